@@ -1,0 +1,237 @@
+/*
+ * libmvs — MI355X (gfx950) transform-space voting core of the partial-symmetry
+ * detector.  C ABI: plain pointers and sizes, no C++ or torch types.
+ *
+ * Two layers:
+ *   1. Host layer (mvs_curvature, mvs_prune, mvs_pairing, mvs_transformations,
+ *      mvs_clustering): borrowed HOST views in, host results out, synchronous.
+ *      Each call is the drop-in replacement of one method of the reference's
+ *      `symmetry::Backend` (include/symmetry/pipeline.h:54-69) with the same
+ *      argument meaning, output order and error classes.
+ *   2. Device layer (mvs_dev_*): DEVICE pointers in HBM, ordered on the
+ *      session's stream; votes never leave HBM.  This is the fast path.
+ *
+ * Status codes map to the reference's exception classes:
+ *   MVS_EINVAL    std::invalid_argument (contract / size violations)
+ *   MVS_EBUDGET   std::runtime_error whose message contains "budget"
+ *   MVS_EOVERFLOW std::overflow_error (int32 / size overflow)
+ *   MVS_EHIP      std::runtime_error "HIP: ..." (device errors)
+ *   MVS_ENODEV    std::runtime_error "HIP: no available device"
+ *   MVS_ERUNTIME  other std::runtime_error (mean-shift budgets, ...)
+ * mvs_last_error() returns the message of the calling thread's last failure.
+ *
+ * Sessions are not thread-safe; use one per host thread.  Calls on one
+ * session are serialised on its stream.
+ */
+#ifndef MVS_H
+#define MVS_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MVS_OK = 0,
+  MVS_EINVAL = 1,
+  MVS_EBUDGET = 2,
+  MVS_EOVERFLOW = 3,
+  MVS_EHIP = 4,
+  MVS_ENODEV = 5,
+  MVS_ERUNTIME = 6
+};
+
+/* Arithmetic of the pairing predicate, frames, votes and mean-shift. */
+enum {
+  MVS_FP64 = 0,          /* plain FP64 throughout (reference arithmetic) */
+  MVS_FP32_GUARDED = 1   /* FP32 with FP64 recheck inside guard bands */
+};
+
+typedef struct mvs_session mvs_session;
+
+/* Edge record with the field order of symmetry::Edge
+ * (include/symmetry/pipeline.h:11-13). */
+typedef struct {
+  int32_t lo, hi, a, b, face0, face1;
+} mvs_edge;
+
+/* std::mt19937_64 state in libstdc++ layout (x[312], index p); the state the
+ * reference's clustering consumes (src/cpu/clustering.cpp:51). */
+typedef struct {
+  uint64_t x[312];
+  uint64_t p;
+} mvs_mt64;
+
+/* SoA point view: x, y, z arrays of `count` doubles (PointsView,
+ * include/symmetry/storage.h:49-57). */
+typedef struct {
+  uint64_t count;
+  const double *x, *y, *z;
+} mvs_points;
+
+/* Topology view (TopologyView, include/symmetry/pipeline.h:14-20).
+ * Only normals, face_normals, neighbors and edges are read by curvature. */
+typedef struct {
+  mvs_points normals;
+  mvs_points face_normals;
+  const uint64_t *neighbor_offsets; /* count+1, first 0 */
+  const int32_t *neighbor_indices;
+  uint64_t edge_count;
+  const mvs_edge *edges;
+} mvs_topology;
+
+/* Feature lanes in FeaturesView order (include/symmetry/storage.h:213-225):
+ * 0-2 normal xyz, 3-5 dmin xyz, 6-8 dmax xyz, 9 kmin, 10 kmax; plus valid. */
+typedef struct {
+  uint64_t count;
+  const double *lane[11];
+  const uint8_t *valid;
+} mvs_features;
+typedef struct {
+  double *lane[11];
+  uint8_t *valid;
+} mvs_features_out;
+
+/* Mesh preparation outputs (prepare(), src/core/prepare.cpp:102-149, with the
+ * topology of :13-58).  Caller-allocated arrays with the stated capacities;
+ * counts are written back. */
+typedef struct {
+  int64_t face_count;       /* usable faces after degenerate rejection */
+  int32_t *f0, *f1, *f2;    /* capacity nf */
+  double *nx, *ny, *nz;     /* vertex normals, nv */
+  double *fnx, *fny, *fnz;  /* face normals, capacity nf */
+  double *weights;          /* vertex area weights, nv */
+  uint64_t *nbr_off;        /* nv+1 */
+  int32_t *nbr_idx;         /* capacity 6*nf */
+  int64_t nbr_nnz;
+  uint64_t *inc_off;        /* nv+1 */
+  int32_t *inc_idx;         /* capacity 3*nf */
+  int64_t inc_nnz;
+  mvs_edge *edges;          /* capacity 3*nf */
+  int64_t edge_count;
+  double box_size, center[3];
+  int32_t rejected_faces, nonmanifold_edges, orientation_conflicts;
+} mvs_prepared;
+
+/* prepare() (src/core/prepare.cpp:102-149) on the host: validation,
+ * degenerate-face rejection, face/vertex normals, area weights, sorted rings,
+ * incident faces and the (lo, hi)-ordered edge list, box size and centre.
+ * Errors carry the reference's messages (MVS_ERUNTIME). */
+int mvs_prepare(int64_t nv, const double *x, const double *y, const double *z, int64_t nf,
+                const int32_t *f0, const int32_t *f1, const int32_t *f2, mvs_prepared *out);
+
+/* std::mt19937_64 on a caller-held state (libstdc++ layout). */
+void mvs_mt64_seed(mvs_mt64 *g, uint64_t seed);
+uint64_t mvs_mt64_next(mvs_mt64 *g);
+
+/* Sampling step of Backend::select (src/core/sampling.cpp:5-24): weighted
+ * sampling without replacement by exponential keys, one draw per
+ * positive-weight vertex; ceil(nv*ratio) ids (at most the positive-weight
+ * count), ascending.  out capacity nv.  Runs on the host like the reference. */
+int mvs_sample_vertices(uint64_t nv, const double *weights, double ratio, mvs_mt64 *rng,
+                        int32_t *out, uint64_t *count);
+
+/* ------------------------------------------------------------- sessions */
+int mvs_session_create(int device, mvs_session **out);
+void mvs_session_destroy(mvs_session *s);
+const char *mvs_last_error(void);
+/* Stream the device layer runs on (default: a stream the session owns).
+ * Pass a hipStream_t; 0 selects the session's own stream again. */
+int mvs_session_set_stream(mvs_session *s, void *hip_stream);
+int mvs_session_sync(mvs_session *s);
+/* Device time (ms) of the last mvs_dev_* launch group, measured with HIP
+ * events on the session stream; see mvs_dev_timing for per-kernel times. */
+int mvs_session_enable_timing(mvs_session *s, int enable);
+int mvs_dev_timing(mvs_session *s, const char *kernel, double *ms, uint64_t *launches);
+
+/* ------------------------------------------------------------ host layer */
+
+/* Backend::curvature (pipeline.h:57; src/cpu/curvature.cpp:14-79).
+ * Reads mesh vertices and the topology; writes 11 lanes + valid. */
+int mvs_curvature(mvs_session *s, const mvs_points *vertices, const mvs_topology *topo,
+                  int smoothing, const mvs_features_out *out);
+
+/* Prune step of Backend::select (pipeline.h:58-59; src/cpu/selection.cpp:21-27):
+ * keeps sample ids with valid && max(|kmin|,|kmax|)>1e-10 &&
+ * |kmin-kmax|/max > 1-prune, order preserved.  out capacity: sample_count. */
+int mvs_prune(mvs_session *s, const mvs_features *f, uint64_t sample_count,
+              const int32_t *sample, double prune, int32_t *out, uint64_t *count);
+
+/* Backend::pairing (pipeline.h:60-61; src/cpu/pairing.cpp:14-35).  Computes
+ * the pair set in CPU order and keeps it on the device; *count receives P.
+ * Fetch with mvs_pairing_fetch.  threshold in (0,1], box > 0 finite. */
+int mvs_pairing(mvs_session *s, const mvs_points *vertices, const mvs_features *f,
+                uint64_t selected_count, const int32_t *selected, double box,
+                double threshold, uint64_t max_pairs, int precision, uint64_t *count);
+int mvs_pairing_fetch(mvs_session *s, int32_t *a, int32_t *b);
+
+/* Backend::transformations (pipeline.h:62-63; src/cpu/transformations.cpp:17-73).
+ * *count receives V; mvs_transformations_fetch writes pair ids, kinds
+ * (0 rigid, 1 reflection) and 28 lanes: rotation[9], translation[3],
+ * plane_normal[3], plane_offset, descriptor[12] (storage.h:329-348). */
+int mvs_transformations(mvs_session *s, const mvs_points *vertices, const mvs_features *f,
+                        uint64_t pair_count, const int32_t *a, const int32_t *b,
+                        double box, const double center[3], uint64_t max_votes,
+                        uint64_t *count);
+int mvs_transformations_fetch(mvs_session *s, int32_t *pair, uint8_t *kind,
+                              double *const lanes[28]);
+
+/* Backend::clustering (pipeline.h:64-65; src/cpu/clustering.cpp:32-208):
+ * sequential-seed Epanechnikov mean-shift with the reference's seeding,
+ * merge, winner and extraction rules.  desc: 12 lanes, rotation: 9 lanes
+ * (used for the degenerate-frame fallback).  rng is consumed exactly as the
+ * reference consumes it.  Results: *n_clusters records and *n_members
+ * member ids; fetch with mvs_clustering_fetch. */
+int mvs_clustering(mvs_session *s, uint64_t vote_count, const uint8_t *kind,
+                   const double *const desc[12], const double *const rotation[9],
+                   double box, const double center[3], double bandwidth,
+                   int max_iterations, int max_clusters, mvs_mt64 *rng,
+                   uint64_t *n_clusters, uint64_t *n_members, int32_t *limited_iterations);
+int mvs_clustering_fetch(mvs_session *s, uint8_t *reflection, double *rotation9,
+                         double *translation3, double *plane_normal3, double *plane_offset,
+                         uint64_t *member_offsets, int32_t *members);
+
+/* ---------------------------------------------------------- device layer
+ * All pointers below are device pointers.  Work is ordered on the session
+ * stream; the call returns once the launches are queued unless it must read
+ * a count back (documented per call). */
+
+/* Curvature on device-resident inputs.  feat: 11*n SoA (stride n), valid: n.
+ * precision MVS_FP64 only for this stage (frames are reused by both paths). */
+int mvs_dev_curvature(mvs_session *s, uint64_t n, const double *x, const double *y,
+                      const double *z, const double *normals3 /* 3*n SoA */,
+                      uint64_t faces, const double *face_normals3 /* 3*faces SoA */,
+                      const uint64_t *nbr_off, const int32_t *nbr_idx, uint64_t edges,
+                      const mvs_edge *edge, int smoothing, double *feat, uint8_t *valid);
+
+/* Pair + vote planning on device-resident features.  Reads the selected
+ * list, sorts the curvature signatures, counts pairs per source and returns
+ * the totals (synchronises once to read them).  Budget errors are raised
+ * before any output is written. */
+int mvs_dev_pair_count(mvs_session *s, uint64_t n, const double *x, const double *y,
+                       const double *z, const double *feat, uint64_t selected_count,
+                       const int32_t *selected, double box, double threshold,
+                       uint64_t max_pairs, int precision, uint64_t *pairs);
+/* Writes the planned pairs in CPU order: a/b hold SELECTED ORDINALS when
+ * ordinals != 0, else mesh vertex ids.  Capacity: the count returned above. */
+int mvs_dev_pair_fill(mvs_session *s, int32_t *a, int32_t *b, int ordinals);
+
+/* Compact vote emission for pairs given as selected ordinals (a < b).
+ * Counts reflection votes (synchronises once) unless max_votes == 0 is
+ * passed to mvs_dev_vote_emit directly after a count. Layout (SoA, s = 8
+ * bytes for MVS_FP64, 4 bytes for MVS_FP32_GUARDED):
+ *   rigid stream, 2 per pair (slot 2k + (sign<0)): q[4] (w,x,y,z, w >= 0),
+ *       t[3], pair id (int32)
+ *   reflection stream, in pair order: n[3], d, pair id (int32)
+ * Global vote order (reference order) is recoverable: pair k's votes start
+ * at 2k + (reflections before k); its reflection vote, if any, comes first. */
+int mvs_dev_vote_count(mvs_session *s, uint64_t pair_count, const int32_t *a,
+                       const int32_t *b, int precision, uint64_t *reflections);
+int mvs_dev_vote_emit(mvs_session *s, uint64_t pair_count, const int32_t *a,
+                      const int32_t *b, double box, const double center[3], int precision,
+                      void *rigid_q4, void *rigid_t3, int32_t *rigid_pair, void *refl_n3,
+                      void *refl_d, int32_t *refl_pair);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

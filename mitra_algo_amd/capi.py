@@ -1,0 +1,377 @@
+"""ctypes binding of libmvs (include/mvs.h).
+
+The library is built in-tree by mitra_algo_amd/build.py.  Loading fails
+loudly when it is missing: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmvs.so")
+
+OK, EINVAL, EBUDGET, EOVERFLOW, EHIP, ENODEV, ERUNTIME = 0, 1, 2, 3, 4, 5, 6
+FP64, FP32_GUARDED = 0, 1
+
+EXPORTS = [
+    "mvs_prepare", "mvs_mt64_seed", "mvs_mt64_next", "mvs_sample_vertices",
+    "mvs_session_create", "mvs_session_destroy", "mvs_last_error", "mvs_session_set_stream",
+    "mvs_session_sync", "mvs_session_enable_timing", "mvs_dev_timing",
+    "mvs_curvature", "mvs_prune", "mvs_pairing", "mvs_pairing_fetch", "mvs_transformations",
+    "mvs_transformations_fetch", "mvs_clustering", "mvs_clustering_fetch",
+    "mvs_dev_curvature", "mvs_dev_pair_count", "mvs_dev_pair_fill", "mvs_dev_vote_count",
+    "mvs_dev_vote_emit",
+]
+
+
+class MvsError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class MvsInvalidArgument(MvsError, ValueError):
+    """std::invalid_argument in the reference."""
+
+
+class MvsBudgetError(MvsError):
+    """std::runtime_error carrying "budget" in the reference."""
+
+
+class MvsOverflowError(MvsError, OverflowError):
+    """std::overflow_error in the reference."""
+
+
+def _raise(code: int, msg: str):
+    if code == EINVAL:
+        raise MvsInvalidArgument(code, msg)
+    if code == EBUDGET:
+        raise MvsBudgetError(code, msg)
+    if code == EOVERFLOW:
+        raise MvsOverflowError(code, msg)
+    raise MvsError(code, msg)
+
+
+class MT64(C.Structure):
+    """std::mt19937_64 state, libstdc++ layout."""
+    _fields_ = [("x", C.c_uint64 * 312), ("p", C.c_uint64)]
+
+    def copy(self) -> "MT64":
+        o = MT64()
+        C.memmove(C.byref(o), C.byref(self), C.sizeof(MT64))
+        return o
+
+    def __eq__(self, other) -> bool:
+        return bytes(self) == bytes(other)
+
+
+class Edge(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("lo", "hi", "a", "b", "face0", "face1")]
+
+
+class Points(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p)]
+
+
+class TopologyView(C.Structure):
+    _fields_ = [("normals", Points), ("face_normals", Points), ("neighbor_offsets", C.c_void_p),
+                ("neighbor_indices", C.c_void_p), ("edge_count", C.c_uint64),
+                ("edges", C.c_void_p)]
+
+
+class FeaturesView(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("lane", C.c_void_p * 11), ("valid", C.c_void_p)]
+
+
+class FeaturesOut(C.Structure):
+    _fields_ = [("lane", C.c_void_p * 11), ("valid", C.c_void_p)]
+
+
+class Prepared(C.Structure):
+    _fields_ = [
+        ("face_count", C.c_int64),
+        ("f0", C.c_void_p), ("f1", C.c_void_p), ("f2", C.c_void_p),
+        ("nx", C.c_void_p), ("ny", C.c_void_p), ("nz", C.c_void_p),
+        ("fnx", C.c_void_p), ("fny", C.c_void_p), ("fnz", C.c_void_p),
+        ("weights", C.c_void_p),
+        ("nbr_off", C.c_void_p), ("nbr_idx", C.c_void_p), ("nbr_nnz", C.c_int64),
+        ("inc_off", C.c_void_p), ("inc_idx", C.c_void_p), ("inc_nnz", C.c_int64),
+        ("edges", C.c_void_p), ("edge_count", C.c_int64),
+        ("box_size", C.c_double), ("center", C.c_double * 3),
+        ("rejected_faces", C.c_int32), ("nonmanifold_edges", C.c_int32),
+        ("orientation_conflicts", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run mitra_algo_amd/build.py "
+                              "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.mvs_last_error.restype = C.c_char_p
+        L.mvs_mt64_next.restype = C.c_uint64
+        L.mvs_mt64_next.argtypes = [C.POINTER(MT64)]
+        L.mvs_mt64_seed.argtypes = [C.POINTER(MT64), C.c_uint64]
+        L.mvs_mt64_seed.restype = None
+        L.mvs_session_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != OK:
+        _raise(rc, lib().mvs_last_error().decode())
+
+
+def ptr(a) -> C.c_void_p:
+    """Address of a numpy array or a torch tensor (device or host)."""
+    if a is None:
+        return C.c_void_p(0)
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(C.c_void_p)
+    if isinstance(a, int):
+        return C.c_void_p(a)
+    return C.c_void_p(a.data_ptr())
+
+
+def rng(seed: int) -> MT64:
+    g = MT64()
+    lib().mvs_mt64_seed(C.byref(g), C.c_uint64(seed))
+    return g
+
+
+def mt_next(g: MT64) -> int:
+    return int(lib().mvs_mt64_next(C.byref(g)))
+
+
+def prepare(vertices: np.ndarray, faces: np.ndarray) -> dict:
+    """prepare() of the reference (host code in libmvs)."""
+    v = np.ascontiguousarray(vertices, dtype=np.float64)
+    f = np.ascontiguousarray(faces, dtype=np.int32)
+    nv, nf = len(v), len(f)
+    x, y, z = (np.ascontiguousarray(v[:, k]) for k in range(3))
+    f0, f1, f2 = (np.ascontiguousarray(f[:, k]) for k in range(3))
+    cap = max(nf, 1)
+    out = {k: np.zeros(cap, np.int32) for k in ("f0", "f1", "f2")}
+    out.update({k: np.zeros(max(nv, 1)) for k in ("nx", "ny", "nz", "weights")})
+    out.update({k: np.zeros(cap) for k in ("fnx", "fny", "fnz")})
+    out["nbr_off"] = np.zeros(nv + 1, np.uint64)
+    out["nbr_idx"] = np.zeros(6 * cap, np.int32)
+    out["inc_off"] = np.zeros(nv + 1, np.uint64)
+    out["inc_idx"] = np.zeros(3 * cap, np.int32)
+    out["edges"] = np.zeros((3 * cap, 6), np.int32)
+    P = Prepared()
+    for k, a in out.items():
+        setattr(P, k, ptr(a))
+    check(lib().mvs_prepare(C.c_int64(nv), ptr(x), ptr(y), ptr(z), C.c_int64(nf), ptr(f0),
+                            ptr(f1), ptr(f2), C.byref(P)))
+    F = P.face_count
+    return dict(
+        faces=np.stack([out["f0"][:F], out["f1"][:F], out["f2"][:F]], 1),
+        normals=np.stack([out["nx"][:nv], out["ny"][:nv], out["nz"][:nv]]),
+        face_normals=np.stack([out["fnx"][:F], out["fny"][:F], out["fnz"][:F]]),
+        weights=out["weights"][:nv].copy(), nbr_off=out["nbr_off"],
+        nbr_idx=out["nbr_idx"][:P.nbr_nnz].copy(), inc_off=out["inc_off"],
+        inc_idx=out["inc_idx"][:P.inc_nnz].copy(), edges=out["edges"][:P.edge_count].copy(),
+        box_size=P.box_size, center=np.array(list(P.center)),
+        rejected_faces=P.rejected_faces)
+
+
+def sample_vertices(weights: np.ndarray, ratio: float, g: MT64) -> np.ndarray:
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    out = np.zeros(max(len(w), 1), np.int32)
+    cnt = C.c_uint64()
+    check(lib().mvs_sample_vertices(C.c_uint64(len(w)), ptr(w), C.c_double(ratio), C.byref(g),
+                                    ptr(out), C.byref(cnt)))
+    return out[:cnt.value].copy()
+
+
+class Session:
+    """One libmvs session (device context + stream + scratch)."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        check(lib().mvs_session_create(C.c_int(device), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().mvs_session_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_stream(self, stream_handle: int):
+        check(lib().mvs_session_set_stream(self.h, C.c_void_p(stream_handle)))
+
+    def sync(self):
+        check(lib().mvs_session_sync(self.h))
+
+    def enable_timing(self, on: bool = True):
+        check(lib().mvs_session_enable_timing(self.h, C.c_int(1 if on else 0)))
+
+    def timing(self, kernel: str):
+        ms = C.c_double()
+        n = C.c_uint64()
+        check(lib().mvs_dev_timing(self.h, kernel.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    # ------------------------------------------------------ host layer
+    def curvature(self, vertices: np.ndarray, topo: dict, smoothing: int):
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        n = len(v)
+        x, y, z = (np.ascontiguousarray(v[:, k]) for k in range(3))
+        nrm = np.ascontiguousarray(topo["normals"], dtype=np.float64)
+        fn = np.ascontiguousarray(topo["face_normals"], dtype=np.float64)
+        off = np.ascontiguousarray(topo["nbr_off"], dtype=np.uint64)
+        idx = np.ascontiguousarray(topo["nbr_idx"], dtype=np.int32)
+        edges = np.ascontiguousarray(topo["edges"], dtype=np.int32)
+        T = TopologyView(Points(n, ptr(nrm[0]), ptr(nrm[1]), ptr(nrm[2])),
+                         Points(fn.shape[1], ptr(fn[0]), ptr(fn[1]), ptr(fn[2])), ptr(off),
+                         ptr(idx), len(edges), ptr(edges))
+        feat = np.zeros((11, n))
+        valid = np.zeros(n, np.uint8)
+        O = FeaturesOut()
+        for k in range(11):
+            O.lane[k] = feat[k].ctypes.data
+        O.valid = valid.ctypes.data
+        check(lib().mvs_curvature(self.h, C.byref(Points(n, ptr(x), ptr(y), ptr(z))),
+                                  C.byref(T), C.c_int(smoothing), C.byref(O)))
+        return feat, valid
+
+    @staticmethod
+    def _features(feat: np.ndarray, valid: np.ndarray | None):
+        feat = np.ascontiguousarray(feat, dtype=np.float64)
+        n = feat.shape[1]
+        if valid is None:
+            valid = np.ones(n, np.uint8)
+        valid = np.ascontiguousarray(valid, dtype=np.uint8)
+        F = FeaturesView()
+        F.count = n
+        for k in range(11):
+            F.lane[k] = feat[k].ctypes.data
+        F.valid = valid.ctypes.data
+        return F, (feat, valid)
+
+    def prune(self, feat, valid, sample, prune: float) -> np.ndarray:
+        F, keep = self._features(feat, valid)
+        s = np.ascontiguousarray(sample, dtype=np.int32)
+        out = np.zeros(max(len(s), 1), np.int32)
+        cnt = C.c_uint64()
+        check(lib().mvs_prune(self.h, C.byref(F), C.c_uint64(len(s)), ptr(s), C.c_double(prune),
+                              ptr(out), C.byref(cnt)))
+        return out[:cnt.value].copy()
+
+    def pairing(self, vertices, feat, selected, box: float, threshold: float,
+                max_pairs: int = 200000, precision: int = FP64):
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        x, y, z = (np.ascontiguousarray(v[:, k]) for k in range(3))
+        F, keep = self._features(feat, None)
+        s = np.ascontiguousarray(selected, dtype=np.int32)
+        cnt = C.c_uint64()
+        check(lib().mvs_pairing(self.h, C.byref(Points(len(v), ptr(x), ptr(y), ptr(z))),
+                                C.byref(F), C.c_uint64(len(s)), ptr(s), C.c_double(box),
+                                C.c_double(threshold), C.c_uint64(max_pairs), C.c_int(precision),
+                                C.byref(cnt)))
+        P = cnt.value
+        a = np.zeros(max(P, 1), np.int32)
+        b = np.zeros(max(P, 1), np.int32)
+        if P:
+            check(lib().mvs_pairing_fetch(self.h, ptr(a), ptr(b)))
+        return a[:P], b[:P]
+
+    def transformations(self, vertices, feat, pa, pb, box: float, center,
+                        max_votes: int = 600000):
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        x, y, z = (np.ascontiguousarray(v[:, k]) for k in range(3))
+        F, keep = self._features(feat, None)
+        pa = np.ascontiguousarray(pa, dtype=np.int32)
+        pb = np.ascontiguousarray(pb, dtype=np.int32)
+        ctr = np.ascontiguousarray(center, dtype=np.float64)
+        cnt = C.c_uint64()
+        check(lib().mvs_transformations(self.h, C.byref(Points(len(v), ptr(x), ptr(y), ptr(z))),
+                                        C.byref(F), C.c_uint64(len(pa)), ptr(pa), ptr(pb),
+                                        C.c_double(box), ptr(ctr), C.c_uint64(max_votes),
+                                        C.byref(cnt)))
+        V = cnt.value
+        pair = np.zeros(max(V, 1), np.int32)
+        kind = np.zeros(max(V, 1), np.uint8)
+        lanes = np.zeros((28, max(V, 1)))
+        if V:
+            arr = (C.c_void_p * 28)(*[lanes[k].ctypes.data for k in range(28)])
+            check(lib().mvs_transformations_fetch(self.h, ptr(pair), ptr(kind), arr))
+        return pair[:V], kind[:V], np.ascontiguousarray(lanes[:, :V])
+
+    def clustering(self, kind, desc, rotation, box: float, center, bandwidth: float,
+                   max_iterations: int, max_clusters: int, g: MT64):
+        kind = np.ascontiguousarray(kind, dtype=np.uint8)
+        V = len(kind)
+        desc = np.ascontiguousarray(desc, dtype=np.float64).reshape(12, V)
+        rotation = np.ascontiguousarray(rotation, dtype=np.float64).reshape(9, V)
+        dptr = (C.c_void_p * 12)(*[desc[k].ctypes.data for k in range(12)])
+        rptr = (C.c_void_p * 9)(*[rotation[k].ctypes.data for k in range(9)])
+        ctr = np.ascontiguousarray(center, dtype=np.float64)
+        nc, nm, lim = C.c_uint64(), C.c_uint64(), C.c_int32()
+        check(lib().mvs_clustering(self.h, C.c_uint64(V), ptr(kind), dptr, rptr, C.c_double(box),
+                                   ptr(ctr), C.c_double(bandwidth), C.c_int(max_iterations),
+                                   C.c_int(max_clusters), C.byref(g), C.byref(nc), C.byref(nm),
+                                   C.byref(lim)))
+        n, m = nc.value, nm.value
+        refl = np.zeros(max(n, 1), np.uint8)
+        rot = np.zeros((max(n, 1), 9))
+        tr = np.zeros((max(n, 1), 3))
+        pn = np.zeros((max(n, 1), 3))
+        po = np.zeros(max(n, 1))
+        off = np.zeros(n + 1, np.uint64)
+        mem = np.zeros(max(m, 1), np.int32)
+        check(lib().mvs_clustering_fetch(self.h, ptr(refl), ptr(rot), ptr(tr), ptr(pn), ptr(po),
+                                         ptr(off), ptr(mem)))
+        return dict(reflection=refl[:n], rotation=rot[:n], translation=tr[:n],
+                    plane_normal=pn[:n], plane_offset=po[:n], member_off=off,
+                    members=mem[:m], limited_iterations=lim.value)
+
+    # ---------------------------------------------------- device layer
+    def dev_curvature(self, n, x, y, z, normals3, nf, fn3, off, idx, ne, edges, smoothing,
+                      feat, valid):
+        check(lib().mvs_dev_curvature(self.h, C.c_uint64(n), ptr(x), ptr(y), ptr(z), ptr(normals3),
+                                      C.c_uint64(nf), ptr(fn3), ptr(off), ptr(idx), C.c_uint64(ne),
+                                      ptr(edges), C.c_int(smoothing), ptr(feat), ptr(valid)))
+
+    def dev_pair_count(self, n, x, y, z, feat, S, sel, box, t, max_pairs, precision) -> int:
+        P = C.c_uint64()
+        check(lib().mvs_dev_pair_count(self.h, C.c_uint64(n), ptr(x), ptr(y), ptr(z), ptr(feat),
+                                       C.c_uint64(S), ptr(sel), C.c_double(box), C.c_double(t),
+                                       C.c_uint64(max_pairs), C.c_int(precision), C.byref(P)))
+        return P.value
+
+    def dev_pair_fill(self, a, b, ordinals: bool):
+        check(lib().mvs_dev_pair_fill(self.h, ptr(a), ptr(b), C.c_int(1 if ordinals else 0)))
+
+    def dev_vote_count(self, P, a, b, precision) -> int:
+        R = C.c_uint64()
+        check(lib().mvs_dev_vote_count(self.h, C.c_uint64(P), ptr(a), ptr(b), C.c_int(precision),
+                                       C.byref(R)))
+        return R.value
+
+    def dev_vote_emit(self, P, a, b, box, center, precision, q4, t3, rp, n3, d1, fp):
+        ctr = (C.c_double * 3)(*[float(c) for c in center])
+        check(lib().mvs_dev_vote_emit(self.h, C.c_uint64(P), ptr(a), ptr(b), C.c_double(box), ctr,
+                                      C.c_int(precision), ptr(q4), ptr(t3), ptr(rp), ptr(n3),
+                                      ptr(d1), ptr(fp)))

@@ -1,0 +1,194 @@
+// Shared host/device infrastructure of libmvs: status codes, error transport,
+// device buffers, the session and small device math helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mvs.h"
+
+namespace mvs {
+
+// ------------------------------------------------------------------ errors
+// Exceptions inside the library carry an mvs status code; every extern "C"
+// entry point converts them back to (code, thread-local message).
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] inline void raise(int code, const std::string &msg) { throw Error(code, msg); }
+inline void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    raise(MVS_EHIP, std::string("HIP: ") + what + ": " + hipGetErrorString(e));
+}
+#define MVS_HIP(call) ::mvs::hip_check((call), #call)
+void set_error(int code, const char *msg);
+template <class F> int guarded(F &&f) {
+  try {
+    f();
+    return MVS_OK;
+  } catch (const Error &e) {
+    set_error(e.code, e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_error(MVS_EHIP, "host allocation failed");
+    return MVS_EHIP;
+  } catch (const std::exception &e) {
+    set_error(MVS_ERUNTIME, e.what());
+    return MVS_ERUNTIME;
+  }
+}
+
+// ---------------------------------------------------------------- buffers
+// Device buffer owned by the caller's scope; freed on unwind.
+template <class T> struct DBuf {
+  T *ptr = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  explicit DBuf(size_t count) { alloc(count); }
+  void alloc(size_t count) {
+    release();
+    if (count > SIZE_MAX / sizeof(T))
+      raise(MVS_EOVERFLOW, "device allocation size overflow");
+    n = count;
+    if (count)
+      MVS_HIP(hipMalloc(reinterpret_cast<void **>(&ptr), count * sizeof(T)));
+  }
+  void ensure(size_t count) {
+    if (count > n)
+      alloc(count);
+  }
+  void release() {
+    if (ptr)
+      (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : ptr(o.ptr), n(o.n) { o.ptr = nullptr, o.n = 0; }
+  DBuf &operator=(DBuf &&o) noexcept {
+    release();
+    ptr = o.ptr, n = o.n;
+    o.ptr = nullptr, o.n = 0;
+    return *this;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// ---------------------------------------------------------------- session
+struct Session;  // defined in session.hip
+Session *as_session(mvs_session *s);
+
+// Stream-ordered helpers bound to a session.
+hipStream_t session_stream(Session *s);
+void *scratch(Session *s, int slot, size_t bytes);  // grow-only per-slot arena
+template <class T> T *scratch_as(Session *s, int slot, size_t count) {
+  return static_cast<T *>(scratch(s, slot, count * sizeof(T)));
+}
+void *host_stage(Session *s, size_t bytes);  // pinned staging for small D2H
+void copy_h2d(Session *s, void *dst, const void *src, size_t bytes);
+void copy_d2h(Session *s, void *dst, const void *src, size_t bytes);
+void sync(Session *s);
+
+// Scratch slots (each slot is an independent grow-only allocation).
+enum Slot : int {
+  kSlotSort0 = 0, kSlotSort1, kSlotSort2, kSlotSort3, kSlotSortHist,
+  kSlotScan, kSlotFlags, kSlotSamples, kSlotWindows, kSlotCounts, kSlotOffsets,
+  kSlotRefl, kSlotBlock, kSlotMisc0, kSlotMisc1, kSlotMisc2, kSlotMisc3,
+  // pairing plan (persist between count, fill and vote emission)
+  kSlotPlanSamples32, kSlotPlanKeys, kSlotPlanOrd, kSlotPlanSortedK, kSlotPlanSortedK32,
+  kSlotPlanWin, kSlotPlanCounts, kSlotPlanOffsets, kSlotPlanLong, kSlotPlanStats,
+  kSlotVoteBlock, kSlotVoteBlockOff,
+  kSlotMs0, kSlotMs1, kSlotMs2, kSlotMs3, kSlotMs4, kSlotMs5, kSlotMs6, kSlotMs7,
+  kSlotCount
+};
+
+// ------------------------------------------------------------- primitives
+// Exclusive prefix sum of n uint64 values into out[0..n] (out[n] = total).
+void exclusive_scan_u64(Session *s, const uint64_t *in, uint64_t *out, uint64_t n);
+// Exclusive prefix sum of n uint32 values into uint64 out[0..n].
+void exclusive_scan_u32(Session *s, const uint32_t *in, uint64_t *out, uint64_t n);
+// Stable LSD radix sort of (key, value) by the low `bits` bits of key.
+void radix_sort_pairs_u64(Session *s, const uint64_t *keys_in, const uint32_t *vals_in,
+                          uint64_t *keys_out, uint32_t *vals_out, uint64_t n, int bits);
+
+// std::mt19937_64 step on a state in libstdc++ layout (sampling.cpp).
+uint64_t mt64_next(mvs_mt64 *g);
+
+inline unsigned blocks_for(uint64_t n, unsigned threads) {
+  return static_cast<unsigned>((n + threads - 1) / threads);
+}
+
+// ------------------------------------------------------------ device math
+// All FP64 helpers compile with -ffp-contract=off so every product and sum
+// rounds as written, matching the reference CPU build
+// (src/core/math.cpp:9-38, 113-119).
+struct V3 {
+  double x, y, z;
+};
+__host__ __device__ inline V3 v3(double x, double y, double z) { return {x, y, z}; }
+__host__ __device__ inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ inline V3 vdivs(V3 a, double s) { return a * (1 / s); }
+__host__ __device__ inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__host__ __device__ inline double norm(V3 a) { return sqrt(dot(a, a)); }
+__host__ __device__ inline V3 unit(V3 a) {
+  double n = norm(a);
+  return n > 1e-14 ? vdivs(a, n) : V3{0, 0, 0};
+}
+__host__ __device__ inline double comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+struct M3 {
+  double m[9];
+};
+__host__ __device__ inline V3 apply(const M3 &m, V3 v) {
+  return {m.m[0] * v.x + m.m[1] * v.y + m.m[2] * v.z, m.m[3] * v.x + m.m[4] * v.y + m.m[5] * v.z,
+          m.m[6] * v.x + m.m[7] * v.y + m.m[8] * v.z};
+}
+__host__ __device__ inline M3 frame(V3 a, V3 b, V3 c) {
+  return {{a.x, b.x, c.x, a.y, b.y, c.y, a.z, b.z, c.z}};
+}
+// R = B * A^T accumulated from zero in k order (math.cpp:31-38).
+__host__ __device__ inline M3 mul_bt(const M3 &b, const M3 &a) {
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k)
+        s += b.m[i * 3 + k] * a.m[j * 3 + k];
+      r.m[i * 3 + j] = s;
+    }
+  return r;
+}
+__host__ __device__ inline M3 reflection(V3 n) {
+  M3 r = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      r.m[i * 3 + j] -= 2 * comp(n, i) * comp(n, j);
+  return r;
+}
+
+// Wavefront helpers (64 lanes on CDNA).
+constexpr int kWave = 64;
+__device__ inline int lane_id() { return __lane_id(); }
+__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+__device__ inline int lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0));
+}
+template <class T> __device__ inline T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1)
+    v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+}  // namespace mvs

@@ -1,0 +1,454 @@
+// Mean-shift over transform votes.
+//
+// Sequential-seed mode (the reference's semantics, src/cpu/clustering.cpp:
+// 32-208): one trajectory at a time, each run entirely inside one launch of
+// k_trajectory (a single 1024-thread workgroup): per iteration the distances
+// of all same-kind votes are evaluated in parallel, the in-support votes are
+// compacted in vote order, and their weights and weighted descriptors are
+// summed IN VOTE ORDER by one lane per component, so every sum rounds exactly
+// as the reference's sequential loop does.  The host keeps the reference's
+// driver: seed = k-th unvisited vote (Fenwick tree instead of rebuilding the
+// unvisited list), merge into the first same-kind centre within h^2/4,
+// winners by largest support count, extraction, stable sort, truncation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "session.hpp"
+
+namespace mvs {
+namespace {
+
+constexpr int kTrajThreads = 1024;
+
+struct TrajState {
+  double mean[12];
+  int32_t converged, exhausted, iterations, touched;
+};
+
+// support[] must be zero on entry for every vote; on exit the touched votes
+// are listed in touched_ids/touched_counts and their support entries reset.
+__global__ void __launch_bounds__(kTrajThreads)
+    k_trajectory(const double *desc, const uint8_t *kind, uint64_t V, int32_t seed, double h2,
+                 int max_iterations, uint32_t *support, int32_t *list_j, double *list_w,
+                 int32_t *touched_ids, uint32_t *touched_counts, TrajState *out) {
+  __shared__ double mean[12];
+  __shared__ double sums[13];
+  __shared__ uint32_t wsum[kTrajThreads / kWave + 1];
+  __shared__ uint32_t n_in, n_touched;
+  __shared__ int flag_exhausted, flag_converged, iters;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const bool refl = kind[seed] != 0;
+  const int dim = refl ? 4 : 12;
+  if (tid < 12)
+    mean[tid] = desc[static_cast<uint64_t>(tid) * V + seed];
+  if (tid == 0) {
+    n_touched = 0;
+    flag_exhausted = flag_converged = 0;
+    iters = 0;
+  }
+  __syncthreads();
+  for (int it = 0; it < max_iterations; ++it) {
+    if (tid == 0)
+      n_in = 0;
+    __syncthreads();
+    double m[12];
+    for (int k = 0; k < 12; ++k)
+      m[k] = mean[k];
+    for (uint64_t base = 0; base < V; base += kTrajThreads) {
+      uint64_t j = base + tid;
+      bool in = false;
+      double w = 0;
+      if (j < V && (kind[j] != 0) == refl) {
+        double s = 0;
+        for (int k = 0; k < dim; ++k) {
+          double d = m[k] - desc[static_cast<uint64_t>(k) * V + j];
+          s += d * d;
+        }
+        if (s < h2) {
+          in = true;
+          w = 1 - s / h2;
+        }
+      }
+      // ordered compaction (vote order) of the in-support votes
+      uint64_t mask = ballot(in);
+      if (lane == 0)
+        wsum[wave] = static_cast<uint32_t>(__popcll(mask));
+      __syncthreads();
+      if (tid == 0) {
+        uint32_t run = n_in;
+        for (int q = 0; q < kTrajThreads / kWave; ++q) {
+          uint32_t c = wsum[q];
+          wsum[q] = run;
+          run += c;
+        }
+        wsum[kTrajThreads / kWave] = run;
+      }
+      __syncthreads();
+      if (in) {
+        uint32_t pos = wsum[wave] + static_cast<uint32_t>(lanes_below(mask));
+        list_j[pos] = static_cast<int32_t>(j);
+        list_w[pos] = w;
+        if (support[j]++ == 0) {
+          uint32_t t = atomicAdd(&n_touched, 1u);
+          touched_ids[t] = static_cast<int32_t>(j);
+        }
+      }
+      __syncthreads();
+      if (tid == 0)
+        n_in = wsum[kTrajThreads / kWave];
+      __syncthreads();
+    }
+    // ordered sums: lane 0 the weight, lane k the k-th weighted component
+    __threadfence_block();
+    __syncthreads();
+    const uint32_t N = n_in;
+    if (tid <= dim) {
+      double acc = 0;
+      if (tid == 0) {
+        for (uint32_t e = 0; e < N; ++e)
+          acc += list_w[e];
+      } else {
+        const double *col = desc + static_cast<uint64_t>(tid - 1) * V;
+        for (uint32_t e = 0; e < N; ++e)
+          acc += list_w[e] * col[list_j[e]];
+      }
+      sums[tid] = acc;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      iters = it + 1;
+      double weight = sums[0];
+      if (weight <= 1e-14) {
+        flag_exhausted = 1;
+      } else {
+        double movement = 0;
+        double next[12];
+        for (int k = 0; k < dim; ++k)
+          next[k] = sums[k + 1] / weight;
+        for (int k = 0; k < dim; ++k) {
+          double d = mean[k] - next[k];
+          movement += d * d;
+        }
+        for (int k = 0; k < dim; ++k)
+          mean[k] = next[k];
+        if (movement < 1e-6 * h2)
+          flag_converged = 1;
+      }
+    }
+    __syncthreads();
+    if (flag_exhausted || flag_converged)
+      break;
+  }
+  __syncthreads();
+  const uint32_t T = n_touched;
+  for (uint32_t e = tid; e < T; e += kTrajThreads) {
+    int32_t j = touched_ids[e];
+    touched_counts[e] = support[j];
+    support[j] = 0;
+  }
+  if (tid == 0) {
+    for (int k = 0; k < 12; ++k)
+      out->mean[k] = k < dim ? mean[k] : 0.0;
+    out->converged = flag_converged;
+    out->exhausted = flag_exhausted;
+    out->iterations = iters;
+    out->touched = static_cast<int32_t>(T);
+  }
+}
+
+// Fenwick tree over the unvisited votes: k-th unvisited in ascending order.
+struct Fenwick {
+  std::vector<int64_t> t;
+  int64_t n = 0, total = 0, top = 1;
+  explicit Fenwick(int64_t count) : t(count + 1, 0), n(count) {
+    for (int64_t i = 1; i <= n; ++i) {
+      t[i] += 1;
+      int64_t j = i + (i & -i);
+      if (j <= n)
+        t[j] += t[i];
+    }
+    total = n;
+    while (top * 2 <= n)
+      top *= 2;
+  }
+  void remove(int64_t idx) {
+    --total;
+    for (int64_t i = idx + 1; i <= n; i += i & -i)
+      t[i] -= 1;
+  }
+  int64_t kth(int64_t k) const {  // 0-based
+    int64_t pos = 0;
+    for (int64_t step = top; step; step >>= 1)
+      if (pos + step <= n && t[pos + step] <= k) {
+        pos += step;
+        k -= t[pos];
+      }
+    return pos;  // index (0-based) of the k-th one
+  }
+};
+
+struct Entry {
+  int32_t vote;
+  uint64_t count;
+};
+
+double ddist(const double *a, const double *b, int dim) {
+  double s = 0;
+  for (int i = 0; i < dim; ++i) {
+    double d = a[i] - b[i];
+    s += d * d;
+  }
+  return s;
+}
+
+}  // namespace
+
+struct ClusterResult {
+  std::vector<uint8_t> refl;
+  std::vector<double> rot, trans, pn, po;
+  std::vector<uint64_t> off;
+  std::vector<int32_t> members;
+  int32_t limited = 0;
+};
+
+// Host driver over device trajectories (clustering.cpp:32-154).
+static void clustering_run(Session *s, uint64_t V, const uint8_t *kind_h, const double *desc_d,
+                           const uint8_t *kind_d, const double *const rot_h[9], double box,
+                           const double center[3], double bandwidth, int max_iterations,
+                           int max_clusters, mvs_mt64 *rng, ClusterResult &res,
+                           const double *const desc_h[12]) {
+  const double h = bandwidth, h2 = h * h;
+  uint32_t *support = scratch_as<uint32_t>(s, kSlotMs0, V ? V : 1);
+  int32_t *list_j = scratch_as<int32_t>(s, kSlotMs1, V ? V : 1);
+  double *list_w = scratch_as<double>(s, kSlotMs2, V ? V : 1);
+  int32_t *touched = scratch_as<int32_t>(s, kSlotMs3, V ? V : 1);
+  uint32_t *tcount = scratch_as<uint32_t>(s, kSlotMs4, V ? V : 1);
+  TrajState *st = scratch_as<TrajState>(s, kSlotMs5, 1);
+  if (V)
+    MVS_HIP(hipMemsetAsync(support, 0, V * sizeof(uint32_t), s->stream));
+  struct Center {
+    double v[12];
+    bool refl;
+  };
+  std::vector<Center> centers;
+  std::vector<std::vector<Entry>> counters;
+  std::vector<uint8_t> visited(V, 0);
+  Fenwick fw(static_cast<int64_t>(V));
+  uint64_t support_entries = 0;
+  int32_t limited = 0;
+  std::vector<int32_t> ids;
+  std::vector<uint32_t> cnts;
+  TrajState hs;
+  while (fw.total > 0) {
+    uint64_t r = mt64_next(rng) % static_cast<uint64_t>(fw.total);
+    int64_t seed = fw.kth(static_cast<int64_t>(r));
+    bool refl = kind_h[seed] != 0;
+    int dim = refl ? 4 : 12;
+    MVS_LAUNCH(s, "ms_trajectory", k_trajectory, 1, kTrajThreads, 0, desc_d, kind_d, V,
+               static_cast<int32_t>(seed), h2, max_iterations, support, list_j, list_w, touched,
+               tcount, st);
+    copy_d2h(s, &hs, st, sizeof hs);
+    uint32_t T = static_cast<uint32_t>(hs.touched);
+    ids.resize(T);
+    cnts.resize(T);
+    copy_d2h(s, ids.data(), touched, T * sizeof(int32_t));
+    copy_d2h(s, cnts.data(), tcount, T * sizeof(uint32_t));
+    std::vector<Entry> sup(T);
+    for (uint32_t e = 0; e < T; ++e)
+      sup[e] = {ids[e], cnts[e]};
+    std::sort(sup.begin(), sup.end(), [](const Entry &a, const Entry &b) { return a.vote < b.vote; });
+    for (const Entry &e : sup)
+      if (!visited[e.vote]) {
+        visited[e.vote] = 1;
+        fw.remove(e.vote);
+      }
+    if (hs.exhausted && !visited[seed]) {
+      visited[seed] = 1;
+      fw.remove(seed);
+    }
+    if (!hs.converged)
+      ++limited;
+    int64_t merge = -1;
+    for (size_t c = 0; c < centers.size(); ++c)
+      if (centers[c].refl == refl && ddist(hs.mean, centers[c].v, dim) < h2 / 4) {
+        merge = static_cast<int64_t>(c);
+        break;
+      }
+    if (merge < 0) {
+      Center c{};
+      for (int k = 0; k < dim; ++k)
+        c.v[k] = hs.mean[k];
+      c.refl = refl;
+      centers.push_back(c);
+      support_entries += sup.size();
+      counters.push_back(std::move(sup));
+    } else {
+      auto &cm = counters[merge];
+      double old = static_cast<double>(cm.size()), now = static_cast<double>(sup.size());
+      if (old + now > 0)
+        for (int k = 0; k < dim; ++k)
+          centers[merge].v[k] = (old * centers[merge].v[k] + now * hs.mean[k]) / (old + now);
+      size_t before = cm.size();
+      std::vector<Entry> merged;
+      merged.reserve(cm.size() + sup.size());
+      size_t i = 0, j = 0;
+      while (i < cm.size() || j < sup.size()) {
+        if (j >= sup.size() || (i < cm.size() && cm[i].vote < sup[j].vote))
+          merged.push_back(cm[i++]);
+        else if (i >= cm.size() || sup[j].vote < cm[i].vote)
+          merged.push_back(sup[j++]);
+        else {
+          Entry e = cm[i++];
+          e.count += sup[j++].count;
+          merged.push_back(e);
+        }
+      }
+      cm.swap(merged);
+      support_entries += cm.size() - before;
+    }
+    if (support_entries > 5000000)
+      raise(MVS_ERUNTIME, "Mean-shift support budget exceeded; lower --sample-ratio");
+    if (centers.size() > 20000)
+      raise(MVS_ERUNTIME, "Too many mean-shift modes; increase --bandwidth or lower --sample-ratio");
+  }
+  // winners: strictly larger count wins, earlier cluster keeps ties
+  std::vector<uint64_t> best(V, 0);
+  std::vector<int32_t> winner(V, -1);
+  for (size_t c = 0; c < counters.size(); ++c)
+    for (const Entry &e : counters[c])
+      if (e.count > best[e.vote]) {
+        best[e.vote] = e.count;
+        winner[e.vote] = static_cast<int32_t>(c);
+      }
+  std::vector<std::vector<int32_t>> members(centers.size());
+  for (uint64_t i = 0; i < V; ++i)
+    if (winner[i] >= 0)
+      members[winner[i]].push_back(static_cast<int32_t>(i));
+  struct Out {
+    bool refl;
+    double rot[9], trans[3], pn[3], po;
+    std::vector<int32_t> votes;
+  };
+  std::vector<Out> outs;
+  const V3 origin = v3(center[0], center[1], center[2]);
+  for (size_t c = 0; c < centers.size(); ++c) {
+    if (members[c].empty())
+      continue;
+    Out o{};
+    o.refl = centers[c].refl;
+    o.votes = std::move(members[c]);
+    const double *d = centers[c].v;
+    if (o.refl) {
+      V3 n = v3(d[0], d[1], d[2]);
+      double length = norm(n);
+      V3 pn = unit(n);
+      double off = d[3] * box / length - dot(pn, origin);
+      M3 r = reflection(pn);
+      V3 t = pn * (-2 * off);
+      std::memcpy(o.rot, r.m, sizeof o.rot);
+      o.trans[0] = t.x, o.trans[1] = t.y, o.trans[2] = t.z;
+      o.pn[0] = pn.x, o.pn[1] = pn.y, o.pn[2] = pn.z;
+      o.po = off;
+    } else {
+      M3 m;
+      for (int k = 0; k < 9; ++k)
+        m.m[k] = d[k] * 2;
+      V3 c0 = v3(m.m[0], m.m[3], m.m[6]), c1 = v3(m.m[1], m.m[4], m.m[7]);
+      V3 a = unit(c0);
+      V3 b = unit(c1 - a * dot(a, c1));
+      if (norm(a) < 0.5 || norm(b) < 0.5) {
+        int32_t v0 = o.votes[0];
+        for (int k = 0; k < 9; ++k)
+          m.m[k] = rot_h[k][v0];
+      } else
+        m = frame(a, b, cross(a, b));
+      V3 t = v3(d[9], d[10], d[11]);
+      V3 tr = (t * box + origin) - apply(m, origin);
+      std::memcpy(o.rot, m.m, sizeof o.rot);
+      o.trans[0] = tr.x, o.trans[1] = tr.y, o.trans[2] = tr.z;
+    }
+    outs.push_back(std::move(o));
+  }
+  std::stable_sort(outs.begin(), outs.end(),
+                   [](const Out &a, const Out &b) { return a.votes.size() > b.votes.size(); });
+  if (outs.size() > static_cast<size_t>(max_clusters))
+    outs.resize(max_clusters);
+  res = ClusterResult{};
+  res.limited = limited;
+  res.off.push_back(0);
+  for (const Out &o : outs) {
+    res.refl.push_back(o.refl);
+    res.rot.insert(res.rot.end(), o.rot, o.rot + 9);
+    res.trans.insert(res.trans.end(), o.trans, o.trans + 3);
+    res.pn.insert(res.pn.end(), o.pn, o.pn + 3);
+    res.po.push_back(o.po);
+    res.members.insert(res.members.end(), o.votes.begin(), o.votes.end());
+    res.off.push_back(res.members.size());
+  }
+  (void)desc_h;
+}
+
+}  // namespace mvs
+
+using namespace mvs;
+
+extern "C" {
+
+int mvs_clustering(mvs_session *sp, uint64_t V, const uint8_t *kind, const double *const desc[12],
+                   const double *const rotation[9], double box, const double center[3],
+                   double bandwidth, int max_iterations, int max_clusters, mvs_mt64 *rng,
+                   uint64_t *n_clusters, uint64_t *n_members, int32_t *limited) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    if (!rng || !n_clusters || !n_members || !limited || !center || (V && (!kind || !desc)))
+      raise(MVS_EINVAL, "null argument");
+    if (V > static_cast<uint64_t>(INT32_MAX))
+      raise(MVS_EOVERFLOW, "vote ids exceed int32 range");
+    if (!(bandwidth > 0) || max_iterations < 1 || max_clusters < 1)
+      raise(MVS_EINVAL, "invalid mean-shift parameters");
+    DBuf<double> d(12 * (V ? V : 1));
+    DBuf<uint8_t> k(V ? V : 1);
+    for (int c = 0; c < 12; ++c)
+      copy_h2d(s, d.ptr + c * V, desc[c], V * 8);
+    copy_h2d(s, k.ptr, kind, V);
+    ClusterResult res;
+    clustering_run(s, V, kind, d.ptr, k.ptr, rotation, box, center, bandwidth, max_iterations,
+                   max_clusters, rng, res, desc);
+    s->cl_refl = std::move(res.refl);
+    s->cl_rot = std::move(res.rot);
+    s->cl_trans = std::move(res.trans);
+    s->cl_pn = std::move(res.pn);
+    s->cl_po = std::move(res.po);
+    s->cl_off = std::move(res.off);
+    s->cl_members = std::move(res.members);
+    *n_clusters = s->cl_refl.size();
+    *n_members = s->cl_members.size();
+    *limited = res.limited;
+  });
+}
+
+int mvs_clustering_fetch(mvs_session *sp, uint8_t *reflection, double *rotation9,
+                         double *translation3, double *plane_normal3, double *plane_offset,
+                         uint64_t *member_offsets, int32_t *members) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    auto put = [](auto *dst, const auto &v) {
+      if (dst && !v.empty())
+        std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    put(reflection, s->cl_refl);
+    put(rotation9, s->cl_rot);
+    put(translation3, s->cl_trans);
+    put(plane_normal3, s->cl_pn);
+    put(plane_offset, s->cl_po);
+    put(member_offsets, s->cl_off);
+    put(members, s->cl_members);
+  });
+}
+
+}  // extern "C"

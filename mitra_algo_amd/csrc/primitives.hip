@@ -1,0 +1,270 @@
+// Device primitives: exclusive scans and the LDS-staged stable LSD radix sort
+// used for the curvature-signature sort.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "session.hpp"
+
+namespace mvs {
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
+
+// Block-wide exclusive scan of one value per thread; returns the total.
+template <class T> __device__ T block_exclusive_scan(T v, T *lds, T &total) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  T x = v;
+  for (int o = 1; o < kWave; o <<= 1) {
+    T y = __shfl_up(x, o, kWave);
+    if (lane >= o)
+      x += y;
+  }
+  if (lane == kWave - 1)
+    lds[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T run = 0;
+    for (int w = 0; w < kScanThreads / kWave; ++w) {
+      T t = lds[w];
+      lds[w] = run;
+      run += t;
+    }
+    lds[kScanThreads / kWave] = run;
+  }
+  __syncthreads();
+  T out = x - v + lds[wave];
+  total = lds[kScanThreads / kWave];
+  __syncthreads();
+  return out;
+}
+
+template <class TIn>
+__global__ void __launch_bounds__(kScanThreads)
+    k_scan_reduce(const TIn *in, uint64_t n, uint64_t *partial) {
+  __shared__ uint64_t lds[kScanThreads / kWave + 1];
+  uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+  uint64_t sum = 0;
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n)
+      sum += static_cast<uint64_t>(in[base + i]);
+  uint64_t total;
+  (void)block_exclusive_scan<uint64_t>(sum, lds, total);
+  if (threadIdx.x == 0)
+    partial[blockIdx.x] = total;
+}
+
+template <class TIn>
+__global__ void __launch_bounds__(kScanThreads)
+    k_scan_apply(const TIn *in, uint64_t n, const uint64_t *partial_scanned, uint64_t *out) {
+  __shared__ uint64_t lds[kScanThreads / kWave + 1];
+  uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+  uint64_t v[kScanItems];
+  uint64_t sum = 0;
+  for (int i = 0; i < kScanItems; ++i) {
+    v[i] = base + i < n ? static_cast<uint64_t>(in[base + i]) : 0;
+    sum += v[i];
+  }
+  uint64_t total;
+  uint64_t run = block_exclusive_scan<uint64_t>(sum, lds, total) + partial_scanned[blockIdx.x];
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n)
+      out[base + i] = run;
+    run += v[i];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+    out[n] = partial_scanned[blockIdx.x] + total;
+}
+
+__global__ void k_zero_u64(uint64_t *p) { *p = 0; }
+
+// Recursive three-phase scan.  `pool` holds the partial arrays of all levels
+// (carved from one scratch slot, sized by scan_pool_words).
+template <class TIn>
+void exclusive_scan_level(Session *s, const TIn *in, uint64_t *out, uint64_t n, uint64_t *pool) {
+  if (n == 0) {
+    MVS_LAUNCH(s, "scan_zero", k_zero_u64, 1, 1, 0, out);
+    return;
+  }
+  unsigned nb = blocks_for(n, kScanTile);
+  uint64_t *part = pool, *part_scan = pool + nb;
+  if (nb == 1) {
+    MVS_LAUNCH(s, "scan_zero", k_zero_u64, 1, 1, 0, part_scan);
+  } else {
+    MVS_LAUNCH(s, "scan_reduce", (k_scan_reduce<TIn>), nb, kScanThreads, 0, in, n, part);
+    exclusive_scan_level<uint64_t>(s, part, part_scan, nb, pool + 2 * nb + 1);
+  }
+  MVS_LAUNCH(s, "scan_apply", (k_scan_apply<TIn>), nb, kScanThreads, 0, in, n, part_scan, out);
+}
+uint64_t scan_pool_words(uint64_t n) {
+  uint64_t words = 0;
+  while (true) {
+    uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 0)
+      nb = 1;
+    words += 2 * nb + 1;
+    if (nb == 1)
+      return words;
+    n = nb;
+  }
+}
+template <class TIn>
+void exclusive_scan_impl(Session *s, const TIn *in, uint64_t *out, uint64_t n, int slot) {
+  uint64_t *pool = scratch_as<uint64_t>(s, slot, scan_pool_words(n));
+  exclusive_scan_level<TIn>(s, in, out, n, pool);
+}
+
+// ------------------------------------------------------------- radix sort
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;  // 2048 keys per tile
+constexpr int kSortWaves = kSortThreads / kWave;      // 4
+constexpr int kRadix = 256;
+
+__global__ void __launch_bounds__(kSortThreads)
+    k_radix_hist(const uint64_t *keys, uint64_t n, int shift, unsigned tiles, uint32_t *hist) {
+  __shared__ uint32_t h[kRadix];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile;
+  for (int i = 0; i < kSortItems; ++i) {
+    uint64_t j = base + i * kSortThreads + threadIdx.x;
+    if (j < n)
+      atomicAdd(&h[(keys[j] >> shift) & 0xFF], 1u);
+  }
+  __syncthreads();
+  // digit-major layout: hist[d * tiles + tile]
+  hist[static_cast<uint64_t>(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of one tile.  Wave w owns keys [w*512, (w+1)*512) of the
+// tile, processed in 8 rounds of 64; ranks are stable within the tile, the
+// tile is re-ordered in LDS and written out in digit runs (coalesced).
+__global__ void __launch_bounds__(kSortThreads)
+    k_radix_scatter(const uint64_t *keys_in, const uint32_t *vals_in, uint64_t n, int shift,
+                    unsigned tiles, const uint64_t *offsets, uint64_t *keys_out,
+                    uint32_t *vals_out) {
+  __shared__ uint32_t wave_hist[kSortWaves][kRadix];
+  __shared__ uint32_t tile_off[kRadix];
+  __shared__ uint64_t s_keys[kSortTile];
+  __shared__ uint32_t s_vals[kSortTile];
+  __shared__ uint64_t g_off[kRadix];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  for (int w = 0; w < kSortWaves; ++w)
+    wave_hist[w][threadIdx.x] = 0;
+  g_off[threadIdx.x] = offsets[static_cast<uint64_t>(threadIdx.x) * tiles + blockIdx.x];
+  __syncthreads();
+  const uint64_t tile_base = static_cast<uint64_t>(blockIdx.x) * kSortTile;
+  uint64_t k[kSortItems];
+  uint32_t v[kSortItems];
+  uint32_t rank[kSortItems];
+  uint32_t dig[kSortItems];
+  for (int r = 0; r < kSortItems; ++r) {
+    uint64_t j = tile_base + wave * (kSortItems * kWave) + r * kWave + lane;
+    bool in = j < n;
+    k[r] = in ? keys_in[j] : 0;
+    v[r] = in ? vals_in[j] : 0;
+    uint32_t d = in ? static_cast<uint32_t>((k[r] >> shift) & 0xFF) : 0u;
+    dig[r] = d;
+    // lanes holding the same digit (only valid lanes participate)
+    uint64_t match = ballot(in);
+    for (int b = 0; b < 8; ++b) {
+      uint64_t bb = ballot(((d >> b) & 1u) != 0);
+      match &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t base = wave_hist[wave][d];
+    uint32_t below = static_cast<uint32_t>(lanes_below(match));
+    rank[r] = base + below;
+    // wave executes in lockstep: all reads above precede this update
+    if (in && below == 0)
+      wave_hist[wave][d] = base + static_cast<uint32_t>(__popcll(match));
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // per-digit offsets: tile exclusive scan over digits, then waves
+  {
+    const int d = threadIdx.x;
+    uint32_t tot = 0;
+    uint32_t pw[kSortWaves];
+    for (int w = 0; w < kSortWaves; ++w) {
+      pw[w] = tot;
+      tot += wave_hist[w][d];
+    }
+    __syncthreads();
+    for (int w = 0; w < kSortWaves; ++w)
+      wave_hist[w][d] = pw[w];
+    // exclusive scan of tot over digits (256 threads = 4 waves)
+    uint32_t x = tot;
+    for (int o = 1; o < kWave; o <<= 1) {
+      uint32_t y = __shfl_up(x, o, kWave);
+      if (lane >= o)
+        x += y;
+    }
+    __shared__ uint32_t wsum[kSortWaves];
+    if (lane == kWave - 1)
+      wsum[wave] = x;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int w = 0; w < wave; ++w)
+      add += wsum[w];
+    tile_off[d] = x - tot + add;
+  }
+  __syncthreads();
+  for (int r = 0; r < kSortItems; ++r) {
+    uint64_t j = tile_base + wave * (kSortItems * kWave) + r * kWave + lane;
+    if (j < n) {
+      uint32_t p = tile_off[dig[r]] + wave_hist[wave][dig[r]] + rank[r];
+      s_keys[p] = k[r];
+      s_vals[p] = v[r];
+    }
+  }
+  __syncthreads();
+  uint64_t count = n - tile_base < kSortTile ? n - tile_base : kSortTile;
+  for (int i = threadIdx.x; i < static_cast<int>(count); i += kSortThreads) {
+    uint64_t key = s_keys[i];
+    uint32_t d = static_cast<uint32_t>((key >> shift) & 0xFF);
+    uint64_t dst = g_off[d] + (i - tile_off[d]);
+    keys_out[dst] = key;
+    vals_out[dst] = s_vals[i];
+  }
+}
+
+}  // namespace
+
+void exclusive_scan_u64(Session *s, const uint64_t *in, uint64_t *out, uint64_t n) {
+  exclusive_scan_impl<uint64_t>(s, in, out, n, kSlotScan);
+}
+void exclusive_scan_u32(Session *s, const uint32_t *in, uint64_t *out, uint64_t n) {
+  exclusive_scan_impl<uint32_t>(s, in, out, n, kSlotScan);
+}
+
+void radix_sort_pairs_u64(Session *s, const uint64_t *keys_in, const uint32_t *vals_in,
+                          uint64_t *keys_out, uint32_t *vals_out, uint64_t n, int bits) {
+  if (n == 0)
+    return;
+  unsigned tiles = blocks_for(n, kSortTile);
+  uint64_t *k2 = scratch_as<uint64_t>(s, kSlotSort0, n);
+  uint32_t *v2 = scratch_as<uint32_t>(s, kSlotSort1, n);
+  uint32_t *hist = scratch_as<uint32_t>(s, kSlotSortHist, static_cast<size_t>(tiles) * kRadix);
+  uint64_t *off = scratch_as<uint64_t>(s, kSlotSort2, static_cast<size_t>(tiles) * kRadix + 1);
+  int passes = (bits + 7) / 8;
+  // ping-pong so that the last pass writes keys_out/vals_out
+  const uint64_t *ksrc = keys_in;
+  const uint32_t *vsrc = vals_in;
+  for (int p = 0; p < passes; ++p) {
+    bool last_to_out = ((passes - 1 - p) % 2) == 0;
+    uint64_t *kdst = last_to_out ? keys_out : k2;
+    uint32_t *vdst = last_to_out ? vals_out : v2;
+    int shift = 8 * p;
+    MVS_LAUNCH(s, "radix_hist", k_radix_hist, tiles, kSortThreads, 0, ksrc, n, shift, tiles,
+               hist);
+    exclusive_scan_u32(s, hist, off, static_cast<uint64_t>(tiles) * kRadix);
+    MVS_LAUNCH(s, "radix_scatter", k_radix_scatter, tiles, kSortThreads, 0, ksrc, vsrc, n, shift,
+               tiles, off, kdst, vdst);
+    ksrc = kdst;
+    vsrc = vdst;
+  }
+}
+
+}  // namespace mvs

@@ -1,0 +1,193 @@
+// Session lifetime, error transport, scratch arenas, stream and timing.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "session.hpp"
+
+namespace mvs {
+
+namespace {
+thread_local std::string g_error;
+thread_local int g_code = MVS_OK;
+}  // namespace
+
+void set_error(int code, const char *msg) {
+  g_code = code;
+  g_error = msg;
+}
+
+Session *as_session(mvs_session *s) {
+  if (!s)
+    raise(MVS_EINVAL, "null session");
+  return reinterpret_cast<Session *>(s);
+}
+
+hipStream_t session_stream(Session *s) { return s->stream; }
+
+void *scratch(Session *s, int slot, size_t bytes) {
+  auto &b = s->slots[slot];
+  if ((bytes ? bytes : 1) > b.n) {
+    // queued work may still read the old allocation
+    MVS_HIP(hipStreamSynchronize(s->stream));
+    b.alloc(bytes ? bytes : 1);
+  }
+  return b.ptr;
+}
+
+void *host_stage(Session *s, size_t bytes) {
+  if (bytes > s->stage_bytes) {
+    if (s->stage)
+      (void)hipHostFree(s->stage);
+    s->stage = nullptr;
+    s->stage_bytes = 0;
+    MVS_HIP(hipHostMalloc(&s->stage, bytes, hipHostMallocDefault));
+    s->stage_bytes = bytes;
+  }
+  return s->stage;
+}
+
+void copy_h2d(Session *s, void *dst, const void *src, size_t bytes) {
+  if (bytes)
+    MVS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream));
+}
+void copy_d2h(Session *s, void *dst, const void *src, size_t bytes) {
+  if (!bytes)
+    return;
+  MVS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream));
+  MVS_HIP(hipStreamSynchronize(s->stream));
+}
+void sync(Session *s) { MVS_HIP(hipStreamSynchronize(s->stream)); }
+
+void Session::begin(const char *name) {
+  if (!timing)
+    return;
+  hipEvent_t a, b;
+  if (event_pool.size() >= 2) {
+    a = event_pool.back();
+    event_pool.pop_back();
+    b = event_pool.back();
+    event_pool.pop_back();
+  } else {
+    MVS_HIP(hipEventCreate(&a));
+    MVS_HIP(hipEventCreate(&b));
+  }
+  MVS_HIP(hipEventRecord(a, stream));
+  pending.push_back({name, a, b});
+}
+void Session::end() {
+  if (!timing || pending.empty())
+    return;
+  MVS_HIP(hipEventRecord(pending.back().stop, stream));
+}
+void Session::collect() {
+  if (pending.empty())
+    return;
+  MVS_HIP(hipStreamSynchronize(stream));
+  for (auto &p : pending) {
+    float ms = 0;
+    MVS_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
+    auto &acc = timings[p.name];
+    acc.first += ms;
+    acc.second += 1;
+    event_pool.push_back(p.start);
+    event_pool.push_back(p.stop);
+  }
+  pending.clear();
+}
+void Session::check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    raise(MVS_EHIP, std::string("HIP: launch ") + what + ": " + hipGetErrorString(e));
+}
+
+Session::~Session() {
+  for (auto &p : pending) {
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  for (auto e : event_pool)
+    (void)hipEventDestroy(e);
+  if (stage)
+    (void)hipHostFree(stage);
+  if (own_stream)
+    (void)hipStreamDestroy(own_stream);
+}
+
+}  // namespace mvs
+
+using namespace mvs;
+
+extern "C" {
+
+const char *mvs_last_error(void) { return g_error.c_str(); }
+
+int mvs_session_create(int device, mvs_session **out) {
+  return guarded([&] {
+    if (!out)
+      raise(MVS_EINVAL, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+      raise(MVS_ENODEV, "HIP: no available device");
+    if (device < 0 || device >= count)
+      raise(MVS_EINVAL, "device ordinal out of range");
+    MVS_HIP(hipSetDevice(device));
+    auto *s = new Session();
+    s->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete s;
+      hip_check(e, "hipStreamCreateWithFlags");
+    }
+    s->stream = s->own_stream;
+    *out = reinterpret_cast<mvs_session *>(s);
+  });
+}
+
+void mvs_session_destroy(mvs_session *s) {
+  if (!s)
+    return;
+  auto *p = reinterpret_cast<Session *>(s);
+  (void)hipSetDevice(p->device);
+  (void)hipStreamSynchronize(p->stream);
+  delete p;
+}
+
+int mvs_session_set_stream(mvs_session *s, void *stream) {
+  return guarded([&] {
+    Session *p = as_session(s);
+    p->stream = stream ? static_cast<hipStream_t>(stream) : p->own_stream;
+  });
+}
+
+int mvs_session_sync(mvs_session *s) {
+  return guarded([&] { sync(as_session(s)); });
+}
+
+int mvs_session_enable_timing(mvs_session *s, int enable) {
+  return guarded([&] {
+    Session *p = as_session(s);
+    p->collect();
+    p->timing = enable != 0;
+    p->timings.clear();
+  });
+}
+
+int mvs_dev_timing(mvs_session *s, const char *kernel, double *ms, uint64_t *launches) {
+  return guarded([&] {
+    Session *p = as_session(s);
+    p->collect();
+    auto it = p->timings.find(kernel ? kernel : "");
+    if (ms)
+      *ms = it == p->timings.end() ? 0.0 : it->second.first;
+    if (launches)
+      *launches = it == p->timings.end() ? 0 : it->second.second;
+  });
+}
+
+}  // extern "C"

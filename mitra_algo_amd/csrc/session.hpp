@@ -1,0 +1,74 @@
+// Session state shared by the stage translation units.
+#pragma once
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace mvs {
+
+struct PendingTiming {
+  const char *name;
+  hipEvent_t start, stop;
+};
+
+// Device-resident plan kept between the count and fill phases of pairing.
+struct PairPlan {
+  uint64_t S = 0;          // selected samples
+  uint64_t P = 0;          // pairs
+  uint64_t long_sources = 0;
+  double threshold = 0, box = 0;
+  int precision = MVS_FP64;
+  const int32_t *selected = nullptr;  // caller's device list (mesh ids)
+  bool valid = false;
+};
+
+struct VotePlan {
+  uint64_t P = 0, R = 0;  // pairs and reflection votes
+  int precision = MVS_FP64;
+  bool valid = false;
+};
+
+struct Session {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  DBuf<unsigned char> slots[kSlotCount];
+  void *stage = nullptr;
+  size_t stage_bytes = 0;
+  bool timing = false;
+  std::vector<PendingTiming> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, std::pair<double, uint64_t>> timings;
+  PairPlan pair_plan;
+  VotePlan vote_plan;
+  // host-layer results awaiting fetch
+  DBuf<int32_t> out_a, out_b, out_pair;
+  DBuf<uint8_t> out_kind;
+  DBuf<double> out_lanes;
+  uint64_t out_count = 0;
+  // clustering results awaiting fetch
+  std::vector<uint8_t> cl_refl;
+  std::vector<double> cl_rot, cl_trans, cl_pn, cl_po;
+  std::vector<uint64_t> cl_off;
+  std::vector<int32_t> cl_members;
+
+  void begin(const char *name);
+  void end();
+  void collect();
+  void check_launch(const char *what);
+  ~Session();
+};
+
+// Launch helper: records per-kernel HIP events when timing is on.
+#define MVS_LAUNCH(sess, name, kernel, grid, block, lds, ...)                    \
+  do {                                                                          \
+    (sess)->begin(name);                                                         \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, (sess)->stream,     \
+                       __VA_ARGS__);                                             \
+    (sess)->check_launch(name);                                                  \
+    (sess)->end();                                                               \
+  } while (0)
+
+}  // namespace mvs

@@ -1,0 +1,115 @@
+"""Device-resident workloads for the voting core (used by bench.py and the
+GPU tests).  A workload owns one libmvs session and torch device tensors
+holding the mesh, its topology, the features and the selected samples, so a
+timed pass starts with every input already in HBM.
+
+Selection follows the reference's select (src/cpu/selection.cpp:15-29) with
+the sampling replaced as the workload specifies: `samples` uniform vertex ids
+without replacement (seeded), optionally the magnitude drop of
+meshgen.magnitude_keep, then the reference's anisotropy prune (.85).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import capi, meshgen
+
+
+@dataclass
+class PairVoteResult:
+    P: int
+    R: int
+    a: torch.Tensor
+    b: torch.Tensor
+    q4: torch.Tensor
+    t3: torch.Tensor
+    rp: torch.Tensor
+    n3: torch.Tensor
+    d1: torch.Tensor
+    fp: torch.Tensor
+
+    @property
+    def V(self) -> int:
+        return 2 * self.P + self.R
+
+
+class Workload:
+    def __init__(self, mesh: meshgen.HeightField, samples: int, threshold: float,
+                 magnitude_drop: bool, sample_seed: int = 3, smoothing: int = 3,
+                 device: str = "cuda:0"):
+        self.mesh = mesh
+        self.threshold = threshold
+        self.dev = torch.device(device)
+        self.sess = capi.Session(self.dev.index or 0)
+        self.stream = torch.cuda.current_stream(self.dev)
+        self.sess.set_stream(self.stream.cuda_stream)
+        v = mesh.vertices
+        self.topo = capi.prepare(v, mesh.faces)
+        self.box = float(self.topo["box_size"])
+        self.center = np.array(self.topo["center"])
+        n = len(v)
+        self.n = n
+        t = lambda a, dt=None: torch.as_tensor(np.ascontiguousarray(a)).to(self.dev)
+        self.x = t(v[:, 0])
+        self.y = t(v[:, 1])
+        self.z = t(v[:, 2])
+        self.normals = t(self.topo["normals"])
+        self.face_normals = t(self.topo["face_normals"])
+        self.nbr_off = t(self.topo["nbr_off"].view(np.int64))
+        self.nbr_idx = t(self.topo["nbr_idx"])
+        self.edges = t(self.topo["edges"])
+        self.feat = torch.empty((11, n), dtype=torch.float64, device=self.dev)
+        self.valid = torch.empty(n, dtype=torch.uint8, device=self.dev)
+        self.smoothing = smoothing
+        self.curvature()
+        torch.cuda.synchronize(self.dev)
+        feat = self.feat.cpu().numpy()
+        valid = self.valid.cpu().numpy()
+        e = self.topo["edges"]
+        self.mean_edge = float(np.mean(np.linalg.norm(v[e[:, 1]] - v[e[:, 0]], axis=1)))
+        s = meshgen.uniform_samples(samples, n, sample_seed)
+        if magnitude_drop:
+            s = meshgen.magnitude_keep(s, feat[9], feat[10], self.mean_edge)
+        self.selected_np = self.sess.prune(feat, valid, s, 0.85)
+        self.selected = torch.as_tensor(self.selected_np).to(self.dev)
+        self.S = len(self.selected_np)
+        self.feat_np = feat
+        self.valid_np = valid
+
+    def curvature(self):
+        nf = self.face_normals.shape[1]
+        self.sess.dev_curvature(self.n, self.x, self.y, self.z, self.normals, nf,
+                                self.face_normals, self.nbr_off, self.nbr_idx,
+                                self.edges.shape[0], self.edges, self.smoothing, self.feat,
+                                self.valid)
+
+    def pair_votes(self, precision: int, max_pairs: int = 2**31 - 1) -> PairVoteResult:
+        s = self.sess
+        P = s.dev_pair_count(self.n, self.x, self.y, self.z, self.feat, self.S, self.selected,
+                             self.box, self.threshold, max_pairs, precision)
+        a = torch.empty(max(P, 1), dtype=torch.int32, device=self.dev)
+        b = torch.empty(max(P, 1), dtype=torch.int32, device=self.dev)
+        s.dev_pair_fill(a, b, True)
+        R = s.dev_vote_count(P, a, b, precision)
+        dt = torch.float32 if precision == capi.FP32_GUARDED else torch.float64
+        q4 = torch.empty((4, max(2 * P, 1)), dtype=dt, device=self.dev)
+        t3 = torch.empty((3, max(2 * P, 1)), dtype=dt, device=self.dev)
+        rp = torch.empty(max(2 * P, 1), dtype=torch.int32, device=self.dev)
+        n3 = torch.empty((3, max(R, 1)), dtype=dt, device=self.dev)
+        d1 = torch.empty(max(R, 1), dtype=dt, device=self.dev)
+        fp = torch.empty(max(R, 1), dtype=torch.int32, device=self.dev)
+        s.dev_vote_emit(P, a, b, self.box, self.center, precision, q4, t3, rp, n3, d1, fp)
+        return PairVoteResult(P, R, a, b, q4, t3, rp, n3, d1, fp)
+
+
+def cfg1(device: str = "cuda:0") -> Workload:
+    """96x96 height field, 2,000 uniform samples, magnitude drop, t = .05."""
+    return Workload(meshgen.cfg1_mesh(), 2000, 0.05, True, device=device)
+
+
+def cfg2(device: str = "cuda:0", n: int = 1024) -> Workload:
+    """1024x1024 height field (2.09M triangles), 100,000 uniform samples, t = .02."""
+    return Workload(meshgen.cfg2_mesh(n), 100000, 0.02, False, device=device)

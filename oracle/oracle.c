@@ -1,0 +1,906 @@
+/*
+ * CPU oracle (test infrastructure only; see oracle.h for the pinning list).
+ * Plain C11, FP64, single thread.  Built with -ffp-contract=off so that every
+ * product and sum rounds as written, like the reference CPU build.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static _Thread_local char g_error[256];
+const char *oracle_last_error(void) { return g_error; }
+static int fail(int code, const char *msg) {
+  snprintf(g_error, sizeof g_error, "%s", msg);
+  return code;
+}
+
+/* ---------------------------------------------------------------- math ---
+ * src/core/math.cpp:9-21, 23-38, 113-119 */
+typedef struct {
+  double x, y, z;
+} vec3;
+static vec3 vadd(vec3 a, vec3 b) { return (vec3){a.x + b.x, a.y + b.y, a.z + b.z}; }
+static vec3 vsub(vec3 a, vec3 b) { return (vec3){a.x - b.x, a.y - b.y, a.z - b.z}; }
+static vec3 vscale(vec3 a, double s) { return (vec3){a.x * s, a.y * s, a.z * s}; }
+/* math.cpp:12: division by a scalar is a reciprocal multiply. */
+static vec3 vdiv(vec3 a, double s) { return vscale(a, 1 / s); }
+static double vdot(vec3 a, vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static vec3 vcross(vec3 a, vec3 b) {
+  return (vec3){a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+static double vnorm(vec3 a) { return sqrt(vdot(a, a)); }
+static vec3 vunit(vec3 a) {
+  double n = vnorm(a);
+  return n > 1e-14 ? vdiv(a, n) : (vec3){0, 0, 0};
+}
+static double vget(vec3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+/* row-major 3x3 */
+typedef struct {
+  double m[9];
+} mat3;
+static vec3 mapply(const mat3 *m, vec3 v) {
+  return (vec3){m->m[0] * v.x + m->m[1] * v.y + m->m[2] * v.z,
+                m->m[3] * v.x + m->m[4] * v.y + m->m[5] * v.z,
+                m->m[6] * v.x + m->m[7] * v.y + m->m[8] * v.z};
+}
+static mat3 mtranspose(const mat3 *a) {
+  mat3 r = {{a->m[0], a->m[3], a->m[6], a->m[1], a->m[4], a->m[7], a->m[2],
+             a->m[5], a->m[8]}};
+  return r;
+}
+/* math.cpp:31-38: r starts at zero and accumulates k = 0, 1, 2. */
+static mat3 mmul(const mat3 *a, const mat3 *b) {
+  mat3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k)
+        s += a->m[i * 3 + k] * b->m[k * 3 + j];
+      r.m[i * 3 + j] = s;
+    }
+  return r;
+}
+/* math.cpp:113-119 */
+static mat3 mreflection(vec3 n) {
+  mat3 r = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      r.m[i * 3 + j] -= 2 * vget(n, i) * vget(n, j);
+  return r;
+}
+/* transformations.cpp:14-16 / clustering.cpp:21-23: a, b, c are columns. */
+static mat3 mframe(vec3 a, vec3 b, vec3 c) {
+  mat3 r = {{a.x, b.x, c.x, a.y, b.y, c.y, a.z, b.z, c.z}};
+  return r;
+}
+
+/* ------------------------------------------------------------- mt19937_64 ---
+ * The engine the C++ standard specifies ([rand.predef]); state layout and
+ * refill schedule follow libstdc++ (x[312], index p, refill when p >= 312). */
+#define MT_N 312
+#define MT_M 156
+void or_mt64_seed(or_mt64 *g, uint64_t seed) {
+  g->x[0] = seed;
+  for (uint64_t i = 1; i < MT_N; ++i)
+    g->x[i] = 6364136223846793005ULL * (g->x[i - 1] ^ (g->x[i - 1] >> 62)) + i;
+  g->p = MT_N;
+}
+static void mt_refill(or_mt64 *g) {
+  const uint64_t upper = ~0ULL << 31, lower = ~upper, a = 0xB5026F5AA96619E9ULL;
+  uint64_t *x = g->x;
+  for (int k = 0; k < MT_N; ++k) {
+    uint64_t y = (x[k] & upper) | (x[(k + 1) % MT_N] & lower);
+    x[k] = x[(k + MT_M) % MT_N] ^ (y >> 1) ^ ((y & 1) ? a : 0);
+  }
+  g->p = 0;
+}
+uint64_t or_mt64_next(or_mt64 *g) {
+  if (g->p >= MT_N)
+    mt_refill(g);
+  uint64_t z = g->x[g->p++];
+  z ^= (z >> 29) & 0x5555555555555555ULL;
+  z ^= (z << 17) & 0x71D67FFFEDA60000ULL;
+  z ^= (z << 37) & 0xFFF7EEE000000000ULL;
+  z ^= z >> 43;
+  return z;
+}
+
+/* ------------------------------------------------------------- prepare ---
+ * src/core/prepare.cpp:13-58 (topology) and :102-149 (prepare). */
+typedef struct {
+  int32_t lo, hi, face, a, b, order;
+} edge_entry;
+static int cmp_edge_entry(const void *pa, const void *pb) {
+  const edge_entry *a = pa, *b = pb;
+  if (a->lo != b->lo)
+    return a->lo < b->lo ? -1 : 1;
+  if (a->hi != b->hi)
+    return a->hi < b->hi ? -1 : 1;
+  return a->order < b->order ? -1 : (a->order > b->order);
+}
+static int cmp_i32(const void *pa, const void *pb) {
+  int32_t a = *(const int32_t *)pa, b = *(const int32_t *)pb;
+  return a < b ? -1 : (a > b);
+}
+
+int oracle_prepare(int64_t nv, const double *x, const double *y, const double *z,
+                   int64_t nf, const int32_t *f0, const int32_t *f1, const int32_t *f2,
+                   or_prepared *o) {
+  if (nv <= 0 || nf <= 0)
+    return fail(OR_ERUNTIME, "Mesh is empty");
+  if (nv > 2147483647LL || nf > 2147483647LL)
+    return fail(OR_ERUNTIME, "Mesh exceeds 32-bit index capacity");
+  vec3 lo = {x[0], y[0], z[0]}, hi = lo;
+  for (int64_t i = 0; i < nv; ++i) {
+    double v[3] = {x[i], y[i], z[i]};
+    double *l = &lo.x, *h = &hi.x;
+    for (int k = 0; k < 3; ++k) {
+      if (!isfinite(v[k]))
+        return fail(OR_ERUNTIME, "Non-finite coordinate");
+      /* std::min / std::max */
+      l[k] = v[k] < l[k] ? v[k] : l[k];
+      h[k] = h[k] < v[k] ? v[k] : h[k];
+    }
+  }
+  o->box_size = vnorm(vsub(hi, lo));
+  if (o->box_size < 1e-14)
+    return fail(OR_ERUNTIME, "Mesh has zero extent");
+  o->rejected_faces = o->nonmanifold_edges = o->orientation_conflicts = 0;
+  int64_t F = 0;
+  for (int64_t i = 0; i < nf; ++i) {
+    int32_t f[3] = {f0[i], f1[i], f2[i]};
+    for (int k = 0; k < 3; ++k)
+      if (f[k] < 0 || f[k] >= nv)
+        return fail(OR_ERUNTIME, "Face index out of range");
+    vec3 a = {x[f[0]], y[f[0]], z[f[0]]}, b = {x[f[1]], y[f[1]], z[f[1]]},
+         c = {x[f[2]], y[f[2]], z[f[2]]};
+    if (vnorm(vcross(vsub(b, a), vsub(c, a))) <= 1e-14 * o->box_size * o->box_size) {
+      ++o->rejected_faces;
+      continue;
+    }
+    o->f0[F] = f[0];
+    o->f1[F] = f[1];
+    o->f2[F] = f[2];
+    ++F;
+  }
+  o->face_count = F;
+  if (!F)
+    return fail(OR_ERUNTIME, "No nondegenerate triangles");
+
+  /* topology(): normals and weights accumulate in face order, corner order. */
+  for (int64_t i = 0; i < nv; ++i) {
+    o->nx[i] = o->ny[i] = o->nz[i] = 0;
+    o->weights[i] = 0;
+  }
+  int32_t *ring_count = calloc((size_t)nv, sizeof(int32_t));
+  int32_t *inc_count = calloc((size_t)nv, sizeof(int32_t));
+  edge_entry *ent = malloc((size_t)(3 * F) * sizeof(edge_entry));
+  for (int64_t i = 0; i < F; ++i) {
+    int32_t f[3] = {o->f0[i], o->f1[i], o->f2[i]};
+    vec3 p0 = {x[f[0]], y[f[0]], z[f[0]]}, p1 = {x[f[1]], y[f[1]], z[f[1]]},
+         p2 = {x[f[2]], y[f[2]], z[f[2]]};
+    vec3 n = vcross(vsub(p1, p0), vsub(p2, p0));
+    double area = vnorm(n) / 2;
+    n = vunit(n);
+    o->fnx[i] = n.x;
+    o->fny[i] = n.y;
+    o->fnz[i] = n.z;
+    for (int k = 0; k < 3; ++k) {
+      int32_t a = f[k], b = f[(k + 1) % 3];
+      vec3 acc = {o->nx[a], o->ny[a], o->nz[a]};
+      acc = vadd(acc, n);
+      o->nx[a] = acc.x;
+      o->ny[a] = acc.y;
+      o->nz[a] = acc.z;
+      o->weights[a] += area / 3;
+      ring_count[a]++;
+      ring_count[b]++;
+      inc_count[a]++;
+      edge_entry e;
+      e.lo = a < b ? a : b;
+      e.hi = a < b ? b : a;
+      e.face = (int32_t)i;
+      e.a = a;
+      e.b = b;
+      e.order = (int32_t)(3 * i + k);
+      ent[3 * i + k] = e;
+    }
+  }
+  /* ring rows (with duplicates) then sort + unique per row */
+  uint64_t *roff = malloc((size_t)(nv + 1) * sizeof(uint64_t));
+  roff[0] = 0;
+  for (int64_t i = 0; i < nv; ++i)
+    roff[i + 1] = roff[i] + (uint64_t)ring_count[i];
+  int32_t *rbuf = malloc((size_t)roff[nv] * sizeof(int32_t) + 1);
+  int32_t *rfill = calloc((size_t)nv, sizeof(int32_t));
+  o->inc_off[0] = 0;
+  for (int64_t i = 0; i < nv; ++i)
+    o->inc_off[i + 1] = o->inc_off[i] + (uint64_t)inc_count[i];
+  int32_t *ifill = calloc((size_t)nv, sizeof(int32_t));
+  for (int64_t i = 0; i < F; ++i) {
+    int32_t f[3] = {o->f0[i], o->f1[i], o->f2[i]};
+    for (int k = 0; k < 3; ++k) {
+      int32_t a = f[k], b = f[(k + 1) % 3];
+      rbuf[roff[a] + (uint64_t)rfill[a]++] = b;
+      rbuf[roff[b] + (uint64_t)rfill[b]++] = a;
+      o->inc_idx[o->inc_off[a] + (uint64_t)ifill[a]++] = (int32_t)i;
+    }
+  }
+  o->inc_nnz = (int64_t)o->inc_off[nv];
+  int64_t nnz = 0;
+  o->nbr_off[0] = 0;
+  for (int64_t i = 0; i < nv; ++i) {
+    vec3 acc = {o->nx[i], o->ny[i], o->nz[i]};
+    acc = vunit(acc);
+    o->nx[i] = acc.x;
+    o->ny[i] = acc.y;
+    o->nz[i] = acc.z;
+    int32_t *row = rbuf + roff[i];
+    int32_t len = ring_count[i];
+    qsort(row, (size_t)len, sizeof(int32_t), cmp_i32);
+    for (int32_t k = 0; k < len; ++k)
+      if (k == 0 || row[k] != row[k - 1])
+        o->nbr_idx[nnz++] = row[k];
+    o->nbr_off[i + 1] = (uint64_t)nnz;
+  }
+  o->nbr_nnz = nnz;
+  /* edges in (lo, hi) order; entries of one edge keep face/corner order */
+  qsort(ent, (size_t)(3 * F), sizeof(edge_entry), cmp_edge_entry);
+  int64_t E = 0;
+  for (int64_t s = 0; s < 3 * F;) {
+    int64_t t = s + 1;
+    while (t < 3 * F && ent[t].lo == ent[s].lo && ent[t].hi == ent[s].hi)
+      ++t;
+    int64_t size = t - s;
+    if (size > 2)
+      ++o->nonmanifold_edges;
+    if (size == 2 && ent[s].a != ent[s + 1].b)
+      ++o->orientation_conflicts;
+    or_edge e = {ent[s].lo, ent[s].hi, ent[s].a, ent[s].b, ent[s].face,
+                 size == 2 ? ent[s + 1].face : -1};
+    o->edges[E++] = e;
+    s = t;
+  }
+  o->edge_count = E;
+  free(ring_count);
+  free(inc_count);
+  free(ent);
+  free(roff);
+  free(rbuf);
+  free(rfill);
+  free(ifill);
+  if (o->nonmanifold_edges || o->orientation_conflicts)
+    return fail(OR_ERUNTIME, "Mesh must have manifold edges and consistent "
+                             "face winding; repair input mesh first");
+  /* prepare.cpp:144-147: center = (sum of vertices) / N, reciprocal multiply */
+  vec3 c = {0, 0, 0};
+  for (int64_t i = 0; i < nv; ++i)
+    c = vadd(c, (vec3){x[i], y[i], z[i]});
+  c = vdiv(c, (double)nv);
+  o->center[0] = c.x;
+  o->center[1] = c.y;
+  o->center[2] = c.z;
+  return OR_OK;
+}
+
+/* ----------------------------------------------------------- curvature ---
+ * src/cpu/curvature.cpp:14-79 */
+int oracle_curvature(int64_t nv, const double *x, const double *y, const double *z,
+                     const double *nx, const double *ny, const double *nz,
+                     const double *fnx, const double *fny, const double *fnz,
+                     const uint64_t *nbr_off, const int32_t *nbr_idx,
+                     int64_t ne, const or_edge *edges, int smoothing,
+                     double *feat, uint8_t *valid) {
+  for (int64_t k = 0; k < 11 * nv; ++k)
+    feat[k] = 0;
+  for (int64_t i = 0; i < nv; ++i)
+    valid[i] = 0;
+  if (nv <= 0)
+    return OR_OK;
+  double *T = calloc((size_t)(9 * nv), sizeof(double));
+  double *N = malloc((size_t)(9 * nv) * sizeof(double));
+  int *count = calloc((size_t)nv, sizeof(int));
+  double mean = 0;
+  for (int64_t e = 0; e < ne; ++e) {
+    vec3 h = {x[edges[e].hi], y[edges[e].hi], z[edges[e].hi]};
+    vec3 l = {x[edges[e].lo], y[edges[e].lo], z[edges[e].lo]};
+    mean += vnorm(vsub(h, l));
+  }
+  mean /= (double)ne;
+  for (int64_t e = 0; e < ne; ++e) {
+    const or_edge *E = &edges[e];
+    if (E->face1 < 0)
+      continue;
+    vec3 d = vsub((vec3){x[E->b], y[E->b], z[E->b]}, (vec3){x[E->a], y[E->a], z[E->a]});
+    double len = vnorm(d);
+    d = vunit(d);
+    vec3 n1 = {fnx[E->face0], fny[E->face0], fnz[E->face0]};
+    vec3 n2 = {fnx[E->face1], fny[E->face1], fnz[E->face1]};
+    double c = vdot(n1, n2);
+    c = c < -1.0 ? -1.0 : (1.0 < c ? 1.0 : c); /* std::clamp */
+    double beta = atan2(vdot(vcross(n1, n2), d), c);
+    int32_t ends[2] = {E->a, E->b};
+    for (int q = 0; q < 2; ++q) {
+      int32_t v = ends[q];
+      ++count[v];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          T[9 * v + i * 3 + j] += vget(d, i) * vget(d, j) * beta * len / mean;
+    }
+  }
+  for (int64_t i = 0; i < nv; ++i) {
+    int c = count[i] > 1 ? count[i] : 1;
+    for (int k = 0; k < 9; ++k)
+      T[9 * i + k] /= c;
+  }
+  for (int it = 0; it < smoothing; ++it) {
+    for (int64_t i = 0; i < nv; ++i) {
+      for (int k = 0; k < 9; ++k)
+        N[9 * i + k] = T[9 * i + k];
+      for (uint64_t j = nbr_off[i]; j < nbr_off[i + 1]; ++j)
+        for (int k = 0; k < 9; ++k)
+          N[9 * i + k] += T[9 * (int64_t)nbr_idx[j] + k];
+      double deg1 = (double)(nbr_off[i + 1] - nbr_off[i] + 1);
+      for (int k = 0; k < 9; ++k)
+        N[9 * i + k] /= deg1;
+    }
+    double *s = T;
+    T = N;
+    N = s;
+  }
+  for (int64_t i = 0; i < nv; ++i) {
+    vec3 n = {nx[i], ny[i], nz[i]};
+    if (vnorm(n) < 0.5 || nbr_off[i + 1] - nbr_off[i] < 3)
+      continue;
+    vec3 axis = fabs(n.x) < 0.8 ? (vec3){1, 0, 0} : (vec3){0, 1, 0};
+    vec3 u = vunit(vcross(n, axis)), v = vcross(n, u);
+    mat3 t;
+    memcpy(t.m, T + 9 * i, sizeof t.m);
+    double a = vdot(u, mapply(&t, u)), b = vdot(u, mapply(&t, v)),
+           c = vdot(v, mapply(&t, v));
+    double angle = 0.5 * atan2(2 * b, a - c);
+    vec3 high = vadd(vscale(u, cos(angle)), vscale(v, sin(angle)));
+    vec3 low = vunit(vcross(high, n));
+    double delta = hypot(a - c, 2 * b);
+    double kmin = (a + c - delta) / 2, kmax = (a + c + delta) / 2;
+    vec3 dmax = vunit(vcross(n, low));
+    double vals[11] = {n.x, n.y, n.z, low.x, low.y, low.z, dmax.x, dmax.y, dmax.z, kmin, kmax};
+    for (int k = 0; k < 11; ++k)
+      feat[k * nv + i] = vals[k];
+    valid[i] = isfinite(kmin) && isfinite(kmax) && delta > 1e-10;
+  }
+  free(T);
+  free(N);
+  free(count);
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------ sampling ---
+ * src/core/sampling.cpp:5-24: exponential keys, sorted by (key, index). */
+typedef struct {
+  double key;
+  int32_t idx;
+} key_entry;
+static int cmp_key(const void *pa, const void *pb) {
+  const key_entry *a = pa, *b = pb;
+  if (a->key < b->key)
+    return -1;
+  if (b->key < a->key)
+    return 1;
+  return a->idx < b->idx ? -1 : (a->idx > b->idx);
+}
+int oracle_sample_vertices(int64_t nv, const double *w, double ratio, or_mt64 *rng,
+                           int32_t *out, int64_t *count) {
+  key_entry *keys = malloc((size_t)(nv ? nv : 1) * sizeof(key_entry));
+  int64_t nk = 0;
+  for (int64_t i = 0; i < nv; ++i)
+    if (w[i] > 0) {
+      double u = ((double)(or_mt64_next(rng) >> 11) + 0.5) / 9007199254740992.0;
+      keys[nk].key = -log(u) / w[i];
+      keys[nk].idx = (int32_t)i;
+      ++nk;
+    }
+  qsort(keys, (size_t)nk, sizeof(key_entry), cmp_key);
+  int64_t target = (int64_t)ceil((double)nv * ratio);
+  if (target > nk)
+    target = nk;
+  for (int64_t i = 0; i < target; ++i)
+    out[i] = keys[i].idx;
+  qsort(out, (size_t)target, sizeof(int32_t), cmp_i32);
+  *count = target;
+  free(keys);
+  return OR_OK;
+}
+
+/* --------------------------------------------------------------- prune ---
+ * src/cpu/selection.cpp:21-27 */
+int oracle_prune(int64_t nv, const double *kmin, const double *kmax,
+                 const uint8_t *valid, int64_t ns, const int32_t *sample,
+                 double prune, int32_t *out, int64_t *count) {
+  int64_t c = 0;
+  for (int64_t s = 0; s < ns; ++s) {
+    int32_t i = sample[s];
+    if (i < 0 || i >= nv)
+      return fail(OR_EINVAL, "sample vertex index out of range");
+    double a = fabs(kmin[i]), b = fabs(kmax[i]);
+    double maximum = a < b ? b : a;
+    if (valid[i] && maximum > 1e-10 && fabs(kmin[i] - kmax[i]) / maximum > 1 - prune)
+      out[c++] = i;
+  }
+  *count = c;
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------- pairing ---
+ * src/cpu/pairing.cpp:14-35 (brute force over ordinals a < b). */
+int oracle_pairing(int64_t nv, const double *x, const double *y, const double *z,
+                   const double *kmin, const double *kmax, int64_t ns,
+                   const int32_t *sel, double box, double t, uint64_t max_pairs,
+                   int64_t cap, int32_t *out_a, int32_t *out_b, int64_t *count) {
+  int64_t c = 0;
+  double dist = 0.03 * box;
+  for (int64_t s = 0; s < ns; ++s)
+    if (sel[s] < 0 || sel[s] >= nv)
+      return fail(OR_EINVAL, "selected vertex index out of range");
+  for (int64_t a = 0; a < ns; ++a) {
+    int32_t i = sel[a];
+    double pk0 = kmin[i], pk1 = kmax[i];
+    vec3 pi = {x[i], y[i], z[i]};
+    for (int64_t b = a + 1; b < ns; ++b) {
+      int32_t j = sel[b];
+      double qk0 = kmin[j], qk1 = kmax[j];
+      /* std::max over an initializer list: first maximal element */
+      double m = fabs(pk0);
+      double cand[4] = {fabs(pk1), fabs(qk0), fabs(qk1), 1e-10};
+      for (int k = 0; k < 4; ++k)
+        if (m < cand[k])
+          m = cand[k];
+      double tol = t * m;
+      if (fabs(pk0 - qk0) > tol || fabs(pk1 - qk1) > tol ||
+          vnorm(vsub(pi, (vec3){x[j], y[j], z[j]})) < dist)
+        continue;
+      if ((uint64_t)c >= max_pairs)
+        return fail(OR_EBUDGET, "Pair budget exceeded; lower --sample-ratio "
+                                "or tighten --pair-threshold");
+      if (c >= cap)
+        return fail(OR_EOVERFLOW, "pair output capacity exceeded");
+      out_a[c] = i;
+      out_b[c] = j;
+      ++c;
+    }
+  }
+  *count = c;
+  return OR_OK;
+}
+
+/* ----------------------------------------------------- transformations ---
+ * src/cpu/transformations.cpp:17-73 */
+static void put_vote(double *lanes, int64_t cap, int64_t row, const mat3 *r, vec3 t,
+                     vec3 n, double d, const double *desc) {
+  for (int k = 0; k < 9; ++k)
+    lanes[k * cap + row] = r->m[k];
+  double v[7] = {t.x, t.y, t.z, n.x, n.y, n.z, d};
+  for (int k = 0; k < 7; ++k)
+    lanes[(9 + k) * cap + row] = v[k];
+  for (int k = 0; k < 12; ++k)
+    lanes[(16 + k) * cap + row] = desc[k];
+}
+int oracle_transformations(int64_t nv, const double *x, const double *y,
+                           const double *z, const double *feat, int64_t np,
+                           const int32_t *pa, const int32_t *pb, double box,
+                           const double *center, uint64_t max_votes, int64_t cap,
+                           int32_t *pair, uint8_t *kind, double *lanes,
+                           int64_t *count) {
+  vec3 ctr = {center[0], center[1], center[2]};
+  int64_t c = 0;
+#define FEAT(lane, i) feat[(int64_t)(lane)*nv + (i)]
+#define FV(lane, i) ((vec3){FEAT(lane, i), FEAT(lane + 1, i), FEAT(lane + 2, i)})
+  for (int64_t i = 0; i < np; ++i) {
+    if (pa[i] < 0 || pb[i] < 0 || pa[i] >= nv || pb[i] >= nv)
+      return fail(OR_EINVAL, "pair vertex index out of range");
+    int32_t ia = pa[i], ib = pb[i];
+    vec3 p = {x[ia], y[ia], z[ia]}, q = {x[ib], y[ib], z[ib]};
+    vec3 an = FV(0, ia), ad = FV(3, ia), ax = FV(6, ia);
+    vec3 bn = FV(0, ib), bd = FV(3, ib), bx = FV(6, ib);
+    vec3 n = vunit(vsub(q, p));
+    int dom = 0;
+    for (int k = 1; k < 3; ++k)
+      if (fabs(vget(n, k)) > fabs(vget(n, dom)))
+        dom = k;
+    if (vget(n, dom) < 0)
+      n = vscale(n, -1);
+    double d = -vdot(n, vdiv(vadd(p, q), 2));
+    mat3 mirror = mreflection(n);
+    if (vdot(mapply(&mirror, an), bn) > 0.75 &&
+        fabs(vdot(mapply(&mirror, ad), bd)) > 0.75) {
+      if ((uint64_t)c >= max_votes)
+        return fail(OR_EBUDGET, "Vote budget exceeded; lower --sample-ratio or "
+                                "tighten --pair-threshold");
+      if (c >= cap)
+        return fail(OR_EOVERFLOW, "vote output capacity exceeded");
+      double desc[12] = {n.x, n.y, n.z, (d + vdot(n, ctr)) / box};
+      put_vote(lanes, cap, c, &mirror, vscale(n, -2 * d), n, d, desc);
+      pair[c] = (int32_t)i;
+      kind[c] = 1;
+      ++c;
+    }
+    mat3 fa = mframe(ad, ax, an), fat = mtranspose(&fa);
+    for (int s = 0; s < 2; ++s) {
+      double sign = s == 0 ? 1.0 : -1.0;
+      mat3 fb = mframe(vscale(bd, sign), vscale(bx, sign), bn);
+      mat3 r = mmul(&fb, &fat);
+      vec3 t = vsub(q, mapply(&r, p));
+      vec3 cen = vdiv(vsub(vadd(mapply(&r, ctr), t), ctr), box);
+      double desc[12];
+      for (int k = 0; k < 9; ++k)
+        desc[k] = r.m[k] / 2;
+      desc[9] = cen.x;
+      desc[10] = cen.y;
+      desc[11] = cen.z;
+      if ((uint64_t)c >= max_votes)
+        return fail(OR_EBUDGET, "Vote budget exceeded; lower --sample-ratio or "
+                                "tighten --pair-threshold");
+      if (c >= cap)
+        return fail(OR_EOVERFLOW, "vote output capacity exceeded");
+      put_vote(lanes, cap, c, &r, t, (vec3){0, 0, 0}, 0, desc);
+      pair[c] = (int32_t)i;
+      kind[c] = 0;
+      ++c;
+    }
+  }
+#undef FV
+#undef FEAT
+  *count = c;
+  return OR_OK;
+}
+
+/* ---------------------------------------------------------- clustering ---
+ * src/cpu/clustering.cpp:24-208 */
+typedef struct {
+  int32_t vote;
+  uint64_t count;
+} counter_entry;
+typedef struct {
+  counter_entry *e;
+  int64_t n, cap;
+} counter;
+typedef struct {
+  double v[12];
+  int dim;
+  int reflection;
+} center_t;
+
+static double ddist(const double *a, const double *b, int dim) {
+  double s = 0;
+  for (int i = 0; i < dim; ++i) {
+    double d = a[i] - b[i];
+    s += d * d;
+  }
+  return s;
+}
+
+typedef struct {
+  int64_t nv;
+  const uint8_t *kind;
+  const double *desc;
+  int32_t *touched;  /* votes in the current trajectory's support */
+  int64_t ntouched;
+  uint64_t *support; /* dense per-vote iteration counts, zero between calls */
+} ms_work;
+
+/* clustering.cpp:156-185 (trajectory) with the evaluator of :186-208. */
+static void trajectory(ms_work *w, double *mean, int dim, int refl, double h2,
+                       int max_iterations, int *converged, int *exhausted) {
+  *converged = *exhausted = 0;
+  double sum[12];
+  for (int it = 0; it < max_iterations; ++it) {
+    double weight = 0;
+    for (int k = 0; k < dim; ++k)
+      sum[k] = 0;
+    for (int64_t j = 0; j < w->nv; ++j) {
+      if ((w->kind[j] != 0) != refl)
+        continue;
+      double s = 0;
+      for (int k = 0; k < dim; ++k) {
+        double d = mean[k] - w->desc[(int64_t)k * w->nv + j];
+        s += d * d;
+      }
+      if (s < h2) {
+        double wt = 1 - s / h2;
+        weight += wt;
+        if (w->support[j]++ == 0)
+          w->touched[w->ntouched++] = (int32_t)j;
+        for (int k = 0; k < dim; ++k)
+          sum[k] += wt * w->desc[(int64_t)k * w->nv + j];
+      }
+    }
+    if (weight <= 1e-14) {
+      *exhausted = 1;
+      break;
+    }
+    for (int k = 0; k < dim; ++k)
+      sum[k] /= weight;
+    double movement = ddist(mean, sum, dim);
+    for (int k = 0; k < dim; ++k)
+      mean[k] = sum[k];
+    if (movement < 1e-6 * h2) {
+      *converged = 1;
+      break;
+    }
+  }
+}
+
+typedef struct {
+  int reflection;
+  double rot[9], trans[3], pn[3], po;
+  int32_t *votes;
+  int64_t nvotes;
+  int64_t order;
+} cluster_out;
+static int cmp_cluster(const void *pa, const void *pb) {
+  const cluster_out *a = pa, *b = pb;
+  if (a->nvotes != b->nvotes)
+    return a->nvotes > b->nvotes ? -1 : 1; /* descending size */
+  return a->order < b->order ? -1 : (a->order > b->order); /* stable */
+}
+
+int oracle_clustering(int64_t nv, const uint8_t *kind, const double *desc,
+                      const double *rotation, double box, const double *center,
+                      double bandwidth, int max_iterations, int max_clusters,
+                      or_mt64 *rng, int32_t *n_clusters, uint8_t *c_refl,
+                      double *c_rot, double *c_trans, double *c_pn, double *c_po,
+                      uint64_t *member_off, int32_t *members, int32_t *limited) {
+  double h = bandwidth, h2 = h * h;
+  int64_t nc = 0, ccap = 16;
+  center_t *centers = malloc((size_t)ccap * sizeof(center_t));
+  counter *counters = malloc((size_t)ccap * sizeof(counter));
+  uint8_t *visited = calloc((size_t)(nv ? nv : 1), 1);
+  int32_t *seeds = malloc((size_t)(nv ? nv : 1) * sizeof(int32_t));
+  ms_work w = {nv, kind, desc, malloc((size_t)(nv ? nv : 1) * sizeof(int32_t)), 0,
+               calloc((size_t)(nv ? nv : 1), sizeof(uint64_t))};
+  uint64_t support_entries = 0;
+  int32_t lim = 0;
+  int rc = OR_OK;
+  for (;;) {
+    int64_t ns = 0;
+    for (int64_t i = 0; i < nv; ++i)
+      if (!visited[i])
+        seeds[ns++] = (int32_t)i;
+    if (!ns)
+      break;
+    int32_t seed = seeds[or_mt64_next(rng) % (uint64_t)ns];
+    int refl = kind[seed] != 0;
+    int dim = refl ? 4 : 12;
+    double mean[12] = {0};
+    for (int k = 0; k < 12; ++k)
+      mean[k] = desc[(int64_t)k * nv + seed];
+    int conv, exh;
+    w.ntouched = 0;
+    trajectory(&w, mean, dim, refl, h2, max_iterations, &conv, &exh);
+    /* support as (vote, count) sorted by vote (std::map order) */
+    qsort(w.touched, (size_t)w.ntouched, sizeof(int32_t), cmp_i32);
+    counter sup = {malloc((size_t)(w.ntouched ? w.ntouched : 1) * sizeof(counter_entry)),
+                   w.ntouched, w.ntouched};
+    for (int64_t k = 0; k < w.ntouched; ++k) {
+      int32_t j = w.touched[k];
+      sup.e[k].vote = j;
+      sup.e[k].count = w.support[j];
+      w.support[j] = 0;
+      visited[j] = 1;
+    }
+    if (exh)
+      visited[seed] = 1;
+    if (!conv)
+      ++lim;
+    int64_t merge = -1;
+    for (int64_t c = 0; c < nc; ++c)
+      if (centers[c].reflection == refl && ddist(mean, centers[c].v, dim) < h2 / 4) {
+        merge = c;
+        break;
+      }
+    if (merge < 0) {
+      if (nc == ccap) {
+        ccap *= 2;
+        centers = realloc(centers, (size_t)ccap * sizeof(center_t));
+        counters = realloc(counters, (size_t)ccap * sizeof(counter));
+      }
+      memset(&centers[nc], 0, sizeof(center_t));
+      for (int k = 0; k < dim; ++k)
+        centers[nc].v[k] = mean[k];
+      centers[nc].dim = dim;
+      centers[nc].reflection = refl;
+      support_entries += (uint64_t)sup.n;
+      counters[nc++] = sup;
+    } else {
+      double old = (double)counters[merge].n, now = (double)sup.n;
+      if (old + now > 0)
+        for (int k = 0; k < dim; ++k)
+          centers[merge].v[k] = (old * centers[merge].v[k] + now * mean[k]) / (old + now);
+      counter *m = &counters[merge];
+      int64_t before = m->n;
+      counter_entry *merged =
+          malloc((size_t)(m->n + sup.n ? m->n + sup.n : 1) * sizeof(counter_entry));
+      int64_t i = 0, j = 0, k = 0;
+      while (i < m->n || j < sup.n) {
+        if (j >= sup.n || (i < m->n && m->e[i].vote < sup.e[j].vote))
+          merged[k++] = m->e[i++];
+        else if (i >= m->n || sup.e[j].vote < m->e[i].vote)
+          merged[k++] = sup.e[j++];
+        else {
+          merged[k] = m->e[i++];
+          merged[k++].count += sup.e[j++].count;
+        }
+      }
+      free(m->e);
+      m->e = merged;
+      m->n = k;
+      m->cap = k;
+      support_entries += (uint64_t)(m->n - before);
+      free(sup.e);
+    }
+    if (support_entries > 5000000) {
+      rc = fail(OR_ERUNTIME, "Mean-shift support budget exceeded; lower --sample-ratio");
+      goto done;
+    }
+    if (nc > 20000) {
+      rc = fail(OR_ERUNTIME, "Too many mean-shift modes; increase "
+                             "--bandwidth or lower --sample-ratio");
+      goto done;
+    }
+  }
+  {
+    /* winners: strictly larger count wins; earlier cluster keeps ties */
+    uint64_t *best = calloc((size_t)(nv ? nv : 1), sizeof(uint64_t));
+    int32_t *winner = malloc((size_t)(nv ? nv : 1) * sizeof(int32_t));
+    for (int64_t i = 0; i < nv; ++i)
+      winner[i] = -1;
+    for (int64_t c = 0; c < nc; ++c)
+      for (int64_t k = 0; k < counters[c].n; ++k) {
+        counter_entry e = counters[c].e[k];
+        if (e.count > best[e.vote]) {
+          best[e.vote] = e.count;
+          winner[e.vote] = (int32_t)c;
+        }
+      }
+    int64_t *msize = calloc((size_t)(nc ? nc : 1), sizeof(int64_t));
+    for (int64_t i = 0; i < nv; ++i)
+      if (winner[i] >= 0)
+        msize[winner[i]]++;
+    cluster_out *outs = malloc((size_t)(nc ? nc : 1) * sizeof(cluster_out));
+    int64_t no = 0;
+    vec3 origin = {center[0], center[1], center[2]};
+    for (int64_t c = 0; c < nc; ++c) {
+      if (!msize[c])
+        continue;
+      cluster_out *o = &outs[no];
+      memset(o, 0, sizeof *o);
+      o->order = no;
+      o->reflection = centers[c].reflection;
+      o->votes = malloc((size_t)msize[c] * sizeof(int32_t));
+      for (int64_t i = 0; i < nv; ++i)
+        if (winner[i] == c)
+          o->votes[o->nvotes++] = (int32_t)i;
+      const double *dsc = centers[c].v;
+      if (o->reflection) {
+        vec3 n = {dsc[0], dsc[1], dsc[2]};
+        double length = vnorm(n);
+        vec3 pn = vunit(n);
+        double po = dsc[3] * box / length - vdot(pn, origin);
+        mat3 r = mreflection(pn);
+        vec3 t = vscale(pn, -2 * po);
+        memcpy(o->rot, r.m, sizeof o->rot);
+        o->trans[0] = t.x;
+        o->trans[1] = t.y;
+        o->trans[2] = t.z;
+        o->pn[0] = pn.x;
+        o->pn[1] = pn.y;
+        o->pn[2] = pn.z;
+        o->po = po;
+      } else {
+        mat3 m;
+        for (int k = 0; k < 9; ++k)
+          m.m[k] = dsc[k] * 2;
+        vec3 c0 = {m.m[0], m.m[3], m.m[6]}, c1 = {m.m[1], m.m[4], m.m[7]};
+        vec3 a = vunit(c0);
+        vec3 b = vunit(vsub(c1, vscale(a, vdot(a, c1))));
+        if (vnorm(a) < 0.5 || vnorm(b) < 0.5) {
+          int64_t v0 = o->votes[0];
+          for (int k = 0; k < 9; ++k)
+            m.m[k] = rotation[(int64_t)k * nv + v0];
+        } else
+          m = mframe(a, b, vcross(a, b));
+        vec3 t = {dsc[9], dsc[10], dsc[11]};
+        vec3 tr = vsub(vadd(vscale(t, box), origin), mapply(&m, origin));
+        memcpy(o->rot, m.m, sizeof o->rot);
+        o->trans[0] = tr.x;
+        o->trans[1] = tr.y;
+        o->trans[2] = tr.z;
+      }
+      ++no;
+    }
+    qsort(outs, (size_t)no, sizeof(cluster_out), cmp_cluster);
+    int64_t keep = no < max_clusters ? no : max_clusters;
+    member_off[0] = 0;
+    for (int64_t c = 0; c < keep; ++c) {
+      cluster_out *o = &outs[c];
+      c_refl[c] = (uint8_t)o->reflection;
+      for (int k = 0; k < 9; ++k)
+        c_rot[9 * c + k] = o->rot[k];
+      for (int k = 0; k < 3; ++k) {
+        c_trans[3 * c + k] = o->trans[k];
+        c_pn[3 * c + k] = o->pn[k];
+      }
+      c_po[c] = o->po;
+      for (int64_t k = 0; k < o->nvotes; ++k)
+        members[member_off[c] + (uint64_t)k] = o->votes[k];
+      member_off[c + 1] = member_off[c] + (uint64_t)o->nvotes;
+    }
+    *n_clusters = (int32_t)keep;
+    *limited = lim;
+    for (int64_t c = 0; c < no; ++c)
+      free(outs[c].votes);
+    free(outs);
+    free(msize);
+    free(best);
+    free(winner);
+  }
+done:
+  for (int64_t c = 0; c < nc; ++c)
+    free(counters[c].e);
+  free(counters);
+  free(centers);
+  free(visited);
+  free(seeds);
+  free(w.touched);
+  free(w.support);
+  return rc;
+}
+
+/* ------------------------------------------------ batched-seed mean-shift ---
+ * No reference counterpart (the reference runs sequential seeds only). */
+int oracle_meanshift_batched(int64_t nv, int dim, const double *pts,
+                             const uint8_t *kind, int dim_rigid, int dim_refl,
+                             int64_t ns, const double *seeds,
+                             const uint8_t *seed_kind, double h, double cutoff,
+                             int iterations, double *out, double *weight) {
+  if (dim < 1 || dim > 12 || dim_rigid > dim || dim_refl > dim)
+    return fail(OR_EINVAL, "bad dimension");
+  double r2 = cutoff * h * cutoff * h, inv = 1 / (2 * h * h);
+  for (int64_t s = 0; s < ns; ++s) {
+    double m[12] = {0}, acc[12];
+    int refl = seed_kind[s] != 0, d = refl ? dim_refl : dim_rigid;
+    for (int k = 0; k < dim; ++k)
+      m[k] = seeds[(int64_t)k * ns + s];
+    double wsum = 0;
+    for (int it = 0; it < iterations; ++it) {
+      wsum = 0;
+      for (int k = 0; k < d; ++k)
+        acc[k] = 0;
+      for (int64_t j = 0; j < nv; ++j) {
+        if ((kind[j] != 0) != refl)
+          continue;
+        double d2 = 0;
+        for (int k = 0; k < d; ++k) {
+          double e = pts[(int64_t)k * nv + j] - m[k];
+          d2 += e * e;
+        }
+        if (d2 < r2) {
+          double wj = exp(-d2 * inv);
+          wsum += wj;
+          for (int k = 0; k < d; ++k)
+            acc[k] += wj * (pts[(int64_t)k * nv + j] - m[k]);
+        }
+      }
+      if (wsum > 0)
+        for (int k = 0; k < d; ++k)
+          m[k] += acc[k] / wsum;
+    }
+    for (int k = 0; k < dim; ++k)
+      out[(int64_t)k * ns + s] = m[k];
+    weight[s] = wsum;
+  }
+  return OR_OK;
+}

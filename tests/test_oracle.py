@@ -1,0 +1,257 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker:
+  * bit-identical to the reference's own CPU sources compiled into
+    oracle/_ref (every stage, on the reference's example meshes, on the
+    reference's test fixtures and on generated height fields);
+  * the results the reference repository holds (tests/golden/known_answers.json);
+  * closed forms and invariants (mt19937_64 known answer, 513*512/2 dense
+    pairs, rotation orthonormality, R p + t = q, planar mesh -> no features).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import OracleError, read_off
+from tests import fixtures as fx
+
+EXAMPLES = ["reflection", "rigid", "partial", "single"]
+
+
+@pytest.fixture(scope="module")
+def known(golden_dir):
+    with open(os.path.join(golden_dir, "known_answers.json")) as f:
+        return json.load(f)
+
+
+def test_mt19937_64_known_answer(oracle, known):
+    g = oracle.rng(5489)
+    for _ in range(9999):
+        oracle.next(g)
+    assert oracle.next(g) == known["mt19937_64_10000th_default_seed"]["value"]
+
+
+def run_chain(lib, v, f, t=0.08, seed=42, bandwidth=0.08):
+    topo = lib.prepare(v, f)
+    feat, valid = lib.curvature(v, topo, 3)
+    g = lib.rng(seed)
+    sample, pruned = lib.select(topo.weights, feat[9], feat[10], valid, 1.0, 0.85, g)
+    pa, pb = lib.pairing(v, feat[9], feat[10], pruned, topo.box_size, t)
+    pair, kind, lanes = lib.transformations(v, feat, pa, pb, topo.box_size, topo.center)
+    cl = lib.clustering(kind, lanes[16:28], lanes[0:9], topo.box_size, topo.center, bandwidth,
+                        100, 20, g)
+    return dict(topo=topo, feat=feat, valid=valid, sample=sample, pruned=pruned, pa=pa, pb=pb,
+                pair=pair, kind=kind, lanes=lanes, cl=cl, rng=g)
+
+
+@pytest.mark.parametrize("name", EXAMPLES)
+def test_end_to_end_examples_match_reference(oracle, ref, golden_dir, known, name):
+    v, f = read_off(os.path.join(golden_dir, f"{name}.off"))
+    a = run_chain(oracle, v, f)
+    b = run_chain(ref, v, f)
+    for k in ("faces", "normals", "face_normals", "weights", "nbr_off", "nbr_idx", "inc_off",
+              "inc_idx", "edges"):
+        assert np.array_equal(getattr(a["topo"], k), getattr(b["topo"], k)), k
+    assert a["topo"].box_size == b["topo"].box_size
+    assert np.array_equal(a["topo"].center, b["topo"].center)
+    for k in ("feat", "valid", "sample", "pruned", "pa", "pb", "pair", "kind", "lanes"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("reflection", "rotation", "translation", "plane_normal", "plane_offset",
+              "member_off", "members"):
+        assert np.array_equal(getattr(a["cl"], k), getattr(b["cl"], k)), k
+    assert a["rng"] == b["rng"]
+    exp = known["end_to_end"][name]
+    assert (len(v), len(f)) == (exp["vertices"], exp["faces"])
+    assert len(a["pruned"]) == exp["pruned"]
+    assert len(a["pa"]) == exp["pairs"]
+    assert len(a["pair"]) == exp["votes"]
+    assert len(a["cl"]) == exp["clusters"]
+
+
+def test_reflection_default_threshold(oracle, golden_dir, known):
+    v, f = read_off(os.path.join(golden_dir, "reflection.off"))
+    topo = oracle.prepare(v, f)
+    feat, valid = oracle.curvature(v, topo, 3)
+    g = oracle.rng(42)
+    _, pruned = oracle.select(topo.weights, feat[9], feat[10], valid, 1.0, 0.85, g)
+    pa, pb = oracle.pairing(v, feat[9], feat[10], pruned, topo.box_size, 0.15)
+    pair, kind, lanes = oracle.transformations(v, feat, pa, pb, topo.box_size, topo.center)
+    exp = known["reflection_default_threshold"]
+    assert (len(pa), len(pair)) == (exp["pairs"], exp["votes"])
+
+
+def _pairs(lib, fix, t, max_pairs=2000000):
+    return lib.pairing(fix.vertices, fix.kmin, fix.kmax, fix.selected, fix.box, t, max_pairs)
+
+
+def test_pairing_known_answers(oracle, known):
+    dense = fx.PairFixture(513)
+    a, _ = _pairs(oracle, dense, 0.15)
+    assert len(a) == known["pairing_dense_513"]["value"] == 513 * 512 // 2
+    _pairs(oracle, dense, 0.15, max_pairs=len(a))  # exact budget passes
+    with pytest.raises(OracleError, match="budget"):
+        _pairs(oracle, dense, 0.15, max_pairs=len(a) - 1)
+    a, _ = _pairs(oracle, fx.pairing_zeros(), 0.15)
+    assert len(a) == known["pairing_signed_zeros"]["value"]
+    a, _ = _pairs(oracle, fx.pairing_rounded(), math.nextafter(1.0, 0.0))
+    assert len(a) == known["pairing_rounded_near_one"]["value"]
+    bench = fx.pairing_benchmark()
+    a, b = _pairs(oracle, bench, 1e-6, max_pairs=600000)
+    assert len(a) == known["pairing_benchmark_pairs"]["value"]
+    # brute force on the closed form: same group and distinct positions
+    k = bench.kmin
+    grp = np.unique(k, return_counts=True)[1]
+    assert len(a) == int(sum(c * (c - 1) // 2 for c in grp))
+
+
+def test_pairing_disjoint_signatures(oracle, known):
+    sp = fx.pairing_sparse(16384)  # same construction; smaller for CPU time
+    a, _ = _pairs(oracle, sp, 1e-10, max_pairs=1)
+    assert len(a) == known["pairing_disjoint_65536"]["value"]
+
+
+def test_pairing_fixtures_match_reference(oracle, ref):
+    cases = [(fx.pairing_random(), t) for t in fx.RANDOM_THRESHOLDS]
+    dup = fx.pairing_random()
+    dup.selected = np.append(dup.selected, dup.selected[0]).astype(np.int32)
+    cases.append((dup, 0.7))
+    cases += [(fx.pairing_boundary(), t) for t in (0.25, 0.5, 1.0)]
+    cases += [(fx.pairing_edge(t), t) for t in fx.EDGE_THRESHOLDS]
+    cases.append((fx.pairing_rounded(), math.nextafter(1.0, 0.0)))
+    cases.append((fx.pairing_zeros(), 0.15))
+    for fix, t in cases:
+        a = _pairs(oracle, fix, t)
+        b = _pairs(ref, fix, t)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), t
+
+
+def _votes(lib, v, feat, pa, pb, box, ctr, max_votes=600000):
+    return lib.transformations(v, feat, pa, pb, box, ctr, max_votes)
+
+
+def test_transformation_fixtures(oracle, ref):
+    v, feat, pa, pb, box, ctr, n, d, x = fx.mirror_fixture()
+    a = _votes(oracle, v, feat, pa, pb, box, ctr)
+    b = _votes(ref, v, feat, pa, pb, box, ctr)
+    assert all(np.array_equal(p, q) for p, q in zip(a, b))
+    assert len(a[0]) == 3 * len(pa)
+    _check_vote_invariants(v, pa, pb, *a)
+    mixed = fx.mixed_features(feat, n)
+    a = _votes(oracle, v, mixed, pa, pb, box, ctr)
+    b = _votes(ref, v, mixed, pa, pb, box, ctr)
+    assert all(np.array_equal(p, q) for p, q in zip(a, b))
+    assert 2 * len(pa) < len(a[0]) < 3 * len(pa)
+    _votes(oracle, v, mixed, pa, pb, box, ctr, max_votes=len(a[0]))
+    with pytest.raises(OracleError, match="budget"):
+        _votes(oracle, v, mixed, pa, pb, box, ctr, max_votes=1)
+    par = fx.parallel_features(len(v), n, d, x)
+    a = _votes(oracle, v, par, pa, pb, box, ctr)
+    assert len(a[0]) == 2 * len(pa)
+    _check_vote_invariants(v, pa, pb, *a)
+    same = np.zeros(1, np.int32)
+    for f in (par, np.zeros_like(par)):
+        a = _votes(oracle, v, f, same, same, box, ctr)
+        b = _votes(ref, v, f, same, same, box, ctr)
+        assert all(np.array_equal(p, q) for p, q in zip(a, b))
+        assert np.all(np.isfinite(a[2]))
+
+
+def _check_vote_invariants(v, pa, pb, pair, kind, lanes, tol=2e-12):
+    R = lanes[0:9].T.reshape(-1, 3, 3)
+    t = lanes[9:12].T
+    for i in range(len(pair)):
+        p, q = v[pa[pair[i]]], v[pb[pair[i]]]
+        assert np.allclose(R[i] @ R[i].T, np.eye(3), atol=tol * 10)
+        assert abs(np.linalg.det(R[i]) - (-1 if kind[i] else 1)) < 1e-10
+        assert np.allclose(R[i] @ p + t[i], q, atol=1e-10)
+        if kind[i]:
+            assert abs(np.linalg.norm(lanes[12:15, i]) - 1) < 1e-12
+            assert np.all(lanes[20:28, i] == 0)
+
+
+def test_select_fixtures(oracle, ref):
+    w, kmin, kmax, valid = fx.select_fixture(1028)
+    for ratio in (1.0, 0.5, 0.001):
+        g1, g2 = oracle.rng(42), ref.rng(42)
+        s1 = oracle.select(w, kmin, kmax, valid, ratio, 0.85, g1)
+        s2 = ref.select(w, kmin, kmax, valid, ratio, 0.85, g2)
+        assert np.array_equal(s1[0], s2[0]) and np.array_equal(s1[1], s2[1])
+        assert g1 == g2
+    w, kmin, kmax, valid = fx.strict_prune_fixture(1028)
+    g1, g2 = oracle.rng(42), ref.rng(42)
+    s1 = oracle.select(w, kmin, kmax, valid, 1.0, 0.75, g1)
+    s2 = ref.select(w, kmin, kmax, valid, 1.0, 0.75, g2)
+    assert np.array_equal(s1[1], s2[1]) and len(s1[1]) == 1028 * 2 // 3
+
+
+@pytest.mark.parametrize("count", [0, 1, 64, 2048])
+def test_clustering_fixtures_match_reference(oracle, ref, known, count):
+    kind, desc, rot = fx.cluster_votes(count)
+    for iters in (1, 100):
+        g1, g2 = oracle.rng(42), ref.rng(42)
+        a = oracle.clustering(kind, desc, rot, 10.0, [0, 0, 0], 0.03, iters, 4, g1)
+        b = ref.clustering(kind, desc, rot, 10.0, [0, 0, 0], 0.03, iters, 4, g2)
+        assert g1 == g2
+        assert a.limited_iterations == b.limited_iterations
+        for k in ("reflection", "rotation", "translation", "plane_normal", "plane_offset",
+                  "member_off", "members"):
+            assert np.array_equal(getattr(a, k), getattr(b, k)), k
+        if count >= 64:
+            assert len(a) == known["clustering_4mode_clusters"]["value"]
+
+
+def test_curvature_invariants_and_reference(oracle, ref):
+    from mitra_algo_amd import meshgen
+    m = meshgen.cfg1_mesh()
+    ta = oracle.prepare(m.vertices, m.faces)
+    for smooth in (0, 3, 7):
+        fa, va = oracle.curvature(m.vertices, ta, smooth)
+        fb, vb = ref.curvature(m.vertices, ta, smooth)
+        assert np.array_equal(fa, fb) and np.array_equal(va, vb)
+    ok = va.astype(bool)
+    n, dmin, dmax = fa[0:3, ok], fa[3:6, ok], fa[6:9, ok]
+    assert np.allclose(np.linalg.norm(dmin, axis=0), 1, atol=1e-12)
+    assert np.allclose(np.linalg.norm(dmax, axis=0), 1, atol=1e-12)
+    assert np.allclose((n * dmin).sum(0), 0, atol=1e-12)
+    assert np.allclose((n * dmax).sum(0), 0, atol=1e-12)
+    assert np.all(fa[9, ok] <= fa[10, ok])
+    # planar mesh: every feature invalid and zero (tests/cuda_curvature.cpp:77-79)
+    v = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], float)
+    f = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    t = oracle.prepare(v, f)
+    feat, valid = oracle.curvature(v, t, 3)
+    assert not valid.any() and np.all(feat[9:11] == 0)
+
+
+def test_cylinder_curvature_is_anisotropic(oracle):
+    """A height field z = sqrt(r^2 - x^2) curves only along x: kmin ~ 0 and
+    the anisotropy |kmin - kmax| / max ~ 1 at interior vertices."""
+    from mitra_algo_amd import meshgen
+    n, r = 40, 2.0
+    ax = np.linspace(-0.5, 0.5, n)
+    X, Y = np.meshgrid(ax, ax, indexing="xy")
+    Z = np.sqrt(r * r - X * X)
+    v = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    f = meshgen.grid_faces(n)
+    t = oracle.prepare(v, f)
+    feat, valid = oracle.curvature(v, t, 0)
+    inner = np.zeros((n, n), bool)
+    inner[3:-3, 3:-3] = True
+    inner = inner.ravel() & valid.astype(bool)
+    k0, k1 = feat[9, inner], feat[10, inner]
+    m = np.maximum(np.abs(k0), np.abs(k1))
+    assert np.median(np.abs(k0 - k1) / m) > 0.95
+
+
+def test_tetrahedron_topology(oracle):
+    """tests/storage_backend.cpp:132-147: tetrahedron rings, incident faces
+    and (lo, hi)-ordered edges."""
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    f = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]], np.int32)
+    t = oracle.prepare(v, f)
+    assert t.nbr_off.tolist() == [0, 3, 6, 9, 12]
+    assert t.nbr_idx.tolist() == [1, 2, 3, 0, 2, 3, 0, 1, 3, 0, 1, 2]
+    assert [tuple(e[:2]) for e in t.edges] == [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    assert np.all(t.edges[:, 5] >= 0)
+    assert t.inc_idx.tolist()[:3] == [0, 1, 2]
