@@ -203,17 +203,24 @@ int mvs_dev_curvature(mvs_session *s, uint64_t n, const double *x, const double 
                       const uint64_t *nbr_off, const int32_t *nbr_idx, uint64_t edges,
                       const mvs_edge *edge, int smoothing, double *feat, uint8_t *valid);
 
-/* Pair + vote planning on device-resident features.  Reads the selected
- * list, sorts the curvature signatures, counts pairs per source and returns
- * the totals (synchronises once to read them).  Budget errors are raised
- * before any output is written. */
+/* Pair planning on device-resident features (feat: 11*n SoA).  Gathers the
+ * selected samples, sorts the curvature signatures, builds the
+ * (kmin bucket, kmax) index, counts pairs per source and returns the total
+ * (synchronises once to read it).  Budget errors are raised before any
+ * output is written.  center is the model centre: the FP32 path works in
+ * centre-relative coordinates.  The selected list must stay valid until
+ * mvs_dev_pair_fill returns. */
 int mvs_dev_pair_count(mvs_session *s, uint64_t n, const double *x, const double *y,
                        const double *z, const double *feat, uint64_t selected_count,
-                       const int32_t *selected, double box, double threshold,
-                       uint64_t max_pairs, int precision, uint64_t *pairs);
+                       const int32_t *selected, double box, const double center[3],
+                       double threshold, uint64_t max_pairs, int precision, uint64_t *pairs);
 /* Writes the planned pairs in CPU order: a/b hold SELECTED ORDINALS when
  * ordinals != 0, else mesh vertex ids.  Capacity: the count returned above. */
 int mvs_dev_pair_fill(mvs_session *s, int32_t *a, int32_t *b, int ordinals);
+/* Statistics of the last pair count: kmin-window candidates, candidates
+ * after the (kmin bucket, kmax) index, FP64 rechecks of the FP32 screen. */
+int mvs_dev_pair_stats(mvs_session *s, uint64_t *window_candidates, uint64_t *index_candidates,
+                       uint64_t *fp64_rechecks);
 
 /* Compact vote emission for pairs given as selected ordinals (a < b).
  * Counts reflection votes (synchronises once) unless max_votes == 0 is
@@ -230,6 +237,25 @@ int mvs_dev_vote_emit(mvs_session *s, uint64_t pair_count, const int32_t *a,
                       const int32_t *b, double box, const double center[3], int precision,
                       void *rigid_q4, void *rigid_t3, int32_t *rigid_pair, void *refl_n3,
                       void *refl_d, int32_t *refl_pair);
+
+/* Batched-seed mean-shift (no reference counterpart; the reference runs
+ * sequential seeds only, src/cpu/clustering.cpp:44-60).
+ * mvs_dev_ms_points converts compact votes (layout of mvs_dev_vote_emit) to
+ * mean-shift points: rigid -> (t_c[3], axis-angle[3]) as a 6*nrigid SoA,
+ * reflection -> (d_c, n[3]) as a 4*nrefl SoA, in the record precision.
+ * mvs_dev_ms_batched runs `iterations` updates of every seed over one point
+ * set (dim 6 or 4, dim*n SoA) with the kernel
+ *   w = exp(-d^2/2h^2) - exp(-cutoff^2/2) for d < cutoff*h, else 0,
+ * x <- x + sum w (p - x) / sum w.  Seeds and out: dim*ns SoA; weight: ns.
+ * MVS_FP32_GUARDED selects FP32 points and accumulation, MVS_FP64 FP64.
+ * candidates / in_support (optional) receive the numbers of point visits and
+ * of visits inside the support (synchronises when requested). */
+int mvs_dev_ms_points(mvs_session *s, int precision, uint64_t nrigid, const void *q4,
+                      const void *t3, uint64_t nrefl, const void *n3, const void *d1,
+                      void *rigid_pts, void *refl_pts);
+int mvs_dev_ms_batched(mvs_session *s, int precision, int dim, uint64_t n, const void *pts,
+                       uint64_t ns, const void *seeds, int iterations, double h, double cutoff,
+                       void *out, void *weight, uint64_t *candidates, uint64_t *in_support);
 
 #ifdef __cplusplus
 }

@@ -22,8 +22,8 @@ EXPORTS = [
     "mvs_session_sync", "mvs_session_enable_timing", "mvs_dev_timing",
     "mvs_curvature", "mvs_prune", "mvs_pairing", "mvs_pairing_fetch", "mvs_transformations",
     "mvs_transformations_fetch", "mvs_clustering", "mvs_clustering_fetch",
-    "mvs_dev_curvature", "mvs_dev_pair_count", "mvs_dev_pair_fill", "mvs_dev_vote_count",
-    "mvs_dev_vote_emit",
+    "mvs_dev_curvature", "mvs_dev_pair_count", "mvs_dev_pair_fill", "mvs_dev_pair_stats",
+    "mvs_dev_vote_count", "mvs_dev_vote_emit", "mvs_dev_ms_points", "mvs_dev_ms_batched",
 ]
 
 
@@ -354,15 +354,38 @@ class Session:
                                       C.c_uint64(nf), ptr(fn3), ptr(off), ptr(idx), C.c_uint64(ne),
                                       ptr(edges), C.c_int(smoothing), ptr(feat), ptr(valid)))
 
-    def dev_pair_count(self, n, x, y, z, feat, S, sel, box, t, max_pairs, precision) -> int:
+    def dev_pair_count(self, n, x, y, z, feat, S, sel, box, center, t, max_pairs,
+                       precision) -> int:
         P = C.c_uint64()
+        ctr = (C.c_double * 3)(*[float(c) for c in center])
         check(lib().mvs_dev_pair_count(self.h, C.c_uint64(n), ptr(x), ptr(y), ptr(z), ptr(feat),
-                                       C.c_uint64(S), ptr(sel), C.c_double(box), C.c_double(t),
-                                       C.c_uint64(max_pairs), C.c_int(precision), C.byref(P)))
+                                       C.c_uint64(S), ptr(sel), C.c_double(box), ctr,
+                                       C.c_double(t), C.c_uint64(max_pairs), C.c_int(precision),
+                                       C.byref(P)))
         return P.value
 
     def dev_pair_fill(self, a, b, ordinals: bool):
         check(lib().mvs_dev_pair_fill(self.h, ptr(a), ptr(b), C.c_int(1 if ordinals else 0)))
+
+    def dev_pair_stats(self):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib().mvs_dev_pair_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(window_candidates=a.value, index_candidates=b.value, fp64_rechecks=c.value)
+
+    def dev_ms_points(self, precision, nrigid, q4, t3, nrefl, n3, d1, rigid_pts, refl_pts):
+        check(lib().mvs_dev_ms_points(self.h, C.c_int(precision), C.c_uint64(nrigid), ptr(q4),
+                                      ptr(t3), C.c_uint64(nrefl), ptr(n3), ptr(d1),
+                                      ptr(rigid_pts), ptr(refl_pts)))
+
+    def dev_ms_batched(self, precision, dim, n, pts, ns, seeds, iterations, h, cutoff, out,
+                       weight, stats: bool = False):
+        cand, ins = C.c_uint64(), C.c_uint64()
+        check(lib().mvs_dev_ms_batched(self.h, C.c_int(precision), C.c_int(dim), C.c_uint64(n),
+                                       ptr(pts), C.c_uint64(ns), ptr(seeds), C.c_int(iterations),
+                                       C.c_double(h), C.c_double(cutoff), ptr(out), ptr(weight),
+                                       C.byref(cand) if stats else None,
+                                       C.byref(ins) if stats else None))
+        return (cand.value, ins.value) if stats else None
 
     def dev_vote_count(self, P, a, b, precision) -> int:
         R = C.c_uint64()

@@ -118,6 +118,9 @@ void exclusive_scan_u32(Session *s, const uint32_t *in, uint64_t *out, uint64_t 
 void radix_sort_pairs_u64(Session *s, const uint64_t *keys_in, const uint32_t *vals_in,
                           uint64_t *keys_out, uint32_t *vals_out, uint64_t n, int bits);
 
+// Zero device memory with a kernel (no memset nodes when a caller captures).
+void zero_device(Session *s, void *ptr, size_t bytes);
+
 // std::mt19937_64 step on a state in libstdc++ layout (sampling.cpp).
 uint64_t mt64_next(mvs_mt64 *g);
 

@@ -230,7 +230,7 @@ static void clustering_run(Session *s, uint64_t V, const uint8_t *kind_h, const 
   uint32_t *tcount = scratch_as<uint32_t>(s, kSlotMs4, V ? V : 1);
   TrajState *st = scratch_as<TrajState>(s, kSlotMs5, 1);
   if (V)
-    MVS_HIP(hipMemsetAsync(support, 0, V * sizeof(uint32_t), s->stream));
+    zero_device(s, support, V * sizeof(uint32_t));
   struct Center {
     double v[12];
     bool refl;

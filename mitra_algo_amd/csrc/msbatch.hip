@@ -1,0 +1,355 @@
+// Batched-seed mean-shift over compact votes (no reference counterpart: the
+// reference only has sequential seeds, src/cpu/clustering.cpp:44-60).
+//
+// Points: rigid votes become (t_c, r) in R^6 -- translation about the model
+// centre (length units) and the rotation's axis-angle vector (radians);
+// reflection votes become (d_c, n) in R^4 -- plane offset about the centre
+// and unit normal.  One bandwidth h applies to every coordinate.
+// Kernel: Gaussian truncated at cutoff*h and shifted to reach zero there,
+//   w = exp(-d^2 / 2h^2) - exp(-cutoff^2 / 2)   for d < cutoff*h,
+// so the support boundary carries no weight jump.
+//
+// Grid: cells of side >= cutoff*h over the first three coordinates; points
+// are radix-sorted by cell (stable, so sums are deterministic) and indexed by
+// an exclusive scan of the per-cell counts.  With x the fastest cell axis the
+// three cells of one (y, z) row are contiguous, so a query visits nine
+// contiguous point ranges.  One wavefront per seed runs all iterations in
+// registers; the 64 lanes stride the ranges and reduce with shuffles.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "common.hpp"
+#include "session.hpp"
+
+namespace mvs {
+namespace {
+
+constexpr int kThreads = 256;
+
+template <class R> __device__ inline R ms_exp(R x);
+template <> __device__ inline float ms_exp<float>(float x) { return __expf(x); }
+template <> __device__ inline double ms_exp<double>(double x) { return exp(x); }
+
+template <class R>
+__global__ void k_points_rigid(const R *q4, const R *t3, uint64_t n, R *pts) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  R w = q4[i], x = q4[n + i], y = q4[2 * n + i], z = q4[3 * n + i];
+  R s = sqrt(x * x + y * y + z * z);
+  R f = s > R(1e-12) ? R(2) * atan2(s, w) / s : R(2) / (w > R(0.5) ? w : R(1));
+  pts[i] = t3[i];
+  pts[n + i] = t3[n + i];
+  pts[2 * n + i] = t3[2 * n + i];
+  pts[3 * n + i] = f * x;
+  pts[4 * n + i] = f * y;
+  pts[5 * n + i] = f * z;
+}
+
+template <class R>
+__global__ void k_points_refl(const R *n3, const R *d1, uint64_t n, R *pts) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  pts[i] = d1[i];
+  pts[n + i] = n3[i];
+  pts[2 * n + i] = n3[n + i];
+  pts[3 * n + i] = n3[2 * n + i];
+}
+
+// per-block min/max of the first three coordinates
+template <class R>
+__global__ void k_bbox(const R *pts, uint64_t n, double *partial) {
+  __shared__ double s[6][kThreads];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * kThreads)
+    for (int k = 0; k < 3; ++k) {
+      double v = static_cast<double>(pts[k * n + i]);
+      lo[k] = fmin(lo[k], v);
+      hi[k] = fmax(hi[k], v);
+    }
+  for (int k = 0; k < 3; ++k) {
+    s[k][threadIdx.x] = lo[k];
+    s[3 + k][threadIdx.x] = hi[k];
+  }
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int k = 0; k < 3; ++k) {
+        s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + o]);
+        s[3 + k][threadIdx.x] = fmax(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + o]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6)
+    partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+struct Grid {
+  double lo[3];
+  double cell;
+  int dims[3];
+  uint64_t ncells;
+};
+
+template <class R> __device__ inline int cell_of(R v, double lo, double cell, int dim) {
+  double c = floor((static_cast<double>(v) - lo) / cell);
+  int ci = c < 0 ? 0 : (c >= dim ? dim - 1 : static_cast<int>(c));
+  return ci;
+}
+
+template <class R>
+__global__ void k_cell_keys(const R *pts, uint64_t n, Grid g, uint64_t *keys, uint32_t *vals,
+                            uint32_t *counts) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  int cx = cell_of(pts[i], g.lo[0], g.cell, g.dims[0]);
+  int cy = cell_of(pts[n + i], g.lo[1], g.cell, g.dims[1]);
+  int cz = cell_of(pts[2 * n + i], g.lo[2], g.cell, g.dims[2]);
+  uint64_t c = (static_cast<uint64_t>(cz) * g.dims[1] + cy) * g.dims[0] + cx;
+  keys[i] = c;
+  vals[i] = static_cast<uint32_t>(i);
+  atomicAdd(&counts[c], 1u);
+}
+
+template <class R>
+__global__ void k_gather_sorted(const R *pts, uint64_t n, int D, const uint32_t *order, R *out) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  uint32_t o = order[i];
+  for (int k = 0; k < D; ++k)
+    out[k * n + i] = pts[k * n + o];
+}
+
+// One wavefront per seed.
+template <class R, int D>
+__global__ void __launch_bounds__(kThreads)
+    k_ms_query(const R *pts, uint64_t n, const uint64_t *offsets, Grid g, const R *seeds,
+               uint64_t ns, int iterations, R inv2h2, R r2, R floor_w, R *out, R *weight,
+               unsigned long long *stats) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t s = static_cast<uint64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (s >= ns)
+    return;
+  R m[D];
+  for (int k = 0; k < D; ++k)
+    m[k] = seeds[k * ns + s];
+  R wsum = 0;
+  unsigned long long cand = 0, inside = 0;
+  for (int it = 0; it < iterations; ++it) {
+    R acc[D];
+    for (int k = 0; k < D; ++k)
+      acc[k] = 0;
+    R wl = 0;
+    const int cx = cell_of(m[0], g.lo[0], g.cell, g.dims[0]);
+    const int cy = cell_of(m[1], g.lo[1], g.cell, g.dims[1]);
+    const int cz = cell_of(m[2], g.lo[2], g.cell, g.dims[2]);
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.dims[0] ? cx + 1 : cx;
+    for (int dz = -1; dz <= 1; ++dz) {
+      int z = cz + dz;
+      if (z < 0 || z >= g.dims[2])
+        continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        int y = cy + dy;
+        if (y < 0 || y >= g.dims[1])
+          continue;
+        uint64_t row = (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
+        uint64_t beg = offsets[row + x0], end = offsets[row + x1 + 1];
+        cand += end - beg;
+        for (uint64_t j = beg + lane; j < end; j += kWave) {
+          R d[D];
+          R d2 = 0;
+          for (int k = 0; k < D; ++k) {
+            d[k] = pts[k * n + j] - m[k];
+            d2 += d[k] * d[k];
+          }
+          if (d2 < r2) {
+            R w = ms_exp<R>(-d2 * inv2h2) - floor_w;
+            wl += w;
+            for (int k = 0; k < D; ++k)
+              acc[k] += w * d[k];
+            ++inside;
+          }
+        }
+      }
+    }
+    wl = wave_sum(wl);
+    for (int k = 0; k < D; ++k)
+      acc[k] = wave_sum(acc[k]);
+    wsum = wl;
+    if (wl > R(0)) {
+      R inv = R(1) / wl;
+      for (int k = 0; k < D; ++k)
+        m[k] += acc[k] * inv;
+    }
+  }
+  if (lane == 0) {
+    for (int k = 0; k < D; ++k)
+      out[k * ns + s] = m[k];
+    weight[s] = wsum;
+    atomicAdd(&stats[0], cand);
+  }
+  inside = wave_sum(inside);
+  if (lane == 0)
+    atomicAdd(&stats[1], inside);
+}
+
+}  // namespace
+
+// Builds the grid and runs every seed; returns (candidates, in-support) counts
+// through stats_out when given.
+template <class R>
+static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns, const R *seeds,
+                       int iterations, double h, double cutoff, R *out, R *weight,
+                       unsigned long long *stats_out) {
+  if (D != 4 && D != 6)
+    raise(MVS_EINVAL, "batched mean-shift supports 4-D (reflection) and 6-D (rigid) points");
+  if (!(h > 0) || !(cutoff > 0) || iterations < 0)
+    raise(MVS_EINVAL, "invalid mean-shift parameters");
+  if (n > static_cast<uint64_t>(UINT32_MAX))
+    raise(MVS_EOVERFLOW, "too many points");
+  unsigned long long *stats = scratch_as<unsigned long long>(s, kSlotMs7, 4);
+  zero_device(s, stats, 4 * sizeof(unsigned long long));
+  if (ns == 0)
+    return;
+  Grid g{};
+  R *sorted = scratch_as<R>(s, kSlotMs0, static_cast<size_t>(D) * (n ? n : 1));
+  uint64_t *off = nullptr;
+  if (n) {
+    unsigned nb = blocks_for(n, kThreads);
+    if (nb > 1024)
+      nb = 1024;
+    double *partial = scratch_as<double>(s, kSlotMs1, 6 * nb);
+    MVS_LAUNCH(s, "ms_bbox", k_bbox<R>, nb, kThreads, 0, pts, n, partial);
+    double *hp = static_cast<double *>(host_stage(s, 6 * nb * sizeof(double)));
+    MVS_HIP(hipMemcpyAsync(hp, partial, 6 * nb * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    MVS_HIP(hipStreamSynchronize(s->stream));
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (unsigned b = 0; b < nb; ++b)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::fmin(lo[k], hp[6 * b + k]);
+        hi[k] = std::fmax(hi[k], hp[6 * b + 3 + k]);
+      }
+    for (int k = 0; k < 3; ++k)
+      if (!std::isfinite(lo[k]) || !std::isfinite(hi[k]))
+        raise(MVS_EINVAL, "non-finite mean-shift points");
+    double cell = cutoff * h;
+    for (;;) {
+      uint64_t cells = 1;
+      for (int k = 0; k < 3; ++k) {
+        g.dims[k] = static_cast<int>(std::floor((hi[k] - lo[k]) / cell)) + 1;
+        cells *= static_cast<uint64_t>(g.dims[k]);
+      }
+      if (cells <= (1ULL << 24))
+        break;
+      cell *= 2;
+    }
+    for (int k = 0; k < 3; ++k)
+      g.lo[k] = lo[k];
+    g.cell = cell;
+    g.ncells = static_cast<uint64_t>(g.dims[0]) * g.dims[1] * g.dims[2];
+    uint64_t *keys = scratch_as<uint64_t>(s, kSlotMs2, 2 * n);
+    uint32_t *vals = scratch_as<uint32_t>(s, kSlotMs3, 2 * n);
+    uint32_t *counts = scratch_as<uint32_t>(s, kSlotMs4, g.ncells);
+    off = scratch_as<uint64_t>(s, kSlotMs5, g.ncells + 1);
+    zero_device(s, counts, g.ncells * sizeof(uint32_t));
+    MVS_LAUNCH(s, "ms_cell_keys", k_cell_keys<R>, blocks_for(n, kThreads), kThreads, 0, pts, n, g,
+               keys, vals, counts);
+    int bits = 1;
+    while ((1ULL << bits) < g.ncells)
+      ++bits;
+    radix_sort_pairs_u64(s, keys, vals, keys + n, vals + n, n, bits);
+    exclusive_scan_u32(s, counts, off, g.ncells);
+    MVS_LAUNCH(s, "ms_gather", k_gather_sorted<R>, blocks_for(n, kThreads), kThreads, 0, pts, n, D,
+               vals + n, sorted);
+  } else {
+    off = scratch_as<uint64_t>(s, kSlotMs5, 2);
+    zero_device(s, off, 2 * sizeof(uint64_t));
+    g.dims[0] = g.dims[1] = g.dims[2] = 1;
+    g.cell = cutoff * h;
+    g.ncells = 1;
+  }
+  const R inv2h2 = static_cast<R>(1.0 / (2.0 * h * h));
+  const R r2 = static_cast<R>(cutoff * h * cutoff * h);
+  const R fw = static_cast<R>(std::exp(-0.5 * cutoff * cutoff));
+  unsigned grid = blocks_for(ns, kThreads / kWave);
+  if (D == 6)
+    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 6>), grid, kThreads, 0, sorted, n, off, g, seeds, ns,
+               iterations, inv2h2, r2, fw, out, weight, stats);
+  else
+    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 4>), grid, kThreads, 0, sorted, n, off, g, seeds, ns,
+               iterations, inv2h2, r2, fw, out, weight, stats);
+  if (stats_out) {
+    void *hp = host_stage(s, 2 * sizeof(unsigned long long));
+    MVS_HIP(hipMemcpyAsync(hp, stats, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           s->stream));
+    MVS_HIP(hipStreamSynchronize(s->stream));
+    std::memcpy(stats_out, hp, 2 * sizeof(unsigned long long));
+  }
+}
+
+}  // namespace mvs
+
+using namespace mvs;
+
+extern "C" {
+
+int mvs_dev_ms_points(mvs_session *sp, int precision, uint64_t nrigid, const void *q4,
+                      const void *t3, uint64_t nrefl, const void *n3, const void *d1,
+                      void *rigid_pts, void *refl_pts) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    if (precision == MVS_FP32_GUARDED) {
+      if (nrigid)
+        MVS_LAUNCH(s, "ms_points", k_points_rigid<float>, blocks_for(nrigid, kThreads), kThreads,
+                   0, static_cast<const float *>(q4), static_cast<const float *>(t3), nrigid,
+                   static_cast<float *>(rigid_pts));
+      if (nrefl)
+        MVS_LAUNCH(s, "ms_points", k_points_refl<float>, blocks_for(nrefl, kThreads), kThreads, 0,
+                   static_cast<const float *>(n3), static_cast<const float *>(d1), nrefl,
+                   static_cast<float *>(refl_pts));
+    } else if (precision == MVS_FP64) {
+      if (nrigid)
+        MVS_LAUNCH(s, "ms_points", k_points_rigid<double>, blocks_for(nrigid, kThreads), kThreads,
+                   0, static_cast<const double *>(q4), static_cast<const double *>(t3), nrigid,
+                   static_cast<double *>(rigid_pts));
+      if (nrefl)
+        MVS_LAUNCH(s, "ms_points", k_points_refl<double>, blocks_for(nrefl, kThreads), kThreads,
+                   0, static_cast<const double *>(n3), static_cast<const double *>(d1), nrefl,
+                   static_cast<double *>(refl_pts));
+    } else
+      raise(MVS_EINVAL, "unknown precision");
+  });
+}
+
+int mvs_dev_ms_batched(mvs_session *sp, int precision, int dim, uint64_t n, const void *pts,
+                       uint64_t ns, const void *seeds, int iterations, double h, double cutoff,
+                       void *out, void *weight, uint64_t *candidates, uint64_t *in_support) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    unsigned long long st[2] = {0, 0};
+    unsigned long long *want = (candidates || in_support) ? st : nullptr;
+    if (precision == MVS_FP32_GUARDED)
+      ms_batched<float>(s, dim, n, static_cast<const float *>(pts), ns,
+                        static_cast<const float *>(seeds), iterations, h, cutoff,
+                        static_cast<float *>(out), static_cast<float *>(weight), want);
+    else if (precision == MVS_FP64)
+      ms_batched<double>(s, dim, n, static_cast<const double *>(pts), ns,
+                         static_cast<const double *>(seeds), iterations, h, cutoff,
+                         static_cast<double *>(out), static_cast<double *>(weight), want);
+    else
+      raise(MVS_EINVAL, "unknown precision");
+    if (candidates)
+      *candidates = st[0];
+    if (in_support)
+      *in_support = st[1];
+  });
+}
+
+}  // extern "C"

@@ -1,34 +1,35 @@
-// Curvature-signature sort, pruned sample-pair search and vote emission.
+// Curvature-signature sort and pruned sample-pair search.
 //
-// Pairing restates src/cpu/pairing.cpp:14-35: all selected-ordinal pairs
-// a < b with |dkmin| <= t*M, |dkmax| <= t*M (M = max of the four |k| and
-// 1e-10) and |p_a - p_b| >= 0.03*box, in (a asc, b asc) order.
-//   1. gather  : selected ids -> ordinal-order sample SoA (14 lanes) and
-//                order-preserving uint64 keys of kmin (-0 canonicalised);
-//   2. sort    : LDS-staged stable LSD radix sort of (key, ordinal);
-//   3. windows : per source a conservative kmin interval (a superset of every
-//                b the predicate can accept) found by binary search;
-//   4. count   : one wavefront per source walks its window with coalesced
-//                loads of (ordinal, kmin, kmax) and counts accepted b > a;
-//   5. scan    : exclusive prefix of the counts -> output offsets; the total
-//                is read back once and checked against the pair budget
-//                before any output is written;
-//   6. fill    : the same walk appends accepted ordinals to an LDS list;
-//                each element's rank in the list (count of smaller ordinals)
-//                is its position, which restores the reference's order;
-//                sources with more than kCap pairs walk the ordinals in order
-//                instead (k_pair_long), so no list can overflow.
-// Vote emission restates src/cpu/transformations.cpp:17-73 per pair: an
-// optional mirror vote (two dot tests > .75) then two rigid votes
-// R = F_b(+-) F_a^T, t = q - R p.
+// Restates src/cpu/pairing.cpp:14-35: every selected-ordinal pair a < b with
+// |dkmin| <= t*M and |dkmax| <= t*M (M = max of the four |k| and 1e-10) and
+// |p_a - p_b| >= 0.03*box, emitted in (a asc, b asc) order.
+//   1. gather : selected ids -> ordinal-order sample SoA (14 lanes) and a
+//               32-bit order-preserving key of kmin (-0 canonicalised);
+//   2. sort   : LDS-staged stable LSD radix sort of (key, ordinal), 4 passes;
+//   3. index  : the kmin-sorted samples are cut into buckets of 64 and each
+//               bucket is sorted by kmax inside one wavefront (bitonic
+//               network over the 64 lanes), giving a two-level
+//               (kmin bucket, kmax) index;
+//   4. windows: per source a conservative interval for kmin and for kmax
+//               (supersets of everything the predicate can accept);
+//   5. count  : one wavefront per source: each lane binary-searches the kmax
+//               interval inside one bucket of the kmin interval, the wave
+//               concatenates the sub-ranges and tests their candidates with
+//               coalesced loads; accepted b > a are counted;
+//   6. scan   : exclusive prefix of the counts -> output offsets; the total is
+//               read back once and checked against the budget before any
+//               output is written;
+//   7. fill   : the same walk appends accepted ordinals to an LDS list whose
+//               rank order (number of smaller ordinals) restores the
+//               reference's b order; sources with more than kCap pairs walk
+//               the targets in ordinal order instead (k_pair_long).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstring>
 #include <limits>
 
-#include "common.hpp"
-#include "session.hpp"
+#include "pairing.hpp"
 
 namespace mvs {
 namespace {
@@ -36,109 +37,163 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;  // sources per block
 constexpr int kCap = 512;                 // LDS order-restoration capacity per source
+constexpr int kBucket = 64;               // samples per kmin bucket
 constexpr double kEps = 2.220446049250313e-16;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-// Sample SoA lanes (ordinal order).
-enum { L_X = 0, L_Y, L_Z, L_K0, L_K1, L_NX, L_NY, L_NZ, L_DX, L_DY, L_DZ, L_XX, L_XY, L_XZ, L_N };
-
-__device__ inline uint64_t order_key(double v) {
+// 32-bit order-preserving key: the top half of the IEEE order transform.
+// Monotone non-decreasing in v, so windows searched in key space are
+// supersets of the windows in value space.
+__device__ inline uint32_t order_key32(double v) {
   if (v == 0.0)
-    v = 0.0;  // -0 and +0 compare equal: give them one key
+    v = 0.0;  // -0 and +0 compare equal: one key
   uint64_t u = __double_as_longlong(v);
-  return (u >> 63) ? ~u : (u | 0x8000000000000000ULL);
+  u = (u >> 63) ? ~u : (u | 0x8000000000000000ULL);
+  return static_cast<uint32_t>(u >> 32);
 }
 
 // Flags: 1 = selected id out of range, 2 = non-finite coordinate/curvature.
+__device__ inline float wave_sum_max(float v) {
+  for (int o = 32; o > 0; o >>= 1)
+    v = fmaxf(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
 __global__ void k_gather_samples(const double *x, const double *y, const double *z,
                                  const double *feat, uint64_t n, const int32_t *sel, uint64_t S,
                                  double cx, double cy, double cz, double *s64, float *s32,
-                                 uint64_t *keys, uint32_t *vals, unsigned *flags) {
-  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= S)
-    return;
-  int32_t id = sel[i];
-  double v[L_N];
-  if (id < 0 || static_cast<uint64_t>(id) >= n) {
-    atomicOr(flags, 1u);
+                                 uint64_t *keys, uint32_t *vals, unsigned *flags,
+                                 unsigned *rmax_bits) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool live = i < S;
+  float r = 0.0f;
+  if (live) {
+    int32_t id = sel[i];
+    double v[L_N];
+    if (id < 0 || static_cast<uint64_t>(id) >= n) {
+      atomicOr(flags, 1u);
+      for (int k = 0; k < L_N; ++k)
+        v[k] = 0;
+    } else {
+      v[L_X] = x[id];
+      v[L_Y] = y[id];
+      v[L_Z] = z[id];
+      v[L_K0] = feat[9 * n + id];
+      v[L_K1] = feat[10 * n + id];
+      for (int k = 0; k < 9; ++k)
+        v[L_NX + k] = feat[k * n + id];
+      bool fin = isfinite(v[L_X]) && isfinite(v[L_Y]) && isfinite(v[L_Z]) &&
+                 isfinite(v[L_K0]) && isfinite(v[L_K1]);
+      if (!fin) {
+        atomicOr(flags, 2u);
+        for (int k = 0; k < L_N; ++k)
+          v[k] = 0;  // keep every later comparison well defined
+      }
+    }
     for (int k = 0; k < L_N; ++k)
-      v[k] = 0;
-  } else {
-    v[L_X] = x[id];
-    v[L_Y] = y[id];
-    v[L_Z] = z[id];
-    v[L_K0] = feat[9 * n + id];
-    v[L_K1] = feat[10 * n + id];
-    for (int k = 0; k < 9; ++k)
-      v[L_NX + k] = feat[k * n + id];
-    bool fin = isfinite(v[L_X]) && isfinite(v[L_Y]) && isfinite(v[L_Z]) && isfinite(v[L_K0]) &&
-               isfinite(v[L_K1]);
-    if (!fin)
-      atomicOr(flags, 2u);
+      s64[k * S + i] = v[k];
+    if (s32) {
+      float px = static_cast<float>(v[L_X] - cx), py = static_cast<float>(v[L_Y] - cy),
+            pz = static_cast<float>(v[L_Z] - cz);
+      s32[L_X * S + i] = px;
+      s32[L_Y * S + i] = py;
+      s32[L_Z * S + i] = pz;
+      for (int k = L_K0; k < L_N; ++k)
+        s32[k * S + i] = static_cast<float>(v[k]);
+      r = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
+    }
+    keys[i] = order_key32(v[L_K0]);
+    vals[i] = static_cast<uint32_t>(i);
   }
-  for (int k = 0; k < L_N; ++k)
-    s64[k * S + i] = v[k];
+  // largest centre-relative coordinate (sets the FP32 guard bands); all
+  // lanes take part in the reduction
   if (s32) {
-    // FP32 copy: positions relative to the model centre
-    s32[L_X * S + i] = static_cast<float>(v[L_X] - cx);
-    s32[L_Y * S + i] = static_cast<float>(v[L_Y] - cy);
-    s32[L_Z * S + i] = static_cast<float>(v[L_Z] - cz);
-    for (int k = L_K0; k < L_N; ++k)
-      s32[k * S + i] = static_cast<float>(v[k]);
+    r = wave_sum_max(r);
+    if ((threadIdx.x & (kWave - 1)) == 0)
+      atomicMax(rmax_bits, __float_as_uint(r));  // non-negative floats order as uints
   }
-  keys[i] = order_key(v[L_K0]);
-  vals[i] = static_cast<uint32_t>(i);
 }
 
-__global__ void k_sorted_gather(const uint32_t *ord, const double *s64, const float *s32,
-                                uint64_t S, double *sk, float *sk32) {
-  uint64_t j = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= S)
+// One wavefront per bucket of 64 kmin-sorted samples: sort by (kmax, ordinal)
+// with a 64-lane bitonic network and write the bucketed index.
+__global__ void __launch_bounds__(kThreads)
+    k_bucket_index(const uint32_t *ord, const double *s64, const float *s32, uint64_t S,
+                   uint32_t *bord, double *bk0, double *bk1, float *bk0f, float *bk1f) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
+  const uint64_t nb = (S + kBucket - 1) / kBucket;
+  if (b >= nb)
     return;
-  uint32_t o = ord[j];
-  sk[j] = s64[L_K0 * S + o];
-  sk[S + j] = s64[L_K1 * S + o];
-  if (sk32) {
-    sk32[j] = s32[L_K0 * S + o];
-    sk32[S + j] = s32[L_K1 * S + o];
+  const uint64_t j = b * kBucket + lane;
+  uint32_t o = kNone;
+  double key = INFINITY;
+  if (j < S) {
+    o = ord[j];
+    key = s64[L_K1 * S + o];
+  }
+  for (int k = 2; k <= kWave; k <<= 1)
+    for (int m = k >> 1; m > 0; m >>= 1) {
+      double pk = __shfl_xor(key, m, kWave);
+      uint32_t po = __shfl_xor(o, m, kWave);
+      bool up = (lane & k) == 0;
+      bool lower = (lane & m) == 0;
+      bool mine_less = key < pk || (key == pk && o < po);
+      bool keep = (lower == up) ? mine_less : !mine_less;
+      if (!keep) {
+        key = pk;
+        o = po;
+      }
+    }
+  if (j < S) {
+    bord[j] = o;
+    bk1[j] = key;
+    bk0[j] = s64[L_K0 * S + o];
+    if (bk0f) {
+      bk0f[j] = s32[L_K0 * S + o];
+      bk1f[j] = s32[L_K1 * S + o];
+    }
   }
 }
 
-// Conservative kmin window.  If a pair passes, with Ma = max(|ka|,|Ka|,1e-10)
+// Conservative intervals.  If a pair passes, with Ma = max(|ka|,|Ka|,1e-10)
 // and M the pair magnitude: M <= Ma/(1-t) (if the other sample dominates,
-// its magnitude m satisfies m <= Ma + t m), hence |ka - kb| <= t Ma/(1-t).
-// t is inflated by 16 eps to cover the reference's rounding of the
-// difference and of t*M, the radius by another 16 eps, and the endpoints by
-// 16 eps of their magnitude; t >= .5 or denormal-range t use the full range.
-__global__ void k_windows(const double *s64, const double *sk, uint64_t S, double t,
-                          uint64_t *win) {
+// its magnitude m satisfies m <= Ma + t m), hence |ka - kb| <= t Ma/(1-t)
+// and |Ka - Kb| <= t Ma/(1-t).  t is inflated by 16 eps (the reference
+// rounds both the difference and t*M), the radius by 16 eps and the
+// endpoints by 16 eps of their magnitude; t >= .5 or t below the normal
+// range use the full range.
+// win: [0,S) first sorted position, [S,2S) end position; kw: kmax interval.
+__global__ void k_windows(const double *s64, const uint64_t *skeys, uint64_t S, double t,
+                          uint64_t *win, double *kw) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= S)
     return;
   uint64_t lo = 0, hi = S;
+  double klo = -INFINITY, khi = INFINITY;
   double tp = t * (1 + 16 * kEps);
   if (tp < 0.5 && t > 1e-290) {
     double a = s64[L_K0 * S + i], b = s64[L_K1 * S + i];
     double ma = fmax(fmax(fabs(a), fabs(b)), 1e-10);
     double r = tp * ma / (1 - tp) * (1 + 16 * kEps);
-    double pad = 16 * kEps * (fabs(a) + r);
-    double low = a - r - pad, high = a + r + pad;
-    // lower_bound(low): first j with !(sk[j] < low)
+    double pa = 16 * kEps * (fabs(a) + r), pb = 16 * kEps * (fabs(b) + r);
+    uint32_t qlo = order_key32(a - r - pa), qhi = order_key32(a + r + pa);
+    klo = b - r - pb;
+    khi = b + r + pb;
     uint64_t f = 0, c = S;
-    while (c) {
+    while (c) {  // first position with key >= qlo
       uint64_t h = c / 2, m = f + h;
-      if (sk[m] < low) {
+      if (static_cast<uint32_t>(skeys[m]) < qlo) {
         f = m + 1;
         c -= h + 1;
       } else
         c = h;
     }
     lo = f;
-    // upper_bound(high): first j with sk[j] > high
     f = 0;
     c = S;
-    while (c) {
+    while (c) {  // first position with key > qhi
       uint64_t h = c / 2, m = f + h;
-      if (!(sk[m] > high)) {
+      if (static_cast<uint32_t>(skeys[m]) <= qhi) {
         f = m + 1;
         c -= h + 1;
       } else
@@ -148,9 +203,11 @@ __global__ void k_windows(const double *s64, const double *sk, uint64_t S, doubl
   }
   win[i] = lo;
   win[S + i] = hi;
+  kw[2 * i] = klo;
+  kw[2 * i + 1] = khi;
 }
 
-// Exact reference predicate in FP64 (pairing.cpp:22-28 as restated above).
+// Exact reference predicate in FP64 (pairing.cpp:22-28).
 __device__ inline bool accept64(double a0, double a1, double b0, double b1, double ax, double ay,
                                 double az, double bx, double by, double bz, double t,
                                 double dist) {
@@ -194,23 +251,27 @@ __device__ inline int screen32_dist(float dx, float dy, float dz, float dist, fl
 struct PairArgs {
   const double *s64;
   const float *s32;
-  const double *sk;   // sorted kmin | kmax (2*S)
-  const float *sk32;  // sorted FP32 kmin | kmax
-  const uint32_t *ord;
+  const uint32_t *bord;
+  const double *bk0, *bk1;
+  const float *bk0f, *bk1f;
   const uint64_t *win;
+  const double *kw;
   uint64_t S;
   double t, dist;
   float t32, dist32, gk, gd;
-  unsigned long long *stats;  // [0] candidates, [1] FP64 rechecks
+  // distance band = gd0 + gd1 * (largest FP32 coordinate), set per launch
+  float gd0, gd1;
+  const unsigned *rmax_bits;
+  uint32_t *cand, *cand2, *rechk;  // per source statistics
 };
 
-// Decide candidate (source i, sorted slot j holding ordinal o) in the chosen arithmetic.
+// Decide candidate at bucketed position j (ordinal o) for source i.
 template <bool kGuarded>
-__device__ inline bool decide(const PairArgs &A, uint64_t i, uint64_t j, uint32_t o,
-                              double a0, double a1, float f0, float f1, unsigned &rechecks) {
+__device__ inline bool decide(const PairArgs &A, uint64_t i, uint64_t j, uint32_t o, double a0,
+                              double a1, float f0, float f1, unsigned &rechecks) {
   const uint64_t S = A.S;
   if (!kGuarded) {
-    double b0 = A.sk[j], b1 = A.sk[S + j];
+    double b0 = A.bk0[j], b1 = A.bk1[j];
     double m = fmax(fmax(fabs(a0), fabs(a1)), fmax(fmax(fabs(b0), fabs(b1)), 1e-10));
     double tol = A.t * m;
     if (fabs(a0 - b0) > tol || fabs(a1 - b1) > tol)
@@ -218,7 +279,7 @@ __device__ inline bool decide(const PairArgs &A, uint64_t i, uint64_t j, uint32_
     return accept64(a0, a1, b0, b1, A.s64[L_X * S + i], A.s64[L_Y * S + i], A.s64[L_Z * S + i],
                     A.s64[L_X * S + o], A.s64[L_Y * S + o], A.s64[L_Z * S + o], A.t, A.dist);
   } else {
-    int r = screen32(f0, f1, A.sk32[j], A.sk32[S + j], A.t32, A.gk);
+    int r = screen32(f0, f1, A.bk0f[j], A.bk1f[j], A.t32, A.gk);
     if (r == 0)
       return false;
     if (r == 1) {
@@ -229,86 +290,160 @@ __device__ inline bool decide(const PairArgs &A, uint64_t i, uint64_t j, uint32_
         return d == 1;
     }
     ++rechecks;
-    return accept64(a0, a1, A.sk[j], A.sk[S + j], A.s64[L_X * S + i], A.s64[L_Y * S + i],
+    return accept64(a0, a1, A.bk0[j], A.bk1[j], A.s64[L_X * S + i], A.s64[L_Y * S + i],
                     A.s64[L_Z * S + i], A.s64[L_X * S + o], A.s64[L_Y * S + o],
                     A.s64[L_Z * S + o], A.t, A.dist);
   }
 }
 
+// Per-wave LDS for concatenating the bucket sub-ranges.
+struct WaveRanges {
+  uint32_t start[kWave];
+  uint32_t pre[kWave + 1];
+};
+
+// Walks every candidate of source i and calls f(accepted, ordinal) once per
+// lane per candidate chunk (uniform control flow: f sees every lane).
+template <bool kGuarded, class F>
+__device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W, int lane,
+                                   unsigned &rechecks, uint64_t &cand2, F &&f) {
+  const uint64_t S = A.S;
+  const uint64_t jlo = A.win[i], jhi = A.win[S + i];
+  if (jlo >= jhi)
+    return;
+  const double klo = A.kw[2 * i], khi = A.kw[2 * i + 1];
+  const double a0 = A.s64[L_K0 * S + i], a1 = A.s64[L_K1 * S + i];
+  const float f0 = kGuarded ? A.s32[L_K0 * S + i] : 0.f;
+  const float f1 = kGuarded ? A.s32[L_K1 * S + i] : 0.f;
+  const uint64_t b0 = jlo / kBucket, b1 = (jhi - 1) / kBucket;
+  for (uint64_t bb = b0; bb <= b1; bb += kWave) {
+    const uint64_t b = bb + lane;
+    uint32_t u = 0, len = 0;
+    if (b <= b1) {
+      const uint64_t base = b * kBucket;
+      const uint64_t nb = S - base < static_cast<uint64_t>(kBucket) ? S - base : kBucket;
+      uint64_t f = 0, c = nb;
+      while (c) {  // first with kmax >= klo
+        uint64_t h = c / 2, m = f + h;
+        if (A.bk1[base + m] < klo) {
+          f = m + 1;
+          c -= h + 1;
+        } else
+          c = h;
+      }
+      uint64_t lo = f;
+      f = lo;
+      c = nb - lo;
+      while (c) {  // first with kmax > khi
+        uint64_t h = c / 2, m = f + h;
+        if (!(A.bk1[base + m] > khi)) {
+          f = m + 1;
+          c -= h + 1;
+        } else
+          c = h;
+      }
+      u = static_cast<uint32_t>(base + lo);
+      len = static_cast<uint32_t>(f - lo);
+    }
+    // exclusive prefix of the sub-range lengths across the wave
+    uint32_t x = len;
+    for (int o = 1; o < kWave; o <<= 1) {
+      uint32_t y = __shfl_up(x, o, kWave);
+      if (lane >= o)
+        x += y;
+    }
+    const uint32_t total = __shfl(x, kWave - 1, kWave);
+    W.start[lane] = u;
+    W.pre[lane] = x - len;
+    if (lane == 0)
+      W.pre[kWave] = total;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    cand2 += total;
+    for (uint32_t c0 = 0; c0 < total; c0 += kWave) {
+      const uint32_t c = c0 + lane;
+      bool ok = false;
+      uint32_t o = 0;
+      if (c < total) {
+        // owner sub-range: last l with pre[l] <= c
+        int l = 0;
+        for (int step = kWave / 2; step > 0; step >>= 1)
+          if (W.pre[l + step] <= c)
+            l += step;
+        const uint64_t j = W.start[l] + (c - W.pre[l]);
+        o = A.bord[j];
+        if (o != kNone && o > i)
+          ok = decide<kGuarded>(A, i, j, o, a0, a1, f0, f1, rechecks);
+      }
+      f(ok, o);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+template <bool kGuarded> __device__ inline PairArgs with_bands(const PairArgs &in) {
+  PairArgs A = in;
+  if (kGuarded)
+    A.gd = in.gd0 + in.gd1 * __uint_as_float(*in.rmax_bits);
+  return A;
+}
+
 template <bool kGuarded>
 __global__ void __launch_bounds__(kThreads)
-    k_pair_count(PairArgs A, uint64_t *counts, uint32_t *long_list, unsigned *long_count) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
-  if (i >= A.S)
+    k_pair_count(PairArgs Ain, uint64_t *counts, uint32_t *long_list, unsigned *long_count) {
+  __shared__ WaveRanges ranges[kWaves];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kWaves + wave;
+  if (i >= Ain.S)
     return;
-  const uint64_t S = A.S;
-  const uint64_t lo = A.win[i], hi = A.win[S + i];
-  const double a0 = A.s64[L_K0 * S + i], a1 = A.s64[L_K1 * S + i];
-  const float f0 = kGuarded ? A.s32[L_K0 * S + i] : 0.f, f1 = kGuarded ? A.s32[L_K1 * S + i] : 0.f;
-  uint64_t cnt = 0;
+  const PairArgs A = with_bands<kGuarded>(Ain);
+  uint64_t cnt = 0, cand2 = 0;
   unsigned rechecks = 0;
-  for (uint64_t j0 = lo; j0 < hi; j0 += kWave) {
-    uint64_t j = j0 + lane;
-    bool ok = false;
-    if (j < hi) {
-      uint32_t o = A.ord[j];
-      if (o > i)
-        ok = decide<kGuarded>(A, i, j, o, a0, a1, f0, f1, rechecks);
-    }
-    cnt += __popcll(ballot(ok));
-  }
+  walk_source<kGuarded>(A, i, ranges[wave], lane, rechecks, cand2,
+                        [&](bool ok, uint32_t) { cnt += __popcll(ballot(ok)); });
+  rechecks = wave_sum(rechecks);
   if (lane == 0) {
     counts[i] = cnt;
     if (cnt > static_cast<uint64_t>(kCap)) {
       unsigned slot = atomicAdd(long_count, 1u);
       long_list[slot] = static_cast<uint32_t>(i);
     }
-    atomicAdd(&A.stats[0], static_cast<unsigned long long>(hi - lo));
+    uint64_t w = A.win[A.S + i] - A.win[i];
+    A.cand[i] = static_cast<uint32_t>(w < kNone ? w : kNone);
+    A.cand2[i] = static_cast<uint32_t>(cand2 < kNone ? cand2 : kNone);
+    A.rechk[i] = rechecks;
   }
-  rechecks = wave_sum(rechecks);
-  if (lane == 0 && rechecks)
-    atomicAdd(&A.stats[1], static_cast<unsigned long long>(rechecks));
 }
 
 template <bool kGuarded>
 __global__ void __launch_bounds__(kThreads)
-    k_pair_fill(PairArgs A, const uint64_t *counts, const uint64_t *offsets, const int32_t *sel,
+    k_pair_fill(PairArgs Ain, const uint64_t *counts, const uint64_t *offsets, const int32_t *sel,
                 int32_t *out_a, int32_t *out_b) {
+  __shared__ WaveRanges ranges[kWaves];
   __shared__ uint32_t list[kWaves][kCap];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kWaves + wave;
-  if (i >= A.S)
+  if (i >= Ain.S)
     return;
-  const uint64_t S = A.S;
+  const PairArgs A = with_bands<kGuarded>(Ain);
   const uint64_t total = counts[i];
   if (total == 0 || total > static_cast<uint64_t>(kCap))
     return;
-  const uint64_t lo = A.win[i], hi = A.win[S + i];
-  const double a0 = A.s64[L_K0 * S + i], a1 = A.s64[L_K1 * S + i];
-  const float f0 = kGuarded ? A.s32[L_K0 * S + i] : 0.f, f1 = kGuarded ? A.s32[L_K1 * S + i] : 0.f;
-  uint32_t n = 0;
-  unsigned rechecks = 0;
   uint32_t *L = list[wave];
-  for (uint64_t j0 = lo; j0 < hi; j0 += kWave) {
-    uint64_t j = j0 + lane;
-    bool ok = false;
-    uint32_t o = 0;
-    if (j < hi) {
-      o = A.ord[j];
-      if (o > i)
-        ok = decide<kGuarded>(A, i, j, o, a0, a1, f0, f1, rechecks);
-    }
+  uint32_t n = 0;
+  uint64_t cand2 = 0;
+  unsigned rechecks = 0;
+  walk_source<kGuarded>(A, i, ranges[wave], lane, rechecks, cand2, [&](bool ok, uint32_t o) {
     uint64_t mask = ballot(ok);
     uint32_t pos = n + static_cast<uint32_t>(lanes_below(mask));
     if (ok && pos < kCap)
       L[pos] = o;
     n += static_cast<uint32_t>(__popcll(mask));
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  });
   __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   // count and fill make identical decisions; the clamps only keep every
-  // access inside this source's output range should they ever disagree
+  // access inside this source's range should they ever disagree
   if (n > kCap)
     n = kCap;
   if (n > total)
@@ -325,7 +460,6 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // Sources with more than kCap pairs: walk the targets in ordinal order.
-template <bool kGuarded>
 __global__ void __launch_bounds__(kThreads)
     k_pair_long(PairArgs A, const uint32_t *long_list, unsigned long_count,
                 const uint64_t *offsets, const int32_t *sel, int32_t *out_a, int32_t *out_b) {
@@ -356,382 +490,30 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// ------------------------------------------------------------ vote geometry
-// Pair geometry of transformations.cpp:34-47 in FP64.
-struct Geo64 {
-  V3 p, q, an, ad, ax, bn, bd, bx, n;
-  double d;
-  M3 mirror;
-  bool refl;
-};
-__device__ inline Geo64 geo64(V3 p, V3 q, V3 an, V3 ad, V3 ax, V3 bn, V3 bd, V3 bx) {
-  Geo64 g;
-  g.p = p, g.q = q, g.an = an, g.ad = ad, g.ax = ax, g.bn = bn, g.bd = bd, g.bx = bx;
-  V3 n = unit(q - p);
-  int dom = 0;
-  for (int k = 1; k < 3; ++k)
-    if (fabs(comp(n, k)) > fabs(comp(n, dom)))
-      dom = k;
-  if (comp(n, dom) < 0)
-    n = n * -1;
-  g.n = n;
-  g.d = -dot(n, vdivs(p + q, 2));
-  g.mirror = reflection(n);
-  g.refl = dot(apply(g.mirror, an), bn) > 0.75 && fabs(dot(apply(g.mirror, ad), bd)) > 0.75;
-  return g;
-}
-__device__ inline M3 rigid64(const Geo64 &g, double sign) {
-  return mul_bt(frame(g.bd * sign, g.bx * sign, g.bn), frame(g.ad, g.ax, g.an));
-}
-
-// Rotation matrix -> unit quaternion (w, x, y, z), w >= 0 (Shepperd's method).
-template <class R> __device__ inline void quat(const R *m, R *q) {
-  R tr = m[0] + m[4] + m[8];
-  R w, x, y, z;
-  if (tr > R(0)) {
-    R s = sqrt(tr + R(1)) * R(2);
-    w = R(0.25) * s;
-    x = (m[7] - m[5]) / s;
-    y = (m[2] - m[6]) / s;
-    z = (m[3] - m[1]) / s;
-  } else if (m[0] > m[4] && m[0] > m[8]) {
-    R s = sqrt(R(1) + m[0] - m[4] - m[8]) * R(2);
-    w = (m[7] - m[5]) / s;
-    x = R(0.25) * s;
-    y = (m[1] + m[3]) / s;
-    z = (m[2] + m[6]) / s;
-  } else if (m[4] > m[8]) {
-    R s = sqrt(R(1) + m[4] - m[0] - m[8]) * R(2);
-    w = (m[2] - m[6]) / s;
-    x = (m[1] + m[3]) / s;
-    y = R(0.25) * s;
-    z = (m[5] + m[7]) / s;
-  } else {
-    R s = sqrt(R(1) + m[8] - m[0] - m[4]) * R(2);
-    w = (m[3] - m[1]) / s;
-    x = (m[2] + m[6]) / s;
-    y = (m[5] + m[7]) / s;
-    z = R(0.25) * s;
-  }
-  if (w < R(0))
-    w = -w, x = -x, y = -y, z = -z;
-  q[0] = w, q[1] = x, q[2] = y, q[3] = z;
-}
-
-// FP32 vector helpers for the guarded emission.
-struct F3 {
-  float x, y, z;
-};
-__device__ inline F3 f3(float x, float y, float z) { return {x, y, z}; }
-__device__ inline F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ inline F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ inline F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
-__device__ inline float fdot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ inline float fcomp(F3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
-
-// Loads of one sample's 14 lanes (ordinal order).
-__device__ inline V3 lane3(const double *s, uint64_t S, int l, uint64_t i) {
-  return v3(s[l * S + i], s[(l + 1) * S + i], s[(l + 2) * S + i]);
-}
-__device__ inline F3 lane3f(const float *s, uint64_t S, int l, uint64_t i) {
-  return f3(s[l * S + i], s[(l + 1) * S + i], s[(l + 2) * S + i]);
-}
-
-struct VoteArgs {
-  const double *s64;
-  const float *s32;
-  uint64_t S;
-  const int32_t *a, *b;  // pair ordinals
-  uint64_t P;
-  double cx, cy, cz;
-  float grefl, gdom;       // guard bands (FP32 path)
-  unsigned long long *stats;  // [2] reflection rechecks, [3] axis-tie rechecks
-};
-
-// Reflection decision and plane for pair k in the chosen arithmetic.  In the
-// guarded path the FP32 result is used unless a test value lies inside its
-// guard band, in which case the FP64 decision is taken.
-template <bool kGuarded>
-__device__ inline bool pair_reflection(const VoteArgs &A, uint64_t k, unsigned &rr, unsigned &rd) {
-  const uint64_t S = A.S;
-  uint64_t i = static_cast<uint64_t>(A.a[k]), o = static_cast<uint64_t>(A.b[k]);
-  if (!kGuarded) {
-    Geo64 g = geo64(lane3(A.s64, S, L_X, i), lane3(A.s64, S, L_X, o), lane3(A.s64, S, L_NX, i),
-                    lane3(A.s64, S, L_DX, i), lane3(A.s64, S, L_XX, i), lane3(A.s64, S, L_NX, o),
-                    lane3(A.s64, S, L_DX, o), lane3(A.s64, S, L_XX, o));
-    return g.refl;
-  }
-  F3 p = lane3f(A.s32, S, L_X, i), q = lane3f(A.s32, S, L_X, o);
-  F3 dv = q - p;
-  float len = sqrtf(fdot(dv, dv));
-  bool tie = false;
-  F3 n = len > 1e-14f ? dv * (1.0f / len) : f3(0, 0, 0);
-  {
-    float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
-    float m1 = fmaxf(ax, fmaxf(ay, az));
-    float m2 = fmaxf(fminf(ax, ay), fminf(fmaxf(ax, ay), az));
-    tie = m1 - m2 <= A.gdom || !(len > 1e-6f);
-  }
-  if (!tie) {
-    int dom = 0;
-    for (int kk = 1; kk < 3; ++kk)
-      if (fabsf(fcomp(n, kk)) > fabsf(fcomp(n, dom)))
-        dom = kk;
-    if (fcomp(n, dom) < 0)
-      n = n * -1.0f;
-    F3 an = lane3f(A.s32, S, L_NX, i), ad = lane3f(A.s32, S, L_DX, i);
-    F3 bn = lane3f(A.s32, S, L_NX, o), bd = lane3f(A.s32, S, L_DX, o);
-    // (I - 2 n n^T) v = v - 2 (n.v) n
-    F3 ma = an - n * (2.0f * fdot(n, an)), md = ad - n * (2.0f * fdot(n, ad));
-    float r1 = fdot(ma, bn), r2 = fabsf(fdot(md, bd));
-    if (fabsf(r1 - 0.75f) > A.grefl && fabsf(r2 - 0.75f) > A.grefl)
-      return r1 > 0.75f && r2 > 0.75f;
-    if (r1 < 0.75f - A.grefl || r2 < 0.75f - A.grefl)
-      return false;  // one test fails outside its band
-    ++rr;
-  } else {
-    ++rd;
-  }
-  Geo64 g = geo64(lane3(A.s64, S, L_X, i), lane3(A.s64, S, L_X, o), lane3(A.s64, S, L_NX, i),
-                  lane3(A.s64, S, L_DX, i), lane3(A.s64, S, L_XX, i), lane3(A.s64, S, L_NX, o),
-                  lane3(A.s64, S, L_DX, o), lane3(A.s64, S, L_XX, o));
-  return g.refl;
-}
-
-constexpr int kVoteItems = 4;
-constexpr int kVoteTile = kThreads * kVoteItems;  // pairs per block
-
-template <bool kGuarded>
-__global__ void __launch_bounds__(kThreads) k_vote_count(VoteArgs A, uint32_t *block_refl) {
-  __shared__ uint32_t wsum[kWaves];
-  uint64_t base = static_cast<uint64_t>(blockIdx.x) * kVoteTile;
-  unsigned rr = 0, rd = 0;
-  uint32_t c = 0;
-  for (int r = 0; r < kVoteItems; ++r) {
-    uint64_t k = base + r * kThreads + threadIdx.x;
-    if (k < A.P)
-      c += pair_reflection<kGuarded>(A, k, rr, rd);
-  }
-  c = wave_sum(c);
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  if (lane == 0)
-    wsum[wave] = c;
-  __syncthreads();
-  if (threadIdx.x == 0)
-    block_refl[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-}
-
-template <class R> struct VoteOut {
-  R *q4;        // rigid: 4 lanes, stride 2P
-  R *t3;        // rigid: 3 lanes, stride 2P
-  int32_t *rp;  // rigid pair ids
-  R *n3;        // reflection: 3 lanes, stride Rtot
-  R *d1;        // reflection offsets
-  int32_t *fp;  // reflection pair ids
-  uint64_t R_total;
-};
-
-template <bool kGuarded, class R>
-__global__ void __launch_bounds__(kThreads)
-    k_vote_emit(VoteArgs A, const uint64_t *block_off, VoteOut<R> out) {
-  __shared__ uint32_t wsum[kWaves + 1];
-  const uint64_t S = A.S, P = A.P, P2 = 2 * A.P;
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  uint64_t base = static_cast<uint64_t>(blockIdx.x) * kVoteTile;
-  uint64_t run = block_off[blockIdx.x];
-  unsigned rr = 0, rd = 0;
-  const double cx = A.cx, cy = A.cy, cz = A.cz;
-  for (int r = 0; r < kVoteItems; ++r) {
-    uint64_t k = base + r * kThreads + threadIdx.x;
-    bool live = k < P;
-    bool refl = live && pair_reflection<kGuarded>(A, k, rr, rd);
-    // block-wide exclusive prefix of refl in pair order
-    uint64_t m = ballot(refl);
-    uint32_t below = static_cast<uint32_t>(lanes_below(m));
-    if (lane == 0)
-      wsum[wave] = static_cast<uint32_t>(__popcll(m));
-    __syncthreads();
-    uint32_t before = 0;
-    for (int w = 0; w < wave; ++w)
-      before += wsum[w];
-    uint32_t tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    __syncthreads();
-    uint64_t slot = run + before + below;
-    run += tot;
-    if (!live)
-      continue;
-    uint64_t i = static_cast<uint64_t>(A.a[k]), o = static_cast<uint64_t>(A.b[k]);
-    R rq[2][4], rt[2][3];
-    R pn[3], pd = 0;
-    if (!kGuarded) {
-      Geo64 g = geo64(lane3(A.s64, S, L_X, i), lane3(A.s64, S, L_X, o), lane3(A.s64, S, L_NX, i),
-                      lane3(A.s64, S, L_DX, i), lane3(A.s64, S, L_XX, i), lane3(A.s64, S, L_NX, o),
-                      lane3(A.s64, S, L_DX, o), lane3(A.s64, S, L_XX, o));
-      V3 c = v3(cx, cy, cz);
-      for (int sg = 0; sg < 2; ++sg) {
-        M3 rot = rigid64(g, sg == 0 ? 1.0 : -1.0);
-        V3 t = g.q - apply(rot, g.p);
-        V3 tc = apply(rot, c) + t - c;  // translation about the model centre
-        double qq[4];
-        quat<double>(rot.m, qq);
-        for (int kk = 0; kk < 4; ++kk)
-          rq[sg][kk] = static_cast<R>(qq[kk]);
-        rt[sg][0] = static_cast<R>(tc.x), rt[sg][1] = static_cast<R>(tc.y),
-        rt[sg][2] = static_cast<R>(tc.z);
-      }
-      pn[0] = static_cast<R>(g.n.x), pn[1] = static_cast<R>(g.n.y), pn[2] = static_cast<R>(g.n.z);
-      pd = static_cast<R>(g.d + dot(g.n, c));
-    } else {
-      // FP32 frames in centre-relative coordinates: t_c = q' - R p'
-      F3 p = lane3f(A.s32, S, L_X, i), q = lane3f(A.s32, S, L_X, o);
-      F3 an = lane3f(A.s32, S, L_NX, i), ad = lane3f(A.s32, S, L_DX, i),
-         ax = lane3f(A.s32, S, L_XX, i);
-      F3 bn = lane3f(A.s32, S, L_NX, o), bd = lane3f(A.s32, S, L_DX, o),
-         bx = lane3f(A.s32, S, L_XX, o);
-      for (int sg = 0; sg < 2; ++sg) {
-        float s = sg == 0 ? 1.0f : -1.0f;
-        F3 b0 = bd * s, b1 = bx * s;
-        // R = [b0 b1 bn] [ad ax an]^T
-        float m[9];
-        float Bc[3][3] = {{b0.x, b1.x, bn.x}, {b0.y, b1.y, bn.y}, {b0.z, b1.z, bn.z}};
-        float Ac[3][3] = {{ad.x, ax.x, an.x}, {ad.y, ax.y, an.y}, {ad.z, ax.z, an.z}};
-        for (int rI = 0; rI < 3; ++rI)
-          for (int cI = 0; cI < 3; ++cI)
-            m[rI * 3 + cI] = Bc[rI][0] * Ac[cI][0] + Bc[rI][1] * Ac[cI][1] + Bc[rI][2] * Ac[cI][2];
-        float rp0 = m[0] * p.x + m[1] * p.y + m[2] * p.z;
-        float rp1 = m[3] * p.x + m[4] * p.y + m[5] * p.z;
-        float rp2 = m[6] * p.x + m[7] * p.y + m[8] * p.z;
-        float qq[4];
-        quat<float>(m, qq);
-        for (int kk = 0; kk < 4; ++kk)
-          rq[sg][kk] = static_cast<R>(qq[kk]);
-        rt[sg][0] = static_cast<R>(q.x - rp0), rt[sg][1] = static_cast<R>(q.y - rp1),
-        rt[sg][2] = static_cast<R>(q.z - rp2);
-      }
-      if (refl) {
-        F3 dv = q - p;
-        float len = sqrtf(fdot(dv, dv));
-        F3 n = dv * (1.0f / len);
-        float ax_ = fabsf(n.x), ay_ = fabsf(n.y), az_ = fabsf(n.z);
-        float m1 = fmaxf(ax_, fmaxf(ay_, az_));
-        float m2 = fmaxf(fminf(ax_, ay_), fminf(fmaxf(ax_, ay_), az_));
-        if (m1 - m2 <= A.gdom || !(len > 1e-6f)) {
-          Geo64 g = geo64(lane3(A.s64, S, L_X, i), lane3(A.s64, S, L_X, o),
-                          lane3(A.s64, S, L_NX, i), lane3(A.s64, S, L_DX, i),
-                          lane3(A.s64, S, L_XX, i), lane3(A.s64, S, L_NX, o),
-                          lane3(A.s64, S, L_DX, o), lane3(A.s64, S, L_XX, o));
-          V3 c = v3(cx, cy, cz);
-          pn[0] = static_cast<R>(g.n.x), pn[1] = static_cast<R>(g.n.y),
-          pn[2] = static_cast<R>(g.n.z);
-          pd = static_cast<R>(g.d + dot(g.n, c));
-        } else {
-          int dom = 0;
-          for (int kk = 1; kk < 3; ++kk)
-            if (fabsf(fcomp(n, kk)) > fabsf(fcomp(n, dom)))
-              dom = kk;
-          if (fcomp(n, dom) < 0)
-            n = n * -1.0f;
-          // plane offset about the centre: -n . (p' + q')/2
-          pn[0] = static_cast<R>(n.x), pn[1] = static_cast<R>(n.y), pn[2] = static_cast<R>(n.z);
-          pd = static_cast<R>(-fdot(n, (p + q) * 0.5f));
-        }
-      } else {
-        pn[0] = pn[1] = pn[2] = R(0);
-      }
-    }
-    // rigid slots 2k, 2k+1 (sign +1 then -1)
-    for (int kk = 0; kk < 4; ++kk) {
-      out.q4[kk * P2 + 2 * k] = rq[0][kk];
-      out.q4[kk * P2 + 2 * k + 1] = rq[1][kk];
-    }
-    for (int kk = 0; kk < 3; ++kk) {
-      out.t3[kk * P2 + 2 * k] = rt[0][kk];
-      out.t3[kk * P2 + 2 * k + 1] = rt[1][kk];
-    }
-    out.rp[2 * k] = static_cast<int32_t>(k);
-    out.rp[2 * k + 1] = static_cast<int32_t>(k);
-    if (refl && slot < out.R_total) {
-      const uint64_t RT = out.R_total;
-      out.n3[slot] = pn[0];
-      out.n3[RT + slot] = pn[1];
-      out.n3[2 * RT + slot] = pn[2];
-      out.d1[slot] = pd;
-      out.fp[slot] = static_cast<int32_t>(k);
-    }
-  }
-  rr = wave_sum(rr);
-  rd = wave_sum(rd);
-  if (lane == 0 && (rr || rd)) {
-    atomicAdd(&A.stats[2], static_cast<unsigned long long>(rr));
-    atomicAdd(&A.stats[3], static_cast<unsigned long long>(rd));
-  }
-}
-
-// --------------------------------------- reference-layout transformations
-// Backend::transformations output: 28 lanes in global vote order.
-struct TransArgs {
-  const double *x, *y, *z, *feat;
-  uint64_t n;
-  const int32_t *a, *b;
-  uint64_t P;
-  double box, cx, cy, cz;
-};
-__device__ inline Geo64 geo_mesh(const TransArgs &A, uint64_t k) {
-  uint64_t i = static_cast<uint64_t>(A.a[k]), o = static_cast<uint64_t>(A.b[k]);
-  const double *f = A.feat;
-  const uint64_t n = A.n;
-  return geo64(v3(A.x[i], A.y[i], A.z[i]), v3(A.x[o], A.y[o], A.z[o]),
-               v3(f[i], f[n + i], f[2 * n + i]), v3(f[3 * n + i], f[4 * n + i], f[5 * n + i]),
-               v3(f[6 * n + i], f[7 * n + i], f[8 * n + i]), v3(f[o], f[n + o], f[2 * n + o]),
-               v3(f[3 * n + o], f[4 * n + o], f[5 * n + o]),
-               v3(f[6 * n + o], f[7 * n + o], f[8 * n + o]));
-}
-__global__ void k_trans_count(TransArgs A, uint32_t *cnt) {
-  uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k < A.P)
-    cnt[k] = geo_mesh(A, k).refl ? 3u : 2u;
-}
-__device__ inline void put28(double *lanes, uint64_t V, uint64_t row, const M3 &r, V3 t, V3 n,
-                             double d, const double *desc) {
-  for (int k = 0; k < 9; ++k)
-    lanes[k * V + row] = r.m[k];
-  double v[7] = {t.x, t.y, t.z, n.x, n.y, n.z, d};
-  for (int k = 0; k < 7; ++k)
-    lanes[(9 + k) * V + row] = v[k];
-  for (int k = 0; k < 12; ++k)
-    lanes[(16 + k) * V + row] = desc[k];
-}
-__global__ void k_trans_fill(TransArgs A, const uint64_t *off, uint64_t V, double *lanes,
-                             int32_t *pair, uint8_t *kind) {
-  uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= A.P)
-    return;
-  Geo64 g = geo_mesh(A, k);
-  uint64_t row = off[k];
-  V3 c = v3(A.cx, A.cy, A.cz);
-  if (g.refl) {
-    double desc[12] = {g.n.x, g.n.y, g.n.z, (g.d + dot(g.n, c)) / A.box, 0, 0, 0, 0, 0, 0, 0, 0};
-    put28(lanes, V, row, g.mirror, g.n * (-2 * g.d), g.n, g.d, desc);
-    pair[row] = static_cast<int32_t>(k);
-    kind[row] = 1;
-    ++row;
-  }
-  for (int sg = 0; sg < 2; ++sg) {
-    M3 rot = rigid64(g, sg == 0 ? 1.0 : -1.0);
-    V3 t = g.q - apply(rot, g.p);
-    V3 cen = vdivs(apply(rot, c) + t - c, A.box);
-    double desc[12];
-    for (int kk = 0; kk < 9; ++kk)
-      desc[kk] = rot.m[kk] / 2;
-    desc[9] = cen.x, desc[10] = cen.y, desc[11] = cen.z;
-    put28(lanes, V, row, rot, t, v3(0, 0, 0), 0, desc);
-    pair[row] = static_cast<int32_t>(k);
-    kind[row] = 0;
-    ++row;
-  }
-}
-
 __global__ void k_read_total(const uint64_t *off, uint64_t n, uint64_t *dst) { *dst = off[n]; }
+
+__global__ void k_sum_stats(const uint32_t *a, const uint32_t *b, const uint32_t *c, uint64_t n,
+                            unsigned long long *out) {
+  __shared__ unsigned long long s[3][kThreads];
+  unsigned long long x = 0, y = 0, z = 0;
+  for (uint64_t i = threadIdx.x; i < n; i += kThreads) {
+    x += a[i];
+    y += b[i];
+    z += c[i];
+  }
+  s[0][threadIdx.x] = x;
+  s[1][threadIdx.x] = y;
+  s[2][threadIdx.x] = z;
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int k = 0; k < 3; ++k)
+        s[k][threadIdx.x] += s[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3)
+    out[threadIdx.x] = s[threadIdx.x][0];
+}
 
 // -------------------------------------------------------------- prune
 __global__ void k_prune_flags(const double *kmin, const double *kmax, const uint8_t *valid,
@@ -759,75 +541,96 @@ __global__ void k_compact(const int32_t *sample, const uint32_t *flag, const uin
 
 }  // namespace
 
-// Read back a few device words (one synchronisation).
-static void read_back(Session *s, void *dst, const void *src, size_t bytes) {
+void read_back(Session *s, void *dst, const void *src, size_t bytes) {
   void *h = host_stage(s, bytes);
   MVS_HIP(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, s->stream));
   MVS_HIP(hipStreamSynchronize(s->stream));
   std::memcpy(dst, h, bytes);
 }
 
-// Guard-band widths of the FP32 screen (relative).  Each bounds the rounding
-// of the FP32 evaluation (inputs rounded once to FP32, a handful of FP32
-// operations) with a wide margin; see the derivations next to the kernels.
+// Guard-band widths of the FP32 screen (relative).  Curvature: inputs rounded
+// once to FP32 and three FP32 operations give |err(d)| <= 4 eps32 M and
+// |err(tol)| <= 3 eps32 t M, i.e. (4/t + 3) eps32 relative to tol; the band
+// adds a factor 4 and a 1e-4 floor.  Distance: with every FP32 coordinate
+// bounded by rmax (measured during the gather), the computed distance
+// carries at most ~8 eps32 rmax of absolute error; the band is
+// 1e-4 + 16 eps32 rmax / dist (set on the device, see with_bands).
 static float guard_curvature(double t) {
   return static_cast<float>(1e-4 + 16.0 * 5.960464477539063e-08 * (1.0 + 1.0 / t));
-}
-static float guard_distance(double box, double dist) {
-  return static_cast<float>(1e-4 + 16.0 * 5.960464477539063e-08 * box / dist);
 }
 
 struct PairState {
   double *s64;
   float *s32;
-  double *sk;
-  float *sk32;
-  uint32_t *ord;
-  uint64_t *win, *counts, *offsets;
-  uint32_t *long_list;
-  unsigned long long *stats;
+  uint32_t *ord, *bord;
+  double *bk0, *bk1;
+  float *bk0f, *bk1f;
+  uint64_t *keys, *win, *counts, *offsets;
+  double *kw;
+  uint32_t *long_list, *cand, *cand2, *rechk;
 };
 
 static PairState pair_state(Session *s, uint64_t S, bool guarded) {
   PairState st{};
-  st.s64 = scratch_as<double>(s, kSlotSamples, L_N * S);
-  st.s32 = guarded ? scratch_as<float>(s, kSlotPlanSamples32, L_N * S) : nullptr;
-  st.sk = scratch_as<double>(s, kSlotPlanSortedK, 2 * S);
-  st.sk32 = guarded ? scratch_as<float>(s, kSlotPlanSortedK32, 2 * S) : nullptr;
-  st.ord = scratch_as<uint32_t>(s, kSlotPlanOrd, S);
-  st.win = scratch_as<uint64_t>(s, kSlotPlanWin, 2 * S);
-  st.counts = scratch_as<uint64_t>(s, kSlotPlanCounts, S + 1);
-  st.offsets = scratch_as<uint64_t>(s, kSlotPlanOffsets, S + 1);
-  st.long_list = scratch_as<uint32_t>(s, kSlotPlanLong, S + 1);
-  st.stats = scratch_as<unsigned long long>(s, kSlotPlanStats, 8);
+  const uint64_t Sp = S ? S : 1;
+  st.s64 = scratch_as<double>(s, kSlotSamples, L_N * Sp);
+  st.s32 = guarded ? scratch_as<float>(s, kSlotPlanSamples32, L_N * Sp) : nullptr;
+  st.ord = scratch_as<uint32_t>(s, kSlotPlanOrd, 2 * Sp);  // [0,S) sort input, [S,2S) sorted
+  double *bk = scratch_as<double>(s, kSlotPlanSortedK, 2 * Sp);
+  st.bk0 = bk;
+  st.bk1 = bk + Sp;
+  float *bkf = guarded ? scratch_as<float>(s, kSlotPlanSortedK32, 2 * Sp) : nullptr;
+  st.bk0f = bkf;
+  st.bk1f = bkf ? bkf + Sp : nullptr;
+  st.bord = scratch_as<uint32_t>(s, kSlotWindows, Sp);
+  st.keys = scratch_as<uint64_t>(s, kSlotPlanKeys, 2 * Sp);
+  st.win = scratch_as<uint64_t>(s, kSlotPlanWin, 2 * Sp);
+  st.kw = scratch_as<double>(s, kSlotMisc3, 2 * Sp);
+  st.counts = scratch_as<uint64_t>(s, kSlotPlanCounts, Sp + 1);
+  st.offsets = scratch_as<uint64_t>(s, kSlotPlanOffsets, Sp + 1);
+  st.long_list = scratch_as<uint32_t>(s, kSlotPlanLong, Sp + 1);
+  uint32_t *stats = scratch_as<uint32_t>(s, kSlotPlanStats, 3 * Sp);
+  st.cand = stats;
+  st.cand2 = stats + Sp;
+  st.rechk = stats + 2 * Sp;
   return st;
 }
 
-static PairArgs pair_args(const PairState &st, uint64_t S, double t, double box, bool guarded) {
+static PairArgs pair_args(const PairState &st, uint64_t S, double t, double box, bool guarded,
+                          const unsigned *flags) {
   PairArgs A{};
   A.s64 = st.s64;
   A.s32 = st.s32;
-  A.sk = st.sk;
-  A.sk32 = st.sk32;
-  A.ord = st.ord;
+  A.bord = st.bord;
+  A.bk0 = st.bk0;
+  A.bk1 = st.bk1;
+  A.bk0f = st.bk0f;
+  A.bk1f = st.bk1f;
   A.win = st.win;
+  A.kw = st.kw;
   A.S = S;
   A.t = t;
   A.dist = 0.03 * box;
   A.t32 = static_cast<float>(t);
   A.dist32 = static_cast<float>(A.dist);
   A.gk = guarded ? guard_curvature(t) : 0.f;
-  A.gd = guarded ? guard_distance(box, A.dist) : 0.f;
-  A.stats = st.stats;
+  A.gd = 0.f;
+  A.gd0 = 1e-4f;
+  A.gd1 = static_cast<float>(16.0 * 5.960464477539063e-08 / A.dist);
+  A.rmax_bits = flags + kRmaxWord;
+  A.cand = st.cand;
+  A.cand2 = st.cand2;
+  A.rechk = st.rechk;
   return A;
 }
 
-__global__ void k_zero_words(unsigned long long *p, int n) {
-  if (threadIdx.x < n)
-    p[threadIdx.x] = 0;
+const double *plan_samples64(Session *s) {
+  return scratch_as<double>(s, kSlotSamples, L_N * (s->pair_plan.S ? s->pair_plan.S : 1));
+}
+const float *plan_samples32(Session *s) {
+  return scratch_as<float>(s, kSlotPlanSamples32, L_N * (s->pair_plan.S ? s->pair_plan.S : 1));
 }
 
-// count phase shared by the host and device layers
 void pair_count_device(Session *s, uint64_t n, const double *x, const double *y, const double *z,
                        const double *feat, uint64_t S, const int32_t *sel, double box, double t,
                        uint64_t max_pairs, int precision, const double *center, uint64_t *P_out) {
@@ -844,30 +647,29 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   plan.box = box;
   plan.precision = precision;
   plan.selected = sel;
-  if (S < 2) {
+  const bool guarded = precision == MVS_FP32_GUARDED;
+  PairState st = pair_state(s, S, guarded);
+  unsigned *flags = plan_flags(s);
+  zero_device(s, flags, 16 * sizeof(unsigned));
+  double cx = center ? center[0] : 0, cy = center ? center[1] : 0, cz = center ? center[2] : 0;
+  if (S)
+    MVS_LAUNCH(s, "pair_gather", k_gather_samples, blocks_for(S, kThreads), kThreads, 0, x, y, z,
+               feat, n, sel, S, cx, cy, cz, st.s64, st.s32, st.keys, st.ord, flags,
+               flags + kRmaxWord);
+  if (S < 2) {  // no pair possible; like the reference, nothing to validate
     plan.P = 0;
     plan.valid = true;
     *P_out = 0;
-    // keep the sample SoA for vote emission of an empty pair set
     return;
   }
-  const bool guarded = precision == MVS_FP32_GUARDED;
-  PairState st = pair_state(s, S, guarded);
-  uint64_t *keys = scratch_as<uint64_t>(s, kSlotPlanKeys, 2 * S);
-  uint32_t *vals = scratch_as<uint32_t>(s, kSlotSort3, S);
-  unsigned *flags = scratch_as<unsigned>(s, kSlotFlags, 4);
-  MVS_HIP(hipMemsetAsync(flags, 0, 4 * sizeof(unsigned), s->stream));
-  MVS_LAUNCH(s, "zero_stats", k_zero_words, 1, 64, 0, st.stats, 8);
-  double cx = center ? center[0] : 0, cy = center ? center[1] : 0, cz = center ? center[2] : 0;
-  MVS_LAUNCH(s, "pair_gather", k_gather_samples, blocks_for(S, kThreads), kThreads, 0, x, y, z,
-             feat, n, sel, S, cx, cy, cz, st.s64, st.s32, keys, vals, flags);
-  radix_sort_pairs_u64(s, keys, vals, keys + S, st.ord, S, 64);
-  MVS_LAUNCH(s, "pair_sorted_gather", k_sorted_gather, blocks_for(S, kThreads), kThreads, 0,
-             st.ord, st.s64, st.s32, S, st.sk, st.sk32);
-  MVS_LAUNCH(s, "pair_windows", k_windows, blocks_for(S, kThreads), kThreads, 0, st.s64, st.sk, S,
-             t, st.win);
+  radix_sort_pairs_u64(s, st.keys, st.ord, st.keys + S, st.ord + S, S, 32);
+  const uint64_t nb = (S + kBucket - 1) / kBucket;
+  MVS_LAUNCH(s, "pair_index", k_bucket_index, blocks_for(nb, kWaves), kThreads, 0, st.ord + S,
+             st.s64, st.s32, S, st.bord, st.bk0, st.bk1, st.bk0f, st.bk1f);
+  MVS_LAUNCH(s, "pair_windows", k_windows, blocks_for(S, kThreads), kThreads, 0, st.s64,
+             st.keys + S, S, t, st.win, st.kw);
   unsigned *long_count = flags + 1;
-  PairArgs A = pair_args(st, S, t, box, guarded);
+  PairArgs A = pair_args(st, S, t, box, guarded, flags);
   unsigned grid = blocks_for(S, kWaves);
   if (guarded)
     MVS_LAUNCH(s, "pair_count", k_pair_count<true>, grid, kThreads, 0, A, st.counts, st.long_list,
@@ -876,7 +678,7 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
     MVS_LAUNCH(s, "pair_count", k_pair_count<false>, grid, kThreads, 0, A, st.counts,
                st.long_list, long_count);
   exclusive_scan_u64(s, st.counts, st.offsets, S);
-  // read back flags, long count and the total in one synchronisation
+  // flags, long-source count and total read back in one synchronisation
   uint64_t *tot = reinterpret_cast<uint64_t *>(flags + 2);
   MVS_LAUNCH(s, "read_total", k_read_total, 1, 1, 0, st.offsets, S, tot);
   unsigned host[4];
@@ -904,7 +706,7 @@ void pair_fill_device(Session *s, int32_t *a, int32_t *b, bool ordinals) {
   const uint64_t S = plan.S;
   const bool guarded = plan.precision == MVS_FP32_GUARDED;
   PairState st = pair_state(s, S, guarded);
-  PairArgs A = pair_args(st, S, plan.threshold, plan.box, guarded);
+  PairArgs A = pair_args(st, S, plan.threshold, plan.box, guarded, plan_flags(s));
   const int32_t *sel = ordinals ? nullptr : plan.selected;
   unsigned grid = blocks_for(S, kWaves);
   if (guarded)
@@ -913,94 +715,23 @@ void pair_fill_device(Session *s, int32_t *a, int32_t *b, bool ordinals) {
   else
     MVS_LAUNCH(s, "pair_fill", k_pair_fill<false>, grid, kThreads, 0, A, st.counts, st.offsets,
                sel, a, b);
-  if (plan.long_sources) {
-    unsigned lg = blocks_for(plan.long_sources, kWaves);
-    if (guarded)
-      MVS_LAUNCH(s, "pair_long", k_pair_long<true>, lg, kThreads, 0, A, st.long_list,
-                 static_cast<unsigned>(plan.long_sources), st.offsets, sel, a, b);
-    else
-      MVS_LAUNCH(s, "pair_long", k_pair_long<false>, lg, kThreads, 0, A, st.long_list,
-                 static_cast<unsigned>(plan.long_sources), st.offsets, sel, a, b);
-  }
+  if (plan.long_sources)
+    MVS_LAUNCH(s, "pair_long", k_pair_long, blocks_for(plan.long_sources, kWaves), kThreads, 0, A,
+               st.long_list, static_cast<unsigned>(plan.long_sources), st.offsets, sel, a, b);
 }
 
-static VoteArgs vote_args(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
-                          const double *center, bool guarded) {
+void pair_statistics(Session *s, uint64_t out[3]) {
   PairPlan &plan = s->pair_plan;
-  if (!plan.valid)
-    raise(MVS_EINVAL, "vote emission needs the sample plan of a pair count");
-  if (guarded && plan.precision != MVS_FP32_GUARDED)
-    raise(MVS_EINVAL, "FP32-guarded votes need an FP32-guarded pair plan");
-  const uint64_t S = plan.S;
-  VoteArgs A{};
-  A.s64 = scratch_as<double>(s, kSlotSamples, L_N * (S ? S : 1));
-  A.s32 = guarded ? scratch_as<float>(s, kSlotPlanSamples32, L_N * (S ? S : 1)) : nullptr;
-  A.S = S;
-  A.a = a;
-  A.b = b;
-  A.P = P;
-  A.cx = center ? center[0] : 0;
-  A.cy = center ? center[1] : 0;
-  A.cz = center ? center[2] : 0;
-  // |q' - p'| >= 0.03 box for every pair; FP32 centre-relative coordinates
-  // are bounded by box, so the direction carries a relative error of at
-  // most ~ 8 eps32 box / (0.03 box); the bands are 20x wider than that.
-  A.grefl = 2e-4f;
-  A.gdom = 2e-4f;
-  A.stats = scratch_as<unsigned long long>(s, kSlotPlanStats, 8);
-  return A;
-}
-
-void vote_count_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b, int precision,
-                       uint64_t *R_out) {
-  const bool guarded = precision == MVS_FP32_GUARDED;
-  VoteArgs A = vote_args(s, P, a, b, nullptr, guarded);
-  unsigned nb = blocks_for(P ? P : 1, kVoteTile);
-  uint32_t *blk = scratch_as<uint32_t>(s, kSlotVoteBlock, nb);
-  uint64_t *off = scratch_as<uint64_t>(s, kSlotVoteBlockOff, nb + 1);
-  if (P) {
-    if (guarded)
-      MVS_LAUNCH(s, "vote_count", k_vote_count<true>, nb, kThreads, 0, A, blk);
-    else
-      MVS_LAUNCH(s, "vote_count", k_vote_count<false>, nb, kThreads, 0, A, blk);
-  }
-  exclusive_scan_u32(s, blk, off, P ? nb : 0);
-  uint64_t *tot = scratch_as<uint64_t>(s, kSlotFlags, 4) + 1;
-  MVS_LAUNCH(s, "read_total", k_read_total, 1, 1, 0, off, static_cast<uint64_t>(P ? nb : 0), tot);
-  uint64_t R;
-  read_back(s, &R, tot, sizeof R);
-  s->vote_plan.P = P;
-  s->vote_plan.R = R;
-  s->vote_plan.precision = precision;
-  s->vote_plan.valid = true;
-  *R_out = R;
-}
-
-void vote_emit_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
-                      const double *center, int precision, void *q4, void *t3, int32_t *rp,
-                      void *n3, void *d1, int32_t *fp) {
-  const bool guarded = precision == MVS_FP32_GUARDED;
-  VotePlan &vp = s->vote_plan;
-  if (!vp.valid || vp.P != P || vp.precision != precision)
-    raise(MVS_EINVAL, "vote emission without a matching vote count");
-  if (P == 0)
+  out[0] = out[1] = out[2] = 0;
+  if (!plan.valid || plan.S < 2)
     return;
-  VoteArgs A = vote_args(s, P, a, b, center, guarded);
-  unsigned nb = blocks_for(P, kVoteTile);
-  uint64_t *off = scratch_as<uint64_t>(s, kSlotVoteBlockOff, nb + 1);
-  if (guarded) {
-    VoteOut<float> o{static_cast<float *>(q4), static_cast<float *>(t3), rp,
-                     static_cast<float *>(n3), static_cast<float *>(d1), fp, vp.R};
-    MVS_LAUNCH(s, "vote_emit", (k_vote_emit<true, float>), nb, kThreads, 0, A, off, o);
-  } else {
-    VoteOut<double> o{static_cast<double *>(q4), static_cast<double *>(t3), rp,
-                      static_cast<double *>(n3), static_cast<double *>(d1), fp, vp.R};
-    MVS_LAUNCH(s, "vote_emit", (k_vote_emit<false, double>), nb, kThreads, 0, A, off, o);
-  }
-}
-
-void pair_stats(Session *s, unsigned long long out[4]) {
-  read_back(s, out, scratch_as<unsigned long long>(s, kSlotPlanStats, 8), 4 * sizeof(uint64_t));
+  PairState st = pair_state(s, plan.S, plan.precision == MVS_FP32_GUARDED);
+  unsigned long long *dst = scratch_as<unsigned long long>(s, kSlotBlock, 4);
+  MVS_LAUNCH(s, "pair_stats", k_sum_stats, 1, kThreads, 0, st.cand, st.cand2, st.rechk, plan.S,
+             dst);
+  unsigned long long h[3];
+  read_back(s, h, dst, sizeof h);
+  out[0] = h[0], out[1] = h[1], out[2] = h[2];
 }
 
 }  // namespace mvs
@@ -1011,13 +742,13 @@ extern "C" {
 
 int mvs_dev_pair_count(mvs_session *sp, uint64_t n, const double *x, const double *y,
                        const double *z, const double *feat, uint64_t S, const int32_t *sel,
-                       double box, double t, uint64_t max_pairs, int precision,
-                       uint64_t *pairs) {
+                       double box, const double center[3], double t, uint64_t max_pairs,
+                       int precision, uint64_t *pairs) {
   return guarded([&] {
     Session *s = as_session(sp);
-    if (!pairs)
-      raise(MVS_EINVAL, "null output pointer");
-    pair_count_device(s, n, x, y, z, feat, S, sel, box, t, max_pairs, precision, nullptr, pairs);
+    if (!pairs || !center)
+      raise(MVS_EINVAL, "null argument");
+    pair_count_device(s, n, x, y, z, feat, S, sel, box, t, max_pairs, precision, center, pairs);
   });
 }
 
@@ -1025,21 +756,17 @@ int mvs_dev_pair_fill(mvs_session *sp, int32_t *a, int32_t *b, int ordinals) {
   return guarded([&] { pair_fill_device(as_session(sp), a, b, ordinals != 0); });
 }
 
-int mvs_dev_vote_count(mvs_session *sp, uint64_t P, const int32_t *a, const int32_t *b,
-                       int precision, uint64_t *reflections) {
+int mvs_dev_pair_stats(mvs_session *sp, uint64_t *window_candidates, uint64_t *index_candidates,
+                       uint64_t *fp64_rechecks) {
   return guarded([&] {
-    if (!reflections)
-      raise(MVS_EINVAL, "null output pointer");
-    vote_count_device(as_session(sp), P, a, b, precision, reflections);
-  });
-}
-
-int mvs_dev_vote_emit(mvs_session *sp, uint64_t P, const int32_t *a, const int32_t *b, double box,
-                      const double center[3], int precision, void *q4, void *t3, int32_t *rp,
-                      void *n3, void *d1, int32_t *fp) {
-  return guarded([&] {
-    (void)box;
-    vote_emit_device(as_session(sp), P, a, b, center, precision, q4, t3, rp, n3, d1, fp);
+    uint64_t st[3];
+    pair_statistics(as_session(sp), st);
+    if (window_candidates)
+      *window_candidates = st[0];
+    if (index_candidates)
+      *index_candidates = st[1];
+    if (fp64_rechecks)
+      *fp64_rechecks = st[2];
   });
 }
 
@@ -1063,7 +790,7 @@ int mvs_prune(mvs_session *sp, const mvs_features *f, uint64_t ns, const int32_t
     copy_h2d(s, k.ptr + n, f->lane[10], n * 8);
     copy_h2d(s, valid.ptr, f->valid, n);
     copy_h2d(s, dsample.ptr, sample, ns * 4);
-    MVS_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(unsigned), s->stream));
+    zero_device(s, bad.ptr, sizeof(unsigned));
     MVS_LAUNCH(s, "prune_flags", k_prune_flags, blocks_for(ns, 256), 256, 0, k.ptr, k.ptr + n,
                valid.ptr, n, dsample.ptr, ns, 1 - prune, flag.ptr, bad.ptr);
     exclusive_scan_u32(s, flag.ptr, off.ptr, ns);
@@ -1102,9 +829,23 @@ int mvs_pairing(mvs_session *sp, const mvs_points *v, const mvs_features *f, uin
     for (int k = 0; k < 11; ++k)
       copy_h2d(s, feat.ptr + k * n, f->lane[k], n * 8);
     copy_h2d(s, dsel.ptr, selected, S * 4);
+    // FP32 coordinates are taken relative to the bounding-box midpoint, which
+    // keeps them within the box size the guard bands assume
+    double ctr[3] = {0, 0, 0};
+    if (n) {
+      const double *c3[3] = {v->x, v->y, v->z};
+      for (int k = 0; k < 3; ++k) {
+        double lo = c3[k][0], hi = c3[k][0];
+        for (uint64_t i = 1; i < n; ++i) {
+          lo = c3[k][i] < lo ? c3[k][i] : lo;
+          hi = hi < c3[k][i] ? c3[k][i] : hi;
+        }
+        ctr[k] = std::isfinite(lo) && std::isfinite(hi) ? 0.5 * (lo + hi) : 0.0;
+      }
+    }
     uint64_t P = 0;
     pair_count_device(s, n, dv.ptr, dv.ptr + n, dv.ptr + 2 * n, feat.ptr, S, dsel.ptr, box, t,
-                      max_pairs, precision, nullptr, &P);
+                      max_pairs, precision, ctr, &P);
     s->out_a.ensure(P ? P : 1);
     s->out_b.ensure(P ? P : 1);
     pair_fill_device(s, s->out_a.ptr, s->out_b.ptr, false);
@@ -1119,71 +860,6 @@ int mvs_pairing_fetch(mvs_session *sp, int32_t *a, int32_t *b) {
     Session *s = as_session(sp);
     copy_d2h(s, a, s->out_a.ptr, s->out_count * 4);
     copy_d2h(s, b, s->out_b.ptr, s->out_count * 4);
-  });
-}
-
-int mvs_transformations(mvs_session *sp, const mvs_points *v, const mvs_features *f,
-                        uint64_t P, const int32_t *a, const int32_t *b, double box,
-                        const double center[3], uint64_t max_votes, uint64_t *count) {
-  return guarded([&] {
-    Session *s = as_session(sp);
-    if (!v || !f || !count || (P && (!a || !b)) || !center)
-      raise(MVS_EINVAL, "null argument");
-    *count = 0;
-    s->out_count = 0;
-    if (P == 0)
-      return;
-    uint64_t n = v->count;
-    if (f->count != n)
-      raise(MVS_EINVAL, "SoA component lengths differ");
-    if (!(box > 0) || !std::isfinite(box) || !std::isfinite(center[0]) ||
-        !std::isfinite(center[1]) || !std::isfinite(center[2]))
-      raise(MVS_EINVAL, "transformations require a positive finite box size and finite center");
-    if (P > static_cast<uint64_t>(std::numeric_limits<int32_t>::max()))
-      raise(MVS_EOVERFLOW, "pair ids exceed int32 range");
-    for (uint64_t i = 0; i < P; ++i)
-      if (a[i] < 0 || b[i] < 0 || static_cast<uint64_t>(a[i]) >= n ||
-          static_cast<uint64_t>(b[i]) >= n)
-        raise(MVS_EINVAL, "pair vertex index out of range");
-    DBuf<double> dv(3 * n), feat(11 * n);
-    DBuf<int32_t> da(P), db(P);
-    DBuf<uint32_t> cnt(P);
-    DBuf<uint64_t> off(P + 1);
-    copy_h2d(s, dv.ptr, v->x, n * 8);
-    copy_h2d(s, dv.ptr + n, v->y, n * 8);
-    copy_h2d(s, dv.ptr + 2 * n, v->z, n * 8);
-    for (int k = 0; k < 11; ++k)
-      copy_h2d(s, feat.ptr + k * n, f->lane[k], n * 8);
-    copy_h2d(s, da.ptr, a, P * 4);
-    copy_h2d(s, db.ptr, b, P * 4);
-    TransArgs A{dv.ptr, dv.ptr + n, dv.ptr + 2 * n, feat.ptr, n, da.ptr, db.ptr, P,
-                box,    center[0],  center[1],     center[2]};
-    MVS_LAUNCH(s, "trans_count", k_trans_count, blocks_for(P, 256), 256, 0, A, cnt.ptr);
-    exclusive_scan_u32(s, cnt.ptr, off.ptr, P);
-    uint64_t V = 0;
-    copy_d2h(s, &V, off.ptr + P, sizeof V);
-    if (V > max_votes || V > static_cast<uint64_t>(std::numeric_limits<int32_t>::max()))
-      raise(MVS_EBUDGET, "Vote budget exceeded; lower --sample-ratio or tighten --pair-threshold");
-    s->out_lanes.ensure(28 * V);
-    s->out_pair.ensure(V);
-    s->out_kind.ensure(V);
-    MVS_LAUNCH(s, "trans_fill", k_trans_fill, blocks_for(P, 256), 256, 0, A, off.ptr, V,
-               s->out_lanes.ptr, s->out_pair.ptr, s->out_kind.ptr);
-    sync(s);
-    s->out_count = V;
-    *count = V;
-  });
-}
-
-int mvs_transformations_fetch(mvs_session *sp, int32_t *pair, uint8_t *kind,
-                              double *const lanes[28]) {
-  return guarded([&] {
-    Session *s = as_session(sp);
-    uint64_t V = s->out_count;
-    copy_d2h(s, pair, s->out_pair.ptr, V * 4);
-    copy_d2h(s, kind, s->out_kind.ptr, V);
-    for (int k = 0; k < 28; ++k)
-      copy_d2h(s, lanes[k], s->out_lanes.ptr + k * V, V * 8);
   });
 }
 

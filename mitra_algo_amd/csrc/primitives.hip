@@ -230,7 +230,29 @@ __global__ void __launch_bounds__(kSortThreads)
   }
 }
 
+__global__ void k_zero_bytes(unsigned char *p, uint64_t bytes) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint64_t words = bytes / 4;
+  for (uint64_t w = i; w < words; w += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    reinterpret_cast<uint32_t *>(p)[w] = 0u;
+  if (i < bytes - 4 * words)
+    p[4 * words + i] = 0;
+}
+
 }  // namespace
+
+// Kernel fill instead of hipMemsetAsync: stays correct inside captured graphs.
+void zero_device(Session *s, void *ptr, size_t bytes) {
+  if (!bytes)
+    return;
+  if (reinterpret_cast<uintptr_t>(ptr) % 4 != 0)
+    raise(MVS_EINVAL, "zero_device needs 4-byte alignment");
+  unsigned blocks = blocks_for((bytes / 4) ? bytes / 4 : 1, 256);
+  if (blocks > 4096)
+    blocks = 4096;
+  MVS_LAUNCH(s, "zero", k_zero_bytes, blocks, 256, 0, static_cast<unsigned char *>(ptr),
+             static_cast<uint64_t>(bytes));
+}
 
 void exclusive_scan_u64(Session *s, const uint64_t *in, uint64_t *out, uint64_t n) {
   exclusive_scan_impl<uint64_t>(s, in, out, n, kSlotScan);

@@ -89,7 +89,7 @@ class Workload:
     def pair_votes(self, precision: int, max_pairs: int = 2**31 - 1) -> PairVoteResult:
         s = self.sess
         P = s.dev_pair_count(self.n, self.x, self.y, self.z, self.feat, self.S, self.selected,
-                             self.box, self.threshold, max_pairs, precision)
+                             self.box, self.center, self.threshold, max_pairs, precision)
         a = torch.empty(max(P, 1), dtype=torch.int32, device=self.dev)
         b = torch.empty(max(P, 1), dtype=torch.int32, device=self.dev)
         s.dev_pair_fill(a, b, True)

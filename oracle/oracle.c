@@ -869,6 +869,7 @@ int oracle_meanshift_batched(int64_t nv, int dim, const double *pts,
   if (dim < 1 || dim > 12 || dim_rigid > dim || dim_refl > dim)
     return fail(OR_EINVAL, "bad dimension");
   double r2 = cutoff * h * cutoff * h, inv = 1 / (2 * h * h);
+  double floor_w = exp(-0.5 * cutoff * cutoff);
   for (int64_t s = 0; s < ns; ++s) {
     double m[12] = {0}, acc[12];
     int refl = seed_kind[s] != 0, d = refl ? dim_refl : dim_rigid;
@@ -888,7 +889,7 @@ int oracle_meanshift_batched(int64_t nv, int dim, const double *pts,
           d2 += e * e;
         }
         if (d2 < r2) {
-          double wj = exp(-d2 * inv);
+          double wj = exp(-d2 * inv) - floor_w;
           wsum += wj;
           for (int k = 0; k < d; ++k)
             acc[k] += wj * (pts[(int64_t)k * nv + j] - m[k]);

@@ -132,9 +132,11 @@ int oracle_clustering(int64_t nv, const uint8_t *kind, const double *desc,
                       int32_t *limited_iterations);
 
 /* Batched-seed mean-shift (no reference counterpart).  Each seed s runs
- * exactly `iterations` Gaussian-kernel updates over all votes of its kind:
- *   w_j = exp(-d2_j / (2 h^2)) for d2_j < (cutoff*h)^2, else 0
- *   mean <- sum_j w_j x_j / sum_j w_j   (unchanged when sum_j w_j == 0)
+ * exactly `iterations` Gaussian-kernel updates over all votes of its kind,
+ * with the Gaussian truncated at cutoff*h and shifted to reach zero there
+ * (continuous kernel):
+ *   w_j = exp(-d2_j / (2 h^2)) - exp(-cutoff^2 / 2) for d2_j < (cutoff*h)^2
+ *   mean <- mean + sum_j w_j (x_j - mean) / sum_j w_j  (unchanged if no support)
  * points: dim*nv SoA (stride nv), kinds per vote; seeds: dim*ns SoA holding the
  * start positions and seed_kind per seed.  Rigid votes use the first
  * dim_rigid lanes, reflection votes the first dim_reflection lanes.

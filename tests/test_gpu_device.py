@@ -1,0 +1,154 @@
+"""Device-layer parity: the fused fast path (pair count/fill on device-resident
+features, compact vote emission, batched mean-shift) against the oracle.
+
+Bars: pair set byte-equal for both arithmetics; FP64 compact votes equal
+the oracle's rotation / centred translation / plane within 1e-12 (the
+quaternion round trip costs a few ulps); FP32-guarded votes carry exactly
+the FP64 mirror decisions and agree within 2e-5 (rotation entries, plane
+normal) and 2e-5 box (translations, plane offset); batched mean-shift FP64
+within 1e-9 of the oracle, FP32 within 2e-5.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from mitra_algo_amd import capi, meshgen, workload  # noqa: E402
+from oracle.pyoracle import Oracle  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def quat_to_R(q):
+    w, x, y, z = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+        [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+        [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    return workload.cfg1()
+
+
+def _oracle_votes(oracle, wl, pa, pb):
+    v = wl.mesh.vertices
+    return oracle.transformations(v, wl.feat_np, pa, pb, wl.box, wl.center, 10**9)
+
+
+@pytest.mark.parametrize("t", [0.05, 0.15])
+def test_compact_votes_match_oracle(cfg1, oracle, t):
+    wl = cfg1
+    wl.threshold = t
+    v = wl.mesh.vertices
+    pa, pb = oracle.pairing(v, wl.feat_np[9], wl.feat_np[10], wl.selected_np, wl.box, t, 10**8)
+    pair, kind, lanes = _oracle_votes(oracle, wl, pa, pb)
+    ref = {}
+    for prec in (capi.FP64, capi.FP32_GUARDED):
+        r = wl.pair_votes(prec)
+        sel = wl.selected_np
+        a = sel[r.a[:r.P].cpu().numpy()]
+        b = sel[r.b[:r.P].cpu().numpy()]
+        assert np.array_equal(a, pa) and np.array_equal(b, pb)
+        q = r.q4[:, :2 * r.P].double().cpu().numpy()
+        tc = r.t3[:, :2 * r.P].double().cpu().numpy()
+        n3 = r.n3[:, :r.R].double().cpu().numpy()
+        d1 = r.d1[:r.R].double().cpu().numpy()
+        fp = r.fp[:r.R].cpu().numpy()
+        rp = r.rp[:2 * r.P].cpu().numpy()
+        assert np.array_equal(rp, np.repeat(np.arange(r.P), 2))
+        # oracle votes per pair: [mirror], rigid(+), rigid(-)
+        refl_pairs = pair[kind == 1]
+        assert np.array_equal(fp, refl_pairs)
+        rig = np.where(kind == 0)[0]
+        assert len(rig) == 2 * r.P
+        tol = 1e-12 if prec == capi.FP64 else 2e-5
+        R_or = lanes[0:9, rig].T.reshape(-1, 3, 3)
+        R_gpu = np.stack([quat_to_R(q[:, i]) for i in range(2 * r.P)])
+        assert np.max(np.abs(R_gpu - R_or)) < tol * 10
+        tc_or = lanes[25:28, rig] * wl.box
+        assert np.max(np.abs(tc - tc_or)) < tol * 10 * wl.box
+        mir = np.where(kind == 1)[0]
+        assert np.max(np.abs(n3 - lanes[12:15, mir]), initial=0) < tol * 10
+        assert np.max(np.abs(d1 - lanes[19, mir] * wl.box), initial=0) < tol * 10 * wl.box
+        ref[prec] = r
+    st = wl.sess.dev_pair_stats()
+    assert st["index_candidates"] <= st["window_candidates"]
+
+
+def test_pair_stats_and_identity_of_arithmetics():
+    wl = workload.Workload(meshgen.cfg2_mesh(256), 20000, 0.02, False)
+    r64 = wl.pair_votes(capi.FP64)
+    s64 = wl.sess.dev_pair_stats()
+    r32 = wl.pair_votes(capi.FP32_GUARDED)
+    s32 = wl.sess.dev_pair_stats()
+    assert r64.P == r32.P and r64.R == r32.R and r64.P > 0
+    assert torch.equal(r64.a[:r64.P], r32.a[:r32.P]) and torch.equal(r64.b[:r64.P], r32.b[:r32.P])
+    assert torch.equal(r64.fp[:r64.R], r32.fp[:r32.R])
+    assert s32["fp64_rechecks"] < s32["index_candidates"]
+    assert s64["fp64_rechecks"] == 0
+    o = Oracle("oracle")
+    pa, pb = o.pairing(wl.mesh.vertices, wl.feat_np[9], wl.feat_np[10], wl.selected_np, wl.box,
+                       0.02, 10**9)
+    sel = wl.selected_np
+    assert np.array_equal(sel[r64.a[:r64.P].cpu().numpy()], pa)
+    assert np.array_equal(sel[r64.b[:r64.P].cpu().numpy()], pb)
+
+
+def _clusters(D, seed):
+    g = np.random.default_rng(seed)
+    centers = g.uniform(-1, 1, (3, D))
+    pts = [c + g.normal(0, 0.02, (300, D)) for c in centers]
+    pts.append(g.uniform(-1.2, 1.2, (200, D)))
+    P = np.concatenate(pts).T.copy()
+    seeds = P[:, g.choice(P.shape[1], 64, replace=False)].copy()
+    return P, seeds
+
+
+@pytest.mark.parametrize("D", [6, 4])
+def test_batched_meanshift_matches_oracle(oracle, D):
+    P, seeds = _clusters(D, 7 + D)
+    h, cut, iters = 0.05, 3.0, 25
+    want, wsum = oracle.meanshift_batched(P, np.zeros(P.shape[1], np.uint8), D, D, seeds,
+                                          np.zeros(seeds.shape[1], np.uint8), h, cut, iters)
+    s = capi.Session(0)
+    dev = torch.device("cuda:0")
+    s.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    for prec, dt, tol in ((capi.FP64, torch.float64, 1e-9), (capi.FP32_GUARDED, torch.float32,
+                                                             2e-5)):
+        pts = torch.as_tensor(P, dtype=dt, device=dev).contiguous()
+        sd = torch.as_tensor(seeds, dtype=dt, device=dev).contiguous()
+        out = torch.empty_like(sd)
+        w = torch.empty(seeds.shape[1], dtype=dt, device=dev)
+        cand, ins = s.dev_ms_batched(prec, D, P.shape[1], pts, seeds.shape[1], sd, iters, h, cut,
+                                     out, w, stats=True)
+        torch.cuda.synchronize()
+        got = out.double().cpu().numpy()
+        assert np.max(np.abs(got - want)) < tol, float(np.max(np.abs(got - want)))
+        assert np.max(np.abs(w.double().cpu().numpy() - wsum) / (1 + wsum)) < tol * 100
+        assert 0 < ins <= cand
+    s.close()
+
+
+def test_batched_meanshift_empty_and_degenerate(oracle):
+    s = capi.Session(0)
+    dev = torch.device("cuda:0")
+    s.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    pts = torch.zeros((6, 0), dtype=torch.float64, device=dev)
+    sd = torch.ones((6, 3), dtype=torch.float64, device=dev)
+    out = torch.empty_like(sd)
+    w = torch.empty(3, dtype=torch.float64, device=dev)
+    s.dev_ms_batched(capi.FP64, 6, 0, pts, 3, sd, 5, 0.05, 3.0, out, w)
+    torch.cuda.synchronize()
+    assert torch.equal(out, sd) and torch.all(w == 0)
+    # all points identical: one iteration lands exactly on them
+    P = np.tile(np.array([[0.3], [0.1], [-0.2], [0.05], [0.0], [0.01]]), (1, 50))
+    pts = torch.as_tensor(P, device=dev).contiguous()
+    sd = torch.as_tensor(P[:, :2] + 0.01, device=dev).contiguous()
+    out = torch.empty_like(sd)
+    w = torch.empty(2, dtype=torch.float64, device=dev)
+    s.dev_ms_batched(capi.FP64, 6, 50, pts, 2, sd, 3, 0.05, 3.0, out, w)
+    torch.cuda.synchronize()
+    assert np.allclose(out.cpu().numpy(), P[:, :2], atol=1e-15)
+    s.close()
