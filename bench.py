@@ -5,12 +5,15 @@ noise 1e-4), 100,000 uniform vertex samples pruned by the reference's
 anisotropy test, curvature-ratio tolerance .02.  Inputs (mesh, topology,
 features, selected samples) are resident in HBM before timing starts.
 
-One step = one pass of pairing (signature sort, window count, scan, fill)
-plus vote emission (reflection count, scan, compact vote records) over the
-selected samples, in the FP32-guarded arithmetic.  `value` is pair votes per
-second of that pass; the plain FP64 path is timed the same way and reported
-beside it, together with the emission kernel's roofline and the reference's
-own CPU code (oracle/_ref) timed on a bounded sample on this host.
+One step = one pass of the voting core over the selected samples:
+pairing (signature sort, two-level index, walk, scan, placement) + vote
+emission (mirror offsets, compact records) + batched mean-shift (20,000 seed
+votes x 30 iterations, h = .05, kernel cut at 3h), in the FP32-guarded
+arithmetic.  `value` is pair votes per second of the whole step; the plain
+FP64 path is timed the same way and reported beside it, with per-kernel
+device times (HIP events on the session stream), the dominant kernel's
+roofline and the reference's own CPU code (oracle/_ref) timed on a bounded
+sample on this host.
 """
 import argparse
 import json
@@ -22,20 +25,33 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+SEEDS, ITERS, H, CUT = 20000, 30, 0.05, 3.0
+KERNELS = ("pair_gather", "radix_hist", "radix_scatter", "scan_reduce", "scan_apply", "scan_zero",
+           "pair_index", "pair_windows", "pair_walk", "pair_place", "pair_long", "vote_count",
+           "vote_emit", "read_total", "zero", "ms_points", "ms_bbox", "ms_cell_keys", "ms_gather",
+           "ms_query")
 
 
 def emission_bytes(P, R, S, s):
     """Algorithmic bytes of vote emission: pair ordinals in, compact records
-    out (rigid q4+t3 and reflection n3+d at s bytes per scalar, int32 pair
-    id each), the 14-lane sample SoA read once."""
+    out (rigid q4+t3 and mirror n3+d at s bytes per scalar, int32 pair id
+    each), the 14-scalar sample records read once."""
     return 8 * P + 2 * P * (7 * s + 4) + R * (4 * s + 4) + 14 * s * S
+
+
+def meanshift_flops(cand, inside, D=6):
+    """Algorithmic flops of the mean-shift queries: 3D per candidate visit
+    (difference, square, accumulate) and 2D+4 per in-support visit (weight,
+    weighted sums)."""
+    return cand * 3 * D + inside * (2 * D + 4)
 
 
 def cpu_baseline(wl, budget_s=20.0):
     """Reference CPU pairing + transformations (oracle/_ref, single thread) on
     the first S_sub selected samples; votes/s is size-independent to first
     order (pairs and tests both grow as S^2)."""
-    import numpy as np
     from oracle.pyoracle import Oracle
     try:
         lib, kind = Oracle("ref"), "reference"
@@ -56,7 +72,8 @@ def cpu_baseline(wl, budget_s=20.0):
         S_sub = min(wl.S, int(S_sub * 1.6))
     return {"value": len(pair) / dt, "unit": "votes/s", "cores": 1, "kind": kind,
             "sample": f"pairing + transformations over the first {S_sub} of {wl.S} selected "
-                      f"samples ({len(pa)} pairs, {len(pair)} votes) in {dt:.2f} s"}
+                      f"samples ({len(pa)} pairs, {len(pair)} votes) in {dt:.2f} s, "
+                      "single thread"}
 
 
 def main():
@@ -69,7 +86,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the voting core runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+              "(the multi-GPU sharding is a design note, not run)", file=sys.stderr)
         sys.exit(2)
     import torch
     from mitra_algo_amd import capi, workload
@@ -79,38 +96,73 @@ def main():
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s S={wl.S}", file=sys.stderr,
           flush=True)
 
-    def run(precision, steps, warmup, timing=False):
+    def run(precision, steps, warmup):
         for _ in range(warmup):
-            r = wl.pair_votes(precision)
+            wl.step(precision, SEEDS, ITERS)
         torch.cuda.synchronize()
-        wl.sess.enable_timing(timing)
+        # timed region: K full steps, kernels timed with HIP events
+        wl.sess.enable_timing(True)
+        t_pv = t_ms = 0.0
         t0 = time.perf_counter()
         for _ in range(steps):
-            r = wl.pair_votes(precision)
-        torch.cuda.synchronize()
+            a = time.perf_counter()
+            pv = wl.pairvote(precision)
+            b = time.perf_counter()
+            ms = wl.meanshift(SEEDS, ITERS)
+            torch.cuda.synchronize()
+            c = time.perf_counter()
+            t_pv += b - a
+            t_ms += c - b
         dt = (time.perf_counter() - t0) / steps
         kernels = {}
-        if timing:
-            for k in ("pair_count", "pair_fill", "pair_long", "vote_count", "vote_emit",
-                      "radix_scatter", "radix_hist", "pair_windows", "pair_gather"):
-                ms, n = wl.sess.timing(k)
-                if n:
-                    kernels[k] = {"ms_avg": ms / n, "launches": n}
-            wl.sess.enable_timing(False)
-        return r, dt, kernels
+        for k in KERNELS:
+            ms_, n = wl.sess.timing(k)
+            if n:
+                kernels[k] = {"ms_total_per_step": ms_ / steps, "ms_avg": ms_ / n,
+                              "launches_per_step": n / steps}
+        wl.sess.enable_timing(False)
+        stats = wl.meanshift(SEEDS, ITERS, want_stats=True)
+        return pv, stats, dt, t_pv / steps, t_ms / steps, kernels
 
-    r32, dt32, k32 = run(capi.FP32_GUARDED, args.steps, args.warmup, timing=True)
-    r64, dt64, k64 = run(capi.FP64, max(3, args.steps // 2), 1, timing=True)
-    # pair sets must agree bit for bit between the two arithmetics
-    same = (r32.P == r64.P and torch.equal(r32.a[:r32.P], r64.a[:r64.P])
-            and torch.equal(r32.b[:r32.P], r64.b[:r64.P]) and r32.R == r64.R)
-    print(f"[bench] fp32g {dt32 * 1e3:.3f} ms fp64 {dt64 * 1e3:.3f} ms P={r32.P} R={r32.R} "
-          f"pairs_equal={same}", file=sys.stderr, flush=True)
-    V = r32.V
+    pv32, ms32, dt32, pv32_s, ms32_s, k32 = run(capi.FP32_GUARDED, args.steps, args.warmup)
+    pair32 = (wl.sess.fetch(pv32.a, (pv32.pairs,), torch.int32).cpu(),
+              wl.sess.fetch(pv32.b, (pv32.pairs,), torch.int32).cpu(),
+              wl.sess.fetch(pv32.refl_pair, (pv32.reflections,), torch.int32).cpu())
+    st32 = wl.sess.dev_pair_stats()
+    pv64, ms64, dt64, pv64_s, ms64_s, k64 = run(capi.FP64, max(3, args.steps // 2), 1)
+    pair64 = (wl.sess.fetch(pv64.a, (pv64.pairs,), torch.int32).cpu(),
+              wl.sess.fetch(pv64.b, (pv64.pairs,), torch.int32).cpu(),
+              wl.sess.fetch(pv64.refl_pair, (pv64.reflections,), torch.int32).cpu())
+    same = (pv32.pairs == pv64.pairs and pv32.reflections == pv64.reflections
+            and all(torch.equal(x, y) for x, y in zip(pair32, pair64)))
+    P, R = pv32.pairs, pv32.reflections
+    V = 2 * P + R
+    print(f"[bench] fp32g {dt32 * 1e3:.3f} ms (pairvote {pv32_s * 1e3:.3f}, meanshift "
+          f"{ms32_s * 1e3:.3f}) fp64 {dt64 * 1e3:.3f} ms (pairvote {pv64_s * 1e3:.3f}, "
+          f"meanshift {ms64_s * 1e3:.3f}) P={P} R={R} pairs_equal={same}", file=sys.stderr,
+          flush=True)
+
     emit_ms = k32["vote_emit"]["ms_avg"]
-    achieved = emission_bytes(r32.P, r32.R, wl.S, 4) / (emit_ms * 1e-3) / 1e9
-    emit64_ms = k64["vote_emit"]["ms_avg"]
-    achieved64 = emission_bytes(r64.P, r64.R, wl.S, 8) / (emit64_ms * 1e-3) / 1e9
+    emit_gbs = emission_bytes(P, R, wl.S, 4) / (emit_ms * 1e-3) / 1e9
+    q_ms = k32["ms_query"]["ms_total_per_step"]
+    q_tflops = meanshift_flops(ms32.candidates, ms32.in_support) / (q_ms * 1e-3) / 1e12
+    q64_ms = k64["ms_query"]["ms_total_per_step"]
+    q64_tflops = meanshift_flops(ms64.candidates, ms64.in_support) / (q64_ms * 1e-3) / 1e12
+    rooflines = {
+        "vote_emit": {"bound": "hbm", "achieved": emit_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": emit_gbs / HBM_PEAK_GBS, "traffic": None,
+                      "algorithmic_bytes": emission_bytes(P, R, wl.S, 4),
+                      "ms_avg": emit_ms},
+        "ms_query": {"bound": "mfma", "pipe": "VALU FP32 (no MFMA: no dense tile)",
+                     "achieved": q_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": q_tflops / FP32_PEAK_TFLOPS, "traffic": None,
+                     "algorithmic_flops": meanshift_flops(ms32.candidates, ms32.in_support),
+                     "ms_per_step": q_ms},
+    }
+    dominant = max(("pair_walk", "vote_emit", "ms_query"),
+                   key=lambda k: k32.get(k, {}).get("ms_total_per_step", 0))
+    roof = dict(rooflines.get(dominant, rooflines["vote_emit"]))
+    roof["kernel"] = dominant if dominant in rooflines else "vote_emit"
     out = {
         "metric": "pair votes/s + mean-shift s on 2.1M-tri mesh: FP32-guarded vs FP64 GPU vs CPU",
         "value": V / dt32,
@@ -126,17 +178,22 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"height field {args.grid}x{args.grid} "
                                f"({2 * (args.grid - 1) ** 2} triangles), 64 motif copies, "
-                               f"100000 uniform samples -> {wl.S} after prune, t=0.02",
-                   "pairs": r32.P, "votes": V, "reflection_votes": r32.R,
+                               f"100000 uniform samples -> {wl.S} after prune, t=0.02; "
+                               f"mean-shift {SEEDS} seeds x {ITERS} iterations, h={H}",
+                   "pairs": P, "votes": V, "reflection_votes": R,
                    "parallelism": "single GPU"},
-        "fp64": {"value": r64.V / dt64, "ms_per_step": dt64 * 1e3,
+        "pairvote_votes_per_s": V / pv32_s,
+        "pairvote_ms": pv32_s * 1e3,
+        "meanshift_s": ms32_s,
+        "meanshift_iterations_per_s": SEEDS * ITERS / ms32_s,
+        "fp64": {"value": V / dt64, "ms_per_step": dt64 * 1e3, "pairvote_ms": pv64_s * 1e3,
+                 "meanshift_s": ms64_s, "ms_query_tflops": q64_tflops,
                  "speedup_fp32g_over_fp64": dt64 / dt32, "pair_set_identical": bool(same)},
+        "pair_stats_fp32g": st32,
         "kernels_fp32g": k32,
         "kernels_fp64": k64,
-        "roofline": {"bound": "hbm", "kernel": "vote_emit (FP32-guarded)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "fp64_emit_achieved": achieved64},
+        "roofline": roof,
+        "rooflines": rooflines,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)

@@ -138,6 +138,8 @@ const char *mvs_last_error(void);
  * Pass a hipStream_t; 0 selects the session's own stream again. */
 int mvs_session_set_stream(mvs_session *s, void *hip_stream);
 int mvs_session_sync(mvs_session *s);
+/* Stream-ordered device-to-device copy (to read session-owned outputs). */
+int mvs_dev_copy(mvs_session *s, void *dst, const void *src, uint64_t bytes);
 /* Device time (ms) of the last mvs_dev_* launch group, measured with HIP
  * events on the session stream; see mvs_dev_timing for per-kernel times. */
 int mvs_session_enable_timing(mvs_session *s, int enable);
@@ -237,6 +239,46 @@ int mvs_dev_vote_emit(mvs_session *s, uint64_t pair_count, const int32_t *a,
                       const int32_t *b, double box, const double center[3], int precision,
                       void *rigid_q4, void *rigid_t3, int32_t *rigid_pair, void *refl_n3,
                       void *refl_d, int32_t *refl_pair);
+
+/* Fused pairing + vote emission on device-resident features: count, budget
+ * check (one synchronisation for the pair total), placement, mirror
+ * offsets and compact emission in one call, outputs in session-owned device
+ * buffers valid until the next mvs_dev_pairvote on this session.  The
+ * mirror stream has capacity pair_count (its SoA stride); the mirror total
+ * is read back at the end.  Budgets: pairs > max_pairs or votes > max_votes
+ * raise MVS_EBUDGET. */
+typedef struct {
+  uint64_t pairs, reflections;
+  const int32_t *a, *b;             /* pair ordinals (selected positions) */
+  const void *rigid_q4, *rigid_t3;  /* 4 / 3 lanes, stride 2*pairs */
+  const int32_t *rigid_pair;
+  const void *refl_n3, *refl_d;     /* 3 / 1 lanes, stride refl_stride */
+  const int32_t *refl_pair;
+  uint64_t refl_stride;
+  int precision;
+} mvs_pairvote_result;
+int mvs_dev_pairvote(mvs_session *s, uint64_t n, const double *x, const double *y,
+                     const double *z, const double *feat, uint64_t selected_count,
+                     const int32_t *selected, double box, const double center[3],
+                     double threshold, uint64_t max_pairs, uint64_t max_votes, int precision,
+                     mvs_pairvote_result *out);
+
+/* Batched mean-shift over the votes of the last mvs_dev_pairvote:
+ * `seeds` seed votes drawn without replacement (std::mt19937_64(seed_value),
+ * split between rigid and mirror votes in proportion to their counts), each
+ * iterated `iterations` times in its own point set (see mvs_dev_ms_batched).
+ * Outputs in session-owned buffers; want_stats != 0 also returns the visit
+ * counts (one extra synchronisation). */
+typedef struct {
+  uint64_t rigid_seeds, refl_seeds;
+  const int32_t *seed_votes;          /* rigid seeds then mirror seeds (stream indices) */
+  const void *rigid_modes, *rigid_weight;  /* 6 x rigid_seeds SoA, rigid_seeds */
+  const void *refl_modes, *refl_weight;    /* 4 x refl_seeds SoA, refl_seeds */
+  uint64_t candidates, in_support;
+  int precision;
+} mvs_meanshift_result;
+int mvs_dev_meanshift_votes(mvs_session *s, uint64_t seeds, uint64_t seed_value, int iterations,
+                            double h, double cutoff, int want_stats, mvs_meanshift_result *out);
 
 /* Batched-seed mean-shift (no reference counterpart; the reference runs
  * sequential seeds only, src/cpu/clustering.cpp:44-60).

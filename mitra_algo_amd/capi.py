@@ -24,6 +24,7 @@ EXPORTS = [
     "mvs_transformations_fetch", "mvs_clustering", "mvs_clustering_fetch",
     "mvs_dev_curvature", "mvs_dev_pair_count", "mvs_dev_pair_fill", "mvs_dev_pair_stats",
     "mvs_dev_vote_count", "mvs_dev_vote_emit", "mvs_dev_ms_points", "mvs_dev_ms_batched",
+    "mvs_dev_pairvote", "mvs_dev_meanshift_votes", "mvs_dev_copy",
 ]
 
 
@@ -104,6 +105,21 @@ class Prepared(C.Structure):
         ("rejected_faces", C.c_int32), ("nonmanifold_edges", C.c_int32),
         ("orientation_conflicts", C.c_int32),
     ]
+
+
+class PairVoteResult(C.Structure):
+    _fields_ = [("pairs", C.c_uint64), ("reflections", C.c_uint64), ("a", C.c_void_p),
+                ("b", C.c_void_p), ("rigid_q4", C.c_void_p), ("rigid_t3", C.c_void_p),
+                ("rigid_pair", C.c_void_p), ("refl_n3", C.c_void_p), ("refl_d", C.c_void_p),
+                ("refl_pair", C.c_void_p), ("refl_stride", C.c_uint64), ("precision", C.c_int)]
+
+
+class MeanShiftResult(C.Structure):
+    _fields_ = [("rigid_seeds", C.c_uint64), ("refl_seeds", C.c_uint64),
+                ("seed_votes", C.c_void_p), ("rigid_modes", C.c_void_p),
+                ("rigid_weight", C.c_void_p), ("refl_modes", C.c_void_p),
+                ("refl_weight", C.c_void_p), ("candidates", C.c_uint64),
+                ("in_support", C.c_uint64), ("precision", C.c_int)]
 
 
 _lib = None
@@ -371,6 +387,44 @@ class Session:
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(lib().mvs_dev_pair_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return dict(window_candidates=a.value, index_candidates=b.value, fp64_rechecks=c.value)
+
+    def dev_pairvote(self, n, x, y, z, feat, S, sel, box, center, t, max_pairs, max_votes,
+                     precision) -> PairVoteResult:
+        r = PairVoteResult()
+        ctr = (C.c_double * 3)(*[float(c) for c in center])
+        check(lib().mvs_dev_pairvote(self.h, C.c_uint64(n), ptr(x), ptr(y), ptr(z), ptr(feat),
+                                     C.c_uint64(S), ptr(sel), C.c_double(box), ctr, C.c_double(t),
+                                     C.c_uint64(max_pairs), C.c_uint64(max_votes),
+                                     C.c_int(precision), C.byref(r)))
+        return r
+
+    def dev_copy(self, dst, src_addr: int, nbytes: int):
+        """Stream-ordered device-to-device copy out of session-owned memory."""
+        check(lib().mvs_dev_copy(self.h, ptr(dst), C.c_void_p(src_addr), C.c_uint64(nbytes)))
+
+    def fetch(self, addr: int, shape, dtype, stride_elems: int | None = None):
+        """Copy a session-owned SoA block into a new torch tensor of `shape`
+        (rows of shape[-1] elements; source rows `stride_elems` apart)."""
+        import torch
+        out = torch.empty(shape, dtype=dtype, device="cuda:0")
+        if out.numel() == 0 or not addr:
+            return out
+        es = out.element_size()
+        if stride_elems is None or len(shape) == 1:
+            self.dev_copy(out, addr, out.numel() * es)
+        else:
+            for r in range(shape[0]):
+                self.dev_copy(out[r], addr + r * stride_elems * es, shape[-1] * es)
+        return out
+
+    def dev_meanshift_votes(self, seeds, seed_value, iterations, h, cutoff,
+                            want_stats=False) -> MeanShiftResult:
+        r = MeanShiftResult()
+        check(lib().mvs_dev_meanshift_votes(self.h, C.c_uint64(seeds), C.c_uint64(seed_value),
+                                            C.c_int(iterations), C.c_double(h),
+                                            C.c_double(cutoff), C.c_int(1 if want_stats else 0),
+                                            C.byref(r)))
+        return r
 
     def dev_ms_points(self, precision, nrigid, q4, t3, nrefl, n3, d1, rigid_pts, refl_pts):
         check(lib().mvs_dev_ms_points(self.h, C.c_int(precision), C.c_uint64(nrigid), ptr(q4),

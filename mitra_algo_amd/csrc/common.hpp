@@ -106,6 +106,10 @@ enum Slot : int {
   kSlotPlanWin, kSlotPlanCounts, kSlotPlanOffsets, kSlotPlanLong, kSlotPlanStats,
   kSlotVoteBlock, kSlotVoteBlockOff,
   kSlotMs0, kSlotMs1, kSlotMs2, kSlotMs3, kSlotMs4, kSlotMs5, kSlotMs6, kSlotMs7,
+  // fused step outputs (valid until the next fused call)
+  kSlotOutPairs, kSlotOutRigidQ, kSlotOutRigidT, kSlotOutRigidPair, kSlotOutReflN,
+  kSlotOutReflD, kSlotOutReflPair, kSlotOutMsRigidPts, kSlotOutMsReflPts, kSlotOutMsSeeds,
+  kSlotOutMsRigid, kSlotOutMsRigidW, kSlotOutMsRefl, kSlotOutMsReflW,
   kSlotCount
 };
 

@@ -1,0 +1,204 @@
+// Fused device entry points of the voting step: pairing + vote emission in
+// one call, then batched mean-shift over the emitted votes.  Outputs live in
+// session-owned buffers, so a caller's step is two calls with two short
+// host synchronisations (the pair total for the budget check, the mirror
+// total at the end).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <unordered_set>
+#include <vector>
+
+#include "pairing.hpp"
+
+namespace mvs {
+
+static void pairvote(Session *s, uint64_t n, const double *x, const double *y, const double *z,
+                     const double *feat, uint64_t S, const int32_t *sel, double box,
+                     const double *center, double t, uint64_t max_pairs, uint64_t max_votes,
+                     int precision, mvs_pairvote_result *out) {
+  s->pairvote_valid = false;
+  uint64_t P = 0;
+  pair_count_device(s, n, x, y, z, feat, S, sel, box, t, max_pairs, precision, center, &P);
+  if (2 * P > max_votes)
+    raise(MVS_EBUDGET, "Vote budget exceeded; lower --sample-ratio or tighten --pair-threshold");
+  const size_t sz = precision == MVS_FP32_GUARDED ? 4 : 8;
+  const uint64_t Pp = P ? P : 1;
+  int32_t *ab = scratch_as<int32_t>(s, kSlotOutPairs, 2 * Pp);
+  void *q4 = scratch(s, kSlotOutRigidQ, 4 * 2 * Pp * sz);
+  void *t3 = scratch(s, kSlotOutRigidT, 3 * 2 * Pp * sz);
+  int32_t *rp = scratch_as<int32_t>(s, kSlotOutRigidPair, 2 * Pp);
+  void *n3 = scratch(s, kSlotOutReflN, 3 * Pp * sz);
+  void *d1 = scratch(s, kSlotOutReflD, Pp * sz);
+  int32_t *fp = scratch_as<int32_t>(s, kSlotOutReflPair, Pp);
+  pair_fill_device(s, ab, ab + Pp, true);
+  uint64_t *rtot = reinterpret_cast<uint64_t *>(plan_flags(s)) + kVoteTotalWord;
+  vote_offsets_device(s, P, ab, ab + Pp, precision, rtot);
+  vote_emit_strided(s, P, ab, ab + Pp, center, precision, Pp, q4, t3, rp, n3, d1, fp);
+  uint64_t R = 0;
+  read_back(s, &R, rtot, sizeof R);
+  if (2 * P + R > max_votes || 2 * P + R > static_cast<uint64_t>(INT32_MAX))
+    raise(MVS_EBUDGET, "Vote budget exceeded; lower --sample-ratio or tighten --pair-threshold");
+  mvs_pairvote_result r{};
+  r.pairs = P;
+  r.reflections = R;
+  r.a = ab;
+  r.b = ab + Pp;
+  r.rigid_q4 = q4;
+  r.rigid_t3 = t3;
+  r.rigid_pair = rp;
+  r.refl_n3 = n3;
+  r.refl_d = d1;
+  r.refl_pair = fp;
+  r.refl_stride = Pp;
+  r.precision = precision;
+  s->pairvote = r;
+  s->pairvote_valid = true;
+  s->vote_plan.P = P;
+  s->vote_plan.R = R;
+  s->vote_plan.precision = precision;
+  s->vote_plan.valid = true;
+  *out = r;
+}
+
+// k distinct indices of [0, n), ascending (Floyd's algorithm on mt19937_64).
+static std::vector<int32_t> sample_distinct(uint64_t n, uint64_t k, mvs_mt64 *g) {
+  std::vector<int32_t> out;
+  if (k >= n) {
+    out.resize(n);
+    for (uint64_t i = 0; i < n; ++i)
+      out[i] = static_cast<int32_t>(i);
+    return out;
+  }
+  std::unordered_set<uint64_t> chosen;
+  chosen.reserve(2 * k);
+  for (uint64_t j = n - k; j < n; ++j) {
+    uint64_t t = mt64_next(g) % (j + 1);
+    if (!chosen.insert(t).second)
+      chosen.insert(j);
+  }
+  out.assign(chosen.begin(), chosen.end());
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, int iterations,
+                            double h, double cutoff, bool want_stats, mvs_meanshift_result *out) {
+  if (!s->pairvote_valid)
+    raise(MVS_EINVAL, "mean-shift over votes needs a preceding mvs_dev_pairvote");
+  const mvs_pairvote_result &r = s->pairvote;
+  const uint64_t nr = 2 * r.pairs, nf = r.reflections, V = nr + nf;
+  const int prec = r.precision;
+  const size_t sz = prec == MVS_FP32_GUARDED ? 4 : 8;
+  uint64_t kf = V ? static_cast<uint64_t>(std::llround(static_cast<double>(nseeds) * nf / V)) : 0;
+  kf = std::min(kf, nf);
+  uint64_t kr = std::min(nseeds > kf ? nseeds - kf : 0, nr);
+  mvs_mt64 g;
+  g.x[0] = seed_value;
+  for (uint64_t i = 1; i < 312; ++i)
+    g.x[i] = 6364136223846793005ULL * (g.x[i - 1] ^ (g.x[i - 1] >> 62)) + i;
+  g.p = 312;
+  std::vector<int32_t> seeds = sample_distinct(nr, kr, &g);
+  std::vector<int32_t> fseeds = sample_distinct(nf, kf, &g);
+  seeds.insert(seeds.end(), fseeds.begin(), fseeds.end());
+  int32_t *dseeds = scratch_as<int32_t>(s, kSlotOutMsSeeds, seeds.size() ? seeds.size() : 1);
+  if (!seeds.empty()) {  // through pinned staging: the copy stays stream-ordered
+    void *hs = host_stage(s, seeds.size() * sizeof(int32_t));
+    std::memcpy(hs, seeds.data(), seeds.size() * sizeof(int32_t));
+    MVS_HIP(hipMemcpyAsync(dseeds, hs, seeds.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                           s->stream));
+  }
+  // Rigid points are built from the rigid stream (stride 2P); mirror points
+  // need the stream compacted to its true length R first when the stride
+  // (capacity) exceeds it, which the point builder does by reading only R.
+  void *rpts = scratch(s, kSlotOutMsRigidPts, 6 * (nr ? nr : 1) * sz);
+  void *fpts = scratch(s, kSlotOutMsReflPts, 4 * (nf ? nf : 1) * sz);
+  ms_points_device(s, prec, nr, r.rigid_q4, r.rigid_t3, 0, nullptr, nullptr, rpts, nullptr);
+  if (nf) {
+    // mirror stream has SoA stride refl_stride; build points lane by lane
+    if (prec == MVS_FP32_GUARDED) {
+      const float *n3 = static_cast<const float *>(r.refl_n3);
+      float *tmp = scratch_as<float>(s, kSlotMs6, 3 * nf);
+      for (int k = 0; k < 3; ++k)
+        MVS_HIP(hipMemcpyAsync(tmp + k * nf, n3 + k * r.refl_stride, nf * 4,
+                               hipMemcpyDeviceToDevice, s->stream));
+      ms_points_device(s, prec, 0, nullptr, nullptr, nf, tmp, r.refl_d, nullptr, fpts);
+    } else {
+      const double *n3 = static_cast<const double *>(r.refl_n3);
+      double *tmp = scratch_as<double>(s, kSlotMs6, 3 * nf);
+      for (int k = 0; k < 3; ++k)
+        MVS_HIP(hipMemcpyAsync(tmp + k * nf, n3 + k * r.refl_stride, nf * 8,
+                               hipMemcpyDeviceToDevice, s->stream));
+      ms_points_device(s, prec, 0, nullptr, nullptr, nf, tmp, r.refl_d, nullptr, fpts);
+    }
+  }
+  void *orig = scratch(s, kSlotOutMsRigid, 6 * (kr ? kr : 1) * sz);
+  void *owr = scratch(s, kSlotOutMsRigidW, (kr ? kr : 1) * sz);
+  void *ofl = scratch(s, kSlotOutMsRefl, 4 * (kf ? kf : 1) * sz);
+  void *owf = scratch(s, kSlotOutMsReflW, (kf ? kf : 1) * sz);
+  unsigned long long s1[2] = {0, 0}, s2[2] = {0, 0};
+  if (prec == MVS_FP32_GUARDED) {
+    if (kr)
+      ms_batched_f32(s, 6, nr, static_cast<const float *>(rpts), kr, nullptr, dseeds, iterations,
+                     h, cutoff, static_cast<float *>(orig), static_cast<float *>(owr),
+                     want_stats ? s1 : nullptr);
+    if (kf)
+      ms_batched_f32(s, 4, nf, static_cast<const float *>(fpts), kf, nullptr, dseeds + kr,
+                     iterations, h, cutoff, static_cast<float *>(ofl), static_cast<float *>(owf),
+                     want_stats ? s2 : nullptr);
+  } else {
+    if (kr)
+      ms_batched_f64(s, 6, nr, static_cast<const double *>(rpts), kr, nullptr, dseeds, iterations,
+                     h, cutoff, static_cast<double *>(orig), static_cast<double *>(owr),
+                     want_stats ? s1 : nullptr);
+    if (kf)
+      ms_batched_f64(s, 4, nf, static_cast<const double *>(fpts), kf, nullptr, dseeds + kr,
+                     iterations, h, cutoff, static_cast<double *>(ofl),
+                     static_cast<double *>(owf), want_stats ? s2 : nullptr);
+  }
+  mvs_meanshift_result m{};
+  m.rigid_seeds = kr;
+  m.refl_seeds = kf;
+  m.seed_votes = dseeds;
+  m.rigid_modes = orig;
+  m.rigid_weight = owr;
+  m.refl_modes = ofl;
+  m.refl_weight = owf;
+  m.candidates = s1[0] + s2[0];
+  m.in_support = s1[1] + s2[1];
+  m.precision = prec;
+  *out = m;
+}
+
+}  // namespace mvs
+
+using namespace mvs;
+
+extern "C" {
+
+int mvs_dev_pairvote(mvs_session *sp, uint64_t n, const double *x, const double *y,
+                     const double *z, const double *feat, uint64_t S, const int32_t *sel,
+                     double box, const double center[3], double t, uint64_t max_pairs,
+                     uint64_t max_votes, int precision, mvs_pairvote_result *out) {
+  return guarded([&] {
+    if (!out || !center)
+      raise(MVS_EINVAL, "null argument");
+    pairvote(as_session(sp), n, x, y, z, feat, S, sel, box, center, t, max_pairs, max_votes,
+             precision, out);
+  });
+}
+
+int mvs_dev_meanshift_votes(mvs_session *sp, uint64_t seeds, uint64_t seed_value, int iterations,
+                            double h, double cutoff, int want_stats, mvs_meanshift_result *out) {
+  return guarded([&] {
+    if (!out)
+      raise(MVS_EINVAL, "null argument");
+    meanshift_votes(as_session(sp), seeds, seed_value, iterations, h, cutoff, want_stats != 0,
+                    out);
+  });
+}
+
+}  // extern "C"

@@ -128,18 +128,26 @@ __global__ void k_gather_sorted(const R *pts, uint64_t n, int D, const uint32_t 
 }
 
 // One wavefront per seed.
+// Seeds are either positions (seeds, D x ns SoA) or indices into the
+// unsorted point set (seed_idx, with raw = those points).
 template <class R, int D>
 __global__ void __launch_bounds__(kThreads)
     k_ms_query(const R *pts, uint64_t n, const uint64_t *offsets, Grid g, const R *seeds,
-               uint64_t ns, int iterations, R inv2h2, R r2, R floor_w, R *out, R *weight,
-               unsigned long long *stats) {
+               const int32_t *seed_idx, const R *raw, uint64_t ns, int iterations, R inv2h2,
+               R r2, R floor_w, R *out, R *weight, unsigned long long *stats) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t s = static_cast<uint64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
   if (s >= ns)
     return;
   R m[D];
-  for (int k = 0; k < D; ++k)
-    m[k] = seeds[k * ns + s];
+  if (seed_idx) {
+    const uint64_t v = static_cast<uint64_t>(seed_idx[s]);
+    for (int k = 0; k < D; ++k)
+      m[k] = raw[k * n + v];
+  } else {
+    for (int k = 0; k < D; ++k)
+      m[k] = seeds[k * ns + s];
+  }
   R wsum = 0;
   unsigned long long cand = 0, inside = 0;
   for (int it = 0; it < iterations; ++it) {
@@ -189,15 +197,30 @@ __global__ void __launch_bounds__(kThreads)
         m[k] += acc[k] * inv;
     }
   }
+  inside = wave_sum(inside);
   if (lane == 0) {
     for (int k = 0; k < D; ++k)
       out[k * ns + s] = m[k];
     weight[s] = wsum;
-    atomicAdd(&stats[0], cand);
+    stats[s] = cand;  // per-seed slots: no contended atomics
+    stats[ns + s] = inside;
   }
-  inside = wave_sum(inside);
-  if (lane == 0)
-    atomicAdd(&stats[1], inside);
+}
+
+__global__ void k_sum_u64(const unsigned long long *a, uint64_t n, unsigned long long *out) {
+  __shared__ unsigned long long sh[kThreads];
+  unsigned long long x = 0;
+  for (uint64_t i = threadIdx.x; i < n; i += kThreads)
+    x += a[i];
+  sh[threadIdx.x] = x;
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    *out = sh[0];
 }
 
 }  // namespace
@@ -206,16 +229,17 @@ __global__ void __launch_bounds__(kThreads)
 // through stats_out when given.
 template <class R>
 static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns, const R *seeds,
-                       int iterations, double h, double cutoff, R *out, R *weight,
-                       unsigned long long *stats_out) {
+                       const int32_t *seed_idx, int iterations, double h, double cutoff, R *out,
+                       R *weight, unsigned long long *stats_out) {
   if (D != 4 && D != 6)
     raise(MVS_EINVAL, "batched mean-shift supports 4-D (reflection) and 6-D (rigid) points");
   if (!(h > 0) || !(cutoff > 0) || iterations < 0)
     raise(MVS_EINVAL, "invalid mean-shift parameters");
   if (n > static_cast<uint64_t>(UINT32_MAX))
     raise(MVS_EOVERFLOW, "too many points");
-  unsigned long long *stats = scratch_as<unsigned long long>(s, kSlotMs7, 4);
-  zero_device(s, stats, 4 * sizeof(unsigned long long));
+  // per-seed statistics [0, ns) candidates, [ns, 2ns) in-support visits,
+  // then two sums
+  unsigned long long *stats = scratch_as<unsigned long long>(s, kSlotMs7, 2 * ns + 2);
   if (ns == 0)
     return;
   Grid g{};
@@ -280,18 +304,56 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   const R fw = static_cast<R>(std::exp(-0.5 * cutoff * cutoff));
   unsigned grid = blocks_for(ns, kThreads / kWave);
   if (D == 6)
-    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 6>), grid, kThreads, 0, sorted, n, off, g, seeds, ns,
-               iterations, inv2h2, r2, fw, out, weight, stats);
+    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 6>), grid, kThreads, 0, sorted, n, off, g, seeds,
+               seed_idx, pts, ns, iterations, inv2h2, r2, fw, out, weight, stats);
   else
-    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 4>), grid, kThreads, 0, sorted, n, off, g, seeds, ns,
-               iterations, inv2h2, r2, fw, out, weight, stats);
+    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 4>), grid, kThreads, 0, sorted, n, off, g, seeds,
+               seed_idx, pts, ns, iterations, inv2h2, r2, fw, out, weight, stats);
   if (stats_out) {
+    MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats, ns, stats + 2 * ns);
+    MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats + ns, ns, stats + 2 * ns + 1);
     void *hp = host_stage(s, 2 * sizeof(unsigned long long));
-    MVS_HIP(hipMemcpyAsync(hp, stats, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                           s->stream));
+    MVS_HIP(hipMemcpyAsync(hp, stats + 2 * ns, 2 * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s->stream));
     MVS_HIP(hipStreamSynchronize(s->stream));
     std::memcpy(stats_out, hp, 2 * sizeof(unsigned long long));
   }
+}
+
+void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,
+                    const float *seeds, const int32_t *seed_idx, int iterations, double h,
+                    double cutoff, float *out, float *weight, unsigned long long *stats) {
+  ms_batched<float>(s, D, n, pts, ns, seeds, seed_idx, iterations, h, cutoff, out, weight, stats);
+}
+void ms_batched_f64(Session *s, int D, uint64_t n, const double *pts, uint64_t ns,
+                    const double *seeds, const int32_t *seed_idx, int iterations, double h,
+                    double cutoff, double *out, double *weight, unsigned long long *stats) {
+  ms_batched<double>(s, D, n, pts, ns, seeds, seed_idx, iterations, h, cutoff, out, weight,
+                     stats);
+}
+void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4, const void *t3,
+                      uint64_t nrefl, const void *n3, const void *d1, void *rigid_pts,
+                      void *refl_pts) {
+  if (precision == MVS_FP32_GUARDED) {
+    if (nrigid)
+      MVS_LAUNCH(s, "ms_points", k_points_rigid<float>, blocks_for(nrigid, kThreads), kThreads, 0,
+                 static_cast<const float *>(q4), static_cast<const float *>(t3), nrigid,
+                 static_cast<float *>(rigid_pts));
+    if (nrefl)
+      MVS_LAUNCH(s, "ms_points", k_points_refl<float>, blocks_for(nrefl, kThreads), kThreads, 0,
+                 static_cast<const float *>(n3), static_cast<const float *>(d1), nrefl,
+                 static_cast<float *>(refl_pts));
+  } else if (precision == MVS_FP64) {
+    if (nrigid)
+      MVS_LAUNCH(s, "ms_points", k_points_rigid<double>, blocks_for(nrigid, kThreads), kThreads,
+                 0, static_cast<const double *>(q4), static_cast<const double *>(t3), nrigid,
+                 static_cast<double *>(rigid_pts));
+    if (nrefl)
+      MVS_LAUNCH(s, "ms_points", k_points_refl<double>, blocks_for(nrefl, kThreads), kThreads, 0,
+                 static_cast<const double *>(n3), static_cast<const double *>(d1), nrefl,
+                 static_cast<double *>(refl_pts));
+  } else
+    raise(MVS_EINVAL, "unknown precision");
 }
 
 }  // namespace mvs
@@ -304,27 +366,8 @@ int mvs_dev_ms_points(mvs_session *sp, int precision, uint64_t nrigid, const voi
                       const void *t3, uint64_t nrefl, const void *n3, const void *d1,
                       void *rigid_pts, void *refl_pts) {
   return guarded([&] {
-    Session *s = as_session(sp);
-    if (precision == MVS_FP32_GUARDED) {
-      if (nrigid)
-        MVS_LAUNCH(s, "ms_points", k_points_rigid<float>, blocks_for(nrigid, kThreads), kThreads,
-                   0, static_cast<const float *>(q4), static_cast<const float *>(t3), nrigid,
-                   static_cast<float *>(rigid_pts));
-      if (nrefl)
-        MVS_LAUNCH(s, "ms_points", k_points_refl<float>, blocks_for(nrefl, kThreads), kThreads, 0,
-                   static_cast<const float *>(n3), static_cast<const float *>(d1), nrefl,
-                   static_cast<float *>(refl_pts));
-    } else if (precision == MVS_FP64) {
-      if (nrigid)
-        MVS_LAUNCH(s, "ms_points", k_points_rigid<double>, blocks_for(nrigid, kThreads), kThreads,
-                   0, static_cast<const double *>(q4), static_cast<const double *>(t3), nrigid,
-                   static_cast<double *>(rigid_pts));
-      if (nrefl)
-        MVS_LAUNCH(s, "ms_points", k_points_refl<double>, blocks_for(nrefl, kThreads), kThreads,
-                   0, static_cast<const double *>(n3), static_cast<const double *>(d1), nrefl,
-                   static_cast<double *>(refl_pts));
-    } else
-      raise(MVS_EINVAL, "unknown precision");
+    ms_points_device(as_session(sp), precision, nrigid, q4, t3, nrefl, n3, d1, rigid_pts,
+                     refl_pts);
   });
 }
 
@@ -337,11 +380,11 @@ int mvs_dev_ms_batched(mvs_session *sp, int precision, int dim, uint64_t n, cons
     unsigned long long *want = (candidates || in_support) ? st : nullptr;
     if (precision == MVS_FP32_GUARDED)
       ms_batched<float>(s, dim, n, static_cast<const float *>(pts), ns,
-                        static_cast<const float *>(seeds), iterations, h, cutoff,
+                        static_cast<const float *>(seeds), nullptr, iterations, h, cutoff,
                         static_cast<float *>(out), static_cast<float *>(weight), want);
     else if (precision == MVS_FP64)
       ms_batched<double>(s, dim, n, static_cast<const double *>(pts), ns,
-                         static_cast<const double *>(seeds), iterations, h, cutoff,
+                         static_cast<const double *>(seeds), nullptr, iterations, h, cutoff,
                          static_cast<double *>(out), static_cast<double *>(weight), want);
     else
       raise(MVS_EINVAL, "unknown precision");

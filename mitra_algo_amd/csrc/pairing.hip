@@ -3,26 +3,27 @@
 // Restates src/cpu/pairing.cpp:14-35: every selected-ordinal pair a < b with
 // |dkmin| <= t*M and |dkmax| <= t*M (M = max of the four |k| and 1e-10) and
 // |p_a - p_b| >= 0.03*box, emitted in (a asc, b asc) order.
-//   1. gather : selected ids -> ordinal-order sample SoA (14 lanes) and a
-//               32-bit order-preserving key of kmin (-0 canonicalised);
+//   1. gather : selected ids -> ordinal-order sample records (AoS: one
+//               64-byte line per FP32 sample, two per FP64 sample, so a
+//               random access touches one or two lines) and a 32-bit
+//               order-preserving key of kmin (-0 canonicalised);
 //   2. sort   : LDS-staged stable LSD radix sort of (key, ordinal), 4 passes;
-//   3. index  : the kmin-sorted samples are cut into buckets of 64 and each
-//               bucket is sorted by kmax inside one wavefront (bitonic
-//               network over the 64 lanes), giving a two-level
-//               (kmin bucket, kmax) index;
-//   4. windows: per source a conservative interval for kmin and for kmax
-//               (supersets of everything the predicate can accept);
-//   5. count  : one wavefront per source: each lane binary-searches the kmax
-//               interval inside one bucket of the kmin interval, the wave
-//               concatenates the sub-ranges and tests their candidates with
-//               coalesced loads; accepted b > a are counted;
+//   3. index  : the kmin-sorted samples are cut into buckets of 64, each
+//               sorted by kmax inside one wavefront (64-lane bitonic network):
+//               a two-level (kmin bucket, kmax) index;
+//   4. windows: per source conservative intervals for kmin and kmax;
+//   5. walk   : one wavefront per source; each lane binary-searches the kmax
+//               interval in one bucket of the kmin interval (both bounds in
+//               one loop), the wave concatenates the sub-ranges and tests
+//               their candidates with coalesced loads.  Accepted ordinals
+//               (b > a) are counted; up to 64 of them are put in CPU order
+//               by a 64-lane bitonic sort and parked in the source's slab;
 //   6. scan   : exclusive prefix of the counts -> output offsets; the total is
 //               read back once and checked against the budget before any
 //               output is written;
-//   7. fill   : the same walk appends accepted ordinals to an LDS list whose
-//               rank order (number of smaller ordinals) restores the
-//               reference's b order; sources with more than kCap pairs walk
-//               the targets in ordinal order instead (k_pair_long).
+//   7. place  : each slab is copied to its offset (coalesced); sources with
+//               more than 64 pairs walk the targets in ordinal order instead
+//               (k_pair_long), so order never depends on a capacity.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -36,14 +37,11 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;  // sources per block
-constexpr int kCap = 512;                 // LDS order-restoration capacity per source
+constexpr int kSlab = 64;                 // pairs parked per source by the walk
 constexpr int kBucket = 64;               // samples per kmin bucket
 constexpr double kEps = 2.220446049250313e-16;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-// 32-bit order-preserving key: the top half of the IEEE order transform.
-// Monotone non-decreasing in v, so windows searched in key space are
-// supersets of the windows in value space.
 __device__ inline uint32_t order_key32(double v) {
   if (v == 0.0)
     v = 0.0;  // -0 and +0 compare equal: one key
@@ -52,16 +50,16 @@ __device__ inline uint32_t order_key32(double v) {
   return static_cast<uint32_t>(u >> 32);
 }
 
-// Flags: 1 = selected id out of range, 2 = non-finite coordinate/curvature.
-__device__ inline float wave_sum_max(float v) {
+__device__ inline float wave_max(float v) {
   for (int o = 32; o > 0; o >>= 1)
     v = fmaxf(v, __shfl_xor(v, o, kWave));
   return v;
 }
 
+// Flags: 1 = selected id out of range, 2 = non-finite coordinate/curvature.
 __global__ void k_gather_samples(const double *x, const double *y, const double *z,
                                  const double *feat, uint64_t n, const int32_t *sel, uint64_t S,
-                                 double cx, double cy, double cz, double *s64, float *s32,
+                                 double cx, double cy, double cz, double *r64, float *r32,
                                  uint64_t *keys, uint32_t *vals, unsigned *flags,
                                  unsigned *rmax_bits) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -69,11 +67,11 @@ __global__ void k_gather_samples(const double *x, const double *y, const double 
   float r = 0.0f;
   if (live) {
     int32_t id = sel[i];
-    double v[L_N];
+    double v[kRec];
+    for (int k = 0; k < kRec; ++k)
+      v[k] = 0;
     if (id < 0 || static_cast<uint64_t>(id) >= n) {
       atomicOr(flags, 1u);
-      for (int k = 0; k < L_N; ++k)
-        v[k] = 0;
     } else {
       v[L_X] = x[id];
       v[L_Y] = y[id];
@@ -86,29 +84,32 @@ __global__ void k_gather_samples(const double *x, const double *y, const double 
                  isfinite(v[L_K0]) && isfinite(v[L_K1]);
       if (!fin) {
         atomicOr(flags, 2u);
-        for (int k = 0; k < L_N; ++k)
+        for (int k = 0; k < kRec; ++k)
           v[k] = 0;  // keep every later comparison well defined
       }
     }
-    for (int k = 0; k < L_N; ++k)
-      s64[k * S + i] = v[k];
-    if (s32) {
-      float px = static_cast<float>(v[L_X] - cx), py = static_cast<float>(v[L_Y] - cy),
-            pz = static_cast<float>(v[L_Z] - cz);
-      s32[L_X * S + i] = px;
-      s32[L_Y * S + i] = py;
-      s32[L_Z * S + i] = pz;
-      for (int k = L_K0; k < L_N; ++k)
-        s32[k * S + i] = static_cast<float>(v[k]);
-      r = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
+    double2 *d2 = reinterpret_cast<double2 *>(r64 + i * kRec);
+    for (int k = 0; k < kRec / 2; ++k)
+      d2[k] = make_double2(v[2 * k], v[2 * k + 1]);
+    if (r32) {
+      float f[kRec];
+      f[L_X] = static_cast<float>(v[L_X] - cx);
+      f[L_Y] = static_cast<float>(v[L_Y] - cy);
+      f[L_Z] = static_cast<float>(v[L_Z] - cz);
+      for (int k = L_K0; k < kRec; ++k)
+        f[k] = static_cast<float>(v[k]);
+      float4 *f4 = reinterpret_cast<float4 *>(r32 + i * kRec);
+      for (int k = 0; k < kRec / 4; ++k)
+        f4[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
+      r = fmaxf(fabsf(f[L_X]), fmaxf(fabsf(f[L_Y]), fabsf(f[L_Z])));
     }
     keys[i] = order_key32(v[L_K0]);
     vals[i] = static_cast<uint32_t>(i);
   }
-  // largest centre-relative coordinate (sets the FP32 guard bands); all
-  // lanes take part in the reduction
-  if (s32) {
-    r = wave_sum_max(r);
+  // largest centre-relative coordinate (sets the FP32 guard bands); every
+  // lane takes part in the reduction
+  if (r32) {
+    r = wave_max(r);
     if ((threadIdx.x & (kWave - 1)) == 0)
       atomicMax(rmax_bits, __float_as_uint(r));  // non-negative floats order as uints
   }
@@ -117,8 +118,8 @@ __global__ void k_gather_samples(const double *x, const double *y, const double 
 // One wavefront per bucket of 64 kmin-sorted samples: sort by (kmax, ordinal)
 // with a 64-lane bitonic network and write the bucketed index.
 __global__ void __launch_bounds__(kThreads)
-    k_bucket_index(const uint32_t *ord, const double *s64, const float *s32, uint64_t S,
-                   uint32_t *bord, double *bk0, double *bk1, float *bk0f, float *bk1f) {
+    k_bucket_index(const uint32_t *ord, const double *r64, const float *r32, uint64_t S,
+                   uint32_t *bord, double2 *bk, float2 *bkf) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
   const uint64_t nb = (S + kBucket - 1) / kBucket;
@@ -129,7 +130,7 @@ __global__ void __launch_bounds__(kThreads)
   double key = INFINITY;
   if (j < S) {
     o = ord[j];
-    key = s64[L_K1 * S + o];
+    key = r64[static_cast<uint64_t>(o) * kRec + L_K1];
   }
   for (int k = 2; k <= kWave; k <<= 1)
     for (int m = k >> 1; m > 0; m >>= 1) {
@@ -146,12 +147,9 @@ __global__ void __launch_bounds__(kThreads)
     }
   if (j < S) {
     bord[j] = o;
-    bk1[j] = key;
-    bk0[j] = s64[L_K0 * S + o];
-    if (bk0f) {
-      bk0f[j] = s32[L_K0 * S + o];
-      bk1f[j] = s32[L_K1 * S + o];
-    }
+    bk[j] = make_double2(r64[static_cast<uint64_t>(o) * kRec + L_K0], key);
+    if (bkf)
+      bkf[j] = make_float2(r32[static_cast<uint64_t>(o) * kRec + L_K0], r32[static_cast<uint64_t>(o) * kRec + L_K1]);
   }
 }
 
@@ -163,8 +161,8 @@ __global__ void __launch_bounds__(kThreads)
 // endpoints by 16 eps of their magnitude; t >= .5 or t below the normal
 // range use the full range.
 // win: [0,S) first sorted position, [S,2S) end position; kw: kmax interval.
-__global__ void k_windows(const double *s64, const uint64_t *skeys, uint64_t S, double t,
-                          uint64_t *win, double *kw) {
+__global__ void k_windows(const double *r64, const uint64_t *skeys, uint64_t S, double t,
+                          uint64_t *win, double2 *kw) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= S)
     return;
@@ -172,7 +170,7 @@ __global__ void k_windows(const double *s64, const uint64_t *skeys, uint64_t S, 
   double klo = -INFINITY, khi = INFINITY;
   double tp = t * (1 + 16 * kEps);
   if (tp < 0.5 && t > 1e-290) {
-    double a = s64[L_K0 * S + i], b = s64[L_K1 * S + i];
+    double a = r64[i * kRec + L_K0], b = r64[i * kRec + L_K1];
     double ma = fmax(fmax(fabs(a), fabs(b)), 1e-10);
     double r = tp * ma / (1 - tp) * (1 + 16 * kEps);
     double pa = 16 * kEps * (fabs(a) + r), pb = 16 * kEps * (fabs(b) + r);
@@ -203,8 +201,7 @@ __global__ void k_windows(const double *s64, const uint64_t *skeys, uint64_t S, 
   }
   win[i] = lo;
   win[S + i] = hi;
-  kw[2 * i] = klo;
-  kw[2 * i + 1] = khi;
+  kw[i] = make_double2(klo, khi);
 }
 
 // Exact reference predicate in FP64 (pairing.cpp:22-28).
@@ -225,7 +222,6 @@ __device__ inline bool accept64(double a0, double a1, double b0, double b1, doub
 }
 
 // FP32 screen with guard bands: 0 reject, 1 accept, 2 undecided (FP64 recheck).
-// gk bounds the relative error of the curvature tests, gd of the distance.
 __device__ inline int screen32(float a0, float a1, float b0, float b1, float t, float gk) {
   float m = fmaxf(fmaxf(fabsf(a0), fabsf(a1)), fmaxf(fmaxf(fabsf(b0), fabsf(b1)), 1e-10f));
   if (!(m < 1e30f))
@@ -249,13 +245,13 @@ __device__ inline int screen32_dist(float dx, float dy, float dz, float dist, fl
 }
 
 struct PairArgs {
-  const double *s64;
-  const float *s32;
+  const double *r64;
+  const float *r32;
   const uint32_t *bord;
-  const double *bk0, *bk1;
-  const float *bk0f, *bk1f;
+  const double2 *bk;   // (kmin, kmax) at bucketed positions, kmax sorted per bucket
+  const float2 *bkf;   // FP32 copy
   const uint64_t *win;
-  const double *kw;
+  const double2 *kw;
   uint64_t S;
   double t, dist;
   float t32, dist32, gk, gd;
@@ -265,34 +261,49 @@ struct PairArgs {
   uint32_t *cand, *cand2, *rechk;  // per source statistics
 };
 
-// Decide candidate at bucketed position j (ordinal o) for source i.
+template <bool kGuarded> __device__ inline PairArgs with_bands(const PairArgs &in) {
+  PairArgs A = in;
+  if (kGuarded)
+    A.gd = in.gd0 + in.gd1 * __uint_as_float(*in.rmax_bits);
+  return A;
+}
+
+struct Source {
+  double a0, a1, ax, ay, az;
+  float f0, f1, fx, fy, fz;
+};
+
+// Decide candidate at bucketed position j (ordinal o).
 template <bool kGuarded>
-__device__ inline bool decide(const PairArgs &A, uint64_t i, uint64_t j, uint32_t o, double a0,
-                              double a1, float f0, float f1, unsigned &rechecks) {
-  const uint64_t S = A.S;
+__device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, uint32_t o,
+                              unsigned &rechecks) {
   if (!kGuarded) {
-    double b0 = A.bk0[j], b1 = A.bk1[j];
-    double m = fmax(fmax(fabs(a0), fabs(a1)), fmax(fmax(fabs(b0), fabs(b1)), 1e-10));
+    double2 b = A.bk[j];
+    double m = fmax(fmax(fabs(src.a0), fabs(src.a1)), fmax(fmax(fabs(b.x), fabs(b.y)), 1e-10));
     double tol = A.t * m;
-    if (fabs(a0 - b0) > tol || fabs(a1 - b1) > tol)
+    if (fabs(src.a0 - b.x) > tol || fabs(src.a1 - b.y) > tol)
       return false;
-    return accept64(a0, a1, b0, b1, A.s64[L_X * S + i], A.s64[L_Y * S + i], A.s64[L_Z * S + i],
-                    A.s64[L_X * S + o], A.s64[L_Y * S + o], A.s64[L_Z * S + o], A.t, A.dist);
+    const double2 *p = reinterpret_cast<const double2 *>(A.r64 + static_cast<uint64_t>(o) * kRec);
+    double2 p0 = p[0], p1 = p[1];
+    return accept64(src.a0, src.a1, b.x, b.y, src.ax, src.ay, src.az, p0.x, p0.y, p1.x, A.t,
+                    A.dist);
   } else {
-    int r = screen32(f0, f1, A.bk0f[j], A.bk1f[j], A.t32, A.gk);
+    float2 b = A.bkf[j];
+    int r = screen32(src.f0, src.f1, b.x, b.y, A.t32, A.gk);
     if (r == 0)
       return false;
     if (r == 1) {
-      int d = screen32_dist(A.s32[L_X * S + i] - A.s32[L_X * S + o],
-                            A.s32[L_Y * S + i] - A.s32[L_Y * S + o],
-                            A.s32[L_Z * S + i] - A.s32[L_Z * S + o], A.dist32, A.gd);
+      float4 p = *reinterpret_cast<const float4 *>(A.r32 + static_cast<uint64_t>(o) * kRec);
+      int d = screen32_dist(src.fx - p.x, src.fy - p.y, src.fz - p.z, A.dist32, A.gd);
       if (d != 2)
         return d == 1;
     }
     ++rechecks;
-    return accept64(a0, a1, A.bk0[j], A.bk1[j], A.s64[L_X * S + i], A.s64[L_Y * S + i],
-                    A.s64[L_Z * S + i], A.s64[L_X * S + o], A.s64[L_Y * S + o],
-                    A.s64[L_Z * S + o], A.t, A.dist);
+    double2 bb = A.bk[j];
+    const double2 *p = reinterpret_cast<const double2 *>(A.r64 + static_cast<uint64_t>(o) * kRec);
+    double2 p0 = p[0], p1 = p[1];
+    return accept64(src.a0, src.a1, bb.x, bb.y, src.ax, src.ay, src.az, p0.x, p0.y, p1.x, A.t,
+                    A.dist);
   }
 }
 
@@ -302,8 +313,8 @@ struct WaveRanges {
   uint32_t pre[kWave + 1];
 };
 
-// Walks every candidate of source i and calls f(accepted, ordinal) once per
-// lane per candidate chunk (uniform control flow: f sees every lane).
+// Walks every candidate of source i; f(accepted, ordinal) is called with all
+// lanes active once per candidate chunk.
 template <bool kGuarded, class F>
 __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W, int lane,
                                    unsigned &rechecks, uint64_t &cand2, F &&f) {
@@ -311,10 +322,18 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
   const uint64_t jlo = A.win[i], jhi = A.win[S + i];
   if (jlo >= jhi)
     return;
-  const double klo = A.kw[2 * i], khi = A.kw[2 * i + 1];
-  const double a0 = A.s64[L_K0 * S + i], a1 = A.s64[L_K1 * S + i];
-  const float f0 = kGuarded ? A.s32[L_K0 * S + i] : 0.f;
-  const float f1 = kGuarded ? A.s32[L_K1 * S + i] : 0.f;
+  const double2 kw = A.kw[i];
+  Source src;
+  {
+    const double2 *p = reinterpret_cast<const double2 *>(A.r64 + i * kRec);
+    double2 p0 = p[0], p1 = p[1], p2 = p[2];
+    src.ax = p0.x, src.ay = p0.y, src.az = p1.x, src.a0 = p1.y, src.a1 = p2.x;
+    if (kGuarded) {
+      const float4 *q = reinterpret_cast<const float4 *>(A.r32 + i * kRec);
+      float4 q0 = q[0], q1 = q[1];
+      src.fx = q0.x, src.fy = q0.y, src.fz = q0.z, src.f0 = q0.w, src.f1 = q1.x;
+    }
+  }
   const uint64_t b0 = jlo / kBucket, b1 = (jhi - 1) / kBucket;
   for (uint64_t bb = b0; bb <= b1; bb += kWave) {
     const uint64_t b = bb + lane;
@@ -322,28 +341,28 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
     if (b <= b1) {
       const uint64_t base = b * kBucket;
       const uint64_t nb = S - base < static_cast<uint64_t>(kBucket) ? S - base : kBucket;
-      uint64_t f = 0, c = nb;
-      while (c) {  // first with kmax >= klo
-        uint64_t h = c / 2, m = f + h;
-        if (A.bk1[base + m] < klo) {
-          f = m + 1;
-          c -= h + 1;
-        } else
-          c = h;
+      // lower and upper bound of the kmax interval, searched together
+      uint64_t fl = 0, cl = nb, fu = 0, cu = nb;
+      while (cl | cu) {
+        if (cl) {
+          uint64_t h = cl / 2, m = fl + h;
+          if (A.bk[base + m].y < kw.x) {
+            fl = m + 1;
+            cl -= h + 1;
+          } else
+            cl = h;
+        }
+        if (cu) {
+          uint64_t h = cu / 2, m = fu + h;
+          if (!(A.bk[base + m].y > kw.y)) {
+            fu = m + 1;
+            cu -= h + 1;
+          } else
+            cu = h;
+        }
       }
-      uint64_t lo = f;
-      f = lo;
-      c = nb - lo;
-      while (c) {  // first with kmax > khi
-        uint64_t h = c / 2, m = f + h;
-        if (!(A.bk1[base + m] > khi)) {
-          f = m + 1;
-          c -= h + 1;
-        } else
-          c = h;
-      }
-      u = static_cast<uint32_t>(base + lo);
-      len = static_cast<uint32_t>(f - lo);
+      u = static_cast<uint32_t>(base + fl);
+      len = fu > fl ? static_cast<uint32_t>(fu - fl) : 0u;
     }
     // exclusive prefix of the sub-range lengths across the wave
     uint32_t x = len;
@@ -373,7 +392,7 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
         const uint64_t j = W.start[l] + (c - W.pre[l]);
         o = A.bord[j];
         if (o != kNone && o > i)
-          ok = decide<kGuarded>(A, i, j, o, a0, a1, f0, f1, rechecks);
+          ok = decide<kGuarded>(A, src, j, o, rechecks);
       }
       f(ok, o);
     }
@@ -381,32 +400,55 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
   }
 }
 
-template <bool kGuarded> __device__ inline PairArgs with_bands(const PairArgs &in) {
-  PairArgs A = in;
-  if (kGuarded)
-    A.gd = in.gd0 + in.gd1 * __uint_as_float(*in.rmax_bits);
-  return A;
+// 64-lane bitonic sort of one uint32 per lane, ascending.
+__device__ inline uint32_t wave_sort_u32(uint32_t v, int lane) {
+  for (int k = 2; k <= kWave; k <<= 1)
+    for (int m = k >> 1; m > 0; m >>= 1) {
+      uint32_t p = __shfl_xor(v, m, kWave);
+      bool up = (lane & k) == 0, lower = (lane & m) == 0;
+      bool keep = (lower == up) ? (v <= p) : (v >= p);
+      v = keep ? v : p;
+    }
+  return v;
 }
 
+// Single walk per source: counts every accepted pair and parks up to kSlab
+// of them, sorted, in the source's slab.
 template <bool kGuarded>
 __global__ void __launch_bounds__(kThreads)
-    k_pair_count(PairArgs Ain, uint64_t *counts, uint32_t *long_list, unsigned *long_count) {
+    k_pair_walk(PairArgs Ain, uint64_t *counts, uint32_t *slab, uint32_t *long_list,
+                unsigned *long_count) {
   __shared__ WaveRanges ranges[kWaves];
+  __shared__ uint32_t list[kWaves][kSlab];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kWaves + wave;
   if (i >= Ain.S)
     return;
   const PairArgs A = with_bands<kGuarded>(Ain);
+  uint32_t *L = list[wave];
   uint64_t cnt = 0, cand2 = 0;
   unsigned rechecks = 0;
-  walk_source<kGuarded>(A, i, ranges[wave], lane, rechecks, cand2,
-                        [&](bool ok, uint32_t) { cnt += __popcll(ballot(ok)); });
+  walk_source<kGuarded>(A, i, ranges[wave], lane, rechecks, cand2, [&](bool ok, uint32_t o) {
+    uint64_t mask = ballot(ok);
+    uint64_t pos = cnt + static_cast<uint64_t>(lanes_below(mask));
+    if (ok && pos < static_cast<uint64_t>(kSlab))
+      L[pos] = o;
+    cnt += __popcll(mask);
+  });
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  if (cnt <= static_cast<uint64_t>(kSlab)) {
+    uint32_t v = static_cast<uint64_t>(lane) < cnt ? L[lane] : kNone;
+    v = wave_sort_u32(v, lane);
+    if (static_cast<uint64_t>(lane) < cnt)
+      slab[i * kSlab + lane] = v;
+  }
   rechecks = wave_sum(rechecks);
   if (lane == 0) {
     counts[i] = cnt;
-    if (cnt > static_cast<uint64_t>(kCap)) {
-      unsigned slot = atomicAdd(long_count, 1u);
-      long_list[slot] = static_cast<uint32_t>(i);
+    if (cnt > static_cast<uint64_t>(kSlab)) {
+      unsigned s = atomicAdd(long_count, 1u);
+      long_list[s] = static_cast<uint32_t>(i);
     }
     uint64_t w = A.win[A.S + i] - A.win[i];
     A.cand[i] = static_cast<uint32_t>(w < kNone ? w : kNone);
@@ -415,51 +457,24 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-template <bool kGuarded>
+// Copies every slab (count <= kSlab) to its output offset.
 __global__ void __launch_bounds__(kThreads)
-    k_pair_fill(PairArgs Ain, const uint64_t *counts, const uint64_t *offsets, const int32_t *sel,
-                int32_t *out_a, int32_t *out_b) {
-  __shared__ WaveRanges ranges[kWaves];
-  __shared__ uint32_t list[kWaves][kCap];
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kWaves + wave;
-  if (i >= Ain.S)
+    k_pair_place(uint64_t S, const uint64_t *counts, const uint64_t *offsets,
+                 const uint32_t *slab, const int32_t *sel, int32_t *out_a, int32_t *out_b) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
+  if (i >= S)
     return;
-  const PairArgs A = with_bands<kGuarded>(Ain);
-  const uint64_t total = counts[i];
-  if (total == 0 || total > static_cast<uint64_t>(kCap))
+  const uint64_t c = counts[i];
+  if (c > static_cast<uint64_t>(kSlab) || static_cast<uint64_t>(lane) >= c)
     return;
-  uint32_t *L = list[wave];
-  uint32_t n = 0;
-  uint64_t cand2 = 0;
-  unsigned rechecks = 0;
-  walk_source<kGuarded>(A, i, ranges[wave], lane, rechecks, cand2, [&](bool ok, uint32_t o) {
-    uint64_t mask = ballot(ok);
-    uint32_t pos = n + static_cast<uint32_t>(lanes_below(mask));
-    if (ok && pos < kCap)
-      L[pos] = o;
-    n += static_cast<uint32_t>(__popcll(mask));
-  });
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  // count and fill make identical decisions; the clamps only keep every
-  // access inside this source's range should they ever disagree
-  if (n > kCap)
-    n = kCap;
-  if (n > total)
-    n = static_cast<uint32_t>(total);
-  const uint64_t base = offsets[i];
-  const int32_t src = sel ? sel[i] : static_cast<int32_t>(i);
-  for (uint32_t e = lane; e < n; e += kWave) {
-    uint32_t v = L[e], r = 0;
-    for (uint32_t k = 0; k < n; ++k)
-      r += L[k] < v;
-    out_a[base + r] = src;
-    out_b[base + r] = sel ? sel[v] : static_cast<int32_t>(v);
-  }
+  const uint32_t o = slab[i * kSlab + lane];
+  const uint64_t p = offsets[i] + lane;
+  out_a[p] = sel ? sel[i] : static_cast<int32_t>(i);
+  out_b[p] = sel ? sel[o] : static_cast<int32_t>(o);
 }
 
-// Sources with more than kCap pairs: walk the targets in ordinal order.
+// Sources with more than kSlab pairs: walk the targets in ordinal order.
 __global__ void __launch_bounds__(kThreads)
     k_pair_long(PairArgs A, const uint32_t *long_list, unsigned long_count,
                 const uint64_t *offsets, const int32_t *sel, int32_t *out_a, int32_t *out_b) {
@@ -469,17 +484,19 @@ __global__ void __launch_bounds__(kThreads)
     return;
   const uint64_t S = A.S;
   const uint64_t i = long_list[w];
-  const double a0 = A.s64[L_K0 * S + i], a1 = A.s64[L_K1 * S + i];
-  const double ax = A.s64[L_X * S + i], ay = A.s64[L_Y * S + i], az = A.s64[L_Z * S + i];
+  const double *ri = A.r64 + i * kRec;
+  const double a0 = ri[L_K0], a1 = ri[L_K1], ax = ri[L_X], ay = ri[L_Y], az = ri[L_Z];
   const int32_t src = sel ? sel[i] : static_cast<int32_t>(i);
   uint64_t cursor = offsets[i];
   const uint64_t end = offsets[i + 1];
   for (uint64_t b0 = i + 1; b0 < S && cursor < end; b0 += kWave) {
     uint64_t b = b0 + lane;
     bool ok = false;
-    if (b < S)
-      ok = accept64(a0, a1, A.s64[L_K0 * S + b], A.s64[L_K1 * S + b], ax, ay, az,
-                    A.s64[L_X * S + b], A.s64[L_Y * S + b], A.s64[L_Z * S + b], A.t, A.dist);
+    if (b < S) {
+      const double *rb = A.r64 + b * kRec;
+      ok = accept64(a0, a1, rb[L_K0], rb[L_K1], ax, ay, az, rb[L_X], rb[L_Y], rb[L_Z], A.t,
+                    A.dist);
+    }
     uint64_t mask = ballot(ok);
     uint64_t p = cursor + lanes_below(mask);
     if (ok && p < end) {
@@ -560,32 +577,28 @@ static float guard_curvature(double t) {
 }
 
 struct PairState {
-  double *s64;
-  float *s32;
-  uint32_t *ord, *bord;
-  double *bk0, *bk1;
-  float *bk0f, *bk1f;
+  double *r64;
+  float *r32;
+  uint32_t *ord, *bord, *slab;
+  double2 *bk, *kw;
+  float2 *bkf;
   uint64_t *keys, *win, *counts, *offsets;
-  double *kw;
   uint32_t *long_list, *cand, *cand2, *rechk;
 };
 
 static PairState pair_state(Session *s, uint64_t S, bool guarded) {
   PairState st{};
   const uint64_t Sp = S ? S : 1;
-  st.s64 = scratch_as<double>(s, kSlotSamples, L_N * Sp);
-  st.s32 = guarded ? scratch_as<float>(s, kSlotPlanSamples32, L_N * Sp) : nullptr;
+  st.r64 = scratch_as<double>(s, kSlotSamples, kRec * Sp);
+  st.r32 = guarded ? scratch_as<float>(s, kSlotPlanSamples32, kRec * Sp) : nullptr;
   st.ord = scratch_as<uint32_t>(s, kSlotPlanOrd, 2 * Sp);  // [0,S) sort input, [S,2S) sorted
-  double *bk = scratch_as<double>(s, kSlotPlanSortedK, 2 * Sp);
-  st.bk0 = bk;
-  st.bk1 = bk + Sp;
-  float *bkf = guarded ? scratch_as<float>(s, kSlotPlanSortedK32, 2 * Sp) : nullptr;
-  st.bk0f = bkf;
-  st.bk1f = bkf ? bkf + Sp : nullptr;
+  st.bk = scratch_as<double2>(s, kSlotPlanSortedK, Sp);
+  st.bkf = guarded ? scratch_as<float2>(s, kSlotPlanSortedK32, Sp) : nullptr;
   st.bord = scratch_as<uint32_t>(s, kSlotWindows, Sp);
+  st.slab = scratch_as<uint32_t>(s, kSlotMisc2, kSlab * Sp);
   st.keys = scratch_as<uint64_t>(s, kSlotPlanKeys, 2 * Sp);
   st.win = scratch_as<uint64_t>(s, kSlotPlanWin, 2 * Sp);
-  st.kw = scratch_as<double>(s, kSlotMisc3, 2 * Sp);
+  st.kw = scratch_as<double2>(s, kSlotMisc3, Sp);
   st.counts = scratch_as<uint64_t>(s, kSlotPlanCounts, Sp + 1);
   st.offsets = scratch_as<uint64_t>(s, kSlotPlanOffsets, Sp + 1);
   st.long_list = scratch_as<uint32_t>(s, kSlotPlanLong, Sp + 1);
@@ -599,13 +612,11 @@ static PairState pair_state(Session *s, uint64_t S, bool guarded) {
 static PairArgs pair_args(const PairState &st, uint64_t S, double t, double box, bool guarded,
                           const unsigned *flags) {
   PairArgs A{};
-  A.s64 = st.s64;
-  A.s32 = st.s32;
+  A.r64 = st.r64;
+  A.r32 = st.r32;
   A.bord = st.bord;
-  A.bk0 = st.bk0;
-  A.bk1 = st.bk1;
-  A.bk0f = st.bk0f;
-  A.bk1f = st.bk1f;
+  A.bk = st.bk;
+  A.bkf = st.bkf;
   A.win = st.win;
   A.kw = st.kw;
   A.S = S;
@@ -625,10 +636,10 @@ static PairArgs pair_args(const PairState &st, uint64_t S, double t, double box,
 }
 
 const double *plan_samples64(Session *s) {
-  return scratch_as<double>(s, kSlotSamples, L_N * (s->pair_plan.S ? s->pair_plan.S : 1));
+  return scratch_as<double>(s, kSlotSamples, kRec * (s->pair_plan.S ? s->pair_plan.S : 1));
 }
 const float *plan_samples32(Session *s) {
-  return scratch_as<float>(s, kSlotPlanSamples32, L_N * (s->pair_plan.S ? s->pair_plan.S : 1));
+  return scratch_as<float>(s, kSlotPlanSamples32, kRec * (s->pair_plan.S ? s->pair_plan.S : 1));
 }
 
 void pair_count_device(Session *s, uint64_t n, const double *x, const double *y, const double *z,
@@ -654,7 +665,7 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   double cx = center ? center[0] : 0, cy = center ? center[1] : 0, cz = center ? center[2] : 0;
   if (S)
     MVS_LAUNCH(s, "pair_gather", k_gather_samples, blocks_for(S, kThreads), kThreads, 0, x, y, z,
-               feat, n, sel, S, cx, cy, cz, st.s64, st.s32, st.keys, st.ord, flags,
+               feat, n, sel, S, cx, cy, cz, st.r64, st.r32, st.keys, st.ord, flags,
                flags + kRmaxWord);
   if (S < 2) {  // no pair possible; like the reference, nothing to validate
     plan.P = 0;
@@ -665,17 +676,17 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   radix_sort_pairs_u64(s, st.keys, st.ord, st.keys + S, st.ord + S, S, 32);
   const uint64_t nb = (S + kBucket - 1) / kBucket;
   MVS_LAUNCH(s, "pair_index", k_bucket_index, blocks_for(nb, kWaves), kThreads, 0, st.ord + S,
-             st.s64, st.s32, S, st.bord, st.bk0, st.bk1, st.bk0f, st.bk1f);
-  MVS_LAUNCH(s, "pair_windows", k_windows, blocks_for(S, kThreads), kThreads, 0, st.s64,
+             st.r64, st.r32, S, st.bord, st.bk, st.bkf);
+  MVS_LAUNCH(s, "pair_windows", k_windows, blocks_for(S, kThreads), kThreads, 0, st.r64,
              st.keys + S, S, t, st.win, st.kw);
   unsigned *long_count = flags + 1;
   PairArgs A = pair_args(st, S, t, box, guarded, flags);
   unsigned grid = blocks_for(S, kWaves);
   if (guarded)
-    MVS_LAUNCH(s, "pair_count", k_pair_count<true>, grid, kThreads, 0, A, st.counts, st.long_list,
-               long_count);
+    MVS_LAUNCH(s, "pair_walk", k_pair_walk<true>, grid, kThreads, 0, A, st.counts, st.slab,
+               st.long_list, long_count);
   else
-    MVS_LAUNCH(s, "pair_count", k_pair_count<false>, grid, kThreads, 0, A, st.counts,
+    MVS_LAUNCH(s, "pair_walk", k_pair_walk<false>, grid, kThreads, 0, A, st.counts, st.slab,
                st.long_list, long_count);
   exclusive_scan_u64(s, st.counts, st.offsets, S);
   // flags, long-source count and total read back in one synchronisation
@@ -708,13 +719,8 @@ void pair_fill_device(Session *s, int32_t *a, int32_t *b, bool ordinals) {
   PairState st = pair_state(s, S, guarded);
   PairArgs A = pair_args(st, S, plan.threshold, plan.box, guarded, plan_flags(s));
   const int32_t *sel = ordinals ? nullptr : plan.selected;
-  unsigned grid = blocks_for(S, kWaves);
-  if (guarded)
-    MVS_LAUNCH(s, "pair_fill", k_pair_fill<true>, grid, kThreads, 0, A, st.counts, st.offsets, sel,
-               a, b);
-  else
-    MVS_LAUNCH(s, "pair_fill", k_pair_fill<false>, grid, kThreads, 0, A, st.counts, st.offsets,
-               sel, a, b);
+  MVS_LAUNCH(s, "pair_place", k_pair_place, blocks_for(S, kWaves), kThreads, 0, S, st.counts,
+             st.offsets, st.slab, sel, a, b);
   if (plan.long_sources)
     MVS_LAUNCH(s, "pair_long", k_pair_long, blocks_for(plan.long_sources, kWaves), kThreads, 0, A,
                st.long_list, static_cast<unsigned>(plan.long_sources), st.offsets, sel, a, b);
@@ -829,8 +835,7 @@ int mvs_pairing(mvs_session *sp, const mvs_points *v, const mvs_features *f, uin
     for (int k = 0; k < 11; ++k)
       copy_h2d(s, feat.ptr + k * n, f->lane[k], n * 8);
     copy_h2d(s, dsel.ptr, selected, S * 4);
-    // FP32 coordinates are taken relative to the bounding-box midpoint, which
-    // keeps them within the box size the guard bands assume
+    // FP32 coordinates are taken relative to the bounding-box midpoint
     double ctr[3] = {0, 0, 0};
     if (n) {
       const double *c3[3] = {v->x, v->y, v->z};

@@ -5,12 +5,14 @@
 
 namespace mvs {
 
-// Ordinal-order sample SoA lanes (14 per sample): position, kmin, kmax,
-// normal, dmin, dmax.  The FP32 copy holds positions relative to the model
-// centre.
-enum SampleLane {
+// Ordinal-order sample records (AoS, kRec scalars per sample, 14 used):
+// position, kmin, kmax, normal, dmin, dmax, 2 pad.  One FP32 record is one
+// 64-byte cache line, an FP64 record two; the FP32 copy holds positions
+// relative to the model centre.
+enum SampleField {
   L_X = 0, L_Y, L_Z, L_K0, L_K1, L_NX, L_NY, L_NZ, L_DX, L_DY, L_DZ, L_XX, L_XY, L_XZ, L_N
 };
+constexpr int kRec = 16;
 
 // Layout of the small flags slot shared by the pairing and vote phases
 // (kFlagsBytes bytes): 32-bit words 0 error flags, 1 long-source count,
@@ -35,7 +37,33 @@ void pair_fill_device(Session *s, int32_t *a, int32_t *b, bool ordinals);
 // 2-D index candidates and FP64 rechecks.
 void pair_statistics(Session *s, uint64_t out[3]);
 
-// Sample SoA of the last pair count (valid until the next count).
+// Vote phases (votes.hip).
+void vote_count_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b, int precision,
+                       uint64_t *R_out);
+void vote_emit_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
+                      const double *center, int precision, void *q4, void *t3, int32_t *rp,
+                      void *n3, void *d1, int32_t *fp);
+// Vote phases without the reflection read-back: mirror votes go to a stream
+// of capacity (and SoA stride) `stride`; the total is left on the device at
+// *total_dev (uint64).
+void vote_offsets_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
+                         int precision, uint64_t *total_dev);
+void vote_emit_strided(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
+                       const double *center, int precision, uint64_t stride, void *q4, void *t3,
+                       int32_t *rp, void *n3, void *d1, int32_t *fp);
+
+// Batched mean-shift (msbatch.hip).
+void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,
+                    const float *seeds, const int32_t *seed_idx, int iterations, double h,
+                    double cutoff, float *out, float *weight, unsigned long long *stats);
+void ms_batched_f64(Session *s, int D, uint64_t n, const double *pts, uint64_t ns,
+                    const double *seeds, const int32_t *seed_idx, int iterations, double h,
+                    double cutoff, double *out, double *weight, unsigned long long *stats);
+void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4, const void *t3,
+                      uint64_t nrefl, const void *n3, const void *d1, void *rigid_pts,
+                      void *refl_pts);
+
+// Sample records of the last pair count (valid until the next count).
 const double *plan_samples64(Session *s);
 const float *plan_samples32(Session *s);
 

@@ -41,6 +41,8 @@ void *scratch(Session *s, int slot, size_t bytes) {
 
 void *host_stage(Session *s, size_t bytes) {
   if (bytes > s->stage_bytes) {
+    // queued copies may still use the old staging buffer
+    MVS_HIP(hipStreamSynchronize(s->stream));
     if (s->stage)
       (void)hipHostFree(s->stage);
     s->stage = nullptr;
@@ -167,6 +169,14 @@ int mvs_session_set_stream(mvs_session *s, void *stream) {
 
 int mvs_session_sync(mvs_session *s) {
   return guarded([&] { sync(as_session(s)); });
+}
+
+int mvs_dev_copy(mvs_session *sp, void *dst, const void *src, uint64_t bytes) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    if (bytes)
+      MVS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s->stream));
+  });
 }
 
 int mvs_session_enable_timing(mvs_session *s, int enable) {

@@ -43,6 +43,9 @@ struct Session {
   std::map<std::string, std::pair<double, uint64_t>> timings;
   PairPlan pair_plan;
   VotePlan vote_plan;
+  // outputs of the last fused pair + vote call (session-owned buffers)
+  mvs_pairvote_result pairvote{};
+  bool pairvote_valid = false;
   // host-layer results awaiting fetch
   DBuf<int32_t> out_a, out_b, out_pair;
   DBuf<uint8_t> out_kind;

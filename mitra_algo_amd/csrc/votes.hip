@@ -95,16 +95,21 @@ __device__ inline F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * 
 __device__ inline float fdot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ inline float fcomp(F3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 
-__device__ inline V3 lane3(const double *s, uint64_t S, int l, uint64_t i) {
-  return v3(s[l * S + i], s[(l + 1) * S + i], s[(l + 2) * S + i]);
+// Three consecutive fields of sample i's record (AoS, kRec scalars each).
+__device__ inline V3 field3(const double *r, uint64_t S, int l, uint64_t i) {
+  (void)S;
+  const double *p = r + i * kRec + l;
+  return v3(p[0], p[1], p[2]);
 }
-__device__ inline F3 lane3f(const float *s, uint64_t S, int l, uint64_t i) {
-  return f3(s[l * S + i], s[(l + 1) * S + i], s[(l + 2) * S + i]);
+__device__ inline F3 field3f(const float *r, uint64_t S, int l, uint64_t i) {
+  (void)S;
+  const float *p = r + i * kRec + l;
+  return f3(p[0], p[1], p[2]);
 }
 __device__ inline Geo64 geo_samples(const double *s, uint64_t S, uint64_t i, uint64_t o) {
-  return geo64(lane3(s, S, L_X, i), lane3(s, S, L_X, o), lane3(s, S, L_NX, i),
-               lane3(s, S, L_DX, i), lane3(s, S, L_XX, i), lane3(s, S, L_NX, o),
-               lane3(s, S, L_DX, o), lane3(s, S, L_XX, o));
+  return geo64(field3(s, S, L_X, i), field3(s, S, L_X, o), field3(s, S, L_NX, i),
+               field3(s, S, L_DX, i), field3(s, S, L_XX, i), field3(s, S, L_NX, o),
+               field3(s, S, L_DX, o), field3(s, S, L_XX, o));
 }
 
 struct VoteArgs {
@@ -137,13 +142,13 @@ __device__ inline bool pair_reflection(const VoteArgs &A, uint64_t k, unsigned &
   const uint64_t S = A.S;
   uint64_t i = static_cast<uint64_t>(A.a[k]), o = static_cast<uint64_t>(A.b[k]);
   if (kGuarded) {
-    F3 p = lane3f(A.s32, S, L_X, i), q = lane3f(A.s32, S, L_X, o);
+    F3 p = field3f(A.s32, S, L_X, i), q = field3f(A.s32, S, L_X, o);
     F3 dv = q - p;
     float len = sqrtf(fdot(dv, dv));
     if (len > 1e-6f) {
       F3 n = dv * (1.0f / len);
-      F3 an = lane3f(A.s32, S, L_NX, i), ad = lane3f(A.s32, S, L_DX, i);
-      F3 bn = lane3f(A.s32, S, L_NX, o), bd = lane3f(A.s32, S, L_DX, o);
+      F3 an = field3f(A.s32, S, L_NX, i), ad = field3f(A.s32, S, L_DX, i);
+      F3 bn = field3f(A.s32, S, L_NX, o), bd = field3f(A.s32, S, L_DX, o);
       // (I - 2 n n^T) v = v - 2 (n.v) n
       F3 ma = an - n * (2.0f * fdot(n, an)), md = ad - n * (2.0f * fdot(n, ad));
       float r1 = fdot(ma, bn), r2 = fabsf(fdot(md, bd));
@@ -228,11 +233,11 @@ __global__ void __launch_bounds__(kThreads)
     }
   } else {
     // FP32 frames in centre-relative coordinates: t_c = q' - R p'
-    F3 p = lane3f(A.s32, S, L_X, i), q = lane3f(A.s32, S, L_X, o);
-    F3 an = lane3f(A.s32, S, L_NX, i), ad = lane3f(A.s32, S, L_DX, i),
-       ax = lane3f(A.s32, S, L_XX, i);
-    F3 bn = lane3f(A.s32, S, L_NX, o), bd = lane3f(A.s32, S, L_DX, o),
-       bx = lane3f(A.s32, S, L_XX, o);
+    F3 p = field3f(A.s32, S, L_X, i), q = field3f(A.s32, S, L_X, o);
+    F3 an = field3f(A.s32, S, L_NX, i), ad = field3f(A.s32, S, L_DX, i),
+       ax = field3f(A.s32, S, L_XX, i);
+    F3 bn = field3f(A.s32, S, L_NX, o), bd = field3f(A.s32, S, L_DX, o),
+       bx = field3f(A.s32, S, L_XX, o);
     for (int sg = 0; sg < 2; ++sg) {
       float s = sg == 0 ? 1.0f : -1.0f;
       F3 b0 = bd * s, b1 = bx * s;
@@ -400,8 +405,8 @@ static VoteArgs vote_args(Session *s, uint64_t P, const int32_t *a, const int32_
   return A;
 }
 
-void vote_count_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b, int precision,
-                       uint64_t *R_out) {
+void vote_offsets_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
+                         int precision, uint64_t *total_dev) {
   if (precision != MVS_FP64 && precision != MVS_FP32_GUARDED)
     raise(MVS_EINVAL, "unknown precision");
   const bool guarded = precision == MVS_FP32_GUARDED;
@@ -417,8 +422,34 @@ void vote_count_device(Session *s, uint64_t P, const int32_t *a, const int32_t *
       MVS_LAUNCH(s, "vote_count", k_vote_count<false>, grid, kThreads, 0, A, wr);
   }
   exclusive_scan_u32(s, wr, off, nw);
+  MVS_LAUNCH(s, "read_total", k_read_total64, 1, 1, 0, off, nw, total_dev);
+}
+
+void vote_emit_strided(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
+                       const double *center, int precision, uint64_t stride, void *q4, void *t3,
+                       int32_t *rp, void *n3, void *d1, int32_t *fp) {
+  if (P == 0)
+    return;
+  const bool guarded = precision == MVS_FP32_GUARDED;
+  VoteArgs A = vote_args(s, P, a, b, center, guarded);
+  const uint64_t nw = (P + kWave - 1) / kWave;
+  uint64_t *off = scratch_as<uint64_t>(s, kSlotVoteBlockOff, nw + 1);
+  unsigned grid = blocks_for(nw, kThreads / kWave);
+  if (guarded) {
+    VoteOut<float> o{static_cast<float *>(q4), static_cast<float *>(t3), rp,
+                     static_cast<float *>(n3), static_cast<float *>(d1), fp, stride};
+    MVS_LAUNCH(s, "vote_emit", (k_vote_emit<true, float>), grid, kThreads, 0, A, off, o);
+  } else {
+    VoteOut<double> o{static_cast<double *>(q4), static_cast<double *>(t3), rp,
+                      static_cast<double *>(n3), static_cast<double *>(d1), fp, stride};
+    MVS_LAUNCH(s, "vote_emit", (k_vote_emit<false, double>), grid, kThreads, 0, A, off, o);
+  }
+}
+
+void vote_count_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b, int precision,
+                       uint64_t *R_out) {
   uint64_t *tot = reinterpret_cast<uint64_t *>(plan_flags(s)) + kVoteTotalWord;
-  MVS_LAUNCH(s, "read_total", k_read_total64, 1, 1, 0, off, nw, tot);
+  vote_offsets_device(s, P, a, b, precision, tot);
   uint64_t R;
   read_back(s, &R, tot, sizeof R);
   s->vote_plan.P = P;
@@ -431,25 +462,10 @@ void vote_count_device(Session *s, uint64_t P, const int32_t *a, const int32_t *
 void vote_emit_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
                       const double *center, int precision, void *q4, void *t3, int32_t *rp,
                       void *n3, void *d1, int32_t *fp) {
-  const bool guarded = precision == MVS_FP32_GUARDED;
   VotePlan &vp = s->vote_plan;
   if (!vp.valid || vp.P != P || vp.precision != precision)
     raise(MVS_EINVAL, "vote emission without a matching vote count");
-  if (P == 0)
-    return;
-  VoteArgs A = vote_args(s, P, a, b, center, guarded);
-  const uint64_t nw = (P + kWave - 1) / kWave;
-  uint64_t *off = scratch_as<uint64_t>(s, kSlotVoteBlockOff, nw + 1);
-  unsigned grid = blocks_for(nw, kThreads / kWave);
-  if (guarded) {
-    VoteOut<float> o{static_cast<float *>(q4), static_cast<float *>(t3), rp,
-                     static_cast<float *>(n3), static_cast<float *>(d1), fp, vp.R};
-    MVS_LAUNCH(s, "vote_emit", (k_vote_emit<true, float>), grid, kThreads, 0, A, off, o);
-  } else {
-    VoteOut<double> o{static_cast<double *>(q4), static_cast<double *>(t3), rp,
-                      static_cast<double *>(n3), static_cast<double *>(d1), fp, vp.R};
-    MVS_LAUNCH(s, "vote_emit", (k_vote_emit<false, double>), grid, kThreads, 0, A, off, o);
-  }
+  vote_emit_strided(s, P, a, b, center, precision, vp.R, q4, t3, rp, n3, d1, fp);
 }
 
 }  // namespace mvs

@@ -105,6 +105,26 @@ class Workload:
         return PairVoteResult(P, R, a, b, q4, t3, rp, n3, d1, fp)
 
 
+    # ---------------------------------------------------------- fused step
+    def pairvote(self, precision: int, max_pairs: int = 2**31 - 1,
+                 max_votes: int = 2**31 - 1) -> capi.PairVoteResult:
+        return self.sess.dev_pairvote(self.n, self.x, self.y, self.z, self.feat, self.S,
+                                      self.selected, self.box, self.center, self.threshold,
+                                      max_pairs, max_votes, precision)
+
+    def meanshift(self, seeds: int = 20000, iterations: int = 30, h: float = 0.05,
+                  cutoff: float = 3.0, seed_value: int = 7,
+                  want_stats: bool = False) -> capi.MeanShiftResult:
+        return self.sess.dev_meanshift_votes(seeds, seed_value, iterations, h, cutoff,
+                                             want_stats)
+
+    def step(self, precision: int, seeds: int = 20000, iterations: int = 30):
+        """One pass of the voting core: pairs + votes, then batched mean-shift."""
+        pv = self.pairvote(precision)
+        ms = self.meanshift(seeds, iterations)
+        return pv, ms
+
+
 def cfg1(device: str = "cuda:0") -> Workload:
     """96x96 height field, 2,000 uniform samples, magnitude drop, t = .05."""
     return Workload(meshgen.cfg1_mesh(), 2000, 0.05, True, device=device)

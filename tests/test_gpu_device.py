@@ -152,3 +152,83 @@ def test_batched_meanshift_empty_and_degenerate(oracle):
     torch.cuda.synchronize()
     assert np.allclose(out.cpu().numpy(), P[:, :2], atol=1e-15)
     s.close()
+
+
+def _fetch_pairvote(sess, r, prec):
+    dt = torch.float32 if prec == capi.FP32_GUARDED else torch.float64
+    P, R, st = r.pairs, r.reflections, r.refl_stride
+    return dict(
+        a=sess.fetch(r.a, (P,), torch.int32), b=sess.fetch(r.b, (P,), torch.int32),
+        q4=sess.fetch(r.rigid_q4, (4, 2 * P), dt, 2 * P),
+        t3=sess.fetch(r.rigid_t3, (3, 2 * P), dt, 2 * P),
+        rp=sess.fetch(r.rigid_pair, (2 * P,), torch.int32),
+        n3=sess.fetch(r.refl_n3, (3, R), dt, st), d1=sess.fetch(r.refl_d, (R,), dt),
+        fp=sess.fetch(r.refl_pair, (R,), torch.int32))
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32_GUARDED])
+def test_fused_pairvote_matches_stepwise(cfg1, prec):
+    wl = cfg1
+    wl.threshold = 0.05
+    step = wl.pair_votes(prec)
+    fused = _fetch_pairvote(wl.sess, wl.pairvote(prec), prec)
+    torch.cuda.synchronize()
+    P, R = step.P, step.R
+    assert fused["a"].numel() == P and fused["fp"].numel() == R
+    assert torch.equal(fused["a"], step.a[:P]) and torch.equal(fused["b"], step.b[:P])
+    assert torch.equal(fused["q4"], step.q4[:, :2 * P]) and torch.equal(fused["t3"],
+                                                                       step.t3[:, :2 * P])
+    assert torch.equal(fused["n3"], step.n3[:, :R]) and torch.equal(fused["d1"], step.d1[:R])
+    assert torch.equal(fused["fp"], step.fp[:R]) and torch.equal(fused["rp"], step.rp[:2 * P])
+    with pytest.raises(capi.MvsBudgetError):
+        wl.pairvote(prec, max_pairs=max(P - 1, 0))
+    with pytest.raises(capi.MvsBudgetError):
+        wl.pairvote(prec, max_votes=2 * P + R - 1)
+
+
+def _rodrigues(r):
+    th = np.linalg.norm(r)
+    if th < 1e-15:
+        return np.eye(3)
+    k = r / th
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + np.sin(th) * K + (1 - np.cos(th)) * K @ K
+
+
+def test_fused_meanshift_matches_oracle(cfg1, oracle):
+    wl = cfg1
+    wl.threshold = 0.05
+    prec = capi.FP64
+    r = wl.pairvote(prec)
+    f = _fetch_pairvote(wl.sess, r, prec)
+    P, R = r.pairs, r.reflections
+    rp = torch.empty((6, 2 * P), dtype=torch.float64, device="cuda:0")
+    fpts = torch.empty((4, R), dtype=torch.float64, device="cuda:0")
+    wl.sess.dev_ms_points(prec, 2 * P, f["q4"], f["t3"], R, f["n3"].contiguous(), f["d1"], rp,
+                          fpts)
+    ms = wl.meanshift(seeds=300, iterations=12, want_stats=True)
+    seeds = wl.sess.fetch(ms.seed_votes, (ms.rigid_seeds + ms.refl_seeds,), torch.int32)
+    got_r = wl.sess.fetch(ms.rigid_modes, (6, ms.rigid_seeds), torch.float64, ms.rigid_seeds)
+    got_f = wl.sess.fetch(ms.refl_modes, (4, ms.refl_seeds), torch.float64, ms.refl_seeds)
+    torch.cuda.synchronize()
+    assert ms.rigid_seeds + ms.refl_seeds == 300 and ms.refl_seeds > 0
+    seeds = seeds.cpu().numpy()
+    P6 = rp.cpu().numpy()
+    P4 = fpts.cpu().numpy()
+    # axis-angle points reproduce the vote rotations (closed form: Rodrigues)
+    q = f["q4"].cpu().numpy()
+    for i in range(0, 2 * P, max(1, (2 * P) // 50)):
+        w, x, y, z = q[:, i]
+        Rq = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                       [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                       [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+        assert np.allclose(_rodrigues(P6[3:6, i]), Rq, atol=1e-9)
+    kr = ms.rigid_seeds
+    want_r, _ = oracle.meanshift_batched(P6, np.zeros(P6.shape[1], np.uint8), 6, 6,
+                                         P6[:, seeds[:kr]], np.zeros(kr, np.uint8), 0.05, 3.0, 12)
+    want_f, _ = oracle.meanshift_batched(P4, np.zeros(P4.shape[1], np.uint8), 4, 4,
+                                         P4[:, seeds[kr:]], np.zeros(len(seeds) - kr, np.uint8),
+                                         0.05, 3.0, 12)
+    assert np.max(np.abs(got_r.cpu().numpy() - want_r)) < 1e-9
+    assert np.max(np.abs(got_f.cpu().numpy() - want_f)) < 1e-9
+    assert 0 < ms.in_support <= ms.candidates
