@@ -104,6 +104,7 @@ enum Slot : int {
   // pairing plan (persist between count, fill and vote emission)
   kSlotPlanSamples32, kSlotPlanKeys, kSlotPlanOrd, kSlotPlanSortedK, kSlotPlanSortedK32,
   kSlotPlanWin, kSlotPlanCounts, kSlotPlanOffsets, kSlotPlanLong, kSlotPlanStats,
+  kSlotPlanSlab, kSlotPlanKw, kSlotPlanLongCnt, kSlotPlanLongOff,
   kSlotVoteBlock, kSlotVoteBlockOff,
   kSlotMs0, kSlotMs1, kSlotMs2, kSlotMs3, kSlotMs4, kSlotMs5, kSlotMs6, kSlotMs7,
   // fused step outputs (valid until the next fused call)

@@ -9,25 +9,32 @@
 //   w = exp(-d^2 / 2h^2) - exp(-cutoff^2 / 2)   for d < cutoff*h,
 // so the support boundary carries no weight jump.
 //
-// Grid: cells of side >= cutoff*h over the first three coordinates; points
-// are radix-sorted by cell (stable, so sums are deterministic) and indexed by
-// an exclusive scan of the per-cell counts.  With x the fastest cell axis the
-// three cells of one (y, z) row are contiguous, so a query visits nine
-// contiguous point ranges.  One wavefront per seed runs all iterations in
-// registers; the 64 lanes stride the ranges and reduce with shuffles.
+// Grid: cells of side >= cutoff*h over the three coordinates with the
+// widest extent (for height fields: two translation components and the
+// rotation about z), so a cell holds few points outside the support.
+// Points are radix-sorted by cell (stable: sums are deterministic) and
+// indexed by per-cell offsets found by binary search (no atomics).  With the
+// first grid axis fastest, the three cells of one row are contiguous, so a
+// query visits nine contiguous point ranges.  One wavefront per seed runs
+// all iterations in registers; lanes stride the ranges and reduce with
+// shuffles.  Seeds run in cell order and blocks are mapped so that each XCD
+// takes a contiguous run of seeds: neighbouring queries share that XCD's L2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
 
-#include "common.hpp"
-#include "session.hpp"
+#include "pairing.hpp"
 
 namespace mvs {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kWavesPerBlock = kThreads / kWave;
+constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, blocks dealt round-robin
+constexpr int kMaxD = 6;
 
 template <class R> __device__ inline R ms_exp(R x);
 template <> __device__ inline float ms_exp<float>(float x) { return __expf(x); }
@@ -60,61 +67,81 @@ __global__ void k_points_refl(const R *n3, const R *d1, uint64_t n, R *pts) {
   pts[3 * n + i] = n3[2 * n + i];
 }
 
-// per-block min/max of the first three coordinates
+// per-block min/max of every coordinate: partial[b*2*kMaxD + k] (lo), + kMaxD (hi)
 template <class R>
-__global__ void k_bbox(const R *pts, uint64_t n, double *partial) {
-  __shared__ double s[6][kThreads];
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+__global__ void k_bbox(const R *pts, uint64_t n, int D, double *partial) {
+  __shared__ double s[2 * kMaxD][kThreads];
+  double lo[kMaxD], hi[kMaxD];
+  for (int k = 0; k < kMaxD; ++k)
+    lo[k] = INFINITY, hi[k] = -INFINITY;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * kThreads)
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < D; ++k) {
       double v = static_cast<double>(pts[k * n + i]);
       lo[k] = fmin(lo[k], v);
       hi[k] = fmax(hi[k], v);
     }
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < kMaxD; ++k) {
     s[k][threadIdx.x] = lo[k];
-    s[3 + k][threadIdx.x] = hi[k];
+    s[kMaxD + k][threadIdx.x] = hi[k];
   }
   __syncthreads();
   for (int o = kThreads / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o)
-      for (int k = 0; k < 3; ++k) {
+      for (int k = 0; k < kMaxD; ++k) {
         s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + o]);
-        s[3 + k][threadIdx.x] = fmax(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + o]);
+        s[kMaxD + k][threadIdx.x] = fmax(s[kMaxD + k][threadIdx.x], s[kMaxD + k][threadIdx.x + o]);
       }
     __syncthreads();
   }
-  if (threadIdx.x < 6)
-    partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+  if (threadIdx.x < 2 * kMaxD)
+    partial[blockIdx.x * 2 * kMaxD + threadIdx.x] = s[threadIdx.x][0];
 }
 
 struct Grid {
+  int axis[3];
   double lo[3];
   double cell;
   int dims[3];
   uint64_t ncells;
 };
 
-template <class R> __device__ inline int cell_of(R v, double lo, double cell, int dim) {
-  double c = floor((static_cast<double>(v) - lo) / cell);
-  int ci = c < 0 ? 0 : (c >= dim ? dim - 1 : static_cast<int>(c));
-  return ci;
+__device__ inline int cell_of(double v, double lo, double cell, int dim) {
+  double c = floor((v - lo) / cell);
+  return c < 0 ? 0 : (c >= dim ? dim - 1 : static_cast<int>(c));
+}
+template <class R> __device__ inline uint64_t cell_key(const Grid &g, R a, R b, R c) {
+  int x = cell_of(static_cast<double>(a), g.lo[0], g.cell, g.dims[0]);
+  int y = cell_of(static_cast<double>(b), g.lo[1], g.cell, g.dims[1]);
+  int z = cell_of(static_cast<double>(c), g.lo[2], g.cell, g.dims[2]);
+  return (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0] + x;
 }
 
 template <class R>
-__global__ void k_cell_keys(const R *pts, uint64_t n, Grid g, uint64_t *keys, uint32_t *vals,
-                            uint32_t *counts) {
+__global__ void k_cell_keys(const R *pts, uint64_t n, Grid g, uint64_t *keys, uint32_t *vals) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
-  int cx = cell_of(pts[i], g.lo[0], g.cell, g.dims[0]);
-  int cy = cell_of(pts[n + i], g.lo[1], g.cell, g.dims[1]);
-  int cz = cell_of(pts[2 * n + i], g.lo[2], g.cell, g.dims[2]);
-  uint64_t c = (static_cast<uint64_t>(cz) * g.dims[1] + cy) * g.dims[0] + cx;
-  keys[i] = c;
+  keys[i] = cell_key(g, pts[g.axis[0] * n + i], pts[g.axis[1] * n + i], pts[g.axis[2] * n + i]);
   vals[i] = static_cast<uint32_t>(i);
-  atomicAdd(&counts[c], 1u);
+}
+
+// offsets[c] = first sorted position with key >= c, for c in [0, ncells].
+__global__ void k_cell_offsets(const uint64_t *skeys, uint64_t n, uint64_t ncells,
+                               uint64_t *offsets) {
+  uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c > ncells)
+    return;
+  uint64_t f = 0, cnt = n;
+  while (cnt) {
+    uint64_t h = cnt / 2, m = f + h;
+    if (skeys[m] < c) {
+      f = m + 1;
+      cnt -= h + 1;
+    } else
+      cnt = h;
+  }
+  offsets[c] = f;
 }
 
 template <class R>
@@ -127,18 +154,40 @@ __global__ void k_gather_sorted(const R *pts, uint64_t n, int D, const uint32_t 
     out[k * n + i] = pts[k * n + o];
 }
 
-// One wavefront per seed.
-// Seeds are either positions (seeds, D x ns SoA) or indices into the
-// unsorted point set (seed_idx, with raw = those points).
+// Seed start positions -> cell keys (to run seeds in cell order).
+template <class R, int D>
+__global__ void k_seed_keys(const R *seeds, const int32_t *seed_idx, const R *raw, uint64_t n,
+                            uint64_t ns, Grid g, uint64_t *keys, uint32_t *vals) {
+  uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= ns)
+    return;
+  R m[3];
+  for (int k = 0; k < 3; ++k)
+    m[k] = seed_idx ? raw[g.axis[k] * n + static_cast<uint64_t>(seed_idx[s])]
+                    : seeds[g.axis[k] * ns + s];
+  keys[s] = cell_key(g, m[0], m[1], m[2]);
+  vals[s] = static_cast<uint32_t>(s);
+}
+
+// One wavefront per seed.  Seeds are positions (seeds, D x ns SoA) or
+// indices into the unsorted point set (seed_idx with raw = those points);
+// `order` lists the seeds in cell order.
 template <class R, int D>
 __global__ void __launch_bounds__(kThreads)
     k_ms_query(const R *pts, uint64_t n, const uint64_t *offsets, Grid g, const R *seeds,
-               const int32_t *seed_idx, const R *raw, uint64_t ns, int iterations, R inv2h2,
-               R r2, R floor_w, R *out, R *weight, unsigned long long *stats) {
+               const int32_t *seed_idx, const R *raw, const uint32_t *order, uint64_t ns,
+               unsigned logical_blocks, int iterations, R inv2h2, R r2, R floor_w, R *out,
+               R *weight, unsigned long long *stats) {
   const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t s = static_cast<uint64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
-  if (s >= ns)
+  // XCD-aware block order: dies take contiguous runs of (cell-ordered) seeds
+  const unsigned per = (logical_blocks + kXcds - 1) / kXcds;
+  const unsigned logical = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  if (logical >= logical_blocks)
     return;
+  const uint64_t slot = static_cast<uint64_t>(logical) * kWavesPerBlock + threadIdx.x / kWave;
+  if (slot >= ns)
+    return;
+  const uint64_t s = order ? order[slot] : slot;
   R m[D];
   if (seed_idx) {
     const uint64_t v = static_cast<uint64_t>(seed_idx[s]);
@@ -155,9 +204,9 @@ __global__ void __launch_bounds__(kThreads)
     for (int k = 0; k < D; ++k)
       acc[k] = 0;
     R wl = 0;
-    const int cx = cell_of(m[0], g.lo[0], g.cell, g.dims[0]);
-    const int cy = cell_of(m[1], g.lo[1], g.cell, g.dims[1]);
-    const int cz = cell_of(m[2], g.lo[2], g.cell, g.dims[2]);
+    const int cx = cell_of(static_cast<double>(m[g.axis[0]]), g.lo[0], g.cell, g.dims[0]);
+    const int cy = cell_of(static_cast<double>(m[g.axis[1]]), g.lo[1], g.cell, g.dims[1]);
+    const int cz = cell_of(static_cast<double>(m[g.axis[2]]), g.lo[2], g.cell, g.dims[2]);
     const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.dims[0] ? cx + 1 : cx;
     for (int dz = -1; dz <= 1; ++dz) {
       int z = cz + dz;
@@ -225,8 +274,8 @@ __global__ void k_sum_u64(const unsigned long long *a, uint64_t n, unsigned long
 
 }  // namespace
 
-// Builds the grid and runs every seed; returns (candidates, in-support) counts
-// through stats_out when given.
+// Builds the grid and runs every seed; returns (candidates, in-support) visit
+// counts through stats_out when given.
 template <class R>
 static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns, const R *seeds,
                        const int32_t *seed_idx, int iterations, double h, double cutoff, R *out,
@@ -235,61 +284,72 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     raise(MVS_EINVAL, "batched mean-shift supports 4-D (reflection) and 6-D (rigid) points");
   if (!(h > 0) || !(cutoff > 0) || iterations < 0)
     raise(MVS_EINVAL, "invalid mean-shift parameters");
-  if (n > static_cast<uint64_t>(UINT32_MAX))
-    raise(MVS_EOVERFLOW, "too many points");
-  // per-seed statistics [0, ns) candidates, [ns, 2ns) in-support visits,
-  // then two sums
+  if (n > static_cast<uint64_t>(UINT32_MAX) || ns > static_cast<uint64_t>(UINT32_MAX))
+    raise(MVS_EOVERFLOW, "too many points or seeds");
+  // per-seed statistics [0, ns) candidates, [ns, 2ns) in-support visits, 2 sums
   unsigned long long *stats = scratch_as<unsigned long long>(s, kSlotMs7, 2 * ns + 2);
   if (ns == 0)
     return;
   Grid g{};
+  g.axis[0] = 0, g.axis[1] = 1, g.axis[2] = 2;
   R *sorted = scratch_as<R>(s, kSlotMs0, static_cast<size_t>(D) * (n ? n : 1));
   uint64_t *off = nullptr;
   if (n) {
     unsigned nb = blocks_for(n, kThreads);
     if (nb > 1024)
       nb = 1024;
-    double *partial = scratch_as<double>(s, kSlotMs1, 6 * nb);
-    MVS_LAUNCH(s, "ms_bbox", k_bbox<R>, nb, kThreads, 0, pts, n, partial);
-    double *hp = static_cast<double *>(host_stage(s, 6 * nb * sizeof(double)));
-    MVS_HIP(hipMemcpyAsync(hp, partial, 6 * nb * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    double *partial = scratch_as<double>(s, kSlotMs1, 2 * kMaxD * nb);
+    MVS_LAUNCH(s, "ms_bbox", k_bbox<R>, nb, kThreads, 0, pts, n, D, partial);
+    double *hp = static_cast<double *>(host_stage(s, 2 * kMaxD * nb * sizeof(double)));
+    MVS_HIP(hipMemcpyAsync(hp, partial, 2 * kMaxD * nb * sizeof(double), hipMemcpyDeviceToHost,
+                           s->stream));
     MVS_HIP(hipStreamSynchronize(s->stream));
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double lo[kMaxD], hi[kMaxD];
+    for (int k = 0; k < kMaxD; ++k)
+      lo[k] = INFINITY, hi[k] = -INFINITY;
     for (unsigned b = 0; b < nb; ++b)
-      for (int k = 0; k < 3; ++k) {
-        lo[k] = std::fmin(lo[k], hp[6 * b + k]);
-        hi[k] = std::fmax(hi[k], hp[6 * b + 3 + k]);
+      for (int k = 0; k < D; ++k) {
+        lo[k] = std::fmin(lo[k], hp[2 * kMaxD * b + k]);
+        hi[k] = std::fmax(hi[k], hp[2 * kMaxD * b + kMaxD + k]);
       }
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < D; ++k)
       if (!std::isfinite(lo[k]) || !std::isfinite(hi[k]))
         raise(MVS_EINVAL, "non-finite mean-shift points");
+    // the three widest coordinates carry the grid (ties: lower index first)
+    int idx[kMaxD];
+    for (int k = 0; k < D; ++k)
+      idx[k] = k;
+    std::stable_sort(idx, idx + D, [&](int a, int b) { return hi[a] - lo[a] > hi[b] - lo[b]; });
+    for (int k = 0; k < 3; ++k)
+      g.axis[k] = idx[k];
+    std::sort(g.axis, g.axis + 3);
     double cell = cutoff * h;
     for (;;) {
       uint64_t cells = 1;
       for (int k = 0; k < 3; ++k) {
-        g.dims[k] = static_cast<int>(std::floor((hi[k] - lo[k]) / cell)) + 1;
+        int a = g.axis[k];
+        g.dims[k] = static_cast<int>(std::floor((hi[a] - lo[a]) / cell)) + 1;
         cells *= static_cast<uint64_t>(g.dims[k]);
       }
-      if (cells <= (1ULL << 24))
+      if (cells <= (1ULL << 22))
         break;
       cell *= 2;
     }
     for (int k = 0; k < 3; ++k)
-      g.lo[k] = lo[k];
+      g.lo[k] = lo[g.axis[k]];
     g.cell = cell;
     g.ncells = static_cast<uint64_t>(g.dims[0]) * g.dims[1] * g.dims[2];
     uint64_t *keys = scratch_as<uint64_t>(s, kSlotMs2, 2 * n);
     uint32_t *vals = scratch_as<uint32_t>(s, kSlotMs3, 2 * n);
-    uint32_t *counts = scratch_as<uint32_t>(s, kSlotMs4, g.ncells);
     off = scratch_as<uint64_t>(s, kSlotMs5, g.ncells + 1);
-    zero_device(s, counts, g.ncells * sizeof(uint32_t));
     MVS_LAUNCH(s, "ms_cell_keys", k_cell_keys<R>, blocks_for(n, kThreads), kThreads, 0, pts, n, g,
-               keys, vals, counts);
+               keys, vals);
     int bits = 1;
     while ((1ULL << bits) < g.ncells)
       ++bits;
     radix_sort_pairs_u64(s, keys, vals, keys + n, vals + n, n, bits);
-    exclusive_scan_u32(s, counts, off, g.ncells);
+    MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, blocks_for(g.ncells + 1, kThreads), kThreads,
+               0, keys + n, n, g.ncells, off);
     MVS_LAUNCH(s, "ms_gather", k_gather_sorted<R>, blocks_for(n, kThreads), kThreads, 0, pts, n, D,
                vals + n, sorted);
   } else {
@@ -299,16 +359,34 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     g.cell = cutoff * h;
     g.ncells = 1;
   }
+  // seeds in cell order
+  uint64_t *skeys = scratch_as<uint64_t>(s, kSlotMs4, 2 * ns);
+  uint32_t *svals = scratch_as<uint32_t>(s, kSlotMs6, 2 * ns);
+  const uint32_t *order = nullptr;
+  if (n && ns > 1) {
+    if (D == 6)
+      MVS_LAUNCH(s, "ms_seed_keys", (k_seed_keys<R, 6>), blocks_for(ns, kThreads), kThreads, 0,
+                 seeds, seed_idx, pts, n, ns, g, skeys, svals);
+    else
+      MVS_LAUNCH(s, "ms_seed_keys", (k_seed_keys<R, 4>), blocks_for(ns, kThreads), kThreads, 0,
+                 seeds, seed_idx, pts, n, ns, g, skeys, svals);
+    int bits = 1;
+    while ((1ULL << bits) < g.ncells)
+      ++bits;
+    radix_sort_pairs_u64(s, skeys, svals, skeys + ns, svals + ns, ns, bits);
+    order = svals + ns;
+  }
   const R inv2h2 = static_cast<R>(1.0 / (2.0 * h * h));
   const R r2 = static_cast<R>(cutoff * h * cutoff * h);
   const R fw = static_cast<R>(std::exp(-0.5 * cutoff * cutoff));
-  unsigned grid = blocks_for(ns, kThreads / kWave);
+  const unsigned logical = blocks_for(ns, kWavesPerBlock);
+  const unsigned grid = ((logical + kXcds - 1) / kXcds) * kXcds;
   if (D == 6)
     MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 6>), grid, kThreads, 0, sorted, n, off, g, seeds,
-               seed_idx, pts, ns, iterations, inv2h2, r2, fw, out, weight, stats);
+               seed_idx, pts, order, ns, logical, iterations, inv2h2, r2, fw, out, weight, stats);
   else
     MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 4>), grid, kThreads, 0, sorted, n, off, g, seeds,
-               seed_idx, pts, ns, iterations, inv2h2, r2, fw, out, weight, stats);
+               seed_idx, pts, order, ns, logical, iterations, inv2h2, r2, fw, out, weight, stats);
   if (stats_out) {
     MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats, ns, stats + 2 * ns);
     MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats + ns, ns, stats + 2 * ns + 1);

@@ -16,14 +16,16 @@
 //               interval in one bucket of the kmin interval (both bounds in
 //               one loop), the wave concatenates the sub-ranges and tests
 //               their candidates with coalesced loads.  Accepted ordinals
-//               (b > a) are counted; up to 64 of them are put in CPU order
-//               by a 64-lane bitonic sort and parked in the source's slab;
+//               (b > a) are counted; up to 128 of them are put in CPU order
+//               (64-lane bitonic network, or ranks in LDS beyond 64) and
+//               parked in the source's slab;
 //   6. scan   : exclusive prefix of the counts -> output offsets; the total is
 //               read back once and checked against the budget before any
 //               output is written;
 //   7. place  : each slab is copied to its offset (coalesced); sources with
-//               more than 64 pairs walk the targets in ordinal order instead
-//               (k_pair_long), so order never depends on a capacity.
+//               more than 128 pairs walk their targets in ordinal order in
+//               4096-ordinal chunks (count, scan, fill), so order never
+//               depends on a capacity.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -37,7 +39,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;  // sources per block
-constexpr int kSlab = 64;                 // pairs parked per source by the walk
+constexpr int kSlab = 128;                // pairs parked per source by the walk
+constexpr uint64_t kLongChunk = 4096;     // ordinals per wave in the long path
 constexpr int kBucket = 64;               // samples per kmin bucket
 constexpr double kEps = 2.220446049250313e-16;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
@@ -437,11 +440,20 @@ __global__ void __launch_bounds__(kThreads)
   });
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  if (cnt <= static_cast<uint64_t>(kSlab)) {
+  if (cnt <= static_cast<uint64_t>(kWave)) {
     uint32_t v = static_cast<uint64_t>(lane) < cnt ? L[lane] : kNone;
     v = wave_sort_u32(v, lane);
     if (static_cast<uint64_t>(lane) < cnt)
       slab[i * kSlab + lane] = v;
+  } else if (cnt <= static_cast<uint64_t>(kSlab)) {
+    // rank = number of smaller ordinals (ordinals are distinct)
+    const uint32_t c = static_cast<uint32_t>(cnt);
+    for (uint32_t e = lane; e < c; e += kWave) {
+      uint32_t v = L[e], r = 0;
+      for (uint32_t k = 0; k < c; ++k)
+        r += L[k] < v;
+      slab[i * kSlab + r] = v;
+    }
   }
   rechecks = wave_sum(rechecks);
   if (lane == 0) {
@@ -474,25 +486,66 @@ __global__ void __launch_bounds__(kThreads)
   out_b[p] = sel ? sel[o] : static_cast<int32_t>(o);
 }
 
-// Sources with more than kSlab pairs: walk the targets in ordinal order.
+// Sources with more than kSlab pairs walk their targets in ordinal order,
+// split into chunks of kLongChunk ordinals (one wave per (source, chunk)):
+// a counting pass, a scan over the flattened (source, chunk) counts, and a
+// writing pass that places each chunk's pairs after the earlier chunks'.
+__device__ inline void long_range(uint64_t i, uint64_t c, uint64_t S, uint64_t &lo,
+                                  uint64_t &hi) {
+  lo = c * kLongChunk;
+  hi = lo + kLongChunk < S ? lo + kLongChunk : S;
+  if (lo < i + 1)
+    lo = i + 1;
+}
+
 __global__ void __launch_bounds__(kThreads)
-    k_pair_long(PairArgs A, const uint32_t *long_list, unsigned long_count,
-                const uint64_t *offsets, const int32_t *sel, int32_t *out_a, int32_t *out_b) {
+    k_long_count(PairArgs A, const uint32_t *long_list, uint64_t long_count, uint64_t chunks,
+                 uint32_t *chunk_counts) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t w = static_cast<uint64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
-  if (w >= long_count)
+  if (w >= long_count * chunks)
     return;
-  const uint64_t S = A.S;
-  const uint64_t i = long_list[w];
+  const uint64_t i = long_list[w / chunks], c = w % chunks;
+  uint64_t lo, hi;
+  long_range(i, c, A.S, lo, hi);
+  const double *ri = A.r64 + i * kRec;
+  const double a0 = ri[L_K0], a1 = ri[L_K1], ax = ri[L_X], ay = ri[L_Y], az = ri[L_Z];
+  uint32_t cnt = 0;
+  for (uint64_t b0 = lo; b0 < hi; b0 += kWave) {
+    uint64_t b = b0 + lane;
+    bool ok = false;
+    if (b < hi) {
+      const double *rb = A.r64 + b * kRec;
+      ok = accept64(a0, a1, rb[L_K0], rb[L_K1], ax, ay, az, rb[L_X], rb[L_Y], rb[L_Z], A.t,
+                    A.dist);
+    }
+    cnt += static_cast<uint32_t>(__popcll(ballot(ok)));
+  }
+  if (lane == 0)
+    chunk_counts[w] = cnt;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_long_fill(PairArgs A, const uint32_t *long_list, uint64_t long_count, uint64_t chunks,
+                const uint64_t *chunk_off, const uint64_t *offsets, const int32_t *sel,
+                int32_t *out_a, int32_t *out_b) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t w = static_cast<uint64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
+  if (w >= long_count * chunks)
+    return;
+  const uint64_t src_w = w / chunks, c = w % chunks;
+  const uint64_t i = long_list[src_w];
+  uint64_t lo, hi;
+  long_range(i, c, A.S, lo, hi);
   const double *ri = A.r64 + i * kRec;
   const double a0 = ri[L_K0], a1 = ri[L_K1], ax = ri[L_X], ay = ri[L_Y], az = ri[L_Z];
   const int32_t src = sel ? sel[i] : static_cast<int32_t>(i);
-  uint64_t cursor = offsets[i];
+  uint64_t cursor = offsets[i] + (chunk_off[w] - chunk_off[src_w * chunks]);
   const uint64_t end = offsets[i + 1];
-  for (uint64_t b0 = i + 1; b0 < S && cursor < end; b0 += kWave) {
+  for (uint64_t b0 = lo; b0 < hi; b0 += kWave) {
     uint64_t b = b0 + lane;
     bool ok = false;
-    if (b < S) {
+    if (b < hi) {
       const double *rb = A.r64 + b * kRec;
       ok = accept64(a0, a1, rb[L_K0], rb[L_K1], ax, ay, az, rb[L_X], rb[L_Y], rb[L_Z], A.t,
                     A.dist);
@@ -595,10 +648,10 @@ static PairState pair_state(Session *s, uint64_t S, bool guarded) {
   st.bk = scratch_as<double2>(s, kSlotPlanSortedK, Sp);
   st.bkf = guarded ? scratch_as<float2>(s, kSlotPlanSortedK32, Sp) : nullptr;
   st.bord = scratch_as<uint32_t>(s, kSlotWindows, Sp);
-  st.slab = scratch_as<uint32_t>(s, kSlotMisc2, kSlab * Sp);
+  st.slab = scratch_as<uint32_t>(s, kSlotPlanSlab, kSlab * Sp);
   st.keys = scratch_as<uint64_t>(s, kSlotPlanKeys, 2 * Sp);
   st.win = scratch_as<uint64_t>(s, kSlotPlanWin, 2 * Sp);
-  st.kw = scratch_as<double2>(s, kSlotMisc3, Sp);
+  st.kw = scratch_as<double2>(s, kSlotPlanKw, Sp);
   st.counts = scratch_as<uint64_t>(s, kSlotPlanCounts, Sp + 1);
   st.offsets = scratch_as<uint64_t>(s, kSlotPlanOffsets, Sp + 1);
   st.long_list = scratch_as<uint32_t>(s, kSlotPlanLong, Sp + 1);
@@ -721,9 +774,16 @@ void pair_fill_device(Session *s, int32_t *a, int32_t *b, bool ordinals) {
   const int32_t *sel = ordinals ? nullptr : plan.selected;
   MVS_LAUNCH(s, "pair_place", k_pair_place, blocks_for(S, kWaves), kThreads, 0, S, st.counts,
              st.offsets, st.slab, sel, a, b);
-  if (plan.long_sources)
-    MVS_LAUNCH(s, "pair_long", k_pair_long, blocks_for(plan.long_sources, kWaves), kThreads, 0, A,
-               st.long_list, static_cast<unsigned>(plan.long_sources), st.offsets, sel, a, b);
+  if (plan.long_sources) {
+    const uint64_t L = plan.long_sources, chunks = (S + kLongChunk - 1) / kLongChunk;
+    uint32_t *cc = scratch_as<uint32_t>(s, kSlotPlanLongCnt, L * chunks);
+    uint64_t *co = scratch_as<uint64_t>(s, kSlotPlanLongOff, L * chunks + 1);
+    unsigned grid = blocks_for(L * chunks, kWaves);
+    MVS_LAUNCH(s, "pair_long", k_long_count, grid, kThreads, 0, A, st.long_list, L, chunks, cc);
+    exclusive_scan_u32(s, cc, co, L * chunks);
+    MVS_LAUNCH(s, "pair_long", k_long_fill, grid, kThreads, 0, A, st.long_list, L, chunks, co,
+               st.offsets, sel, a, b);
+  }
 }
 
 void pair_statistics(Session *s, uint64_t out[3]) {
