@@ -38,8 +38,7 @@ static void pairvote(Session *s, uint64_t n, const double *x, const double *y, c
   uint64_t *rtot = reinterpret_cast<uint64_t *>(plan_flags(s)) + kVoteTotalWord;
   vote_offsets_device(s, P, ab, ab + Pp, precision, rtot);
   vote_emit_strided(s, P, ab, ab + Pp, center, precision, Pp, q4, t3, rp, n3, d1, fp);
-  uint64_t R = 0;
-  read_back(s, &R, rtot, sizeof R);
+  const uint64_t R = vote_total_checked(s);
   if (2 * P + R > max_votes || 2 * P + R > static_cast<uint64_t>(INT32_MAX))
     raise(MVS_EBUDGET, "Vote budget exceeded; lower --sample-ratio or tighten --pair-threshold");
   mvs_pairvote_result r{};

@@ -478,12 +478,15 @@ __global__ void __launch_bounds__(kThreads)
   if (i >= S)
     return;
   const uint64_t c = counts[i];
-  if (c > static_cast<uint64_t>(kSlab) || static_cast<uint64_t>(lane) >= c)
-    return;
-  const uint32_t o = slab[i * kSlab + lane];
-  const uint64_t p = offsets[i] + lane;
-  out_a[p] = sel ? sel[i] : static_cast<int32_t>(i);
-  out_b[p] = sel ? sel[o] : static_cast<int32_t>(o);
+  if (c > static_cast<uint64_t>(kSlab))
+    return;  // written by the long path
+  // every parked pair, also beyond the first 64 (slabs hold up to kSlab)
+  for (uint64_t e = lane; e < c; e += kWave) {
+    const uint32_t o = slab[i * kSlab + e];
+    const uint64_t p = offsets[i] + e;
+    out_a[p] = sel ? sel[i] : static_cast<int32_t>(i);
+    out_b[p] = sel ? sel[o] : static_cast<int32_t>(o);
+  }
 }
 
 // Sources with more than kSlab pairs walk their targets in ordinal order,

@@ -17,10 +17,12 @@ constexpr int kRec = 16;
 // Layout of the small flags slot shared by the pairing and vote phases
 // (kFlagsBytes bytes): 32-bit words 0 error flags, 1 long-source count,
 // 2-3 pair total (uint64), kRmaxWord the FP32 coordinate bound (float bits);
-// 64-bit word kVoteTotalWord the mirror-vote total.
+// 64-bit word kVoteTotalWord the mirror-vote total, 32-bit word kVoteBadWord
+// (inside 64-bit word kVoteTotalWord + 2) the out-of-range ordinal flag.
 constexpr size_t kFlagsBytes = 256;
 constexpr int kRmaxWord = 8;
 constexpr int kVoteTotalWord = 8;
+constexpr int kVoteBadWord = 2 * (kVoteTotalWord + 2);
 inline unsigned *plan_flags(Session *s) {
   return static_cast<unsigned *>(scratch(s, kSlotFlags, kFlagsBytes));
 }
@@ -51,6 +53,9 @@ void vote_offsets_device(Session *s, uint64_t P, const int32_t *a, const int32_t
 void vote_emit_strided(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
                        const double *center, int precision, uint64_t stride, void *q4, void *t3,
                        int32_t *rp, void *n3, void *d1, int32_t *fp);
+// Synchronous read of the mirror total left by vote_offsets_device; raises
+// MVS_EINVAL when the count pass met a pair ordinal outside [0, S).
+uint64_t vote_total_checked(Session *s);
 
 // Batched mean-shift (msbatch.hip).
 void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,

@@ -123,7 +123,13 @@ struct VoteArgs {
   float g0, g1;       // band = g0 + g1 * (largest FP32 coordinate)
   const unsigned *rmax_bits;
   uint32_t *wave_rr;  // per wave: FP64 rechecks of the mirror tests
+  unsigned *bad;      // set when a pair ordinal is outside [0, S)
 };
+// Both ordinals of pair k index the sample records.
+__device__ inline bool pair_in_range(const int32_t *a, const int32_t *b, uint64_t k, uint64_t S) {
+  int32_t i = a[k], o = b[k];
+  return i >= 0 && o >= 0 && static_cast<uint64_t>(i) < S && static_cast<uint64_t>(o) < S;
+}
 template <bool kGuarded> __device__ inline VoteArgs with_bands(const VoteArgs &in) {
   VoteArgs A = in;
   if (kGuarded) {
@@ -171,7 +177,10 @@ __global__ void __launch_bounds__(kThreads) k_vote_count(VoteArgs Ain, uint32_t 
     return;
   const VoteArgs A = with_bands<kGuarded>(Ain);
   unsigned rr = 0;
-  bool refl = k < A.P && pair_reflection<kGuarded>(A, k, rr);
+  const bool in = k < A.P && pair_in_range(A.a, A.b, k, A.S);
+  if (k < A.P && !in)
+    atomicOr(A.bad, 1u);
+  bool refl = in && pair_reflection<kGuarded>(A, k, rr);
   uint64_t m = ballot(refl);
   rr = wave_sum(rr);
   if (lane == 0) {
@@ -205,7 +214,8 @@ __global__ void __launch_bounds__(kThreads)
     return;
   const VoteArgs A = with_bands<kGuarded>(Ain);
   const uint64_t S = A.S, P2 = 2 * A.P;
-  const bool live = k < A.P;
+  // out-of-range pairs were flagged by the count pass and are skipped here
+  const bool live = k < A.P && pair_in_range(A.a, A.b, k, A.S);
   unsigned rr = 0;
   bool refl = live && pair_reflection<kGuarded>(A, k, rr);
   const uint64_t slot = wave_off[w] + lanes_below(ballot(refl));
@@ -402,6 +412,7 @@ static VoteArgs vote_args(Session *s, uint64_t P, const int32_t *a, const int32_
   A.rmax_bits = plan_flags(s) + kRmaxWord;
   const uint64_t nw = (P + kWave - 1) / kWave;
   A.wave_rr = scratch_as<uint32_t>(s, kSlotRefl, nw ? nw : 1);
+  A.bad = plan_flags(s) + kVoteBadWord;
   return A;
 }
 
@@ -414,6 +425,7 @@ void vote_offsets_device(Session *s, uint64_t P, const int32_t *a, const int32_t
   const uint64_t nw = (P + kWave - 1) / kWave;
   uint32_t *wr = scratch_as<uint32_t>(s, kSlotVoteBlock, nw ? nw : 1);
   uint64_t *off = scratch_as<uint64_t>(s, kSlotVoteBlockOff, nw + 1);
+  zero_device(s, A.bad, sizeof(unsigned));
   if (P) {
     unsigned grid = blocks_for(nw, kThreads / kWave);
     if (guarded)
@@ -446,12 +458,20 @@ void vote_emit_strided(Session *s, uint64_t P, const int32_t *a, const int32_t *
   }
 }
 
+uint64_t vote_total_checked(Session *s) {
+  // words kVoteTotalWord .. kVoteTotalWord + 2: total, (unused), bad flag
+  uint64_t w[3];
+  read_back(s, w, reinterpret_cast<uint64_t *>(plan_flags(s)) + kVoteTotalWord, sizeof w);
+  if (static_cast<uint32_t>(w[2]))
+    raise(MVS_EINVAL, "pair ordinal outside the sample set");
+  return w[0];
+}
+
 void vote_count_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b, int precision,
                        uint64_t *R_out) {
   uint64_t *tot = reinterpret_cast<uint64_t *>(plan_flags(s)) + kVoteTotalWord;
   vote_offsets_device(s, P, a, b, precision, tot);
-  uint64_t R;
-  read_back(s, &R, tot, sizeof R);
+  uint64_t R = vote_total_checked(s);
   s->vote_plan.P = P;
   s->vote_plan.R = R;
   s->vote_plan.precision = precision;

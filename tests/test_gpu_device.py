@@ -37,12 +37,21 @@ def _oracle_votes(oracle, wl, pa, pb):
     return oracle.transformations(v, wl.feat_np, pa, pb, wl.box, wl.center, 10**9)
 
 
-@pytest.mark.parametrize("t", [0.05, 0.15])
+# Per-source pair counts on this workload: t = .05 keeps every source at
+# most 15 partners (slab, one pass of the bitonic sort), t = .15 has sources
+# with 65..72 partners (slab, LDS rank sort), t = .3 has sources beyond 128
+# partners (chunked long path).
+@pytest.mark.parametrize("t", [0.05, 0.15, 0.3])
 def test_compact_votes_match_oracle(cfg1, oracle, t):
     wl = cfg1
     wl.threshold = t
     v = wl.mesh.vertices
     pa, pb = oracle.pairing(v, wl.feat_np[9], wl.feat_np[10], wl.selected_np, wl.box, t, 10**8)
+    _, per_source = np.unique(pa, return_counts=True)
+    if t >= 0.15:
+        assert np.any((per_source > 64) & (per_source <= 128))
+    if t >= 0.3:
+        assert np.any(per_source > 128)
     pair, kind, lanes = _oracle_votes(oracle, wl, pa, pb)
     ref = {}
     for prec in (capi.FP64, capi.FP32_GUARDED):
