@@ -30,8 +30,21 @@ FP64_PEAK_TFLOPS = 78.6
 SEEDS, ITERS, H, CUT = 20000, 30, 0.05, 3.0
 KERNELS = ("pair_gather", "radix_hist", "radix_scatter", "scan_reduce", "scan_apply", "scan_zero",
            "pair_index", "pair_windows", "pair_walk", "pair_place", "pair_long", "vote_count",
-           "vote_emit", "read_total", "zero", "ms_points", "ms_bbox", "ms_cell_keys", "ms_gather",
-           "ms_query")
+           "vote_emit", "read_total", "zero", "ms_seeds", "ms_points", "ms_bbox", "ms_grid",
+           "ms_cell_keys", "ms_cell_offsets", "ms_gather", "ms_seed_keys", "ms_query", "ms_stats")
+# kernels timed with HIP events inside the timed region (the roofline kernels)
+TIMED = ("pair_walk", "vote_emit", "ms_query")
+
+
+def load_traffic():
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or {}."""
+    path = os.path.join(ROOT, "profiles", "traffic_cfg2.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("kernels", {})
+    except (OSError, ValueError):
+        return {}
 
 
 def emission_bytes(P, R, S, s):
@@ -83,6 +96,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 256 MiB copy kernel (torch.neg) three times, the "
+                         "reference for rocprofv3 FETCH_SIZE / WRITE_SIZE scaling")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the voting core runs on a single MI355X "
@@ -91,6 +107,13 @@ def main():
     import torch
     from mitra_algo_amd import capi, workload
 
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     t_setup = time.perf_counter()
     wl = workload.cfg2(n=args.grid)
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s S={wl.S}", file=sys.stderr,
@@ -100,8 +123,8 @@ def main():
         for _ in range(warmup):
             wl.step(precision, SEEDS, ITERS)
         torch.cuda.synchronize()
-        # timed region: K full steps, kernels timed with HIP events
-        wl.sess.enable_timing(True)
+        # timed region: K full steps; the roofline kernels carry HIP events
+        wl.sess.enable_timing(True, only=TIMED)
         t_pv = t_ms = 0.0
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -115,21 +138,32 @@ def main():
             t_ms += c - b
         dt = (time.perf_counter() - t0) / steps
         kernels = {}
-        for k in KERNELS:
+        for k in TIMED:
             ms_, n = wl.sess.timing(k)
             if n:
                 kernels[k] = {"ms_total_per_step": ms_ / steps, "ms_avg": ms_ / n,
                               "launches_per_step": n / steps}
+        # untimed breakdown of every kernel (events around each launch)
+        wl.sess.enable_timing(True)
+        nb = 2
+        for _ in range(nb):
+            wl.step(precision, SEEDS, ITERS)
+        breakdown = {}
+        for k in KERNELS:
+            ms_, n = wl.sess.timing(k)
+            if n:
+                breakdown[k] = {"ms_per_step": ms_ / nb, "ms_avg": ms_ / n,
+                                "launches_per_step": n / nb}
         wl.sess.enable_timing(False)
         stats = wl.meanshift(SEEDS, ITERS, want_stats=True)
-        return pv, stats, dt, t_pv / steps, t_ms / steps, kernels
+        return pv, stats, dt, t_pv / steps, t_ms / steps, kernels, breakdown
 
-    pv32, ms32, dt32, pv32_s, ms32_s, k32 = run(capi.FP32_GUARDED, args.steps, args.warmup)
+    pv32, ms32, dt32, pv32_s, ms32_s, k32, b32 = run(capi.FP32_GUARDED, args.steps, args.warmup)
     pair32 = (wl.sess.fetch(pv32.a, (pv32.pairs,), torch.int32).cpu(),
               wl.sess.fetch(pv32.b, (pv32.pairs,), torch.int32).cpu(),
               wl.sess.fetch(pv32.refl_pair, (pv32.reflections,), torch.int32).cpu())
     st32 = wl.sess.dev_pair_stats()
-    pv64, ms64, dt64, pv64_s, ms64_s, k64 = run(capi.FP64, max(3, args.steps // 2), 1)
+    pv64, ms64, dt64, pv64_s, ms64_s, k64, b64 = run(capi.FP64, max(3, args.steps // 2), 1)
     pair64 = (wl.sess.fetch(pv64.a, (pv64.pairs,), torch.int32).cpu(),
               wl.sess.fetch(pv64.b, (pv64.pairs,), torch.int32).cpu(),
               wl.sess.fetch(pv64.refl_pair, (pv64.reflections,), torch.int32).cpu())
@@ -148,14 +182,17 @@ def main():
     q_tflops = meanshift_flops(ms32.candidates, ms32.in_support) / (q_ms * 1e-3) / 1e12
     q64_ms = k64["ms_query"]["ms_total_per_step"]
     q64_tflops = meanshift_flops(ms64.candidates, ms64.in_support) / (q64_ms * 1e-3) / 1e12
+    traffic = load_traffic()
     rooflines = {
         "vote_emit": {"bound": "hbm", "achieved": emit_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                      "frac": emit_gbs / HBM_PEAK_GBS, "traffic": None,
+                      "frac": emit_gbs / HBM_PEAK_GBS,
+                      "traffic": traffic.get("vote_emit", {}).get("bytes"),
                       "algorithmic_bytes": emission_bytes(P, R, wl.S, 4),
                       "ms_avg": emit_ms},
         "ms_query": {"bound": "mfma", "pipe": "VALU FP32 (no MFMA: no dense tile)",
                      "achieved": q_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": q_tflops / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": q_tflops / FP32_PEAK_TFLOPS,
+                     "traffic": traffic.get("ms_query", {}).get("bytes"),
                      "algorithmic_flops": meanshift_flops(ms32.candidates, ms32.in_support),
                      "ms_per_step": q_ms},
     }
@@ -192,6 +229,8 @@ def main():
         "pair_stats_fp32g": st32,
         "kernels_fp32g": k32,
         "kernels_fp64": k64,
+        "breakdown_fp32g": b32,
+        "breakdown_fp64": b64,
         "roofline": roof,
         "rooflines": rooflines,
     }

@@ -143,6 +143,8 @@ int mvs_dev_copy(mvs_session *s, void *dst, const void *src, uint64_t bytes);
 /* Device time (ms) of the last mvs_dev_* launch group, measured with HIP
  * events on the session stream; see mvs_dev_timing for per-kernel times. */
 int mvs_session_enable_timing(mvs_session *s, int enable);
+/* Restricts timing to the comma-separated kernel names (NULL or "" = all). */
+int mvs_session_timing_filter(mvs_session *s, const char *names);
 int mvs_dev_timing(mvs_session *s, const char *kernel, double *ms, uint64_t *launches);
 
 /* ------------------------------------------------------------ host layer */
@@ -264,11 +266,14 @@ int mvs_dev_pairvote(mvs_session *s, uint64_t n, const double *x, const double *
                      mvs_pairvote_result *out);
 
 /* Batched mean-shift over the votes of the last mvs_dev_pairvote:
- * `seeds` seed votes drawn without replacement (std::mt19937_64(seed_value),
- * split between rigid and mirror votes in proportion to their counts), each
+ * `seeds` seed votes split between rigid and mirror votes in proportion to
+ * their counts; within each stream of n votes, seed j of k is drawn from the
+ * j-th of k equal strata [j n / k, (j + 1) n / k) by a splitmix64 hash of
+ * (seed_value, j), so seeds are distinct and ascending.  Each seed is
  * iterated `iterations` times in its own point set (see mvs_dev_ms_batched).
- * Outputs in session-owned buffers; want_stats != 0 also returns the visit
- * counts (one extra synchronisation). */
+ * Grid, seeds and queries are built on the device; the call synchronises
+ * once at the end (non-finite points raise MVS_EINVAL).  Outputs live in
+ * session-owned buffers; want_stats != 0 also returns the visit counts. */
 typedef struct {
   uint64_t rigid_seeds, refl_seeds;
   const int32_t *seed_votes;          /* rigid seeds then mirror seeds (stream indices) */

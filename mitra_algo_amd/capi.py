@@ -19,7 +19,8 @@ FP64, FP32_GUARDED = 0, 1
 EXPORTS = [
     "mvs_prepare", "mvs_mt64_seed", "mvs_mt64_next", "mvs_sample_vertices",
     "mvs_session_create", "mvs_session_destroy", "mvs_last_error", "mvs_session_set_stream",
-    "mvs_session_sync", "mvs_session_enable_timing", "mvs_dev_timing",
+    "mvs_session_sync", "mvs_session_enable_timing", "mvs_session_timing_filter",
+    "mvs_dev_timing",
     "mvs_curvature", "mvs_prune", "mvs_pairing", "mvs_pairing_fetch", "mvs_transformations",
     "mvs_transformations_fetch", "mvs_clustering", "mvs_clustering_fetch",
     "mvs_dev_curvature", "mvs_dev_pair_count", "mvs_dev_pair_fill", "mvs_dev_pair_stats",
@@ -240,8 +241,11 @@ class Session:
     def sync(self):
         check(lib().mvs_session_sync(self.h))
 
-    def enable_timing(self, on: bool = True):
+    def enable_timing(self, on: bool = True, only=None):
+        """Per-launch HIP-event timing; `only` restricts it to these kernel names."""
         check(lib().mvs_session_enable_timing(self.h, C.c_int(1 if on else 0)))
+        names = ",".join(only) if only else ""
+        check(lib().mvs_session_timing_filter(self.h, names.encode()))
 
     def timing(self, kernel: str):
         ms = C.c_double()

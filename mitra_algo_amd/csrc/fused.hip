@@ -9,12 +9,19 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
-#include <unordered_set>
 #include <vector>
 
 #include "pairing.hpp"
 
 namespace mvs {
+
+// Host twin of the device mixer below (the seed salt).
+static uint64_t mix64_host(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
 
 static void pairvote(Session *s, uint64_t n, const double *x, const double *y, const double *z,
                      const double *feat, uint64_t S, const int32_t *sel, double box,
@@ -63,25 +70,23 @@ static void pairvote(Session *s, uint64_t n, const double *x, const double *y, c
   *out = r;
 }
 
-// k distinct indices of [0, n), ascending (Floyd's algorithm on mt19937_64).
-static std::vector<int32_t> sample_distinct(uint64_t n, uint64_t k, mvs_mt64 *g) {
-  std::vector<int32_t> out;
-  if (k >= n) {
-    out.resize(n);
-    for (uint64_t i = 0; i < n; ++i)
-      out[i] = static_cast<int32_t>(i);
-    return out;
-  }
-  std::unordered_set<uint64_t> chosen;
-  chosen.reserve(2 * k);
-  for (uint64_t j = n - k; j < n; ++j) {
-    uint64_t t = mt64_next(g) % (j + 1);
-    if (!chosen.insert(t).second)
-      chosen.insert(j);
-  }
-  out.assign(chosen.begin(), chosen.end());
-  std::sort(out.begin(), out.end());
-  return out;
+// splitmix64 output function (Steele, Lea and Flood, "Fast splittable
+// pseudorandom number generators", OOPSLA 2014).
+__device__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+// k distinct indices of [0, n) (k <= n), ascending: index j is drawn
+// uniformly from the j-th of k equal strata [j n / k, (j + 1) n / k).
+__global__ void k_pick_seeds(uint64_t n, uint64_t k, uint64_t salt, int32_t *out) {
+  const uint64_t j = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= k)
+    return;
+  const uint64_t lo = j * n / k, hi = (j + 1) * n / k;
+  out[j] = static_cast<int32_t>(lo + mix64(salt ^ mix64(j)) % (hi - lo));
 }
 
 static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, int iterations,
@@ -94,70 +99,43 @@ static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, in
   const size_t sz = prec == MVS_FP32_GUARDED ? 4 : 8;
   uint64_t kf = V ? static_cast<uint64_t>(std::llround(static_cast<double>(nseeds) * nf / V)) : 0;
   kf = std::min(kf, nf);
-  uint64_t kr = std::min(nseeds > kf ? nseeds - kf : 0, nr);
-  mvs_mt64 g;
-  g.x[0] = seed_value;
-  for (uint64_t i = 1; i < 312; ++i)
-    g.x[i] = 6364136223846793005ULL * (g.x[i - 1] ^ (g.x[i - 1] >> 62)) + i;
-  g.p = 312;
-  std::vector<int32_t> seeds = sample_distinct(nr, kr, &g);
-  std::vector<int32_t> fseeds = sample_distinct(nf, kf, &g);
-  seeds.insert(seeds.end(), fseeds.begin(), fseeds.end());
-  int32_t *dseeds = scratch_as<int32_t>(s, kSlotOutMsSeeds, seeds.size() ? seeds.size() : 1);
-  if (!seeds.empty()) {  // through pinned staging: the copy stays stream-ordered
-    void *hs = host_stage(s, seeds.size() * sizeof(int32_t));
-    std::memcpy(hs, seeds.data(), seeds.size() * sizeof(int32_t));
-    MVS_HIP(hipMemcpyAsync(dseeds, hs, seeds.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                           s->stream));
-  }
-  // Rigid points are built from the rigid stream (stride 2P); mirror points
-  // need the stream compacted to its true length R first when the stride
-  // (capacity) exceeds it, which the point builder does by reading only R.
+  const uint64_t kr = std::min(nseeds > kf ? nseeds - kf : 0, nr);
+  int32_t *dseeds = scratch_as<int32_t>(s, kSlotOutMsSeeds, kr + kf ? kr + kf : 1);
+  if (kr)
+    MVS_LAUNCH(s, "ms_seeds", k_pick_seeds, blocks_for(kr, 256), 256, 0, nr, kr,
+               mix64_host(seed_value), dseeds);
+  if (kf)
+    MVS_LAUNCH(s, "ms_seeds", k_pick_seeds, blocks_for(kf, 256), 256, 0, nf, kf,
+               mix64_host(seed_value ^ 0x6d6972726f72ULL), dseeds + kr);
   void *rpts = scratch(s, kSlotOutMsRigidPts, 6 * (nr ? nr : 1) * sz);
   void *fpts = scratch(s, kSlotOutMsReflPts, 4 * (nf ? nf : 1) * sz);
-  ms_points_device(s, prec, nr, r.rigid_q4, r.rigid_t3, 0, nullptr, nullptr, rpts, nullptr);
-  if (nf) {
-    // mirror stream has SoA stride refl_stride; build points lane by lane
-    if (prec == MVS_FP32_GUARDED) {
-      const float *n3 = static_cast<const float *>(r.refl_n3);
-      float *tmp = scratch_as<float>(s, kSlotMs6, 3 * nf);
-      for (int k = 0; k < 3; ++k)
-        MVS_HIP(hipMemcpyAsync(tmp + k * nf, n3 + k * r.refl_stride, nf * 4,
-                               hipMemcpyDeviceToDevice, s->stream));
-      ms_points_device(s, prec, 0, nullptr, nullptr, nf, tmp, r.refl_d, nullptr, fpts);
-    } else {
-      const double *n3 = static_cast<const double *>(r.refl_n3);
-      double *tmp = scratch_as<double>(s, kSlotMs6, 3 * nf);
-      for (int k = 0; k < 3; ++k)
-        MVS_HIP(hipMemcpyAsync(tmp + k * nf, n3 + k * r.refl_stride, nf * 8,
-                               hipMemcpyDeviceToDevice, s->stream));
-      ms_points_device(s, prec, 0, nullptr, nullptr, nf, tmp, r.refl_d, nullptr, fpts);
-    }
-  }
   void *orig = scratch(s, kSlotOutMsRigid, 6 * (kr ? kr : 1) * sz);
   void *owr = scratch(s, kSlotOutMsRigidW, (kr ? kr : 1) * sz);
   void *ofl = scratch(s, kSlotOutMsRefl, 4 * (kf ? kf : 1) * sz);
   void *owf = scratch(s, kSlotOutMsReflW, (kf ? kf : 1) * sz);
-  unsigned long long s1[2] = {0, 0}, s2[2] = {0, 0};
+  ms_points_device(s, prec, kr ? nr : 0, r.rigid_q4, r.rigid_t3, kf ? nf : 0, r.refl_n3,
+                   r.refl_stride, r.refl_d, rpts, fpts);
+  ms_begin(s);
+  const int slot_r = want_stats ? 0 : -1, slot_f = want_stats ? 2 : -1;
   if (prec == MVS_FP32_GUARDED) {
     if (kr)
       ms_batched_f32(s, 6, nr, static_cast<const float *>(rpts), kr, nullptr, dseeds, iterations,
-                     h, cutoff, static_cast<float *>(orig), static_cast<float *>(owr),
-                     want_stats ? s1 : nullptr);
+                     h, cutoff, static_cast<float *>(orig), static_cast<float *>(owr), slot_r);
     if (kf)
       ms_batched_f32(s, 4, nf, static_cast<const float *>(fpts), kf, nullptr, dseeds + kr,
                      iterations, h, cutoff, static_cast<float *>(ofl), static_cast<float *>(owf),
-                     want_stats ? s2 : nullptr);
+                     slot_f);
   } else {
     if (kr)
       ms_batched_f64(s, 6, nr, static_cast<const double *>(rpts), kr, nullptr, dseeds, iterations,
-                     h, cutoff, static_cast<double *>(orig), static_cast<double *>(owr),
-                     want_stats ? s1 : nullptr);
+                     h, cutoff, static_cast<double *>(orig), static_cast<double *>(owr), slot_r);
     if (kf)
       ms_batched_f64(s, 4, nf, static_cast<const double *>(fpts), kf, nullptr, dseeds + kr,
                      iterations, h, cutoff, static_cast<double *>(ofl),
-                     static_cast<double *>(owf), want_stats ? s2 : nullptr);
+                     static_cast<double *>(owf), slot_f);
   }
+  unsigned long long sums[4] = {0, 0, 0, 0};
+  ms_finish(s, sums);
   mvs_meanshift_result m{};
   m.rigid_seeds = kr;
   m.refl_seeds = kf;
@@ -166,8 +144,8 @@ static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, in
   m.rigid_weight = owr;
   m.refl_modes = ofl;
   m.refl_weight = owf;
-  m.candidates = s1[0] + s2[0];
-  m.in_support = s1[1] + s2[1];
+  m.candidates = sums[0] + sums[2];
+  m.in_support = sums[1] + sums[3];
   m.precision = prec;
   *out = m;
 }

@@ -56,15 +56,16 @@ __global__ void k_points_rigid(const R *q4, const R *t3, uint64_t n, R *pts) {
   pts[5 * n + i] = f * z;
 }
 
+// Mirror normals come as 3 rows of stride `stride` (the emission capacity).
 template <class R>
-__global__ void k_points_refl(const R *n3, const R *d1, uint64_t n, R *pts) {
+__global__ void k_points_refl(const R *n3, uint64_t stride, const R *d1, uint64_t n, R *pts) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
   pts[i] = d1[i];
   pts[n + i] = n3[i];
-  pts[2 * n + i] = n3[n + i];
-  pts[3 * n + i] = n3[2 * n + i];
+  pts[2 * n + i] = n3[stride + i];
+  pts[3 * n + i] = n3[2 * stride + i];
 }
 
 // per-block min/max of every coordinate: partial[b*2*kMaxD + k] (lo), + kMaxD (hi)
@@ -105,10 +106,99 @@ struct Grid {
   int dims[3];
   uint64_t ncells;
 };
+constexpr int kCellBits = 22;  // at most 2^22 cells: three 8-bit radix passes
+constexpr uint64_t kMaxCells = 1ULL << kCellBits;
+constexpr unsigned kBboxBlocks = 1024;
 
+// Device-side control block of one mean-shift call: the grid, visit sums of
+// up to two runs and the non-finite-input flag, read back once at the end.
+struct MsCtl {
+  Grid grid;
+  unsigned long long sums[4];
+  unsigned bad;
+};
+
+// One wavefront: reduces the per-block boxes and lays the grid on the three
+// widest coordinates (ties: lower index first), cells of side >= cell0
+// doubled until at most kMaxCells.  No points: a single cell.
+__global__ void k_grid_setup(const double *partial, unsigned nb, int D, double cell0, MsCtl *ctl) {
+  const int lane = threadIdx.x;
+  double lo[kMaxD], hi[kMaxD];
+  for (int k = 0; k < kMaxD; ++k)
+    lo[k] = INFINITY, hi[k] = -INFINITY;
+  for (unsigned b = lane; b < nb; b += kWave)
+    for (int k = 0; k < D; ++k) {
+      lo[k] = fmin(lo[k], partial[b * 2 * kMaxD + k]);
+      hi[k] = fmax(hi[k], partial[b * 2 * kMaxD + kMaxD + k]);
+    }
+  for (int k = 0; k < D; ++k)
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+      lo[k] = fmin(lo[k], __shfl_xor(lo[k], o, kWave));
+      hi[k] = fmax(hi[k], __shfl_xor(hi[k], o, kWave));
+    }
+  if (lane != 0)
+    return;
+  Grid g{};
+  g.axis[0] = 0, g.axis[1] = 1, g.axis[2] = 2;
+  g.cell = cell0;
+  g.dims[0] = g.dims[1] = g.dims[2] = 1;
+  g.ncells = 1;
+  bool finite = true;
+  for (int k = 0; k < D; ++k)
+    finite = finite && isfinite(lo[k]) && isfinite(hi[k]);
+  if (nb && !finite)
+    atomicOr(&ctl->bad, 1u);
+  if (nb && finite) {
+    // stable selection of the three widest extents
+    int idx[kMaxD];
+    for (int k = 0; k < D; ++k)
+      idx[k] = k;
+    for (int k = 1; k < D; ++k) {
+      int v = idx[k], j = k;
+      while (j > 0 && hi[idx[j - 1]] - lo[idx[j - 1]] < hi[v] - lo[v]) {
+        idx[j] = idx[j - 1];
+        --j;
+      }
+      idx[j] = v;
+    }
+    int ax[3] = {idx[0], idx[1], idx[2]};
+    for (int k = 1; k < 3; ++k)
+      for (int j = k; j > 0 && ax[j - 1] > ax[j]; --j) {
+        int t = ax[j];
+        ax[j] = ax[j - 1];
+        ax[j - 1] = t;
+      }
+    double cell = cell0;
+    for (;;) {
+      uint64_t cells = 1;
+      for (int k = 0; k < 3; ++k) {
+        double d = floor((hi[ax[k]] - lo[ax[k]]) / cell) + 1;
+        g.dims[k] = d < 2147483647.0 ? static_cast<int>(d) : 2147483647;
+        cells = cells * static_cast<uint64_t>(g.dims[k]);
+        if (cells > kMaxCells)
+          break;
+      }
+      if (cells <= kMaxCells) {
+        g.ncells = cells;
+        break;
+      }
+      cell *= 2;
+    }
+    for (int k = 0; k < 3; ++k) {
+      g.axis[k] = ax[k];
+      g.lo[k] = lo[ax[k]];
+    }
+    g.cell = cell;
+  }
+  ctl->grid = g;
+}
+
+// Cell index along one axis; NaN and underflow go to cell 0.
 __device__ inline int cell_of(double v, double lo, double cell, int dim) {
   double c = floor((v - lo) / cell);
-  return c < 0 ? 0 : (c >= dim ? dim - 1 : static_cast<int>(c));
+  if (!(c > 0))
+    return 0;
+  return c >= dim ? dim - 1 : static_cast<int>(c);
 }
 template <class R> __device__ inline uint64_t cell_key(const Grid &g, R a, R b, R c) {
   int x = cell_of(static_cast<double>(a), g.lo[0], g.cell, g.dims[0]);
@@ -118,30 +208,34 @@ template <class R> __device__ inline uint64_t cell_key(const Grid &g, R a, R b, 
 }
 
 template <class R>
-__global__ void k_cell_keys(const R *pts, uint64_t n, Grid g, uint64_t *keys, uint32_t *vals) {
+__global__ void k_cell_keys(const R *pts, uint64_t n, const MsCtl *ctl, uint64_t *keys,
+                            uint32_t *vals) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
+  const Grid g = ctl->grid;
   keys[i] = cell_key(g, pts[g.axis[0] * n + i], pts[g.axis[1] * n + i], pts[g.axis[2] * n + i]);
   vals[i] = static_cast<uint32_t>(i);
 }
 
-// offsets[c] = first sorted position with key >= c, for c in [0, ncells].
-__global__ void k_cell_offsets(const uint64_t *skeys, uint64_t n, uint64_t ncells,
+// offsets[c] = first sorted position with key >= c, for c in [0, ncells]
+// (grid-stride: the cell count lives on the device).
+__global__ void k_cell_offsets(const uint64_t *skeys, uint64_t n, const MsCtl *ctl,
                                uint64_t *offsets) {
-  uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c > ncells)
-    return;
-  uint64_t f = 0, cnt = n;
-  while (cnt) {
-    uint64_t h = cnt / 2, m = f + h;
-    if (skeys[m] < c) {
-      f = m + 1;
-      cnt -= h + 1;
-    } else
-      cnt = h;
+  const uint64_t ncells = ctl->grid.ncells;
+  for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c <= ncells;
+       c += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint64_t f = 0, cnt = n;
+    while (cnt) {
+      uint64_t h = cnt / 2, m = f + h;
+      if (skeys[m] < c) {
+        f = m + 1;
+        cnt -= h + 1;
+      } else
+        cnt = h;
+    }
+    offsets[c] = f;
   }
-  offsets[c] = f;
 }
 
 template <class R>
@@ -155,12 +249,13 @@ __global__ void k_gather_sorted(const R *pts, uint64_t n, int D, const uint32_t 
 }
 
 // Seed start positions -> cell keys (to run seeds in cell order).
-template <class R, int D>
+template <class R>
 __global__ void k_seed_keys(const R *seeds, const int32_t *seed_idx, const R *raw, uint64_t n,
-                            uint64_t ns, Grid g, uint64_t *keys, uint32_t *vals) {
+                            uint64_t ns, const MsCtl *ctl, uint64_t *keys, uint32_t *vals) {
   uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= ns)
     return;
+  const Grid g = ctl->grid;
   R m[3];
   for (int k = 0; k < 3; ++k)
     m[k] = seed_idx ? raw[g.axis[k] * n + static_cast<uint64_t>(seed_idx[s])]
@@ -174,7 +269,7 @@ __global__ void k_seed_keys(const R *seeds, const int32_t *seed_idx, const R *ra
 // `order` lists the seeds in cell order.
 template <class R, int D>
 __global__ void __launch_bounds__(kThreads)
-    k_ms_query(const R *pts, uint64_t n, const uint64_t *offsets, Grid g, const R *seeds,
+    k_ms_query(const R *pts, uint64_t n, const uint64_t *offsets, const MsCtl *ctl, const R *seeds,
                const int32_t *seed_idx, const R *raw, const uint32_t *order, uint64_t ns,
                unsigned logical_blocks, int iterations, R inv2h2, R r2, R floor_w, R *out,
                R *weight, unsigned long long *stats) {
@@ -188,6 +283,7 @@ __global__ void __launch_bounds__(kThreads)
   if (slot >= ns)
     return;
   const uint64_t s = order ? order[slot] : slot;
+  const Grid g = ctl->grid;
   R m[D];
   if (seed_idx) {
     const uint64_t v = static_cast<uint64_t>(seed_idx[s]);
@@ -274,106 +370,75 @@ __global__ void k_sum_u64(const unsigned long long *a, uint64_t n, unsigned long
 
 }  // namespace
 
-// Builds the grid and runs every seed; returns (candidates, in-support) visit
-// counts through stats_out when given.
+// Control block of the session's mean-shift calls (slot kSlotMs1, after the
+// per-block box partials).
+static MsCtl *ms_ctl(Session *s) {
+  const size_t part = 2 * kMaxD * kBboxBlocks * sizeof(double);
+  char *base = static_cast<char *>(scratch(s, kSlotMs1, part + sizeof(MsCtl)));
+  return reinterpret_cast<MsCtl *>(base + part);
+}
+
+void ms_begin(Session *s) { zero_device(s, ms_ctl(s), sizeof(MsCtl)); }
+
+void ms_finish(Session *s, unsigned long long sums[4]) {
+  MsCtl h;
+  read_back(s, &h, ms_ctl(s), sizeof h);
+  if (h.bad)
+    raise(MVS_EINVAL, "non-finite mean-shift points");
+  if (sums)
+    std::memcpy(sums, h.sums, sizeof h.sums);
+}
+
+// Builds the grid on the device and runs every seed, with no host
+// synchronisation; visit sums go to ctl->sums[run_slot], [run_slot + 1].
 template <class R>
 static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns, const R *seeds,
                        const int32_t *seed_idx, int iterations, double h, double cutoff, R *out,
-                       R *weight, unsigned long long *stats_out) {
+                       R *weight, int run_slot) {
   if (D != 4 && D != 6)
     raise(MVS_EINVAL, "batched mean-shift supports 4-D (reflection) and 6-D (rigid) points");
-  if (!(h > 0) || !(cutoff > 0) || iterations < 0)
+  if (!(h > 0) || !(cutoff > 0) || !std::isfinite(h * cutoff) || iterations < 0)
     raise(MVS_EINVAL, "invalid mean-shift parameters");
   if (n > static_cast<uint64_t>(UINT32_MAX) || ns > static_cast<uint64_t>(UINT32_MAX))
     raise(MVS_EOVERFLOW, "too many points or seeds");
-  // per-seed statistics [0, ns) candidates, [ns, 2ns) in-support visits, 2 sums
-  unsigned long long *stats = scratch_as<unsigned long long>(s, kSlotMs7, 2 * ns + 2);
+  if (seed_idx && n == 0 && ns)
+    raise(MVS_EINVAL, "seed votes given without points");
   if (ns == 0)
     return;
-  Grid g{};
-  g.axis[0] = 0, g.axis[1] = 1, g.axis[2] = 2;
+  MsCtl *ctl = ms_ctl(s);
+  double *partial = scratch_as<double>(s, kSlotMs1, 2 * kMaxD * kBboxBlocks);
+  // per-seed statistics [0, ns) candidates, [ns, 2ns) in-support visits
+  unsigned long long *stats = scratch_as<unsigned long long>(s, kSlotMs7, 2 * ns);
   R *sorted = scratch_as<R>(s, kSlotMs0, static_cast<size_t>(D) * (n ? n : 1));
-  uint64_t *off = nullptr;
+  uint64_t *off = scratch_as<uint64_t>(s, kSlotMs5, kMaxCells + 1);
+  unsigned nb = 0;
   if (n) {
-    unsigned nb = blocks_for(n, kThreads);
-    if (nb > 1024)
-      nb = 1024;
-    double *partial = scratch_as<double>(s, kSlotMs1, 2 * kMaxD * nb);
+    nb = blocks_for(n, kThreads);
+    if (nb > kBboxBlocks)
+      nb = kBboxBlocks;
     MVS_LAUNCH(s, "ms_bbox", k_bbox<R>, nb, kThreads, 0, pts, n, D, partial);
-    double *hp = static_cast<double *>(host_stage(s, 2 * kMaxD * nb * sizeof(double)));
-    MVS_HIP(hipMemcpyAsync(hp, partial, 2 * kMaxD * nb * sizeof(double), hipMemcpyDeviceToHost,
-                           s->stream));
-    MVS_HIP(hipStreamSynchronize(s->stream));
-    double lo[kMaxD], hi[kMaxD];
-    for (int k = 0; k < kMaxD; ++k)
-      lo[k] = INFINITY, hi[k] = -INFINITY;
-    for (unsigned b = 0; b < nb; ++b)
-      for (int k = 0; k < D; ++k) {
-        lo[k] = std::fmin(lo[k], hp[2 * kMaxD * b + k]);
-        hi[k] = std::fmax(hi[k], hp[2 * kMaxD * b + kMaxD + k]);
-      }
-    for (int k = 0; k < D; ++k)
-      if (!std::isfinite(lo[k]) || !std::isfinite(hi[k]))
-        raise(MVS_EINVAL, "non-finite mean-shift points");
-    // the three widest coordinates carry the grid (ties: lower index first)
-    int idx[kMaxD];
-    for (int k = 0; k < D; ++k)
-      idx[k] = k;
-    std::stable_sort(idx, idx + D, [&](int a, int b) { return hi[a] - lo[a] > hi[b] - lo[b]; });
-    for (int k = 0; k < 3; ++k)
-      g.axis[k] = idx[k];
-    std::sort(g.axis, g.axis + 3);
-    double cell = cutoff * h;
-    for (;;) {
-      uint64_t cells = 1;
-      for (int k = 0; k < 3; ++k) {
-        int a = g.axis[k];
-        g.dims[k] = static_cast<int>(std::floor((hi[a] - lo[a]) / cell)) + 1;
-        cells *= static_cast<uint64_t>(g.dims[k]);
-      }
-      if (cells <= (1ULL << 22))
-        break;
-      cell *= 2;
-    }
-    for (int k = 0; k < 3; ++k)
-      g.lo[k] = lo[g.axis[k]];
-    g.cell = cell;
-    g.ncells = static_cast<uint64_t>(g.dims[0]) * g.dims[1] * g.dims[2];
+  }
+  MVS_LAUNCH(s, "ms_grid", k_grid_setup, 1, kWave, 0, partial, nb, D, cutoff * h, ctl);
+  if (n) {
     uint64_t *keys = scratch_as<uint64_t>(s, kSlotMs2, 2 * n);
     uint32_t *vals = scratch_as<uint32_t>(s, kSlotMs3, 2 * n);
-    off = scratch_as<uint64_t>(s, kSlotMs5, g.ncells + 1);
-    MVS_LAUNCH(s, "ms_cell_keys", k_cell_keys<R>, blocks_for(n, kThreads), kThreads, 0, pts, n, g,
-               keys, vals);
-    int bits = 1;
-    while ((1ULL << bits) < g.ncells)
-      ++bits;
-    radix_sort_pairs_u64(s, keys, vals, keys + n, vals + n, n, bits);
-    MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, blocks_for(g.ncells + 1, kThreads), kThreads,
-               0, keys + n, n, g.ncells, off);
-    MVS_LAUNCH(s, "ms_gather", k_gather_sorted<R>, blocks_for(n, kThreads), kThreads, 0, pts, n, D,
-               vals + n, sorted);
+    MVS_LAUNCH(s, "ms_cell_keys", k_cell_keys<R>, blocks_for(n, kThreads), kThreads, 0, pts, n,
+               ctl, keys, vals);
+    radix_sort_pairs_u64(s, keys, vals, keys + n, vals + n, n, kCellBits);
+    MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 2048, kThreads, 0, keys + n, n, ctl, off);
+    MVS_LAUNCH(s, "ms_gather", k_gather_sorted<R>, blocks_for(n, kThreads), kThreads, 0, pts, n,
+               D, vals + n, sorted);
   } else {
-    off = scratch_as<uint64_t>(s, kSlotMs5, 2);
-    zero_device(s, off, 2 * sizeof(uint64_t));
-    g.dims[0] = g.dims[1] = g.dims[2] = 1;
-    g.cell = cutoff * h;
-    g.ncells = 1;
+    MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 1, kThreads, 0, nullptr, 0, ctl, off);
   }
   // seeds in cell order
-  uint64_t *skeys = scratch_as<uint64_t>(s, kSlotMs4, 2 * ns);
-  uint32_t *svals = scratch_as<uint32_t>(s, kSlotMs6, 2 * ns);
   const uint32_t *order = nullptr;
   if (n && ns > 1) {
-    if (D == 6)
-      MVS_LAUNCH(s, "ms_seed_keys", (k_seed_keys<R, 6>), blocks_for(ns, kThreads), kThreads, 0,
-                 seeds, seed_idx, pts, n, ns, g, skeys, svals);
-    else
-      MVS_LAUNCH(s, "ms_seed_keys", (k_seed_keys<R, 4>), blocks_for(ns, kThreads), kThreads, 0,
-                 seeds, seed_idx, pts, n, ns, g, skeys, svals);
-    int bits = 1;
-    while ((1ULL << bits) < g.ncells)
-      ++bits;
-    radix_sort_pairs_u64(s, skeys, svals, skeys + ns, svals + ns, ns, bits);
+    uint64_t *skeys = scratch_as<uint64_t>(s, kSlotMs4, 2 * ns);
+    uint32_t *svals = scratch_as<uint32_t>(s, kSlotMs6, 2 * ns);
+    MVS_LAUNCH(s, "ms_seed_keys", k_seed_keys<R>, blocks_for(ns, kThreads), kThreads, 0, seeds,
+               seed_idx, pts, n, ns, ctl, skeys, svals);
+    radix_sort_pairs_u64(s, skeys, svals, skeys + ns, svals + ns, ns, kCellBits);
     order = svals + ns;
   }
   const R inv2h2 = static_cast<R>(1.0 / (2.0 * h * h));
@@ -382,36 +447,33 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   const unsigned logical = blocks_for(ns, kWavesPerBlock);
   const unsigned grid = ((logical + kXcds - 1) / kXcds) * kXcds;
   if (D == 6)
-    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 6>), grid, kThreads, 0, sorted, n, off, g, seeds,
+    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 6>), grid, kThreads, 0, sorted, n, off, ctl, seeds,
                seed_idx, pts, order, ns, logical, iterations, inv2h2, r2, fw, out, weight, stats);
   else
-    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 4>), grid, kThreads, 0, sorted, n, off, g, seeds,
+    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 4>), grid, kThreads, 0, sorted, n, off, ctl, seeds,
                seed_idx, pts, order, ns, logical, iterations, inv2h2, r2, fw, out, weight, stats);
-  if (stats_out) {
-    MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats, ns, stats + 2 * ns);
-    MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats + ns, ns, stats + 2 * ns + 1);
-    void *hp = host_stage(s, 2 * sizeof(unsigned long long));
-    MVS_HIP(hipMemcpyAsync(hp, stats + 2 * ns, 2 * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, s->stream));
-    MVS_HIP(hipStreamSynchronize(s->stream));
-    std::memcpy(stats_out, hp, 2 * sizeof(unsigned long long));
+  if (run_slot >= 0) {
+    MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats, ns, ctl->sums + run_slot);
+    MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats + ns, ns,
+               ctl->sums + run_slot + 1);
   }
 }
 
 void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,
                     const float *seeds, const int32_t *seed_idx, int iterations, double h,
-                    double cutoff, float *out, float *weight, unsigned long long *stats) {
-  ms_batched<float>(s, D, n, pts, ns, seeds, seed_idx, iterations, h, cutoff, out, weight, stats);
+                    double cutoff, float *out, float *weight, int run_slot) {
+  ms_batched<float>(s, D, n, pts, ns, seeds, seed_idx, iterations, h, cutoff, out, weight,
+                    run_slot);
 }
 void ms_batched_f64(Session *s, int D, uint64_t n, const double *pts, uint64_t ns,
                     const double *seeds, const int32_t *seed_idx, int iterations, double h,
-                    double cutoff, double *out, double *weight, unsigned long long *stats) {
+                    double cutoff, double *out, double *weight, int run_slot) {
   ms_batched<double>(s, D, n, pts, ns, seeds, seed_idx, iterations, h, cutoff, out, weight,
-                     stats);
+                     run_slot);
 }
 void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4, const void *t3,
-                      uint64_t nrefl, const void *n3, const void *d1, void *rigid_pts,
-                      void *refl_pts) {
+                      uint64_t nrefl, const void *n3, uint64_t n3_stride, const void *d1,
+                      void *rigid_pts, void *refl_pts) {
   if (precision == MVS_FP32_GUARDED) {
     if (nrigid)
       MVS_LAUNCH(s, "ms_points", k_points_rigid<float>, blocks_for(nrigid, kThreads), kThreads, 0,
@@ -419,7 +481,7 @@ void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4
                  static_cast<float *>(rigid_pts));
     if (nrefl)
       MVS_LAUNCH(s, "ms_points", k_points_refl<float>, blocks_for(nrefl, kThreads), kThreads, 0,
-                 static_cast<const float *>(n3), static_cast<const float *>(d1), nrefl,
+                 static_cast<const float *>(n3), n3_stride, static_cast<const float *>(d1), nrefl,
                  static_cast<float *>(refl_pts));
   } else if (precision == MVS_FP64) {
     if (nrigid)
@@ -428,7 +490,7 @@ void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4
                  static_cast<double *>(rigid_pts));
     if (nrefl)
       MVS_LAUNCH(s, "ms_points", k_points_refl<double>, blocks_for(nrefl, kThreads), kThreads, 0,
-                 static_cast<const double *>(n3), static_cast<const double *>(d1), nrefl,
+                 static_cast<const double *>(n3), n3_stride, static_cast<const double *>(d1), nrefl,
                  static_cast<double *>(refl_pts));
   } else
     raise(MVS_EINVAL, "unknown precision");
@@ -444,7 +506,7 @@ int mvs_dev_ms_points(mvs_session *sp, int precision, uint64_t nrigid, const voi
                       const void *t3, uint64_t nrefl, const void *n3, const void *d1,
                       void *rigid_pts, void *refl_pts) {
   return guarded([&] {
-    ms_points_device(as_session(sp), precision, nrigid, q4, t3, nrefl, n3, d1, rigid_pts,
+    ms_points_device(as_session(sp), precision, nrigid, q4, t3, nrefl, n3, nrefl, d1, rigid_pts,
                      refl_pts);
   });
 }
@@ -454,22 +516,23 @@ int mvs_dev_ms_batched(mvs_session *sp, int precision, int dim, uint64_t n, cons
                        void *out, void *weight, uint64_t *candidates, uint64_t *in_support) {
   return guarded([&] {
     Session *s = as_session(sp);
-    unsigned long long st[2] = {0, 0};
-    unsigned long long *want = (candidates || in_support) ? st : nullptr;
+    if (precision != MVS_FP32_GUARDED && precision != MVS_FP64)
+      raise(MVS_EINVAL, "unknown precision");
+    ms_begin(s);
     if (precision == MVS_FP32_GUARDED)
       ms_batched<float>(s, dim, n, static_cast<const float *>(pts), ns,
                         static_cast<const float *>(seeds), nullptr, iterations, h, cutoff,
-                        static_cast<float *>(out), static_cast<float *>(weight), want);
-    else if (precision == MVS_FP64)
+                        static_cast<float *>(out), static_cast<float *>(weight), 0);
+    else
       ms_batched<double>(s, dim, n, static_cast<const double *>(pts), ns,
                          static_cast<const double *>(seeds), nullptr, iterations, h, cutoff,
-                         static_cast<double *>(out), static_cast<double *>(weight), want);
-    else
-      raise(MVS_EINVAL, "unknown precision");
+                         static_cast<double *>(out), static_cast<double *>(weight), 0);
+    unsigned long long sums[4];
+    ms_finish(s, sums);
     if (candidates)
-      *candidates = st[0];
+      *candidates = sums[0];
     if (in_support)
-      *in_support = st[1];
+      *in_support = sums[1];
   });
 }
 

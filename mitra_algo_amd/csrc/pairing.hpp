@@ -57,16 +57,22 @@ void vote_emit_strided(Session *s, uint64_t P, const int32_t *a, const int32_t *
 // MVS_EINVAL when the count pass met a pair ordinal outside [0, S).
 uint64_t vote_total_checked(Session *s);
 
-// Batched mean-shift (msbatch.hip).
+// Batched mean-shift (msbatch.hip).  A call is ms_begin, up to two runs
+// (run_slot 0 and 2 receive the (candidates, in-support) visit sums; -1 for
+// none), then ms_finish: one synchronisation that raises MVS_EINVAL for
+// non-finite points and returns the four sums.
+void ms_begin(Session *s);
+void ms_finish(Session *s, unsigned long long sums[4]);
 void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,
                     const float *seeds, const int32_t *seed_idx, int iterations, double h,
-                    double cutoff, float *out, float *weight, unsigned long long *stats);
+                    double cutoff, float *out, float *weight, int run_slot);
 void ms_batched_f64(Session *s, int D, uint64_t n, const double *pts, uint64_t ns,
                     const double *seeds, const int32_t *seed_idx, int iterations, double h,
-                    double cutoff, double *out, double *weight, unsigned long long *stats);
+                    double cutoff, double *out, double *weight, int run_slot);
+// Compact votes to mean-shift points; mirror normals are 3 rows of n3_stride.
 void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4, const void *t3,
-                      uint64_t nrefl, const void *n3, const void *d1, void *rigid_pts,
-                      void *refl_pts);
+                      uint64_t nrefl, const void *n3, uint64_t n3_stride, const void *d1,
+                      void *rigid_pts, void *refl_pts);
 
 // Sample records of the last pair count (valid until the next count).
 const double *plan_samples64(Session *s);

@@ -66,7 +66,11 @@ void copy_d2h(Session *s, void *dst, const void *src, size_t bytes) {
 void sync(Session *s) { MVS_HIP(hipStreamSynchronize(s->stream)); }
 
 void Session::begin(const char *name) {
+  if (timing_open)  // the previous launch raised before closing its record
+    end();
   if (!timing)
+    return;
+  if (!timing_only.empty() && timing_only.find("," + std::string(name) + ",") == std::string::npos)
     return;
   hipEvent_t a, b;
   if (event_pool.size() >= 2) {
@@ -80,10 +84,12 @@ void Session::begin(const char *name) {
   }
   MVS_HIP(hipEventRecord(a, stream));
   pending.push_back({name, a, b});
+  timing_open = true;
 }
 void Session::end() {
-  if (!timing || pending.empty())
+  if (!timing_open)
     return;
+  timing_open = false;
   MVS_HIP(hipEventRecord(pending.back().stop, stream));
 }
 void Session::collect() {
@@ -185,6 +191,14 @@ int mvs_session_enable_timing(mvs_session *s, int enable) {
     p->collect();
     p->timing = enable != 0;
     p->timings.clear();
+  });
+}
+
+int mvs_session_timing_filter(mvs_session *s, const char *names) {
+  return guarded([&] {
+    Session *p = as_session(s);
+    p->collect();
+    p->timing_only = names && *names ? "," + std::string(names) + "," : std::string();
   });
 }
 

@@ -38,6 +38,8 @@ struct Session {
   void *stage = nullptr;
   size_t stage_bytes = 0;
   bool timing = false;
+  std::string timing_only;  // ",name,name," or empty for every kernel
+  bool timing_open = false;  // begin() recorded a start event for the current launch
   std::vector<PendingTiming> pending;
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, std::pair<double, uint64_t>> timings;
