@@ -140,8 +140,10 @@ int mvs_session_set_stream(mvs_session *s, void *hip_stream);
 int mvs_session_sync(mvs_session *s);
 /* Stream-ordered device-to-device copy (to read session-owned outputs). */
 int mvs_dev_copy(mvs_session *s, void *dst, const void *src, uint64_t bytes);
-/* Device time (ms) of the last mvs_dev_* launch group, measured with HIP
- * events on the session stream; see mvs_dev_timing for per-kernel times. */
+/* Per-launch kernel timing with HIP events on the session stream (clears
+ * the accumulated times).  mvs_dev_timing returns the accumulated device
+ * time (ms) and launch count of one kernel name, or of every kernel when
+ * kernel is NULL. */
 int mvs_session_enable_timing(mvs_session *s, int enable);
 /* Restricts timing to the comma-separated kernel names (NULL or "" = all). */
 int mvs_session_timing_filter(mvs_session *s, const char *names);

@@ -206,11 +206,17 @@ int mvs_dev_timing(mvs_session *s, const char *kernel, double *ms, uint64_t *lau
   return guarded([&] {
     Session *p = as_session(s);
     p->collect();
-    auto it = p->timings.find(kernel ? kernel : "");
+    double t = 0;
+    uint64_t n = 0;
+    for (const auto &kv : p->timings)
+      if (!kernel || kv.first == kernel) {
+        t += kv.second.first;
+        n += kv.second.second;
+      }
     if (ms)
-      *ms = it == p->timings.end() ? 0.0 : it->second.first;
+      *ms = t;
     if (launches)
-      *launches = it == p->timings.end() ? 0 : it->second.second;
+      *launches = n;
   });
 }
 

@@ -82,3 +82,13 @@ def test_sampling_matches_oracle(oracle):
     for _ in range(9999):
         capi.mt_next(g)
     assert capi.mt_next(g) == 9981545732273789042
+
+
+def test_adapter_check_usage():
+    """The reference-driven adapter check builds where the reference is and
+    rejects a missing argument without touching a GPU."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "adapter_check")
+    if not os.path.exists(exe):
+        pytest.skip("built only where the reference sources are")
+    assert subprocess.run([exe], capture_output=True).returncode == 2
