@@ -223,6 +223,7 @@ def main():
         "pairvote_ms": pv32_s * 1e3,
         "meanshift_s": ms32_s,
         "meanshift_iterations_per_s": SEEDS * ITERS / ms32_s,
+        "meanshift_visits": {"candidates": ms32.candidates, "in_support": ms32.in_support},
         "fp64": {"value": V / dt64, "ms_per_step": dt64 * 1e3, "pairvote_ms": pv64_s * 1e3,
                  "meanshift_s": ms64_s, "ms_query_tflops": q64_tflops,
                  "speedup_fp32g_over_fp64": dt64 / dt32, "pair_set_identical": bool(same)},

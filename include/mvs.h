@@ -269,13 +269,15 @@ int mvs_dev_pairvote(mvs_session *s, uint64_t n, const double *x, const double *
 
 /* Batched mean-shift over the votes of the last mvs_dev_pairvote:
  * `seeds` seed votes split between rigid and mirror votes in proportion to
- * their counts; within each stream of n votes, seed j of k is drawn from the
- * j-th of k equal strata [j n / k, (j + 1) n / k) by a splitmix64 hash of
- * (seed_value, j), so seeds are distinct and ascending.  Each seed is
- * iterated `iterations` times in its own point set (see mvs_dev_ms_batched).
- * Grid, seeds and queries are built on the device; the call synchronises
- * once at the end (non-finite points raise MVS_EINVAL).  Outputs live in
- * session-owned buffers; want_stats != 0 also returns the visit counts. */
+ * their counts.  Within each stream of n votes, sorted by grid cell, seed j
+ * of k is drawn from the j-th of k equal strata of the sorted order by a
+ * splitmix64 hash of (seed_value, j): seeds are distinct and spread over the
+ * occupied cells; seed_votes lists their stream indices in that (cell)
+ * order.  Each seed is iterated up to `iterations` times in its own point
+ * set (see mvs_dev_ms_batched).  Grid, seeds and queries are built on the
+ * device; the call synchronises once at the end (non-finite points raise
+ * MVS_EINVAL).  Outputs live in session-owned buffers; want_stats != 0 also
+ * returns the visit counts. */
 typedef struct {
   uint64_t rigid_seeds, refl_seeds;
   const int32_t *seed_votes;          /* rigid seeds then mirror seeds (stream indices) */
@@ -295,7 +297,9 @@ int mvs_dev_meanshift_votes(mvs_session *s, uint64_t seeds, uint64_t seed_value,
  * mvs_dev_ms_batched runs `iterations` updates of every seed over one point
  * set (dim 6 or 4, dim*n SoA) with the kernel
  *   w = exp(-d^2/2h^2) - exp(-cutoff^2/2) for d < cutoff*h, else 0,
- * x <- x + sum w (p - x) / sum w.  Seeds and out: dim*ns SoA; weight: ns.
+ * x <- x + sum w (p - x) / sum w.  A seed stops early once an update leaves
+ * it unchanged (a fixed point: further updates would repeat it exactly).
+ * Seeds and out: dim*ns SoA; weight: ns (sum of weights at the result).
  * MVS_FP32_GUARDED selects FP32 points and accumulation, MVS_FP64 FP64.
  * candidates / in_support (optional) receive the numbers of point visits and
  * of visits inside the support (synchronises when requested). */

@@ -129,6 +129,15 @@ void zero_device(Session *s, void *ptr, size_t bytes);
 // std::mt19937_64 step on a state in libstdc++ layout (sampling.cpp).
 uint64_t mt64_next(mvs_mt64 *g);
 
+// splitmix64 output function (Steele, Lea and Flood, "Fast splittable
+// pseudorandom number generators", OOPSLA 2014): seed-stratum draws.
+__host__ __device__ inline uint64_t splitmix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
 inline unsigned blocks_for(uint64_t n, unsigned threads) {
   return static_cast<unsigned>((n + threads - 1) / threads);
 }

@@ -15,14 +15,6 @@
 
 namespace mvs {
 
-// Host twin of the device mixer below (the seed salt).
-static uint64_t mix64_host(uint64_t z) {
-  z += 0x9e3779b97f4a7c15ULL;
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-  return z ^ (z >> 31);
-}
-
 static void pairvote(Session *s, uint64_t n, const double *x, const double *y, const double *z,
                      const double *feat, uint64_t S, const int32_t *sel, double box,
                      const double *center, double t, uint64_t max_pairs, uint64_t max_votes,
@@ -70,25 +62,6 @@ static void pairvote(Session *s, uint64_t n, const double *x, const double *y, c
   *out = r;
 }
 
-// splitmix64 output function (Steele, Lea and Flood, "Fast splittable
-// pseudorandom number generators", OOPSLA 2014).
-__device__ inline uint64_t mix64(uint64_t z) {
-  z += 0x9e3779b97f4a7c15ULL;
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-  return z ^ (z >> 31);
-}
-
-// k distinct indices of [0, n) (k <= n), ascending: index j is drawn
-// uniformly from the j-th of k equal strata [j n / k, (j + 1) n / k).
-__global__ void k_pick_seeds(uint64_t n, uint64_t k, uint64_t salt, int32_t *out) {
-  const uint64_t j = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= k)
-    return;
-  const uint64_t lo = j * n / k, hi = (j + 1) * n / k;
-  out[j] = static_cast<int32_t>(lo + mix64(salt ^ mix64(j)) % (hi - lo));
-}
-
 static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, int iterations,
                             double h, double cutoff, bool want_stats, mvs_meanshift_result *out) {
   if (!s->pairvote_valid)
@@ -101,12 +74,8 @@ static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, in
   kf = std::min(kf, nf);
   const uint64_t kr = std::min(nseeds > kf ? nseeds - kf : 0, nr);
   int32_t *dseeds = scratch_as<int32_t>(s, kSlotOutMsSeeds, kr + kf ? kr + kf : 1);
-  if (kr)
-    MVS_LAUNCH(s, "ms_seeds", k_pick_seeds, blocks_for(kr, 256), 256, 0, nr, kr,
-               mix64_host(seed_value), dseeds);
-  if (kf)
-    MVS_LAUNCH(s, "ms_seeds", k_pick_seeds, blocks_for(kf, 256), 256, 0, nf, kf,
-               mix64_host(seed_value ^ 0x6d6972726f72ULL), dseeds + kr);
+  const uint64_t salt_r = splitmix64(seed_value);
+  const uint64_t salt_f = splitmix64(seed_value ^ 0x6d6972726f72ULL);  // "mirror"
   void *rpts = scratch(s, kSlotOutMsRigidPts, 6 * (nr ? nr : 1) * sz);
   void *fpts = scratch(s, kSlotOutMsReflPts, 4 * (nf ? nf : 1) * sz);
   void *orig = scratch(s, kSlotOutMsRigid, 6 * (kr ? kr : 1) * sz);
@@ -119,19 +88,21 @@ static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, in
   const int slot_r = want_stats ? 0 : -1, slot_f = want_stats ? 2 : -1;
   if (prec == MVS_FP32_GUARDED) {
     if (kr)
-      ms_batched_f32(s, 6, nr, static_cast<const float *>(rpts), kr, nullptr, dseeds, iterations,
-                     h, cutoff, static_cast<float *>(orig), static_cast<float *>(owr), slot_r);
+      ms_batched_f32(s, 6, nr, static_cast<const float *>(rpts), kr, nullptr, dseeds, salt_r,
+                     iterations, h, cutoff, static_cast<float *>(orig), static_cast<float *>(owr),
+                     slot_r);
     if (kf)
       ms_batched_f32(s, 4, nf, static_cast<const float *>(fpts), kf, nullptr, dseeds + kr,
-                     iterations, h, cutoff, static_cast<float *>(ofl), static_cast<float *>(owf),
-                     slot_f);
+                     salt_f, iterations, h, cutoff, static_cast<float *>(ofl),
+                     static_cast<float *>(owf), slot_f);
   } else {
     if (kr)
-      ms_batched_f64(s, 6, nr, static_cast<const double *>(rpts), kr, nullptr, dseeds, iterations,
-                     h, cutoff, static_cast<double *>(orig), static_cast<double *>(owr), slot_r);
+      ms_batched_f64(s, 6, nr, static_cast<const double *>(rpts), kr, nullptr, dseeds, salt_r,
+                     iterations, h, cutoff, static_cast<double *>(orig),
+                     static_cast<double *>(owr), slot_r);
     if (kf)
       ms_batched_f64(s, 4, nf, static_cast<const double *>(fpts), kf, nullptr, dseeds + kr,
-                     iterations, h, cutoff, static_cast<double *>(ofl),
+                     salt_f, iterations, h, cutoff, static_cast<double *>(ofl),
                      static_cast<double *>(owf), slot_f);
   }
   unsigned long long sums[4] = {0, 0, 0, 0};

@@ -99,8 +99,11 @@ __global__ void k_bbox(const R *pts, uint64_t n, int D, double *partial) {
     partial[blockIdx.x * 2 * kMaxD + threadIdx.x] = s[threadIdx.x][0];
 }
 
+// perm lists the point coordinates in storage (slot) order: the three grid
+// axes ascending, then the remaining coordinates ascending.
 struct Grid {
   int axis[3];
+  int perm[kMaxD];
   double lo[3];
   double cell;
   int dims[3];
@@ -140,6 +143,8 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, int D, double c
     return;
   Grid g{};
   g.axis[0] = 0, g.axis[1] = 1, g.axis[2] = 2;
+  for (int k = 0; k < kMaxD; ++k)
+    g.perm[k] = k;
   g.cell = cell0;
   g.dims[0] = g.dims[1] = g.dims[2] = 1;
   g.ncells = 1;
@@ -186,8 +191,12 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, int D, double c
     }
     for (int k = 0; k < 3; ++k) {
       g.axis[k] = ax[k];
+      g.perm[k] = ax[k];
       g.lo[k] = lo[ax[k]];
     }
+    for (int k = 0, r = 3; k < D; ++k)
+      if (k != ax[0] && k != ax[1] && k != ax[2])
+        g.perm[r++] = k;
     g.cell = cell;
   }
   ctl->grid = g;
@@ -238,41 +247,89 @@ __global__ void k_cell_offsets(const uint64_t *skeys, uint64_t n, const MsCtl *c
   }
 }
 
-template <class R>
-__global__ void k_gather_sorted(const R *pts, uint64_t n, int D, const uint32_t *order, R *out) {
+// Storage of the cell-sorted points: slot coordinates 0-3 (three grid axes
+// and the first remaining one) as one 4-vector, slots 4-5 (6-D only) as a
+// 2-vector, so the three-axis pretest is one coalesced load per candidate.
+template <class R> struct Vec;
+template <> struct Vec<float> {
+  using V4 = float4;
+  using V2 = float2;
+};
+template <> struct Vec<double> {
+  using V4 = double4;
+  using V2 = double2;
+};
+
+template <class R, int D>
+__global__ void k_gather_sorted(const R *pts, uint64_t n, const MsCtl *ctl, const uint32_t *order,
+                                typename Vec<R>::V4 *p4, typename Vec<R>::V2 *q2) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
-  uint32_t o = order[i];
-  for (int k = 0; k < D; ++k)
-    out[k * n + i] = pts[k * n + o];
+  const int *perm = ctl->grid.perm;
+  const uint64_t o = order[i];
+  typename Vec<R>::V4 v;
+  v.x = pts[perm[0] * n + o];
+  v.y = pts[perm[1] * n + o];
+  v.z = pts[perm[2] * n + o];
+  v.w = pts[perm[3] * n + o];
+  p4[i] = v;
+  if (D == 6) {
+    typename Vec<R>::V2 u;
+    u.x = pts[perm[4] * n + o];
+    u.y = pts[perm[5] * n + o];
+    q2[i] = u;
+  }
 }
 
-// Seed start positions -> cell keys (to run seeds in cell order).
+// Seeds taken straight from the cell-sorted points: seed j of k is drawn
+// from the j-th of k equal strata of the sorted order (splitmix64 of
+// (salt, j)), so seeds come out distinct and in cell order.  ids receive the
+// seeds' point indices in the caller's (unsorted) numbering.
+__global__ void k_pick_sorted(uint64_t n, uint64_t k, uint64_t salt, const uint32_t *sorted_ids,
+                              uint32_t *pos, int32_t *ids) {
+  const uint64_t j = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= k)
+    return;
+  const uint64_t lo = j * n / k, hi = (j + 1) * n / k;
+  const uint64_t p = lo + splitmix64(salt ^ splitmix64(j)) % (hi - lo);
+  pos[j] = static_cast<uint32_t>(p);
+  ids[j] = static_cast<int32_t>(sorted_ids[p]);
+}
+
+// Explicit seed positions -> cell keys (to run those seeds in cell order).
 template <class R>
-__global__ void k_seed_keys(const R *seeds, const int32_t *seed_idx, const R *raw, uint64_t n,
-                            uint64_t ns, const MsCtl *ctl, uint64_t *keys, uint32_t *vals) {
+__global__ void k_seed_keys(const R *seeds, uint64_t ns, const MsCtl *ctl, uint64_t *keys,
+                            uint32_t *vals) {
   uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= ns)
     return;
   const Grid g = ctl->grid;
-  R m[3];
-  for (int k = 0; k < 3; ++k)
-    m[k] = seed_idx ? raw[g.axis[k] * n + static_cast<uint64_t>(seed_idx[s])]
-                    : seeds[g.axis[k] * ns + s];
-  keys[s] = cell_key(g, m[0], m[1], m[2]);
+  keys[s] = cell_key(g, seeds[g.axis[0] * ns + s], seeds[g.axis[1] * ns + s],
+                     seeds[g.axis[2] * ns + s]);
   vals[s] = static_cast<uint32_t>(s);
 }
 
-// One wavefront per seed.  Seeds are positions (seeds, D x ns SoA) or
-// indices into the unsorted point set (seed_idx with raw = those points);
-// `order` lists the seeds in cell order.
+// Per-wave ranges of the 3 x 3 neighbourhood rows (each row: three cells
+// contiguous in the sorted order), flattened across the lanes.
+struct RowRanges {
+  uint32_t beg[9];
+  uint32_t pre[10];
+};
+
+// One wavefront per seed.  Seeds are sorted points (seed_pos, already in
+// cell order) or explicit positions (seeds, D x ns SoA, run in `order`).
+// All iterations run in registers; a seed whose update leaves every
+// coordinate unchanged has reached a fixed point of the iteration and stops
+// (the remaining iterations would repeat it bit for bit).
 template <class R, int D>
 __global__ void __launch_bounds__(kThreads)
-    k_ms_query(const R *pts, uint64_t n, const uint64_t *offsets, const MsCtl *ctl, const R *seeds,
-               const int32_t *seed_idx, const R *raw, const uint32_t *order, uint64_t ns,
-               unsigned logical_blocks, int iterations, R inv2h2, R r2, R floor_w, R *out,
-               R *weight, unsigned long long *stats) {
+    k_ms_query(const typename Vec<R>::V4 *p4, const typename Vec<R>::V2 *q2,
+               const uint64_t *offsets, const MsCtl *ctl, const uint32_t *seed_pos,
+               const R *seeds, const uint32_t *order, uint64_t ns, unsigned logical_blocks,
+               int iterations, R inv2h2, R r2, R floor_w, R *out, R *weight,
+               unsigned long long *stats) {
+  __shared__ RowRanges rows[kWavesPerBlock];
   const int lane = threadIdx.x & (kWave - 1);
   // XCD-aware block order: dies take contiguous runs of (cell-ordered) seeds
   const unsigned per = (logical_blocks + kXcds - 1) / kXcds;
@@ -282,70 +339,110 @@ __global__ void __launch_bounds__(kThreads)
   const uint64_t slot = static_cast<uint64_t>(logical) * kWavesPerBlock + threadIdx.x / kWave;
   if (slot >= ns)
     return;
-  const uint64_t s = order ? order[slot] : slot;
+  RowRanges &W = rows[threadIdx.x / kWave];
   const Grid g = ctl->grid;
-  R m[D];
-  if (seed_idx) {
-    const uint64_t v = static_cast<uint64_t>(seed_idx[s]);
-    for (int k = 0; k < D; ++k)
-      m[k] = raw[k * n + v];
+  const uint64_t s = order ? order[slot] : slot;
+  R m[D];  // slot order
+  if (seed_pos) {
+    const uint64_t p = seed_pos[s];
+    const typename Vec<R>::V4 v = p4[p];
+    m[0] = v.x, m[1] = v.y, m[2] = v.z, m[3] = v.w;
+    if (D == 6) {
+      const typename Vec<R>::V2 u = q2[p];
+      m[4 % D] = u.x, m[5 % D] = u.y;
+    }
   } else {
     for (int k = 0; k < D; ++k)
-      m[k] = seeds[k * ns + s];
+      m[k] = seeds[g.perm[k] * ns + s];
   }
   R wsum = 0;
   unsigned long long cand = 0, inside = 0;
+  const int dz = lane / 3 - 1, dy = lane % 3 - 1;
   for (int it = 0; it < iterations; ++it) {
+    const int cx = cell_of(static_cast<double>(m[0]), g.lo[0], g.cell, g.dims[0]);
+    const int cy = cell_of(static_cast<double>(m[1]), g.lo[1], g.cell, g.dims[1]);
+    const int cz = cell_of(static_cast<double>(m[2]), g.lo[2], g.cell, g.dims[2]);
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.dims[0] ? cx + 1 : cx;
+    uint32_t beg = 0, len = 0;
+    if (lane < 9) {
+      const int z = cz + dz, y = cy + dy;
+      if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
+        const uint64_t row = (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
+        const uint64_t b = offsets[row + x0];
+        beg = static_cast<uint32_t>(b);
+        len = static_cast<uint32_t>(offsets[row + x1 + 1] - b);
+      }
+    }
+    uint32_t x = len;  // inclusive prefix over lanes 0..8
+    for (int o = 1; o < 16; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, kWave);
+      if (lane >= o)
+        x += y;
+    }
+    const uint32_t total = __shfl(x, 8, kWave);
+    if (lane < 9) {
+      W.beg[lane] = beg;
+      W.pre[lane] = x - len;
+    }
+    if (lane == 8)
+      W.pre[9] = x;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    cand += total;
     R acc[D];
     for (int k = 0; k < D; ++k)
       acc[k] = 0;
     R wl = 0;
-    const int cx = cell_of(static_cast<double>(m[g.axis[0]]), g.lo[0], g.cell, g.dims[0]);
-    const int cy = cell_of(static_cast<double>(m[g.axis[1]]), g.lo[1], g.cell, g.dims[1]);
-    const int cz = cell_of(static_cast<double>(m[g.axis[2]]), g.lo[2], g.cell, g.dims[2]);
-    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.dims[0] ? cx + 1 : cx;
-    for (int dz = -1; dz <= 1; ++dz) {
-      int z = cz + dz;
-      if (z < 0 || z >= g.dims[2])
-        continue;
-      for (int dy = -1; dy <= 1; ++dy) {
-        int y = cy + dy;
-        if (y < 0 || y >= g.dims[1])
-          continue;
-        uint64_t row = (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
-        uint64_t beg = offsets[row + x0], end = offsets[row + x1 + 1];
-        cand += end - beg;
-        for (uint64_t j = beg + lane; j < end; j += kWave) {
-          R d[D];
-          R d2 = 0;
-          for (int k = 0; k < D; ++k) {
-            d[k] = pts[k * n + j] - m[k];
-            d2 += d[k] * d[k];
-          }
-          if (d2 < r2) {
-            R w = ms_exp<R>(-d2 * inv2h2) - floor_w;
-            wl += w;
-            for (int k = 0; k < D; ++k)
-              acc[k] += w * d[k];
-            ++inside;
-          }
+    int r = 0;
+    for (uint32_t c = lane; c < total; c += kWave) {
+      while (c >= W.pre[r + 1])
+        ++r;
+      const uint64_t j = W.beg[r] + (c - W.pre[r]);
+      const typename Vec<R>::V4 p = p4[j];
+      R d[D];
+      d[0] = p.x - m[0];
+      d[1] = p.y - m[1];
+      d[2] = p.z - m[2];
+      const R dg = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (dg < r2) {
+        d[3] = p.w - m[3];
+        R dd = dg + d[3] * d[3];
+        if (D == 6) {
+          const typename Vec<R>::V2 q = q2[j];
+          d[4 % D] = q.x - m[4 % D];
+          d[5 % D] = q.y - m[5 % D];
+          dd += d[4 % D] * d[4 % D] + d[5 % D] * d[5 % D];
+        }
+        if (dd < r2) {
+          const R w = ms_exp<R>(-dd * inv2h2) - floor_w;
+          wl += w;
+          for (int k = 0; k < D; ++k)
+            acc[k] += w * d[k];
+          ++inside;
         }
       }
     }
+    __builtin_amdgcn_wave_barrier();  // rows are rewritten next iteration
     wl = wave_sum(wl);
     for (int k = 0; k < D; ++k)
       acc[k] = wave_sum(acc[k]);
     wsum = wl;
+    bool moved = false;
     if (wl > R(0)) {
-      R inv = R(1) / wl;
-      for (int k = 0; k < D; ++k)
-        m[k] += acc[k] * inv;
+      const R inv = R(1) / wl;
+      for (int k = 0; k < D; ++k) {
+        const R nk = m[k] + acc[k] * inv;
+        moved = moved || nk != m[k];
+        m[k] = nk;
+      }
     }
+    if (!moved)
+      break;
   }
   inside = wave_sum(inside);
   if (lane == 0) {
     for (int k = 0; k < D; ++k)
-      out[k * ns + s] = m[k];
+      out[g.perm[k] * ns + s] = m[k];
     weight[s] = wsum;
     stats[s] = cand;  // per-seed slots: no contended atomics
     stats[ns + s] = inside;
@@ -391,25 +488,33 @@ void ms_finish(Session *s, unsigned long long sums[4]) {
 
 // Builds the grid on the device and runs every seed, with no host
 // synchronisation; visit sums go to ctl->sums[run_slot], [run_slot + 1].
+// Seeds: explicit positions (`seeds`, D x ns SoA), or, when seed_ids is
+// given, ns points drawn from the sorted points (salt), whose indices are
+// written to seed_ids.
 template <class R>
 static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns, const R *seeds,
-                       const int32_t *seed_idx, int iterations, double h, double cutoff, R *out,
-                       R *weight, int run_slot) {
+                       int32_t *seed_ids, uint64_t salt, int iterations, double h, double cutoff,
+                       R *out, R *weight, int run_slot) {
+  using V4 = typename Vec<R>::V4;
+  using V2 = typename Vec<R>::V2;
   if (D != 4 && D != 6)
     raise(MVS_EINVAL, "batched mean-shift supports 4-D (reflection) and 6-D (rigid) points");
   if (!(h > 0) || !(cutoff > 0) || !std::isfinite(h * cutoff) || iterations < 0)
     raise(MVS_EINVAL, "invalid mean-shift parameters");
   if (n > static_cast<uint64_t>(UINT32_MAX) || ns > static_cast<uint64_t>(UINT32_MAX))
     raise(MVS_EOVERFLOW, "too many points or seeds");
-  if (seed_idx && n == 0 && ns)
-    raise(MVS_EINVAL, "seed votes given without points");
+  if (seed_ids && ns > n)
+    raise(MVS_EINVAL, "more seed votes than votes");
   if (ns == 0)
     return;
   MsCtl *ctl = ms_ctl(s);
   double *partial = scratch_as<double>(s, kSlotMs1, 2 * kMaxD * kBboxBlocks);
   // per-seed statistics [0, ns) candidates, [ns, 2ns) in-support visits
   unsigned long long *stats = scratch_as<unsigned long long>(s, kSlotMs7, 2 * ns);
-  R *sorted = scratch_as<R>(s, kSlotMs0, static_cast<size_t>(D) * (n ? n : 1));
+  const uint64_t np = n ? n : 1;
+  char *sorted = static_cast<char *>(scratch(s, kSlotMs0, np * (sizeof(V4) + sizeof(V2))));
+  V4 *p4 = reinterpret_cast<V4 *>(sorted);
+  V2 *q2 = reinterpret_cast<V2 *>(sorted + np * sizeof(V4));
   uint64_t *off = scratch_as<uint64_t>(s, kSlotMs5, kMaxCells + 1);
   unsigned nb = 0;
   if (n) {
@@ -419,25 +524,36 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     MVS_LAUNCH(s, "ms_bbox", k_bbox<R>, nb, kThreads, 0, pts, n, D, partial);
   }
   MVS_LAUNCH(s, "ms_grid", k_grid_setup, 1, kWave, 0, partial, nb, D, cutoff * h, ctl);
+  uint32_t *sorted_ids = nullptr;
   if (n) {
     uint64_t *keys = scratch_as<uint64_t>(s, kSlotMs2, 2 * n);
     uint32_t *vals = scratch_as<uint32_t>(s, kSlotMs3, 2 * n);
     MVS_LAUNCH(s, "ms_cell_keys", k_cell_keys<R>, blocks_for(n, kThreads), kThreads, 0, pts, n,
                ctl, keys, vals);
     radix_sort_pairs_u64(s, keys, vals, keys + n, vals + n, n, kCellBits);
+    sorted_ids = vals + n;
     MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 2048, kThreads, 0, keys + n, n, ctl, off);
-    MVS_LAUNCH(s, "ms_gather", k_gather_sorted<R>, blocks_for(n, kThreads), kThreads, 0, pts, n,
-               D, vals + n, sorted);
+    if (D == 6)
+      MVS_LAUNCH(s, "ms_gather", (k_gather_sorted<R, 6>), blocks_for(n, kThreads), kThreads, 0,
+                 pts, n, ctl, sorted_ids, p4, q2);
+    else
+      MVS_LAUNCH(s, "ms_gather", (k_gather_sorted<R, 4>), blocks_for(n, kThreads), kThreads, 0,
+                 pts, n, ctl, sorted_ids, p4, q2);
   } else {
     MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 1, kThreads, 0, nullptr, 0, ctl, off);
   }
   // seeds in cell order
-  const uint32_t *order = nullptr;
-  if (n && ns > 1) {
+  const uint32_t *order = nullptr, *seed_pos = nullptr;
+  if (seed_ids) {
+    uint32_t *pos = scratch_as<uint32_t>(s, kSlotMs6, ns);
+    MVS_LAUNCH(s, "ms_seeds", k_pick_sorted, blocks_for(ns, kThreads), kThreads, 0, n, ns, salt,
+               sorted_ids, pos, seed_ids);
+    seed_pos = pos;
+  } else if (n && ns > 1) {
     uint64_t *skeys = scratch_as<uint64_t>(s, kSlotMs4, 2 * ns);
     uint32_t *svals = scratch_as<uint32_t>(s, kSlotMs6, 2 * ns);
     MVS_LAUNCH(s, "ms_seed_keys", k_seed_keys<R>, blocks_for(ns, kThreads), kThreads, 0, seeds,
-               seed_idx, pts, n, ns, ctl, skeys, svals);
+               ns, ctl, skeys, svals);
     radix_sort_pairs_u64(s, skeys, svals, skeys + ns, svals + ns, ns, kCellBits);
     order = svals + ns;
   }
@@ -447,11 +563,11 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   const unsigned logical = blocks_for(ns, kWavesPerBlock);
   const unsigned grid = ((logical + kXcds - 1) / kXcds) * kXcds;
   if (D == 6)
-    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 6>), grid, kThreads, 0, sorted, n, off, ctl, seeds,
-               seed_idx, pts, order, ns, logical, iterations, inv2h2, r2, fw, out, weight, stats);
+    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 6>), grid, kThreads, 0, p4, q2, off, ctl, seed_pos,
+               seeds, order, ns, logical, iterations, inv2h2, r2, fw, out, weight, stats);
   else
-    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 4>), grid, kThreads, 0, sorted, n, off, ctl, seeds,
-               seed_idx, pts, order, ns, logical, iterations, inv2h2, r2, fw, out, weight, stats);
+    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 4>), grid, kThreads, 0, p4, q2, off, ctl, seed_pos,
+               seeds, order, ns, logical, iterations, inv2h2, r2, fw, out, weight, stats);
   if (run_slot >= 0) {
     MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats, ns, ctl->sums + run_slot);
     MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats + ns, ns,
@@ -460,15 +576,15 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
 }
 
 void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,
-                    const float *seeds, const int32_t *seed_idx, int iterations, double h,
-                    double cutoff, float *out, float *weight, int run_slot) {
-  ms_batched<float>(s, D, n, pts, ns, seeds, seed_idx, iterations, h, cutoff, out, weight,
+                    const float *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
+                    double h, double cutoff, float *out, float *weight, int run_slot) {
+  ms_batched<float>(s, D, n, pts, ns, seeds, seed_ids, salt, iterations, h, cutoff, out, weight,
                     run_slot);
 }
 void ms_batched_f64(Session *s, int D, uint64_t n, const double *pts, uint64_t ns,
-                    const double *seeds, const int32_t *seed_idx, int iterations, double h,
-                    double cutoff, double *out, double *weight, int run_slot) {
-  ms_batched<double>(s, D, n, pts, ns, seeds, seed_idx, iterations, h, cutoff, out, weight,
+                    const double *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
+                    double h, double cutoff, double *out, double *weight, int run_slot) {
+  ms_batched<double>(s, D, n, pts, ns, seeds, seed_ids, salt, iterations, h, cutoff, out, weight,
                      run_slot);
 }
 void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4, const void *t3,
@@ -521,11 +637,11 @@ int mvs_dev_ms_batched(mvs_session *sp, int precision, int dim, uint64_t n, cons
     ms_begin(s);
     if (precision == MVS_FP32_GUARDED)
       ms_batched<float>(s, dim, n, static_cast<const float *>(pts), ns,
-                        static_cast<const float *>(seeds), nullptr, iterations, h, cutoff,
+                        static_cast<const float *>(seeds), nullptr, 0, iterations, h, cutoff,
                         static_cast<float *>(out), static_cast<float *>(weight), 0);
     else
       ms_batched<double>(s, dim, n, static_cast<const double *>(pts), ns,
-                         static_cast<const double *>(seeds), nullptr, iterations, h, cutoff,
+                         static_cast<const double *>(seeds), nullptr, 0, iterations, h, cutoff,
                          static_cast<double *>(out), static_cast<double *>(weight), 0);
     unsigned long long sums[4];
     ms_finish(s, sums);

@@ -63,12 +63,15 @@ uint64_t vote_total_checked(Session *s);
 // non-finite points and returns the four sums.
 void ms_begin(Session *s);
 void ms_finish(Session *s, unsigned long long sums[4]);
+// Seeds: explicit positions (D x ns SoA), or, with seed_ids, ns of the n
+// points drawn stratified over the cell-sorted order with `salt` (indices
+// written to seed_ids, in cell order).
 void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,
-                    const float *seeds, const int32_t *seed_idx, int iterations, double h,
-                    double cutoff, float *out, float *weight, int run_slot);
+                    const float *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
+                    double h, double cutoff, float *out, float *weight, int run_slot);
 void ms_batched_f64(Session *s, int D, uint64_t n, const double *pts, uint64_t ns,
-                    const double *seeds, const int32_t *seed_idx, int iterations, double h,
-                    double cutoff, double *out, double *weight, int run_slot);
+                    const double *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
+                    double h, double cutoff, double *out, double *weight, int run_slot);
 // Compact votes to mean-shift points; mirror normals are 3 rows of n3_stride.
 void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4, const void *t3,
                       uint64_t nrefl, const void *n3, uint64_t n3_stride, const void *d1,

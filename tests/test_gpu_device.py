@@ -222,10 +222,10 @@ def test_fused_meanshift_matches_oracle(cfg1, oracle):
     torch.cuda.synchronize()
     assert ms.rigid_seeds + ms.refl_seeds == 300 and ms.refl_seeds > 0
     seeds = seeds.cpu().numpy()
-    # stratified draws: distinct, ascending, inside each stream
+    # stratified draws over the cell order: distinct, inside each stream
     sr, sf = seeds[:ms.rigid_seeds], seeds[ms.rigid_seeds:]
-    assert np.all(np.diff(sr) > 0) and sr[0] >= 0 and sr[-1] < 2 * P
-    assert np.all(np.diff(sf) > 0) and sf[0] >= 0 and sf[-1] < R
+    assert len(np.unique(sr)) == len(sr) and sr.min() >= 0 and sr.max() < 2 * P
+    assert len(np.unique(sf)) == len(sf) and sf.min() >= 0 and sf.max() < R
     P6 = rp.cpu().numpy()
     P4 = fpts.cpu().numpy()
     # axis-angle points reproduce the vote rotations (closed form: Rodrigues)
