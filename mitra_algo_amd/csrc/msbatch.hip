@@ -9,16 +9,18 @@
 //   w = exp(-d^2 / 2h^2) - exp(-cutoff^2 / 2)   for d < cutoff*h,
 // so the support boundary carries no weight jump.
 //
-// Grid: cells of side >= cutoff*h over the three coordinates with the
-// widest extent (for height fields: two translation components and the
-// rotation about z), so a cell holds few points outside the support.
-// Points are radix-sorted by cell (stable: sums are deterministic) and
-// indexed by per-cell offsets found by binary search (no atomics).  With the
-// first grid axis fastest, the three cells of one row are contiguous, so a
-// query visits nine contiguous point ranges.  One wavefront per seed runs
-// all iterations in registers; lanes stride the ranges and reduce with
-// shuffles.  Seeds run in cell order and blocks are mapped so that each XCD
-// takes a contiguous run of seeds: neighbouring queries share that XCD's L2.
+// Grid: cells of side >= cutoff*h over the three coordinates of largest
+// variance (for height fields: the two horizontal translation components and
+// the rotation about z for rigid votes, plane offset and the horizontal
+// normal components for mirror votes), so a cell holds few points outside
+// the support.  Points are radix-sorted by cell (stable: sums are
+// deterministic) and indexed by per-cell offsets found by binary search (no
+// atomics).  With the first grid axis fastest, the three cells of one row
+// are contiguous, so a query visits nine contiguous point ranges.  One
+// wavefront per seed runs all iterations in registers; lanes stride the
+// ranges and reduce with shuffles.  Seeds run in cell order and blocks are
+// mapped so that each XCD takes a contiguous run of seeds: neighbouring
+// queries share that XCD's L2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,35 +70,46 @@ __global__ void k_points_refl(const R *n3, uint64_t stride, const R *d1, uint64_
   pts[3 * n + i] = n3[2 * stride + i];
 }
 
-// per-block min/max of every coordinate: partial[b*2*kMaxD + k] (lo), + kMaxD (hi)
-template <class R>
-__global__ void k_bbox(const R *pts, uint64_t n, int D, double *partial) {
-  __shared__ double s[2 * kMaxD][kThreads];
-  double lo[kMaxD], hi[kMaxD];
+// Per-block box and moments of every coordinate: partial[b * kPart + k]
+// holds min (k < kMaxD), max, sum and sum of squares (FP64) in blocks of
+// kMaxD.
+constexpr int kPart = 4 * kMaxD;
+template <class R, int D>
+__global__ void __launch_bounds__(kThreads) k_bbox(const R *pts, uint64_t n, double *partial) {
+  __shared__ double sh[kWavesPerBlock][kPart];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  double v[kPart];
   for (int k = 0; k < kMaxD; ++k)
-    lo[k] = INFINITY, hi[k] = -INFINITY;
+    v[k] = INFINITY, v[kMaxD + k] = -INFINITY, v[2 * kMaxD + k] = 0, v[3 * kMaxD + k] = 0;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * kThreads)
     for (int k = 0; k < D; ++k) {
-      double v = static_cast<double>(pts[k * n + i]);
-      lo[k] = fmin(lo[k], v);
-      hi[k] = fmax(hi[k], v);
+      const double x = static_cast<double>(pts[k * n + i]);
+      v[k] = fmin(v[k], x);
+      v[kMaxD + k] = fmax(v[kMaxD + k], x);
+      v[2 * kMaxD + k] += x;
+      v[3 * kMaxD + k] += x * x;
     }
-  for (int k = 0; k < kMaxD; ++k) {
-    s[k][threadIdx.x] = lo[k];
-    s[kMaxD + k][threadIdx.x] = hi[k];
-  }
+  for (int o = kWave / 2; o > 0; o >>= 1)
+    for (int k = 0; k < kMaxD; ++k) {
+      v[k] = fmin(v[k], __shfl_xor(v[k], o, kWave));
+      v[kMaxD + k] = fmax(v[kMaxD + k], __shfl_xor(v[kMaxD + k], o, kWave));
+      v[2 * kMaxD + k] += __shfl_xor(v[2 * kMaxD + k], o, kWave);
+      v[3 * kMaxD + k] += __shfl_xor(v[3 * kMaxD + k], o, kWave);
+    }
+  if (lane == 0)
+    for (int k = 0; k < kPart; ++k)
+      sh[wave][k] = v[k];
   __syncthreads();
-  for (int o = kThreads / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o)
-      for (int k = 0; k < kMaxD; ++k) {
-        s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + o]);
-        s[kMaxD + k][threadIdx.x] = fmax(s[kMaxD + k][threadIdx.x], s[kMaxD + k][threadIdx.x + o]);
-      }
-    __syncthreads();
+  if (threadIdx.x < kPart) {
+    const int k = threadIdx.x;
+    double r = sh[0][k];
+    for (int w = 1; w < kWavesPerBlock; ++w) {
+      const double x = sh[w][k];
+      r = k < kMaxD ? fmin(r, x) : (k < 2 * kMaxD ? fmax(r, x) : r + x);
+    }
+    partial[blockIdx.x * kPart + k] = r;
   }
-  if (threadIdx.x < 2 * kMaxD)
-    partial[blockIdx.x * 2 * kMaxD + threadIdx.x] = s[threadIdx.x][0];
 }
 
 // perm lists the point coordinates in storage (slot) order: the three grid
@@ -111,7 +124,7 @@ struct Grid {
 };
 constexpr int kCellBits = 22;  // at most 2^22 cells: three 8-bit radix passes
 constexpr uint64_t kMaxCells = 1ULL << kCellBits;
-constexpr unsigned kBboxBlocks = 1024;
+constexpr unsigned kBboxBlocks = 256;
 
 // Device-side control block of one mean-shift call: the grid, visit sums of
 // up to two runs and the non-finite-input flag, read back once at the end.
@@ -121,26 +134,39 @@ struct MsCtl {
   unsigned bad;
 };
 
-// One wavefront: reduces the per-block boxes and lays the grid on the three
-// widest coordinates (ties: lower index first), cells of side >= cell0
-// doubled until at most kMaxCells.  No points: a single cell.
-__global__ void k_grid_setup(const double *partial, unsigned nb, int D, double cell0, MsCtl *ctl) {
+// One wavefront: reduces the per-block boxes and moments and lays the grid
+// on the three coordinates of largest spread (variance; ties: lower index
+// first).  Spread, not extent: a few outlying votes can stretch a coordinate
+// whose votes are otherwise concentrated.  Cells have side >= cell0, doubled
+// until at most kMaxCells.  No points: a single cell.
+template <int D>
+__global__ void k_grid_setup(const double *partial, unsigned nb, uint64_t n, double cell0,
+                             MsCtl *ctl) {
   const int lane = threadIdx.x;
-  double lo[kMaxD], hi[kMaxD];
+  double lo[kMaxD], hi[kMaxD], sum[kMaxD], sq[kMaxD];
   for (int k = 0; k < kMaxD; ++k)
-    lo[k] = INFINITY, hi[k] = -INFINITY;
+    lo[k] = INFINITY, hi[k] = -INFINITY, sum[k] = 0, sq[k] = 0;
   for (unsigned b = lane; b < nb; b += kWave)
     for (int k = 0; k < D; ++k) {
-      lo[k] = fmin(lo[k], partial[b * 2 * kMaxD + k]);
-      hi[k] = fmax(hi[k], partial[b * 2 * kMaxD + kMaxD + k]);
+      lo[k] = fmin(lo[k], partial[b * kPart + k]);
+      hi[k] = fmax(hi[k], partial[b * kPart + kMaxD + k]);
+      sum[k] += partial[b * kPart + 2 * kMaxD + k];
+      sq[k] += partial[b * kPart + 3 * kMaxD + k];
     }
   for (int k = 0; k < D; ++k)
     for (int o = kWave / 2; o > 0; o >>= 1) {
       lo[k] = fmin(lo[k], __shfl_xor(lo[k], o, kWave));
       hi[k] = fmax(hi[k], __shfl_xor(hi[k], o, kWave));
+      sum[k] += __shfl_xor(sum[k], o, kWave);
+      sq[k] += __shfl_xor(sq[k], o, kWave);
     }
   if (lane != 0)
     return;
+  double var[kMaxD];
+  for (int k = 0; k < D; ++k) {
+    const double mean = n ? sum[k] / static_cast<double>(n) : 0.0;
+    var[k] = n ? sq[k] / static_cast<double>(n) - mean * mean : 0.0;
+  }
   Grid g{};
   g.axis[0] = 0, g.axis[1] = 1, g.axis[2] = 2;
   for (int k = 0; k < kMaxD; ++k)
@@ -154,34 +180,39 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, int D, double c
   if (nb && !finite)
     atomicOr(&ctl->bad, 1u);
   if (nb && finite) {
-    // stable selection of the three widest extents
-    int idx[kMaxD];
-    for (int k = 0; k < D; ++k)
-      idx[k] = k;
-    for (int k = 1; k < D; ++k) {
-      int v = idx[k], j = k;
-      while (j > 0 && hi[idx[j - 1]] - lo[idx[j - 1]] < hi[v] - lo[v]) {
-        idx[j] = idx[j - 1];
-        --j;
-      }
-      idx[j] = v;
+    // the three largest variances (ties: lower index first), as a bit mask;
+    // fixed-trip loops only, so everything stays in registers
+    unsigned sel = 0;
+    for (int k = 0; k < D; ++k) {
+      int rank = 0;
+      for (int j = 0; j < D; ++j)
+        rank += var[j] > var[k] || (var[j] == var[k] && j < k);
+      sel |= rank < 3 ? 1u << k : 0u;
     }
-    int ax[3] = {idx[0], idx[1], idx[2]};
-    for (int k = 1; k < 3; ++k)
-      for (int j = k; j > 0 && ax[j - 1] > ax[j]; --j) {
-        int t = ax[j];
-        ax[j] = ax[j - 1];
-        ax[j - 1] = t;
-      }
+    unsigned rest = ((1u << D) - 1) & ~sel;
+    int perm[kMaxD] = {0, 1, 2, 3, 4, 5};
+    for (int k = 0; k < 3; ++k) {
+      perm[k] = __builtin_ctz(sel);
+      sel &= sel - 1;
+    }
+    for (int k = 3; k < D; ++k) {
+      perm[k] = __builtin_ctz(rest);
+      rest &= rest - 1;
+    }
+    double alo[3], ext[3];
+    for (int k = 0; k < 3; ++k) {
+      alo[k] = 0, ext[k] = 0;
+      for (int j = 0; j < D; ++j)
+        if (perm[k] == j)
+          alo[k] = lo[j], ext[k] = hi[j] - lo[j];
+    }
     double cell = cell0;
     for (;;) {
       uint64_t cells = 1;
       for (int k = 0; k < 3; ++k) {
-        double d = floor((hi[ax[k]] - lo[ax[k]]) / cell) + 1;
+        const double d = floor(ext[k] / cell) + 1;
         g.dims[k] = d < 2147483647.0 ? static_cast<int>(d) : 2147483647;
-        cells = cells * static_cast<uint64_t>(g.dims[k]);
-        if (cells > kMaxCells)
-          break;
+        cells = cells > kMaxCells ? cells : cells * static_cast<uint64_t>(g.dims[k]);
       }
       if (cells <= kMaxCells) {
         g.ncells = cells;
@@ -190,13 +221,11 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, int D, double c
       cell *= 2;
     }
     for (int k = 0; k < 3; ++k) {
-      g.axis[k] = ax[k];
-      g.perm[k] = ax[k];
-      g.lo[k] = lo[ax[k]];
+      g.axis[k] = perm[k];
+      g.lo[k] = alo[k];
     }
-    for (int k = 0, r = 3; k < D; ++k)
-      if (k != ax[0] && k != ax[1] && k != ax[2])
-        g.perm[r++] = k;
+    for (int k = 0; k < kMaxD; ++k)
+      g.perm[k] = perm[k];
     g.cell = cell;
   }
   ctl->grid = g;
@@ -310,15 +339,11 @@ __global__ void k_seed_keys(const R *seeds, uint64_t ns, const MsCtl *ctl, uint6
   vals[s] = static_cast<uint32_t>(s);
 }
 
-// Per-wave ranges of the 3 x 3 neighbourhood rows (each row: three cells
-// contiguous in the sorted order), flattened across the lanes.
-struct RowRanges {
-  uint32_t beg[9];
-  uint32_t pre[10];
-};
-
 // One wavefront per seed.  Seeds are sorted points (seed_pos, already in
 // cell order) or explicit positions (seeds, D x ns SoA, run in `order`).
+// Per iteration the wave walks the nine rows of the 3 x 3 neighbourhood
+// (each row three cells, contiguous in the sorted order) with the records of
+// the next 64 candidates in flight while the current ones are tested.
 // All iterations run in registers; a seed whose update leaves every
 // coordinate unchanged has reached a fixed point of the iteration and stops
 // (the remaining iterations would repeat it bit for bit).
@@ -329,7 +354,6 @@ __global__ void __launch_bounds__(kThreads)
                const R *seeds, const uint32_t *order, uint64_t ns, unsigned logical_blocks,
                int iterations, R inv2h2, R r2, R floor_w, R *out, R *weight,
                unsigned long long *stats) {
-  __shared__ RowRanges rows[kWavesPerBlock];
   const int lane = threadIdx.x & (kWave - 1);
   // XCD-aware block order: dies take contiguous runs of (cell-ordered) seeds
   const unsigned per = (logical_blocks + kXcds - 1) / kXcds;
@@ -339,7 +363,6 @@ __global__ void __launch_bounds__(kThreads)
   const uint64_t slot = static_cast<uint64_t>(logical) * kWavesPerBlock + threadIdx.x / kWave;
   if (slot >= ns)
     return;
-  RowRanges &W = rows[threadIdx.x / kWave];
   const Grid g = ctl->grid;
   const uint64_t s = order ? order[slot] : slot;
   R m[D];  // slot order
@@ -363,66 +386,68 @@ __global__ void __launch_bounds__(kThreads)
     const int cy = cell_of(static_cast<double>(m[1]), g.lo[1], g.cell, g.dims[1]);
     const int cz = cell_of(static_cast<double>(m[2]), g.lo[2], g.cell, g.dims[2]);
     const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.dims[0] ? cx + 1 : cx;
-    uint32_t beg = 0, len = 0;
+    // lanes 0..8 fetch the ranges of the 3 x 3 rows (three cells each)
+    uint32_t rb = 0, re = 0;
     if (lane < 9) {
       const int z = cz + dz, y = cy + dy;
       if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
         const uint64_t row = (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
-        const uint64_t b = offsets[row + x0];
-        beg = static_cast<uint32_t>(b);
-        len = static_cast<uint32_t>(offsets[row + x1 + 1] - b);
+        rb = static_cast<uint32_t>(offsets[row + x0]);
+        re = static_cast<uint32_t>(offsets[row + x1 + 1]);
       }
     }
-    uint32_t x = len;  // inclusive prefix over lanes 0..8
-    for (int o = 1; o < 16; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o, kWave);
-      if (lane >= o)
-        x += y;
-    }
-    const uint32_t total = __shfl(x, 8, kWave);
-    if (lane < 9) {
-      W.beg[lane] = beg;
-      W.pre[lane] = x - len;
-    }
-    if (lane == 8)
-      W.pre[9] = x;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    cand += total;
     R acc[D];
     for (int k = 0; k < D; ++k)
       acc[k] = 0;
     R wl = 0;
-    int r = 0;
-    for (uint32_t c = lane; c < total; c += kWave) {
-      while (c >= W.pre[r + 1])
-        ++r;
-      const uint64_t j = W.beg[r] + (c - W.pre[r]);
-      const typename Vec<R>::V4 p = p4[j];
-      R d[D];
-      d[0] = p.x - m[0];
-      d[1] = p.y - m[1];
-      d[2] = p.z - m[2];
-      const R dg = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-      if (dg < r2) {
-        d[3] = p.w - m[3];
-        R dd = dg + d[3] * d[3];
-        if (D == 6) {
-          const typename Vec<R>::V2 q = q2[j];
-          d[4 % D] = q.x - m[4 % D];
-          d[5 % D] = q.y - m[5 % D];
-          dd += d[4 % D] * d[4 % D] + d[5 % D] * d[5 % D];
+    for (int r = 0; r < 9; ++r) {
+      const uint32_t beg = __builtin_amdgcn_readlane(rb, r);
+      const uint32_t end = __builtin_amdgcn_readlane(re, r);
+      cand += end - beg;
+      // software pipeline: the next batch's records load while this one runs
+      uint32_t j = beg + lane;
+      typename Vec<R>::V4 p{};
+      typename Vec<R>::V2 q{};
+      if (j < end) {
+        p = p4[j];
+        if (D == 6)
+          q = q2[j];
+      }
+      for (uint32_t b0 = beg; b0 < end; b0 += kWave) {
+        const uint32_t jn = j + kWave;
+        typename Vec<R>::V4 pn{};
+        typename Vec<R>::V2 qn{};
+        if (jn < end) {
+          pn = p4[jn];
+          if (D == 6)
+            qn = q2[jn];
         }
-        if (dd < r2) {
-          const R w = ms_exp<R>(-dd * inv2h2) - floor_w;
-          wl += w;
+        if (j < end) {
+          R d[D];
+          d[0] = p.x - m[0];
+          d[1] = p.y - m[1];
+          d[2] = p.z - m[2];
+          d[3] = p.w - m[3];
+          if (D == 6) {
+            d[4 % D] = q.x - m[4 % D];
+            d[5 % D] = q.y - m[5 % D];
+          }
+          R dd = 0;
           for (int k = 0; k < D; ++k)
-            acc[k] += w * d[k];
-          ++inside;
+            dd += d[k] * d[k];
+          if (dd < r2) {
+            const R w = ms_exp<R>(-dd * inv2h2) - floor_w;
+            wl += w;
+            for (int k = 0; k < D; ++k)
+              acc[k] += w * d[k];
+            ++inside;
+          }
         }
+        p = pn;
+        q = qn;
+        j = jn;
       }
     }
-    __builtin_amdgcn_wave_barrier();  // rows are rewritten next iteration
     wl = wave_sum(wl);
     for (int k = 0; k < D; ++k)
       acc[k] = wave_sum(acc[k]);
@@ -470,7 +495,7 @@ __global__ void k_sum_u64(const unsigned long long *a, uint64_t n, unsigned long
 // Control block of the session's mean-shift calls (slot kSlotMs1, after the
 // per-block box partials).
 static MsCtl *ms_ctl(Session *s) {
-  const size_t part = 2 * kMaxD * kBboxBlocks * sizeof(double);
+  const size_t part = kPart * kBboxBlocks * sizeof(double);
   char *base = static_cast<char *>(scratch(s, kSlotMs1, part + sizeof(MsCtl)));
   return reinterpret_cast<MsCtl *>(base + part);
 }
@@ -508,7 +533,7 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   if (ns == 0)
     return;
   MsCtl *ctl = ms_ctl(s);
-  double *partial = scratch_as<double>(s, kSlotMs1, 2 * kMaxD * kBboxBlocks);
+  double *partial = scratch_as<double>(s, kSlotMs1, kPart * kBboxBlocks);
   // per-seed statistics [0, ns) candidates, [ns, 2ns) in-support visits
   unsigned long long *stats = scratch_as<unsigned long long>(s, kSlotMs7, 2 * ns);
   const uint64_t np = n ? n : 1;
@@ -521,9 +546,15 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     nb = blocks_for(n, kThreads);
     if (nb > kBboxBlocks)
       nb = kBboxBlocks;
-    MVS_LAUNCH(s, "ms_bbox", k_bbox<R>, nb, kThreads, 0, pts, n, D, partial);
+    if (D == 6)
+      MVS_LAUNCH(s, "ms_bbox", (k_bbox<R, 6>), nb, kThreads, 0, pts, n, partial);
+    else
+      MVS_LAUNCH(s, "ms_bbox", (k_bbox<R, 4>), nb, kThreads, 0, pts, n, partial);
   }
-  MVS_LAUNCH(s, "ms_grid", k_grid_setup, 1, kWave, 0, partial, nb, D, cutoff * h, ctl);
+  if (D == 6)
+    MVS_LAUNCH(s, "ms_grid", k_grid_setup<6>, 1, kWave, 0, partial, nb, n, cutoff * h, ctl);
+  else
+    MVS_LAUNCH(s, "ms_grid", k_grid_setup<4>, 1, kWave, 0, partial, nb, n, cutoff * h, ctl);
   uint32_t *sorted_ids = nullptr;
   if (n) {
     uint64_t *keys = scratch_as<uint64_t>(s, kSlotMs2, 2 * n);

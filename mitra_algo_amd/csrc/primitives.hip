@@ -55,10 +55,27 @@ __global__ void __launch_bounds__(kScanThreads)
     partial[blockIdx.x] = total;
 }
 
-template <class TIn>
+// Writes the exclusive scan of one tile.  The tile's offset is
+// partial[blockIdx.x] (kSelfPrefix false: already scanned totals; nullptr
+// for a single tile) or, with kSelfPrefix, the sum of the raw tile totals
+// partial[0 .. blockIdx.x), which the block adds up itself: two launches per
+// scan for up to kSelfPrefixTiles tiles.
+constexpr unsigned kSelfPrefixTiles = 8192;
+template <class TIn, bool kSelfPrefix>
 __global__ void __launch_bounds__(kScanThreads)
-    k_scan_apply(const TIn *in, uint64_t n, const uint64_t *partial_scanned, uint64_t *out) {
+    k_scan_apply(const TIn *in, uint64_t n, const uint64_t *partial, uint64_t *out) {
   __shared__ uint64_t lds[kScanThreads / kWave + 1];
+  uint64_t tile_off = 0;
+  if (kSelfPrefix) {
+    uint64_t pre = 0;
+    for (unsigned i = threadIdx.x; i < blockIdx.x; i += kScanThreads)
+      pre += partial[i];
+    uint64_t tot;
+    (void)block_exclusive_scan<uint64_t>(pre, lds, tot);
+    tile_off = tot;
+  } else if (partial) {
+    tile_off = partial[blockIdx.x];
+  }
   uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
   uint64_t v[kScanItems];
   uint64_t sum = 0;
@@ -67,20 +84,22 @@ __global__ void __launch_bounds__(kScanThreads)
     sum += v[i];
   }
   uint64_t total;
-  uint64_t run = block_exclusive_scan<uint64_t>(sum, lds, total) + partial_scanned[blockIdx.x];
+  uint64_t run = block_exclusive_scan<uint64_t>(sum, lds, total) + tile_off;
   for (int i = 0; i < kScanItems; ++i) {
     if (base + i < n)
       out[base + i] = run;
     run += v[i];
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
-    out[n] = partial_scanned[blockIdx.x] + total;
+    out[n] = tile_off + total;
 }
 
 __global__ void k_zero_u64(uint64_t *p) { *p = 0; }
 
-// Recursive three-phase scan.  `pool` holds the partial arrays of all levels
-// (carved from one scratch slot, sized by scan_pool_words).
+// Exclusive scan: one launch for one tile, two (tile totals, then tiles that
+// add their own prefix) up to kSelfPrefixTiles tiles, recursive beyond.
+// `pool` holds the partial arrays of all levels (carved from one scratch
+// slot, sized by scan_pool_words).
 template <class TIn>
 void exclusive_scan_level(Session *s, const TIn *in, uint64_t *out, uint64_t n, uint64_t *pool) {
   if (n == 0) {
@@ -90,12 +109,18 @@ void exclusive_scan_level(Session *s, const TIn *in, uint64_t *out, uint64_t n, 
   unsigned nb = blocks_for(n, kScanTile);
   uint64_t *part = pool, *part_scan = pool + nb;
   if (nb == 1) {
-    MVS_LAUNCH(s, "scan_zero", k_zero_u64, 1, 1, 0, part_scan);
-  } else {
-    MVS_LAUNCH(s, "scan_reduce", (k_scan_reduce<TIn>), nb, kScanThreads, 0, in, n, part);
-    exclusive_scan_level<uint64_t>(s, part, part_scan, nb, pool + 2 * nb + 1);
+    MVS_LAUNCH(s, "scan_apply", (k_scan_apply<TIn, false>), 1, kScanThreads, 0, in, n, nullptr,
+               out);
+    return;
   }
-  MVS_LAUNCH(s, "scan_apply", (k_scan_apply<TIn>), nb, kScanThreads, 0, in, n, part_scan, out);
+  MVS_LAUNCH(s, "scan_reduce", (k_scan_reduce<TIn>), nb, kScanThreads, 0, in, n, part);
+  if (nb <= kSelfPrefixTiles) {
+    MVS_LAUNCH(s, "scan_apply", (k_scan_apply<TIn, true>), nb, kScanThreads, 0, in, n, part, out);
+    return;
+  }
+  exclusive_scan_level<uint64_t>(s, part, part_scan, nb, pool + 2 * nb + 1);
+  MVS_LAUNCH(s, "scan_apply", (k_scan_apply<TIn, false>), nb, kScanThreads, 0, in, n, part_scan,
+             out);
 }
 uint64_t scan_pool_words(uint64_t n) {
   uint64_t words = 0;
