@@ -38,6 +38,12 @@ constexpr int kWavesPerBlock = kThreads / kWave;
 constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, blocks dealt round-robin
 constexpr int kMaxD = 6;
 
+// Fused multiply-add, spelled out: the library builds without contraction
+// (the pair and vote arithmetic must round as written), the mean-shift sums
+// have no such constraint.
+__device__ inline float ms_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double ms_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <class R> __device__ inline R ms_exp(R x);
 template <> __device__ inline float ms_exp<float>(float x) { return __expf(x); }
 template <> __device__ inline double ms_exp<double>(double x) { return exp(x); }
@@ -125,6 +131,8 @@ struct Grid {
 constexpr int kCellBits = 22;  // at most 2^22 cells: three 8-bit radix passes
 constexpr uint64_t kMaxCells = 1ULL << kCellBits;
 constexpr unsigned kBboxBlocks = 256;
+// below this many seeds a seed gets a whole 256-lane block (four waves)
+constexpr uint64_t kWideSeeds = 8192;
 
 // Device-side control block of one mean-shift call: the grid, visit sums of
 // up to two runs and the non-finite-input flag, read back once at the end.
@@ -339,30 +347,42 @@ __global__ void k_seed_keys(const R *seeds, uint64_t ns, const MsCtl *ctl, uint6
   vals[s] = static_cast<uint32_t>(s);
 }
 
-// One wavefront per seed.  Seeds are sorted points (seed_pos, already in
-// cell order) or explicit positions (seeds, D x ns SoA, run in `order`).
-// Per iteration the wave walks the nine rows of the 3 x 3 neighbourhood
-// (each row three cells, contiguous in the sorted order) with the records of
-// the next 64 candidates in flight while the current ones are tested.
-// All iterations run in registers; a seed whose update leaves every
-// coordinate unchanged has reached a fixed point of the iteration and stops
-// (the remaining iterations would repeat it bit for bit).
-template <class R, int D>
+// Mean-shift queries.  A seed is served by W wavefronts (W = 1: four seeds
+// per 256-lane block; W = 4: one seed per block, for few seeds with many
+// candidates).  Seeds are sorted points (seed_pos, already in cell order) or
+// explicit positions (seeds, D x ns SoA, run in `order`).  All iterations
+// run in registers.  Per iteration the nine rows of the 3 x 3 neighbourhood
+// (each row three cells, contiguous in the sorted order) are walked as one
+// flattened sequence of candidates: the row bounds live in scalar registers
+// and each lane finds its row with eight compares, so short rows do not
+// leave lanes idle; the next batch's records are in flight while the current
+// batch is tested, and batch k goes to wave k mod W.  With W = 4 the waves
+// add their partial sums through LDS in a fixed order, so every wave holds
+// the same new mean.  A seed whose update leaves every coordinate unchanged
+// has reached a fixed point of the iteration and stops (further iterations
+// would repeat it bit for bit).
+template <class R, int D, int W>
 __global__ void __launch_bounds__(kThreads)
     k_ms_query(const typename Vec<R>::V4 *p4, const typename Vec<R>::V2 *q2,
                const uint64_t *offsets, const MsCtl *ctl, const uint32_t *seed_pos,
                const R *seeds, const uint32_t *order, uint64_t ns, unsigned logical_blocks,
                int iterations, R inv2h2, R r2, R floor_w, R *out, R *weight,
                unsigned long long *stats) {
+  static_assert(W == 1 || W == kWavesPerBlock, "one seed per wave or per block");
+  __shared__ R part[kWavesPerBlock][D + 1];
   const int lane = threadIdx.x & (kWave - 1);
+  // wave index, made visibly wave-uniform so the row cursor stays scalar
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int wsub = W == 1 ? 0 : wave;  // this wave's share of the seed's batches
   // XCD-aware block order: dies take contiguous runs of (cell-ordered) seeds
   const unsigned per = (logical_blocks + kXcds - 1) / kXcds;
   const unsigned logical = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
   if (logical >= logical_blocks)
     return;
-  const uint64_t slot = static_cast<uint64_t>(logical) * kWavesPerBlock + threadIdx.x / kWave;
+  const uint64_t slot = W == 1 ? static_cast<uint64_t>(logical) * kWavesPerBlock + wave
+                               : static_cast<uint64_t>(logical);
   if (slot >= ns)
-    return;
+    return;  // W = 4: the whole block leaves together
   const Grid g = ctl->grid;
   const uint64_t s = order ? order[slot] : slot;
   R m[D];  // slot order
@@ -387,70 +407,119 @@ __global__ void __launch_bounds__(kThreads)
     const int cz = cell_of(static_cast<double>(m[2]), g.lo[2], g.cell, g.dims[2]);
     const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.dims[0] ? cx + 1 : cx;
     // lanes 0..8 fetch the ranges of the 3 x 3 rows (three cells each)
-    uint32_t rb = 0, re = 0;
+    uint32_t rb = 0, len = 0;
     if (lane < 9) {
       const int z = cz + dz, y = cy + dy;
       if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
         const uint64_t row = (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
-        rb = static_cast<uint32_t>(offsets[row + x0]);
-        re = static_cast<uint32_t>(offsets[row + x1 + 1]);
+        const uint64_t b = offsets[row + x0];
+        rb = static_cast<uint32_t>(b);
+        len = static_cast<uint32_t>(offsets[row + x1 + 1] - b);
       }
     }
+    uint32_t incl = len;  // inclusive prefix over lanes 0..8
+    for (int o = 1; o < 16; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o, kWave);
+      if (lane >= o)
+        incl += y;
+    }
+    // row r covers flattened [start_r, start_r + len_r), start_r = incl_{r-1}
+    // (lane r holds start_r in `startv`, begin_r - start_r in `basev`); a
+    // flattened index c maps to the record c + base of its row
+    const uint32_t startv = __shfl_up(incl, 1, kWave) * (lane > 0 ? 1u : 0u);
+    const uint32_t basev = rb - startv;
+    const uint32_t total = __builtin_amdgcn_readlane(incl, 8);
+    if (wsub == 0)
+      cand += total;
     R acc[D];
     for (int k = 0; k < D; ++k)
       acc[k] = 0;
     R wl = 0;
-    for (int r = 0; r < 9; ++r) {
-      const uint32_t beg = __builtin_amdgcn_readlane(rb, r);
-      const uint32_t end = __builtin_amdgcn_readlane(re, r);
-      cand += end - beg;
-      // software pipeline: the next batch's records load while this one runs
-      uint32_t j = beg + lane;
-      typename Vec<R>::V4 p{};
-      typename Vec<R>::V2 q{};
-      if (j < end) {
-        p = p4[j];
-        if (D == 6)
-          q = q2[j];
+    uint32_t in32 = 0;
+    // scalar row cursor: rc is the row of the window's first index; lanes
+    // past the next row start(s) take those rows' bases
+    int rc = 0;
+    auto locate = [&](uint32_t c0, uint32_t c) {
+      while (rc < 8 && static_cast<uint32_t>(__builtin_amdgcn_readlane(startv, rc + 1)) <= c0)
+        ++rc;
+      uint32_t b = __builtin_amdgcn_readlane(basev, rc);
+      for (int r = rc + 1; r < 9; ++r) {
+        const uint32_t st = __builtin_amdgcn_readlane(startv, r);
+        if (st > c0 + (kWave - 1))
+          break;
+        b = c >= st ? static_cast<uint32_t>(__builtin_amdgcn_readlane(basev, r)) : b;
       }
-      for (uint32_t b0 = beg; b0 < end; b0 += kWave) {
-        const uint32_t jn = j + kWave;
-        typename Vec<R>::V4 pn{};
-        typename Vec<R>::V2 qn{};
-        if (jn < end) {
-          pn = p4[jn];
-          if (D == 6)
-            qn = q2[jn];
-        }
-        if (j < end) {
-          R d[D];
-          d[0] = p.x - m[0];
-          d[1] = p.y - m[1];
-          d[2] = p.z - m[2];
-          d[3] = p.w - m[3];
-          if (D == 6) {
-            d[4 % D] = q.x - m[4 % D];
-            d[5 % D] = q.y - m[5 % D];
-          }
-          R dd = 0;
-          for (int k = 0; k < D; ++k)
-            dd += d[k] * d[k];
-          if (dd < r2) {
-            const R w = ms_exp<R>(-dd * inv2h2) - floor_w;
-            wl += w;
-            for (int k = 0; k < D; ++k)
-              acc[k] += w * d[k];
-            ++inside;
-          }
-        }
-        p = pn;
-        q = qn;
-        j = jn;
+      return b + c;
+    };
+    constexpr uint32_t kStep = static_cast<uint32_t>(W) * kWave;
+    // two register sets: batch k + 1 loads while batch k is tested.  Loads
+    // are unconditional (lanes past the end re-read the last candidate), so
+    // the compiler can wait for one batch while the next is in flight.
+    const uint32_t last = total ? total - 1 : 0;
+    auto fetch = [&](uint32_t cb, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
+      const uint32_t cw = cb < last ? cb : last;
+      const uint32_t c = cb + lane < last ? cb + lane : last;
+      const uint32_t j = locate(cw, c);
+      p = p4[j];
+      if (D == 6)
+        q = q2[j];
+    };
+    auto test = [&](uint32_t cb, const typename Vec<R>::V4 &p, const typename Vec<R>::V2 &q) {
+      if (cb + lane >= total)
+        return;
+      R d[D];
+      d[0] = p.x - m[0];
+      d[1] = p.y - m[1];
+      d[2] = p.z - m[2];
+      d[3] = p.w - m[3];
+      if (D == 6) {
+        d[4 % D] = q.x - m[4 % D];
+        d[5 % D] = q.y - m[5 % D];
+      }
+      R dd = d[0] * d[0];
+      for (int k = 1; k < D; ++k)
+        dd = ms_fma(d[k], d[k], dd);
+      if (dd < r2) {
+        const R w = ms_exp<R>(-dd * inv2h2) - floor_w;
+        wl += w;
+        for (int k = 0; k < D; ++k)
+          acc[k] = ms_fma(w, d[k], acc[k]);
+        ++in32;
+      }
+    };
+    uint32_t c0 = static_cast<uint32_t>(wsub) * kWave;
+    if (c0 < total) {
+      typename Vec<R>::V4 pa, pb;
+      typename Vec<R>::V2 qa{}, qb{};
+      fetch(c0, pa, qa);
+      for (; c0 < total; c0 += 2 * kStep) {
+        fetch(c0 + kStep, pb, qb);
+        test(c0, pa, qa);
+        fetch(c0 + 2 * kStep, pa, qa);
+        test(c0 + kStep, pb, qb);
       }
     }
+    inside += in32;
     wl = wave_sum(wl);
     for (int k = 0; k < D; ++k)
       acc[k] = wave_sum(acc[k]);
+    if (W > 1) {
+      if (lane == 0) {
+        for (int k = 0; k < D; ++k)
+          part[wave][k] = acc[k];
+        part[wave][D] = wl;
+      }
+      __syncthreads();
+      wl = part[0][D];
+      for (int k = 0; k < D; ++k)
+        acc[k] = part[0][k];
+      for (int w = 1; w < W; ++w) {
+        wl += part[w][D];
+        for (int k = 0; k < D; ++k)
+          acc[k] += part[w][k];
+      }
+      __syncthreads();  // part is rewritten next iteration
+    }
     wsum = wl;
     bool moved = false;
     if (wl > R(0)) {
@@ -462,10 +531,17 @@ __global__ void __launch_bounds__(kThreads)
       }
     }
     if (!moved)
-      break;
+      break;  // uniform across the seed's waves: they hold the same mean
   }
   inside = wave_sum(inside);
-  if (lane == 0) {
+  if (W > 1) {
+    __shared__ unsigned long long ins[kWavesPerBlock];
+    if (lane == 0)
+      ins[wave] = inside;
+    __syncthreads();
+    inside = ins[0] + ins[1] + ins[2] + ins[3];
+  }
+  if (lane == 0 && wsub == 0) {
     for (int k = 0; k < D; ++k)
       out[g.perm[k] * ns + s] = m[k];
     weight[s] = wsum;
@@ -591,14 +667,22 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   const R inv2h2 = static_cast<R>(1.0 / (2.0 * h * h));
   const R r2 = static_cast<R>(cutoff * h * cutoff * h);
   const R fw = static_cast<R>(std::exp(-0.5 * cutoff * cutoff));
-  const unsigned logical = blocks_for(ns, kWavesPerBlock);
+  // few seeds: four waves per seed, so the launch still fills the chip
+  const bool wide = ns < kWideSeeds;
+  const unsigned logical = wide ? static_cast<unsigned>(ns) : blocks_for(ns, kWavesPerBlock);
   const unsigned grid = ((logical + kXcds - 1) / kXcds) * kXcds;
-  if (D == 6)
-    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 6>), grid, kThreads, 0, p4, q2, off, ctl, seed_pos,
-               seeds, order, ns, logical, iterations, inv2h2, r2, fw, out, weight, stats);
+#define MVS_MS_QUERY(DD, WW)                                                                    \
+  MVS_LAUNCH(s, "ms_query", (k_ms_query<R, DD, WW>), grid, kThreads, 0, p4, q2, off, ctl,      \
+             seed_pos, seeds, order, ns, logical, iterations, inv2h2, r2, fw, out, weight, stats)
+  if (D == 6 && wide)
+    MVS_MS_QUERY(6, kWavesPerBlock);
+  else if (D == 6)
+    MVS_MS_QUERY(6, 1);
+  else if (wide)
+    MVS_MS_QUERY(4, kWavesPerBlock);
   else
-    MVS_LAUNCH(s, "ms_query", (k_ms_query<R, 4>), grid, kThreads, 0, p4, q2, off, ctl, seed_pos,
-               seeds, order, ns, logical, iterations, inv2h2, r2, fw, out, weight, stats);
+    MVS_MS_QUERY(4, 1);
+#undef MVS_MS_QUERY
   if (run_slot >= 0) {
     MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats, ns, ctl->sums + run_slot);
     MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats + ns, ns,
