@@ -9,18 +9,19 @@
 //   w = exp(-d^2 / 2h^2) - exp(-cutoff^2 / 2)   for d < cutoff*h,
 // so the support boundary carries no weight jump.
 //
-// Grid: cells of side >= cutoff*h over the three coordinates of largest
-// variance (for height fields: the two horizontal translation components and
-// the rotation about z for rigid votes, plane offset and the horizontal
-// normal components for mirror votes), so a cell holds few points outside
-// the support.  Points are radix-sorted by cell (stable: sums are
+// Grid: cells of side >= cutoff*h / reach over the three coordinates of
+// largest variance (for height fields: the two horizontal translation
+// components and the rotation about z for rigid votes, plane offset and the
+// horizontal normal components for mirror votes), so a cell holds few points
+// outside the support; reach is 2 (half-width cells) when that grid fits in
+// 2^22 cells, else 1.  Points are radix-sorted by cell (stable: sums are
 // deterministic) and indexed by per-cell offsets found by binary search (no
-// atomics).  With the first grid axis fastest, the three cells of one row
-// are contiguous, so a query visits nine contiguous point ranges.  One
-// wavefront per seed runs all iterations in registers; lanes stride the
-// ranges and reduce with shuffles.  Seeds run in cell order and blocks are
-// mapped so that each XCD takes a contiguous run of seeds: neighbouring
-// queries share that XCD's L2.
+// atomics).  With the first grid axis fastest, the 2 reach + 1 cells of one
+// row are contiguous, so a query visits (2 reach + 1)^2 contiguous point
+// ranges (9 or 25).  Queries run all iterations in registers and reduce
+// with shuffles.  Seeds run in cell order and blocks are mapped so that each
+// XCD takes a contiguous run of seeds: neighbouring queries share that XCD's
+// L2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -120,14 +121,20 @@ __global__ void __launch_bounds__(kThreads) k_bbox(const R *pts, uint64_t n, dou
 
 // perm lists the point coordinates in storage (slot) order: the three grid
 // axes ascending, then the remaining coordinates ascending.
+// reach: neighbourhood half-width in cells (cell side >= cutoff*h / reach),
+// so a query visits (2 reach + 1)^2 rows of 2 reach + 1 cells.
 struct Grid {
   int axis[3];
   int perm[kMaxD];
   double lo[3];
   double cell;
   int dims[3];
+  int reach;
   uint64_t ncells;
 };
+constexpr int kMaxReach = 2;
+constexpr uint32_t kMapPerWave = 1024;  // LDS candidate-map entries per wave
+constexpr int kMaxRows = (2 * kMaxReach + 1) * (2 * kMaxReach + 1);
 constexpr int kCellBits = 22;  // at most 2^22 cells: three 8-bit radix passes
 constexpr uint64_t kMaxCells = 1ULL << kCellBits;
 constexpr unsigned kBboxBlocks = 256;
@@ -181,6 +188,7 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, uint64_t n, dou
     g.perm[k] = k;
   g.cell = cell0;
   g.dims[0] = g.dims[1] = g.dims[2] = 1;
+  g.reach = 1;
   g.ncells = 1;
   bool finite = true;
   for (int k = 0; k < D; ++k)
@@ -214,19 +222,25 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, uint64_t n, dou
         if (perm[k] == j)
           alo[k] = lo[j], ext[k] = hi[j] - lo[j];
     }
-    double cell = cell0;
-    for (;;) {
+    // half-width cells (reach 2) when that grid fits, else cells of side
+    // cutoff*h (reach 1), doubled until the grid fits
+    auto fits = [&](double cell) {
       uint64_t cells = 1;
       for (int k = 0; k < 3; ++k) {
         const double d = floor(ext[k] / cell) + 1;
         g.dims[k] = d < 2147483647.0 ? static_cast<int>(d) : 2147483647;
         cells = cells > kMaxCells ? cells : cells * static_cast<uint64_t>(g.dims[k]);
       }
-      if (cells <= kMaxCells) {
-        g.ncells = cells;
-        break;
-      }
-      cell *= 2;
+      g.ncells = cells;
+      return cells <= kMaxCells;
+    };
+    double cell = cell0 / kMaxReach;
+    g.reach = kMaxReach;
+    if (!fits(cell)) {
+      g.reach = 1;
+      cell = cell0;
+      while (!fits(cell))
+        cell *= 2;
     }
     for (int k = 0; k < 3; ++k) {
       g.axis[k] = perm[k];
@@ -351,11 +365,12 @@ __global__ void k_seed_keys(const R *seeds, uint64_t ns, const MsCtl *ctl, uint6
 // per 256-lane block; W = 4: one seed per block, for few seeds with many
 // candidates).  Seeds are sorted points (seed_pos, already in cell order) or
 // explicit positions (seeds, D x ns SoA, run in `order`).  All iterations
-// run in registers.  Per iteration the nine rows of the 3 x 3 neighbourhood
-// (each row three cells, contiguous in the sorted order) are walked as one
-// flattened sequence of candidates: the row bounds live in scalar registers
-// and each lane finds its row with eight compares, so short rows do not
-// leave lanes idle; the next batch's records are in flight while the current
+// run in registers.  Per iteration the rows of the neighbourhood (each row
+// 2 reach + 1 cells, contiguous in the sorted order) are walked as one
+// flattened sequence of candidates: the row bounds sit in lanes 0..24, a
+// scalar cursor follows the row of the current 64-candidate window and lanes
+// past a row start take that row's base, so short rows do not leave lanes
+// idle; the next batch's records are in flight while the current
 // batch is tested, and batch k goes to wave k mod W.  With W = 4 the waves
 // add their partial sums through LDS in a fixed order, so every wave holds
 // the same new mean.  A seed whose update leaves every coordinate unchanged
@@ -370,10 +385,14 @@ __global__ void __launch_bounds__(kThreads)
                unsigned long long *stats) {
   static_assert(W == 1 || W == kWavesPerBlock, "one seed per wave or per block");
   __shared__ R part[kWavesPerBlock][D + 1];
+  // candidate -> record map: one slice per wave (W = 1) or one per block
+  constexpr uint32_t kMapCap = W == 1 ? kMapPerWave : kMapPerWave * kWavesPerBlock;
+  __shared__ uint32_t cmap[kMapPerWave * kWavesPerBlock];
   const int lane = threadIdx.x & (kWave - 1);
   // wave index, made visibly wave-uniform so the row cursor stays scalar
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
   const int wsub = W == 1 ? 0 : wave;  // this wave's share of the seed's batches
+  uint32_t *map = W == 1 ? cmap + wave * kMapPerWave : cmap;
   // XCD-aware block order: dies take contiguous runs of (cell-ordered) seeds
   const unsigned per = (logical_blocks + kXcds - 1) / kXcds;
   const unsigned logical = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
@@ -400,15 +419,17 @@ __global__ void __launch_bounds__(kThreads)
   }
   R wsum = 0;
   unsigned long long cand = 0, inside = 0;
-  const int dz = lane / 3 - 1, dy = lane % 3 - 1;
+  const int reach = g.reach, span = 2 * reach + 1, nrows = span * span;
+  const int dz = lane / span - reach, dy = lane % span - reach;
   for (int it = 0; it < iterations; ++it) {
     const int cx = cell_of(static_cast<double>(m[0]), g.lo[0], g.cell, g.dims[0]);
     const int cy = cell_of(static_cast<double>(m[1]), g.lo[1], g.cell, g.dims[1]);
     const int cz = cell_of(static_cast<double>(m[2]), g.lo[2], g.cell, g.dims[2]);
-    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.dims[0] ? cx + 1 : cx;
-    // lanes 0..8 fetch the ranges of the 3 x 3 rows (three cells each)
+    const int x0 = cx > reach ? cx - reach : 0;
+    const int x1 = cx + reach < g.dims[0] ? cx + reach : g.dims[0] - 1;
+    // lanes 0..nrows-1 fetch the ranges of the neighbourhood rows
     uint32_t rb = 0, len = 0;
-    if (lane < 9) {
+    if (lane < nrows) {
       const int z = cz + dz, y = cy + dy;
       if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
         const uint64_t row = (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
@@ -417,18 +438,16 @@ __global__ void __launch_bounds__(kThreads)
         len = static_cast<uint32_t>(offsets[row + x1 + 1] - b);
       }
     }
-    uint32_t incl = len;  // inclusive prefix over lanes 0..8
-    for (int o = 1; o < 16; o <<= 1) {
+    uint32_t incl = len;  // inclusive prefix over lanes 0..nrows-1
+    for (int o = 1; o < 32; o <<= 1) {
       const uint32_t y = __shfl_up(incl, o, kWave);
       if (lane >= o)
         incl += y;
     }
-    // row r covers flattened [start_r, start_r + len_r), start_r = incl_{r-1}
-    // (lane r holds start_r in `startv`, begin_r - start_r in `basev`); a
-    // flattened index c maps to the record c + base of its row
+    // row r covers flattened [start_r, start_r + len_r), start_r = incl_{r-1};
+    // lane r holds start_r (lane nrows: the total) and the row's first record
     const uint32_t startv = __shfl_up(incl, 1, kWave) * (lane > 0 ? 1u : 0u);
-    const uint32_t basev = rb - startv;
-    const uint32_t total = __builtin_amdgcn_readlane(incl, 8);
+    const uint32_t total = __builtin_amdgcn_readlane(incl, nrows - 1);
     if (wsub == 0)
       cand += total;
     R acc[D];
@@ -436,36 +455,9 @@ __global__ void __launch_bounds__(kThreads)
       acc[k] = 0;
     R wl = 0;
     uint32_t in32 = 0;
-    // scalar row cursor: rc is the row of the window's first index; lanes
-    // past the next row start(s) take those rows' bases
-    int rc = 0;
-    auto locate = [&](uint32_t c0, uint32_t c) {
-      while (rc < 8 && static_cast<uint32_t>(__builtin_amdgcn_readlane(startv, rc + 1)) <= c0)
-        ++rc;
-      uint32_t b = __builtin_amdgcn_readlane(basev, rc);
-      for (int r = rc + 1; r < 9; ++r) {
-        const uint32_t st = __builtin_amdgcn_readlane(startv, r);
-        if (st > c0 + (kWave - 1))
-          break;
-        b = c >= st ? static_cast<uint32_t>(__builtin_amdgcn_readlane(basev, r)) : b;
-      }
-      return b + c;
-    };
-    constexpr uint32_t kStep = static_cast<uint32_t>(W) * kWave;
-    // two register sets: batch k + 1 loads while batch k is tested.  Loads
-    // are unconditional (lanes past the end re-read the last candidate), so
-    // the compiler can wait for one batch while the next is in flight.
-    const uint32_t last = total ? total - 1 : 0;
-    auto fetch = [&](uint32_t cb, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
-      const uint32_t cw = cb < last ? cb : last;
-      const uint32_t c = cb + lane < last ? cb + lane : last;
-      const uint32_t j = locate(cw, c);
-      p = p4[j];
-      if (D == 6)
-        q = q2[j];
-    };
-    auto test = [&](uint32_t cb, const typename Vec<R>::V4 &p, const typename Vec<R>::V2 &q) {
-      if (cb + lane >= total)
+    auto test = [&](uint32_t c, uint32_t end, const typename Vec<R>::V4 &p,
+                    const typename Vec<R>::V2 &q) {
+      if (c >= end)
         return;
       R d[D];
       d[0] = p.x - m[0];
@@ -487,16 +479,53 @@ __global__ void __launch_bounds__(kThreads)
         ++in32;
       }
     };
-    uint32_t c0 = static_cast<uint32_t>(wsub) * kWave;
-    if (c0 < total) {
-      typename Vec<R>::V4 pa, pb;
-      typename Vec<R>::V2 qa{}, qb{};
-      fetch(c0, pa, qa);
-      for (; c0 < total; c0 += 2 * kStep) {
-        fetch(c0 + kStep, pb, qb);
-        test(c0, pa, qa);
-        fetch(c0 + 2 * kStep, pa, qa);
-        test(c0 + kStep, pb, qb);
+    // Chunks of the flattened range: the map (LDS) sends candidate c to its
+    // record index, written row by row (rows split across the seed's waves),
+    // then the waves walk the chunk in 64-candidate batches, batch k + 1
+    // loading while batch k is tested (loads are unconditional: lanes past
+    // the end re-read the chunk's last candidate).
+    constexpr uint32_t kStep = static_cast<uint32_t>(W) * kWave;
+    for (uint32_t cbase = 0; cbase < total; cbase += kMapCap) {
+      const uint32_t cend = total - cbase < kMapCap ? total : cbase + kMapCap;
+      for (int r = wsub; r < nrows; r += W) {
+        const uint32_t rs = __builtin_amdgcn_readlane(startv, r);
+        const uint32_t re = __builtin_amdgcn_readlane(startv, r + 1);
+        const uint32_t lo = rs > cbase ? rs : cbase, hi = re < cend ? re : cend;
+        const uint32_t b = __builtin_amdgcn_readlane(rb, r);
+        for (uint32_t c = lo + lane; c < hi; c += kWave)
+          map[c - cbase] = b + (c - rs);
+      }
+      if (W > 1) {
+        __syncthreads();
+      } else {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      }
+      const uint32_t lastc = cend - 1 - cbase;
+      auto fetch = [&](uint32_t c, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
+        const uint32_t k = c - cbase < lastc ? c - cbase : lastc;
+        const uint32_t j = map[k];
+        p = p4[j];
+        if (D == 6)
+          q = q2[j];
+      };
+      uint32_t c0 = cbase + static_cast<uint32_t>(wsub) * kWave;
+      if (c0 < cend) {
+        typename Vec<R>::V4 pa, pb;
+        typename Vec<R>::V2 qa{}, qb{};
+        fetch(c0 + lane, pa, qa);
+        for (; c0 < cend; c0 += 2 * kStep) {
+          fetch(c0 + kStep + lane, pb, qb);
+          test(c0 + lane, cend, pa, qa);
+          fetch(c0 + 2 * kStep + lane, pa, qa);
+          test(c0 + kStep + lane, cend, pb, qb);
+        }
+      }
+      if (W > 1) {
+        __syncthreads();  // the map is rewritten by the next chunk
+      } else {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       }
     }
     inside += in32;

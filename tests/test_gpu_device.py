@@ -105,19 +105,22 @@ def test_pair_stats_and_identity_of_arithmetics():
     assert np.array_equal(sel[r64.b[:r64.P].cpu().numpy()], pb)
 
 
-def _clusters(D, seed):
+def _clusters(D, seed, spread=1.0):
     g = np.random.default_rng(seed)
-    centers = g.uniform(-1, 1, (3, D))
+    centers = g.uniform(-spread, spread, (3, D))
     pts = [c + g.normal(0, 0.02, (300, D)) for c in centers]
-    pts.append(g.uniform(-1.2, 1.2, (200, D)))
+    pts.append(g.uniform(-1.2 * spread, 1.2 * spread, (200, D)))
     P = np.concatenate(pts).T.copy()
     seeds = P[:, g.choice(P.shape[1], 64, replace=False)].copy()
     return P, seeds
 
 
+# spread 1: half-width cells (reach 2); spread 8: cells of side 3h (reach 1);
+# spread 200: reach 1 with cells doubled until the grid fits 2^22 cells
+@pytest.mark.parametrize("spread", [1.0, 8.0, 200.0])
 @pytest.mark.parametrize("D", [6, 4])
-def test_batched_meanshift_matches_oracle(oracle, D):
-    P, seeds = _clusters(D, 7 + D)
+def test_batched_meanshift_matches_oracle(oracle, D, spread):
+    P, seeds = _clusters(D, 7 + D, spread)
     h, cut, iters = 0.05, 3.0, 25
     want, wsum = oracle.meanshift_batched(P, np.zeros(P.shape[1], np.uint8), D, D, seeds,
                                           np.zeros(seeds.shape[1], np.uint8), h, cut, iters)
