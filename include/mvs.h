@@ -289,6 +289,32 @@ typedef struct {
 int mvs_dev_meanshift_votes(mvs_session *s, uint64_t seeds, uint64_t seed_value, int iterations,
                             double h, double cutoff, int want_stats, mvs_meanshift_result *out);
 
+/* Batches of meshes (no reference counterpart: the reference runs one mesh
+ * per detect call, src/core/pipeline.cpp:5-52).  The meshes' vertices and
+ * features are concatenated (x, y, z, feat exactly as for one mesh, n = all
+ * vertices); `selected` (device) lists every mesh's sample vertex ids as ids
+ * into the concatenation, mesh m owning samples [sample_offsets[m],
+ * sample_offsets[m + 1]).  box (meshes) and center (3 * meshes) describe each
+ * mesh; the three metadata arrays are HOST arrays.  Pairs form inside each
+ * mesh only.  For every mesh the pairs, their order and their compact votes
+ * equal those of mvs_dev_pairvote on that mesh alone, with batch-wide
+ * ordinals (mesh m's local ordinal = ordinal - sample_offsets[m]).  Budgets
+ * apply to the whole batch.  Outputs: the fused result over all meshes, in
+ * mesh order, and per-mesh starts of the pair list and of the mirror stream
+ * (device arrays of meshes + 1). */
+typedef struct {
+  mvs_pairvote_result all;
+  uint32_t meshes;
+  const uint64_t *pair_offsets;  /* device, meshes + 1 */
+  const uint64_t *refl_offsets;  /* device, meshes + 1 */
+} mvs_batch_pairvote_result;
+int mvs_dev_batch_pairvote(mvs_session *s, uint64_t n, const double *x, const double *y,
+                           const double *z, const double *feat, uint32_t meshes,
+                           const uint64_t *sample_offsets, const int32_t *selected,
+                           const double *box, const double *center, double threshold,
+                           uint64_t max_pairs, uint64_t max_votes, int precision,
+                           mvs_batch_pairvote_result *out);
+
 /* Batched-seed mean-shift (no reference counterpart; the reference runs
  * sequential seeds only, src/cpu/clustering.cpp:44-60).
  * mvs_dev_ms_points converts compact votes (layout of mvs_dev_vote_emit) to

@@ -25,7 +25,7 @@ EXPORTS = [
     "mvs_transformations_fetch", "mvs_clustering", "mvs_clustering_fetch",
     "mvs_dev_curvature", "mvs_dev_pair_count", "mvs_dev_pair_fill", "mvs_dev_pair_stats",
     "mvs_dev_vote_count", "mvs_dev_vote_emit", "mvs_dev_ms_points", "mvs_dev_ms_batched",
-    "mvs_dev_pairvote", "mvs_dev_meanshift_votes", "mvs_dev_copy",
+    "mvs_dev_pairvote", "mvs_dev_meanshift_votes", "mvs_dev_copy", "mvs_dev_batch_pairvote",
 ]
 
 
@@ -113,6 +113,11 @@ class PairVoteResult(C.Structure):
                 ("b", C.c_void_p), ("rigid_q4", C.c_void_p), ("rigid_t3", C.c_void_p),
                 ("rigid_pair", C.c_void_p), ("refl_n3", C.c_void_p), ("refl_d", C.c_void_p),
                 ("refl_pair", C.c_void_p), ("refl_stride", C.c_uint64), ("precision", C.c_int)]
+
+
+class BatchPairVoteResult(C.Structure):
+    _fields_ = [("all", PairVoteResult), ("meshes", C.c_uint32), ("pair_offsets", C.c_void_p),
+                ("refl_offsets", C.c_void_p)]
 
 
 class MeanShiftResult(C.Structure):
@@ -391,6 +396,21 @@ class Session:
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(lib().mvs_dev_pair_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return dict(window_candidates=a.value, index_candidates=b.value, fp64_rechecks=c.value)
+
+    def dev_batch_pairvote(self, n, x, y, z, feat, sample_offsets, sel, boxes, centers, t,
+                           max_pairs, max_votes, precision) -> BatchPairVoteResult:
+        """sample_offsets (meshes + 1), boxes (meshes), centers (meshes x 3): host arrays."""
+        off = np.ascontiguousarray(sample_offsets, dtype=np.uint64)
+        bx = np.ascontiguousarray(boxes, dtype=np.float64)
+        ct = np.ascontiguousarray(centers, dtype=np.float64)
+        meshes = len(off) - 1
+        assert len(bx) == meshes and ct.size == 3 * meshes
+        r = BatchPairVoteResult()
+        check(lib().mvs_dev_batch_pairvote(
+            self.h, C.c_uint64(n), ptr(x), ptr(y), ptr(z), ptr(feat), C.c_uint32(meshes),
+            ptr(off), ptr(sel), ptr(bx), ptr(ct), C.c_double(t), C.c_uint64(max_pairs),
+            C.c_uint64(max_votes), C.c_int(precision), C.byref(r)))
+        return r
 
     def dev_pairvote(self, n, x, y, z, feat, S, sel, box, center, t, max_pairs, max_votes,
                      precision) -> PairVoteResult:

@@ -111,6 +111,9 @@ enum Slot : int {
   kSlotOutPairs, kSlotOutRigidQ, kSlotOutRigidT, kSlotOutRigidPair, kSlotOutReflN,
   kSlotOutReflD, kSlotOutReflPair, kSlotOutMsRigidPts, kSlotOutMsReflPts, kSlotOutMsSeeds,
   kSlotOutMsRigid, kSlotOutMsRigidW, kSlotOutMsRefl, kSlotOutMsReflW,
+  // batches of meshes
+  kSlotBatchMeta, kSlotBatchSeg, kSlotBatchPairOff, kSlotBatchReflOff, kSlotBatchMs0,
+  kSlotBatchMs1, kSlotBatchMs2, kSlotBatchMs3, kSlotBatchMs4, kSlotBatchMs5,
   kSlotCount
 };
 

@@ -15,13 +15,14 @@
 
 namespace mvs {
 
-static void pairvote(Session *s, uint64_t n, const double *x, const double *y, const double *z,
-                     const double *feat, uint64_t S, const int32_t *sel, double box,
-                     const double *center, double t, uint64_t max_pairs, uint64_t max_votes,
-                     int precision, mvs_pairvote_result *out) {
+void pairvote(Session *s, uint64_t n, const double *x, const double *y, const double *z,
+              const double *feat, uint64_t S, const int32_t *sel, double box,
+              const double *center, double t, uint64_t max_pairs, uint64_t max_votes,
+              int precision, mvs_pairvote_result *out, const Segs &segs) {
   s->pairvote_valid = false;
   uint64_t P = 0;
-  pair_count_device(s, n, x, y, z, feat, S, sel, box, t, max_pairs, precision, center, &P);
+  pair_count_device(s, n, x, y, z, feat, S, sel, box, t, max_pairs, precision, center, &P,
+                    segs);
   if (2 * P > max_votes)
     raise(MVS_EBUDGET, "Vote budget exceeded; lower --sample-ratio or tighten --pair-threshold");
   const size_t sz = precision == MVS_FP32_GUARDED ? 4 : 8;
@@ -35,8 +36,8 @@ static void pairvote(Session *s, uint64_t n, const double *x, const double *y, c
   int32_t *fp = scratch_as<int32_t>(s, kSlotOutReflPair, Pp);
   pair_fill_device(s, ab, ab + Pp, true);
   uint64_t *rtot = reinterpret_cast<uint64_t *>(plan_flags(s)) + kVoteTotalWord;
-  vote_offsets_device(s, P, ab, ab + Pp, precision, rtot);
-  vote_emit_strided(s, P, ab, ab + Pp, center, precision, Pp, q4, t3, rp, n3, d1, fp);
+  vote_offsets_device(s, P, ab, ab + Pp, precision, rtot, segs);
+  vote_emit_strided(s, P, ab, ab + Pp, center, precision, Pp, q4, t3, rp, n3, d1, fp, segs);
   const uint64_t R = vote_total_checked(s);
   if (2 * P + R > max_votes || 2 * P + R > static_cast<uint64_t>(INT32_MAX))
     raise(MVS_EBUDGET, "Vote budget exceeded; lower --sample-ratio or tighten --pair-threshold");
@@ -135,7 +136,7 @@ int mvs_dev_pairvote(mvs_session *sp, uint64_t n, const double *x, const double 
     if (!out || !center)
       raise(MVS_EINVAL, "null argument");
     pairvote(as_session(sp), n, x, y, z, feat, S, sel, box, center, t, max_pairs, max_votes,
-             precision, out);
+             precision, out, Segs{});
   });
 }
 

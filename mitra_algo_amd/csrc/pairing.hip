@@ -60,15 +60,22 @@ __device__ inline float wave_max(float v) {
 }
 
 // Flags: 1 = selected id out of range, 2 = non-finite coordinate/curvature.
+// Batches: the centre is the sample's mesh centre and the sort key carries
+// the mesh id above the 32-bit curvature key, so each mesh sorts apart.
 __global__ void k_gather_samples(const double *x, const double *y, const double *z,
                                  const double *feat, uint64_t n, const int32_t *sel, uint64_t S,
-                                 double cx, double cy, double cz, double *r64, float *r32,
-                                 uint64_t *keys, uint32_t *vals, unsigned *flags,
+                                 double cx, double cy, double cz, Segs sg, double *r64,
+                                 float *r32, uint64_t *keys, uint32_t *vals, unsigned *flags,
                                  unsigned *rmax_bits) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool live = i < S;
   float r = 0.0f;
   if (live) {
+    uint64_t mesh = 0;
+    if (sg.seg) {
+      mesh = sg.seg[i];
+      cx = sg.center[3 * mesh], cy = sg.center[3 * mesh + 1], cz = sg.center[3 * mesh + 2];
+    }
     int32_t id = sel[i];
     double v[kRec];
     for (int k = 0; k < kRec; ++k)
@@ -106,7 +113,7 @@ __global__ void k_gather_samples(const double *x, const double *y, const double 
         f4[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
       r = fmaxf(fabsf(f[L_X]), fmaxf(fabsf(f[L_Y]), fabsf(f[L_Z])));
     }
-    keys[i] = order_key32(v[L_K0]);
+    keys[i] = (mesh << 32) | order_key32(v[L_K0]);
     vals[i] = static_cast<uint32_t>(i);
   }
   // largest centre-relative coordinate (sets the FP32 guard bands); every
@@ -162,14 +169,15 @@ __global__ void __launch_bounds__(kThreads)
 // and |Ka - Kb| <= t Ma/(1-t).  t is inflated by 16 eps (the reference
 // rounds both the difference and t*M), the radius by 16 eps and the
 // endpoints by 16 eps of their magnitude; t >= .5 or t below the normal
-// range use the full range.
+// range use the sample's whole mesh.
 // win: [0,S) first sorted position, [S,2S) end position; kw: kmax interval.
 __global__ void k_windows(const double *r64, const uint64_t *skeys, uint64_t S, double t,
-                          uint64_t *win, double2 *kw) {
+                          const uint32_t *seg, uint64_t *win, double2 *kw) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= S)
     return;
-  uint64_t lo = 0, hi = S;
+  const uint64_t mesh = seg ? static_cast<uint64_t>(seg[i]) << 32 : 0;
+  uint32_t qlo = 0, qhi = 0xFFFFFFFFu;
   double klo = -INFINITY, khi = INFINITY;
   double tp = t * (1 + 16 * kEps);
   if (tp < 0.5 && t > 1e-290) {
@@ -177,33 +185,34 @@ __global__ void k_windows(const double *r64, const uint64_t *skeys, uint64_t S, 
     double ma = fmax(fmax(fabs(a), fabs(b)), 1e-10);
     double r = tp * ma / (1 - tp) * (1 + 16 * kEps);
     double pa = 16 * kEps * (fabs(a) + r), pb = 16 * kEps * (fabs(b) + r);
-    uint32_t qlo = order_key32(a - r - pa), qhi = order_key32(a + r + pa);
+    qlo = order_key32(a - r - pa);
+    qhi = order_key32(a + r + pa);
     klo = b - r - pb;
     khi = b + r + pb;
-    uint64_t f = 0, c = S;
-    while (c) {  // first position with key >= qlo
-      uint64_t h = c / 2, m = f + h;
-      if (static_cast<uint32_t>(skeys[m]) < qlo) {
-        f = m + 1;
-        c -= h + 1;
-      } else
-        c = h;
-    }
-    lo = f;
-    f = 0;
-    c = S;
-    while (c) {  // first position with key > qhi
-      uint64_t h = c / 2, m = f + h;
-      if (static_cast<uint32_t>(skeys[m]) <= qhi) {
-        f = m + 1;
-        c -= h + 1;
-      } else
-        c = h;
-    }
-    hi = f;
+  }
+  const uint64_t kl = mesh | qlo, kh = mesh | qhi;
+  uint64_t f = 0, c = S;
+  while (c) {  // first position with key >= kl
+    uint64_t h = c / 2, m = f + h;
+    if (skeys[m] < kl) {
+      f = m + 1;
+      c -= h + 1;
+    } else
+      c = h;
+  }
+  const uint64_t lo = f;
+  f = 0;
+  c = S;
+  while (c) {  // first position with key > kh
+    uint64_t h = c / 2, m = f + h;
+    if (skeys[m] <= kh) {
+      f = m + 1;
+      c -= h + 1;
+    } else
+      c = h;
   }
   win[i] = lo;
-  win[S + i] = hi;
+  win[S + i] = f;
   kw[i] = make_double2(klo, khi);
 }
 
@@ -262,6 +271,7 @@ struct PairArgs {
   float gd0, gd1;
   const unsigned *rmax_bits;
   uint32_t *cand, *cand2, *rechk;  // per source statistics
+  Segs sg;                         // batches: per-mesh box and ordinal range
 };
 
 template <bool kGuarded> __device__ inline PairArgs with_bands(const PairArgs &in) {
@@ -274,7 +284,22 @@ template <bool kGuarded> __device__ inline PairArgs with_bands(const PairArgs &i
 struct Source {
   double a0, a1, ax, ay, az;
   float f0, f1, fx, fy, fz;
+  double dist;    // 0.03 x the mesh's box size
+  float dist32;
+  uint64_t ohi;   // end of the source's mesh in ordinals
 };
+
+// Minimum pair distance and ordinal end of source i's mesh.
+__device__ inline void source_mesh(const PairArgs &A, uint64_t i, double &dist, uint64_t &ohi) {
+  if (A.sg.seg) {
+    const uint32_t m = A.sg.seg[i];
+    dist = 0.03 * A.sg.box[m];
+    ohi = A.sg.ostart[m + 1];
+  } else {
+    dist = A.dist;
+    ohi = A.S;
+  }
+}
 
 // Decide candidate at bucketed position j (ordinal o).
 template <bool kGuarded>
@@ -289,7 +314,7 @@ __device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, 
     const double2 *p = reinterpret_cast<const double2 *>(A.r64 + static_cast<uint64_t>(o) * kRec);
     double2 p0 = p[0], p1 = p[1];
     return accept64(src.a0, src.a1, b.x, b.y, src.ax, src.ay, src.az, p0.x, p0.y, p1.x, A.t,
-                    A.dist);
+                    src.dist);
   } else {
     float2 b = A.bkf[j];
     int r = screen32(src.f0, src.f1, b.x, b.y, A.t32, A.gk);
@@ -297,7 +322,7 @@ __device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, 
       return false;
     if (r == 1) {
       float4 p = *reinterpret_cast<const float4 *>(A.r32 + static_cast<uint64_t>(o) * kRec);
-      int d = screen32_dist(src.fx - p.x, src.fy - p.y, src.fz - p.z, A.dist32, A.gd);
+      int d = screen32_dist(src.fx - p.x, src.fy - p.y, src.fz - p.z, src.dist32, A.gd);
       if (d != 2)
         return d == 1;
     }
@@ -306,7 +331,7 @@ __device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, 
     const double2 *p = reinterpret_cast<const double2 *>(A.r64 + static_cast<uint64_t>(o) * kRec);
     double2 p0 = p[0], p1 = p[1];
     return accept64(src.a0, src.a1, bb.x, bb.y, src.ax, src.ay, src.az, p0.x, p0.y, p1.x, A.t,
-                    A.dist);
+                    src.dist);
   }
 }
 
@@ -336,6 +361,8 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
       float4 q0 = q[0], q1 = q[1];
       src.fx = q0.x, src.fy = q0.y, src.fz = q0.z, src.f0 = q0.w, src.f1 = q1.x;
     }
+    source_mesh(A, i, src.dist, src.ohi);
+    src.dist32 = static_cast<float>(src.dist);
   }
   const uint64_t b0 = jlo / kBucket, b1 = (jhi - 1) / kBucket;
   for (uint64_t bb = b0; bb <= b1; bb += kWave) {
@@ -394,7 +421,7 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
             l += step;
         const uint64_t j = W.start[l] + (c - W.pre[l]);
         o = A.bord[j];
-        if (o != kNone && o > i)
+        if (o != kNone && o > i && o < src.ohi)
           ok = decide<kGuarded>(A, src, j, o, rechecks);
       }
       f(ok, o);
@@ -493,10 +520,10 @@ __global__ void __launch_bounds__(kThreads)
 // split into chunks of kLongChunk ordinals (one wave per (source, chunk)):
 // a counting pass, a scan over the flattened (source, chunk) counts, and a
 // writing pass that places each chunk's pairs after the earlier chunks'.
-__device__ inline void long_range(uint64_t i, uint64_t c, uint64_t S, uint64_t &lo,
+__device__ inline void long_range(uint64_t i, uint64_t c, uint64_t ohi, uint64_t &lo,
                                   uint64_t &hi) {
   lo = c * kLongChunk;
-  hi = lo + kLongChunk < S ? lo + kLongChunk : S;
+  hi = lo + kLongChunk < ohi ? lo + kLongChunk : ohi;
   if (lo < i + 1)
     lo = i + 1;
 }
@@ -509,8 +536,10 @@ __global__ void __launch_bounds__(kThreads)
   if (w >= long_count * chunks)
     return;
   const uint64_t i = long_list[w / chunks], c = w % chunks;
-  uint64_t lo, hi;
-  long_range(i, c, A.S, lo, hi);
+  double dist;
+  uint64_t ohi, lo, hi;
+  source_mesh(A, i, dist, ohi);
+  long_range(i, c, ohi, lo, hi);
   const double *ri = A.r64 + i * kRec;
   const double a0 = ri[L_K0], a1 = ri[L_K1], ax = ri[L_X], ay = ri[L_Y], az = ri[L_Z];
   uint32_t cnt = 0;
@@ -520,7 +549,7 @@ __global__ void __launch_bounds__(kThreads)
     if (b < hi) {
       const double *rb = A.r64 + b * kRec;
       ok = accept64(a0, a1, rb[L_K0], rb[L_K1], ax, ay, az, rb[L_X], rb[L_Y], rb[L_Z], A.t,
-                    A.dist);
+                    dist);
     }
     cnt += static_cast<uint32_t>(__popcll(ballot(ok)));
   }
@@ -538,8 +567,10 @@ __global__ void __launch_bounds__(kThreads)
     return;
   const uint64_t src_w = w / chunks, c = w % chunks;
   const uint64_t i = long_list[src_w];
-  uint64_t lo, hi;
-  long_range(i, c, A.S, lo, hi);
+  double dist;
+  uint64_t ohi, lo, hi;
+  source_mesh(A, i, dist, ohi);
+  long_range(i, c, ohi, lo, hi);
   const double *ri = A.r64 + i * kRec;
   const double a0 = ri[L_K0], a1 = ri[L_K1], ax = ri[L_X], ay = ri[L_Y], az = ri[L_Z];
   const int32_t src = sel ? sel[i] : static_cast<int32_t>(i);
@@ -551,7 +582,7 @@ __global__ void __launch_bounds__(kThreads)
     if (b < hi) {
       const double *rb = A.r64 + b * kRec;
       ok = accept64(a0, a1, rb[L_K0], rb[L_K1], ax, ay, az, rb[L_X], rb[L_Y], rb[L_Z], A.t,
-                    A.dist);
+                    dist);
     }
     uint64_t mask = ballot(ok);
     uint64_t p = cursor + lanes_below(mask);
@@ -665,8 +696,9 @@ static PairState pair_state(Session *s, uint64_t S, bool guarded) {
   return st;
 }
 
+// box: the mesh's box size, or the smallest box of a batch (guard bands).
 static PairArgs pair_args(const PairState &st, uint64_t S, double t, double box, bool guarded,
-                          const unsigned *flags) {
+                          const unsigned *flags, const Segs &segs) {
   PairArgs A{};
   A.r64 = st.r64;
   A.r32 = st.r32;
@@ -688,7 +720,12 @@ static PairArgs pair_args(const PairState &st, uint64_t S, double t, double box,
   A.cand = st.cand;
   A.cand2 = st.cand2;
   A.rechk = st.rechk;
+  A.sg = segs;
   return A;
+}
+
+const uint64_t *plan_pair_offsets(Session *s) {
+  return scratch_as<uint64_t>(s, kSlotPlanOffsets, (s->pair_plan.S ? s->pair_plan.S : 1) + 1);
 }
 
 const double *plan_samples64(Session *s) {
@@ -700,7 +737,8 @@ const float *plan_samples32(Session *s) {
 
 void pair_count_device(Session *s, uint64_t n, const double *x, const double *y, const double *z,
                        const double *feat, uint64_t S, const int32_t *sel, double box, double t,
-                       uint64_t max_pairs, int precision, const double *center, uint64_t *P_out) {
+                       uint64_t max_pairs, int precision, const double *center, uint64_t *P_out,
+                       const Segs &segs) {
   if (precision != MVS_FP64 && precision != MVS_FP32_GUARDED)
     raise(MVS_EINVAL, "unknown precision");
   if (!(box > 0) || !std::isfinite(box) || !(t > 0 && t <= 1))
@@ -714,6 +752,7 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   plan.box = box;
   plan.precision = precision;
   plan.selected = sel;
+  plan.segs = segs;
   const bool guarded = precision == MVS_FP32_GUARDED;
   PairState st = pair_state(s, S, guarded);
   unsigned *flags = plan_flags(s);
@@ -721,7 +760,7 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   double cx = center ? center[0] : 0, cy = center ? center[1] : 0, cz = center ? center[2] : 0;
   if (S)
     MVS_LAUNCH(s, "pair_gather", k_gather_samples, blocks_for(S, kThreads), kThreads, 0, x, y, z,
-               feat, n, sel, S, cx, cy, cz, st.r64, st.r32, st.keys, st.ord, flags,
+               feat, n, sel, S, cx, cy, cz, segs, st.r64, st.r32, st.keys, st.ord, flags,
                flags + kRmaxWord);
   if (S < 2) {  // no pair possible; like the reference, nothing to validate
     plan.P = 0;
@@ -729,14 +768,18 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
     *P_out = 0;
     return;
   }
-  radix_sort_pairs_u64(s, st.keys, st.ord, st.keys + S, st.ord + S, S, 32);
+  int key_bits = 32;  // batches: the mesh id sits above the curvature key
+  if (segs.seg)
+    while (key_bits < 64 && (s->batch_meshes >> (key_bits - 32)) != 0)
+      ++key_bits;
+  radix_sort_pairs_u64(s, st.keys, st.ord, st.keys + S, st.ord + S, S, key_bits);
   const uint64_t nb = (S + kBucket - 1) / kBucket;
   MVS_LAUNCH(s, "pair_index", k_bucket_index, blocks_for(nb, kWaves), kThreads, 0, st.ord + S,
              st.r64, st.r32, S, st.bord, st.bk, st.bkf);
   MVS_LAUNCH(s, "pair_windows", k_windows, blocks_for(S, kThreads), kThreads, 0, st.r64,
-             st.keys + S, S, t, st.win, st.kw);
+             st.keys + S, S, t, segs.seg, st.win, st.kw);
   unsigned *long_count = flags + 1;
-  PairArgs A = pair_args(st, S, t, box, guarded, flags);
+  PairArgs A = pair_args(st, S, t, box, guarded, flags, segs);
   unsigned grid = blocks_for(S, kWaves);
   if (guarded)
     MVS_LAUNCH(s, "pair_walk", k_pair_walk<true>, grid, kThreads, 0, A, st.counts, st.slab,
@@ -773,7 +816,7 @@ void pair_fill_device(Session *s, int32_t *a, int32_t *b, bool ordinals) {
   const uint64_t S = plan.S;
   const bool guarded = plan.precision == MVS_FP32_GUARDED;
   PairState st = pair_state(s, S, guarded);
-  PairArgs A = pair_args(st, S, plan.threshold, plan.box, guarded, plan_flags(s));
+  PairArgs A = pair_args(st, S, plan.threshold, plan.box, guarded, plan_flags(s), plan.segs);
   const int32_t *sel = ordinals ? nullptr : plan.selected;
   MVS_LAUNCH(s, "pair_place", k_pair_place, blocks_for(S, kWaves), kThreads, 0, S, st.counts,
              st.offsets, st.slab, sel, a, b);

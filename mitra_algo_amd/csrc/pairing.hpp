@@ -33,7 +33,8 @@ void read_back(Session *s, void *dst, const void *src, size_t bytes);
 // Pairing phases (pairing.hip).
 void pair_count_device(Session *s, uint64_t n, const double *x, const double *y, const double *z,
                        const double *feat, uint64_t S, const int32_t *sel, double box, double t,
-                       uint64_t max_pairs, int precision, const double *center, uint64_t *P_out);
+                       uint64_t max_pairs, int precision, const double *center, uint64_t *P_out,
+                       const Segs &segs = Segs{});
 void pair_fill_device(Session *s, int32_t *a, int32_t *b, bool ordinals);
 // Sums of per-source statistics of the last count: window candidates,
 // 2-D index candidates and FP64 rechecks.
@@ -49,10 +50,10 @@ void vote_emit_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b
 // of capacity (and SoA stride) `stride`; the total is left on the device at
 // *total_dev (uint64).
 void vote_offsets_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
-                         int precision, uint64_t *total_dev);
+                         int precision, uint64_t *total_dev, const Segs &segs = Segs{});
 void vote_emit_strided(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
                        const double *center, int precision, uint64_t stride, void *q4, void *t3,
-                       int32_t *rp, void *n3, void *d1, int32_t *fp);
+                       int32_t *rp, void *n3, void *d1, int32_t *fp, const Segs &segs = Segs{});
 // Synchronous read of the mirror total left by vote_offsets_device; raises
 // MVS_EINVAL when the count pass met a pair ordinal outside [0, S).
 uint64_t vote_total_checked(Session *s);
@@ -76,6 +77,28 @@ void ms_batched_f64(Session *s, int D, uint64_t n, const double *pts, uint64_t n
 void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4, const void *t3,
                       uint64_t nrefl, const void *n3, uint64_t n3_stride, const void *d1,
                       void *rigid_pts, void *refl_pts);
+
+// Fused pair + vote step (fused.hip); segs describes a batch of meshes.
+void pairvote(Session *s, uint64_t n, const double *x, const double *y, const double *z,
+              const double *feat, uint64_t S, const int32_t *sel, double box,
+              const double *center, double t, uint64_t max_pairs, uint64_t max_votes,
+              int precision, mvs_pairvote_result *out, const Segs &segs);
+// Per-ordinal pair offsets of the last pair count ([S + 1], device).
+const uint64_t *plan_pair_offsets(Session *s);
+
+// Batches (batch.hip): per-mesh metadata on the device and the segment view.
+struct BatchLayout {
+  Segs segs;
+  uint32_t meshes = 0;
+  uint64_t S = 0;       // samples of all meshes
+  double min_box = 0;   // smallest box (guard bands)
+};
+BatchLayout batch_setup(Session *s, uint32_t meshes, const uint64_t *soff, const double *box,
+                        const double *center);
+void batch_pairvote(Session *s, uint64_t n, const double *x, const double *y, const double *z,
+                    const double *feat, const BatchLayout &L, const int32_t *sel, double t,
+                    uint64_t max_pairs, uint64_t max_votes, int precision,
+                    mvs_batch_pairvote_result *out);
 
 // Sample records of the last pair count (valid until the next count).
 const double *plan_samples64(Session *s);

@@ -122,6 +122,8 @@ Session::~Session() {
     (void)hipEventDestroy(e);
   if (stage)
     (void)hipHostFree(stage);
+  if (batch_stage)
+    (void)hipHostFree(batch_stage);
   if (own_stream)
     (void)hipStreamDestroy(own_stream);
 }

@@ -13,6 +13,17 @@ struct PendingTiming {
   hipEvent_t start, stop;
 };
 
+// Batch layout (several meshes in one call).  Ordinals of mesh m are the
+// contiguous range [ostart[m], ostart[m + 1]); vertex ids in the selected
+// list index the concatenated vertex and feature arrays.  seg == nullptr
+// means one mesh whose box and centre are passed as plain values.
+struct Segs {
+  const uint32_t *seg = nullptr;     // per ordinal: mesh id
+  const uint64_t *ostart = nullptr;  // per mesh + 1: first ordinal (device)
+  const double *box = nullptr;       // per mesh: box size (device)
+  const double *center = nullptr;    // per mesh: 3 doubles (device)
+};
+
 // Device-resident plan kept between the count and fill phases of pairing.
 struct PairPlan {
   uint64_t S = 0;          // selected samples
@@ -21,6 +32,8 @@ struct PairPlan {
   double threshold = 0, box = 0;
   int precision = MVS_FP64;
   const int32_t *selected = nullptr;  // caller's device list (mesh ids)
+  double min_box = 0;                 // smallest box of the batch (guard bands)
+  Segs segs;
   bool valid = false;
 };
 
@@ -37,6 +50,10 @@ struct Session {
   DBuf<unsigned char> slots[kSlotCount];
   void *stage = nullptr;
   size_t stage_bytes = 0;
+  // pinned copy of the current batch's per-mesh metadata (a captured graph
+  // re-reads it at every replay, so nothing else writes it)
+  void *batch_stage = nullptr;
+  size_t batch_stage_bytes = 0;
   bool timing = false;
   std::string timing_only;  // ",name,name," or empty for every kernel
   bool timing_open = false;  // begin() recorded a start event for the current launch
@@ -45,6 +62,7 @@ struct Session {
   std::map<std::string, std::pair<double, uint64_t>> timings;
   PairPlan pair_plan;
   VotePlan vote_plan;
+  uint64_t batch_meshes = 0;  // mesh count of the current batch call (sort key width)
   // outputs of the last fused pair + vote call (session-owned buffers)
   mvs_pairvote_result pairvote{};
   bool pairvote_valid = false;

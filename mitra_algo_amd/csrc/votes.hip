@@ -124,7 +124,15 @@ struct VoteArgs {
   const unsigned *rmax_bits;
   uint32_t *wave_rr;  // per wave: FP64 rechecks of the mirror tests
   unsigned *bad;      // set when a pair ordinal is outside [0, S)
+  Segs sg;            // batches: the centre is the pair's mesh centre
 };
+// Model centre of the mesh that sample i belongs to.
+__device__ inline V3 centre_of(const VoteArgs &A, uint64_t i) {
+  if (!A.sg.seg)
+    return v3(A.cx, A.cy, A.cz);
+  const uint64_t m = A.sg.seg[i];
+  return v3(A.sg.center[3 * m], A.sg.center[3 * m + 1], A.sg.center[3 * m + 2]);
+}
 // Both ordinals of pair k index the sample records.
 __device__ inline bool pair_in_range(const int32_t *a, const int32_t *b, uint64_t k, uint64_t S) {
   int32_t i = a[k], o = b[k];
@@ -225,7 +233,7 @@ __global__ void __launch_bounds__(kThreads)
   R rq[2][4], rt[2][3], pn[3] = {R(0), R(0), R(0)}, pd = R(0);
   if (!kGuarded) {
     Geo64 g = geo_samples(A.s64, S, i, o);
-    const V3 c = v3(A.cx, A.cy, A.cz);
+    const V3 c = centre_of(A, i);
     for (int sg = 0; sg < 2; ++sg) {
       M3 rot = rigid64(g, sg == 0 ? 1.0 : -1.0);
       V3 t = g.q - apply(rot, g.p);
@@ -275,7 +283,7 @@ __global__ void __launch_bounds__(kThreads)
       if (m1 - m2 <= A.gdom || !(len > 1e-6f)) {
         // near-tied dominant axis: the sign of n comes from FP64
         Geo64 g = geo_samples(A.s64, S, i, o);
-        const V3 c = v3(A.cx, A.cy, A.cz);
+        const V3 c = centre_of(A, i);
         pn[0] = static_cast<R>(g.n.x), pn[1] = static_cast<R>(g.n.y),
         pn[2] = static_cast<R>(g.n.z);
         pd = static_cast<R>(g.d + dot(g.n, c));
@@ -385,7 +393,7 @@ __global__ void k_read_total64(const uint64_t *off, uint64_t n, uint64_t *dst) {
 }  // namespace
 
 static VoteArgs vote_args(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
-                          const double *center, bool guarded) {
+                          const double *center, bool guarded, const Segs &segs) {
   PairPlan &plan = s->pair_plan;
   if (!plan.valid)
     raise(MVS_EINVAL, "vote emission needs the sample plan of a pair count");
@@ -413,15 +421,16 @@ static VoteArgs vote_args(Session *s, uint64_t P, const int32_t *a, const int32_
   const uint64_t nw = (P + kWave - 1) / kWave;
   A.wave_rr = scratch_as<uint32_t>(s, kSlotRefl, nw ? nw : 1);
   A.bad = plan_flags(s) + kVoteBadWord;
+  A.sg = segs;
   return A;
 }
 
 void vote_offsets_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
-                         int precision, uint64_t *total_dev) {
+                         int precision, uint64_t *total_dev, const Segs &segs) {
   if (precision != MVS_FP64 && precision != MVS_FP32_GUARDED)
     raise(MVS_EINVAL, "unknown precision");
   const bool guarded = precision == MVS_FP32_GUARDED;
-  VoteArgs A = vote_args(s, P, a, b, nullptr, guarded);
+  VoteArgs A = vote_args(s, P, a, b, nullptr, guarded, segs);
   const uint64_t nw = (P + kWave - 1) / kWave;
   uint32_t *wr = scratch_as<uint32_t>(s, kSlotVoteBlock, nw ? nw : 1);
   uint64_t *off = scratch_as<uint64_t>(s, kSlotVoteBlockOff, nw + 1);
@@ -439,11 +448,11 @@ void vote_offsets_device(Session *s, uint64_t P, const int32_t *a, const int32_t
 
 void vote_emit_strided(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
                        const double *center, int precision, uint64_t stride, void *q4, void *t3,
-                       int32_t *rp, void *n3, void *d1, int32_t *fp) {
+                       int32_t *rp, void *n3, void *d1, int32_t *fp, const Segs &segs) {
   if (P == 0)
     return;
   const bool guarded = precision == MVS_FP32_GUARDED;
-  VoteArgs A = vote_args(s, P, a, b, center, guarded);
+  VoteArgs A = vote_args(s, P, a, b, center, guarded, segs);
   const uint64_t nw = (P + kWave - 1) / kWave;
   uint64_t *off = scratch_as<uint64_t>(s, kSlotVoteBlockOff, nw + 1);
   unsigned grid = blocks_for(nw, kThreads / kWave);

@@ -21,6 +21,11 @@ Workloads (sizes as in the reference's headline configurations):
   * cfg2_mesh(): 1024 x 1024 grid on [-4, 4]^2 (2,093,058 triangles); 64
     copies on the 8 x 8 lattice of cell centres, each rotated U(0, 2 pi) and
     x-mirrored with probability .5 (seed 1); noise N(0, 1e-4) from seed 2.
+  * batch_spec() / batch_mesh(): mesh m of a batch draws, from PCG64 seeded
+    with (seed, m), a grid side log-uniform in [32, 512] (rounded), 1-16
+    motif copies placed on the first cells of a k x k lattice of unit cells
+    (k = ceil(sqrt(copies)), domain [-k/2, k/2]^2), each rotated U(0, 2 pi)
+    and x-mirrored with probability .5, and N(0, 1e-4) noise.
 """
 from __future__ import annotations
 
@@ -176,3 +181,36 @@ def magnitude_keep(samples: np.ndarray, kmin: np.ndarray, kmax: np.ndarray,
     s = samples
     mag = (np.abs(kmin[s]) + np.abs(kmax[s])) / mean_edge
     return s[mag >= threshold]
+
+
+@dataclass
+class BatchMeshSpec:
+    side: int
+    copies: list
+    half: float
+    noise_seed: int
+
+
+def batch_spec(index: int, seed: int = 4, min_side: int = 32, max_side: int = 512) -> BatchMeshSpec:
+    """Parameters of mesh `index` of a batch (see the module docstring)."""
+    g = np.random.default_rng([seed, index])
+    side = int(round(math.exp(g.uniform(math.log(min_side), math.log(max_side)))))
+    count = int(g.integers(1, 17))
+    k = int(math.ceil(math.sqrt(count)))
+    theta = g.uniform(0.0, 2 * math.pi, count)
+    mirror = g.uniform(0.0, 1.0, count) < 0.5
+    copies = []
+    for c in range(count):
+        i, j = c % k, c // k
+        copies.append(Copy(-(k - 1) / 2 + i, -(k - 1) / 2 + j, float(theta[c]), bool(mirror[c])))
+    return BatchMeshSpec(side, copies, k / 2.0, int(g.integers(0, 2**31)))
+
+
+def batch_mesh(spec: BatchMeshSpec) -> HeightField:
+    return height_field(spec.side, spec.half, spec.copies, Motif.make(0), 1e-4, spec.noise_seed)
+
+
+def batch_samples(nvertices: int, index: int, seed: int = 4) -> np.ndarray:
+    """clamp(0.05 N, 200, 20000) uniform vertex samples of mesh `index`."""
+    count = int(min(max(round(0.05 * nvertices), 200), 20000))
+    return uniform_samples(count, nvertices, seed * 1000003 + index)

@@ -133,3 +133,68 @@ def cfg1(device: str = "cuda:0") -> Workload:
 def cfg2(device: str = "cuda:0", n: int = 1024) -> Workload:
     """1024x1024 height field (2.09M triangles), 100,000 uniform samples, t = .02."""
     return Workload(meshgen.cfg2_mesh(n), 100000, 0.02, False, device=device)
+
+
+class Batch:
+    """A batch of height-field meshes (meshgen.batch_spec) resident on one GPU:
+    concatenated vertices and features, per-mesh sample offsets, boxes and
+    centres, ready for the batch pair + vote and mean-shift entry points.
+    Each mesh also keeps its own feature array for one-mesh reference runs."""
+
+    def __init__(self, count: int = 512, seed: int = 4, threshold: float = 0.05,
+                 min_side: int = 32, max_side: int = 512, smoothing: int = 3,
+                 device: str = "cuda:0", keep_single: bool = False):
+        self.dev = torch.device(device)
+        self.threshold = threshold
+        self.sess = capi.Session(self.dev.index or 0)
+        self.stream = torch.cuda.current_stream(self.dev)
+        self.sess.set_stream(self.stream.cuda_stream)
+        xs, ys, zs, feats, sels = [], [], [], [], []
+        self.vbase, self.soff, self.boxes, self.centers = [0], [0], [], []
+        self.single = []
+        for m in range(count):
+            spec = meshgen.batch_spec(m, seed, min_side, max_side)
+            mesh = meshgen.batch_mesh(spec)
+            v = mesh.vertices
+            topo = capi.prepare(v, mesh.faces)
+            t = lambda a: torch.as_tensor(np.ascontiguousarray(a)).to(self.dev)
+            x, y, z = t(v[:, 0]), t(v[:, 1]), t(v[:, 2])
+            n = len(v)
+            feat = torch.empty((11, n), dtype=torch.float64, device=self.dev)
+            valid = torch.empty(n, dtype=torch.uint8, device=self.dev)
+            self.sess.dev_curvature(n, x, y, z, t(topo["normals"]), topo["face_normals"].shape[1],
+                                    t(topo["face_normals"]), t(topo["nbr_off"].view(np.int64)),
+                                    t(topo["nbr_idx"]), topo["edges"].shape[0], t(topo["edges"]),
+                                    smoothing, feat, valid)
+            fnp = feat.cpu().numpy()
+            sel = self.sess.prune(fnp, valid.cpu().numpy(), meshgen.batch_samples(n, m, seed),
+                                  0.85)
+            box, center = float(topo["box_size"]), np.array(topo["center"])
+            xs.append(x), ys.append(y), zs.append(z), feats.append(feat)
+            sels.append(torch.as_tensor(sel.astype(np.int32) + self.vbase[-1]))
+            if keep_single:
+                self.single.append(dict(n=n, x=x, y=y, z=z, feat=feat, S=len(sel),
+                                        sel=torch.as_tensor(sel.astype(np.int32)).to(self.dev),
+                                        box=box, center=center, side=spec.side))
+            self.vbase.append(self.vbase[-1] + n)
+            self.soff.append(self.soff[-1] + len(sel))
+            self.boxes.append(box)
+            self.centers.append(center)
+        self.count = count
+        self.n = self.vbase[-1]
+        self.x = torch.cat(xs)
+        self.y = torch.cat(ys)
+        self.z = torch.cat(zs)
+        self.feat = torch.cat(feats, dim=1).contiguous()
+        self.selected = torch.cat(sels).to(self.dev)
+        self.S = self.soff[-1]
+        self.soff = np.array(self.soff, dtype=np.uint64)
+        self.boxes = np.array(self.boxes)
+        self.centers = np.array(self.centers)
+        torch.cuda.synchronize(self.dev)
+
+    def pairvote(self, precision: int, max_pairs: int = 2**31 - 1,
+                 max_votes: int = 2**31 - 1) -> capi.BatchPairVoteResult:
+        return self.sess.dev_batch_pairvote(self.n, self.x, self.y, self.z, self.feat, self.soff,
+                                            self.selected, self.boxes, self.centers,
+                                            self.threshold, max_pairs, max_votes, precision)

@@ -1,0 +1,60 @@
+"""Batches of meshes: one batch pair + vote pass over several meshes of
+uneven size equals, mesh by mesh, the one-mesh pass (pairs, their order and
+the compact votes bit for bit), for both arithmetics."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from mitra_algo_amd import capi, workload  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def batch():
+    return workload.Batch(count=10, seed=11, min_side=24, max_side=96, keep_single=True)
+
+
+def _fetch(sess, r, prec):
+    dt = torch.float32 if prec == capi.FP32_GUARDED else torch.float64
+    P, R, st = r.pairs, r.reflections, r.refl_stride
+    f = dict(a=sess.fetch(r.a, (P,), torch.int32), b=sess.fetch(r.b, (P,), torch.int32),
+             q4=sess.fetch(r.rigid_q4, (4, 2 * P), dt, 2 * P),
+             t3=sess.fetch(r.rigid_t3, (3, 2 * P), dt, 2 * P),
+             n3=sess.fetch(r.refl_n3, (3, R), dt, st), d1=sess.fetch(r.refl_d, (R,), dt),
+             fp=sess.fetch(r.refl_pair, (R,), torch.int32))
+    return {k: v.cpu().clone() for k, v in f.items()}
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32_GUARDED])
+def test_batch_pairvote_equals_single_meshes(batch, prec):
+    r = batch.pairvote(prec)
+    allr = _fetch(batch.sess, r.all, prec)
+    M = batch.count
+    poff = batch.sess.fetch(r.pair_offsets, (M + 1,), torch.int64).cpu().numpy()
+    roff = batch.sess.fetch(r.refl_offsets, (M + 1,), torch.int64).cpu().numpy()
+    torch.cuda.synchronize()
+    assert poff[0] == 0 and poff[-1] == r.all.pairs and roff[-1] == r.all.reflections
+    single = capi.Session(0)
+    single.set_stream(torch.cuda.current_stream().cuda_stream)
+    total = 0
+    for m, s in enumerate(batch.single):
+        p0, p1, r0, r1 = poff[m], poff[m + 1], roff[m], roff[m + 1]
+        o = int(batch.soff[m])
+        one = _fetch(single, single.dev_pairvote(s["n"], s["x"], s["y"], s["z"], s["feat"],
+                                                s["S"], s["sel"], s["box"], s["center"],
+                                                batch.threshold, 2**31 - 1, 2**31 - 1, prec),
+                     prec)
+        torch.cuda.synchronize()
+        assert one["a"].numel() == p1 - p0 and one["fp"].numel() == r1 - r0, m
+        assert torch.equal(allr["a"][p0:p1] - o, one["a"]) and torch.equal(
+            allr["b"][p0:p1] - o, one["b"]), m
+        assert torch.equal(allr["q4"][:, 2 * p0:2 * p1], one["q4"]), m
+        assert torch.equal(allr["t3"][:, 2 * p0:2 * p1], one["t3"]), m
+        assert torch.equal(allr["fp"][r0:r1] - int(p0), one["fp"]), m
+        assert torch.equal(allr["n3"][:, r0:r1], one["n3"]) and torch.equal(
+            allr["d1"][r0:r1], one["d1"]), m
+        total += p1 - p0
+    assert total == r.all.pairs > 0
+    single.close()

@@ -92,3 +92,18 @@ def test_adapter_check_usage():
     if not os.path.exists(exe):
         pytest.skip("built only where the reference sources are")
     assert subprocess.run([exe], capture_output=True).returncode == 2
+
+
+def test_batch_mesh_specs():
+    """Batch meshes: sides log-uniform in [32, 512], 1-16 copies inside the
+    domain, deterministic per (seed, index); samples clamp(0.05 N, 200, 20000)."""
+    sides = [meshgen.batch_spec(m).side for m in range(512)]
+    assert min(sides) >= 32 and max(sides) <= 512
+    assert np.median(np.log(sides)) == pytest.approx(np.log(128), abs=0.25)
+    for m in range(20):
+        a, b = meshgen.batch_spec(m, 4), meshgen.batch_spec(m, 4)
+        assert a == b and 1 <= len(a.copies) <= 16
+        assert all(abs(c.cx) < a.half and abs(c.cy) < a.half for c in a.copies)
+    assert len(meshgen.batch_samples(32 * 32, 0)) == 200
+    assert len(meshgen.batch_samples(100 * 100, 0)) == 500
+    assert len(meshgen.batch_samples(1024 * 1024, 0)) == 20000
