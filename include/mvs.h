@@ -315,6 +315,26 @@ int mvs_dev_batch_pairvote(mvs_session *s, uint64_t n, const double *x, const do
                            uint64_t max_pairs, uint64_t max_votes, int precision,
                            mvs_batch_pairvote_result *out);
 
+/* Batched mean-shift over the votes of the last mvs_dev_batch_pairvote:
+ * mesh m draws ns = min(V_m, ceil(seeds_per_sample * S_m)) seeds (S_m its
+ * samples, V_m its votes), split between its rigid and mirror votes as
+ * mvs_dev_meanshift_votes splits `seeds`; each mesh's seeds, grids and
+ * iterations equal that one-mesh call with seeds = ns.  Outputs: modes and
+ * weights in the layout of mvs_meanshift_result with all.rigid_seeds /
+ * all.refl_seeds the capacities (SoA strides), mesh m's seeds at
+ * [offsets[m], offsets[m + 1]) of each stream (device arrays of meshes + 1);
+ * seed vote ids: rigid at [0, capacity), mirror after them, as indices into
+ * the batch's rigid / mirror streams. */
+typedef struct {
+  mvs_meanshift_result all;
+  uint32_t meshes;
+  const uint64_t *rigid_seed_offsets;  /* device, meshes + 1 */
+  const uint64_t *refl_seed_offsets;   /* device, meshes + 1 */
+} mvs_batch_meanshift_result;
+int mvs_dev_batch_meanshift(mvs_session *s, double seeds_per_sample, uint64_t seed_value,
+                            int iterations, double h, double cutoff, int want_stats,
+                            mvs_batch_meanshift_result *out);
+
 /* Batched-seed mean-shift (no reference counterpart; the reference runs
  * sequential seeds only, src/cpu/clustering.cpp:44-60).
  * mvs_dev_ms_points converts compact votes (layout of mvs_dev_vote_emit) to

@@ -26,6 +26,7 @@ EXPORTS = [
     "mvs_dev_curvature", "mvs_dev_pair_count", "mvs_dev_pair_fill", "mvs_dev_pair_stats",
     "mvs_dev_vote_count", "mvs_dev_vote_emit", "mvs_dev_ms_points", "mvs_dev_ms_batched",
     "mvs_dev_pairvote", "mvs_dev_meanshift_votes", "mvs_dev_copy", "mvs_dev_batch_pairvote",
+    "mvs_dev_batch_meanshift",
 ]
 
 
@@ -126,6 +127,11 @@ class MeanShiftResult(C.Structure):
                 ("rigid_weight", C.c_void_p), ("refl_modes", C.c_void_p),
                 ("refl_weight", C.c_void_p), ("candidates", C.c_uint64),
                 ("in_support", C.c_uint64), ("precision", C.c_int)]
+
+
+class BatchMeanShiftResult(C.Structure):
+    _fields_ = [("all", MeanShiftResult), ("meshes", C.c_uint32),
+                ("rigid_seed_offsets", C.c_void_p), ("refl_seed_offsets", C.c_void_p)]
 
 
 _lib = None
@@ -410,6 +416,15 @@ class Session:
             self.h, C.c_uint64(n), ptr(x), ptr(y), ptr(z), ptr(feat), C.c_uint32(meshes),
             ptr(off), ptr(sel), ptr(bx), ptr(ct), C.c_double(t), C.c_uint64(max_pairs),
             C.c_uint64(max_votes), C.c_int(precision), C.byref(r)))
+        return r
+
+    def dev_batch_meanshift(self, seeds_per_sample, seed_value, iterations, h, cutoff,
+                            want_stats=False) -> BatchMeanShiftResult:
+        r = BatchMeanShiftResult()
+        check(lib().mvs_dev_batch_meanshift(self.h, C.c_double(seeds_per_sample),
+                                            C.c_uint64(seed_value), C.c_int(iterations),
+                                            C.c_double(h), C.c_double(cutoff),
+                                            C.c_int(1 if want_stats else 0), C.byref(r)))
         return r
 
     def dev_pairvote(self, n, x, y, z, feat, S, sel, box, center, t, max_pairs, max_votes,

@@ -62,6 +62,30 @@ __global__ void k_mesh_offsets(const uint64_t *soff, const uint64_t *pair_off, u
   roff[m] = f;
 }
 
+// Seeds per mesh: ns = min(V, ceil(ratio S)) split between the rigid and
+// mirror streams as the one-mesh step splits them (mirror share rounded,
+// then each capped by its stream).  Rigid point segments start at 2 x the
+// mesh's first pair.
+__global__ void k_batch_seeds(const uint64_t *soff, const uint64_t *poff, const uint64_t *roff,
+                              uint32_t meshes, double ratio, uint64_t *kr, uint64_t *kf,
+                              uint64_t *rseg) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m > meshes)
+    return;
+  rseg[m] = 2 * poff[m];
+  if (m == meshes)
+    return;
+  const uint64_t S = soff[m + 1] - soff[m], nr = 2 * (poff[m + 1] - poff[m]);
+  const uint64_t nf = roff[m + 1] - roff[m], V = nr + nf;
+  const double want = ceil(ratio * static_cast<double>(S));
+  const uint64_t ns = want < static_cast<double>(V) ? static_cast<uint64_t>(want) : V;
+  uint64_t f = V ? static_cast<uint64_t>(llround(static_cast<double>(ns) * nf / V)) : 0;
+  f = f < nf ? f : nf;
+  const uint64_t r = ns > f ? ns - f : 0;
+  kr[m] = r < nr ? r : nr;
+  kf[m] = f;
+}
+
 }  // namespace
 
 // Uploads the batch's per-mesh metadata (offsets, boxes, centres) through
@@ -118,6 +142,9 @@ BatchLayout batch_setup(Session *s, uint32_t meshes, const uint64_t *soff, const
                meshes, S, seg);
   L.segs.seg = seg;
   s->batch_meshes = meshes;
+  s->batch_meta_soff = L.segs.ostart;
+  s->batch_soff.assign(soff, soff + meshes + 1);
+  s->batch_pv_valid = false;
   return L;
 }
 
@@ -138,6 +165,96 @@ void batch_pairvote(Session *s, uint64_t n, const double *x, const double *y, co
   out->meshes = L.meshes;
   out->pair_offsets = poff;
   out->refl_offsets = roff;
+  s->batch_pv = *out;
+  s->batch_pv_valid = true;
+}
+
+void batch_meanshift(Session *s, const uint64_t *soff_host, double ratio, uint64_t seed_value,
+                     int iterations, double h, double cutoff, bool want_stats,
+                     mvs_batch_meanshift_result *out) {
+  if (!s->batch_pv_valid)
+    raise(MVS_EINVAL, "batch mean-shift needs a preceding mvs_dev_batch_pairvote");
+  if (!(ratio >= 0) || !std::isfinite(ratio))
+    raise(MVS_EINVAL, "invalid seed ratio");
+  const mvs_batch_pairvote_result &b = s->batch_pv;
+  const mvs_pairvote_result &r = b.all;
+  const uint32_t meshes = b.meshes;
+  const int prec = r.precision;
+  const size_t sz = prec == MVS_FP32_GUARDED ? 4 : 8;
+  const uint64_t nr = 2 * r.pairs, nf = r.reflections;
+  // seed capacity: sum over meshes of ceil(ratio S_m), each stream
+  uint64_t cap = 0;
+  for (uint32_t m = 0; m < meshes; ++m)
+    cap += static_cast<uint64_t>(std::ceil(ratio * static_cast<double>(soff_host[m + 1] -
+                                                                       soff_host[m])));
+  const uint64_t cr = std::min(cap, nr), cf = std::min(cap, nf);
+  uint64_t *kr = scratch_as<uint64_t>(s, kSlotBatchMs4, 6 * (meshes + 1));
+  uint64_t *kf = kr + (meshes + 1), *rseg = kf + (meshes + 1);
+  uint64_t *offr = rseg + (meshes + 1), *offf = offr + (meshes + 1);
+  MVS_LAUNCH(s, "batch_seeds", k_batch_seeds, blocks_for(meshes + 1, kThreads), kThreads, 0,
+             s->batch_meta_soff, b.pair_offsets, b.refl_offsets, meshes, ratio, kr, kf, rseg);
+  exclusive_scan_u64(s, kr, offr, meshes);
+  exclusive_scan_u64(s, kf, offf, meshes);
+  int32_t *ids = scratch_as<int32_t>(s, kSlotOutMsSeeds, cr + cf ? cr + cf : 1);
+  uint32_t *smesh = scratch_as<uint32_t>(s, kSlotBatchMs5, cr + cf ? cr + cf : 1);
+  void *rpts = scratch(s, kSlotOutMsRigidPts, 6 * (nr ? nr : 1) * sz);
+  void *fpts = scratch(s, kSlotOutMsReflPts, 4 * (nf ? nf : 1) * sz);
+  void *orig = scratch(s, kSlotOutMsRigid, 6 * (cr ? cr : 1) * sz);
+  void *owr = scratch(s, kSlotOutMsRigidW, (cr ? cr : 1) * sz);
+  void *ofl = scratch(s, kSlotOutMsRefl, 4 * (cf ? cf : 1) * sz);
+  void *owf = scratch(s, kSlotOutMsReflW, (cf ? cf : 1) * sz);
+  ms_points_device(s, prec, cr ? nr : 0, r.rigid_q4, r.rigid_t3, cf ? nf : 0, r.refl_n3,
+                   r.refl_stride, r.refl_d, rpts, fpts);
+  MsSegs sr, sf;
+  sr.meshes = sf.meshes = meshes;
+  sr.seg_off = rseg;
+  sr.seed_off = offr;
+  sr.seed_mesh = smesh;
+  sr.cells_bound = 2 * nr + 1024ULL * meshes;
+  sf.seg_off = b.refl_offsets;
+  sf.seed_off = offf;
+  sf.seed_mesh = smesh + cr;
+  sf.cells_bound = 2 * nf + 1024ULL * meshes;
+  const uint64_t salt_r = splitmix64(seed_value);
+  const uint64_t salt_f = splitmix64(seed_value ^ 0x6d6972726f72ULL);  // "mirror"
+  ms_begin(s);
+  const int slot_r = want_stats ? 0 : -1, slot_f = want_stats ? 2 : -1;
+  if (prec == MVS_FP32_GUARDED) {
+    if (cr)
+      ms_batched_f32(s, 6, nr, static_cast<const float *>(rpts), cr, nullptr, ids, salt_r,
+                     iterations, h, cutoff, static_cast<float *>(orig), static_cast<float *>(owr),
+                     slot_r, &sr);
+    if (cf)
+      ms_batched_f32(s, 4, nf, static_cast<const float *>(fpts), cf, nullptr, ids + cr, salt_f,
+                     iterations, h, cutoff, static_cast<float *>(ofl), static_cast<float *>(owf),
+                     slot_f, &sf);
+  } else {
+    if (cr)
+      ms_batched_f64(s, 6, nr, static_cast<const double *>(rpts), cr, nullptr, ids, salt_r,
+                     iterations, h, cutoff, static_cast<double *>(orig),
+                     static_cast<double *>(owr), slot_r, &sr);
+    if (cf)
+      ms_batched_f64(s, 4, nf, static_cast<const double *>(fpts), cf, nullptr, ids + cr, salt_f,
+                     iterations, h, cutoff, static_cast<double *>(ofl),
+                     static_cast<double *>(owf), slot_f, &sf);
+  }
+  unsigned long long sums[4] = {0, 0, 0, 0};
+  ms_finish(s, sums);
+  mvs_meanshift_result m{};
+  m.rigid_seeds = cr;
+  m.refl_seeds = cf;
+  m.seed_votes = ids;
+  m.rigid_modes = orig;
+  m.rigid_weight = owr;
+  m.refl_modes = ofl;
+  m.refl_weight = owf;
+  m.candidates = sums[0] + sums[2];
+  m.in_support = sums[1] + sums[3];
+  m.precision = prec;
+  out->all = m;
+  out->meshes = meshes;
+  out->rigid_seed_offsets = offr;
+  out->refl_seed_offsets = offf;
 }
 
 }  // namespace mvs
@@ -159,6 +276,18 @@ int mvs_dev_batch_pairvote(mvs_session *sp, uint64_t n, const double *x, const d
     BatchLayout L = batch_setup(s, meshes, sample_offsets, box, center);
     batch_pairvote(s, n, x, y, z, feat, L, selected, threshold, max_pairs, max_votes, precision,
                    out);
+  });
+}
+
+int mvs_dev_batch_meanshift(mvs_session *sp, double seeds_per_sample, uint64_t seed_value,
+                            int iterations, double h, double cutoff, int want_stats,
+                            mvs_batch_meanshift_result *out) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    if (!out)
+      raise(MVS_EINVAL, "null argument");
+    batch_meanshift(s, s->batch_soff.data(), seeds_per_sample, seed_value, iterations, h, cutoff,
+                    want_stats != 0, out);
   });
 }
 

@@ -20,6 +20,7 @@ void pairvote(Session *s, uint64_t n, const double *x, const double *y, const do
               const double *center, double t, uint64_t max_pairs, uint64_t max_votes,
               int precision, mvs_pairvote_result *out, const Segs &segs) {
   s->pairvote_valid = false;
+  s->batch_pv_valid = false;  // a batch result's buffers are about to be reused
   uint64_t P = 0;
   pair_count_device(s, n, x, y, z, feat, S, sel, box, t, max_pairs, precision, center, &P,
                     segs);

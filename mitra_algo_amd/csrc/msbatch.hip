@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <limits>
 
@@ -119,6 +120,64 @@ __global__ void __launch_bounds__(kThreads) k_bbox(const R *pts, uint64_t n, dou
   }
 }
 
+// Segments (batches of meshes): item i of a point or seed list belongs to
+// the last m with off[m] <= i (off: meshes + 1 starts); off == nullptr means
+// one segment.
+__device__ inline uint32_t seg_of(const uint64_t *off, uint32_t meshes, uint64_t i) {
+  if (!off)
+    return 0;
+  uint32_t f = 0, c = meshes + 1;
+  while (c) {
+    const uint32_t h = c / 2, m = f + h;
+    if (off[m] <= i) {
+      f = m + 1;
+      c -= h + 1;
+    } else
+      c = h;
+  }
+  return f - 1;
+}
+
+// Box and moments of one segment per block (layout of k_bbox).
+template <class R, int D>
+__global__ void __launch_bounds__(kThreads)
+    k_seg_moments(const R *pts, uint64_t n, const uint64_t *seg_off, double *partial) {
+  __shared__ double sh[kWavesPerBlock][kPart];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const uint64_t b = seg_off[blockIdx.x], e = seg_off[blockIdx.x + 1];
+  double v[kPart];
+  for (int k = 0; k < kMaxD; ++k)
+    v[k] = INFINITY, v[kMaxD + k] = -INFINITY, v[2 * kMaxD + k] = 0, v[3 * kMaxD + k] = 0;
+  for (uint64_t i = b + threadIdx.x; i < e; i += kThreads)
+    for (int k = 0; k < D; ++k) {
+      const double x = static_cast<double>(pts[k * n + i]);
+      v[k] = fmin(v[k], x);
+      v[kMaxD + k] = fmax(v[kMaxD + k], x);
+      v[2 * kMaxD + k] += x;
+      v[3 * kMaxD + k] += x * x;
+    }
+  for (int o = kWave / 2; o > 0; o >>= 1)
+    for (int k = 0; k < kMaxD; ++k) {
+      v[k] = fmin(v[k], __shfl_xor(v[k], o, kWave));
+      v[kMaxD + k] = fmax(v[kMaxD + k], __shfl_xor(v[kMaxD + k], o, kWave));
+      v[2 * kMaxD + k] += __shfl_xor(v[2 * kMaxD + k], o, kWave);
+      v[3 * kMaxD + k] += __shfl_xor(v[3 * kMaxD + k], o, kWave);
+    }
+  if (lane == 0)
+    for (int k = 0; k < kPart; ++k)
+      sh[wave][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < kPart) {
+    const int k = threadIdx.x;
+    double r = sh[0][k];
+    for (int w = 1; w < kWavesPerBlock; ++w) {
+      const double x = sh[w][k];
+      r = k < kMaxD ? fmin(r, x) : (k < 2 * kMaxD ? fmax(r, x) : r + x);
+    }
+    partial[static_cast<uint64_t>(blockIdx.x) * kPart + k] = r;
+  }
+}
+
 // perm lists the point coordinates in storage (slot) order: the three grid
 // axes ascending, then the remaining coordinates ascending.
 // reach: neighbourhood half-width in cells (cell side >= cutoff*h / reach),
@@ -138,8 +197,6 @@ constexpr int kMaxRows = (2 * kMaxReach + 1) * (2 * kMaxReach + 1);
 constexpr int kCellBits = 22;  // at most 2^22 cells: three 8-bit radix passes
 constexpr uint64_t kMaxCells = 1ULL << kCellBits;
 constexpr unsigned kBboxBlocks = 256;
-// below this many seeds a seed gets a whole 256-lane block (four waves)
-constexpr uint64_t kWideSeeds = 8192;
 
 // Device-side control block of one mean-shift call: the grid, visit sums of
 // up to two runs and the non-finite-input flag, read back once at the end.
@@ -149,15 +206,23 @@ struct MsCtl {
   unsigned bad;
 };
 
-// One wavefront: reduces the per-block boxes and moments and lays the grid
-// on the three coordinates of largest spread (variance; ties: lower index
-// first).  Spread, not extent: a few outlying votes can stretch a coordinate
-// whose votes are otherwise concentrated.  Cells have side >= cell0, doubled
-// until at most kMaxCells.  No points: a single cell.
+// One wavefront per segment: reduces the segment's nb partial boxes and
+// moments and lays its grid on the three coordinates of largest spread
+// (variance; ties: lower index first).  Spread, not extent: a few outlying
+// votes can stretch a coordinate whose votes are otherwise concentrated.
+// Cells have side >= cell0 / 2 when that grid has at most
+// clamp(2 n, 1024, kMaxCells) cells, else side cell0, doubled until it fits.
+// No points: a single cell.  ncells[m] (optional) receives the cell count.
 template <int D>
-__global__ void k_grid_setup(const double *partial, unsigned nb, uint64_t n, double cell0,
-                             MsCtl *ctl) {
+__global__ void k_grid_setup(const double *partial, unsigned nb, const uint64_t *seg_off,
+                             uint64_t n, double cell0, Grid *grids, uint64_t *ncells,
+                             unsigned *bad) {
   const int lane = threadIdx.x;
+  const uint32_t mesh = blockIdx.x;
+  partial += static_cast<uint64_t>(mesh) * nb * kPart;
+  if (seg_off)
+    n = seg_off[mesh + 1] - seg_off[mesh];
+  const uint64_t cap = n < kMaxCells / 2 ? (2 * n > 1024 ? 2 * n : 1024) : kMaxCells;
   double lo[kMaxD], hi[kMaxD], sum[kMaxD], sq[kMaxD];
   for (int k = 0; k < kMaxD; ++k)
     lo[k] = INFINITY, hi[k] = -INFINITY, sum[k] = 0, sq[k] = 0;
@@ -193,9 +258,9 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, uint64_t n, dou
   bool finite = true;
   for (int k = 0; k < D; ++k)
     finite = finite && isfinite(lo[k]) && isfinite(hi[k]);
-  if (nb && !finite)
-    atomicOr(&ctl->bad, 1u);
-  if (nb && finite) {
+  if (nb && n && !finite)
+    atomicOr(bad, 1u);
+  if (nb && n && finite) {
     // the three largest variances (ties: lower index first), as a bit mask;
     // fixed-trip loops only, so everything stays in registers
     unsigned sel = 0;
@@ -229,10 +294,10 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, uint64_t n, dou
       for (int k = 0; k < 3; ++k) {
         const double d = floor(ext[k] / cell) + 1;
         g.dims[k] = d < 2147483647.0 ? static_cast<int>(d) : 2147483647;
-        cells = cells > kMaxCells ? cells : cells * static_cast<uint64_t>(g.dims[k]);
+        cells = cells > cap ? cells : cells * static_cast<uint64_t>(g.dims[k]);
       }
       g.ncells = cells;
-      return cells <= kMaxCells;
+      return cells <= cap;
     };
     double cell = cell0 / kMaxReach;
     g.reach = kMaxReach;
@@ -250,7 +315,9 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, uint64_t n, dou
       g.perm[k] = perm[k];
     g.cell = cell;
   }
-  ctl->grid = g;
+  grids[mesh] = g;
+  if (ncells)
+    ncells[mesh] = g.ncells;
 }
 
 // Cell index along one axis; NaN and underflow go to cell 0.
@@ -267,22 +334,27 @@ template <class R> __device__ inline uint64_t cell_key(const Grid &g, R a, R b, 
   return (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0] + x;
 }
 
+// Global cell id = the segment's cell base + its local cell id, so the
+// sorted order is segment-major.
 template <class R>
-__global__ void k_cell_keys(const R *pts, uint64_t n, const MsCtl *ctl, uint64_t *keys,
+__global__ void k_cell_keys(const R *pts, uint64_t n, const Grid *grids, const uint64_t *seg_off,
+                            uint32_t meshes, const uint64_t *cellbase, uint64_t *keys,
                             uint32_t *vals) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
-  const Grid g = ctl->grid;
-  keys[i] = cell_key(g, pts[g.axis[0] * n + i], pts[g.axis[1] * n + i], pts[g.axis[2] * n + i]);
+  const uint32_t m = seg_of(seg_off, meshes, i);
+  const Grid g = grids[m];
+  keys[i] = (cellbase ? cellbase[m] : 0) +
+            cell_key(g, pts[g.axis[0] * n + i], pts[g.axis[1] * n + i], pts[g.axis[2] * n + i]);
   vals[i] = static_cast<uint32_t>(i);
 }
 
 // offsets[c] = first sorted position with key >= c, for c in [0, ncells]
-// (grid-stride: the cell count lives on the device).
-__global__ void k_cell_offsets(const uint64_t *skeys, uint64_t n, const MsCtl *ctl,
+// (grid-stride: the cell count lives on the device, at *total).
+__global__ void k_cell_offsets(const uint64_t *skeys, uint64_t n, const uint64_t *total,
                                uint64_t *offsets) {
-  const uint64_t ncells = ctl->grid.ncells;
+  const uint64_t ncells = *total;
   for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c <= ncells;
        c += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     uint64_t f = 0, cnt = n;
@@ -312,13 +384,14 @@ template <> struct Vec<double> {
 };
 
 template <class R, int D>
-__global__ void k_gather_sorted(const R *pts, uint64_t n, const MsCtl *ctl, const uint32_t *order,
+__global__ void k_gather_sorted(const R *pts, uint64_t n, const Grid *grids,
+                                const uint64_t *seg_off, uint32_t meshes, const uint32_t *order,
                                 typename Vec<R>::V4 *p4, typename Vec<R>::V2 *q2) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
-  const int *perm = ctl->grid.perm;
   const uint64_t o = order[i];
+  const int *perm = grids[seg_of(seg_off, meshes, o)].perm;
   typename Vec<R>::V4 v;
   v.x = pts[perm[0] * n + o];
   v.y = pts[perm[1] * n + o];
@@ -333,29 +406,39 @@ __global__ void k_gather_sorted(const R *pts, uint64_t n, const MsCtl *ctl, cons
   }
 }
 
-// Seeds taken straight from the cell-sorted points: seed j of k is drawn
-// from the j-th of k equal strata of the sorted order (splitmix64 of
-// (salt, j)), so seeds come out distinct and in cell order.  ids receive the
-// seeds' point indices in the caller's (unsorted) numbering.
-__global__ void k_pick_sorted(uint64_t n, uint64_t k, uint64_t salt, const uint32_t *sorted_ids,
-                              uint32_t *pos, int32_t *ids) {
+// Seeds taken straight from the cell-sorted points: seed j of the k seeds of
+// a segment of n points is drawn from the j-th of k equal strata of the
+// segment's sorted range (splitmix64 of (salt, j)), so seeds come out
+// distinct and in cell order.  ids receive the seeds' point indices in the
+// caller's (unsorted) numbering, seed_mesh their segment.  Segments: seed_off
+// (meshes + 1) splits the seeds, seg_off the points; one segment otherwise.
+__global__ void k_pick_sorted(const uint64_t *seg_off, const uint64_t *seed_off, uint32_t meshes,
+                              uint64_t n, uint64_t k, uint64_t salt, const uint32_t *sorted_ids,
+                              uint32_t *pos, int32_t *ids, uint32_t *seed_mesh) {
   const uint64_t j = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= k)
+  if (j >= (seed_off ? seed_off[meshes] : k))
     return;
-  const uint64_t lo = j * n / k, hi = (j + 1) * n / k;
-  const uint64_t p = lo + splitmix64(salt ^ splitmix64(j)) % (hi - lo);
+  const uint32_t m = seg_of(seed_off, meshes, j);
+  const uint64_t jl = seed_off ? j - seed_off[m] : j;
+  const uint64_t km = seed_off ? seed_off[m + 1] - seed_off[m] : k;
+  const uint64_t base = seg_off ? seg_off[m] : 0;
+  const uint64_t nm = seg_off ? seg_off[m + 1] - base : n;
+  const uint64_t lo = jl * nm / km, hi = (jl + 1) * nm / km;
+  const uint64_t p = base + lo + splitmix64(salt ^ splitmix64(jl)) % (hi - lo);
   pos[j] = static_cast<uint32_t>(p);
   ids[j] = static_cast<int32_t>(sorted_ids[p]);
+  if (seed_mesh)
+    seed_mesh[j] = m;
 }
 
 // Explicit seed positions -> cell keys (to run those seeds in cell order).
 template <class R>
-__global__ void k_seed_keys(const R *seeds, uint64_t ns, const MsCtl *ctl, uint64_t *keys,
+__global__ void k_seed_keys(const R *seeds, uint64_t ns, const Grid *grids, uint64_t *keys,
                             uint32_t *vals) {
   uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= ns)
     return;
-  const Grid g = ctl->grid;
+  const Grid g = grids[0];
   keys[s] = cell_key(g, seeds[g.axis[0] * ns + s], seeds[g.axis[1] * ns + s],
                      seeds[g.axis[2] * ns + s]);
   vals[s] = static_cast<uint32_t>(s);
@@ -379,7 +462,8 @@ __global__ void k_seed_keys(const R *seeds, uint64_t ns, const MsCtl *ctl, uint6
 template <class R, int D, int W>
 __global__ void __launch_bounds__(kThreads)
     k_ms_query(const typename Vec<R>::V4 *p4, const typename Vec<R>::V2 *q2,
-               const uint64_t *offsets, const MsCtl *ctl, const uint32_t *seed_pos,
+               const uint64_t *offsets, const Grid *grids, const uint64_t *cellbase,
+               const uint32_t *seed_mesh, const uint64_t *ns_dev, const uint32_t *seed_pos,
                const R *seeds, const uint32_t *order, uint64_t ns, unsigned logical_blocks,
                int iterations, R inv2h2, R r2, R floor_w, R *out, R *weight,
                unsigned long long *stats) {
@@ -400,10 +484,13 @@ __global__ void __launch_bounds__(kThreads)
     return;
   const uint64_t slot = W == 1 ? static_cast<uint64_t>(logical) * kWavesPerBlock + wave
                                : static_cast<uint64_t>(logical);
-  if (slot >= ns)
+  if (slot >= (ns_dev ? *ns_dev : ns))
     return;  // W = 4: the whole block leaves together
-  const Grid g = ctl->grid;
   const uint64_t s = order ? order[slot] : slot;
+  // the seed's segment: its grid, and its cells' base in the offsets
+  const uint32_t gm = seed_mesh ? seed_mesh[s] : 0;
+  const Grid g = grids[gm];
+  const uint64_t cb = cellbase ? cellbase[gm] : 0;
   R m[D];  // slot order
   if (seed_pos) {
     const uint64_t p = seed_pos[s];
@@ -432,7 +519,7 @@ __global__ void __launch_bounds__(kThreads)
     if (lane < nrows) {
       const int z = cz + dz, y = cy + dy;
       if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
-        const uint64_t row = (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
+        const uint64_t row = cb + (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
         const uint64_t b = offsets[row + x0];
         rb = static_cast<uint32_t>(b);
         len = static_cast<uint32_t>(offsets[row + x1 + 1] - b);
@@ -616,15 +703,17 @@ void ms_finish(Session *s, unsigned long long sums[4]) {
     std::memcpy(sums, h.sums, sizeof h.sums);
 }
 
-// Builds the grid on the device and runs every seed, with no host
+// Builds the grid(s) on the device and runs every seed, with no host
 // synchronisation; visit sums go to ctl->sums[run_slot], [run_slot + 1].
 // Seeds: explicit positions (`seeds`, D x ns SoA), or, when seed_ids is
 // given, ns points drawn from the sorted points (salt), whose indices are
-// written to seed_ids.
+// written to seed_ids.  With segments (batches of meshes: seg->meshes
+// grids, drawn seeds only) ns is the seed capacity and seg->seed_off holds
+// the per-mesh seed counts' starts on the device.
 template <class R>
 static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns, const R *seeds,
                        int32_t *seed_ids, uint64_t salt, int iterations, double h, double cutoff,
-                       R *out, R *weight, int run_slot) {
+                       R *out, R *weight, int run_slot, const MsSegs *seg) {
   using V4 = typename Vec<R>::V4;
   using V2 = typename Vec<R>::V2;
   if (D != 4 && D != 6)
@@ -633,63 +722,103 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     raise(MVS_EINVAL, "invalid mean-shift parameters");
   if (n > static_cast<uint64_t>(UINT32_MAX) || ns > static_cast<uint64_t>(UINT32_MAX))
     raise(MVS_EOVERFLOW, "too many points or seeds");
-  if (seed_ids && ns > n)
+  if (seed_ids && !seg && ns > n)
     raise(MVS_EINVAL, "more seed votes than votes");
+  if (seg && !seed_ids)
+    raise(MVS_EINVAL, "segmented mean-shift draws its seeds");
   if (ns == 0)
     return;
+  const uint32_t meshes = seg ? seg->meshes : 1;
+  const uint64_t *seg_off = seg ? seg->seg_off : nullptr;
   MsCtl *ctl = ms_ctl(s);
-  double *partial = scratch_as<double>(s, kSlotMs1, kPart * kBboxBlocks);
   // per-seed statistics [0, ns) candidates, [ns, 2ns) in-support visits
   unsigned long long *stats = scratch_as<unsigned long long>(s, kSlotMs7, 2 * ns);
+  if (seg)  // seeds past the drawn count leave their slots untouched
+    zero_device(s, stats, 2 * ns * sizeof(unsigned long long));
   const uint64_t np = n ? n : 1;
   char *sorted = static_cast<char *>(scratch(s, kSlotMs0, np * (sizeof(V4) + sizeof(V2))));
   V4 *p4 = reinterpret_cast<V4 *>(sorted);
   V2 *q2 = reinterpret_cast<V2 *>(sorted + np * sizeof(V4));
-  uint64_t *off = scratch_as<uint64_t>(s, kSlotMs5, kMaxCells + 1);
+  const uint64_t cells_cap = seg ? std::max<uint64_t>(seg->cells_bound, 1) : kMaxCells;
+  uint64_t *off = scratch_as<uint64_t>(s, kSlotMs5, cells_cap + 1);
+  Grid *grids = seg ? scratch_as<Grid>(s, kSlotBatchMs0, meshes) : &ctl->grid;
+  uint64_t *cellbase = nullptr;
+  const uint64_t *total_cells = reinterpret_cast<const uint64_t *>(
+      reinterpret_cast<const char *>(grids) + offsetof(Grid, ncells));
+  // boxes and moments: per-block partials of the one point set, or one
+  // block per segment
   unsigned nb = 0;
-  if (n) {
-    nb = blocks_for(n, kThreads);
-    if (nb > kBboxBlocks)
-      nb = kBboxBlocks;
+  double *partial = nullptr;
+  if (!seg) {
+    partial = scratch_as<double>(s, kSlotMs1, kPart * kBboxBlocks);
+    if (n) {
+      nb = blocks_for(n, kThreads);
+      if (nb > kBboxBlocks)
+        nb = kBboxBlocks;
+      if (D == 6)
+        MVS_LAUNCH(s, "ms_bbox", (k_bbox<R, 6>), nb, kThreads, 0, pts, n, partial);
+      else
+        MVS_LAUNCH(s, "ms_bbox", (k_bbox<R, 4>), nb, kThreads, 0, pts, n, partial);
+    }
+  } else {
+    partial = scratch_as<double>(s, kSlotBatchMs1, kPart * meshes);
+    nb = 1;
     if (D == 6)
-      MVS_LAUNCH(s, "ms_bbox", (k_bbox<R, 6>), nb, kThreads, 0, pts, n, partial);
+      MVS_LAUNCH(s, "ms_bbox", (k_seg_moments<R, 6>), meshes, kThreads, 0, pts, n, seg_off,
+                 partial);
     else
-      MVS_LAUNCH(s, "ms_bbox", (k_bbox<R, 4>), nb, kThreads, 0, pts, n, partial);
+      MVS_LAUNCH(s, "ms_bbox", (k_seg_moments<R, 4>), meshes, kThreads, 0, pts, n, seg_off,
+                 partial);
   }
+  uint64_t *ncell = seg ? scratch_as<uint64_t>(s, kSlotBatchMs2, meshes + 1) : nullptr;
   if (D == 6)
-    MVS_LAUNCH(s, "ms_grid", k_grid_setup<6>, 1, kWave, 0, partial, nb, n, cutoff * h, ctl);
+    MVS_LAUNCH(s, "ms_grid", k_grid_setup<6>, meshes, kWave, 0, partial, nb, seg_off, n,
+               cutoff * h, grids, ncell, &ctl->bad);
   else
-    MVS_LAUNCH(s, "ms_grid", k_grid_setup<4>, 1, kWave, 0, partial, nb, n, cutoff * h, ctl);
+    MVS_LAUNCH(s, "ms_grid", k_grid_setup<4>, meshes, kWave, 0, partial, nb, seg_off, n,
+               cutoff * h, grids, ncell, &ctl->bad);
+  int key_bits = kCellBits;
+  if (seg) {  // cells of mesh m follow those of meshes < m
+    cellbase = scratch_as<uint64_t>(s, kSlotBatchMs3, meshes + 1);
+    exclusive_scan_u64(s, ncell, cellbase, meshes);
+    total_cells = cellbase + meshes;
+    key_bits = 1;
+    while (key_bits < 63 && (1ULL << key_bits) < cells_cap)
+      ++key_bits;
+  }
   uint32_t *sorted_ids = nullptr;
   if (n) {
     uint64_t *keys = scratch_as<uint64_t>(s, kSlotMs2, 2 * n);
     uint32_t *vals = scratch_as<uint32_t>(s, kSlotMs3, 2 * n);
     MVS_LAUNCH(s, "ms_cell_keys", k_cell_keys<R>, blocks_for(n, kThreads), kThreads, 0, pts, n,
-               ctl, keys, vals);
-    radix_sort_pairs_u64(s, keys, vals, keys + n, vals + n, n, kCellBits);
+               grids, seg_off, meshes, cellbase, keys, vals);
+    radix_sort_pairs_u64(s, keys, vals, keys + n, vals + n, n, key_bits);
     sorted_ids = vals + n;
-    MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 2048, kThreads, 0, keys + n, n, ctl, off);
+    MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 2048, kThreads, 0, keys + n, n, total_cells,
+               off);
     if (D == 6)
       MVS_LAUNCH(s, "ms_gather", (k_gather_sorted<R, 6>), blocks_for(n, kThreads), kThreads, 0,
-                 pts, n, ctl, sorted_ids, p4, q2);
+                 pts, n, grids, seg_off, meshes, sorted_ids, p4, q2);
     else
       MVS_LAUNCH(s, "ms_gather", (k_gather_sorted<R, 4>), blocks_for(n, kThreads), kThreads, 0,
-                 pts, n, ctl, sorted_ids, p4, q2);
+                 pts, n, grids, seg_off, meshes, sorted_ids, p4, q2);
   } else {
-    MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 1, kThreads, 0, nullptr, 0, ctl, off);
+    MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 1, kThreads, 0, nullptr, 0, total_cells,
+               off);
   }
   // seeds in cell order
   const uint32_t *order = nullptr, *seed_pos = nullptr;
   if (seed_ids) {
     uint32_t *pos = scratch_as<uint32_t>(s, kSlotMs6, ns);
-    MVS_LAUNCH(s, "ms_seeds", k_pick_sorted, blocks_for(ns, kThreads), kThreads, 0, n, ns, salt,
-               sorted_ids, pos, seed_ids);
+    MVS_LAUNCH(s, "ms_seeds", k_pick_sorted, blocks_for(ns, kThreads), kThreads, 0, seg_off,
+               seg ? seg->seed_off : nullptr, meshes, n, ns, salt, sorted_ids, pos, seed_ids,
+               seg ? seg->seed_mesh : nullptr);
     seed_pos = pos;
   } else if (n && ns > 1) {
     uint64_t *skeys = scratch_as<uint64_t>(s, kSlotMs4, 2 * ns);
     uint32_t *svals = scratch_as<uint32_t>(s, kSlotMs6, 2 * ns);
     MVS_LAUNCH(s, "ms_seed_keys", k_seed_keys<R>, blocks_for(ns, kThreads), kThreads, 0, seeds,
-               ns, ctl, skeys, svals);
+               ns, grids, skeys, svals);
     radix_sort_pairs_u64(s, skeys, svals, skeys + ns, svals + ns, ns, kCellBits);
     order = svals + ns;
   }
@@ -697,12 +826,15 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   const R r2 = static_cast<R>(cutoff * h * cutoff * h);
   const R fw = static_cast<R>(std::exp(-0.5 * cutoff * cutoff));
   // few seeds: four waves per seed, so the launch still fills the chip
-  const bool wide = ns < kWideSeeds;
+  const bool wide = ns < s->tune.wide_seeds;
   const unsigned logical = wide ? static_cast<unsigned>(ns) : blocks_for(ns, kWavesPerBlock);
   const unsigned grid = ((logical + kXcds - 1) / kXcds) * kXcds;
-#define MVS_MS_QUERY(DD, WW)                                                                    \
-  MVS_LAUNCH(s, "ms_query", (k_ms_query<R, DD, WW>), grid, kThreads, 0, p4, q2, off, ctl,      \
-             seed_pos, seeds, order, ns, logical, iterations, inv2h2, r2, fw, out, weight, stats)
+  const uint32_t *seed_mesh = seg ? seg->seed_mesh : nullptr;
+  const uint64_t *ns_dev = seg ? seg->seed_off + meshes : nullptr;
+#define MVS_MS_QUERY(DD, WW)                                                                   \
+  MVS_LAUNCH(s, "ms_query", (k_ms_query<R, DD, WW>), grid, kThreads, 0, p4, q2, off, grids,   \
+             cellbase, seed_mesh, ns_dev, seed_pos, seeds, order, ns, logical, iterations,    \
+             inv2h2, r2, fw, out, weight, stats)
   if (D == 6 && wide)
     MVS_MS_QUERY(6, kWavesPerBlock);
   else if (D == 6)
@@ -721,15 +853,17 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
 
 void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,
                     const float *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
-                    double h, double cutoff, float *out, float *weight, int run_slot) {
+                    double h, double cutoff, float *out, float *weight, int run_slot,
+                    const MsSegs *seg) {
   ms_batched<float>(s, D, n, pts, ns, seeds, seed_ids, salt, iterations, h, cutoff, out, weight,
-                    run_slot);
+                    run_slot, seg);
 }
 void ms_batched_f64(Session *s, int D, uint64_t n, const double *pts, uint64_t ns,
                     const double *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
-                    double h, double cutoff, double *out, double *weight, int run_slot) {
+                    double h, double cutoff, double *out, double *weight, int run_slot,
+                    const MsSegs *seg) {
   ms_batched<double>(s, D, n, pts, ns, seeds, seed_ids, salt, iterations, h, cutoff, out, weight,
-                     run_slot);
+                     run_slot, seg);
 }
 void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4, const void *t3,
                       uint64_t nrefl, const void *n3, uint64_t n3_stride, const void *d1,
@@ -782,11 +916,11 @@ int mvs_dev_ms_batched(mvs_session *sp, int precision, int dim, uint64_t n, cons
     if (precision == MVS_FP32_GUARDED)
       ms_batched<float>(s, dim, n, static_cast<const float *>(pts), ns,
                         static_cast<const float *>(seeds), nullptr, 0, iterations, h, cutoff,
-                        static_cast<float *>(out), static_cast<float *>(weight), 0);
+                        static_cast<float *>(out), static_cast<float *>(weight), 0, nullptr);
     else
       ms_batched<double>(s, dim, n, static_cast<const double *>(pts), ns,
                          static_cast<const double *>(seeds), nullptr, 0, iterations, h, cutoff,
-                         static_cast<double *>(out), static_cast<double *>(weight), 0);
+                         static_cast<double *>(out), static_cast<double *>(weight), 0, nullptr);
     unsigned long long sums[4];
     ms_finish(s, sums);
     if (candidates)

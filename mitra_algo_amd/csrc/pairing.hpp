@@ -62,6 +62,14 @@ uint64_t vote_total_checked(Session *s);
 // (run_slot 0 and 2 receive the (candidates, in-support) visit sums; -1 for
 // none), then ms_finish: one synchronisation that raises MVS_EINVAL for
 // non-finite points and returns the four sums.
+// Segments of one mean-shift run (batches of meshes).
+struct MsSegs {
+  uint32_t meshes = 1;
+  const uint64_t *seg_off = nullptr;   // device, meshes + 1: each mesh's points
+  const uint64_t *seed_off = nullptr;  // device, meshes + 1: each mesh's seeds
+  uint32_t *seed_mesh = nullptr;       // device, seed capacity: mesh of each seed
+  uint64_t cells_bound = 0;            // bound on the total grid cells
+};
 void ms_begin(Session *s);
 void ms_finish(Session *s, unsigned long long sums[4]);
 // Seeds: explicit positions (D x ns SoA), or, with seed_ids, ns of the n
@@ -69,10 +77,12 @@ void ms_finish(Session *s, unsigned long long sums[4]);
 // written to seed_ids, in cell order).
 void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,
                     const float *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
-                    double h, double cutoff, float *out, float *weight, int run_slot);
+                    double h, double cutoff, float *out, float *weight, int run_slot,
+                    const MsSegs *seg = nullptr);
 void ms_batched_f64(Session *s, int D, uint64_t n, const double *pts, uint64_t ns,
                     const double *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
-                    double h, double cutoff, double *out, double *weight, int run_slot);
+                    double h, double cutoff, double *out, double *weight, int run_slot,
+                    const MsSegs *seg = nullptr);
 // Compact votes to mean-shift points; mirror normals are 3 rows of n3_stride.
 void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4, const void *t3,
                       uint64_t nrefl, const void *n3, uint64_t n3_stride, const void *d1,

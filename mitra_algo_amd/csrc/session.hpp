@@ -43,6 +43,12 @@ struct VotePlan {
   bool valid = false;
 };
 
+// Launch parameters picked by the autotuner (defaults: the hand-set values).
+struct Tuning {
+  uint64_t wide_seeds = 8192;  // below this many seeds, a seed gets four waves
+  unsigned walk_quota = 1;     // sources per wave in the pair walk
+};
+
 struct Session {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -62,7 +68,12 @@ struct Session {
   std::map<std::string, std::pair<double, uint64_t>> timings;
   PairPlan pair_plan;
   VotePlan vote_plan;
-  uint64_t batch_meshes = 0;  // mesh count of the current batch call (sort key width)
+  uint64_t batch_meshes = 0;
+  const uint64_t *batch_meta_soff = nullptr;  // device copy of the sample offsets
+  std::vector<uint64_t> batch_soff;           // host copy
+  mvs_batch_pairvote_result batch_pv{};
+  bool batch_pv_valid = false;
+  Tuning tune;  // mesh count of the current batch call (sort key width)
   // outputs of the last fused pair + vote call (session-owned buffers)
   mvs_pairvote_result pairvote{};
   bool pairvote_valid = false;

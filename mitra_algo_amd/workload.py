@@ -198,3 +198,15 @@ class Batch:
         return self.sess.dev_batch_pairvote(self.n, self.x, self.y, self.z, self.feat, self.soff,
                                             self.selected, self.boxes, self.centers,
                                             self.threshold, max_pairs, max_votes, precision)
+
+    def meanshift(self, seeds_per_sample: float = 0.2, iterations: int = 20, h: float = 0.05,
+                  cutoff: float = 3.0, seed_value: int = 7,
+                  want_stats: bool = False) -> capi.BatchMeanShiftResult:
+        return self.sess.dev_batch_meanshift(seeds_per_sample, seed_value, iterations, h, cutoff,
+                                             want_stats)
+
+    def step(self, precision: int, seeds_per_sample: float = 0.2, iterations: int = 20):
+        """Pairs + votes of every mesh, then mean-shift of every mesh."""
+        pv = self.pairvote(precision)
+        ms = self.meanshift(seeds_per_sample, iterations)
+        return pv, ms

@@ -58,3 +58,52 @@ def test_batch_pairvote_equals_single_meshes(batch, prec):
         total += p1 - p0
     assert total == r.all.pairs > 0
     single.close()
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32_GUARDED])
+def test_batch_meanshift_equals_single_meshes(batch, prec):
+    """Per mesh: the same seeds as the one-mesh call with seeds = min(V, ceil(
+    .2 S)), and the same modes (FP64 1e-9, FP32 2e-5: a launch may serve a
+    seed with one or four waves, which changes the summation order)."""
+    dt = torch.float32 if prec == capi.FP32_GUARDED else torch.float64
+    tol = 1e-9 if prec == capi.FP64 else 2e-5
+    ratio, iters = 0.2, 12
+    r = batch.pairvote(prec)
+    M = batch.count
+    poff = batch.sess.fetch(r.pair_offsets, (M + 1,), torch.int64).cpu().numpy()
+    roff = batch.sess.fetch(r.refl_offsets, (M + 1,), torch.int64).cpu().numpy()
+    ms = batch.meanshift(ratio, iters)
+    cr, cf = ms.all.rigid_seeds, ms.all.refl_seeds
+    so_r = batch.sess.fetch(ms.rigid_seed_offsets, (M + 1,), torch.int64).cpu().numpy()
+    so_f = batch.sess.fetch(ms.refl_seed_offsets, (M + 1,), torch.int64).cpu().numpy()
+    ids = batch.sess.fetch(ms.all.seed_votes, (cr + cf,), torch.int32).cpu().numpy()
+    mr = batch.sess.fetch(ms.all.rigid_modes, (6, cr), dt, cr).double().cpu().numpy()
+    mf = batch.sess.fetch(ms.all.refl_modes, (4, cf), dt, cf).double().cpu().numpy()
+    torch.cuda.synchronize()
+    single = capi.Session(0)
+    single.set_stream(torch.cuda.current_stream().cuda_stream)
+    checked = 0
+    for m, s in enumerate(batch.single):
+        S = int(batch.soff[m + 1] - batch.soff[m])
+        P, R = int(poff[m + 1] - poff[m]), int(roff[m + 1] - roff[m])
+        V = 2 * P + R
+        ns = min(V, int(np.ceil(ratio * S)))
+        assert (so_r[m + 1] - so_r[m]) + (so_f[m + 1] - so_f[m]) == ns, m
+        if ns == 0:
+            continue
+        single.dev_pairvote(s["n"], s["x"], s["y"], s["z"], s["feat"], s["S"], s["sel"],
+                            s["box"], s["center"], batch.threshold, 2**31 - 1, 2**31 - 1, prec)
+        one = single.dev_meanshift_votes(ns, 7, iters, 0.05, 3.0, False)
+        kr, kf = one.rigid_seeds, one.refl_seeds
+        oid = single.fetch(one.seed_votes, (kr + kf,), torch.int32).cpu().numpy()
+        orig = single.fetch(one.rigid_modes, (6, kr), dt, kr).double().cpu().numpy()
+        ofl = single.fetch(one.refl_modes, (4, kf), dt, kf).double().cpu().numpy()
+        torch.cuda.synchronize()
+        assert kr == so_r[m + 1] - so_r[m] and kf == so_f[m + 1] - so_f[m], m
+        assert np.array_equal(ids[so_r[m]:so_r[m + 1]] - 2 * poff[m], oid[:kr]), m
+        assert np.array_equal(ids[cr + so_f[m]:cr + so_f[m + 1]] - roff[m], oid[kr:]), m
+        assert np.max(np.abs(mr[:, so_r[m]:so_r[m + 1]] - orig), initial=0) < tol, m
+        assert np.max(np.abs(mf[:, so_f[m]:so_f[m + 1]] - ofl), initial=0) < tol, m
+        checked += 1
+    assert checked > 0
+    single.close()
