@@ -335,6 +335,29 @@ int mvs_dev_batch_meanshift(mvs_session *s, double seeds_per_sample, uint64_t se
                             int iterations, double h, double cutoff, int want_stats,
                             mvs_batch_meanshift_result *out);
 
+/* One batch step (mvs_dev_batch_pairvote then mvs_dev_batch_meanshift with
+ * these arguments, want_stats off) as a HIP graph owned by the session.
+ * mvs_batch_graph_capture runs the step once (results in *pv, *ms, whose
+ * buffers every replay rewrites) while recording each count the step reads
+ * back, then captures the step on the session's own stream with those
+ * counts frozen; each read-back site becomes a device check.
+ * mvs_batch_graph_launch replays it with no host synchronisation, ordered
+ * after the session stream's queued work and before its later work.
+ * mvs_batch_graph_check synchronises and raises MVS_ERUNTIME when a replay
+ * read counts other than the captured ones (its inputs changed; the replay
+ * stayed inside the captured buffers).  Any call that grows a session buffer,
+ * and any new batch call, drops the graph. */
+int mvs_batch_graph_capture(mvs_session *s, uint64_t n, const double *x, const double *y,
+                            const double *z, const double *feat, uint32_t meshes,
+                            const uint64_t *sample_offsets, const int32_t *selected,
+                            const double *box, const double *center, double threshold,
+                            uint64_t max_pairs, uint64_t max_votes, int precision,
+                            double seeds_per_sample, uint64_t seed_value, int iterations,
+                            double h, double cutoff, mvs_batch_pairvote_result *pv,
+                            mvs_batch_meanshift_result *ms);
+int mvs_batch_graph_launch(mvs_session *s);
+int mvs_batch_graph_check(mvs_session *s);
+
 /* Batched-seed mean-shift (no reference counterpart; the reference runs
  * sequential seeds only, src/cpu/clustering.cpp:44-60).
  * mvs_dev_ms_points converts compact votes (layout of mvs_dev_vote_emit) to

@@ -26,7 +26,8 @@ EXPORTS = [
     "mvs_dev_curvature", "mvs_dev_pair_count", "mvs_dev_pair_fill", "mvs_dev_pair_stats",
     "mvs_dev_vote_count", "mvs_dev_vote_emit", "mvs_dev_ms_points", "mvs_dev_ms_batched",
     "mvs_dev_pairvote", "mvs_dev_meanshift_votes", "mvs_dev_copy", "mvs_dev_batch_pairvote",
-    "mvs_dev_batch_meanshift",
+    "mvs_dev_batch_meanshift", "mvs_batch_graph_capture", "mvs_batch_graph_launch",
+    "mvs_batch_graph_check",
 ]
 
 
@@ -426,6 +427,27 @@ class Session:
                                             C.c_double(h), C.c_double(cutoff),
                                             C.c_int(1 if want_stats else 0), C.byref(r)))
         return r
+
+    def batch_graph_capture(self, n, x, y, z, feat, sample_offsets, sel, boxes, centers, t,
+                            max_pairs, max_votes, precision, seeds_per_sample, seed_value,
+                            iterations, h, cutoff):
+        off = np.ascontiguousarray(sample_offsets, dtype=np.uint64)
+        bx = np.ascontiguousarray(boxes, dtype=np.float64)
+        ct = np.ascontiguousarray(centers, dtype=np.float64)
+        pv, ms = BatchPairVoteResult(), BatchMeanShiftResult()
+        check(lib().mvs_batch_graph_capture(
+            self.h, C.c_uint64(n), ptr(x), ptr(y), ptr(z), ptr(feat), C.c_uint32(len(off) - 1),
+            ptr(off), ptr(sel), ptr(bx), ptr(ct), C.c_double(t), C.c_uint64(max_pairs),
+            C.c_uint64(max_votes), C.c_int(precision), C.c_double(seeds_per_sample),
+            C.c_uint64(seed_value), C.c_int(iterations), C.c_double(h), C.c_double(cutoff),
+            C.byref(pv), C.byref(ms)))
+        return pv, ms
+
+    def batch_graph_launch(self):
+        check(lib().mvs_batch_graph_launch(self.h))
+
+    def batch_graph_check(self):
+        check(lib().mvs_batch_graph_check(self.h))
 
     def dev_pairvote(self, n, x, y, z, feat, S, sel, box, center, t, max_pairs, max_votes,
                      precision) -> PairVoteResult:

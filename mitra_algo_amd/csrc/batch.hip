@@ -42,13 +42,14 @@ __global__ void k_seg_fill(const uint64_t *soff, uint32_t meshes, uint64_t S, ui
 
 // Per-mesh starts of the pair list (from the per-source offsets) and of the
 // mirror stream (first mirror vote whose pair is in the mesh).
+// Offsets are clamped to the pair count P the step was sized with.
 __global__ void k_mesh_offsets(const uint64_t *soff, const uint64_t *pair_off, uint32_t meshes,
-                               const int32_t *refl_pair, uint64_t R, uint64_t *poff,
+                               uint64_t P, const int32_t *refl_pair, uint64_t R, uint64_t *poff,
                                uint64_t *roff) {
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m > meshes)
     return;
-  const uint64_t p = pair_off[soff[m]];
+  const uint64_t p = pair_off[soff[m]] < P ? pair_off[soff[m]] : P;
   poff[m] = p;
   uint64_t f = 0, c = R;
   while (c) {
@@ -111,6 +112,24 @@ BatchLayout batch_setup(Session *s, uint32_t meshes, const uint64_t *soff, const
   const uint64_t S = soff[meshes];
   const size_t nb = (meshes + 1) * sizeof(uint64_t) + meshes * sizeof(double) +
                     3 * meshes * sizeof(double);
+  BatchLayout L;
+  L.meshes = meshes;
+  L.S = S;
+  L.min_box = min_box;
+  char *d = static_cast<char *>(scratch(s, kSlotBatchMeta, nb));
+  L.segs.ostart = reinterpret_cast<const uint64_t *>(d);
+  L.segs.box = reinterpret_cast<const double *>(d + (meshes + 1) * sizeof(uint64_t));
+  L.segs.center = reinterpret_cast<const double *>(d + (meshes + 1) * sizeof(uint64_t) +
+                                                   meshes * sizeof(double));
+  uint32_t *seg = scratch_as<uint32_t>(s, kSlotBatchSeg, S ? S : 1);
+  L.segs.seg = seg;
+  s->batch_meshes = meshes;
+  s->batch_meta_soff = L.segs.ostart;
+  s->batch_soff.assign(soff, soff + meshes + 1);
+  s->batch_pv_valid = false;
+  if (s->replay == Replay::kCapture)
+    return L;  // the recorded step left this batch's metadata and ids in place
+  s->drop_graph();  // a captured step would now see other metadata
   // the previous batch's upload may still be reading the pinned buffer
   MVS_HIP(hipStreamSynchronize(s->stream));
   if (nb > s->batch_stage_bytes) {
@@ -126,25 +145,10 @@ BatchLayout batch_setup(Session *s, uint32_t meshes, const uint64_t *soff, const
   std::memcpy(h + (meshes + 1) * sizeof(uint64_t), box, meshes * sizeof(double));
   std::memcpy(h + (meshes + 1) * sizeof(uint64_t) + meshes * sizeof(double), center,
               3 * meshes * sizeof(double));
-  char *d = static_cast<char *>(scratch(s, kSlotBatchMeta, nb));
   MVS_HIP(hipMemcpyAsync(d, h, nb, hipMemcpyHostToDevice, s->stream));
-  BatchLayout L;
-  L.meshes = meshes;
-  L.S = S;
-  L.min_box = min_box;
-  L.segs.ostart = reinterpret_cast<const uint64_t *>(d);
-  L.segs.box = reinterpret_cast<const double *>(d + (meshes + 1) * sizeof(uint64_t));
-  L.segs.center = reinterpret_cast<const double *>(d + (meshes + 1) * sizeof(uint64_t) +
-                                                   meshes * sizeof(double));
-  uint32_t *seg = scratch_as<uint32_t>(s, kSlotBatchSeg, S ? S : 1);
   if (S)
     MVS_LAUNCH(s, "batch_seg", k_seg_fill, blocks_for(S, kThreads), kThreads, 0, L.segs.ostart,
                meshes, S, seg);
-  L.segs.seg = seg;
-  s->batch_meshes = meshes;
-  s->batch_meta_soff = L.segs.ostart;
-  s->batch_soff.assign(soff, soff + meshes + 1);
-  s->batch_pv_valid = false;
   return L;
 }
 
@@ -159,8 +163,8 @@ void batch_pairvote(Session *s, uint64_t n, const double *x, const double *y, co
   uint64_t *poff = scratch_as<uint64_t>(s, kSlotBatchPairOff, L.meshes + 1);
   uint64_t *roff = scratch_as<uint64_t>(s, kSlotBatchReflOff, L.meshes + 1);
   MVS_LAUNCH(s, "batch_offsets", k_mesh_offsets, blocks_for(L.meshes + 1, kThreads), kThreads, 0,
-             L.segs.ostart, plan_pair_offsets(s), L.meshes, r.refl_pair, r.reflections, poff,
-             roff);
+             L.segs.ostart, plan_pair_offsets(s), L.meshes, r.pairs, r.refl_pair, r.reflections,
+             poff, roff);
   out->all = r;
   out->meshes = L.meshes;
   out->pair_offsets = poff;
@@ -257,6 +261,131 @@ void batch_meanshift(Session *s, const uint64_t *soff_host, double ratio, uint64
   out->refl_seed_offsets = offf;
 }
 
+// Everything one batch step needs, as passed to the C entry points.
+struct BatchStepArgs {
+  uint64_t n;
+  const double *x, *y, *z, *feat;
+  uint32_t meshes;
+  const uint64_t *soff;
+  const int32_t *sel;
+  const double *box, *center;
+  double threshold;
+  uint64_t max_pairs, max_votes;
+  int precision;
+  double seeds_per_sample;
+  uint64_t seed_value;
+  int iterations;
+  double h, cutoff;
+};
+
+static void batch_step(Session *s, const BatchStepArgs &a, mvs_batch_pairvote_result *pv,
+                       mvs_batch_meanshift_result *ms) {
+  BatchLayout L = batch_setup(s, a.meshes, a.soff, a.box, a.center);
+  batch_pairvote(s, a.n, a.x, a.y, a.z, a.feat, L, a.sel, a.threshold, a.max_pairs, a.max_votes,
+                 a.precision, pv);
+  batch_meanshift(s, a.soff, a.seeds_per_sample, a.seed_value, a.iterations, a.h, a.cutoff,
+                  false, ms);
+}
+
+// Records the step eagerly, then captures it on the session's own stream with
+// the recorded counts frozen (see Replay).
+static void batch_graph_capture(Session *s, const BatchStepArgs &a, mvs_batch_pairvote_result *pv,
+                                mvs_batch_meanshift_result *ms) {
+  s->drop_graph();
+  hipStream_t user = s->stream;
+  MVS_HIP(hipStreamSynchronize(user));  // the caller's inputs are ready
+  const bool timing = s->timing;
+  struct Restore {
+    Session *s;
+    hipStream_t user;
+    bool timing;
+    ~Restore() {
+      s->stream = user;
+      s->replay = Replay::kOff;
+      s->timing = timing;
+    }
+  } restore{s, user, timing};
+  s->stream = s->own_stream;  // capture needs a stream other than the legacy one
+  s->timing = false;
+  s->replay = Replay::kRecord;
+  s->rec.clear();
+  batch_step(s, a, pv, ms);
+  // recorded values on the device, mismatch word cleared
+  s->rec_off.clear();
+  size_t total = 0;
+  for (const auto &v : s->rec) {
+    s->rec_off.push_back(total);
+    total += v.size();
+  }
+  std::vector<unsigned char> flat;
+  flat.reserve(total);
+  for (const auto &v : s->rec)
+    flat.insert(flat.end(), v.begin(), v.end());
+  s->rec_dev.ensure(total ? total : 1);
+  if (total)
+    MVS_HIP(hipMemcpy(s->rec_dev.ptr, flat.data(), total, hipMemcpyHostToDevice));
+  s->mismatch_dev = plan_flags(s) + kGraphMismatchWord;
+  zero_device(s, s->mismatch_dev, sizeof(unsigned));
+  MVS_HIP(hipStreamSynchronize(s->stream));
+  s->replay = Replay::kCapture;
+  s->rec_pos = 0;
+  MVS_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
+  hipGraph_t graph = nullptr;
+  try {
+    mvs_batch_pairvote_result pv2{};
+    mvs_batch_meanshift_result ms2{};
+    batch_step(s, a, &pv2, &ms2);
+  } catch (...) {
+    (void)hipStreamEndCapture(s->stream, &graph);
+    if (graph)
+      (void)hipGraphDestroy(graph);
+    throw;
+  }
+  MVS_HIP(hipStreamEndCapture(s->stream, &graph));
+  hipError_t e = hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    s->graph_exec = nullptr;
+    hip_check(e, "hipGraphInstantiate");
+  }
+}
+
+// Replays the captured step on the session's own stream, ordered after the
+// caller's stream and before its later work.
+static void batch_graph_launch(Session *s) {
+  if (!s->graph_exec)
+    raise(MVS_EINVAL, "no captured batch step on this session");
+  if (!s->graph_events[0]) {
+    MVS_HIP(hipEventCreateWithFlags(&s->graph_events[0], hipEventDisableTiming));
+    MVS_HIP(hipEventCreateWithFlags(&s->graph_events[1], hipEventDisableTiming));
+  }
+  const bool other = s->stream != s->own_stream;
+  if (other) {
+    MVS_HIP(hipEventRecord(s->graph_events[0], s->stream));
+    MVS_HIP(hipStreamWaitEvent(s->own_stream, s->graph_events[0], 0));
+  }
+  MVS_HIP(hipGraphLaunch(s->graph_exec, s->own_stream));
+  if (other) {
+    MVS_HIP(hipEventRecord(s->graph_events[1], s->own_stream));
+    MVS_HIP(hipStreamWaitEvent(s->stream, s->graph_events[1], 0));
+  }
+}
+
+static void batch_graph_check(Session *s) {
+  if (!s->graph_exec)
+    raise(MVS_EINVAL, "no captured batch step on this session");
+  MVS_HIP(hipStreamSynchronize(s->own_stream));
+  unsigned mm = 0;
+  MVS_HIP(hipMemcpy(&mm, s->mismatch_dev, sizeof mm, hipMemcpyDeviceToHost));
+  if (mm) {
+    const unsigned zero = 0;
+    MVS_HIP(hipMemcpy(s->mismatch_dev, &zero, sizeof zero, hipMemcpyHostToDevice));
+    raise(MVS_ERUNTIME,
+          "a graph replay read counts other than the captured step's: its inputs changed; "
+          "capture the step again");
+  }
+}
+
 }  // namespace mvs
 
 using namespace mvs;
@@ -289,6 +418,33 @@ int mvs_dev_batch_meanshift(mvs_session *sp, double seeds_per_sample, uint64_t s
     batch_meanshift(s, s->batch_soff.data(), seeds_per_sample, seed_value, iterations, h, cutoff,
                     want_stats != 0, out);
   });
+}
+
+int mvs_batch_graph_capture(mvs_session *sp, uint64_t n, const double *x, const double *y,
+                            const double *z, const double *feat, uint32_t meshes,
+                            const uint64_t *sample_offsets, const int32_t *selected,
+                            const double *box, const double *center, double threshold,
+                            uint64_t max_pairs, uint64_t max_votes, int precision,
+                            double seeds_per_sample, uint64_t seed_value, int iterations,
+                            double h, double cutoff, mvs_batch_pairvote_result *pv,
+                            mvs_batch_meanshift_result *ms) {
+  return guarded([&] {
+    if (!pv || !ms)
+      raise(MVS_EINVAL, "null argument");
+    BatchStepArgs a{n,         x,        y,         z,         feat,      meshes,
+                    sample_offsets,      selected,  box,       center,    threshold,
+                    max_pairs, max_votes, precision, seeds_per_sample, seed_value,
+                    iterations, h,       cutoff};
+    batch_graph_capture(as_session(sp), a, pv, ms);
+  });
+}
+
+int mvs_batch_graph_launch(mvs_session *sp) {
+  return guarded([&] { batch_graph_launch(as_session(sp)); });
+}
+
+int mvs_batch_graph_check(mvs_session *sp) {
+  return guarded([&] { batch_graph_check(as_session(sp)); });
 }
 
 }  // extern "C"

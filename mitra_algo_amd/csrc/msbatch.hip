@@ -695,8 +695,15 @@ static MsCtl *ms_ctl(Session *s) {
 void ms_begin(Session *s) { zero_device(s, ms_ctl(s), sizeof(MsCtl)); }
 
 void ms_finish(Session *s, unsigned long long sums[4]) {
-  MsCtl h;
-  read_back(s, &h, ms_ctl(s), sizeof h);
+  // the visit sums and the flag only (a captured step compares what it reads)
+  struct {
+    unsigned long long sums[4];
+    unsigned bad;
+  } h;
+  static_assert(offsetof(MsCtl, bad) == offsetof(MsCtl, sums) + 4 * sizeof(unsigned long long),
+                "sums and flag are adjacent");
+  read_back(s, &h, reinterpret_cast<const char *>(ms_ctl(s)) + offsetof(MsCtl, sums),
+            4 * sizeof(unsigned long long) + sizeof(unsigned));
   if (h.bad)
     raise(MVS_EINVAL, "non-finite mean-shift points");
   if (sums)

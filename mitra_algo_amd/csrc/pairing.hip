@@ -497,9 +497,12 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // Copies every slab (count <= kSlab) to its output offset.
+// Writes stay below `cap` (the pair capacity), also when a graph replay
+// meets counts that differ from the ones it was captured with.
 __global__ void __launch_bounds__(kThreads)
     k_pair_place(uint64_t S, const uint64_t *counts, const uint64_t *offsets,
-                 const uint32_t *slab, const int32_t *sel, int32_t *out_a, int32_t *out_b) {
+                 const uint32_t *slab, const int32_t *sel, int32_t *out_a, int32_t *out_b,
+                 uint64_t cap) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
   if (i >= S)
@@ -511,6 +514,8 @@ __global__ void __launch_bounds__(kThreads)
   for (uint64_t e = lane; e < c; e += kWave) {
     const uint32_t o = slab[i * kSlab + e];
     const uint64_t p = offsets[i] + e;
+    if (p >= cap)
+      break;
     out_a[p] = sel ? sel[i] : static_cast<int32_t>(i);
     out_b[p] = sel ? sel[o] : static_cast<int32_t>(o);
   }
@@ -560,7 +565,7 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void __launch_bounds__(kThreads)
     k_long_fill(PairArgs A, const uint32_t *long_list, uint64_t long_count, uint64_t chunks,
                 const uint64_t *chunk_off, const uint64_t *offsets, const int32_t *sel,
-                int32_t *out_a, int32_t *out_b) {
+                int32_t *out_a, int32_t *out_b, uint64_t cap) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t w = static_cast<uint64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
   if (w >= long_count * chunks)
@@ -575,7 +580,7 @@ __global__ void __launch_bounds__(kThreads)
   const double a0 = ri[L_K0], a1 = ri[L_K1], ax = ri[L_X], ay = ri[L_Y], az = ri[L_Z];
   const int32_t src = sel ? sel[i] : static_cast<int32_t>(i);
   uint64_t cursor = offsets[i] + (chunk_off[w] - chunk_off[src_w * chunks]);
-  const uint64_t end = offsets[i + 1];
+  const uint64_t end = offsets[i + 1] < cap ? offsets[i + 1] : cap;
   for (uint64_t b0 = lo; b0 < hi; b0 += kWave) {
     uint64_t b = b0 + lane;
     bool ok = false;
@@ -595,6 +600,16 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 __global__ void k_read_total(const uint64_t *off, uint64_t n, uint64_t *dst) { *dst = off[n]; }
+
+// Sets *mismatch when the device bytes differ from the recorded ones.
+__global__ void k_check_equal(const unsigned char *a, const unsigned char *b, uint64_t bytes,
+                              unsigned *mismatch) {
+  bool diff = false;
+  for (uint64_t i = threadIdx.x; i < bytes; i += blockDim.x)
+    diff = diff || a[i] != b[i];
+  if (diff)
+    atomicOr(mismatch, 1u);
+}
 
 __global__ void k_sum_stats(const uint32_t *a, const uint32_t *b, const uint32_t *c, uint64_t n,
                             unsigned long long *out) {
@@ -646,10 +661,26 @@ __global__ void k_compact(const int32_t *sample, const uint32_t *flag, const uin
 }  // namespace
 
 void read_back(Session *s, void *dst, const void *src, size_t bytes) {
+  if (s->replay == Replay::kCapture) {
+    // a captured step reads nothing back: it takes the recorded value and
+    // leaves a device check that the replayed value is the same
+    if (s->rec_pos >= s->rec.size() || s->rec[s->rec_pos].size() != bytes)
+      raise(MVS_EINVAL, "graph capture diverged from the recorded step");
+    std::memcpy(dst, s->rec[s->rec_pos].data(), bytes);
+    MVS_LAUNCH(s, "graph_check", k_check_equal, 1, kThreads, 0,
+               static_cast<const unsigned char *>(src), s->rec_dev.ptr + s->rec_off[s->rec_pos],
+               static_cast<uint64_t>(bytes), s->mismatch_dev);
+    ++s->rec_pos;
+    return;
+  }
   void *h = host_stage(s, bytes);
   MVS_HIP(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, s->stream));
   MVS_HIP(hipStreamSynchronize(s->stream));
   std::memcpy(dst, h, bytes);
+  if (s->replay == Replay::kRecord) {
+    const unsigned char *b = static_cast<const unsigned char *>(dst);
+    s->rec.emplace_back(b, b + bytes);
+  }
 }
 
 // Guard-band widths of the FP32 screen (relative).  Curvature: inputs rounded
@@ -819,7 +850,7 @@ void pair_fill_device(Session *s, int32_t *a, int32_t *b, bool ordinals) {
   PairArgs A = pair_args(st, S, plan.threshold, plan.box, guarded, plan_flags(s), plan.segs);
   const int32_t *sel = ordinals ? nullptr : plan.selected;
   MVS_LAUNCH(s, "pair_place", k_pair_place, blocks_for(S, kWaves), kThreads, 0, S, st.counts,
-             st.offsets, st.slab, sel, a, b);
+             st.offsets, st.slab, sel, a, b, plan.P);
   if (plan.long_sources) {
     const uint64_t L = plan.long_sources, chunks = (S + kLongChunk - 1) / kLongChunk;
     uint32_t *cc = scratch_as<uint32_t>(s, kSlotPlanLongCnt, L * chunks);
@@ -828,7 +859,7 @@ void pair_fill_device(Session *s, int32_t *a, int32_t *b, bool ordinals) {
     MVS_LAUNCH(s, "pair_long", k_long_count, grid, kThreads, 0, A, st.long_list, L, chunks, cc);
     exclusive_scan_u32(s, cc, co, L * chunks);
     MVS_LAUNCH(s, "pair_long", k_long_fill, grid, kThreads, 0, A, st.long_list, L, chunks, co,
-               st.offsets, sel, a, b);
+               st.offsets, sel, a, b, plan.P);
   }
 }
 

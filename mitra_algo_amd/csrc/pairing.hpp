@@ -18,11 +18,13 @@ constexpr int kRec = 16;
 // (kFlagsBytes bytes): 32-bit words 0 error flags, 1 long-source count,
 // 2-3 pair total (uint64), kRmaxWord the FP32 coordinate bound (float bits);
 // 64-bit word kVoteTotalWord the mirror-vote total, 32-bit word kVoteBadWord
-// (inside 64-bit word kVoteTotalWord + 2) the out-of-range ordinal flag.
+// (inside 64-bit word kVoteTotalWord + 2) the out-of-range ordinal flag,
+// 32-bit word kGraphMismatchWord the replay check of a captured step.
 constexpr size_t kFlagsBytes = 256;
 constexpr int kRmaxWord = 8;
 constexpr int kVoteTotalWord = 8;
 constexpr int kVoteBadWord = 2 * (kVoteTotalWord + 2);
+constexpr int kGraphMismatchWord = 40;  // set by a graph replay that read other values
 inline unsigned *plan_flags(Session *s) {
   return static_cast<unsigned *>(scratch(s, kSlotFlags, kFlagsBytes));
 }
@@ -109,6 +111,9 @@ void batch_pairvote(Session *s, uint64_t n, const double *x, const double *y, co
                     const double *feat, const BatchLayout &L, const int32_t *sel, double t,
                     uint64_t max_pairs, uint64_t max_votes, int precision,
                     mvs_batch_pairvote_result *out);
+void batch_meanshift(Session *s, const uint64_t *soff_host, double ratio, uint64_t seed_value,
+                     int iterations, double h, double cutoff, bool want_stats,
+                     mvs_batch_meanshift_result *out);
 
 // Sample records of the last pair count (valid until the next count).
 const double *plan_samples64(Session *s);

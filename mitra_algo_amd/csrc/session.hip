@@ -32,6 +32,9 @@ hipStream_t session_stream(Session *s) { return s->stream; }
 void *scratch(Session *s, int slot, size_t bytes) {
   auto &b = s->slots[slot];
   if ((bytes ? bytes : 1) > b.n) {
+    if (s->replay == Replay::kCapture)
+      raise(MVS_EINVAL, "graph capture needs the buffers of its recorded step");
+    s->drop_graph();  // a captured graph points at the old allocation
     // queued work may still read the old allocation
     MVS_HIP(hipStreamSynchronize(s->stream));
     b.alloc(bytes ? bytes : 1);
@@ -113,7 +116,19 @@ void Session::check_launch(const char *what) {
     raise(MVS_EHIP, std::string("HIP: launch ") + what + ": " + hipGetErrorString(e));
 }
 
+void Session::drop_graph() {
+  if (graph_exec) {
+    (void)hipStreamSynchronize(own_stream);
+    (void)hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+  }
+}
+
 Session::~Session() {
+  drop_graph();
+  for (auto e : graph_events)
+    if (e)
+      (void)hipEventDestroy(e);
   for (auto &p : pending) {
     (void)hipEventDestroy(p.start);
     (void)hipEventDestroy(p.stop);

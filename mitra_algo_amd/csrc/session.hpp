@@ -43,6 +43,11 @@ struct VotePlan {
   bool valid = false;
 };
 
+// Record / replay of a step for graph capture: kRecord runs eagerly and keeps
+// every value read back; kCapture runs under stream capture and takes those
+// values instead of reading (each site leaves a device check).
+enum class Replay { kOff, kRecord, kCapture };
+
 // Launch parameters picked by the autotuner (defaults: the hand-set values).
 struct Tuning {
   uint64_t wide_seeds = 8192;  // below this many seeds, a seed gets four waves
@@ -68,12 +73,22 @@ struct Session {
   std::map<std::string, std::pair<double, uint64_t>> timings;
   PairPlan pair_plan;
   VotePlan vote_plan;
-  uint64_t batch_meshes = 0;
+  uint64_t batch_meshes = 0;  // mesh count of the current batch call (sort key width)
   const uint64_t *batch_meta_soff = nullptr;  // device copy of the sample offsets
   std::vector<uint64_t> batch_soff;           // host copy
   mvs_batch_pairvote_result batch_pv{};
   bool batch_pv_valid = false;
-  Tuning tune;  // mesh count of the current batch call (sort key width)
+  Tuning tune;
+  // graph of one batch step (see Replay)
+  Replay replay = Replay::kOff;
+  std::vector<std::vector<unsigned char>> rec;  // values read back while recording
+  std::vector<size_t> rec_off;                  // their offsets in rec_dev
+  size_t rec_pos = 0;
+  DBuf<unsigned char> rec_dev;                  // the recorded values on the device
+  unsigned *mismatch_dev = nullptr;             // set by a replay that read other values
+  hipGraphExec_t graph_exec = nullptr;
+  hipEvent_t graph_events[2] = {nullptr, nullptr};  // orders replays with the caller's stream
+  void drop_graph();
   // outputs of the last fused pair + vote call (session-owned buffers)
   mvs_pairvote_result pairvote{};
   bool pairvote_valid = false;

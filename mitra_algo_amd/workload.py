@@ -210,3 +210,14 @@ class Batch:
         pv = self.pairvote(precision)
         ms = self.meanshift(seeds_per_sample, iterations)
         return pv, ms
+
+    def capture(self, precision: int, seeds_per_sample: float = 0.2, iterations: int = 20,
+                h: float = 0.05, cutoff: float = 3.0, seed_value: int = 7):
+        """Captures one batch step as a HIP graph (see mvs_batch_graph_capture)."""
+        return self.sess.batch_graph_capture(
+            self.n, self.x, self.y, self.z, self.feat, self.soff, self.selected, self.boxes,
+            self.centers, self.threshold, 2**31 - 1, 2**31 - 1, precision, seeds_per_sample,
+            seed_value, iterations, h, cutoff)
+
+    def replay(self):
+        self.sess.batch_graph_launch()

@@ -107,3 +107,44 @@ def test_batch_meanshift_equals_single_meshes(batch, prec):
         checked += 1
     assert checked > 0
     single.close()
+
+
+def test_batch_graph_replays_the_captured_step(batch):
+    """A captured batch step replays bit for bit; a replay over changed inputs
+    is reported by the check (and stays inside the captured buffers)."""
+    prec = capi.FP32_GUARDED
+    pv, ms = batch.capture(prec, 0.2, 12)
+    M = batch.count
+    first = _fetch(batch.sess, pv.all, prec)
+    cr = ms.all.rigid_seeds
+    modes = batch.sess.fetch(ms.all.rigid_modes, (6, cr), torch.float32, cr).cpu().clone()
+    poff = batch.sess.fetch(pv.pair_offsets, (M + 1,), torch.int64).cpu().clone()
+    torch.cuda.synchronize()
+    for _ in range(3):
+        batch.replay()
+        batch.sess.batch_graph_check()
+        again = _fetch(batch.sess, pv.all, prec)
+        torch.cuda.synchronize()
+        for k in first:
+            assert torch.equal(first[k], again[k]), k
+        assert torch.equal(modes, batch.sess.fetch(ms.all.rigid_modes, (6, cr), torch.float32,
+                                                   cr).cpu())
+        assert torch.equal(poff, batch.sess.fetch(pv.pair_offsets, (M + 1,), torch.int64).cpu())
+    # other curvatures on one mesh (kmin sign flipped on every other vertex:
+    # the ratio test is invariant under a common scale or sign, not under
+    # this): other pair counts, reported by the check
+    saved = batch.feat.clone()
+    lo, hi = batch.vbase[0], batch.vbase[1]
+    batch.feat[9, lo:hi:2] *= -1
+    torch.cuda.synchronize()
+    batch.replay()
+    with pytest.raises(capi.MvsError):
+        batch.sess.batch_graph_check()
+    batch.feat.copy_(saved)
+    torch.cuda.synchronize()
+    batch.replay()
+    batch.sess.batch_graph_check()
+    again = _fetch(batch.sess, pv.all, prec)
+    torch.cuda.synchronize()
+    for k in first:
+        assert torch.equal(first[k], again[k]), k
