@@ -358,6 +358,33 @@ int mvs_batch_graph_capture(mvs_session *s, uint64_t n, const double *x, const d
 int mvs_batch_graph_launch(mvs_session *s);
 int mvs_batch_graph_check(mvs_session *s);
 
+/* Launch table of a session: pair-walk work quota (sources a persistent wave
+ * claims at a time; 0 = one wave per source) and resident blocks, and the
+ * seed count below which a mean-shift seed gets a whole 256-lane block.
+ * mvs_batch_autotune captures the batch step under each candidate table,
+ * times three graph replays each (median, ms), writes every candidate with
+ * its time to `table` (capacity >= 16), keeps the fastest on the session and
+ * leaves the step captured with it (results in *pv, *ms as for
+ * mvs_batch_graph_capture).  Pairs and votes do not depend on the table;
+ * mean-shift modes only through the order of their FP sums (one wave or a
+ * block per seed). */
+typedef struct {
+  uint32_t walk_quota, walk_blocks;
+  uint64_t wide_seeds;
+  double ms;
+} mvs_tune_entry;
+int mvs_batch_autotune(mvs_session *s, uint64_t n, const double *x, const double *y,
+                       const double *z, const double *feat, uint32_t meshes,
+                       const uint64_t *sample_offsets, const int32_t *selected,
+                       const double *box, const double *center, double threshold,
+                       uint64_t max_pairs, uint64_t max_votes, int precision,
+                       double seeds_per_sample, uint64_t seed_value, int iterations, double h,
+                       double cutoff, mvs_tune_entry *table, uint32_t table_capacity,
+                       uint32_t *entries, uint32_t *best, mvs_batch_pairvote_result *pv,
+                       mvs_batch_meanshift_result *ms);
+int mvs_session_get_tuning(mvs_session *s, mvs_tune_entry *out);
+int mvs_session_set_tuning(mvs_session *s, const mvs_tune_entry *in);
+
 /* Batched-seed mean-shift (no reference counterpart; the reference runs
  * sequential seeds only, src/cpu/clustering.cpp:44-60).
  * mvs_dev_ms_points converts compact votes (layout of mvs_dev_vote_emit) to

@@ -27,7 +27,8 @@ EXPORTS = [
     "mvs_dev_vote_count", "mvs_dev_vote_emit", "mvs_dev_ms_points", "mvs_dev_ms_batched",
     "mvs_dev_pairvote", "mvs_dev_meanshift_votes", "mvs_dev_copy", "mvs_dev_batch_pairvote",
     "mvs_dev_batch_meanshift", "mvs_batch_graph_capture", "mvs_batch_graph_launch",
-    "mvs_batch_graph_check",
+    "mvs_batch_graph_check", "mvs_batch_autotune", "mvs_session_get_tuning",
+    "mvs_session_set_tuning",
 ]
 
 
@@ -128,6 +129,15 @@ class MeanShiftResult(C.Structure):
                 ("rigid_weight", C.c_void_p), ("refl_modes", C.c_void_p),
                 ("refl_weight", C.c_void_p), ("candidates", C.c_uint64),
                 ("in_support", C.c_uint64), ("precision", C.c_int)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("walk_quota", C.c_uint32), ("walk_blocks", C.c_uint32),
+                ("wide_seeds", C.c_uint64), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {"walk_quota": self.walk_quota, "walk_blocks": self.walk_blocks,
+                "wide_seeds": self.wide_seeds, "ms": self.ms}
 
 
 class BatchMeanShiftResult(C.Structure):
@@ -442,6 +452,33 @@ class Session:
             C.c_uint64(seed_value), C.c_int(iterations), C.c_double(h), C.c_double(cutoff),
             C.byref(pv), C.byref(ms)))
         return pv, ms
+
+    def batch_autotune(self, n, x, y, z, feat, sample_offsets, sel, boxes, centers, t,
+                       max_pairs, max_votes, precision, seeds_per_sample, seed_value,
+                       iterations, h, cutoff):
+        """Returns (table of candidate dicts with their replay ms, best index, pv, ms)."""
+        off = np.ascontiguousarray(sample_offsets, dtype=np.uint64)
+        bx = np.ascontiguousarray(boxes, dtype=np.float64)
+        ct = np.ascontiguousarray(centers, dtype=np.float64)
+        table = (TuneEntry * 16)()
+        n_ent, best = C.c_uint32(), C.c_uint32()
+        pv, ms = BatchPairVoteResult(), BatchMeanShiftResult()
+        check(lib().mvs_batch_autotune(
+            self.h, C.c_uint64(n), ptr(x), ptr(y), ptr(z), ptr(feat), C.c_uint32(len(off) - 1),
+            ptr(off), ptr(sel), ptr(bx), ptr(ct), C.c_double(t), C.c_uint64(max_pairs),
+            C.c_uint64(max_votes), C.c_int(precision), C.c_double(seeds_per_sample),
+            C.c_uint64(seed_value), C.c_int(iterations), C.c_double(h), C.c_double(cutoff),
+            table, C.c_uint32(16), C.byref(n_ent), C.byref(best), C.byref(pv), C.byref(ms)))
+        return [table[i].as_dict() for i in range(n_ent.value)], best.value, pv, ms
+
+    def get_tuning(self) -> dict:
+        e = TuneEntry()
+        check(lib().mvs_session_get_tuning(self.h, C.byref(e)))
+        return e.as_dict()
+
+    def set_tuning(self, walk_quota: int, walk_blocks: int, wide_seeds: int):
+        e = TuneEntry(walk_quota, walk_blocks, wide_seeds, 0.0)
+        check(lib().mvs_session_set_tuning(self.h, C.byref(e)))
 
     def batch_graph_launch(self):
         check(lib().mvs_batch_graph_launch(self.h))

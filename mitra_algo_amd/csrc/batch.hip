@@ -386,6 +386,62 @@ static void batch_graph_check(Session *s) {
   }
 }
 
+// Autotuner: captures the batch step under each candidate launch table and
+// times graph replays (HIP events on the replay stream, median of three);
+// the fastest table stays on the session, with its graph captured.
+static const Tuning kCandidates[] = {
+    // walk quota, walk blocks: one wave per source, or persistent claims of
+    // 1-16 sources from 1024-4096 resident blocks; wide seeds: when a
+    // mean-shift seed gets a whole block
+    {8192, 0, 2048},   {8192, 1, 2048}, {8192, 4, 2048}, {8192, 16, 2048}, {8192, 4, 1024},
+    {8192, 4, 4096},   {0, 0, 2048},    {0, 4, 2048},    {65536, 0, 2048}, {65536, 4, 2048},
+};
+
+static void batch_autotune(Session *s, const BatchStepArgs &a, mvs_tune_entry *table,
+                           uint32_t cap, uint32_t *entries, uint32_t *best,
+                           mvs_batch_pairvote_result *pv, mvs_batch_meanshift_result *ms) {
+  const uint32_t n = sizeof(kCandidates) / sizeof(kCandidates[0]);
+  if (!table || cap < n || !entries || !best)
+    raise(MVS_EINVAL, "the tuning table needs room for every candidate");
+  hipEvent_t ev[2];
+  MVS_HIP(hipEventCreate(&ev[0]));
+  MVS_HIP(hipEventCreate(&ev[1]));
+  struct Events {
+    hipEvent_t *e;
+    ~Events() {
+      (void)hipEventDestroy(e[0]);
+      (void)hipEventDestroy(e[1]);
+    }
+  } guard{ev};
+  uint32_t arg = 0;
+  for (uint32_t c = 0; c < n; ++c) {
+    s->tune = kCandidates[c];
+    batch_graph_capture(s, a, pv, ms);
+    batch_graph_launch(s);  // warm
+    batch_graph_check(s);
+    float t[3];
+    for (int r = 0; r < 3; ++r) {
+      MVS_HIP(hipEventRecord(ev[0], s->own_stream));
+      MVS_HIP(hipGraphLaunch(s->graph_exec, s->own_stream));
+      MVS_HIP(hipEventRecord(ev[1], s->own_stream));
+      MVS_HIP(hipEventSynchronize(ev[1]));
+      MVS_HIP(hipEventElapsedTime(&t[r], ev[0], ev[1]));
+    }
+    batch_graph_check(s);
+    std::sort(t, t + 3);
+    table[c].walk_quota = kCandidates[c].walk_quota;
+    table[c].walk_blocks = kCandidates[c].walk_blocks;
+    table[c].wide_seeds = kCandidates[c].wide_seeds;
+    table[c].ms = t[1];
+    if (table[c].ms < table[arg].ms)
+      arg = c;
+  }
+  *entries = n;
+  *best = arg;
+  s->tune = kCandidates[arg];
+  batch_graph_capture(s, a, pv, ms);
+}
+
 }  // namespace mvs
 
 using namespace mvs;
@@ -436,6 +492,50 @@ int mvs_batch_graph_capture(mvs_session *sp, uint64_t n, const double *x, const 
                     max_pairs, max_votes, precision, seeds_per_sample, seed_value,
                     iterations, h,       cutoff};
     batch_graph_capture(as_session(sp), a, pv, ms);
+  });
+}
+
+int mvs_batch_autotune(mvs_session *sp, uint64_t n, const double *x, const double *y,
+                       const double *z, const double *feat, uint32_t meshes,
+                       const uint64_t *sample_offsets, const int32_t *selected,
+                       const double *box, const double *center, double threshold,
+                       uint64_t max_pairs, uint64_t max_votes, int precision,
+                       double seeds_per_sample, uint64_t seed_value, int iterations, double h,
+                       double cutoff, mvs_tune_entry *table, uint32_t table_capacity,
+                       uint32_t *entries, uint32_t *best, mvs_batch_pairvote_result *pv,
+                       mvs_batch_meanshift_result *ms) {
+  return guarded([&] {
+    if (!pv || !ms)
+      raise(MVS_EINVAL, "null argument");
+    BatchStepArgs a{n,         x,        y,         z,         feat,      meshes,
+                    sample_offsets,      selected,  box,       center,    threshold,
+                    max_pairs, max_votes, precision, seeds_per_sample, seed_value,
+                    iterations, h,       cutoff};
+    batch_autotune(as_session(sp), a, table, table_capacity, entries, best, pv, ms);
+  });
+}
+
+int mvs_session_get_tuning(mvs_session *sp, mvs_tune_entry *out) {
+  return guarded([&] {
+    if (!out)
+      raise(MVS_EINVAL, "null argument");
+    const Tuning &t = as_session(sp)->tune;
+    out->walk_quota = t.walk_quota;
+    out->walk_blocks = t.walk_blocks;
+    out->wide_seeds = t.wide_seeds;
+    out->ms = 0;
+  });
+}
+
+int mvs_session_set_tuning(mvs_session *sp, const mvs_tune_entry *in) {
+  return guarded([&] {
+    if (!in || in->walk_blocks == 0)
+      raise(MVS_EINVAL, "invalid tuning entry");
+    Session *s = as_session(sp);
+    s->drop_graph();  // a captured step keeps the launch table it was captured with
+    s->tune.walk_quota = in->walk_quota;
+    s->tune.walk_blocks = in->walk_blocks;
+    s->tune.wide_seeds = in->wide_seeds;
   });
 }
 

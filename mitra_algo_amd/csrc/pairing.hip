@@ -28,6 +28,7 @@
 //               depends on a capacity.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -443,56 +444,80 @@ __device__ inline uint32_t wave_sort_u32(uint32_t v, int lane) {
 }
 
 // Single walk per source: counts every accepted pair and parks up to kSlab
-// of them, sorted, in the source's slab.
+// of them, sorted, in the source's slab.  quota == 0: wave w of the grid
+// walks source w.  quota > 0 (persistent): the resident waves claim `quota`
+// sources at a time from the queue counter until every source is taken, so
+// sources of uneven cost (and meshes of uneven size) balance across the
+// chip; every wave's last claim returns a start past the end, so all exit.
 template <bool kGuarded>
 __global__ void __launch_bounds__(kThreads)
     k_pair_walk(PairArgs Ain, uint64_t *counts, uint32_t *slab, uint32_t *long_list,
-                unsigned *long_count) {
+                unsigned *long_count, unsigned quota, unsigned *queue) {
   __shared__ WaveRanges ranges[kWaves];
   __shared__ uint32_t list[kWaves][kSlab];
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kWaves + wave;
-  if (i >= Ain.S)
-    return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
   const PairArgs A = with_bands<kGuarded>(Ain);
   uint32_t *L = list[wave];
-  uint64_t cnt = 0, cand2 = 0;
-  unsigned rechecks = 0;
-  walk_source<kGuarded>(A, i, ranges[wave], lane, rechecks, cand2, [&](bool ok, uint32_t o) {
-    uint64_t mask = ballot(ok);
-    uint64_t pos = cnt + static_cast<uint64_t>(lanes_below(mask));
-    if (ok && pos < static_cast<uint64_t>(kSlab))
-      L[pos] = o;
-    cnt += __popcll(mask);
-  });
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  if (cnt <= static_cast<uint64_t>(kWave)) {
-    uint32_t v = static_cast<uint64_t>(lane) < cnt ? L[lane] : kNone;
-    v = wave_sort_u32(v, lane);
-    if (static_cast<uint64_t>(lane) < cnt)
-      slab[i * kSlab + lane] = v;
-  } else if (cnt <= static_cast<uint64_t>(kSlab)) {
-    // rank = number of smaller ordinals (ordinals are distinct)
-    const uint32_t c = static_cast<uint32_t>(cnt);
-    for (uint32_t e = lane; e < c; e += kWave) {
-      uint32_t v = L[e], r = 0;
-      for (uint32_t k = 0; k < c; ++k)
-        r += L[k] < v;
-      slab[i * kSlab + r] = v;
+  auto walk_one = [&](uint64_t i) {
+    uint64_t cnt = 0, cand2 = 0;
+    unsigned rechecks = 0;
+    walk_source<kGuarded>(A, i, ranges[wave], lane, rechecks, cand2, [&](bool ok, uint32_t o) {
+      uint64_t mask = ballot(ok);
+      uint64_t pos = cnt + static_cast<uint64_t>(lanes_below(mask));
+      if (ok && pos < static_cast<uint64_t>(kSlab))
+        L[pos] = o;
+      cnt += __popcll(mask);
+    });
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (cnt <= static_cast<uint64_t>(kWave)) {
+      uint32_t v = static_cast<uint64_t>(lane) < cnt ? L[lane] : kNone;
+      v = wave_sort_u32(v, lane);
+      if (static_cast<uint64_t>(lane) < cnt)
+        slab[i * kSlab + lane] = v;
+    } else if (cnt <= static_cast<uint64_t>(kSlab)) {
+      // rank = number of smaller ordinals (ordinals are distinct)
+      const uint32_t c = static_cast<uint32_t>(cnt);
+      for (uint32_t e = lane; e < c; e += kWave) {
+        uint32_t v = L[e], r = 0;
+        for (uint32_t k = 0; k < c; ++k)
+          r += L[k] < v;
+        slab[i * kSlab + r] = v;
+      }
     }
+    rechecks = wave_sum(rechecks);
+    if (lane == 0) {
+      counts[i] = cnt;
+      if (cnt > static_cast<uint64_t>(kSlab)) {
+        unsigned s = atomicAdd(long_count, 1u);
+        long_list[s] = static_cast<uint32_t>(i);
+      }
+      uint64_t w = A.win[A.S + i] - A.win[i];
+      A.cand[i] = static_cast<uint32_t>(w < kNone ? w : kNone);
+      A.cand2[i] = static_cast<uint32_t>(cand2 < kNone ? cand2 : kNone);
+      A.rechk[i] = rechecks;
+    }
+    // the next source rewrites this wave's list and ranges
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  };
+  if (quota == 0) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kWaves + wave;
+    if (i < A.S)
+      walk_one(i);
+    return;
   }
-  rechecks = wave_sum(rechecks);
-  if (lane == 0) {
-    counts[i] = cnt;
-    if (cnt > static_cast<uint64_t>(kSlab)) {
-      unsigned s = atomicAdd(long_count, 1u);
-      long_list[s] = static_cast<uint32_t>(i);
-    }
-    uint64_t w = A.win[A.S + i] - A.win[i];
-    A.cand[i] = static_cast<uint32_t>(w < kNone ? w : kNone);
-    A.cand2[i] = static_cast<uint32_t>(cand2 < kNone ? cand2 : kNone);
-    A.rechk[i] = rechecks;
+  for (;;) {
+    unsigned base = 0;
+    if (lane == 0)
+      base = atomicAdd(queue, quota);
+    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, kWave));
+    if (base >= A.S)
+      break;
+    const uint64_t end = base + quota < A.S ? base + quota : A.S;
+    for (uint64_t i = base; i < end; ++i)
+      walk_one(i);
   }
 }
 
@@ -810,14 +835,18 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   MVS_LAUNCH(s, "pair_windows", k_windows, blocks_for(S, kThreads), kThreads, 0, st.r64,
              st.keys + S, S, t, segs.seg, st.win, st.kw);
   unsigned *long_count = flags + 1;
+  unsigned *queue = flags + 12;  // cleared with the flags above
   PairArgs A = pair_args(st, S, t, box, guarded, flags, segs);
+  const unsigned quota = s->tune.walk_quota;
   unsigned grid = blocks_for(S, kWaves);
+  if (quota)  // persistent: a fixed set of resident blocks
+    grid = std::min<unsigned>(grid, s->tune.walk_blocks);
   if (guarded)
     MVS_LAUNCH(s, "pair_walk", k_pair_walk<true>, grid, kThreads, 0, A, st.counts, st.slab,
-               st.long_list, long_count);
+               st.long_list, long_count, quota, queue);
   else
     MVS_LAUNCH(s, "pair_walk", k_pair_walk<false>, grid, kThreads, 0, A, st.counts, st.slab,
-               st.long_list, long_count);
+               st.long_list, long_count, quota, queue);
   exclusive_scan_u64(s, st.counts, st.offsets, S);
   // flags, long-source count and total read back in one synchronisation
   uint64_t *tot = reinterpret_cast<uint64_t *>(flags + 2);

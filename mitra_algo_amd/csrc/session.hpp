@@ -51,7 +51,8 @@ enum class Replay { kOff, kRecord, kCapture };
 // Launch parameters picked by the autotuner (defaults: the hand-set values).
 struct Tuning {
   uint64_t wide_seeds = 8192;  // below this many seeds, a seed gets four waves
-  unsigned walk_quota = 1;     // sources per wave in the pair walk
+  unsigned walk_quota = 0;     // sources per claim of a persistent pair walk (0: one per wave)
+  unsigned walk_blocks = 2048; // resident blocks of the persistent pair walk
 };
 
 struct Session {

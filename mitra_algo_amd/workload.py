@@ -221,3 +221,11 @@ class Batch:
 
     def replay(self):
         self.sess.batch_graph_launch()
+
+    def autotune(self, precision: int, seeds_per_sample: float = 0.2, iterations: int = 20,
+                 h: float = 0.05, cutoff: float = 3.0, seed_value: int = 7):
+        """Picks the launch table by timing graph replays (mvs_batch_autotune)."""
+        return self.sess.batch_autotune(
+            self.n, self.x, self.y, self.z, self.feat, self.soff, self.selected, self.boxes,
+            self.centers, self.threshold, 2**31 - 1, 2**31 - 1, precision, seeds_per_sample,
+            seed_value, iterations, h, cutoff)

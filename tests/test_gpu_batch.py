@@ -148,3 +148,33 @@ def test_batch_graph_replays_the_captured_step(batch):
     torch.cuda.synchronize()
     for k in first:
         assert torch.equal(first[k], again[k]), k
+
+
+def test_batch_autotune_keeps_results(batch):
+    """Every candidate launch table gives the same pairs and votes; the tuned
+    table stays on the session with its graph captured."""
+    prec = capi.FP32_GUARDED
+    batch.sess.set_tuning(0, 2048, 8192)
+    eager = _fetch(batch.sess, batch.pairvote(prec).all, prec)
+    torch.cuda.synchronize()
+    table, best, pv, ms = batch.autotune(prec, 0.2, 8)
+    assert len(table) >= 8 and 0 <= best < len(table)
+    assert all(t["ms"] > 0 for t in table)
+    assert min(t["ms"] for t in table) == table[best]["ms"]
+    tuned = batch.sess.get_tuning()
+    assert (tuned["walk_quota"], tuned["walk_blocks"], tuned["wide_seeds"]) == (
+        table[best]["walk_quota"], table[best]["walk_blocks"], table[best]["wide_seeds"])
+    batch.replay()
+    batch.sess.batch_graph_check()
+    got = _fetch(batch.sess, pv.all, prec)
+    torch.cuda.synchronize()
+    for k in eager:
+        assert torch.equal(eager[k], got[k]), k
+    # every persistent quota gives the same pairs (eager path)
+    for quota in (1, 3, 16):
+        batch.sess.set_tuning(quota, 64, 8192)
+        again = _fetch(batch.sess, batch.pairvote(prec).all, prec)
+        torch.cuda.synchronize()
+        for k in eager:
+            assert torch.equal(eager[k], again[k]), (quota, k)
+    batch.sess.set_tuning(0, 2048, 8192)
