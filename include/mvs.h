@@ -385,6 +385,31 @@ int mvs_batch_autotune(mvs_session *s, uint64_t n, const double *x, const double
 int mvs_session_get_tuning(mvs_session *s, mvs_tune_entry *out);
 int mvs_session_set_tuning(mvs_session *s, const mvs_tune_entry *in);
 
+/* Mode refinement (no reference counterpart for the energy).
+ * mvs_bvh_build keeps a BVH of the mesh surface on the session: median split
+ * of face centroids on the widest box axis, at most 8 faces per leaf, as the
+ * reference's verification surface (src/cpu/surface.h:12-96); coordinates
+ * relative to `center`, in FP32.  Host arrays in.
+ * mvs_dev_closest_points: for n centre-relative FP32 points (3 x n SoA) the
+ * closest surface point q (3 x n), the unit normal of its face (3 x n), the
+ * squared distance and the face id (optional); device arrays.
+ * mvs_dev_align_energy: per mode, the point-to-plane energy
+ * E = sum_j (n_j . (T(p_j) - q_j))^2 over its npts points (pts, q, nrm:
+ * modes x 3 x npts, centre-relative FP32) and its gradient (modes x P) and
+ * Gauss-Newton matrix 2 sum J^T J (modes x P x P, optional):
+ *   kind 0, rigid, P = 6: theta = (w, v), T(p) = exp([w]x) (R0 p + t0) + v,
+ *          base = modes x 12: R0 row-major then t0;
+ *   kind 1, mirror, P = 4: theta = (m, d), T(p) = p - 2 (u.p + d) u, u = m/|m|.
+ * FP32 arithmetic, sums in a fixed order. */
+int mvs_bvh_build(mvs_session *s, uint64_t nv, const double *x, const double *y,
+                  const double *z, uint64_t nf, const int32_t *f0, const int32_t *f1,
+                  const int32_t *f2, const double center[3]);
+int mvs_dev_closest_points(mvs_session *s, uint64_t n, const float *pts, float *q, float *nrm,
+                           float *dist2, int32_t *face);
+int mvs_dev_align_energy(mvs_session *s, int kind, uint32_t modes, uint32_t npts,
+                         const float *theta, const float *base, const float *pts, const float *q,
+                         const float *nrm, float *energy, float *grad, float *hess);
+
 /* Batched-seed mean-shift (no reference counterpart; the reference runs
  * sequential seeds only, src/cpu/clustering.cpp:44-60).
  * mvs_dev_ms_points converts compact votes (layout of mvs_dev_vote_emit) to

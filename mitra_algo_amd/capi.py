@@ -28,7 +28,8 @@ EXPORTS = [
     "mvs_dev_pairvote", "mvs_dev_meanshift_votes", "mvs_dev_copy", "mvs_dev_batch_pairvote",
     "mvs_dev_batch_meanshift", "mvs_batch_graph_capture", "mvs_batch_graph_launch",
     "mvs_batch_graph_check", "mvs_batch_autotune", "mvs_session_get_tuning",
-    "mvs_session_set_tuning",
+    "mvs_session_set_tuning", "mvs_bvh_build", "mvs_dev_closest_points",
+    "mvs_dev_align_energy",
 ]
 
 
@@ -479,6 +480,26 @@ class Session:
     def set_tuning(self, walk_quota: int, walk_blocks: int, wide_seeds: int):
         e = TuneEntry(walk_quota, walk_blocks, wide_seeds, 0.0)
         check(lib().mvs_session_set_tuning(self.h, C.byref(e)))
+
+    # ------------------------------------------------------ mode refinement
+    def bvh_build(self, vertices: np.ndarray, faces: np.ndarray, center):
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        x, y, z = (np.ascontiguousarray(v[:, k]) for k in range(3))
+        f0, f1, f2 = (np.ascontiguousarray(f[:, k]) for k in range(3))
+        ctr = (C.c_double * 3)(*[float(c) for c in center])
+        check(lib().mvs_bvh_build(self.h, C.c_uint64(len(v)), ptr(x), ptr(y), ptr(z),
+                                  C.c_uint64(len(f)), ptr(f0), ptr(f1), ptr(f2), ctr))
+
+    def dev_closest_points(self, n, pts, q, nrm, dist2, face=None):
+        check(lib().mvs_dev_closest_points(self.h, C.c_uint64(n), ptr(pts), ptr(q), ptr(nrm),
+                                           ptr(dist2), ptr(face)))
+
+    def dev_align_energy(self, kind, modes, npts, theta, base, pts, q, nrm, energy, grad,
+                         hess=None):
+        check(lib().mvs_dev_align_energy(self.h, C.c_int(kind), C.c_uint32(modes),
+                                         C.c_uint32(npts), ptr(theta), ptr(base), ptr(pts),
+                                         ptr(q), ptr(nrm), ptr(energy), ptr(grad), ptr(hess)))
 
     def batch_graph_launch(self):
         check(lib().mvs_batch_graph_launch(self.h))

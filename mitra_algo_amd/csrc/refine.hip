@@ -1,0 +1,474 @@
+// Mode refinement: closest points on the mesh through a BVH, and the
+// point-to-plane alignment energy of a transform with its adjoint (gradient)
+// and Gauss-Newton matrix.  No reference counterpart for the energy; the
+// closest-point query follows the reference's surface queries
+// (src/cpu/surface.h:12-96, src/core/math.cpp:72-112): a median-split BVH
+// with at most 8 faces per leaf and the exact point-triangle distance.
+//
+// Coordinates are relative to the model centre (FP32).
+//   rigid:   T(p) = R(w) (R0 p + t0) + v,  parameters (w, v), R(w) = exp([w]x)
+//   mirror:  T(p) = p - 2 (u.p + d) u,     u = m / |m|, parameters (m, d)
+// Residual of point j: r_j = n_j . (T(p_j) - q_j) with q_j, n_j the closest
+// mesh point of the current transform and its face normal (held fixed
+// while the parameters move).  E = sum r^2, g = dE/dtheta,
+// H = 2 sum J_j^T J_j (J_j = dr_j/dtheta).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "pairing.hpp"
+
+namespace mvs {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kLeaf = 8;
+constexpr int kStack = 32;  // median split: depth <= log2(faces / 8) + 1 < 32
+
+// BVH node: bounds, then (left, right) for an inner node or
+// (-(first + 1), end) into the leaf-ordered faces for a leaf.
+struct alignas(16) Node {
+  float lo[3];
+  int a;
+  float hi[3];
+  int b;
+};
+
+__device__ inline float box_dist2(const Node &n, float px, float py, float pz) {
+  float dx = fmaxf(fmaxf(n.lo[0] - px, 0.f), px - n.hi[0]);
+  float dy = fmaxf(fmaxf(n.lo[1] - py, 0.f), py - n.hi[1]);
+  float dz = fmaxf(fmaxf(n.lo[2] - pz, 0.f), pz - n.hi[2]);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+struct P3 {
+  float x, y, z;
+};
+__device__ inline P3 p3(float x, float y, float z) { return {x, y, z}; }
+__device__ inline P3 sub(P3 a, P3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline P3 add(P3 a, P3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline P3 mul(P3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ inline float dot3(P3 a, P3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline P3 cross3(P3 a, P3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// Closest point of segment ab to p.
+__device__ inline P3 closest_on_segment(P3 p, P3 a, P3 b) {
+  P3 d = sub(b, a);
+  float l = dot3(d, d);
+  float t = l > 0.f ? fminf(fmaxf(dot3(sub(p, a), d) / l, 0.f), 1.f) : 0.f;
+  return add(a, mul(d, t));
+}
+
+// Closest point of triangle abc to p by Voronoi regions (Ericson, Real-Time
+// Collision Detection, 2005, section 5.1.5); a degenerate triangle falls back
+// to its edges.
+__device__ inline P3 closest_on_triangle(P3 p, P3 a, P3 b, P3 c) {
+  P3 ab = sub(b, a), ac = sub(c, a);
+  P3 nn = cross3(ab, ac);
+  if (dot3(nn, nn) < 1e-24f) {
+    P3 q0 = closest_on_segment(p, a, b), q1 = closest_on_segment(p, a, c),
+       q2 = closest_on_segment(p, b, c);
+    float e0 = dot3(sub(p, q0), sub(p, q0)), e1 = dot3(sub(p, q1), sub(p, q1)),
+          e2 = dot3(sub(p, q2), sub(p, q2));
+    return e0 <= e1 && e0 <= e2 ? q0 : (e1 <= e2 ? q1 : q2);
+  }
+  P3 ap = sub(p, a);
+  float d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  if (d1 <= 0.f && d2 <= 0.f)
+    return a;
+  P3 bp = sub(p, b);
+  float d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  if (d3 >= 0.f && d4 <= d3)
+    return b;
+  float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f)
+    return add(a, mul(ab, d1 / (d1 - d3)));
+  P3 cp = sub(p, c);
+  float d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+  if (d6 >= 0.f && d5 <= d6)
+    return c;
+  float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f)
+    return add(a, mul(ac, d2 / (d2 - d6)));
+  float va = d3 * d6 - d5 * d4;
+  if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f)
+    return add(b, mul(sub(c, b), (d4 - d3) / ((d4 - d3) + (d5 - d6))));
+  float den = 1.f / (va + vb + vc);
+  return add(a, add(mul(ab, vb * den), mul(ac, vc * den)));
+}
+
+// One thread per query: depth-first, nearer child first, subtrees whose box
+// is farther than the best distance skipped; the stack lives in LDS.
+__global__ void __launch_bounds__(kThreads)
+    k_closest(const Node *nodes, const float *tri, const float *fnrm, const int32_t *face_id,
+              uint64_t n, const float *pts, float *q, float *nrm, float *dist2, int32_t *face) {
+  __shared__ int stack[kStack][kThreads];
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const P3 p = p3(pts[i], pts[n + i], pts[2 * n + i]);
+  float best = INFINITY;
+  P3 bq = p;
+  int bf = -1;
+  int top = 0;
+  stack[top++][threadIdx.x] = 0;
+  while (top > 0) {
+    const Node nd = nodes[stack[--top][threadIdx.x]];
+    if (box_dist2(nd, p.x, p.y, p.z) > best)
+      continue;
+    if (nd.a < 0) {
+      for (int k = -nd.a - 1; k < nd.b; ++k) {
+        const float *t = tri + 9 * static_cast<uint64_t>(k);
+        P3 c = closest_on_triangle(p, p3(t[0], t[1], t[2]), p3(t[3], t[4], t[5]),
+                                   p3(t[6], t[7], t[8]));
+        P3 e = sub(p, c);
+        float d = dot3(e, e);
+        if (d < best) {
+          best = d;
+          bq = c;
+          bf = k;
+        }
+      }
+    } else {
+      const Node l = nodes[nd.a], r = nodes[nd.b];
+      const float dl = box_dist2(l, p.x, p.y, p.z), dr = box_dist2(r, p.x, p.y, p.z);
+      // push the farther child first so the nearer one is searched first
+      if (dl <= dr) {
+        if (dr <= best && top < kStack)
+          stack[top++][threadIdx.x] = nd.b;
+        if (dl <= best && top < kStack)
+          stack[top++][threadIdx.x] = nd.a;
+      } else {
+        if (dl <= best && top < kStack)
+          stack[top++][threadIdx.x] = nd.a;
+        if (dr <= best && top < kStack)
+          stack[top++][threadIdx.x] = nd.b;
+      }
+    }
+  }
+  q[i] = bq.x, q[n + i] = bq.y, q[2 * n + i] = bq.z;
+  if (bf >= 0) {
+    nrm[i] = fnrm[3 * bf], nrm[n + i] = fnrm[3 * bf + 1], nrm[2 * n + i] = fnrm[3 * bf + 2];
+  } else {
+    nrm[i] = nrm[n + i] = nrm[2 * n + i] = 0.f;
+  }
+  dist2[i] = best;
+  if (face)
+    face[i] = bf >= 0 ? face_id[bf] : -1;
+}
+
+// Rodrigues coefficients of R(w) = I + A K + B K^2 (K = [w]x, t = |w|),
+// A = sin t / t, B = (1 - cos t) / t^2, and their derivatives divided by t,
+// A'/t and B'/t; Taylor series below t = 0.1, where the closed forms cancel
+// in FP32.
+__device__ inline void rodrigues(float t, float &A, float &B, float &dA, float &dB) {
+  const float t2 = t * t, t4 = t2 * t2;
+  if (t < 0.1f) {
+    A = 1.f - t2 / 6.f + t4 / 120.f;
+    B = 0.5f - t2 / 24.f + t4 / 720.f;
+    dA = -1.f / 3.f + t2 / 30.f - t4 / 840.f;
+    dB = -1.f / 12.f + t2 / 180.f - t4 / 6720.f;
+  } else {
+    const float s = sinf(t), c = cosf(t);
+    A = s / t;
+    B = (1.f - c) / t2;
+    dA = (t * c - s) / (t2 * t);
+    dB = (t * s - 2.f * (1.f - c)) / (t2 * t2);
+  }
+}
+
+template <int NP> __device__ inline void block_sum(float *v, float *sh) {
+  // v[0..NP) summed over the block into v (every thread), fixed order
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  for (int k = 0; k < NP; ++k)
+    for (int o = kWave / 2; o > 0; o >>= 1)
+      v[k] += __shfl_xor(v[k], o, kWave);
+  if (lane == 0)
+    for (int k = 0; k < NP; ++k)
+      sh[wave * NP + k] = v[k];
+  __syncthreads();
+  for (int k = 0; k < NP; ++k) {
+    float s = 0.f;
+    for (int w = 0; w < kThreads / kWave; ++w)
+      s += sh[w * NP + k];
+    v[k] = s;
+  }
+}
+
+// One block per mode: residuals of its npts points, E, g (NP) and the upper
+// triangle of H (NP (NP + 1) / 2), summed in FP32 with a fixed order.
+template <int KIND>
+__global__ void __launch_bounds__(kThreads)
+    k_align(uint32_t modes, uint32_t npts, const float *theta, const float *base,
+            const float *pts, const float *q, const float *nrm, float *energy, float *grad,
+            float *hess) {
+  constexpr int NP = KIND == 0 ? 6 : 4;
+  constexpr int NH = NP * (NP + 1) / 2;
+  constexpr int NV = 1 + NP + NH;
+  __shared__ float sh[(kThreads / kWave) * NV];
+  const uint32_t m = blockIdx.x;
+  if (m >= modes)
+    return;
+  const float *th = theta + static_cast<uint64_t>(m) * NP;
+  const uint64_t off = static_cast<uint64_t>(m) * npts * 3;
+  const float *px = pts + off, *py = px + npts, *pz = py + npts;
+  const float *qx = q + off, *qy = qx + npts, *qz = qy + npts;
+  const float *nx = nrm + off, *ny = nx + npts, *nz = ny + npts;
+  float acc[NV];
+  for (int k = 0; k < NV; ++k)
+    acc[k] = 0.f;
+  // per-mode constants
+  P3 w{}, v{}, u{};
+  float A = 1, B = 0.5f, dA = 0, dB = 0, R0[9] = {}, t0[3] = {}, d = 0, inv_m = 1;
+  if (KIND == 0) {
+    w = p3(th[0], th[1], th[2]);
+    v = p3(th[3], th[4], th[5]);
+    rodrigues(sqrtf(dot3(w, w)), A, B, dA, dB);
+    for (int k = 0; k < 9; ++k)
+      R0[k] = base[12 * m + k];
+    for (int k = 0; k < 3; ++k)
+      t0[k] = base[12 * m + 9 + k];
+  } else {
+    P3 mm = p3(th[0], th[1], th[2]);
+    inv_m = 1.f / sqrtf(dot3(mm, mm));
+    u = mul(mm, inv_m);
+    d = th[3];
+  }
+  for (uint32_t j = threadIdx.x; j < npts; j += kThreads) {
+    const P3 p = p3(px[j], py[j], pz[j]);
+    const P3 qq = p3(qx[j], qy[j], qz[j]);
+    const P3 n = p3(nx[j], ny[j], nz[j]);
+    float J[NP];
+    float r;
+    if (KIND == 0) {
+      const P3 y = p3(R0[0] * p.x + R0[1] * p.y + R0[2] * p.z + t0[0],
+                      R0[3] * p.x + R0[4] * p.y + R0[5] * p.z + t0[1],
+                      R0[6] * p.x + R0[7] * p.y + R0[8] * p.z + t0[2]);
+      const P3 ky = cross3(w, y), kky = cross3(w, ky);
+      const P3 T = add(add(add(y, mul(ky, A)), mul(kky, B)), v);
+      r = dot3(n, sub(T, qq));
+      // dr/dw = A (y x n) + B ((Ky) x n + y x (n x w)) + dA (n.Ky) w + dB (n.K^2 y) w
+      const P3 g1 = cross3(y, n), g2 = add(cross3(ky, n), cross3(y, cross3(n, w)));
+      const float s1 = dA * dot3(n, ky), s2 = dB * dot3(n, kky);
+      J[0] = A * g1.x + B * g2.x + (s1 + s2) * w.x;
+      J[1] = A * g1.y + B * g2.y + (s1 + s2) * w.y;
+      J[2] = A * g1.z + B * g2.z + (s1 + s2) * w.z;
+      J[3] = n.x, J[4] = n.y, J[5] = n.z;  // dr/dv
+    } else {
+      const float s = dot3(u, p) + d;
+      const P3 T = sub(p, mul(u, 2.f * s));
+      r = dot3(n, sub(T, qq));
+      // dr/du = -2 ((n.u) p + s n); dr/dm = (I - u u^T) dr/du / |m|; dr/dd = -2 n.u
+      const float nu = dot3(n, u);
+      const P3 gu = mul(add(mul(p, nu), mul(n, s)), -2.f);
+      const P3 gm = mul(sub(gu, mul(u, dot3(u, gu))), inv_m);
+      J[0] = gm.x, J[1] = gm.y, J[2] = gm.z, J[3] = -2.f * nu;
+    }
+    acc[0] = __builtin_fmaf(r, r, acc[0]);
+    for (int k = 0; k < NP; ++k)
+      acc[1 + k] = __builtin_fmaf(2.f * r, J[k], acc[1 + k]);
+    int h = 1 + NP;
+    for (int a = 0; a < NP; ++a)
+      for (int b = a; b < NP; ++b) {
+        acc[h] = __builtin_fmaf(2.f * J[a], J[b], acc[h]);
+        ++h;
+      }
+  }
+  block_sum<NV>(acc, sh);
+  if (threadIdx.x == 0) {
+    energy[m] = acc[0];
+    for (int k = 0; k < NP; ++k)
+      grad[static_cast<uint64_t>(m) * NP + k] = acc[1 + k];
+    if (hess) {
+      int h = 1 + NP;
+      for (int a = 0; a < NP; ++a)
+        for (int b = a; b < NP; ++b) {
+          hess[(static_cast<uint64_t>(m) * NP + a) * NP + b] = acc[h];
+          hess[(static_cast<uint64_t>(m) * NP + b) * NP + a] = acc[h];
+          ++h;
+        }
+    }
+  }
+}
+
+// ------------------------------------------------------------- BVH build
+struct BuildNode {
+  float lo[3], hi[3];
+  int left = -1, right = -1, begin = 0, end = 0;
+};
+
+}  // namespace
+
+// Host median-split build over face centroids (the widest box axis, at most
+// kLeaf faces per leaf), in centre-relative FP32.
+static void bvh_build(Session *s, uint64_t nv, const double *x, const double *y, const double *z,
+               uint64_t nf, const int32_t *f0, const int32_t *f1, const int32_t *f2,
+               const double *center) {
+  if (nf == 0 || nf > static_cast<uint64_t>(INT32_MAX / 2))
+    raise(MVS_EINVAL, "a BVH needs between 1 and 2^30 faces");
+  for (uint64_t f = 0; f < nf; ++f)
+    for (int32_t v : {f0[f], f1[f], f2[f]})
+      if (v < 0 || static_cast<uint64_t>(v) >= nv)
+        raise(MVS_EINVAL, "face vertex index out of range");
+  std::vector<float> vx(nv), vy(nv), vz(nv);
+  for (uint64_t i = 0; i < nv; ++i) {
+    vx[i] = static_cast<float>(x[i] - center[0]);
+    vy[i] = static_cast<float>(y[i] - center[1]);
+    vz[i] = static_cast<float>(z[i] - center[2]);
+  }
+  std::vector<int> ids(nf);
+  std::iota(ids.begin(), ids.end(), 0);
+  std::vector<float> cen(3 * nf);
+  for (uint64_t f = 0; f < nf; ++f) {
+    const int32_t v[3] = {f0[f], f1[f], f2[f]};
+    cen[3 * f] = (vx[v[0]] + vx[v[1]] + vx[v[2]]) / 3.f;
+    cen[3 * f + 1] = (vy[v[0]] + vy[v[1]] + vy[v[2]]) / 3.f;
+    cen[3 * f + 2] = (vz[v[0]] + vz[v[1]] + vz[v[2]]) / 3.f;
+  }
+  std::vector<BuildNode> nodes;
+  nodes.reserve(2 * (nf / kLeaf + 1));
+  // iterative build: (node index, begin, end)
+  struct Job {
+    int node, begin, end;
+  };
+  std::vector<Job> jobs;
+  nodes.emplace_back();
+  jobs.push_back({0, 0, static_cast<int>(nf)});
+  while (!jobs.empty()) {
+    Job jb = jobs.back();
+    jobs.pop_back();
+    BuildNode nd;
+    for (int k = 0; k < 3; ++k)
+      nd.lo[k] = INFINITY, nd.hi[k] = -INFINITY;
+    for (int i = jb.begin; i < jb.end; ++i) {
+      const int f = ids[i];
+      for (int32_t v : {f0[f], f1[f], f2[f]}) {
+        const float c[3] = {vx[v], vy[v], vz[v]};
+        for (int k = 0; k < 3; ++k)
+          nd.lo[k] = std::min(nd.lo[k], c[k]), nd.hi[k] = std::max(nd.hi[k], c[k]);
+      }
+    }
+    nd.begin = jb.begin;
+    nd.end = jb.end;
+    if (jb.end - jb.begin > kLeaf) {
+      int axis = 0;
+      for (int k = 1; k < 3; ++k)
+        if (nd.hi[k] - nd.lo[k] > nd.hi[axis] - nd.lo[axis])
+          axis = k;
+      const int mid = (jb.begin + jb.end) / 2;
+      std::nth_element(ids.begin() + jb.begin, ids.begin() + mid, ids.begin() + jb.end,
+                       [&](int a, int b) {
+                         return cen[3 * a + axis] < cen[3 * b + axis] ||
+                                (cen[3 * a + axis] == cen[3 * b + axis] && a < b);
+                       });
+      nd.left = static_cast<int>(nodes.size());
+      nd.right = nd.left + 1;
+      nodes.emplace_back();
+      nodes.emplace_back();
+      jobs.push_back({nd.right, mid, jb.end});
+      jobs.push_back({nd.left, jb.begin, mid});
+    }
+    nodes[jb.node] = nd;
+  }
+  std::vector<Node> dn(nodes.size());
+  for (size_t i = 0; i < nodes.size(); ++i) {
+    const BuildNode &b = nodes[i];
+    for (int k = 0; k < 3; ++k)
+      dn[i].lo[k] = b.lo[k], dn[i].hi[k] = b.hi[k];
+    if (b.left >= 0)
+      dn[i].a = b.left, dn[i].b = b.right;
+    else
+      dn[i].a = -(b.begin + 1), dn[i].b = b.end;
+  }
+  std::vector<float> tri(9 * nf), fn(3 * nf);
+  std::vector<int32_t> fid(nf);
+  for (uint64_t k = 0; k < nf; ++k) {
+    const int f = ids[k];
+    fid[k] = f;
+    const int32_t v[3] = {f0[f], f1[f], f2[f]};
+    for (int c = 0; c < 3; ++c)
+      tri[9 * k + 3 * c] = vx[v[c]], tri[9 * k + 3 * c + 1] = vy[v[c]],
+                   tri[9 * k + 3 * c + 2] = vz[v[c]];
+    // unit face normal from the FP64 coordinates
+    const double ax = x[v[1]] - x[v[0]], ay = y[v[1]] - y[v[0]], az = z[v[1]] - z[v[0]];
+    const double bx = x[v[2]] - x[v[0]], by = y[v[2]] - y[v[0]], bz = z[v[2]] - z[v[0]];
+    double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+    const double l = std::sqrt(cx * cx + cy * cy + cz * cz);
+    if (l > 0)
+      cx /= l, cy /= l, cz /= l;
+    fn[3 * k] = static_cast<float>(cx), fn[3 * k + 1] = static_cast<float>(cy),
+           fn[3 * k + 2] = static_cast<float>(cz);
+  }
+  s->bvh_nodes.alloc(dn.size() * sizeof(Node));
+  s->bvh_tri.alloc(tri.size() * sizeof(float));
+  s->bvh_fn.alloc(fn.size() * sizeof(float));
+  s->bvh_fid.alloc(fid.size() * sizeof(int32_t));
+  MVS_HIP(hipMemcpy(s->bvh_nodes.ptr, dn.data(), dn.size() * sizeof(Node), hipMemcpyHostToDevice));
+  MVS_HIP(hipMemcpy(s->bvh_tri.ptr, tri.data(), tri.size() * sizeof(float),
+                    hipMemcpyHostToDevice));
+  MVS_HIP(hipMemcpy(s->bvh_fn.ptr, fn.data(), fn.size() * sizeof(float), hipMemcpyHostToDevice));
+  MVS_HIP(hipMemcpy(s->bvh_fid.ptr, fid.data(), fid.size() * sizeof(int32_t),
+                    hipMemcpyHostToDevice));
+  s->bvh_faces = nf;
+}
+
+}  // namespace mvs
+
+using namespace mvs;
+
+extern "C" {
+
+int mvs_bvh_build(mvs_session *sp, uint64_t nv, const double *x, const double *y,
+                  const double *z, uint64_t nf, const int32_t *f0, const int32_t *f1,
+                  const int32_t *f2, const double center[3]) {
+  return guarded([&] {
+    if (!x || !y || !z || !f0 || !f1 || !f2 || !center)
+      raise(MVS_EINVAL, "null argument");
+    bvh_build(as_session(sp), nv, x, y, z, nf, f0, f1, f2, center);
+  });
+}
+
+int mvs_dev_closest_points(mvs_session *sp, uint64_t n, const float *pts, float *q, float *nrm,
+                           float *dist2, int32_t *face) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    if (!s->bvh_faces)
+      raise(MVS_EINVAL, "closest points need a BVH (mvs_bvh_build)");
+    if (n)
+      MVS_LAUNCH(s, "closest", k_closest, blocks_for(n, kThreads), kThreads, 0,
+                 reinterpret_cast<const Node *>(s->bvh_nodes.ptr),
+                 reinterpret_cast<const float *>(s->bvh_tri.ptr),
+                 reinterpret_cast<const float *>(s->bvh_fn.ptr),
+                 reinterpret_cast<const int32_t *>(s->bvh_fid.ptr), n, pts, q, nrm, dist2, face);
+  });
+}
+
+int mvs_dev_align_energy(mvs_session *sp, int kind, uint32_t modes, uint32_t npts,
+                         const float *theta, const float *base, const float *pts, const float *q,
+                         const float *nrm, float *energy, float *grad, float *hess) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    if (kind != 0 && kind != 1)
+      raise(MVS_EINVAL, "kind is 0 (rigid) or 1 (mirror)");
+    if (kind == 0 && !base)
+      raise(MVS_EINVAL, "rigid modes need their base transforms");
+    if (!theta || !pts || !q || !nrm || !energy || !grad)
+      raise(MVS_EINVAL, "null argument");
+    if (modes == 0)
+      return;
+    if (kind == 0)
+      MVS_LAUNCH(s, "align", k_align<0>, modes, kThreads, 0, modes, npts, theta, base, pts, q,
+                 nrm, energy, grad, hess);
+    else
+      MVS_LAUNCH(s, "align", k_align<1>, modes, kThreads, 0, modes, npts, theta, base, pts, q,
+                 nrm, energy, grad, hess);
+  });
+}
+
+}  // extern "C"

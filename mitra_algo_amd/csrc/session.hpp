@@ -90,6 +90,9 @@ struct Session {
   hipGraphExec_t graph_exec = nullptr;
   hipEvent_t graph_events[2] = {nullptr, nullptr};  // orders replays with the caller's stream
   void drop_graph();
+  // mesh surface for closest-point queries (centre-relative FP32 BVH)
+  DBuf<unsigned char> bvh_nodes, bvh_tri, bvh_fn, bvh_fid;
+  uint64_t bvh_faces = 0;
   // outputs of the last fused pair + vote call (session-owned buffers)
   mvs_pairvote_result pairvote{};
   bool pairvote_valid = false;

@@ -3,6 +3,7 @@
 // same C signatures as oracle.h, prefixed ref_.  Used only to pin the oracle
 // restatement and as the CPU baseline; never linked into the product.
 #include "oracle.h"
+#include "symmetry/math.h"
 #include "symmetry/pipeline.h"
 #include "symmetry/sampling.h"
 
@@ -285,4 +286,11 @@ int ref_clustering(int64_t nv, const uint8_t *kind, const double *desc,
                 r.clusters.votes.indices.size() * sizeof(int32_t));
   });
 }
+// The reference's exact squared point-triangle distance (math.cpp:72-112).
+double ref_point_triangle_distance2(const double *p, const double *a, const double *b,
+                                    const double *c) {
+  return point_triangle_distance2(Vec{p[0], p[1], p[2]}, Vec{a[0], a[1], a[2]},
+                                  Vec{b[0], b[1], b[2]}, Vec{c[0], c[1], c[2]});
+}
+
 } // extern "C"
