@@ -173,6 +173,122 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_refine(args):
+    """Config 5: the 64 heaviest distinct FP32 modes of the 96x96 height field
+    (2,000 samples, t = .05, mean-shift h = .05), 20,000 patch points each
+    (the reference's verification rule at the unrefined mode), 10 damped
+    Gauss-Newton steps per mode on the point-to-plane energy with closest
+    points recomputed each step.  One step = the 10 iterations of all modes
+    from the unrefined state; value = modes refined per second."""
+    import numpy as np
+    import torch
+    from mitra_algo_amd import refine, workload
+    npts, top, iters = 20000, 64, 10
+    wl = workload.cfg1()
+    modes = refine.top_modes(wl, top)
+    patches = refine.Patches(wl.sess, wl.mesh.vertices, wl.mesh.faces, wl.center, wl.box)
+    idx = patches.select(modes, npts)
+    probs = [p for p in (patches.problem(modes, idx, refine.RIGID)[0],
+                         patches.problem(modes, idx, refine.MIRROR)[0]) if p is not None]
+    start = [(p.base.clone() if p.base is not None else None, p.theta.clone()) for p in probs]
+
+    def reset():
+        for p, (b, t) in zip(probs, start):
+            p.base = b.clone() if b is not None else None
+            p.theta = t.clone()
+
+    def step():
+        reset()
+        return [refine.refine(p, steps=iters) for p in probs]
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    wl.sess.enable_timing(True, only=("closest", "align"))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        hist = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    kern = {}
+    for k in ("closest", "align"):
+        ms_, n = wl.sess.timing(k)
+        if n:
+            kern[k] = {"ms_total_per_step": ms_ / args.steps, "ms_avg": ms_ / n,
+                       "launches_per_step": n / args.steps}
+    wl.sess.enable_timing(False)
+    M = sum(p.modes for p in probs)
+    # bytes per launch: closest reads 12 and writes 28 bytes per point (all
+    # modes of a kind in one launch); align reads 36 bytes per point
+    launches = 2 * (iters + 1) * len(probs)
+    per_launch_pts = M * npts / len(probs)
+    roofs = {}
+    for k, bpp in (("closest", 40), ("align", 36)):
+        if k in kern:
+            b = bpp * per_launch_pts
+            gbs = b / (kern[k]["ms_avg"] * 1e-3) / 1e9
+            roofs[k] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": b,
+                        "ms_avg": kern[k]["ms_avg"]}
+    dom = max(roofs, key=lambda k: kern[k]["ms_total_per_step"])
+    roof = dict(roofs[dom], kernel=dom)
+    resid = {}
+    for p, h in zip(probs, hist):
+        before, after = refine.rms_residual(h[0], npts), refine.rms_residual(h[-1], npts)
+        resid["rigid" if p.kind == refine.RIGID else "mirror"] = {
+            "modes": p.modes, "rms_before_median": float(np.median(before)),
+            "rms_after_median": float(np.median(after)), "rms_before_max": float(before.max()),
+            "rms_after_max": float(after.max()),
+            "modes_lowered": int(np.sum(after < before))}
+    out = {
+        "metric": "modes/s: 10 Gauss-Newton steps of point-to-plane refinement (top-64 modes)",
+        "value": M / dt, "unit": "modes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"96x96 height field (18050 triangles), 2000 samples, t=0.05; "
+                               f"top {top} FP32 modes ({M} with support), {npts} patch points "
+                               f"each, {iters} Gauss-Newton steps", "parallelism": "single GPU",
+                   "support": [m.support for m in modes]},
+        "residual": resid, "kernels": kern, "roofline": roof, "rooflines": roofs,
+        "launches_per_step": launches,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_refine(probs)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_refine(probs):
+    """The FP64 CPU reference of one energy + gradient evaluation of every
+    mode (closest points of the last GPU step held fixed), one thread; the
+    GPU's FP32 gradients at the same inputs are compared with it."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    from oracle import refine_ref as rr
+    evals, worst, t = 0, 0.0, 0.0
+    with threadpool_limits(1):
+        for p in probs:
+            E, g, _ = p.evaluate(p.theta, want_hessian=False)
+            g = g.double().cpu().numpy()
+            th = p.theta.double().cpu().numpy()
+            for j in range(p.modes):
+                P, Q, N = (x[j].double().cpu().numpy().T for x in (p.pts, p.q, p.nrm))
+                R0 = t0 = None
+                if p.base is not None:
+                    b = p.base[j].double().cpu().numpy()
+                    R0, t0 = b[:9].reshape(3, 3), b[9:]
+                a = time.perf_counter()
+                rr.energy(p.kind, th[j], P, Q, N, R0, t0)
+                g64 = rr.gradient(p.kind, th[j], P, Q, N, R0, t0)
+                t += time.perf_counter() - a
+                evals += 1
+                if np.linalg.norm(g64) >= 1.0:
+                    worst = max(worst, float(np.linalg.norm(g[j] - g64) / np.linalg.norm(g64)))
+    return {"value": evals / t, "unit": "mode evaluations/s (energy + gradient, 20,000 points)",
+            "cores": 1, "kind": "port",
+            "sample": f"{evals} modes at their refined parameters, FP64 numpy",
+            "gpu_fp32_gradient_rel_err_max": worst}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,8 +296,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("cfg2", "batch"), default="cfg2",
-                    help="cfg2: the 2.1M-triangle mesh (default); batch: 512 meshes")
+    ap.add_argument("--workload", choices=("cfg2", "batch", "refine"), default="cfg2",
+                    help="cfg2: the 2.1M-triangle mesh (default); batch: 512 meshes; "
+                         "refine: Gauss-Newton refinement of the top-64 modes of the 96x96 mesh")
     ap.add_argument("--meshes", type=int, default=512)
     ap.add_argument("--check-meshes", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -194,6 +311,9 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "refine":
+        bench_refine(args)
         return
     import torch
     from mitra_algo_amd import capi, workload

@@ -440,6 +440,8 @@ int mvs_dev_closest_points(mvs_session *sp, uint64_t n, const float *pts, float 
     Session *s = as_session(sp);
     if (!s->bvh_faces)
       raise(MVS_EINVAL, "closest points need a BVH (mvs_bvh_build)");
+    if (n && (!pts || !q || !nrm || !dist2))
+      raise(MVS_EINVAL, "null argument");
     if (n)
       MVS_LAUNCH(s, "closest", k_closest, blocks_for(n, kThreads), kThreads, 0,
                  reinterpret_cast<const Node *>(s->bvh_nodes.ptr),

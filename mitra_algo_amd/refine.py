@@ -130,8 +130,9 @@ def align_energy(theta: torch.Tensor, problem: Problem) -> torch.Tensor:
 
 def refine(problem: Problem, steps: int = 10, damping: float = 1e-3):
     """Damped Gauss-Newton (Levenberg-Marquardt) steps; closest points are
-    recomputed at the start of every step.  Rigid twists are folded into the
-    base after each accepted step.  Returns the per-step energies (steps + 1
+    recomputed at the start of every step.  A step is kept when it lowers the
+    energy (with the step's closest points), else the damping grows.  Rigid
+    twists are folded into the base after each accepted step.  Returns the per-step energies (steps + 1
     rows: before the first step and after each, with fresh closest points)."""
     M, P = problem.modes, NPARAM[problem.kind]
     dev = problem.pts.device
@@ -152,13 +153,16 @@ def refine(problem: Problem, steps: int = 10, damping: float = 1e-3):
         ok = (E1 < E0) & torch.isfinite(E1)
         lam = torch.where(ok, lam / 3, lam * 4)
         if problem.kind == RIGID:
-            R = rotation_torch(delta[:, :3])
+            # fold the accepted twist into the base: the twist restarts at zero
+            R = rotation_torch(trial[:, :3])
             R0 = problem.base[:, :9].double().reshape(-1, 3, 3)
             t0 = problem.base[:, 9:].double()
             R0n = (R[:, :, :, None] * R0[:, None, :, :]).sum(2)
-            t0n = (R * t0[:, None, :]).sum(2) + delta[:, 3:6]
+            t0n = (R * t0[:, None, :]).sum(2) + trial[:, 3:6]
             nb = torch.cat([R0n.reshape(-1, 9), t0n], 1).float()
             problem.base = torch.where(ok[:, None], nb, problem.base).contiguous()
+            problem.theta = torch.where(ok[:, None], torch.zeros_like(problem.theta),
+                                        problem.theta).contiguous()
         else:
             m = trial[:, :3] / torch.linalg.norm(trial[:, :3], dim=1, keepdim=True)
             nt = torch.cat([m, trial[:, 3:4]], 1).float()
@@ -167,17 +171,20 @@ def refine(problem: Problem, steps: int = 10, damping: float = 1e-3):
 
 
 def surface_samples(vertices: np.ndarray, faces: np.ndarray, count: int,
-                    seed: int) -> np.ndarray:
+                    seed: int):
     """`count` points uniform on the surface (area-weighted faces, uniform
-    barycentric coordinates), PCG64(seed); (count, 3) FP64."""
+    barycentric coordinates), PCG64(seed): ((count, 3) FP64 points, their
+    unit face normals)."""
     g = np.random.default_rng(seed)
     a, b, c = (vertices[faces[:, k]] for k in range(3))
-    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)
+    cr = np.cross(b - a, c - a)
+    area = 0.5 * np.linalg.norm(cr, axis=1)
     f = g.choice(len(faces), size=count, p=area / area.sum())
     r1, r2 = g.uniform(size=count), g.uniform(size=count)
     s = np.sqrt(r1)
-    return ((1 - s)[:, None] * a[f] + (s * (1 - r2))[:, None] * b[f] +
-            (s * r2)[:, None] * c[f])
+    pts = ((1 - s)[:, None] * a[f] + (s * (1 - r2))[:, None] * b[f] +
+           (s * r2)[:, None] * c[f])
+    return pts, cr[f] / (2 * area[f])[:, None]
 
 
 def merge_modes(points: np.ndarray, weight: np.ndarray, radius: float) -> np.ndarray:
@@ -203,3 +210,134 @@ def axis_angle_to_matrix(r: np.ndarray) -> np.ndarray:
     k = r / t
     K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
     return np.eye(3) + np.sin(t) * K + (1 - np.cos(t)) * K @ K
+
+
+# ---------------------------------------------------------------- modes + patches
+VERIFY_THRESHOLD = 0.025  # the reference's default (include/symmetry/options.h:7)
+
+
+@dataclass
+class Mode:
+    kind: int
+    weight: float
+    coords: np.ndarray   # mean-shift point: rigid (t_c, axis-angle), mirror (d_c, n)
+    support: int = 0     # candidate surface points passing the patch rule
+
+
+def top_modes(wl, count: int = 64, seeds: int = 20000, iterations: int = 30,
+              h: float = 0.05, cutoff: float = 3.0, seed_value: int = 7) -> list:
+    """Distinct FP32 modes of a workload, heaviest first: mean-shift over the
+    votes of the FP32-guarded pair pass, modes within h/2 of a heavier one of
+    the same kind dropped (the reference's merge radius, squared h^2/4,
+    src/cpu/clustering.cpp), then the `count` heaviest over both kinds."""
+    wl.pairvote(capi.FP32_GUARDED)
+    ms = wl.meanshift(seeds, iterations, h, cutoff, seed_value)
+    s = wl.sess
+    out = []
+    for kind, k, modes, weight, dim in ((RIGID, ms.rigid_seeds, ms.rigid_modes,
+                                         ms.rigid_weight, 6),
+                                        (MIRROR, ms.refl_seeds, ms.refl_modes,
+                                         ms.refl_weight, 4)):
+        if k == 0:
+            continue
+        pts = s.fetch(modes, (dim, k), torch.float32, k).double().cpu().numpy()
+        w = s.fetch(weight, (k,), torch.float32).double().cpu().numpy()
+        for i in merge_modes(pts, w, h / 2):
+            out.append(Mode(kind, float(w[i]), pts[:, i].copy()))
+    out.sort(key=lambda m: -m.weight)
+    return out[:count]
+
+
+def mode_transform(m: Mode):
+    """(3x3 linear part, translation) of a mode in centre-relative coordinates."""
+    if m.kind == RIGID:
+        return axis_angle_to_matrix(m.coords[3:6]), m.coords[0:3].copy()
+    u = m.coords[1:4] / np.linalg.norm(m.coords[1:4])
+    return np.eye(3) - 2 * np.outer(u, u), -2 * m.coords[0] * u
+
+
+class Patches:
+    """Candidate surface points of one mesh (centre-relative FP32 on the
+    device, with their face normals) and the session's BVH of that mesh."""
+
+    def __init__(self, sess: capi.Session, vertices: np.ndarray, faces: np.ndarray,
+                 center, box: float, candidates: int = 262144, seed: int = 11,
+                 device: str = "cuda:0"):
+        self.sess = sess
+        self.box = box
+        self.tau = VERIFY_THRESHOLD * box
+        sess.bvh_build(vertices, faces, center)
+        p, n = surface_samples(vertices, faces, candidates, seed)
+        p = p - np.asarray(center, dtype=np.float64)
+        self.pts = torch.as_tensor(np.ascontiguousarray(p.T), dtype=torch.float32).to(device)
+        self.nrm = torch.as_tensor(np.ascontiguousarray(n.T), dtype=torch.float32).to(device)
+        self.count = candidates
+
+    def select(self, modes: list, npts: int, chunk: int = 16) -> list:
+        """Patch point indices per mode: the candidates p whose image T(p)
+        passes the reference's verification rule for the A side
+        (src/cpu/verification.cpp:58-80): |T(p) - p| >= 2 tau, for mirrors
+        u.p + d < -tau, the closest surface point within tau and the mapped
+        normal within 60 degrees of that face's normal (tau = 0.025 box).
+        The first `npts` passing candidates (in candidate order) form the
+        patch, repeated cyclically when fewer pass; None when none does."""
+        dev = self.pts.device
+        out = []
+        for c0 in range(0, len(modes), chunk):
+            ms = modes[c0:c0 + chunk]
+            L = torch.tensor(np.stack([mode_transform(m)[0] for m in ms]),
+                             dtype=torch.float32, device=dev)
+            t = torch.tensor(np.stack([mode_transform(m)[1] for m in ms]),
+                             dtype=torch.float32, device=dev)
+            tp = (L[:, :, :, None] * self.pts[None, None, :, :]).sum(2) + t[:, :, None]
+            tn = (L[:, :, :, None] * self.nrm[None, None, :, :]).sum(2)
+            M, C = len(ms), self.count
+            flat = tp.permute(1, 0, 2).reshape(3, M * C).contiguous()
+            q = torch.empty_like(flat)
+            qn = torch.empty_like(flat)
+            d2 = torch.empty(M * C, dtype=torch.float32, device=dev)
+            self.sess.dev_closest_points(M * C, flat, q, qn, d2)
+            qn = qn.reshape(3, M, C).permute(1, 0, 2)
+            d2 = d2.reshape(M, C)
+            move = torch.linalg.norm(tp - self.pts[None], dim=1)
+            ok = (move >= 2 * self.tau) & (d2 <= self.tau * self.tau) & \
+                ((tn * qn).sum(1) >= 0.5)
+            for j, m in enumerate(ms):
+                if m.kind == MIRROR:
+                    u = m.coords[1:4] / np.linalg.norm(m.coords[1:4])
+                    side = (torch.tensor(u, dtype=torch.float32, device=dev)[:, None]
+                            * self.pts).sum(0) + float(m.coords[0])
+                    ok[j] &= side < -self.tau
+                idx = torch.nonzero(ok[j]).flatten()
+                m.support = int(idx.numel())
+                if m.support == 0:
+                    out.append(None)
+                    continue
+                reps = -(-npts // m.support)
+                out.append(idx.repeat(reps)[:npts])
+        return out
+
+    def problem(self, modes: list, idx: list, kind: int) -> tuple:
+        """The Problem over the modes of `kind` with a patch; returns it with
+        the positions (in `modes`) of the modes it holds."""
+        pick = [i for i, m in enumerate(modes) if m.kind == kind and idx[i] is not None]
+        if not pick:
+            return None, pick
+        dev = self.pts.device
+        pts = torch.stack([self.pts[:, idx[i]] for i in pick]).contiguous()
+        if kind == RIGID:
+            base = torch.tensor(np.stack([np.concatenate([mode_transform(modes[i])[0].ravel(),
+                                                          modes[i].coords[0:3]])
+                                          for i in pick]), dtype=torch.float32, device=dev)
+            theta = torch.zeros(len(pick), 6, dtype=torch.float32, device=dev)
+        else:
+            base = None
+            theta = torch.tensor(np.stack([np.concatenate([modes[i].coords[1:4],
+                                                           modes[i].coords[0:1]])
+                                           for i in pick]), dtype=torch.float32, device=dev)
+        return Problem(self.sess, kind, pts, base, theta.contiguous()), pick
+
+
+def rms_residual(energy: np.ndarray, npts: int) -> np.ndarray:
+    """Root-mean-square point-to-plane residual from per-mode energies."""
+    return np.sqrt(np.maximum(energy, 0) / npts)

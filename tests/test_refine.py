@@ -83,3 +83,15 @@ def test_closest_brute_is_the_minimum():
         q, d2, f = rr.closest_brute(p, V, F)
         assert d2 == pytest.approx(min(((p - rr.closest_on_triangle(p, *V[t])) ** 2).sum()
                                        for t in F))
+
+
+def test_vectorised_closest_points_match_voronoi_form():
+    g = np.random.default_rng(21)
+    V = g.normal(0, 1, (30, 3))
+    F = np.array([g.choice(30, 3, replace=False) for _ in range(40)])
+    P = np.concatenate([g.normal(0, 1.5, (150, 3)), V[F[:20]].mean(1)])
+    q, d2, f = rr.closest_points(P, V, F)
+    for i, p in enumerate(P):
+        bq, bd2, bf = rr.closest_brute(p, V, F)
+        assert abs(d2[i] - bd2) <= 1e-12 * (1 + bd2)
+        assert np.linalg.norm(q[i] - bq) <= 1e-9 or abs(d2[i] - bd2) <= 1e-12

@@ -1,0 +1,141 @@
+"""Host logic of mode refinement on the CPU (no GPU): the patch rule, the
+problem layout, the damped Gauss-Newton loop and the folding of rigid twists,
+driven through a stand-in session whose closest points and energies come from
+the FP64 reference (oracle/refine_ref.py).  The HIP kernels themselves are
+checked in test_gpu_refine.py."""
+import math
+
+import numpy as np
+import torch
+
+from mitra_algo_amd import meshgen, refine
+from oracle import refine_ref as rr
+
+
+class CpuSession:
+    """The three refinement calls of a session, in FP64 on the host."""
+
+    def bvh_build(self, vertices, faces, center):
+        self.V = vertices - np.asarray(center)
+        self.F = np.asarray(faces)
+        a, b, c = (self.V[self.F[:, k]] for k in range(3))
+        n = np.cross(b - a, c - a)
+        self.fn = n / np.linalg.norm(n, axis=1, keepdims=True)
+
+    def dev_closest_points(self, n, pts, q, nrm, dist2, face=None):
+        P = pts.double().numpy().T
+        qq, d2, f = rr.closest_points(P, self.V, self.F)
+        q.copy_(torch.from_numpy(np.ascontiguousarray(qq.T)))
+        nrm.copy_(torch.from_numpy(np.ascontiguousarray(self.fn[f].T)))
+        dist2.copy_(torch.from_numpy(d2))
+        if face is not None:
+            face.copy_(torch.from_numpy(f.astype(np.int32)))
+
+    def dev_align_energy(self, kind, modes, npts, theta, base, pts, q, nrm, energy, grad,
+                         hess=None):
+        for j in range(modes):
+            P, Q, N = (t[j].double().numpy().T for t in (pts, q, nrm))
+            R0 = t0 = None
+            if kind == 0:
+                b = base[j].double().numpy()
+                R0, t0 = b[:9].reshape(3, 3), b[9:]
+            th = theta[j].double().numpy()
+            energy[j] = rr.energy(kind, th, P, Q, N, R0, t0)
+            grad[j] = torch.from_numpy(rr.gradient(kind, th, P, Q, N, R0, t0))
+            if hess is not None:
+                def res(x):
+                    return np.einsum("ij,ij->i", N, rr.transform(kind, x, P, R0, t0) - Q)
+                J = np.stack([(res(th + 1e-6 * e) - res(th - 1e-6 * e)) / 2e-6
+                              for e in np.eye(len(th))], 1)
+                hess[j] = torch.from_numpy(2 * J.T @ J)
+
+
+# An odd grid side keeps the alternating diagonals symmetric under x -> -x
+# and under the half turn, so these meshes carry exact symmetries (up to
+# the 1e-4 noise).
+HALF_TURN = [meshgen.Copy(0.4, 0.4, 0.0, False), meshgen.Copy(-0.4, -0.4, math.pi, False)]
+MIRROR_X = [meshgen.Copy(0.4, 0.0, 0.0, False), meshgen.Copy(-0.4, 0.0, 0.0, True)]
+
+
+def _setup(modes, copies=HALF_TURN, npts=400, candidates=3000):
+    mesh = meshgen.height_field(21, 1.0, copies, meshgen.Motif.make(0), 1e-4, 2)
+    V, F = mesh.vertices, mesh.faces
+    center = 0.5 * (V.min(0) + V.max(0))
+    box = float(np.linalg.norm(V.max(0) - V.min(0)))
+    s = CpuSession()
+    patches = refine.Patches(s, V, F, center, box, candidates=candidates, seed=3, device="cpu")
+    idx = patches.select(modes, npts)
+    return s, patches, idx
+
+
+def test_rigid_refinement_removes_a_tilt():
+    # the half turn about the vertical axis through the centre maps the mesh
+    # onto itself (the copies at +-(0.4, 0.4) turned by 0 and pi); start
+    # tilted by 0.03 rad about x and lifted by 0.01
+    aa = np.array([0.03, 0.0, math.pi])
+    m = refine.Mode(refine.RIGID, 1.0, np.concatenate([[0.0, 0.0, 0.01], aa]))
+    s, patches, idx = _setup([m])
+    assert idx[0] is not None and m.support > 0
+    prob, pick = patches.problem([m], idx, refine.RIGID)
+    assert pick == [0] and prob.pts.shape == (1, 3, 400)
+    hist = refine.refine(prob, steps=10)
+    before, after = refine.rms_residual(hist[[0, -1]], 400)[:, 0]
+    assert after < 0.05 * before, (before, after)
+    R = prob.base[0, :9].double().reshape(3, 3).numpy()
+    # the refined motion keeps the ground plane: its z axis is (0, 0, 1) and
+    # it no longer lifts
+    assert abs(R[2, 2] - 1) < 1e-4
+    assert abs(float(prob.base[0, 11])) < 2e-3
+    assert np.allclose(R @ R.T, np.eye(3), atol=1e-5)
+    assert torch.count_nonzero(prob.theta) == 0
+
+
+def test_mirror_refinement_removes_a_tilt():
+    # the mesh is symmetric about x = 0; start from the plane x = 0.03 with
+    # its normal tilted 0.04 out of the horizontal
+    u = np.array([1.0, 0.0, 0.04])
+    u /= np.linalg.norm(u)
+    m = refine.Mode(refine.MIRROR, 1.0, np.concatenate([[-0.03], u]))
+    s, patches, idx = _setup([m], MIRROR_X)
+    assert idx[0] is not None
+    prob, pick = patches.problem([m], idx, refine.MIRROR)
+    hist = refine.refine(prob, steps=10)
+    before, after = refine.rms_residual(hist[[0, -1]], 400)[:, 0]
+    assert after < 0.05 * before, (before, after)
+    n = prob.theta[0, :3].double().numpy()
+    assert abs(np.linalg.norm(n) - 1) < 1e-6
+    assert abs(n[2]) < 1e-3 and abs(n[1]) < 1e-3
+    assert abs(float(prob.theta[0, 3])) < 1e-3
+
+
+def test_patch_rule_and_fill():
+    ident = refine.Mode(refine.RIGID, 2.0, np.zeros(6))     # moves nothing: no support
+    turn = refine.Mode(refine.RIGID, 1.0, np.array([0, 0, 0, 0, 0, math.pi]))
+    s, patches, idx = _setup([ident, turn], npts=5000)
+    assert idx[0] is None and ident.support == 0
+    assert idx[1] is not None and 0 < turn.support < 3000
+    got = idx[1].numpy()
+    # the first `support` entries are the passing candidates in order, then
+    # they repeat
+    assert np.all(np.diff(got[:turn.support]) > 0)
+    assert np.array_equal(got[turn.support:], got[:5000 - turn.support])
+    prob, pick = patches.problem([ident, turn], idx, refine.RIGID)
+    assert pick == [1]
+    mir, fpick = patches.problem([ident, turn], idx, refine.MIRROR)
+    assert mir is None and fpick == []
+
+
+def test_merge_modes_keeps_heaviest_distinct():
+    pts = np.array([[0.0, 0.01, 1.0, 1.03, np.nan],
+                    [0.0, 0.0, 0.0, 0.0, 0.0]])
+    w = np.array([1.0, 3.0, 2.0, 0.5, 9.0])
+    assert list(refine.merge_modes(pts, w, 0.025)) == [1, 2, 3]
+
+
+def test_rotation_torch_matches_reference():
+    g = np.random.default_rng(2)
+    w = np.concatenate([g.normal(0, 1, (6, 3)), g.normal(0, 1e-8, (2, 3)), np.zeros((1, 3))])
+    R = refine.rotation_torch(torch.from_numpy(w)).numpy()
+    for k in range(len(w)):
+        assert np.allclose(R[k], rr.rotation(w[k]), atol=1e-14)
+        assert np.allclose(refine.axis_angle_to_matrix(w[k]), rr.rotation(w[k]), atol=1e-14)
