@@ -134,8 +134,13 @@ int mvs_sample_vertices(uint64_t nv, const double *weights, double ratio, mvs_mt
 int mvs_session_create(int device, mvs_session **out);
 void mvs_session_destroy(mvs_session *s);
 const char *mvs_last_error(void);
-/* Stream the device layer runs on (default: a stream the session owns).
- * Pass a hipStream_t; 0 selects the session's own stream again. */
+/* Stream the device layer runs on (default: a non-blocking stream the
+ * session owns).  Pass a hipStream_t: 0 is the null stream (ordered against
+ * every blocking stream, torch's default stream among them); MVS_OWN_STREAM
+ * selects the session's own stream again.  Work on the session stream is
+ * ordered only against that stream: a caller whose buffers are produced or
+ * recycled on another stream passes that stream here. */
+#define MVS_OWN_STREAM ((void *)(intptr_t)-1)
 int mvs_session_set_stream(mvs_session *s, void *hip_stream);
 int mvs_session_sync(mvs_session *s);
 /* Stream-ordered device-to-device copy (to read session-owned outputs). */

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmvs.so")
 
 OK, EINVAL, EBUDGET, EOVERFLOW, EHIP, ENODEV, ERUNTIME = 0, 1, 2, 3, 4, 5, 6
 FP64, FP32_GUARDED = 0, 1
+OWN_STREAM = 2**64 - 1  # MVS_OWN_STREAM: (void *)-1
 
 EXPORTS = [
     "mvs_prepare", "mvs_mt64_seed", "mvs_mt64_next", "mvs_sample_vertices",
@@ -259,7 +260,12 @@ class Session:
         self.close()
 
     def set_stream(self, stream_handle: int):
+        """Run on this hipStream_t (0: the null stream, torch's default)."""
         check(lib().mvs_session_set_stream(self.h, C.c_void_p(stream_handle)))
+
+    def use_own_stream(self):
+        """Run on the session's own non-blocking stream again."""
+        check(lib().mvs_session_set_stream(self.h, C.c_void_p(OWN_STREAM)))
 
     def sync(self):
         check(lib().mvs_session_sync(self.h))

@@ -186,7 +186,7 @@ void mvs_session_destroy(mvs_session *s) {
 int mvs_session_set_stream(mvs_session *s, void *stream) {
   return guarded([&] {
     Session *p = as_session(s);
-    p->stream = stream ? static_cast<hipStream_t>(stream) : p->own_stream;
+    p->stream = stream == MVS_OWN_STREAM ? p->own_stream : static_cast<hipStream_t>(stream);
   });
 }
 

@@ -116,19 +116,25 @@ def test_batch_graph_replays_the_captured_step(batch):
     pv, ms = batch.capture(prec, 0.2, 12)
     M = batch.count
     first = _fetch(batch.sess, pv.all, prec)
-    cr = ms.all.rigid_seeds
-    modes = batch.sess.fetch(ms.all.rigid_modes, (6, cr), torch.float32, cr).cpu().clone()
+    # the seeds written: [0, offsets[M]) of the capacity all.rigid_seeds
+    cap = ms.all.rigid_seeds
+    cr = int(batch.sess.fetch(ms.rigid_seed_offsets, (M + 1,), torch.int64).cpu()[M])
+    assert 0 < cr <= cap
+    modes = batch.sess.fetch(ms.all.rigid_modes, (6, cr), torch.float32, cap).cpu().clone()
+    assert torch.isfinite(modes).all()
     poff = batch.sess.fetch(pv.pair_offsets, (M + 1,), torch.int64).cpu().clone()
     torch.cuda.synchronize()
-    for _ in range(3):
+    for it in range(3):
         batch.replay()
         batch.sess.batch_graph_check()
         again = _fetch(batch.sess, pv.all, prec)
         torch.cuda.synchronize()
         for k in first:
-            assert torch.equal(first[k], again[k]), k
-        assert torch.equal(modes, batch.sess.fetch(ms.all.rigid_modes, (6, cr), torch.float32,
-                                                   cr).cpu())
+            assert torch.equal(first[k], again[k]), (it, k)
+        got = batch.sess.fetch(ms.all.rigid_modes, (6, cr), torch.float32, cap).cpu()
+        diff = (got != modes).any(0)
+        assert not diff.any(), (it, int(diff.sum()), cr, torch.nonzero(diff).flatten()[:8].tolist(),
+                                float((got - modes).abs().max()))
         assert torch.equal(poff, batch.sess.fetch(pv.pair_offsets, (M + 1,), torch.int64).cpu())
     # other curvatures on one mesh (kmin sign flipped on every other vertex:
     # the ratio test is invariant under a common scale or sign, not under

@@ -75,6 +75,7 @@ def test_closest_points_match_brute_force():
     V, F = mesh.vertices, mesh.faces
     ctr = V.mean(0)
     s = capi.Session(0)
+    s.set_stream(torch.cuda.current_stream().cuda_stream)
     s.bvh_build(V, F, ctr)
     g = np.random.default_rng(5)
     on = V[F[g.integers(0, len(F), 64)]].mean(1)              # face centroids
@@ -139,31 +140,41 @@ def test_adjoint_matches_fp64_reference_and_differences(cfg1_modes, kind, pertur
     th = theta.double().cpu().numpy()
     # FP32 rounding of T(p) - q leaves each residual off by at most about
     # DELTA (coordinates within ~1.5 of the centre); by Cauchy-Schwarz the
-    # energy is then off by at most 2 sqrt(E N) DELTA + N DELTA^2, and the
-    # gradient by about 2 DELTA sqrt(N) |J_row| ~ 2e-4 for random rounding:
-    # a gradient below GMIN cannot be checked to 1e-3 relative.
-    DELTA, GMIN = 3e-7, 1.0
-    worst, skipped = 0.0, 0
+    # energy is then off by at most 2 sqrt(E N) DELTA + N DELTA^2 and the
+    # gradient by at most 2 DELTA sqrt(N) |J|_F (|J|_F^2 = trace(H) / 2)
+    # plus its own summation error.
+    # Gradients far above that bound are held to 1e-3 relative error against
+    # both the FP64 reference gradient and FP64 central differences; the
+    # rest (stationary to FP32 resolution) to the bound itself.
+    DELTA = 3e-7
+    worst, small = 0.0, []
     for j in range(prob.modes):
         P, Q, N, R0, t0 = _host(prob, j)
         e64 = rr.energy(kind, th[j], P, Q, N, R0, t0)
         g64 = rr.gradient(kind, th[j], P, Q, N, R0, t0)
         fd = _fd(kind, th[j], P, Q, N, R0, t0)
+        Hj = _jtj(kind, th[j], P, Q, N, R0, t0)
+        assert np.linalg.norm(H[j] - Hj) <= 1e-3 * np.linalg.norm(Hj), j
         tol = 1e-4 * e64 + 2 * np.sqrt(e64 * NPTS) * DELTA + NPTS * DELTA ** 2
         assert abs(E[j] - e64) <= tol, (j, E[j], e64)
         gn = np.linalg.norm(g64)
-        if gn < GMIN:
-            skipped += 1
+        jf = np.sqrt(max(np.trace(Hj), 0) / 2)
+        # rounding of the residuals, plus the FP32 sum of 2 r J (about 90
+        # sequential adds per thread and tree levels: 1e-5 of sum |2 r J|)
+        bound = 2 * DELTA * np.sqrt(NPTS) * jf + 1e-5 * 2 * np.sqrt(e64) * jf
+        err = np.linalg.norm(grad[j] - g64)
+        assert err <= 1e-3 * gn + bound, (j, err, gn, bound)
+        if gn < 1e3 * bound:
+            small.append((j, float(gn), float(np.sqrt(e64 / NPTS)), float(bound)))
             continue
-        r1 = np.linalg.norm(grad[j] - g64) / gn
+        r1 = err / gn
         r2 = np.linalg.norm(grad[j] - fd) / np.linalg.norm(fd)
         worst = max(worst, r1, r2)
         assert r1 < 1e-3 and r2 < 1e-3, (j, r1, r2, grad[j], g64, fd)
-        Hj = _jtj(kind, th[j], P, Q, N, R0, t0)
-        assert np.linalg.norm(H[j] - Hj) <= 1e-3 * np.linalg.norm(Hj), j
-    print(f"kind {kind} perturb {perturb}: worst relative gradient error {worst:.2e}, "
-          f"{skipped} of {prob.modes} modes below the checkable gradient")
-    assert skipped <= prob.modes // 10
+    print(f"kind {kind} perturb {perturb}: worst relative gradient error {worst:.2e} over "
+          f"{prob.modes - len(small)} of {prob.modes} modes; near-stationary "
+          f"(mode, |g|, rms residual, bound): {small}")
+    assert len(small) <= prob.modes // 2
 
 
 def test_autograd_backward_is_the_adjoint(cfg1_modes):
