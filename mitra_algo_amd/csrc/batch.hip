@@ -63,6 +63,37 @@ __global__ void k_mesh_offsets(const uint64_t *soff, const uint64_t *pair_off, u
   roff[m] = f;
 }
 
+// Makes off[0..n) non-decreasing (a running maximum) and at most cap, in
+// one block.  Every consumer of per-mesh offsets takes off[m + 1] - off[m]
+// as a count; a graph replay over changed inputs searches streams whose
+// tails still hold the captured step's entries, so offsets found there need
+// not be sorted.  Sorted offsets within capacity pass unchanged.
+constexpr int kMonoThreads = 1024;
+__global__ void __launch_bounds__(kMonoThreads) k_monotone(uint64_t *off, uint64_t n,
+                                                           uint64_t cap) {
+  __shared__ uint64_t sh[kMonoThreads];
+  uint64_t carry = 0;
+  for (uint64_t base = 0; base < n; base += kMonoThreads) {
+    const uint64_t i = base + threadIdx.x;
+    uint64_t v = i < n ? off[i] : 0;
+    v = v < cap ? v : cap;
+    v = v > carry ? v : carry;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < kMonoThreads; o <<= 1) {
+      const uint64_t t = threadIdx.x >= static_cast<unsigned>(o) ? sh[threadIdx.x - o] : 0;
+      __syncthreads();
+      v = v > t ? v : t;
+      sh[threadIdx.x] = v;
+      __syncthreads();
+    }
+    if (i < n)
+      off[i] = v;
+    carry = sh[kMonoThreads - 1];
+    __syncthreads();
+  }
+}
+
 // Seeds per mesh: ns = min(V, ceil(ratio S)) split between the rigid and
 // mirror streams as the one-mesh step splits them (mirror share rounded,
 // then each capped by its stream).  Rigid point segments start at 2 x the
@@ -165,6 +196,9 @@ void batch_pairvote(Session *s, uint64_t n, const double *x, const double *y, co
   MVS_LAUNCH(s, "batch_offsets", k_mesh_offsets, blocks_for(L.meshes + 1, kThreads), kThreads, 0,
              L.segs.ostart, plan_pair_offsets(s), L.meshes, r.pairs, r.refl_pair, r.reflections,
              poff, roff);
+  MVS_LAUNCH(s, "batch_offsets", k_monotone, 1, kMonoThreads, 0, poff, L.meshes + 1, r.pairs);
+  MVS_LAUNCH(s, "batch_offsets", k_monotone, 1, kMonoThreads, 0, roff, L.meshes + 1,
+             r.reflections);
   out->all = r;
   out->meshes = L.meshes;
   out->pair_offsets = poff;
@@ -199,6 +233,9 @@ void batch_meanshift(Session *s, const uint64_t *soff_host, double ratio, uint64
              s->batch_meta_soff, b.pair_offsets, b.refl_offsets, meshes, ratio, kr, kf, rseg);
   exclusive_scan_u64(s, kr, offr, meshes);
   exclusive_scan_u64(s, kf, offf, meshes);
+  // seed slots stay inside the capacities (a replay over changed inputs)
+  MVS_LAUNCH(s, "batch_seeds", k_monotone, 1, kMonoThreads, 0, offr, meshes + 1, cr);
+  MVS_LAUNCH(s, "batch_seeds", k_monotone, 1, kMonoThreads, 0, offf, meshes + 1, cf);
   int32_t *ids = scratch_as<int32_t>(s, kSlotOutMsSeeds, cr + cf ? cr + cf : 1);
   uint32_t *smesh = scratch_as<uint32_t>(s, kSlotBatchMs5, cr + cf ? cr + cf : 1);
   void *rpts = scratch(s, kSlotOutMsRigidPts, 6 * (nr ? nr : 1) * sz);
