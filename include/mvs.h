@@ -349,9 +349,12 @@ int mvs_dev_batch_meanshift(mvs_session *s, double seeds_per_sample, uint64_t se
  * mvs_batch_graph_launch replays it with no host synchronisation, ordered
  * after the session stream's queued work and before its later work.
  * mvs_batch_graph_check synchronises and raises MVS_ERUNTIME when a replay
- * read counts other than the captured ones (its inputs changed; the replay
- * stayed inside the captured buffers).  Any call that grows a session buffer,
- * and any new batch call, drops the graph. */
+ * read counts other than the captured ones.  Replays are for inputs with the
+ * captured counts (the same meshes and curvatures, as when a step is timed or
+ * repeated): capture again after inputs change.  Writes are clamped to the
+ * captured capacities, but a replay over inputs with other counts is not a
+ * supported use.  Any call that grows a session buffer, and any new batch
+ * call, drops the graph. */
 int mvs_batch_graph_capture(mvs_session *s, uint64_t n, const double *x, const double *y,
                             const double *z, const double *feat, uint32_t meshes,
                             const uint64_t *sample_offsets, const int32_t *selected,

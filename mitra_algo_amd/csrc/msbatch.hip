@@ -335,19 +335,25 @@ template <class R> __device__ inline uint64_t cell_key(const Grid &g, R a, R b, 
 }
 
 // Global cell id = the segment's cell base + its local cell id, so the
-// sorted order is segment-major.
+// sorted order is segment-major.  Points past the last segment (a graph
+// replay whose inputs gave fewer votes than the captured step) take the key
+// `beyond`, larger than every cell id: they sort last and no cell holds them.
 template <class R>
 __global__ void k_cell_keys(const R *pts, uint64_t n, const Grid *grids, const uint64_t *seg_off,
-                            uint32_t meshes, const uint64_t *cellbase, uint64_t *keys,
-                            uint32_t *vals) {
+                            uint32_t meshes, const uint64_t *cellbase, uint64_t beyond,
+                            uint64_t *keys, uint32_t *vals) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
   const uint32_t m = seg_of(seg_off, meshes, i);
+  vals[i] = static_cast<uint32_t>(i);
+  if (m >= meshes) {
+    keys[i] = beyond;
+    return;
+  }
   const Grid g = grids[m];
   keys[i] = (cellbase ? cellbase[m] : 0) +
             cell_key(g, pts[g.axis[0] * n + i], pts[g.axis[1] * n + i], pts[g.axis[2] * n + i]);
-  vals[i] = static_cast<uint32_t>(i);
 }
 
 // offsets[c] = first sorted position with key >= c, for c in [0, ncells]
@@ -391,7 +397,10 @@ __global__ void k_gather_sorted(const R *pts, uint64_t n, const Grid *grids,
   if (i >= n)
     return;
   const uint64_t o = order[i];
-  const int *perm = grids[seg_of(seg_off, meshes, o)].perm;
+  const uint32_t m = seg_of(seg_off, meshes, o);
+  if (m >= meshes)
+    return;  // past the last segment: in no cell, never read
+  const int *perm = grids[m].perm;
   typename Vec<R>::V4 v;
   v.x = pts[perm[0] * n + o];
   v.y = pts[perm[1] * n + o];
@@ -789,8 +798,8 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     cellbase = scratch_as<uint64_t>(s, kSlotBatchMs3, meshes + 1);
     exclusive_scan_u64(s, ncell, cellbase, meshes);
     total_cells = cellbase + meshes;
-    key_bits = 1;
-    while (key_bits < 63 && (1ULL << key_bits) < cells_cap)
+    key_bits = 1;  // room for the key cells_cap, past every cell
+    while (key_bits < 63 && (1ULL << key_bits) <= cells_cap)
       ++key_bits;
   }
   uint32_t *sorted_ids = nullptr;
@@ -798,7 +807,7 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     uint64_t *keys = scratch_as<uint64_t>(s, kSlotMs2, 2 * n);
     uint32_t *vals = scratch_as<uint32_t>(s, kSlotMs3, 2 * n);
     MVS_LAUNCH(s, "ms_cell_keys", k_cell_keys<R>, blocks_for(n, kThreads), kThreads, 0, pts, n,
-               grids, seg_off, meshes, cellbase, keys, vals);
+               grids, seg_off, meshes, cellbase, cells_cap, keys, vals);
     radix_sort_pairs_u64(s, keys, vals, keys + n, vals + n, n, key_bits);
     sorted_ids = vals + n;
     MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 2048, kThreads, 0, keys + n, n, total_cells,
