@@ -401,6 +401,11 @@ int mvs_session_set_tuning(mvs_session *s, const mvs_tune_entry *in);
  * mvs_dev_closest_points: for n centre-relative FP32 points (3 x n SoA) the
  * closest surface point q (3 x n), the unit normal of its face (3 x n), the
  * squared distance and the face id (optional); device arrays.
+ * mvs_dev_closest_modes: the same query for the transformed points of many
+ * modes in one launch: point j of mode m is mapped by the mode's transform
+ * (kind, theta and base as for mvs_dev_align_energy) before the query; pts
+ * is modes x 3 x npts, or one 3 x npts set that every mode maps when
+ * shared_pts != 0; q and nrm are modes x 3 x npts, dist2 modes x npts.
  * mvs_dev_align_energy: per mode, the point-to-plane energy
  * E = sum_j (n_j . (T(p_j) - q_j))^2 over its npts points (pts, q, nrm:
  * modes x 3 x npts, centre-relative FP32) and its gradient (modes x P) and
@@ -414,6 +419,9 @@ int mvs_bvh_build(mvs_session *s, uint64_t nv, const double *x, const double *y,
                   const int32_t *f2, const double center[3]);
 int mvs_dev_closest_points(mvs_session *s, uint64_t n, const float *pts, float *q, float *nrm,
                            float *dist2, int32_t *face);
+int mvs_dev_closest_modes(mvs_session *s, int kind, uint32_t modes, uint32_t npts,
+                          const float *theta, const float *base, const float *pts,
+                          int shared_pts, float *q, float *nrm, float *dist2);
 int mvs_dev_align_energy(mvs_session *s, int kind, uint32_t modes, uint32_t npts,
                          const float *theta, const float *base, const float *pts, const float *q,
                          const float *nrm, float *energy, float *grad, float *hess);

@@ -30,7 +30,7 @@ EXPORTS = [
     "mvs_dev_batch_meanshift", "mvs_batch_graph_capture", "mvs_batch_graph_launch",
     "mvs_batch_graph_check", "mvs_batch_autotune", "mvs_session_get_tuning",
     "mvs_session_set_tuning", "mvs_bvh_build", "mvs_dev_closest_points",
-    "mvs_dev_align_energy",
+    "mvs_dev_closest_modes", "mvs_dev_align_energy",
 ]
 
 
@@ -500,6 +500,12 @@ class Session:
     def dev_closest_points(self, n, pts, q, nrm, dist2, face=None):
         check(lib().mvs_dev_closest_points(self.h, C.c_uint64(n), ptr(pts), ptr(q), ptr(nrm),
                                            ptr(dist2), ptr(face)))
+
+    def dev_closest_modes(self, kind, modes, npts, theta, base, pts, shared, q, nrm, dist2):
+        check(lib().mvs_dev_closest_modes(self.h, C.c_int(kind), C.c_uint32(modes),
+                                          C.c_uint32(npts), ptr(theta), ptr(base), ptr(pts),
+                                          C.c_int(1 if shared else 0), ptr(q), ptr(nrm),
+                                          ptr(dist2)))
 
     def dev_align_energy(self, kind, modes, npts, theta, base, pts, q, nrm, energy, grad,
                          hess=None):

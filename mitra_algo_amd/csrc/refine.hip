@@ -103,24 +103,22 @@ __device__ inline P3 closest_on_triangle(P3 p, P3 a, P3 b, P3 c) {
   return add(a, add(mul(ab, vb * den), mul(ac, vc * den)));
 }
 
-// One thread per query: depth-first, nearer child first, subtrees whose box
-// is farther than the best distance skipped; the stack lives in LDS.
-__global__ void __launch_bounds__(kThreads)
-    k_closest(const Node *nodes, const float *tri, const float *fnrm, const int32_t *face_id,
-              uint64_t n, const float *pts, float *q, float *nrm, float *dist2, int32_t *face) {
-  __shared__ int stack[kStack][kThreads];
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n)
-    return;
-  const P3 p = p3(pts[i], pts[n + i], pts[2 * n + i]);
-  float best = INFINITY;
-  P3 bq = p;
-  int bf = -1;
+// Nearest surface point to p: depth-first, nearer child first, subtrees
+// whose box is farther than the best distance skipped; `stack` is this
+// thread's column of an LDS stack (stride kThreads).
+struct Hit {
+  P3 q;
+  float d2;
+  int k;  // leaf-ordered face, -1 when none
+};
+__device__ inline Hit nearest(const Node *nodes, const float *tri, P3 p, int *stack) {
+  Hit h{p, INFINITY, -1};
   int top = 0;
-  stack[top++][threadIdx.x] = 0;
+  stack[0] = 0;
+  ++top;
   while (top > 0) {
-    const Node nd = nodes[stack[--top][threadIdx.x]];
-    if (box_dist2(nd, p.x, p.y, p.z) > best)
+    const Node nd = nodes[stack[--top * kThreads]];
+    if (box_dist2(nd, p.x, p.y, p.z) > h.d2)
       continue;
     if (nd.a < 0) {
       for (int k = -nd.a - 1; k < nd.b; ++k) {
@@ -129,38 +127,45 @@ __global__ void __launch_bounds__(kThreads)
                                    p3(t[6], t[7], t[8]));
         P3 e = sub(p, c);
         float d = dot3(e, e);
-        if (d < best) {
-          best = d;
-          bq = c;
-          bf = k;
+        if (d < h.d2) {
+          h.d2 = d;
+          h.q = c;
+          h.k = k;
         }
       }
     } else {
       const Node l = nodes[nd.a], r = nodes[nd.b];
       const float dl = box_dist2(l, p.x, p.y, p.z), dr = box_dist2(r, p.x, p.y, p.z);
       // push the farther child first so the nearer one is searched first
-      if (dl <= dr) {
-        if (dr <= best && top < kStack)
-          stack[top++][threadIdx.x] = nd.b;
-        if (dl <= best && top < kStack)
-          stack[top++][threadIdx.x] = nd.a;
-      } else {
-        if (dl <= best && top < kStack)
-          stack[top++][threadIdx.x] = nd.a;
-        if (dr <= best && top < kStack)
-          stack[top++][threadIdx.x] = nd.b;
-      }
+      const int near = dl <= dr ? nd.a : nd.b, far = dl <= dr ? nd.b : nd.a;
+      const float dnear = dl <= dr ? dl : dr, dfar = dl <= dr ? dr : dl;
+      if (dfar <= h.d2 && top < kStack)
+        stack[top++ * kThreads] = far;
+      if (dnear <= h.d2 && top < kStack)
+        stack[top++ * kThreads] = near;
     }
   }
-  q[i] = bq.x, q[n + i] = bq.y, q[2 * n + i] = bq.z;
-  if (bf >= 0) {
-    nrm[i] = fnrm[3 * bf], nrm[n + i] = fnrm[3 * bf + 1], nrm[2 * n + i] = fnrm[3 * bf + 2];
+  return h;
+}
+
+// One thread per query point (3 x n SoA).
+__global__ void __launch_bounds__(kThreads)
+    k_closest(const Node *nodes, const float *tri, const float *fnrm, const int32_t *face_id,
+              uint64_t n, const float *pts, float *q, float *nrm, float *dist2, int32_t *face) {
+  __shared__ int stack[kStack * kThreads];
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const Hit h = nearest(nodes, tri, p3(pts[i], pts[n + i], pts[2 * n + i]), stack + threadIdx.x);
+  q[i] = h.q.x, q[n + i] = h.q.y, q[2 * n + i] = h.q.z;
+  if (h.k >= 0) {
+    nrm[i] = fnrm[3 * h.k], nrm[n + i] = fnrm[3 * h.k + 1], nrm[2 * n + i] = fnrm[3 * h.k + 2];
   } else {
     nrm[i] = nrm[n + i] = nrm[2 * n + i] = 0.f;
   }
-  dist2[i] = best;
+  dist2[i] = h.d2;
   if (face)
-    face[i] = bf >= 0 ? face_id[bf] : -1;
+    face[i] = h.k >= 0 ? face_id[h.k] : -1;
 }
 
 // Rodrigues coefficients of R(w) = I + A K + B K^2 (K = [w]x, t = |w|),
@@ -181,6 +186,77 @@ __device__ inline void rodrigues(float t, float &A, float &B, float &dA, float &
     dA = (t * c - s) / (t2 * t);
     dB = (t * s - 2.f * (1.f - c)) / (t2 * t2);
   }
+}
+
+// Transform of mode m applied to p (the energy's T, in FP32).
+template <int KIND> struct ModeTransform {
+  float M[9], t[3];
+  __device__ ModeTransform(const float *theta, const float *base, uint32_t m) {
+    if (KIND == 0) {
+      const float *th = theta + 6 * static_cast<uint64_t>(m);
+      const float *b = base + 12 * static_cast<uint64_t>(m);
+      const P3 w = p3(th[0], th[1], th[2]);
+      float A, B, dA, dB;
+      rodrigues(sqrtf(dot3(w, w)), A, B, dA, dB);
+      // R(w) = I + A K + B K^2, K = [w]x; M = R(w) R0, t = R(w) t0 + v
+      const float K[9] = {0.f, -w.z, w.y, w.z, 0.f, -w.x, -w.y, w.x, 0.f};
+      float Rw[9];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          float kk = 0.f;
+          for (int e = 0; e < 3; ++e)
+            kk += K[3 * r + e] * K[3 * e + c];
+          Rw[3 * r + c] = (r == c ? 1.f : 0.f) + A * K[3 * r + c] + B * kk;
+        }
+      for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c)
+          M[3 * r + c] = Rw[3 * r] * b[c] + Rw[3 * r + 1] * b[3 + c] + Rw[3 * r + 2] * b[6 + c];
+        t[r] = Rw[3 * r] * b[9] + Rw[3 * r + 1] * b[10] + Rw[3 * r + 2] * b[11] + th[3 + r];
+      }
+    } else {
+      const float *th = theta + 4 * static_cast<uint64_t>(m);
+      const P3 mm = p3(th[0], th[1], th[2]);
+      const P3 u = mul(mm, 1.f / sqrtf(dot3(mm, mm)));
+      const float uu[3] = {u.x, u.y, u.z};
+      for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c)
+          M[3 * r + c] = (r == c ? 1.f : 0.f) - 2.f * uu[r] * uu[c];
+        t[r] = -2.f * th[3] * uu[r];
+      }
+    }
+  }
+  __device__ P3 operator()(P3 p) const {
+    return p3(M[0] * p.x + M[1] * p.y + M[2] * p.z + t[0],
+              M[3] * p.x + M[4] * p.y + M[5] * p.z + t[1],
+              M[6] * p.x + M[7] * p.y + M[8] * p.z + t[2]);
+  }
+};
+
+// Closest points of every mode's transformed points in one launch: thread
+// (m, j) maps point j of mode m (pts: modes x 3 x npts, or one shared
+// 3 x npts set when pts_stride == 0) and writes q, nrm (modes x 3 x npts)
+// and dist2 (modes x npts).  Blocks never span two modes.
+template <int KIND>
+__global__ void __launch_bounds__(kThreads)
+    k_closest_modes(const Node *nodes, const float *tri, const float *fnrm, uint32_t modes,
+                    uint32_t npts, unsigned blocks_per_mode, const float *theta,
+                    const float *base, const float *pts, uint64_t pts_stride, float *q,
+                    float *nrm, float *dist2) {
+  __shared__ int stack[kStack * kThreads];
+  const uint32_t m = blockIdx.x / blocks_per_mode;
+  const uint64_t j = static_cast<uint64_t>(blockIdx.x % blocks_per_mode) * kThreads + threadIdx.x;
+  if (m >= modes || j >= npts)
+    return;
+  const ModeTransform<KIND> T(theta, base, m);
+  const float *pm = pts + m * pts_stride;
+  const P3 p = T(p3(pm[j], pm[npts + j], pm[2 * static_cast<uint64_t>(npts) + j]));
+  const Hit h = nearest(nodes, tri, p, stack + threadIdx.x);
+  const uint64_t o = static_cast<uint64_t>(m) * 3 * npts + j;
+  q[o] = h.q.x, q[o + npts] = h.q.y, q[o + 2 * static_cast<uint64_t>(npts)] = h.q.z;
+  const float *f = h.k >= 0 ? fnrm + 3 * h.k : nullptr;
+  nrm[o] = f ? f[0] : 0.f, nrm[o + npts] = f ? f[1] : 0.f;
+  nrm[o + 2 * static_cast<uint64_t>(npts)] = f ? f[2] : 0.f;
+  dist2[static_cast<uint64_t>(m) * npts + j] = h.d2;
 }
 
 template <int NP> __device__ inline void block_sum(float *v, float *sh) {
@@ -448,6 +524,37 @@ int mvs_dev_closest_points(mvs_session *sp, uint64_t n, const float *pts, float 
                  reinterpret_cast<const float *>(s->bvh_tri.ptr),
                  reinterpret_cast<const float *>(s->bvh_fn.ptr),
                  reinterpret_cast<const int32_t *>(s->bvh_fid.ptr), n, pts, q, nrm, dist2, face);
+  });
+}
+
+int mvs_dev_closest_modes(mvs_session *sp, int kind, uint32_t modes, uint32_t npts,
+                          const float *theta, const float *base, const float *pts,
+                          int shared_pts, float *q, float *nrm, float *dist2) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    if (!s->bvh_faces)
+      raise(MVS_EINVAL, "closest points need a BVH (mvs_bvh_build)");
+    if (kind != 0 && kind != 1)
+      raise(MVS_EINVAL, "kind is 0 (rigid) or 1 (mirror)");
+    if (kind == 0 && !base)
+      raise(MVS_EINVAL, "rigid modes need their base transforms");
+    if (!theta || !pts || !q || !nrm || !dist2)
+      raise(MVS_EINVAL, "null argument");
+    if (modes == 0 || npts == 0)
+      return;
+    const unsigned per = blocks_for(npts, kThreads);
+    if (static_cast<uint64_t>(per) * modes > 0x7fffffffULL)
+      raise(MVS_EOVERFLOW, "too many closest-point queries for one launch");
+    const uint64_t stride = shared_pts ? 0 : 3ULL * npts;
+    const Node *nodes = reinterpret_cast<const Node *>(s->bvh_nodes.ptr);
+    const float *tri = reinterpret_cast<const float *>(s->bvh_tri.ptr);
+    const float *fn = reinterpret_cast<const float *>(s->bvh_fn.ptr);
+    if (kind == 0)
+      MVS_LAUNCH(s, "closest", k_closest_modes<0>, per * modes, kThreads, 0, nodes, tri, fn,
+                 modes, npts, per, theta, base, pts, stride, q, nrm, dist2);
+    else
+      MVS_LAUNCH(s, "closest", k_closest_modes<1>, per * modes, kThreads, 0, nodes, tri, fn,
+                 modes, npts, per, theta, base, pts, stride, q, nrm, dist2);
   });
 }
 

@@ -82,18 +82,13 @@ class Problem:
         return p - 2 * s[:, None, :] * u[:, :, None]
 
     def update_closest(self, theta: torch.Tensor) -> torch.Tensor:
-        """Closest mesh points and normals of T(p); returns the squared distances."""
-        tp = self.transformed(theta).contiguous()
-        M, N = self.modes, self.npts
-        # the kernel takes one 3 x n SoA: lay the modes side by side
-        flat = tp.permute(1, 0, 2).reshape(3, M * N).contiguous()
-        q = torch.empty_like(flat)
-        nrm = torch.empty_like(flat)
-        d2 = torch.empty(M * N, dtype=torch.float32, device=flat.device)
-        self.sess.dev_closest_points(M * N, flat, q, nrm, d2)
-        self.q = q.reshape(3, M, N).permute(1, 0, 2).contiguous()
-        self.nrm = nrm.reshape(3, M, N).permute(1, 0, 2).contiguous()
-        return d2.reshape(M, N)
+        """Closest mesh points and normals of T(p) (one launch transforms and
+        queries every mode's points); returns the squared distances."""
+        theta = theta.detach().float().contiguous()
+        d2 = torch.empty(self.modes, self.npts, dtype=torch.float32, device=self.pts.device)
+        self.sess.dev_closest_modes(self.kind, self.modes, self.npts, theta, self.base,
+                                    self.pts, False, self.q, self.nrm, d2)
+        return d2
 
     def evaluate(self, theta: torch.Tensor, want_hessian: bool = True):
         """(E, g, H) of every mode at theta with the current closest points."""
@@ -132,41 +127,48 @@ def refine(problem: Problem, steps: int = 10, damping: float = 1e-3):
     """Damped Gauss-Newton (Levenberg-Marquardt) steps; closest points are
     recomputed at the start of every step.  A step is kept when it lowers the
     energy (with the step's closest points), else the damping grows.  Rigid
-    twists are folded into the base after each accepted step.  Returns the per-step energies (steps + 1
-    rows: before the first step and after each, with fresh closest points)."""
-    M, P = problem.modes, NPARAM[problem.kind]
+    twists are folded into the base after each accepted step.  Energies,
+    gradients and Gauss-Newton matrices come from the device; the 6x6 / 4x4
+    solves and parameter updates run on the host in FP64.  Returns the
+    per-step energies (steps + 1 rows: before the first step and after each,
+    with fresh closest points)."""
+    M = problem.modes
     dev = problem.pts.device
-    lam = torch.full((M,), damping, dtype=torch.float64, device=dev)
+    lam = np.full(M, damping)
+    theta = problem.theta.double().cpu().numpy()
+    base = problem.base.double().cpu().numpy() if problem.base is not None else None
     history = []
     for it in range(steps + 1):
         problem.update_closest(problem.theta)
         E0, g, H = problem.evaluate(problem.theta)
-        history.append(E0.double().cpu().numpy())
+        E0 = E0.double().cpu().numpy()
+        history.append(E0)
         if it == steps:
             break
-        Hd = H.double()
-        diag = torch.diagonal(Hd, dim1=1, dim2=2)
-        A = Hd + torch.diag_embed(lam[:, None] * (diag + 1e-12 * (1 + diag.sum(1, keepdim=True))))
-        delta = -torch.linalg.solve(A, g.double()[:, :, None])[:, :, 0]
-        trial = problem.theta.double() + delta
-        E1, _, _ = problem.evaluate(trial.float(), want_hessian=False)
-        ok = (E1 < E0) & torch.isfinite(E1)
-        lam = torch.where(ok, lam / 3, lam * 4)
+        g = g.double().cpu().numpy()
+        H = H.double().cpu().numpy()
+        diag = np.diagonal(H, axis1=1, axis2=2)
+        A = H + (lam[:, None] * (diag + 1e-12 * (1 + diag.sum(1, keepdims=True))))[:, :, None] \
+            * np.eye(H.shape[1])[None]
+        trial = theta - np.linalg.solve(A, g[:, :, None])[:, :, 0]
+        E1, _, _ = problem.evaluate(torch.from_numpy(trial).float().to(dev),
+                                    want_hessian=False)
+        E1 = E1.double().cpu().numpy()
+        ok = (E1 < E0) & np.isfinite(E1)
+        lam = np.where(ok, lam / 3, lam * 4)
         if problem.kind == RIGID:
             # fold the accepted twist into the base: the twist restarts at zero
-            R = rotation_torch(trial[:, :3])
-            R0 = problem.base[:, :9].double().reshape(-1, 3, 3)
-            t0 = problem.base[:, 9:].double()
-            R0n = (R[:, :, :, None] * R0[:, None, :, :]).sum(2)
-            t0n = (R * t0[:, None, :]).sum(2) + trial[:, 3:6]
-            nb = torch.cat([R0n.reshape(-1, 9), t0n], 1).float()
-            problem.base = torch.where(ok[:, None], nb, problem.base).contiguous()
-            problem.theta = torch.where(ok[:, None], torch.zeros_like(problem.theta),
-                                        problem.theta).contiguous()
+            for j in np.nonzero(ok)[0]:
+                R = axis_angle_to_matrix(trial[j, :3])
+                R0 = base[j, :9].reshape(3, 3)
+                base[j] = np.concatenate([(R @ R0).ravel(), R @ base[j, 9:] + trial[j, 3:6]])
+                theta[j] = 0
+            problem.base = torch.from_numpy(base).float().to(dev).contiguous()
         else:
-            m = trial[:, :3] / torch.linalg.norm(trial[:, :3], dim=1, keepdim=True)
-            nt = torch.cat([m, trial[:, 3:4]], 1).float()
-            problem.theta = torch.where(ok[:, None], nt, problem.theta).contiguous()
+            for j in np.nonzero(ok)[0]:
+                theta[j] = np.concatenate([trial[j, :3] / np.linalg.norm(trial[j, :3]),
+                                           trial[j, 3:4]])
+        problem.theta = torch.from_numpy(theta).float().to(dev).contiguous()
     return np.stack(history)
 
 
@@ -185,6 +187,17 @@ def surface_samples(vertices: np.ndarray, faces: np.ndarray, count: int,
     pts = ((1 - s)[:, None] * a[f] + (s * (1 - r2))[:, None] * b[f] +
            (s * r2)[:, None] * c[f])
     return pts, cr[f] / (2 * area[f])[:, None]
+
+
+def morton_codes(p: np.ndarray, bits: int = 10) -> np.ndarray:
+    """3-D Morton (Z-order) codes of points, `bits` per axis over their box."""
+    lo, hi = p.min(0), p.max(0)
+    q = ((p - lo) / np.where(hi > lo, hi - lo, 1) * ((1 << bits) - 1)).astype(np.uint64)
+    code = np.zeros(len(p), dtype=np.uint64)
+    for b in range(bits):
+        for a in range(3):
+            code |= ((q[:, a] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + a)
+    return code
 
 
 def merge_modes(points: np.ndarray, weight: np.ndarray, radius: float) -> np.ndarray:
@@ -269,6 +282,10 @@ class Patches:
         sess.bvh_build(vertices, faces, center)
         p, n = surface_samples(vertices, faces, candidates, seed)
         p = p - np.asarray(center, dtype=np.float64)
+        # Morton order: neighbouring candidates are neighbours on the surface,
+        # so a wavefront's closest-point queries walk the same BVH nodes
+        order = np.argsort(morton_codes(p), kind="stable")
+        p, n = p[order], n[order]
         self.pts = torch.as_tensor(np.ascontiguousarray(p.T), dtype=torch.float32).to(device)
         self.nrm = torch.as_tensor(np.ascontiguousarray(n.T), dtype=torch.float32).to(device)
         self.count = candidates
@@ -279,26 +296,27 @@ class Patches:
         (src/cpu/verification.cpp:58-80): |T(p) - p| >= 2 tau, for mirrors
         u.p + d < -tau, the closest surface point within tau and the mapped
         normal within 60 degrees of that face's normal (tau = 0.025 box).
-        The first `npts` passing candidates (in candidate order) form the
-        patch, repeated cyclically when fewer pass; None when none does."""
+        The patch takes `npts` of the passing candidates evenly spread over
+        their (Morton) order, or all of them repeated cyclically when fewer
+        pass; None when none does."""
         dev = self.pts.device
         out = []
         for c0 in range(0, len(modes), chunk):
             ms = modes[c0:c0 + chunk]
+            M, C = len(ms), self.count
             L = torch.tensor(np.stack([mode_transform(m)[0] for m in ms]),
                              dtype=torch.float32, device=dev)
             t = torch.tensor(np.stack([mode_transform(m)[1] for m in ms]),
                              dtype=torch.float32, device=dev)
+            # each mode as the kernel's rigid transform (linear part, offset)
+            base = torch.cat([L.reshape(M, 9), t], 1).contiguous()
+            zero = torch.zeros(M, 6, dtype=torch.float32, device=dev)
+            q = torch.empty(M, 3, C, dtype=torch.float32, device=dev)
+            qn = torch.empty_like(q)
+            d2 = torch.empty(M, C, dtype=torch.float32, device=dev)
+            self.sess.dev_closest_modes(RIGID, M, C, zero, base, self.pts, True, q, qn, d2)
             tp = (L[:, :, :, None] * self.pts[None, None, :, :]).sum(2) + t[:, :, None]
             tn = (L[:, :, :, None] * self.nrm[None, None, :, :]).sum(2)
-            M, C = len(ms), self.count
-            flat = tp.permute(1, 0, 2).reshape(3, M * C).contiguous()
-            q = torch.empty_like(flat)
-            qn = torch.empty_like(flat)
-            d2 = torch.empty(M * C, dtype=torch.float32, device=dev)
-            self.sess.dev_closest_points(M * C, flat, q, qn, d2)
-            qn = qn.reshape(3, M, C).permute(1, 0, 2)
-            d2 = d2.reshape(M, C)
             move = torch.linalg.norm(tp - self.pts[None], dim=1)
             ok = (move >= 2 * self.tau) & (d2 <= self.tau * self.tau) & \
                 ((tn * qn).sum(1) >= 0.5)
@@ -312,9 +330,11 @@ class Patches:
                 m.support = int(idx.numel())
                 if m.support == 0:
                     out.append(None)
-                    continue
-                reps = -(-npts // m.support)
-                out.append(idx.repeat(reps)[:npts])
+                elif m.support >= npts:
+                    k = torch.arange(npts, device=dev, dtype=torch.int64)
+                    out.append(idx[(k * m.support) // npts])
+                else:
+                    out.append(idx.repeat(-(-npts // m.support))[:npts])
         return out
 
     def problem(self, modes: list, idx: list, kind: int) -> tuple:

@@ -110,6 +110,54 @@ def test_closest_points_match_brute_force():
         assert np.linalg.norm(nrm[i] - fn) <= 1e-5, i
 
 
+@pytest.mark.parametrize("kind", [0, 1])
+def test_closest_modes_matches_closest_points(cfg1_modes, kind):
+    """The fused transform + query equals the query of the transformed points
+    (transforms formed in FP64 on the host: distances agree to FP32 rounding)."""
+    from mitra_algo_amd import refine
+    wl, modes, patches, rig, mir = cfg1_modes
+    prob = rig if kind == 0 else mir
+    g = torch.Generator().manual_seed(3)
+    theta = prob.theta + (torch.randn(prob.theta.shape, generator=g) * 1e-2).to(
+        prob.theta.device)
+    d2 = prob.update_closest(theta)
+    q1, n1 = prob.q.clone(), prob.nrm.clone()
+    tp = prob.transformed(theta)
+    M, N = prob.modes, prob.npts
+    flat = tp.permute(1, 0, 2).reshape(3, M * N).contiguous()
+    q2, n2 = torch.empty_like(flat), torch.empty_like(flat)
+    e2 = torch.empty(M * N, dtype=torch.float32, device=flat.device)
+    prob.sess.dev_closest_points(M * N, flat, q2, n2, e2)
+    q2 = q2.reshape(3, M, N).permute(1, 0, 2)
+    e2 = e2.reshape(M, N)
+    torch.cuda.synchronize()
+    dd = (d2.sqrt() - e2.sqrt()).abs()
+    assert float(dd.max()) < 2e-5, float(dd.max())
+    # same closest point except where two faces tie within rounding
+    far = (q1 - q2).norm(dim=1) > 1e-4
+    assert float(far.float().mean()) < 1e-3
+    # shared candidates: every mode maps the same 3 x n set
+    C = 4096
+    base = torch.eye(3, device=flat.device).reshape(1, 9).repeat(2, 1)
+    base = torch.cat([base, torch.tensor([[0.0, 0, 0], [0.1, -0.2, 0.05]],
+                                         device=flat.device)], 1).contiguous()
+    zero = torch.zeros(2, 6, device=flat.device)
+    cand = patches.pts[:, :C].contiguous()
+    qs = torch.empty(2, 3, C, device=flat.device)
+    ns_ = torch.empty_like(qs)
+    ds = torch.empty(2, C, device=flat.device)
+    prob.sess.dev_closest_modes(refine.RIGID, 2, C, zero, base, cand, True, qs, ns_, ds)
+    torch.cuda.synchronize()
+    # mode 0 is the identity: the candidates lie on the surface
+    assert float(ds[0].max()) < 1e-10
+    shifted = (cand + base[1, 9:, None]).contiguous()
+    qx, nx = torch.empty_like(shifted), torch.empty_like(shifted)
+    dx = torch.empty(C, device=flat.device)
+    prob.sess.dev_closest_points(C, shifted, qx, nx, dx)
+    torch.cuda.synchronize()
+    assert float((ds[1].sqrt() - dx.sqrt()).abs().max()) < 2e-6
+
+
 def test_modes_and_patches(cfg1_modes):
     wl, modes, patches, rig, mir = cfg1_modes
     assert len(modes) == TOP

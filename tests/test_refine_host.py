@@ -31,6 +31,19 @@ class CpuSession:
         if face is not None:
             face.copy_(torch.from_numpy(f.astype(np.int32)))
 
+    def dev_closest_modes(self, kind, modes, npts, theta, base, pts, shared, q, nrm, dist2):
+        for j in range(modes):
+            P = (pts if shared else pts[j]).double().numpy().T
+            R0 = t0 = None
+            if kind == 0:
+                b = base[j].double().numpy()
+                R0, t0 = b[:9].reshape(3, 3), b[9:]
+            T = rr.transform(kind, theta[j].double().numpy(), P, R0, t0)
+            qq, d2, f = rr.closest_points(T, self.V, self.F)
+            q[j] = torch.from_numpy(np.ascontiguousarray(qq.T))
+            nrm[j] = torch.from_numpy(np.ascontiguousarray(self.fn[f].T))
+            dist2[j] = torch.from_numpy(d2)
+
     def dev_align_energy(self, kind, modes, npts, theta, base, pts, q, nrm, energy, grad,
                          hess=None):
         for j in range(modes):
@@ -115,14 +128,31 @@ def test_patch_rule_and_fill():
     assert idx[0] is None and ident.support == 0
     assert idx[1] is not None and 0 < turn.support < 3000
     got = idx[1].numpy()
-    # the first `support` entries are the passing candidates in order, then
-    # they repeat
+    # every passing candidate in order, then again from the start
     assert np.all(np.diff(got[:turn.support]) > 0)
     assert np.array_equal(got[turn.support:], got[:5000 - turn.support])
+    # more passing candidates than patch points: an even spread over them
+    s2, patches2, idx2 = _setup([turn], npts=1000)
+    got2 = idx2[0].numpy()
+    assert len(got2) == 1000 and np.all(np.diff(got2) > 0)
+    assert got2[0] == got[0] and got2[-1] >= got[turn.support - 1] - 5
     prob, pick = patches.problem([ident, turn], idx, refine.RIGID)
     assert pick == [1]
     mir, fpick = patches.problem([ident, turn], idx, refine.MIRROR)
     assert mir is None and fpick == []
+
+
+def test_morton_codes_order_space():
+    g = np.random.default_rng(4)
+    p = g.uniform(-1, 1, (4000, 3))
+    c = refine.morton_codes(p)
+    assert c.dtype == np.uint64 and len(np.unique(c)) > 3900
+    # corner cells: the low corner first, the high corner last
+    o = np.argsort(c)
+    assert np.all(p[o[:20]] < 0) and np.all(p[o[-20:]] > 0)
+    # consecutive points in Morton order are close on average
+    d = np.linalg.norm(np.diff(p[o], axis=0), axis=1)
+    assert d.mean() < 0.2 * np.linalg.norm(np.diff(p, axis=0), axis=1).mean()
 
 
 def test_merge_modes_keeps_heaviest_distinct():
