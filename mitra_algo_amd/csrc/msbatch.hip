@@ -46,9 +46,14 @@ constexpr int kMaxD = 6;
 __device__ inline float ms_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ inline double ms_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-template <class R> __device__ inline R ms_exp(R x);
-template <> __device__ inline float ms_exp<float>(float x) { return __expf(x); }
-template <> __device__ inline double ms_exp<double>(double x) { return exp(x); }
+// exp(-d^2 / 2h^2) from d^2 and a folded scale c: FP32 as 2^(d^2 c) with
+// c = -log2(e) / 2h^2 (one multiply and v_exp_f32), FP64 as exp(d^2 c) with
+// c = -1 / 2h^2.  An infinite d^2 (the sentinel record) gives 0.
+template <class R> __device__ inline R ms_gauss(R dd, R c);
+template <> __device__ inline float ms_gauss<float>(float dd, float c) {
+  return __builtin_amdgcn_exp2f(dd * c);
+}
+template <> __device__ inline double ms_gauss<double>(double dd, double c) { return exp(dd * c); }
 
 template <class R>
 __global__ void k_points_rigid(const R *q4, const R *t3, uint64_t n, R *pts) {
@@ -396,6 +401,16 @@ __global__ void k_gather_sorted(const R *pts, uint64_t n, const Grid *grids,
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
+  if (i == 0) {  // the sentinel record at [n]: outside every support
+    typename Vec<R>::V4 far;
+    far.x = far.y = far.z = far.w = R(1e30);
+    p4[n] = far;
+    if (D == 6) {
+      typename Vec<R>::V2 farq;
+      farq.x = farq.y = R(1e30);
+      q2[n] = farq;
+    }
+  }
   const uint64_t o = order[i];
   const uint32_t m = seg_of(seg_off, meshes, o);
   if (m >= meshes)
@@ -474,7 +489,7 @@ __global__ void __launch_bounds__(kThreads)
                const uint64_t *offsets, const Grid *grids, const uint64_t *cellbase,
                const uint32_t *seed_mesh, const uint64_t *ns_dev, const uint32_t *seed_pos,
                const R *seeds, const uint32_t *order, uint64_t ns, unsigned logical_blocks,
-               int iterations, R inv2h2, R r2, R floor_w, R *out, R *weight,
+               uint32_t sentinel, int iterations, R gscale, R r2, R floor_w, R *out, R *weight,
                unsigned long long *stats) {
   static_assert(W == 1 || W == kWavesPerBlock, "one seed per wave or per block");
   __shared__ R part[kWavesPerBlock][D + 1];
@@ -551,10 +566,10 @@ __global__ void __launch_bounds__(kThreads)
       acc[k] = 0;
     R wl = 0;
     uint32_t in32 = 0;
-    auto test = [&](uint32_t c, uint32_t end, const typename Vec<R>::V4 &p,
-                    const typename Vec<R>::V2 &q) {
-      if (c >= end)
-        return;
+    // Candidates of one batch are consecutive in a row of cells, so whole
+    // batches tend to fall outside the support together: the branch lets
+    // the wave skip the weight.  The sentinel record is never inside.
+    auto test = [&](const typename Vec<R>::V4 &p, const typename Vec<R>::V2 &q) {
       R d[D];
       d[0] = p.x - m[0];
       d[1] = p.y - m[1];
@@ -568,7 +583,7 @@ __global__ void __launch_bounds__(kThreads)
       for (int k = 1; k < D; ++k)
         dd = ms_fma(d[k], d[k], dd);
       if (dd < r2) {
-        const R w = ms_exp<R>(-dd * inv2h2) - floor_w;
+        const R w = ms_gauss<R>(dd, gscale) - floor_w;
         wl += w;
         for (int k = 0; k < D; ++k)
           acc[k] = ms_fma(w, d[k], acc[k]);
@@ -578,11 +593,13 @@ __global__ void __launch_bounds__(kThreads)
     // Chunks of the flattened range: the map (LDS) sends candidate c to its
     // record index, written row by row (rows split across the seed's waves),
     // then the waves walk the chunk in 64-candidate batches, batch k + 1
-    // loading while batch k is tested (loads are unconditional: lanes past
-    // the end re-read the chunk's last candidate).
+    // loading while batch k is tested.  Loads are unconditional: lanes past
+    // the chunk's end read the map's last slot, the sentinel record, which
+    // lies outside every support.
     constexpr uint32_t kStep = static_cast<uint32_t>(W) * kWave;
-    for (uint32_t cbase = 0; cbase < total; cbase += kMapCap) {
-      const uint32_t cend = total - cbase < kMapCap ? total : cbase + kMapCap;
+    constexpr uint32_t kChunk = kMapCap - 1;  // the last slot holds the sentinel
+    for (uint32_t cbase = 0; cbase < total; cbase += kChunk) {
+      const uint32_t cend = total - cbase < kChunk ? total : cbase + kChunk;
       for (int r = wsub; r < nrows; r += W) {
         const uint32_t rs = __builtin_amdgcn_readlane(startv, r);
         const uint32_t re = __builtin_amdgcn_readlane(startv, r + 1);
@@ -591,15 +608,17 @@ __global__ void __launch_bounds__(kThreads)
         for (uint32_t c = lo + lane; c < hi; c += kWave)
           map[c - cbase] = b + (c - rs);
       }
+      const uint32_t send = cend - cbase;  // the sentinel's slot
+      if (wsub == 0 && lane == 0)
+        map[send] = sentinel;
       if (W > 1) {
         __syncthreads();
       } else {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       }
-      const uint32_t lastc = cend - 1 - cbase;
       auto fetch = [&](uint32_t c, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
-        const uint32_t k = c - cbase < lastc ? c - cbase : lastc;
+        const uint32_t k = c - cbase < send ? c - cbase : send;
         const uint32_t j = map[k];
         p = p4[j];
         if (D == 6)
@@ -611,10 +630,14 @@ __global__ void __launch_bounds__(kThreads)
         typename Vec<R>::V2 qa{}, qb{};
         fetch(c0 + lane, pa, qa);
         for (; c0 < cend; c0 += 2 * kStep) {
+          // the scheduling barriers keep each prefetch ahead of the tests
+          // (a load sunk into the iteration that reads it waits in full)
           fetch(c0 + kStep + lane, pb, qb);
-          test(c0 + lane, cend, pa, qa);
+          __builtin_amdgcn_sched_barrier(0);
+          test(pa, qa);
           fetch(c0 + 2 * kStep + lane, pa, qa);
-          test(c0 + kStep + lane, cend, pb, qb);
+          __builtin_amdgcn_sched_barrier(0);
+          test(pb, qb);
         }
       }
       if (W > 1) {
@@ -752,9 +775,11 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   if (seg)  // seeds past the drawn count leave their slots untouched
     zero_device(s, stats, 2 * ns * sizeof(unsigned long long));
   const uint64_t np = n ? n : 1;
-  char *sorted = static_cast<char *>(scratch(s, kSlotMs0, np * (sizeof(V4) + sizeof(V2))));
+  // cell-sorted records, then the sentinel record at [n]
+  char *sorted =
+      static_cast<char *>(scratch(s, kSlotMs0, (np + 1) * (sizeof(V4) + sizeof(V2))));
   V4 *p4 = reinterpret_cast<V4 *>(sorted);
-  V2 *q2 = reinterpret_cast<V2 *>(sorted + np * sizeof(V4));
+  V2 *q2 = reinterpret_cast<V2 *>(sorted + (np + 1) * sizeof(V4));
   const uint64_t cells_cap = seg ? std::max<uint64_t>(seg->cells_bound, 1) : kMaxCells;
   uint64_t *off = scratch_as<uint64_t>(s, kSlotMs5, cells_cap + 1);
   Grid *grids = seg ? scratch_as<Grid>(s, kSlotBatchMs0, meshes) : &ctl->grid;
@@ -838,7 +863,9 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     radix_sort_pairs_u64(s, skeys, svals, skeys + ns, svals + ns, ns, kCellBits);
     order = svals + ns;
   }
-  const R inv2h2 = static_cast<R>(1.0 / (2.0 * h * h));
+  // the Gaussian's scale, folded as ms_gauss takes it
+  const R gscale = static_cast<R>(sizeof(R) == 4 ? -1.4426950408889634 / (2.0 * h * h)
+                                                 : -1.0 / (2.0 * h * h));
   const R r2 = static_cast<R>(cutoff * h * cutoff * h);
   const R fw = static_cast<R>(std::exp(-0.5 * cutoff * cutoff));
   // few seeds: four waves per seed, so the launch still fills the chip
@@ -849,8 +876,8 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   const uint64_t *ns_dev = seg ? seg->seed_off + meshes : nullptr;
 #define MVS_MS_QUERY(DD, WW)                                                                   \
   MVS_LAUNCH(s, "ms_query", (k_ms_query<R, DD, WW>), grid, kThreads, 0, p4, q2, off, grids,   \
-             cellbase, seed_mesh, ns_dev, seed_pos, seeds, order, ns, logical, iterations,    \
-             inv2h2, r2, fw, out, weight, stats)
+             cellbase, seed_mesh, ns_dev, seed_pos, seeds, order, ns, logical,                \
+             static_cast<uint32_t>(n), iterations, gscale, r2, fw, out, weight, stats)
   if (D == 6 && wide)
     MVS_MS_QUERY(6, kWavesPerBlock);
   else if (D == 6)
