@@ -532,21 +532,65 @@ __global__ void __launch_bounds__(kThreads)
   unsigned long long cand = 0, inside = 0;
   const int reach = g.reach, span = 2 * reach + 1, nrows = span * span;
   const int dz = lane / span - reach, dy = lane % span - reach;
+  // row cutting (below): slab slop, support radius with margins, 1 / cell
+  const double slop =
+      1e-9 * (1.0 + fabs(g.lo[1]) + fabs(g.lo[2]) + g.cell * (g.dims[1] + g.dims[2]));
+  const double rlim = sqrt(static_cast<double>(r2)) * (1.0 + 1e-5) + slop;
+  const double rlim2 = rlim * rlim, inv_cell = 1.0 / g.cell;
   for (int it = 0; it < iterations; ++it) {
     const int cx = cell_of(static_cast<double>(m[0]), g.lo[0], g.cell, g.dims[0]);
     const int cy = cell_of(static_cast<double>(m[1]), g.lo[1], g.cell, g.dims[1]);
     const int cz = cell_of(static_cast<double>(m[2]), g.lo[2], g.cell, g.dims[2]);
     const int x0 = cx > reach ? cx - reach : 0;
     const int x1 = cx + reach < g.dims[0] ? cx + reach : g.dims[0] - 1;
-    // lanes 0..nrows-1 fetch the ranges of the neighbourhood rows
+    // lanes 0..nrows-1 fetch the ranges of the neighbourhood rows, each cut
+    // to the cells its slice of the support ball can reach: a point of row
+    // (y, z) is at least ey, ez from the seed in those two coordinates, so
+    // within the support only if |x - m0| <= sqrt(rlim^2 - ey^2 - ez^2).
+    // Cells are monotone in each coordinate (also where clamped at the grid
+    // edges), rlim and the slab bounds carry margins far above the rounding
+    // of the FP32 / FP64 distance test: no point inside the support is cut.
+    // The cut pays where a seed has thousands of candidates (a block per
+    // seed); with one wave per seed and a few hundred candidates its cost
+    // exceeds what it saves, so those rows stay whole.
     uint32_t rb = 0, len = 0;
-    if (lane < nrows) {
+    if (W == 1 && lane < nrows) {
       const int z = cz + dz, y = cy + dy;
       if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
         const uint64_t row = cb + (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
         const uint64_t b = offsets[row + x0];
         rb = static_cast<uint32_t>(b);
         len = static_cast<uint32_t>(offsets[row + x1 + 1] - b);
+      }
+    } else if (W > 1 && lane < nrows) {
+      const int z = cz + dz, y = cy + dy;
+      if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
+        const double my = static_cast<double>(m[1]), mz = static_cast<double>(m[2]);
+        double ey = y > cy ? g.lo[1] + y * g.cell - my
+                           : (y < cy ? my - (g.lo[1] + (y + 1) * g.cell) : 0.0);
+        double ez = z > cz ? g.lo[2] + z * g.cell - mz
+                           : (z < cz ? mz - (g.lo[2] + (z + 1) * g.cell) : 0.0);
+        ey = ey > slop ? ey - slop : 0.0;
+        ez = ez > slop ? ez - slop : 0.0;
+        const double rem = rlim2 - ey * ey - ez * ez;
+        if (rem > 0) {
+          // half-width (FP32 root, relative margin 1e-6) and its cells, by
+          // the inverse cell side with a margin of 1e-6 cells: xa never
+          // exceeds, xb never falls short of, the cells the divisions of
+          // cell_of give to points inside [mx - hx, mx + hx]
+          const double hx = static_cast<double>(sqrtf(static_cast<float>(rem))) * (1.0 + 1e-6);
+          const double mx = static_cast<double>(m[0]);
+          const double ua = floor((mx - hx - g.lo[0]) * inv_cell - 1e-6);
+          const double ub = floor((mx + hx - g.lo[0]) * inv_cell + 1e-6);
+          const int xa = ua > x0 ? (ua < x1 + 1 ? static_cast<int>(ua) : x1 + 1) : x0;
+          const int xb = ub < x1 ? (ub > x0 - 1 ? static_cast<int>(ub) : x0 - 1) : x1;
+          if (xa <= xb) {
+            const uint64_t row = cb + (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
+            const uint64_t b = offsets[row + xa];
+            rb = static_cast<uint32_t>(b);
+            len = static_cast<uint32_t>(offsets[row + xb + 1] - b);
+          }
+        }
       }
     }
     uint32_t incl = len;  // inclusive prefix over lanes 0..nrows-1

@@ -143,6 +143,34 @@ def test_batched_meanshift_matches_oracle(oracle, D, spread):
     s.close()
 
 
+@pytest.mark.parametrize("spread", [1.0, 8.0])
+@pytest.mark.parametrize("D", [6, 4])
+def test_meanshift_support_visits_match_brute_force(D, spread):
+    """One update from each seed visits exactly the points inside its support
+    (counted by brute force in FP64): the grid walk, with its rows cut to the
+    support ball, drops no point of the support."""
+    P, seeds = _clusters(D, 3 + D, spread)
+    h, cut = 0.05, 3.0
+    s = capi.Session(0)
+    dev = torch.device("cuda:0")
+    s.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    pts = torch.as_tensor(P, dtype=torch.float64, device=dev).contiguous()
+    sd = torch.as_tensor(seeds, dtype=torch.float64, device=dev).contiguous()
+    out = torch.empty_like(sd)
+    w = torch.empty(seeds.shape[1], dtype=torch.float64, device=dev)
+    cand, ins = s.dev_ms_batched(capi.FP64, D, P.shape[1], pts, seeds.shape[1], sd, 1, h, cut,
+                                 out, w, stats=True)
+    torch.cuda.synchronize()
+    s.close()
+    d2 = ((P[:, None, :] - seeds[:, :, None]) ** 2).sum(0)
+    inside = int(np.sum(d2 < (cut * h) ** 2))
+    # points within rounding of the boundary could go either way; there are none
+    near = int(np.sum(np.abs(d2 - (cut * h) ** 2) < 1e-12))
+    assert near == 0
+    assert ins == inside, (ins, inside)
+    assert cand >= ins
+
+
 def test_batched_meanshift_empty_and_degenerate(oracle):
     s = capi.Session(0)
     dev = torch.device("cuda:0")
