@@ -106,6 +106,25 @@ __device__ inline F3 field3f(const float *r, uint64_t S, int l, uint64_t i) {
   const float *p = r + i * kRec + l;
   return f3(p[0], p[1], p[2]);
 }
+// A whole FP32 sample record (one 64-byte line) in registers: four 16-byte
+// loads issued together, so a pair's two records arrive in one round trip.
+struct Rec32 {
+  float4 v[kRec / 4];
+};
+__device__ inline Rec32 load_rec32(const float *s32, uint64_t i) {
+  const float4 *p = reinterpret_cast<const float4 *>(s32 + i * kRec);
+  Rec32 r;
+  for (int k = 0; k < kRec / 4; ++k)
+    r.v[k] = p[k];
+  return r;
+}
+__device__ inline float rget(const Rec32 &r, int l) {
+  const float4 &q = r.v[l >> 2];
+  const int c = l & 3;
+  return c == 0 ? q.x : (c == 1 ? q.y : (c == 2 ? q.z : q.w));
+}
+__device__ inline F3 rf3(const Rec32 &r, int l) { return f3(rget(r, l), rget(r, l + 1), rget(r, l + 2)); }
+
 __device__ inline Geo64 geo_samples(const double *s, uint64_t S, uint64_t i, uint64_t o) {
   return geo64(field3(s, S, L_X, i), field3(s, S, L_X, o), field3(s, S, L_NX, i),
                field3(s, S, L_DX, i), field3(s, S, L_XX, i), field3(s, S, L_NX, o),
@@ -151,18 +170,19 @@ template <bool kGuarded> __device__ inline VoteArgs with_bands(const VoteArgs &i
 // Mirror decision for pair k.  The FP32 result stands unless a test value is
 // inside its guard band or the direction is degenerate; then FP64 decides.
 // (The decision does not depend on the sign of n, so axis ties do not matter.)
+// ra, rb: the FP32 records of samples i and o (read only when kGuarded).
 template <bool kGuarded>
-__device__ inline bool pair_reflection(const VoteArgs &A, uint64_t k, unsigned &rr) {
+__device__ inline bool pair_reflection(const VoteArgs &A, uint64_t i, uint64_t o,
+                                       const Rec32 &ra, const Rec32 &rb, unsigned &rr) {
   const uint64_t S = A.S;
-  uint64_t i = static_cast<uint64_t>(A.a[k]), o = static_cast<uint64_t>(A.b[k]);
   if (kGuarded) {
-    F3 p = field3f(A.s32, S, L_X, i), q = field3f(A.s32, S, L_X, o);
+    F3 p = rf3(ra, L_X), q = rf3(rb, L_X);
     F3 dv = q - p;
     float len = sqrtf(fdot(dv, dv));
     if (len > 1e-6f) {
       F3 n = dv * (1.0f / len);
-      F3 an = field3f(A.s32, S, L_NX, i), ad = field3f(A.s32, S, L_DX, i);
-      F3 bn = field3f(A.s32, S, L_NX, o), bd = field3f(A.s32, S, L_DX, o);
+      F3 an = rf3(ra, L_NX), ad = rf3(ra, L_DX);
+      F3 bn = rf3(rb, L_NX), bd = rf3(rb, L_DX);
       // (I - 2 n n^T) v = v - 2 (n.v) n
       F3 ma = an - n * (2.0f * fdot(n, an)), md = ad - n * (2.0f * fdot(n, ad));
       float r1 = fdot(ma, bn), r2 = fabsf(fdot(md, bd));
@@ -185,10 +205,19 @@ __global__ void __launch_bounds__(kThreads) k_vote_count(VoteArgs Ain, uint32_t 
     return;
   const VoteArgs A = with_bands<kGuarded>(Ain);
   unsigned rr = 0;
-  const bool in = k < A.P && pair_in_range(A.a, A.b, k, A.S);
-  if (k < A.P && !in)
+  const bool inp = k < A.P;
+  const int32_t ia = inp ? A.a[k] : 0, ib = inp ? A.b[k] : 0;
+  const bool in = inp && ia >= 0 && ib >= 0 && static_cast<uint64_t>(ia) < A.S &&
+                  static_cast<uint64_t>(ib) < A.S;
+  if (inp && !in)
     atomicOr(A.bad, 1u);
-  bool refl = in && pair_reflection<kGuarded>(A, k, rr);
+  const uint64_t i = in ? static_cast<uint64_t>(ia) : 0, o = in ? static_cast<uint64_t>(ib) : 0;
+  Rec32 ra{}, rb{};
+  if (kGuarded) {  // sample 0 stands in for dead lanes: every load is in range
+    ra = load_rec32(A.s32, i);
+    rb = load_rec32(A.s32, o);
+  }
+  bool refl = in && pair_reflection<kGuarded>(A, i, o, ra, rb, rr);
   uint64_t m = ballot(refl);
   rr = wave_sum(rr);
   if (lane == 0) {
@@ -222,14 +251,23 @@ __global__ void __launch_bounds__(kThreads)
     return;
   const VoteArgs A = with_bands<kGuarded>(Ain);
   const uint64_t S = A.S, P2 = 2 * A.P;
-  // out-of-range pairs were flagged by the count pass and are skipped here
-  const bool live = k < A.P && pair_in_range(A.a, A.b, k, A.S);
+  // out-of-range pairs were flagged by the count pass and are skipped here;
+  // the pair's ordinals and (FP32) records are read once, up front
+  const bool inp = k < A.P;
+  const int32_t ia = inp ? A.a[k] : 0, ib = inp ? A.b[k] : 0;
+  const bool live = inp && ia >= 0 && ib >= 0 && static_cast<uint64_t>(ia) < S &&
+                    static_cast<uint64_t>(ib) < S;
+  const uint64_t i = live ? static_cast<uint64_t>(ia) : 0, o = live ? static_cast<uint64_t>(ib) : 0;
+  Rec32 ra{}, rb{};
+  if (kGuarded) {  // sample 0 stands in for dead lanes: every load is in range
+    ra = load_rec32(A.s32, i);
+    rb = load_rec32(A.s32, o);
+  }
   unsigned rr = 0;
-  bool refl = live && pair_reflection<kGuarded>(A, k, rr);
+  bool refl = live && pair_reflection<kGuarded>(A, i, o, ra, rb, rr);
   const uint64_t slot = wave_off[w] + lanes_below(ballot(refl));
   if (!live)
     return;
-  const uint64_t i = static_cast<uint64_t>(A.a[k]), o = static_cast<uint64_t>(A.b[k]);
   R rq[2][4], rt[2][3], pn[3] = {R(0), R(0), R(0)}, pd = R(0);
   if (!kGuarded) {
     Geo64 g = geo_samples(A.s64, S, i, o);
@@ -251,11 +289,9 @@ __global__ void __launch_bounds__(kThreads)
     }
   } else {
     // FP32 frames in centre-relative coordinates: t_c = q' - R p'
-    F3 p = field3f(A.s32, S, L_X, i), q = field3f(A.s32, S, L_X, o);
-    F3 an = field3f(A.s32, S, L_NX, i), ad = field3f(A.s32, S, L_DX, i),
-       ax = field3f(A.s32, S, L_XX, i);
-    F3 bn = field3f(A.s32, S, L_NX, o), bd = field3f(A.s32, S, L_DX, o),
-       bx = field3f(A.s32, S, L_XX, o);
+    F3 p = rf3(ra, L_X), q = rf3(rb, L_X);
+    F3 an = rf3(ra, L_NX), ad = rf3(ra, L_DX), ax = rf3(ra, L_XX);
+    F3 bn = rf3(rb, L_NX), bd = rf3(rb, L_DX), bx = rf3(rb, L_XX);
     for (int sg = 0; sg < 2; ++sg) {
       float s = sg == 0 ? 1.0f : -1.0f;
       F3 b0 = bd * s, b1 = bx * s;
