@@ -23,6 +23,23 @@ namespace mvs {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr unsigned kXcds = 8;  // MI355X: 8 dies, blocks dealt to them round-robin
+
+// Wave w of a pair pass, with blocks remapped so that each die takes a
+// contiguous run of 64-pair chunks: consecutive pairs share their sources,
+// whose records then stay in that die's L2.  Launch grids are rounded up to
+// a multiple of kXcds (waves past the pairs return at once).
+__device__ inline uint64_t pair_wave(uint64_t P) {
+  const uint64_t nw = (P + kWave - 1) / kWave;
+  const unsigned nb = static_cast<unsigned>((nw + kThreads / kWave - 1) / (kThreads / kWave));
+  const unsigned per = (nb + kXcds - 1) / kXcds;
+  const unsigned logical = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  return static_cast<uint64_t>(logical) * (kThreads / kWave) + threadIdx.x / kWave;
+}
+inline unsigned pair_grid(uint64_t nw) {
+  const unsigned nb = blocks_for(nw, kThreads / kWave);
+  return ((nb + kXcds - 1) / kXcds) * kXcds;
+}
 
 // Pair geometry of transformations.cpp:34-47 in FP64.
 struct Geo64 {
@@ -199,7 +216,7 @@ __device__ inline bool pair_reflection(const VoteArgs &A, uint64_t i, uint64_t o
 template <bool kGuarded>
 __global__ void __launch_bounds__(kThreads) k_vote_count(VoteArgs Ain, uint32_t *wave_refl) {
   const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t w = static_cast<uint64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  const uint64_t w = pair_wave(Ain.P);
   const uint64_t k = w * kWave + lane;
   if (w * kWave >= Ain.P)
     return;
@@ -245,7 +262,7 @@ __global__ void __launch_bounds__(kThreads)
     k_vote_emit(VoteArgs Ain, const uint64_t *wave_off, VoteOut<R> out) {
   using V2 = typename Vec2T<R>::T;
   const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t w = static_cast<uint64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  const uint64_t w = pair_wave(Ain.P);
   const uint64_t k = w * kWave + lane;
   if (w * kWave >= Ain.P)
     return;
@@ -472,7 +489,7 @@ void vote_offsets_device(Session *s, uint64_t P, const int32_t *a, const int32_t
   uint64_t *off = scratch_as<uint64_t>(s, kSlotVoteBlockOff, nw + 1);
   zero_device(s, A.bad, sizeof(unsigned));
   if (P) {
-    unsigned grid = blocks_for(nw, kThreads / kWave);
+    unsigned grid = pair_grid(nw);
     if (guarded)
       MVS_LAUNCH(s, "vote_count", k_vote_count<true>, grid, kThreads, 0, A, wr);
     else
@@ -491,7 +508,7 @@ void vote_emit_strided(Session *s, uint64_t P, const int32_t *a, const int32_t *
   VoteArgs A = vote_args(s, P, a, b, center, guarded, segs);
   const uint64_t nw = (P + kWave - 1) / kWave;
   uint64_t *off = scratch_as<uint64_t>(s, kSlotVoteBlockOff, nw + 1);
-  unsigned grid = blocks_for(nw, kThreads / kWave);
+  unsigned grid = pair_grid(nw);
   if (guarded) {
     VoteOut<float> o{static_cast<float *>(q4), static_cast<float *>(t3), rp,
                      static_cast<float *>(n3), static_cast<float *>(d1), fp, stride};
