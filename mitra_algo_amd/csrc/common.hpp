@@ -106,7 +106,7 @@ enum Slot : int {
   kSlotPlanWin, kSlotPlanCounts, kSlotPlanOffsets, kSlotPlanLong, kSlotPlanStats,
   kSlotPlanSlab, kSlotPlanKw, kSlotPlanLongCnt, kSlotPlanLongOff,
   kSlotVoteBlock, kSlotVoteBlockOff,
-  kSlotMs0, kSlotMs1, kSlotMs2, kSlotMs3, kSlotMs4, kSlotMs5, kSlotMs6, kSlotMs7,
+  kSlotMs0, kSlotMs1, kSlotMs2, kSlotMs3, kSlotMs4, kSlotMs5, kSlotMs6, kSlotMs7, kSlotMs8,
   // fused step outputs (valid until the next fused call)
   kSlotOutPairs, kSlotOutRigidQ, kSlotOutRigidT, kSlotOutRigidPair, kSlotOutReflN,
   kSlotOutReflD, kSlotOutReflPair, kSlotOutMsRigidPts, kSlotOutMsReflPts, kSlotOutMsSeeds,

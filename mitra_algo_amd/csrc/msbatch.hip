@@ -339,17 +339,42 @@ template <class R> __device__ inline uint64_t cell_key(const Grid &g, R a, R b, 
   return (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0] + x;
 }
 
+// Storage of the cell-sorted points: slot coordinates 0-3 (three grid axes
+// and the first remaining one) as one 4-vector, slots 4-5 (6-D only) as a
+// 2-vector, so the three-axis pretest is one coalesced load per candidate.
+template <class R> struct Vec;
+template <> struct Vec<float> {
+  using V4 = float4;
+  using V2 = float2;
+};
+template <> struct Vec<double> {
+  using V4 = double4;
+  using V2 = double2;
+};
+
 // Global cell id = the segment's cell base + its local cell id, so the
 // sorted order is segment-major.  Points past the last segment (a graph
 // replay whose inputs gave fewer votes than the captured step) take the key
 // `beyond`, larger than every cell id: they sort last and no cell holds them.
-template <class R>
+// The pass also copies each point into `aos` (kAos scalars per point,
+// coordinates in input order), so the gather after the sort reads one
+// 32- or 64-byte record per point instead of D scattered scalars.
+constexpr int kAos = 8;
+template <class R, int D>
 __global__ void k_cell_keys(const R *pts, uint64_t n, const Grid *grids, const uint64_t *seg_off,
                             uint32_t meshes, const uint64_t *cellbase, uint64_t beyond,
-                            uint64_t *keys, uint32_t *vals) {
+                            uint64_t *keys, uint32_t *vals, R *aos) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
+  R v[kAos];
+  for (int k = 0; k < kAos; ++k)
+    v[k] = k < D ? pts[k * n + i] : R(0);
+  for (int k = 0; k < kAos; k += 2) {  // 8- or 16-byte stores, record-contiguous
+    typename Vec<R>::V2 u;
+    u.x = v[k], u.y = v[k + 1];
+    *reinterpret_cast<typename Vec<R>::V2 *>(aos + i * kAos + k) = u;
+  }
   const uint32_t m = seg_of(seg_off, meshes, i);
   vals[i] = static_cast<uint32_t>(i);
   if (m >= meshes) {
@@ -357,8 +382,13 @@ __global__ void k_cell_keys(const R *pts, uint64_t n, const Grid *grids, const u
     return;
   }
   const Grid g = grids[m];
-  keys[i] = (cellbase ? cellbase[m] : 0) +
-            cell_key(g, pts[g.axis[0] * n + i], pts[g.axis[1] * n + i], pts[g.axis[2] * n + i]);
+  R a = v[0], b = v[0], c = v[0];
+  for (int k = 1; k < D; ++k) {  // the grid axes' coordinates, without dynamic indexing
+    a = g.axis[0] == k ? v[k] : a;
+    b = g.axis[1] == k ? v[k] : b;
+    c = g.axis[2] == k ? v[k] : c;
+  }
+  keys[i] = (cellbase ? cellbase[m] : 0) + cell_key(g, a, b, c);
 }
 
 // offsets[c] = first sorted position with key >= c, for c in [0, ncells]
@@ -381,21 +411,8 @@ __global__ void k_cell_offsets(const uint64_t *skeys, uint64_t n, const uint64_t
   }
 }
 
-// Storage of the cell-sorted points: slot coordinates 0-3 (three grid axes
-// and the first remaining one) as one 4-vector, slots 4-5 (6-D only) as a
-// 2-vector, so the three-axis pretest is one coalesced load per candidate.
-template <class R> struct Vec;
-template <> struct Vec<float> {
-  using V4 = float4;
-  using V2 = float2;
-};
-template <> struct Vec<double> {
-  using V4 = double4;
-  using V2 = double2;
-};
-
 template <class R, int D>
-__global__ void k_gather_sorted(const R *pts, uint64_t n, const Grid *grids,
+__global__ void k_gather_sorted(const R *aos, uint64_t n, const Grid *grids,
                                 const uint64_t *seg_off, uint32_t meshes, const uint32_t *order,
                                 typename Vec<R>::V4 *p4, typename Vec<R>::V2 *q2) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -416,16 +433,26 @@ __global__ void k_gather_sorted(const R *pts, uint64_t n, const Grid *grids,
   if (m >= meshes)
     return;  // past the last segment: in no cell, never read
   const int *perm = grids[m].perm;
-  typename Vec<R>::V4 v;
-  v.x = pts[perm[0] * n + o];
-  v.y = pts[perm[1] * n + o];
-  v.z = pts[perm[2] * n + o];
-  v.w = pts[perm[3] * n + o];
-  p4[i] = v;
+  R v[kAos];
+  for (int k = 0; k < kAos; k += 2) {
+    const typename Vec<R>::V2 u = *reinterpret_cast<const typename Vec<R>::V2 *>(aos + o * kAos + k);
+    v[k] = u.x, v[k + 1] = u.y;
+  }
+  // slot j holds coordinate perm[j] (selects, no dynamic register indexing)
+  R w[D];
+  for (int j = 0; j < D; ++j) {
+    const int pj = perm[j];
+    R x = v[0];
+    for (int k = 1; k < D; ++k)
+      x = pj == k ? v[k] : x;
+    w[j] = x;
+  }
+  typename Vec<R>::V4 a;
+  a.x = w[0], a.y = w[1], a.z = w[2], a.w = w[3];
+  p4[i] = a;
   if (D == 6) {
     typename Vec<R>::V2 u;
-    u.x = pts[perm[4] * n + o];
-    u.y = pts[perm[5] * n + o];
+    u.x = w[4 % D], u.y = w[5 % D];
     q2[i] = u;
   }
 }
@@ -875,18 +902,23 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   if (n) {
     uint64_t *keys = scratch_as<uint64_t>(s, kSlotMs2, 2 * n);
     uint32_t *vals = scratch_as<uint32_t>(s, kSlotMs3, 2 * n);
-    MVS_LAUNCH(s, "ms_cell_keys", k_cell_keys<R>, blocks_for(n, kThreads), kThreads, 0, pts, n,
-               grids, seg_off, meshes, cellbase, cells_cap, keys, vals);
+    R *aos = scratch_as<R>(s, kSlotMs8, kAos * n);
+    if (D == 6)
+      MVS_LAUNCH(s, "ms_cell_keys", (k_cell_keys<R, 6>), blocks_for(n, kThreads), kThreads, 0,
+                 pts, n, grids, seg_off, meshes, cellbase, cells_cap, keys, vals, aos);
+    else
+      MVS_LAUNCH(s, "ms_cell_keys", (k_cell_keys<R, 4>), blocks_for(n, kThreads), kThreads, 0,
+                 pts, n, grids, seg_off, meshes, cellbase, cells_cap, keys, vals, aos);
     radix_sort_pairs_u64(s, keys, vals, keys + n, vals + n, n, key_bits);
     sorted_ids = vals + n;
     MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 2048, kThreads, 0, keys + n, n, total_cells,
                off);
     if (D == 6)
       MVS_LAUNCH(s, "ms_gather", (k_gather_sorted<R, 6>), blocks_for(n, kThreads), kThreads, 0,
-                 pts, n, grids, seg_off, meshes, sorted_ids, p4, q2);
+                 aos, n, grids, seg_off, meshes, sorted_ids, p4, q2);
     else
       MVS_LAUNCH(s, "ms_gather", (k_gather_sorted<R, 4>), blocks_for(n, kThreads), kThreads, 0,
-                 pts, n, grids, seg_off, meshes, sorted_ids, p4, q2);
+                 aos, n, grids, seg_off, meshes, sorted_ids, p4, q2);
   } else {
     MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 1, kThreads, 0, nullptr, 0, total_cells,
                off);
