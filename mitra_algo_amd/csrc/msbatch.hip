@@ -196,7 +196,7 @@ struct Grid {
   int reach;
   uint64_t ncells;
 };
-constexpr int kMaxReach = 2;
+constexpr int kMaxReach = 3;  // 7 x 7 = 49 rows: one row range per lane
 constexpr uint32_t kMapPerWave = 1024;  // LDS candidate-map entries per wave
 constexpr int kMaxRows = (2 * kMaxReach + 1) * (2 * kMaxReach + 1);
 constexpr int kCellBits = 22;  // at most 2^22 cells: three 8-bit radix passes
@@ -220,8 +220,8 @@ struct MsCtl {
 // No points: a single cell.  ncells[m] (optional) receives the cell count.
 template <int D>
 __global__ void k_grid_setup(const double *partial, unsigned nb, const uint64_t *seg_off,
-                             uint64_t n, double cell0, Grid *grids, uint64_t *ncells,
-                             unsigned *bad) {
+                             uint64_t n, double cell0, int max_reach, Grid *grids,
+                             uint64_t *ncells, unsigned *bad) {
   const int lane = threadIdx.x;
   const uint32_t mesh = blockIdx.x;
   partial += static_cast<uint64_t>(mesh) * nb * kPart;
@@ -292,8 +292,9 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, const uint64_t 
         if (perm[k] == j)
           alo[k] = lo[j], ext[k] = hi[j] - lo[j];
     }
-    // half-width cells (reach 2) when that grid fits, else cells of side
-    // cutoff*h (reach 1), doubled until the grid fits
+    // cells of side cutoff*h / reach for the largest reach up to max_reach
+    // whose grid fits, else cells of side cutoff*h (reach 1), doubled until
+    // the grid fits
     auto fits = [&](double cell) {
       uint64_t cells = 1;
       for (int k = 0; k < 3; ++k) {
@@ -304,14 +305,15 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, const uint64_t 
       g.ncells = cells;
       return cells <= cap;
     };
-    double cell = cell0 / kMaxReach;
-    g.reach = kMaxReach;
-    if (!fits(cell)) {
-      g.reach = 1;
-      cell = cell0;
-      while (!fits(cell))
-        cell *= 2;
+    int reach = max_reach;
+    double cell = cell0 / reach;
+    while (reach > 1 && !fits(cell)) {
+      --reach;
+      cell = cell0 / reach;
     }
+    g.reach = reach;
+    while (!fits(cell))
+      cell *= 2;
     for (int k = 0; k < 3; ++k) {
       g.axis[k] = perm[k];
       g.lo[k] = alo[k];
@@ -495,6 +497,55 @@ __global__ void k_seed_keys(const R *seeds, uint64_t ns, const Grid *grids, uint
   vals[s] = static_cast<uint32_t>(s);
 }
 
+// Cross-lane steps through DPP: register moves between the lanes of a
+// 16-lane row, with no LDS round trip (the query runs them every iteration).
+template <int kCtrl, int kRowMask = 0xF>
+__device__ inline uint32_t dpp_u32(uint32_t old, uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+      static_cast<int>(old), static_cast<int>(v), kCtrl, kRowMask, 0xF, true));
+}
+// Inclusive prefix sum over the 64 lanes: Hillis-Steele inside each row
+// (row_shr 1, 2, 4, 8), then row 0 into row 1 and row 2 into row 3
+// (row_bcast:15), then rows 0-1 into rows 2-3 (row_bcast:31).
+__device__ inline uint32_t wave_incl_scan(uint32_t x) {
+  x += dpp_u32<0x111>(0u, x);
+  x += dpp_u32<0x112>(0u, x);
+  x += dpp_u32<0x114>(0u, x);
+  x += dpp_u32<0x118>(0u, x);
+  x += dpp_u32<0x142, 0xA>(0u, x);
+  x += dpp_u32<0x143, 0xC>(0u, x);
+  return x;
+}
+template <int kCtrl> __device__ inline float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kCtrl, 0xF, 0xF, true));
+}
+template <int kCtrl> __device__ inline double dpp_f(double v) {
+  const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+  const uint32_t lo = static_cast<uint32_t>(
+      __builtin_amdgcn_mov_dpp(static_cast<int>(static_cast<uint32_t>(b)), kCtrl, 0xF, 0xF, true));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(
+      static_cast<int>(static_cast<uint32_t>(b >> 32)), kCtrl, 0xF, 0xF, true));
+  return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+__device__ inline float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ inline double lane_value(double v, int l) {
+  const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(b), l);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(b >> 32), l);
+  return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+// Sum over the 64 lanes, the same (wave-uniform) value in every lane and a
+// fixed order: quads (quad_perm), rows (row_ror 4, 8), then the four rows.
+template <class R> __device__ inline R wave_total(R v) {
+  v += dpp_f<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_f<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_f<0x124>(v);  // row_ror:4
+  v += dpp_f<0x128>(v);  // row_ror:8
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
 // Mean-shift queries.  A seed is served by W wavefronts (W = 1: four seeds
 // per 256-lane block; W = 4: one seed per block, for few seeds with many
 // candidates).  Seeds are sorted points (seed_pos, already in cell order) or
@@ -620,15 +671,10 @@ __global__ void __launch_bounds__(kThreads)
         }
       }
     }
-    uint32_t incl = len;  // inclusive prefix over lanes 0..nrows-1
-    for (int o = 1; o < 32; o <<= 1) {
-      const uint32_t y = __shfl_up(incl, o, kWave);
-      if (lane >= o)
-        incl += y;
-    }
+    const uint32_t incl = wave_incl_scan(len);  // lanes >= nrows add 0
     // row r covers flattened [start_r, start_r + len_r), start_r = incl_{r-1};
     // lane r holds start_r (lane nrows: the total) and the row's first record
-    const uint32_t startv = __shfl_up(incl, 1, kWave) * (lane > 0 ? 1u : 0u);
+    const uint32_t startv = incl - len;
     const uint32_t total = __builtin_amdgcn_readlane(incl, nrows - 1);
     if (wsub == 0)
       cand += total;
@@ -688,9 +734,12 @@ __global__ void __launch_bounds__(kThreads)
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       }
-      auto fetch = [&](uint32_t c, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
-        const uint32_t k = c - cbase < send ? c - cbase : send;
-        const uint32_t j = map[k];
+      // three stages per batch: map slot (LDS), record (global), test; the
+      // map read runs two batches ahead and the record load one batch ahead
+      auto slot = [&](uint32_t c) -> uint32_t {
+        return map[c - cbase < send ? c - cbase : send];
+      };
+      auto load = [&](uint32_t j, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
         p = p4[j];
         if (D == 6)
           q = q2[j];
@@ -699,14 +748,17 @@ __global__ void __launch_bounds__(kThreads)
       if (c0 < cend) {
         typename Vec<R>::V4 pa, pb;
         typename Vec<R>::V2 qa{}, qb{};
-        fetch(c0 + lane, pa, qa);
+        uint32_t ja = slot(c0 + lane), jb = slot(c0 + kStep + lane);
+        load(ja, pa, qa);
         for (; c0 < cend; c0 += 2 * kStep) {
           // the scheduling barriers keep each prefetch ahead of the tests
           // (a load sunk into the iteration that reads it waits in full)
-          fetch(c0 + kStep + lane, pb, qb);
+          load(jb, pb, qb);
+          ja = slot(c0 + 2 * kStep + lane);
           __builtin_amdgcn_sched_barrier(0);
           test(pa, qa);
-          fetch(c0 + 2 * kStep + lane, pa, qa);
+          load(ja, pa, qa);
+          jb = slot(c0 + 3 * kStep + lane);
           __builtin_amdgcn_sched_barrier(0);
           test(pb, qb);
         }
@@ -719,9 +771,9 @@ __global__ void __launch_bounds__(kThreads)
       }
     }
     inside += in32;
-    wl = wave_sum(wl);
+    wl = wave_total(wl);
     for (int k = 0; k < D; ++k)
-      acc[k] = wave_sum(acc[k]);
+      acc[k] = wave_total(acc[k]);
     if (W > 1) {
       if (lane == 0) {
         for (int k = 0; k < D; ++k)
@@ -883,12 +935,18 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
                  partial);
   }
   uint64_t *ncell = seg ? scratch_as<uint64_t>(s, kSlotBatchMs2, meshes + 1) : nullptr;
+  // few seeds: four waves per seed, so the launch still fills the chip.  A
+  // seed with a block of its own walks finer cells (reach 3, rows cut to the
+  // support ball); with one wave per seed the per-row cost of finer cells
+  // outweighs the candidates they save (reach 2).
+  const bool wide = ns < s->tune.wide_seeds;
+  const int max_reach = wide ? kMaxReach : 2;
   if (D == 6)
     MVS_LAUNCH(s, "ms_grid", k_grid_setup<6>, meshes, kWave, 0, partial, nb, seg_off, n,
-               cutoff * h, grids, ncell, &ctl->bad);
+               cutoff * h, max_reach, grids, ncell, &ctl->bad);
   else
     MVS_LAUNCH(s, "ms_grid", k_grid_setup<4>, meshes, kWave, 0, partial, nb, seg_off, n,
-               cutoff * h, grids, ncell, &ctl->bad);
+               cutoff * h, max_reach, grids, ncell, &ctl->bad);
   int key_bits = kCellBits;
   if (seg) {  // cells of mesh m follow those of meshes < m
     cellbase = scratch_as<uint64_t>(s, kSlotBatchMs3, meshes + 1);
@@ -944,8 +1002,6 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
                                                  : -1.0 / (2.0 * h * h));
   const R r2 = static_cast<R>(cutoff * h * cutoff * h);
   const R fw = static_cast<R>(std::exp(-0.5 * cutoff * cutoff));
-  // few seeds: four waves per seed, so the launch still fills the chip
-  const bool wide = ns < s->tune.wide_seeds;
   const unsigned logical = wide ? static_cast<unsigned>(ns) : blocks_for(ns, kWavesPerBlock);
   const unsigned grid = ((logical + kXcds - 1) / kXcds) * kXcds;
   const uint32_t *seed_mesh = seg ? seg->seed_mesh : nullptr;
