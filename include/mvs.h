@@ -406,6 +406,11 @@ int mvs_session_set_tuning(mvs_session *s, const mvs_tune_entry *in);
  * (kind, theta and base as for mvs_dev_align_energy) before the query; pts
  * is modes x 3 x npts, or one 3 x npts set that every mode maps when
  * shared_pts != 0; q and nrm are modes x 3 x npts, dist2 modes x npts.
+ * face (optional, modes x npts, in and out): a face id per point that
+ * starts the search (its distance bounds the tree walk; -1 for none), then
+ * the face of the closest point.  Passing the previous step's faces makes
+ * repeated queries of slowly moving points cheap; the result is the same
+ * nearest point (between faces at exactly equal distance, the given one).
  * mvs_dev_align_energy: per mode, the point-to-plane energy
  * E = sum_j (n_j . (T(p_j) - q_j))^2 over its npts points (pts, q, nrm:
  * modes x 3 x npts, centre-relative FP32) and its gradient (modes x P) and
@@ -421,7 +426,7 @@ int mvs_dev_closest_points(mvs_session *s, uint64_t n, const float *pts, float *
                            float *dist2, int32_t *face);
 int mvs_dev_closest_modes(mvs_session *s, int kind, uint32_t modes, uint32_t npts,
                           const float *theta, const float *base, const float *pts,
-                          int shared_pts, float *q, float *nrm, float *dist2);
+                          int shared_pts, float *q, float *nrm, float *dist2, int32_t *face);
 int mvs_dev_align_energy(mvs_session *s, int kind, uint32_t modes, uint32_t npts,
                          const float *theta, const float *base, const float *pts, const float *q,
                          const float *nrm, float *energy, float *grad, float *hess);

@@ -501,11 +501,12 @@ class Session:
         check(lib().mvs_dev_closest_points(self.h, C.c_uint64(n), ptr(pts), ptr(q), ptr(nrm),
                                            ptr(dist2), ptr(face)))
 
-    def dev_closest_modes(self, kind, modes, npts, theta, base, pts, shared, q, nrm, dist2):
+    def dev_closest_modes(self, kind, modes, npts, theta, base, pts, shared, q, nrm, dist2,
+                          face=None):
         check(lib().mvs_dev_closest_modes(self.h, C.c_int(kind), C.c_uint32(modes),
                                           C.c_uint32(npts), ptr(theta), ptr(base), ptr(pts),
                                           C.c_int(1 if shared else 0), ptr(q), ptr(nrm),
-                                          ptr(dist2)))
+                                          ptr(dist2), ptr(face)))
 
     def dev_align_energy(self, kind, modes, npts, theta, base, pts, q, nrm, energy, grad,
                          hess=None):

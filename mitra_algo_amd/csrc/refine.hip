@@ -111,8 +111,8 @@ struct Hit {
   float d2;
   int k;  // leaf-ordered face, -1 when none
 };
-__device__ inline Hit nearest(const Node *nodes, const float *tri, P3 p, int *stack) {
-  Hit h{p, INFINITY, -1};
+// h: the best hit known before the search (d2 = INFINITY: none).
+__device__ inline Hit nearest(const Node *nodes, const float *tri, P3 p, int *stack, Hit h) {
   int top = 0;
   stack[0] = 0;
   ++top;
@@ -156,7 +156,8 @@ __global__ void __launch_bounds__(kThreads)
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
-  const Hit h = nearest(nodes, tri, p3(pts[i], pts[n + i], pts[2 * n + i]), stack + threadIdx.x);
+  const P3 p = p3(pts[i], pts[n + i], pts[2 * n + i]);
+  const Hit h = nearest(nodes, tri, p, stack + threadIdx.x, Hit{p, INFINITY, -1});
   q[i] = h.q.x, q[n + i] = h.q.y, q[2 * n + i] = h.q.z;
   if (h.k >= 0) {
     nrm[i] = fnrm[3 * h.k], nrm[n + i] = fnrm[3 * h.k + 1], nrm[2 * n + i] = fnrm[3 * h.k + 2];
@@ -241,7 +242,8 @@ __global__ void __launch_bounds__(kThreads)
     k_closest_modes(const Node *nodes, const float *tri, const float *fnrm, uint32_t modes,
                     uint32_t npts, unsigned blocks_per_mode, const float *theta,
                     const float *base, const float *pts, uint64_t pts_stride, float *q,
-                    float *nrm, float *dist2) {
+                    float *nrm, float *dist2, int32_t *face, const int32_t *face_id,
+                    const int32_t *leaf_of, uint32_t nfaces) {
   __shared__ int stack[kStack * kThreads];
   const uint32_t m = blockIdx.x / blocks_per_mode;
   const uint64_t j = static_cast<uint64_t>(blockIdx.x % blocks_per_mode) * kThreads + threadIdx.x;
@@ -250,13 +252,30 @@ __global__ void __launch_bounds__(kThreads)
   const ModeTransform<KIND> T(theta, base, m);
   const float *pm = pts + m * pts_stride;
   const P3 p = T(p3(pm[j], pm[npts + j], pm[2 * static_cast<uint64_t>(npts) + j]));
-  const Hit h = nearest(nodes, tri, p, stack + threadIdx.x);
+  const uint64_t fo = static_cast<uint64_t>(m) * npts + j;
+  // warm start: the face given for this point (the previous step's) bounds
+  // the search from the start; the tree then finds anything nearer
+  Hit init{p, INFINITY, -1};
+  if (face) {
+    const int32_t f = face[fo];
+    if (f >= 0 && static_cast<uint32_t>(f) < nfaces) {
+      const int k = leaf_of[f];
+      const float *t = tri + 9 * static_cast<uint64_t>(k);
+      const P3 c = closest_on_triangle(p, p3(t[0], t[1], t[2]), p3(t[3], t[4], t[5]),
+                                       p3(t[6], t[7], t[8]));
+      const P3 e = sub(p, c);
+      init = Hit{c, dot3(e, e), k};
+    }
+  }
+  const Hit h = nearest(nodes, tri, p, stack + threadIdx.x, init);
+  if (face)
+    face[fo] = h.k >= 0 ? face_id[h.k] : -1;
   const uint64_t o = static_cast<uint64_t>(m) * 3 * npts + j;
   q[o] = h.q.x, q[o + npts] = h.q.y, q[o + 2 * static_cast<uint64_t>(npts)] = h.q.z;
   const float *f = h.k >= 0 ? fnrm + 3 * h.k : nullptr;
   nrm[o] = f ? f[0] : 0.f, nrm[o + npts] = f ? f[1] : 0.f;
   nrm[o + 2 * static_cast<uint64_t>(npts)] = f ? f[2] : 0.f;
-  dist2[static_cast<uint64_t>(m) * npts + j] = h.d2;
+  dist2[fo] = h.d2;
 }
 
 template <int NP> __device__ inline void block_sum(float *v, float *sh) {
@@ -463,10 +482,11 @@ static void bvh_build(Session *s, uint64_t nv, const double *x, const double *y,
       dn[i].a = -(b.begin + 1), dn[i].b = b.end;
   }
   std::vector<float> tri(9 * nf), fn(3 * nf);
-  std::vector<int32_t> fid(nf);
+  std::vector<int32_t> fid(nf), leaf_of(nf);
   for (uint64_t k = 0; k < nf; ++k) {
     const int f = ids[k];
     fid[k] = f;
+    leaf_of[f] = static_cast<int32_t>(k);
     const int32_t v[3] = {f0[f], f1[f], f2[f]};
     for (int c = 0; c < 3; ++c)
       tri[9 * k + 3 * c] = vx[v[c]], tri[9 * k + 3 * c + 1] = vy[v[c]],
@@ -485,6 +505,9 @@ static void bvh_build(Session *s, uint64_t nv, const double *x, const double *y,
   s->bvh_tri.alloc(tri.size() * sizeof(float));
   s->bvh_fn.alloc(fn.size() * sizeof(float));
   s->bvh_fid.alloc(fid.size() * sizeof(int32_t));
+  s->bvh_leaf_of.alloc(leaf_of.size() * sizeof(int32_t));
+  MVS_HIP(hipMemcpy(s->bvh_leaf_of.ptr, leaf_of.data(), leaf_of.size() * sizeof(int32_t),
+                    hipMemcpyHostToDevice));
   MVS_HIP(hipMemcpy(s->bvh_nodes.ptr, dn.data(), dn.size() * sizeof(Node), hipMemcpyHostToDevice));
   MVS_HIP(hipMemcpy(s->bvh_tri.ptr, tri.data(), tri.size() * sizeof(float),
                     hipMemcpyHostToDevice));
@@ -529,7 +552,7 @@ int mvs_dev_closest_points(mvs_session *sp, uint64_t n, const float *pts, float 
 
 int mvs_dev_closest_modes(mvs_session *sp, int kind, uint32_t modes, uint32_t npts,
                           const float *theta, const float *base, const float *pts,
-                          int shared_pts, float *q, float *nrm, float *dist2) {
+                          int shared_pts, float *q, float *nrm, float *dist2, int32_t *face) {
   return guarded([&] {
     Session *s = as_session(sp);
     if (!s->bvh_faces)
@@ -549,12 +572,17 @@ int mvs_dev_closest_modes(mvs_session *sp, int kind, uint32_t modes, uint32_t np
     const Node *nodes = reinterpret_cast<const Node *>(s->bvh_nodes.ptr);
     const float *tri = reinterpret_cast<const float *>(s->bvh_tri.ptr);
     const float *fn = reinterpret_cast<const float *>(s->bvh_fn.ptr);
+    const int32_t *fid = reinterpret_cast<const int32_t *>(s->bvh_fid.ptr);
+    const int32_t *leaf_of = reinterpret_cast<const int32_t *>(s->bvh_leaf_of.ptr);
+    const uint32_t nf = static_cast<uint32_t>(s->bvh_faces);
     if (kind == 0)
       MVS_LAUNCH(s, "closest", k_closest_modes<0>, per * modes, kThreads, 0, nodes, tri, fn,
-                 modes, npts, per, theta, base, pts, stride, q, nrm, dist2);
+                 modes, npts, per, theta, base, pts, stride, q, nrm, dist2, face, fid, leaf_of,
+                 nf);
     else
       MVS_LAUNCH(s, "closest", k_closest_modes<1>, per * modes, kThreads, 0, nodes, tri, fn,
-                 modes, npts, per, theta, base, pts, stride, q, nrm, dist2);
+                 modes, npts, per, theta, base, pts, stride, q, nrm, dist2, face, fid, leaf_of,
+                 nf);
   });
 }
 

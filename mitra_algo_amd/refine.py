@@ -60,6 +60,10 @@ class Problem:
     def __post_init__(self):
         self.q = torch.empty_like(self.pts)
         self.nrm = torch.empty_like(self.pts)
+        # each point's face from the previous query (-1: none yet): the next
+        # query starts from it
+        self.face = torch.full((self.pts.shape[0], self.pts.shape[2]), -1, dtype=torch.int32,
+                               device=self.pts.device)
 
     @property
     def modes(self) -> int:
@@ -87,7 +91,7 @@ class Problem:
         theta = theta.detach().float().contiguous()
         d2 = torch.empty(self.modes, self.npts, dtype=torch.float32, device=self.pts.device)
         self.sess.dev_closest_modes(self.kind, self.modes, self.npts, theta, self.base,
-                                    self.pts, False, self.q, self.nrm, d2)
+                                    self.pts, False, self.q, self.nrm, d2, self.face)
         return d2
 
     def evaluate(self, theta: torch.Tensor, want_hessian: bool = True):

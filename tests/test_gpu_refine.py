@@ -133,6 +133,17 @@ def test_closest_modes_matches_closest_points(cfg1_modes, kind):
     torch.cuda.synchronize()
     dd = (d2.sqrt() - e2.sqrt()).abs()
     assert float(dd.max()) < 2e-5, float(dd.max())
+    # warm start from faces: from the right faces, from wrong ones (shifted
+    # by one point), from none: the same nearest distances
+    for hint in (prob.face.clone(), torch.roll(prob.face, 1, dims=1),
+                 torch.full_like(prob.face, -1)):
+        d3 = torch.empty_like(d2)
+        q3, n3 = torch.empty_like(prob.q), torch.empty_like(prob.nrm)
+        prob.sess.dev_closest_modes(prob.kind, M, N, theta.float().contiguous(), prob.base,
+                                    prob.pts, False, q3, n3, d3, hint)
+        torch.cuda.synchronize()
+        assert torch.equal(d3, d2)
+        assert int((hint < 0).sum()) == 0
     # same closest point except where two faces tie within rounding
     far = (q1 - q2).norm(dim=1) > 1e-4
     assert float(far.float().mean()) < 1e-3

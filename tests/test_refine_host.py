@@ -31,7 +31,8 @@ class CpuSession:
         if face is not None:
             face.copy_(torch.from_numpy(f.astype(np.int32)))
 
-    def dev_closest_modes(self, kind, modes, npts, theta, base, pts, shared, q, nrm, dist2):
+    def dev_closest_modes(self, kind, modes, npts, theta, base, pts, shared, q, nrm, dist2,
+                          face=None):
         for j in range(modes):
             P = (pts if shared else pts[j]).double().numpy().T
             R0 = t0 = None
@@ -43,6 +44,8 @@ class CpuSession:
             q[j] = torch.from_numpy(np.ascontiguousarray(qq.T))
             nrm[j] = torch.from_numpy(np.ascontiguousarray(self.fn[f].T))
             dist2[j] = torch.from_numpy(d2)
+            if face is not None:
+                face[j] = torch.from_numpy(f.astype(np.int32))
 
     def dev_align_energy(self, kind, modes, npts, theta, base, pts, q, nrm, energy, grad,
                          hess=None):
