@@ -497,54 +497,6 @@ __global__ void k_seed_keys(const R *seeds, uint64_t ns, const Grid *grids, uint
   vals[s] = static_cast<uint32_t>(s);
 }
 
-// Cross-lane steps through DPP: register moves between the lanes of a
-// 16-lane row, with no LDS round trip (the query runs them every iteration).
-template <int kCtrl, int kRowMask = 0xF>
-__device__ inline uint32_t dpp_u32(uint32_t old, uint32_t v) {
-  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
-      static_cast<int>(old), static_cast<int>(v), kCtrl, kRowMask, 0xF, true));
-}
-// Inclusive prefix sum over the 64 lanes: Hillis-Steele inside each row
-// (row_shr 1, 2, 4, 8), then row 0 into row 1 and row 2 into row 3
-// (row_bcast:15), then rows 0-1 into rows 2-3 (row_bcast:31).
-__device__ inline uint32_t wave_incl_scan(uint32_t x) {
-  x += dpp_u32<0x111>(0u, x);
-  x += dpp_u32<0x112>(0u, x);
-  x += dpp_u32<0x114>(0u, x);
-  x += dpp_u32<0x118>(0u, x);
-  x += dpp_u32<0x142, 0xA>(0u, x);
-  x += dpp_u32<0x143, 0xC>(0u, x);
-  return x;
-}
-template <int kCtrl> __device__ inline float dpp_f(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kCtrl, 0xF, 0xF, true));
-}
-template <int kCtrl> __device__ inline double dpp_f(double v) {
-  const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
-  const uint32_t lo = static_cast<uint32_t>(
-      __builtin_amdgcn_mov_dpp(static_cast<int>(static_cast<uint32_t>(b)), kCtrl, 0xF, 0xF, true));
-  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(
-      static_cast<int>(static_cast<uint32_t>(b >> 32)), kCtrl, 0xF, 0xF, true));
-  return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
-}
-__device__ inline float lane_value(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-__device__ inline double lane_value(double v, int l) {
-  const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
-  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(b), l);
-  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(b >> 32), l);
-  return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
-}
-// Sum over the 64 lanes, the same (wave-uniform) value in every lane and a
-// fixed order: quads (quad_perm), rows (row_ror 4, 8), then the four rows.
-template <class R> __device__ inline R wave_total(R v) {
-  v += dpp_f<0xB1>(v);   // quad_perm [1, 0, 3, 2]
-  v += dpp_f<0x4E>(v);   // quad_perm [2, 3, 0, 1]
-  v += dpp_f<0x124>(v);  // row_ror:4
-  v += dpp_f<0x128>(v);  // row_ror:8
-  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
-}
 
 // Mean-shift queries.  A seed is served by W wavefronts (W = 1: four seeds
 // per 256-lane block; W = 4: one seed per block, for few seeds with many

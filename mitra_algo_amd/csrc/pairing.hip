@@ -283,7 +283,8 @@ template <bool kGuarded> __device__ inline PairArgs with_bands(const PairArgs &i
 }
 
 struct Source {
-  double a0, a1, ax, ay, az;
+  uint64_t i;     // the source's ordinal
+  double a0, a1, ax, ay, az;  // FP64 fields (the FP64 walk only)
   float f0, f1, fx, fy, fz;
   double dist;    // 0.03 x the mesh's box size
   float dist32;
@@ -327,12 +328,15 @@ __device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, 
       if (d != 2)
         return d == 1;
     }
+    // FP64 recheck: the source's FP64 fields are read here, not kept live
+    // through the walk (rechecks are rare; registers are not)
     ++rechecks;
     double2 bb = A.bk[j];
     const double2 *p = reinterpret_cast<const double2 *>(A.r64 + static_cast<uint64_t>(o) * kRec);
     double2 p0 = p[0], p1 = p[1];
-    return accept64(src.a0, src.a1, bb.x, bb.y, src.ax, src.ay, src.az, p0.x, p0.y, p1.x, A.t,
-                    src.dist);
+    const double2 *ps = reinterpret_cast<const double2 *>(A.r64 + src.i * kRec);
+    const double2 s0 = ps[0], s1 = ps[1], s2 = ps[2];
+    return accept64(s1.y, s2.x, bb.x, bb.y, s0.x, s0.y, s1.x, p0.x, p0.y, p1.x, A.t, src.dist);
   }
 }
 
@@ -353,11 +357,13 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
     return;
   const double2 kw = A.kw[i];
   Source src;
+  src.i = i;
   {
-    const double2 *p = reinterpret_cast<const double2 *>(A.r64 + i * kRec);
-    double2 p0 = p[0], p1 = p[1], p2 = p[2];
-    src.ax = p0.x, src.ay = p0.y, src.az = p1.x, src.a0 = p1.y, src.a1 = p2.x;
-    if (kGuarded) {
+    if (!kGuarded) {
+      const double2 *p = reinterpret_cast<const double2 *>(A.r64 + i * kRec);
+      double2 p0 = p[0], p1 = p[1], p2 = p[2];
+      src.ax = p0.x, src.ay = p0.y, src.az = p1.x, src.a0 = p1.y, src.a1 = p2.x;
+    } else {
       const float4 *q = reinterpret_cast<const float4 *>(A.r32 + i * kRec);
       float4 q0 = q[0], q1 = q[1];
       src.fx = q0.x, src.fy = q0.y, src.fz = q0.z, src.f0 = q0.w, src.f1 = q1.x;
@@ -373,10 +379,11 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
       const uint64_t base = b * kBucket;
       const uint64_t nb = S - base < static_cast<uint64_t>(kBucket) ? S - base : kBucket;
       // lower and upper bound of the kmax interval, searched together
-      uint64_t fl = 0, cl = nb, fu = 0, cu = nb;
+      // (32-bit counters: a bucket holds at most kBucket entries)
+      uint32_t fl = 0, cl = static_cast<uint32_t>(nb), fu = 0, cu = static_cast<uint32_t>(nb);
       while (cl | cu) {
         if (cl) {
-          uint64_t h = cl / 2, m = fl + h;
+          uint32_t h = cl / 2, m = fl + h;
           if (A.bk[base + m].y < kw.x) {
             fl = m + 1;
             cl -= h + 1;
@@ -384,7 +391,7 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
             cl = h;
         }
         if (cu) {
-          uint64_t h = cu / 2, m = fu + h;
+          uint32_t h = cu / 2, m = fu + h;
           if (!(A.bk[base + m].y > kw.y)) {
             fu = m + 1;
             cu -= h + 1;
@@ -396,13 +403,8 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
       len = fu > fl ? static_cast<uint32_t>(fu - fl) : 0u;
     }
     // exclusive prefix of the sub-range lengths across the wave
-    uint32_t x = len;
-    for (int o = 1; o < kWave; o <<= 1) {
-      uint32_t y = __shfl_up(x, o, kWave);
-      if (lane >= o)
-        x += y;
-    }
-    const uint32_t total = __shfl(x, kWave - 1, kWave);
+    const uint32_t x = wave_incl_scan(len);
+    const uint32_t total = __builtin_amdgcn_readlane(x, kWave - 1);
     W.start[lane] = u;
     W.pre[lane] = x - len;
     if (lane == 0)
