@@ -303,12 +303,24 @@ __device__ inline void source_mesh(const PairArgs &A, uint64_t i, double &dist, 
   }
 }
 
-// Decide candidate at bucketed position j (ordinal o).
+// Curvatures of bucketed position j: FP32 for the guarded walk, FP64 for
+// the plain one (loaded beside the ordinal, not after it).
+template <bool kGuarded> struct BucketK;
+template <> struct BucketK<true> {
+  using T = float2;
+  __device__ static T load(const PairArgs &A, uint64_t j) { return A.bkf[j]; }
+};
+template <> struct BucketK<false> {
+  using T = double2;
+  __device__ static T load(const PairArgs &A, uint64_t j) { return A.bk[j]; }
+};
+
+// Decide candidate at bucketed position j (ordinal o, curvatures bkj).
 template <bool kGuarded>
 __device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, uint32_t o,
-                              unsigned &rechecks) {
-  if (!kGuarded) {
-    double2 b = A.bk[j];
+                              const typename BucketK<kGuarded>::T &bkj, unsigned &rechecks) {
+  if constexpr (!kGuarded) {
+    const double2 b = bkj;
     double m = fmax(fmax(fabs(src.a0), fabs(src.a1)), fmax(fmax(fabs(b.x), fabs(b.y)), 1e-10));
     double tol = A.t * m;
     if (fabs(src.a0 - b.x) > tol || fabs(src.a1 - b.y) > tol)
@@ -318,7 +330,7 @@ __device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, 
     return accept64(src.a0, src.a1, b.x, b.y, src.ax, src.ay, src.az, p0.x, p0.y, p1.x, A.t,
                     src.dist);
   } else {
-    float2 b = A.bkf[j];
+    const float2 b = bkj;
     int r = screen32(src.f0, src.f1, b.x, b.y, A.t32, A.gk);
     if (r == 0)
       return false;
@@ -424,8 +436,9 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
             l += step;
         const uint64_t j = W.start[l] + (c - W.pre[l]);
         o = A.bord[j];
+        const typename BucketK<kGuarded>::T bkj = BucketK<kGuarded>::load(A, j);
         if (o != kNone && o > i && o < src.ohi)
-          ok = decide<kGuarded>(A, src, j, o, rechecks);
+          ok = decide<kGuarded>(A, src, j, o, bkj, rechecks);
       }
       f(ok, o);
     }
