@@ -28,7 +28,8 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 SEEDS, ITERS, H, CUT = 20000, 30, 0.05, 3.0
-KERNELS = ("pair_gather", "radix_hist", "radix_scatter", "scan_reduce", "scan_apply", "scan_zero",
+KERNELS = ("batch_seg", "batch_offsets", "batch_seeds",
+           "pair_gather", "radix_hist", "radix_scatter", "scan_reduce", "scan_apply", "scan_zero",
            "pair_index", "pair_windows", "pair_walk", "pair_place", "pair_long", "vote_count",
            "vote_emit", "read_total", "zero", "ms_seeds", "ms_points", "ms_bbox", "ms_grid",
            "ms_cell_keys", "ms_cell_offsets", "ms_gather", "ms_seed_keys", "ms_query", "ms_stats")
@@ -126,6 +127,16 @@ def bench_batch(args):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t1) / args.steps
     wl.sess.batch_graph_check()
+    # untimed breakdown: one eager step with an event pair around every launch
+    wl.sess.enable_timing(True)
+    wl.step(prec, ratio, iters)
+    torch.cuda.synchronize()
+    breakdown = {}
+    for k in KERNELS:
+        ms_, n = wl.sess.timing(k)
+        if n:
+            breakdown[k] = {"ms": ms_, "launches": n}
+    wl.sess.enable_timing(False)
     # per-mesh pairs against one-mesh runs (a spread of meshes)
     M = wl.count
     poff = wl.sess.fetch(pv.pair_offsets, (M + 1,), torch.int64).cpu().numpy()
@@ -168,6 +179,7 @@ def bench_batch(args):
         "eager_step_ms": eager_ms,
         "graph_speedup_over_eager": eager_ms / (dt * 1e3),
         "autotune": {"table": table, "best": best, "seconds": tune_s},
+        "breakdown_eager_step": breakdown,
         "per_mesh_pairs_bit_exact": {"meshes_checked": len(picks), "equal": equal},
     }
     print(json.dumps(out), flush=True)
