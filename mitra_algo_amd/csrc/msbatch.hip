@@ -143,17 +143,22 @@ __device__ inline uint32_t seg_of(const uint64_t *off, uint32_t meshes, uint64_t
   return f - 1;
 }
 
-// Box and moments of one segment per block (layout of k_bbox).
+// Box and moments of the segments (layout of k_bbox): kSegBlocks blocks per
+// segment, block p of segment m taking every kSegBlocks-th run of kThreads
+// points, so one large mesh among small ones does not hold the launch.
+constexpr unsigned kSegBlocks = 8;
 template <class R, int D>
 __global__ void __launch_bounds__(kThreads)
     k_seg_moments(const R *pts, uint64_t n, const uint64_t *seg_off, double *partial) {
   __shared__ double sh[kWavesPerBlock][kPart];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const uint64_t b = seg_off[blockIdx.x], e = seg_off[blockIdx.x + 1];
+  const uint32_t mesh = blockIdx.x / kSegBlocks, part = blockIdx.x % kSegBlocks;
+  const uint64_t b = seg_off[mesh], e = seg_off[mesh + 1];
   double v[kPart];
   for (int k = 0; k < kMaxD; ++k)
     v[k] = INFINITY, v[kMaxD + k] = -INFINITY, v[2 * kMaxD + k] = 0, v[3 * kMaxD + k] = 0;
-  for (uint64_t i = b + threadIdx.x; i < e; i += kThreads)
+  for (uint64_t i = b + static_cast<uint64_t>(part) * kThreads + threadIdx.x; i < e;
+       i += static_cast<uint64_t>(kSegBlocks) * kThreads)
     for (int k = 0; k < D; ++k) {
       const double x = static_cast<double>(pts[k * n + i]);
       v[k] = fmin(v[k], x);
@@ -877,14 +882,14 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
         MVS_LAUNCH(s, "ms_bbox", (k_bbox<R, 4>), nb, kThreads, 0, pts, n, partial);
     }
   } else {
-    partial = scratch_as<double>(s, kSlotBatchMs1, kPart * meshes);
-    nb = 1;
+    partial = scratch_as<double>(s, kSlotBatchMs1, static_cast<size_t>(kPart) * meshes * kSegBlocks);
+    nb = kSegBlocks;
     if (D == 6)
-      MVS_LAUNCH(s, "ms_bbox", (k_seg_moments<R, 6>), meshes, kThreads, 0, pts, n, seg_off,
-                 partial);
+      MVS_LAUNCH(s, "ms_bbox", (k_seg_moments<R, 6>), meshes * kSegBlocks, kThreads, 0, pts, n,
+                 seg_off, partial);
     else
-      MVS_LAUNCH(s, "ms_bbox", (k_seg_moments<R, 4>), meshes, kThreads, 0, pts, n, seg_off,
-                 partial);
+      MVS_LAUNCH(s, "ms_bbox", (k_seg_moments<R, 4>), meshes * kSegBlocks, kThreads, 0, pts, n,
+                 seg_off, partial);
   }
   uint64_t *ncell = seg ? scratch_as<uint64_t>(s, kSlotBatchMs2, meshes + 1) : nullptr;
   // few seeds: four waves per seed, so the launch still fills the chip.  A
