@@ -674,13 +674,36 @@ __global__ void __launch_bounds__(kThreads)
     constexpr uint32_t kChunk = kMapCap - 1;  // the last slot holds the sentinel
     for (uint32_t cbase = 0; cbase < total; cbase += kChunk) {
       const uint32_t cend = total - cbase < kChunk ? total : cbase + kChunk;
-      for (int r = wsub; r < nrows; r += W) {
-        const uint32_t rs = __builtin_amdgcn_readlane(startv, r);
-        const uint32_t re = __builtin_amdgcn_readlane(startv, r + 1);
-        const uint32_t lo = rs > cbase ? rs : cbase, hi = re < cend ? re : cend;
-        const uint32_t b = __builtin_amdgcn_readlane(rb, r);
-        for (uint32_t c = lo + lane; c < hi; c += kWave)
-          map[c - cbase] = b + (c - rs);
+      // With one wave per seed and short rows (at most kShort candidates a
+      // row on average: small meshes), each lane writes the first kShort slots of
+      // its own row and the wave fills the rest of the longer rows; else
+      // the wave fills the rows one by one.
+      constexpr uint32_t kShort = 8;
+      if (W == 1 && total <= kShort * static_cast<uint32_t>(nrows)) {
+        const uint32_t lo = startv > cbase ? startv : cbase, hi = incl < cend ? incl : cend;
+        for (uint32_t e = 0; e < kShort; ++e)
+          if (lo + e < hi)
+            map[lo + e - cbase] = rb + (lo + e - startv);
+        uint64_t longer = ballot(hi > lo + kShort);
+        while (longer) {
+          const int r = __builtin_ctzll(longer);
+          longer &= longer - 1;
+          const uint32_t rs = __builtin_amdgcn_readlane(startv, r);
+          const uint32_t ra = __builtin_amdgcn_readlane(lo, r) + kShort;
+          const uint32_t rz = __builtin_amdgcn_readlane(hi, r);
+          const uint32_t b = __builtin_amdgcn_readlane(rb, r);
+          for (uint32_t c = ra + lane; c < rz; c += kWave)
+            map[c - cbase] = b + (c - rs);
+        }
+      } else {
+        for (int r = wsub; r < nrows; r += W) {
+          const uint32_t rs = __builtin_amdgcn_readlane(startv, r);
+          const uint32_t re = __builtin_amdgcn_readlane(startv, r + 1);
+          const uint32_t lo = rs > cbase ? rs : cbase, hi = re < cend ? re : cend;
+          const uint32_t b = __builtin_amdgcn_readlane(rb, r);
+          for (uint32_t c = lo + lane; c < hi; c += kWave)
+            map[c - cbase] = b + (c - rs);
+        }
       }
       const uint32_t send = cend - cbase;  // the sentinel's slot
       if (wsub == 0 && lane == 0)
