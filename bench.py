@@ -269,6 +269,100 @@ def bench_refine(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_precision(args):
+    """Config 3: the 96x96 height field (2,000 samples, t = .05) in both
+    arithmetics.  The FP32-guarded pair and mirror sets are compared with
+    the FP64 ones (they must be equal), FP64 rechecks counted, and the
+    mean-shift of both paths is run from the same seed positions (every
+    k-th FP64 vote, at most 2,000 per kind, 30 iterations, h = .05, cut
+    3h): max / median error of the FP32 modes against the FP64 GPU modes in
+    radians (rotation angle of R32^T R64; angle between mirror normals) and
+    length units (translation, mirror offset)."""
+    import numpy as np
+    import torch
+    from scipy.spatial.transform import Rotation
+    from mitra_algo_amd import capi, workload
+    wl = workload.cfg1()
+    res = {}
+    for prec in (capi.FP64, capi.FP32_GUARDED):
+        dt = torch.float32 if prec == capi.FP32_GUARDED else torch.float64
+        for _ in range(args.warmup):
+            wl.pairvote(prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            pv = wl.pairvote(prec)
+        torch.cuda.synchronize()
+        ms_pv = (time.perf_counter() - t0) / args.steps * 1e3
+        st = wl.sess.dev_pair_stats()
+        P, R = pv.pairs, pv.reflections
+        f = wl.sess.fetch
+        pairs = (f(pv.a, (P,), torch.int32).cpu(), f(pv.b, (P,), torch.int32).cpu(),
+                 f(pv.refl_pair, (R,), torch.int32).cpu())
+        q4 = f(pv.rigid_q4, (4, 2 * P), dt, 2 * P)
+        t3 = f(pv.rigid_t3, (3, 2 * P), dt, 2 * P)
+        n3 = f(pv.refl_n3, (3, R), dt, pv.refl_stride)
+        d1 = f(pv.refl_d, (R,), dt)
+        rp = torch.empty((6, 2 * P), dtype=dt, device="cuda")
+        fpts = torch.empty((4, R), dtype=dt, device="cuda")
+        wl.sess.dev_ms_points(prec, 2 * P, q4, t3, R, n3, d1, rp, fpts)
+        torch.cuda.synchronize()
+        res[prec] = {"pairs": pairs, "stats": st, "ms": ms_pv, "pts": {6: rp, 4: fpts}}
+    a64, a32 = res[capi.FP64], res[capi.FP32_GUARDED]
+    same = all(torch.equal(x, y) for x, y in zip(a64["pairs"], a32["pairs"]))
+    modes = {}
+    for D, name in ((6, "rigid"), (4, "mirror")):
+        p64, p32 = a64["pts"][D], a32["pts"][D]
+        n = p64.shape[1]
+        if n == 0:
+            continue
+        idx = torch.arange(0, n, max(1, n // 2000), device="cuda")[:2000]
+        s64 = p64[:, idx].contiguous()
+        s32 = s64.float().contiguous()
+        ns = s64.shape[1]
+        o64, o32 = torch.empty_like(s64), torch.empty_like(s32)
+        w64 = torch.empty(ns, dtype=torch.float64, device="cuda")
+        w32 = torch.empty(ns, dtype=torch.float32, device="cuda")
+        wl.sess.dev_ms_batched(capi.FP64, D, n, p64, ns, s64, 30, 0.05, 3.0, o64, w64)
+        wl.sess.dev_ms_batched(capi.FP32_GUARDED, D, n, p32, ns, s32, 30, 0.05, 3.0, o32, w32)
+        torch.cuda.synchronize()
+        x64, x32 = o64.cpu().numpy(), o32.double().cpu().numpy()
+        if D == 6:
+            lin = np.linalg.norm(x32[0:3] - x64[0:3], axis=0)
+            R64 = Rotation.from_rotvec(x64[3:6].T)
+            R32 = Rotation.from_rotvec(x32[3:6].T)
+            ang = (R32.inv() * R64).magnitude()
+        else:
+            lin = np.abs(x32[0] - x64[0])
+            u64 = x64[1:4] / np.linalg.norm(x64[1:4], axis=0)
+            u32 = x32[1:4] / np.linalg.norm(x32[1:4], axis=0)
+            ang = np.arccos(np.clip((u64 * u32).sum(0), -1, 1))
+        # the bar: modes within 1e-5 rad and 1e-6 length units of FP64
+        modes[name] = {"seeds": int(ns), "units_max": float(lin.max()),
+                       "units_p99": float(np.percentile(lin, 99)),
+                       "units_median": float(np.median(lin)), "rad_max": float(ang.max()),
+                       "rad_p99": float(np.percentile(ang, 99)),
+                       "rad_median": float(np.median(ang)),
+                       "seeds_over_1e-6_units": int(np.sum(lin > 1e-6)),
+                       "seeds_over_1e-5_rad": int(np.sum(ang > 1e-5))}
+    out = {
+        "metric": "config 3: FP32-guarded vs FP64 pair sets and mean-shift modes (96x96 mesh)",
+        "value": a64["ms"] / a32["ms"], "unit": "x (FP64 pairvote ms / FP32-guarded pairvote ms)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": a32["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (FP64 recheck in guard bands) vs f64", "data": "synthetic",
+        "config": {"workload": "96x96 height field (18050 triangles), 2000 uniform samples, "
+                               "magnitude drop, t=0.05; mean-shift 30 iterations, h=0.05, cut 3h",
+                   "pairs": len(a64["pairs"][0]), "reflections": len(a64["pairs"][2]),
+                   "parallelism": "single GPU"},
+        "pair_set_identical": bool(same),
+        "pair_stats_fp32g": a32["stats"],
+        "pairvote_ms": {"fp64": a64["ms"], "fp32g": a32["ms"]},
+        "mode_error_fp32_vs_fp64": modes,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline_refine(probs):
     """The FP64 CPU reference of one energy + gradient evaluation of every
     mode (closest points of the last GPU step held fixed), one thread; the
@@ -308,9 +402,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("cfg2", "batch", "refine"), default="cfg2",
+    ap.add_argument("--workload", choices=("cfg2", "batch", "refine", "precision"),
+                    default="cfg2",
                     help="cfg2: the 2.1M-triangle mesh (default); batch: 512 meshes; "
-                         "refine: Gauss-Newton refinement of the top-64 modes of the 96x96 mesh")
+                         "refine: Gauss-Newton refinement of the top-64 modes of the 96x96 mesh; "
+                         "precision: FP32-guarded against FP64 on the 96x96 mesh")
     ap.add_argument("--meshes", type=int, default=512)
     ap.add_argument("--check-meshes", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -326,6 +422,9 @@ def main():
         return
     if args.workload == "refine":
         bench_refine(args)
+        return
+    if args.workload == "precision":
+        bench_precision(args)
         return
     import torch
     from mitra_algo_amd import capi, workload
