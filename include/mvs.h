@@ -201,6 +201,58 @@ int mvs_clustering_fetch(mvs_session *s, uint8_t *reflection, double *rotation9,
                          double *translation3, double *plane_normal3, double *plane_offset,
                          uint64_t *member_offsets, int32_t *members);
 
+/* Totals of the last mvs_verification call, summed over clusters. */
+typedef struct {
+  uint64_t side_a, side_b;        /* accepted vertices */
+  uint64_t patches_a, patches_b;  /* patches */
+} mvs_verification_counts;
+
+/* One side of a Verification (include/symmetry/storage.h:371-374): the side
+ * as a CSR over clusters (side_offsets: clusters+1, side: side total), and
+ * its patches as a grouped CSR (patch_groups: clusters+1 patch offsets,
+ * patch_offsets: patches+1 member offsets, patch_members: side total).
+ * NULL members are skipped by the fetch. */
+typedef struct {
+  uint64_t *side_offsets;
+  int32_t *side;
+  uint64_t *patch_groups;
+  uint64_t *patch_offsets;
+  int32_t *patch_members;
+} mvs_verification_side;
+
+/* Backend::verification (pipeline.h:67-68; src/cpu/verification.cpp:50-110).
+ * For every cluster: side A holds the vertices p whose forward image R p + t,
+ * side B those whose inverse image R^T (p - t), moves at least 2 tau, lies on
+ * the far side of a reflection plane by more than tau, is within tau of the
+ * mesh surface and carries the vertex normal to within 60 degrees of the
+ * nearest face's normal; tau = verify_threshold * box.  Patches are the
+ * connected components of each side over the one-ring graph, ordered by
+ * their smallest vertex, members ascending (verification.cpp:22-49); the
+ * neighbour rings must be symmetric, as prepare() produces them.  rms and
+ * coverage follow verification.cpp:95-96.  The surface search reproduces the
+ * reference's tree and visiting order, so the nearest face, and with it every
+ * set, equals the reference's; the squared distances are bit-identical.
+ * vertices/normals: nv points; faces: face_count triangles (f0, f1, f2);
+ * weights: nv vertex areas; neighbor_offsets: nv+1.  Clusters: cluster_count
+ * records (reflection flag, rotation row-major 9, translation 3, plane
+ * normal 3, plane offset).  Fetch with mvs_verification_fetch. */
+int mvs_verification(mvs_session *s, const mvs_points *vertices, uint64_t face_count,
+                     const int32_t *f0, const int32_t *f1, const int32_t *f2,
+                     const mvs_points *normals, const double *weights,
+                     const uint64_t *neighbor_offsets, const int32_t *neighbor_indices,
+                     uint64_t cluster_count, const uint8_t *reflection, const double *rotation9,
+                     const double *translation3, const double *plane_normal3,
+                     const double *plane_offset, double verify_threshold, double box,
+                     mvs_verification_counts *counts);
+/* Copies the last verification out: both sides, coverage and rms
+ * (cluster_count each).  Any pointer may be NULL. */
+int mvs_verification_fetch(mvs_session *s, const mvs_verification_side *side_a,
+                           const mvs_verification_side *side_b, double *coverage, double *rms);
+/* Per-row results of the last verification, 2*cluster_count*nv rows ordered
+ * (cluster, side, vertex): accepted 0/1 and the nearest squared distance
+ * (NaN for rows rejected before the surface search).  Either may be NULL. */
+int mvs_verification_rows(mvs_session *s, uint8_t *accepted, double *dist2);
+
 /* ---------------------------------------------------------- device layer
  * All pointers below are device pointers.  Work is ordered on the session
  * stream; the call returns once the launches are queued unless it must read

@@ -9,8 +9,7 @@
 //   pairing         -> mvs_pairing + mvs_pairing_fetch (FP64 predicate)
 //   transformations -> mvs_transformations + mvs_transformations_fetch
 //   clustering      -> mvs_clustering + mvs_clustering_fetch
-//   verification    -> the reference's own CPU verification (not on the
-//                      accelerated path)
+//   verification    -> mvs_verification + mvs_verification_fetch
 // A session (stream + scratch) lives for one call only, so no device state
 // survives between calls, as in src/cuda/backend.cu:341-351.
 #include "hip_backend.hpp"
@@ -136,7 +135,6 @@ public:
     mvs_session *probe = nullptr;
     check(mvs_session_create(device_, &probe));  // MVS_ENODEV without a GPU
     mvs_session_destroy(probe);
-    cpu_ = make_cpu_backend();
   }
   const char *name() const noexcept override { return "hip"; }
 
@@ -289,13 +287,54 @@ public:
 
   Verification verification(MeshView m, TopologyView t, ClustersView cl, ModelContext ctx,
                             const Options &o) const override {
-    return cpu_->verification(m, t, cl, ctx, o);
+    Call c(device_, profile_, "verification");
+    const Offset n = m.vertices.size(), F = m.faces.size(), C = cl.size();
+    std::vector<uint8_t> refl(C);
+    std::vector<double> r9(9 * C), t3(3 * C), n3(3 * C), po(C);
+    for (Offset i = 0; i < C; ++i) {
+      const ClusterRecord &rec = cl.records[i];
+      refl[i] = rec.reflection;
+      std::memcpy(&r9[9 * i], rec.rotation.data(), 9 * sizeof(double));
+      for (int k = 0; k < 3; ++k) {
+        t3[3 * i + k] = rec.translation[k];
+        n3[3 * i + k] = rec.plane_normal[k];
+      }
+      po[i] = rec.plane_offset;
+    }
+    const mvs_points verts = points(m.vertices), normals = points(t.normals);
+    const uint64_t zero = 0;
+    mvs_verification_counts k{};
+    check(mvs_verification(c.get(), &verts, F, m.faces.component[0].data,
+                           m.faces.component[1].data, m.faces.component[2].data, &normals,
+                           t.weights.data, t.neighbors.offsets.size() ? t.neighbors.offsets.data
+                                                                      : &zero,
+                           t.neighbors.indices.data, C, refl.data(), r9.data(), t3.data(),
+                           n3.data(), po.data(), o.verify_threshold, ctx.box_size, &k));
+    Verification r;
+    Csr *sides[2] = {&r.side_a, &r.side_b};
+    GroupedCsr *patches[2] = {&r.patches_a, &r.patches_b};
+    const uint64_t totals[2] = {k.side_a, k.side_b}, npatch[2] = {k.patches_a, k.patches_b};
+    mvs_verification_side out[2];
+    for (int s = 0; s < 2; ++s) {
+      sides[s]->offsets.resize(C + 1);
+      sides[s]->indices.resize(totals[s]);
+      patches[s]->groups.resize(C + 1);
+      patches[s]->rows.offsets.resize(npatch[s] + 1);
+      patches[s]->rows.indices.resize(totals[s]);
+      out[s] = {sides[s]->offsets.data(), sides[s]->indices.data(), patches[s]->groups.data(),
+                patches[s]->rows.offsets.data(), patches[s]->rows.indices.data()};
+    }
+    r.coverage.resize(C);
+    r.rms.resize(C);
+    check(mvs_verification_fetch(c.get(), &out[0], &out[1], r.coverage.data(), r.rms.data()));
+    c.h2d = 56 * n + 12 * F + 4 * t.neighbors.indices.size() + 136 * C;
+    c.d2h = 16 * (k.side_a + k.side_b);
+    return r;
   }
 
 private:
   bool profile_;
   int device_;
-  std::unique_ptr<Backend> cpu_;
 };
 
 }  // namespace

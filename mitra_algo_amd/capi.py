@@ -31,6 +31,7 @@ EXPORTS = [
     "mvs_batch_graph_check", "mvs_batch_autotune", "mvs_session_get_tuning",
     "mvs_session_set_tuning", "mvs_bvh_build", "mvs_dev_closest_points",
     "mvs_dev_closest_modes", "mvs_dev_align_energy",
+    "mvs_verification", "mvs_verification_fetch", "mvs_verification_rows",
 ]
 
 
@@ -140,6 +141,17 @@ class TuneEntry(C.Structure):
     def as_dict(self):
         return {"walk_quota": self.walk_quota, "walk_blocks": self.walk_blocks,
                 "wide_seeds": self.wide_seeds, "ms": self.ms}
+
+
+class VerificationCounts(C.Structure):
+    _fields_ = [("side_a", C.c_uint64), ("side_b", C.c_uint64), ("patches_a", C.c_uint64),
+                ("patches_b", C.c_uint64)]
+
+
+class VerificationSide(C.Structure):
+    _fields_ = [("side_offsets", C.c_void_p), ("side", C.c_void_p),
+                ("patch_groups", C.c_void_p), ("patch_offsets", C.c_void_p),
+                ("patch_members", C.c_void_p)]
 
 
 class BatchMeanShiftResult(C.Structure):
@@ -395,6 +407,62 @@ class Session:
         return dict(reflection=refl[:n], rotation=rot[:n], translation=tr[:n],
                     plane_normal=pn[:n], plane_offset=po[:n], member_off=off,
                     members=mem[:m], limited_iterations=lim.value)
+
+    def verification(self, vertices, faces, topo: dict, clusters: dict,
+                     verify_threshold: float, box: float) -> dict:
+        """Backend::verification (src/cpu/verification.cpp:50-110).  topo holds
+        normals (3, N), weights (N), nbr_off, nbr_idx; clusters holds
+        reflection (C), rotation (C, 9), translation (C, 3), plane_normal
+        (C, 3), plane_offset (C).  Returns the Verification fields: side_a /
+        side_b as (offsets, indices), patches_a / patches_b as (groups,
+        row offsets, members), coverage and rms."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        n = len(v)
+        x, y, z = (np.ascontiguousarray(v[:, k]) for k in range(3))
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+        f0, f1, f2 = (np.ascontiguousarray(f[:, k]) for k in range(3))
+        nrm = np.ascontiguousarray(topo["normals"], dtype=np.float64).reshape(3, n)
+        w = np.ascontiguousarray(topo["weights"], dtype=np.float64)
+        off = np.ascontiguousarray(topo["nbr_off"], dtype=np.uint64)
+        idx = np.ascontiguousarray(topo["nbr_idx"], dtype=np.int32)
+        refl = np.ascontiguousarray(clusters["reflection"], dtype=np.uint8)
+        nc = len(refl)
+        rot = np.ascontiguousarray(clusters["rotation"], dtype=np.float64).reshape(nc, 9)
+        tr = np.ascontiguousarray(clusters["translation"], dtype=np.float64).reshape(nc, 3)
+        pn = np.ascontiguousarray(clusters["plane_normal"], dtype=np.float64).reshape(nc, 3)
+        po = np.ascontiguousarray(clusters["plane_offset"], dtype=np.float64).reshape(nc)
+        cnt = VerificationCounts()
+        check(lib().mvs_verification(
+            self.h, C.byref(Points(n, ptr(x), ptr(y), ptr(z))), C.c_uint64(len(f)), ptr(f0),
+            ptr(f1), ptr(f2), C.byref(Points(n, ptr(nrm[0]), ptr(nrm[1]), ptr(nrm[2]))), ptr(w),
+            ptr(off), ptr(idx), C.c_uint64(nc), ptr(refl), ptr(rot), ptr(tr), ptr(pn), ptr(po),
+            C.c_double(verify_threshold), C.c_double(box), C.byref(cnt)))
+        out, sides = {}, []
+        for name, total, patches in (("a", cnt.side_a, cnt.patches_a),
+                                     ("b", cnt.side_b, cnt.patches_b)):
+            so = np.zeros(nc + 1, np.uint64)
+            sv = np.zeros(max(total, 1), np.int32)
+            pg = np.zeros(nc + 1, np.uint64)
+            pof = np.zeros(patches + 1, np.uint64)
+            pm = np.zeros(max(total, 1), np.int32)
+            sides.append(VerificationSide(ptr(so), ptr(sv), ptr(pg), ptr(pof), ptr(pm)))
+            out["side_" + name] = (so, sv[:total])
+            out["patches_" + name] = (pg, pof, pm[:total])
+        cov = np.zeros(max(nc, 1))
+        rms = np.zeros(max(nc, 1))
+        check(lib().mvs_verification_fetch(self.h, C.byref(sides[0]), C.byref(sides[1]),
+                                           ptr(cov), ptr(rms)))
+        out["coverage"], out["rms"] = cov[:nc], rms[:nc]
+        return out
+
+    def verification_rows(self, clusters: int, n: int):
+        """Per-row results of the last verification: accepted (clusters, 2, n)
+        and the nearest squared distance (NaN where the search did not run)."""
+        acc = np.zeros(max(2 * clusters * n, 1), np.uint8)
+        d2 = np.zeros(max(2 * clusters * n, 1))
+        check(lib().mvs_verification_rows(self.h, ptr(acc), ptr(d2)))
+        m = 2 * clusters * n
+        return acc[:m].reshape(clusters, 2, n), d2[:m].reshape(clusters, 2, n)
 
     # ---------------------------------------------------- device layer
     def dev_curvature(self, n, x, y, z, normals3, nf, fn3, off, idx, ne, edges, smoothing,

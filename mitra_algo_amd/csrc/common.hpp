@@ -114,6 +114,10 @@ enum Slot : int {
   // batches of meshes
   kSlotBatchMeta, kSlotBatchSeg, kSlotBatchPairOff, kSlotBatchReflOff, kSlotBatchMs0,
   kSlotBatchMs1, kSlotBatchMs2, kSlotBatchMs3, kSlotBatchMs4, kSlotBatchMs5,
+  // verification
+  kSlotVerNodes, kSlotVerFaces, kSlotVerPoints, kSlotVerMotions, kSlotVerNbrOff,
+  kSlotVerNbrIdx, kSlotVerAccepted, kSlotVerLabel, kSlotVerDist, kSlotVerPos, kSlotVerBounds,
+  kSlotVerVid, kSlotVerRoot, kSlotVerD2,
   kSlotCount
 };
 

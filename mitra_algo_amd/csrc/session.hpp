@@ -106,6 +106,11 @@ struct Session {
   std::vector<double> cl_rot, cl_trans, cl_pn, cl_po;
   std::vector<uint64_t> cl_off;
   std::vector<int32_t> cl_members;
+  // verification results awaiting fetch (index 0: side A, 1: side B)
+  std::vector<uint64_t> ver_side_off[2], ver_groups[2], ver_patch_off[2];
+  std::vector<int32_t> ver_side[2], ver_members[2];
+  std::vector<double> ver_cov, ver_rms;
+  uint64_t ver_rows = 0, ver_nv = 0;  // per-row results stay on the device
 
   void begin(const char *name);
   void end();

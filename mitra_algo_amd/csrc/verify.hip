@@ -1,0 +1,652 @@
+// Verification of detected transforms against the mesh surface
+// (Backend::verification, include/symmetry/pipeline.h:67-68;
+// src/cpu/verification.cpp:50-110) and its patches (connected components of
+// the accepted vertices over the one-ring graph, verification.cpp:22-49).
+//
+// For every cluster c and vertex i, two rows: side A maps p_i forward
+// (q = R p + t), side B backward (q = R^T (p - t)).  A row is accepted when
+//   |q - p| >= 2 tau, the reflection side test passes (A: n.p + d < -tau,
+//   B: n.p + d > tau; rigid clusters skip it), the nearest mesh face to q is
+//   within tau (squared distance <= tau^2) and the transformed vertex normal
+//   meets that face's unit normal at dot >= 0.5,
+// with tau = verify_threshold * box.  rms = sqrt(sum of side-A squared
+// distances / |A|), coverage = min(1, area(A) + area(B) over the total area).
+//
+// Exactness.  Every FP64 expression keeps the reference's operation order
+// (this library builds with -ffp-contract=off).  The nearest face comes from
+// a tree built exactly like the reference's surface tree
+// (src/cpu/surface.h:16-48): boxes over face corners, split at the median
+// face (std::nth_element by centroid along the widest axis) while a node holds
+// more than 8 faces, left half first; and it is searched in the same order
+// (surface.h:57-81: skip a node whose box is farther than the best distance,
+// scan a leaf's faces in order keeping strictly smaller distances, visit the
+// nearer child first).  The minimum distance is therefore the same value, and
+// among faces at exactly that distance the same face wins, so the normal test
+// and with it the accepted sets are the reference's.
+//
+// Device work: one thread per row (k_verify, FP64, per-thread stack in LDS),
+// a union-find over the accepted vertices whose roots are the smallest member
+// (k_cc_hook / k_cc_flatten), and an ordered compaction of the accepted rows.
+// The host groups the compact lists into patches in O(|side|).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "session.hpp"
+
+namespace mvs {
+namespace {
+
+constexpr int kVThreads = 256;
+constexpr int kLeafFaces = 8;
+constexpr int kFaceWords = 12;  // a, b, c, unit normal
+
+// Tree node: box, children (left < 0 for a leaf) and the leaf's face range
+// in leaf order.
+struct alignas(16) SNode {
+  double lo[3], hi[3];
+  int32_t left, right, begin, end;
+};
+static_assert(sizeof(SNode) == 64, "node layout");
+
+// One cluster's motion (ClusterRecord, include/symmetry/storage.h:365-370).
+struct Motion {
+  double r[9], t[3], n[3], off;
+  int32_t reflection, pad;
+};
+
+// ------------------------------------------------------------ host tree
+struct TreeBuilder {
+  const double *x, *y, *z;
+  const int32_t *f[3];
+  std::vector<double> cen[3];
+  std::vector<int> ids;
+  std::vector<SNode> nodes;
+  int depth = 0;
+
+  double coord(int v, int k) const { return k == 0 ? x[v] : (k == 1 ? y[v] : z[v]); }
+
+  // surface.h:16-48, pre-order node numbering
+  int build(int begin, int end, int level) {
+    depth = std::max(depth, level);
+    SNode n;
+    for (int k = 0; k < 3; ++k)
+      n.lo[k] = 1e300, n.hi[k] = -1e300;
+    for (int i = begin; i < end; ++i)
+      for (int c = 0; c < 3; ++c) {
+        const int v = f[c][ids[i]];
+        for (int k = 0; k < 3; ++k) {
+          n.lo[k] = std::min(n.lo[k], coord(v, k));
+          n.hi[k] = std::max(n.hi[k], coord(v, k));
+        }
+      }
+    n.left = n.right = -1;
+    n.begin = begin;
+    n.end = end;
+    const int index = static_cast<int>(nodes.size());
+    nodes.push_back(n);
+    if (end - begin > kLeafFaces) {
+      const double ex = n.hi[0] - n.lo[0], ey = n.hi[1] - n.lo[1], ez = n.hi[2] - n.lo[2];
+      int axis = ey > ex ? 1 : 0;
+      if (ez > (axis == 1 ? ey : ex))
+        axis = 2;
+      const int mid = (begin + end) / 2;
+      const double *c = cen[axis].data();
+      std::nth_element(ids.begin() + begin, ids.begin() + mid, ids.begin() + end,
+                       [c](int a, int b) { return c[a] < c[b]; });
+      const int l = build(begin, mid, level + 1);
+      const int r = build(mid, end, level + 1);
+      nodes[index].left = l;
+      nodes[index].right = r;
+    }
+    return index;
+  }
+};
+
+// -------------------------------------------------------- device geometry
+// std::max({a, b, c}) and std::clamp as the reference evaluates them.
+__device__ inline double max3(double a, double b, double c) {
+  double r = a;
+  if (r < b)
+    r = b;
+  if (r < c)
+    r = c;
+  return r;
+}
+__device__ inline double clamp01(double v) { return v < 0.0 ? 0.0 : (1.0 < v ? 1.0 : v); }
+__device__ inline double min2(double a, double b) { return b < a ? b : a; }
+
+// Squared distance from p to a node's box (surface.h:49-56).
+__device__ inline double box_bound(V3 p, const SNode &n) {
+  double s = 0;
+  double d = max3(n.lo[0] - p.x, 0.0, p.x - n.hi[0]);
+  s += d * d;
+  d = max3(n.lo[1] - p.y, 0.0, p.y - n.hi[1]);
+  s += d * d;
+  d = max3(n.lo[2] - p.z, 0.0, p.z - n.hi[2]);
+  s += d * d;
+  return s;
+}
+
+// Squared distance from p to segment ab (src/core/math.cpp:65-71).
+__device__ inline double seg_d2(V3 p, V3 a, V3 b) {
+  V3 d = b - a;
+  double l = dot(d, d);
+  double t = l > 1e-14 ? clamp01(dot(p - a, d) / l) : 0.0;
+  V3 e = p - (a + d * t);
+  return dot(e, e);
+}
+
+// Squared distance from p to triangle abc by the Voronoi regions of the
+// vertices, edges and face (Ericson, Real-Time Collision Detection, 2005,
+// 5.1.5), with the reference's degenerate fallback and operation order
+// (src/core/math.cpp:72-112).
+__device__ inline double tri_d2(V3 p, V3 a, V3 b, V3 c) {
+  V3 ab = b - a, ac = c - a, ap = p - a;
+  if (dot(cross(ab, ac), cross(ab, ac)) < 1e-14 * 1e-14)
+    return min2(min2(seg_d2(p, a, b), seg_d2(p, a, c)), seg_d2(p, b, c));
+  double d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0 && d2 <= 0)
+    return dot(ap, ap);
+  V3 bp = p - b;
+  double d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0 && d4 <= d3)
+    return dot(bp, bp);
+  double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+    V3 e = p - (a + ab * (d1 / (d1 - d3)));
+    return dot(e, e);
+  }
+  V3 cp = p - c;
+  double d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0 && d5 <= d6)
+    return dot(cp, cp);
+  double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+    V3 e = p - (a + ac * (d2 / (d2 - d6)));
+    return dot(e, e);
+  }
+  double va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    V3 e = p - (b + (c - b) * ((d4 - d3) / ((d4 - d3) + (d5 - d6))));
+    return dot(e, e);
+  }
+  V3 e = p - (a + ab * (vb / (va + vb + vc)) + ac * (vc / (va + vb + vc)));
+  double r = dot(e, e);
+  return 0.0 < r ? r : 0.0;
+}
+
+__device__ inline V3 face_corner(const double *fr, int k) {
+  return v3(fr[3 * k], fr[3 * k + 1], fr[3 * k + 2]);
+}
+
+// ------------------------------------------------------------- kernels
+// Row r = (2c + s) * nv + i.  accepted[r] in {0, 1}; label[r] = i when
+// accepted, else -1 (the union-find parent array); dist2[r] = the nearest
+// squared distance when the row reached the surface search, else NaN.
+__global__ void __launch_bounds__(kVThreads)
+k_verify(const double *__restrict__ vx, const double *__restrict__ vy,
+         const double *__restrict__ vz, const double *__restrict__ nx,
+         const double *__restrict__ ny, const double *__restrict__ nz, uint32_t nv,
+         uint64_t rows, const SNode *__restrict__ nodes, const double *__restrict__ faces,
+         const Motion *__restrict__ motions, double threshold,
+         uint32_t *__restrict__ accepted, int32_t *__restrict__ label,
+         double *__restrict__ dist2) {
+  extern __shared__ int stack_lds[];
+  int *stack = stack_lds + threadIdx.x;
+  const uint64_t row = static_cast<uint64_t>(blockIdx.x) * kVThreads + threadIdx.x;
+  if (row >= rows)
+    return;
+  const uint64_t seg = row / nv;
+  const uint32_t i = static_cast<uint32_t>(row - seg * nv);
+  const Motion &mo = motions[seg >> 1];
+  const bool inverse = seg & 1;
+  uint32_t ok = 0;
+  double best = __builtin_nan("");
+  const V3 p = v3(vx[i], vy[i], vz[i]);
+  M3 r;
+  for (int k = 0; k < 9; ++k)
+    r.m[k] = mo.r[k];
+  if (inverse) {
+    // transpose(c.rotation) (verification.cpp:59)
+    M3 t = {{r.m[0], r.m[3], r.m[6], r.m[1], r.m[4], r.m[7], r.m[2], r.m[5], r.m[8]}};
+    r = t;
+  }
+  const V3 tr = v3(mo.t[0], mo.t[1], mo.t[2]);
+  const V3 q = inverse ? apply(r, p - tr) : apply(r, p) + tr;
+  bool go = !(norm(q - p) < threshold * 2);
+  if (go && mo.reflection) {
+    const double side = dot(v3(mo.n[0], mo.n[1], mo.n[2]), p) + mo.off;
+    go = inverse ? !(side <= threshold) : !(side >= -threshold);
+  }
+  if (go) {
+    best = __builtin_inf();
+    int hit = -1, top = 1;
+    stack[0] = 0;
+    while (top > 0) {
+      const SNode nd = nodes[stack[--top * kVThreads]];
+      if (box_bound(q, nd) > best)
+        continue;
+      if (nd.left < 0) {
+        for (int k = nd.begin; k < nd.end; ++k) {
+          const double *fr = faces + static_cast<uint64_t>(k) * kFaceWords;
+          const double d = tri_d2(q, face_corner(fr, 0), face_corner(fr, 1), face_corner(fr, 2));
+          if (d < best) {
+            best = d;
+            hit = k;
+          }
+        }
+      } else {
+        int first = nd.left, second = nd.right;
+        if (box_bound(q, nodes[first]) > box_bound(q, nodes[second])) {
+          first = nd.right;
+          second = nd.left;
+        }
+        // depth-first, `first` on top: at most one waiting sibling per level
+        // plus the two children, so tree depth + 1 entries suffice
+        stack[top++ * kVThreads] = second;
+        stack[top++ * kVThreads] = first;
+      }
+    }
+    if (hit >= 0 && !(best > threshold * threshold)) {
+      const double *fr = faces + static_cast<uint64_t>(hit) * kFaceWords;
+      const V3 fn = face_corner(fr, 3);
+      const V3 vn = apply(r, v3(nx[i], ny[i], nz[i]));
+      ok = dot(vn, fn) < 0.5 ? 0u : 1u;
+    }
+  }
+  accepted[row] = ok;
+  label[row] = ok ? static_cast<int32_t>(i) : -1;
+  dist2[row] = best;
+}
+
+__device__ inline int32_t load_parent(const int32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Root of x: parents only point to smaller vertices, so the walk ends.
+__device__ inline int32_t cc_find(const int32_t *par, int32_t x) {
+  int32_t p = load_parent(par + x);
+  while (p != x) {
+    x = p;
+    p = load_parent(par + x);
+  }
+  return x;
+}
+
+// Union-find over the accepted vertices of each (cluster, side) segment:
+// every ring edge (u, v), u < v, with both ends accepted joins their trees by
+// hooking the larger root under the smaller (compare-and-swap on the root), so
+// every parent is smaller than its child and each root is the smallest
+// member of its component (Jaiganesh and Burtscher, "A high-performance
+// connected components implementation for GPUs", HPDC 2018).
+__global__ void __launch_bounds__(kVThreads)
+k_cc_hook(const uint64_t *__restrict__ nbr_off, const int32_t *__restrict__ nbr_idx,
+          uint32_t nv, uint64_t rows, int32_t *label) {
+  const uint64_t row = static_cast<uint64_t>(blockIdx.x) * kVThreads + threadIdx.x;
+  if (row >= rows)
+    return;
+  const uint64_t seg = row / nv;
+  const int32_t u = static_cast<int32_t>(row - seg * nv);
+  int32_t *par = label + seg * nv;
+  if (load_parent(par + u) < 0)
+    return;
+  for (uint64_t j = nbr_off[u]; j < nbr_off[u + 1]; ++j) {
+    const int32_t v = nbr_idx[j];
+    if (v <= u || load_parent(par + v) < 0)
+      continue;
+    while (true) {
+      const int32_t a = cc_find(par, u), b = cc_find(par, v);
+      if (a == b)
+        break;
+      const int32_t hi = a > b ? a : b, lo = a > b ? b : a;
+      int32_t expect = hi;
+      if (__hip_atomic_compare_exchange_strong(par + hi, &expect, lo, __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        break;
+    }
+  }
+}
+
+// label[r] = root of r's vertex (the smallest member of its patch).
+__global__ void __launch_bounds__(kVThreads)
+k_cc_flatten(uint32_t nv, uint64_t rows, int32_t *label) {
+  const uint64_t row = static_cast<uint64_t>(blockIdx.x) * kVThreads + threadIdx.x;
+  if (row >= rows)
+    return;
+  const uint64_t seg = row / nv;
+  const int32_t u = static_cast<int32_t>(row - seg * nv);
+  int32_t *par = label + seg * nv;
+  if (load_parent(par + u) < 0)
+    return;
+  const int32_t root = cc_find(par, u);
+  __hip_atomic_store(par + u, root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Accepted rows in row order: vertex, patch root and squared distance.
+__global__ void __launch_bounds__(kVThreads)
+k_compact(const uint32_t *__restrict__ accepted, const uint64_t *__restrict__ pos,
+          const int32_t *__restrict__ label, const double *__restrict__ dist2, uint32_t nv,
+          uint64_t rows, int32_t *__restrict__ vid, int32_t *__restrict__ root,
+          double *__restrict__ d2) {
+  const uint64_t row = static_cast<uint64_t>(blockIdx.x) * kVThreads + threadIdx.x;
+  if (row >= rows || !accepted[row])
+    return;
+  const uint64_t o = pos[row];
+  const uint64_t seg = row / nv;
+  vid[o] = static_cast<int32_t>(row - seg * nv);
+  root[o] = label[row];
+  d2[o] = dist2[row];
+}
+
+// First compact position of every segment: bnd[k] = pos[k * nv], k <= segs.
+__global__ void k_bounds(const uint64_t *__restrict__ pos, uint32_t nv, uint32_t segs,
+                         uint64_t *__restrict__ bnd) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= segs)
+    bnd[k] = pos[static_cast<uint64_t>(k) * nv];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- host side
+namespace {
+
+struct VerifyInput {
+  const mvs_points *vertices;
+  uint64_t nf;
+  const int32_t *f[3];
+  const mvs_points *normals;
+  const double *weights;
+  const uint64_t *nbr_off;
+  const int32_t *nbr_idx;
+  uint64_t nc;
+  const uint8_t *reflection;
+  const double *rotation9, *translation3, *plane_normal3, *plane_offset;
+  double verify_threshold, box;
+};
+
+// Groups one segment's compact rows (ascending vertices with their patch
+// roots) into patches: a root is its patch's smallest vertex, so patches
+// appear in the order of their roots, which is the reference's order of
+// breadth-first seeds (verification.cpp:27-47); members stay ascending.
+void group_patches(const int32_t *vid, const int32_t *root, uint64_t n,
+                   std::vector<int32_t> &slot_of, std::vector<uint64_t> &offsets,
+                   std::vector<int32_t> &members) {
+  std::vector<uint64_t> count;
+  for (uint64_t k = 0; k < n; ++k) {
+    if (root[k] == vid[k]) {
+      slot_of[vid[k]] = static_cast<int32_t>(count.size());
+      count.push_back(0);
+    }
+    ++count[static_cast<size_t>(slot_of[root[k]])];
+  }
+  const uint64_t base = members.size();
+  std::vector<uint64_t> next(count.size());
+  uint64_t run = base;
+  for (size_t g = 0; g < count.size(); ++g) {
+    next[g] = run;
+    run += count[g];
+    offsets.push_back(run);
+  }
+  members.resize(run);
+  for (uint64_t k = 0; k < n; ++k)
+    members[next[static_cast<size_t>(slot_of[root[k]])]++] = vid[k];
+}
+
+void verification(Session *s, const VerifyInput &in, mvs_verification_counts *counts) {
+  const uint64_t nv = in.vertices->count, nf = in.nf, nc = in.nc;
+  if (nv > static_cast<uint64_t>(INT32_MAX) || nf > static_cast<uint64_t>(INT32_MAX))
+    raise(MVS_EOVERFLOW, "verification: vertex and face ids must fit int32");
+  if (in.normals->count != nv)
+    raise(MVS_EINVAL, "verification: normals and vertices differ in size");
+  if (nc > 0 && 2 * nc > UINT32_MAX / 2)
+    raise(MVS_EOVERFLOW, "verification: too many clusters");
+  for (uint64_t f = 0; f < nf; ++f)
+    for (int c = 0; c < 3; ++c)
+      if (in.f[c][f] < 0 || static_cast<uint64_t>(in.f[c][f]) >= nv)
+        raise(MVS_EINVAL, "verification: face vertex index out of range");
+  if (in.nbr_off[0] != 0)
+    raise(MVS_EINVAL, "verification: neighbor offsets must start at 0");
+  for (uint64_t i = 0; i < nv; ++i)
+    if (in.nbr_off[i + 1] < in.nbr_off[i])
+      raise(MVS_EINVAL, "verification: neighbor offsets must not decrease");
+  const uint64_t nnz = in.nbr_off[nv];
+  for (uint64_t j = 0; j < nnz; ++j)
+    if (in.nbr_idx[j] < 0 || static_cast<uint64_t>(in.nbr_idx[j]) >= nv)
+      raise(MVS_EINVAL, "verification: neighbor index out of range");
+
+  const double threshold = in.verify_threshold * in.box;
+  const uint64_t segs = 2 * nc, rows = segs * nv;
+  // results of this call (fetch reads them)
+  for (int sd = 0; sd < 2; ++sd) {
+    s->ver_side_off[sd].assign(1, 0);
+    s->ver_side[sd].clear();
+    s->ver_groups[sd].assign(1, 0);
+    s->ver_patch_off[sd].assign(1, 0);
+    s->ver_members[sd].clear();
+  }
+  s->ver_cov.assign(nc, 0.0);
+  s->ver_rms.assign(nc, 0.0);
+  s->ver_rows = 0;
+  s->ver_nv = nv;
+
+  std::vector<uint64_t> bnd(segs + 1, 0);
+  std::vector<int32_t> vid, root;
+  std::vector<double> d2;
+  if (rows > 0) {
+    // tree (surface.h:89-94); an empty surface is a single empty leaf
+    TreeBuilder tb;
+    tb.x = in.vertices->x, tb.y = in.vertices->y, tb.z = in.vertices->z;
+    for (int c = 0; c < 3; ++c)
+      tb.f[c] = in.f[c];
+    for (int k = 0; k < 3; ++k) {
+      tb.cen[k].resize(nf);
+      for (uint64_t f = 0; f < nf; ++f)
+        tb.cen[k][f] = (tb.coord(in.f[0][f], k) + tb.coord(in.f[1][f], k) +
+                        tb.coord(in.f[2][f], k)) / 3;
+    }
+    tb.ids.resize(nf);
+    std::iota(tb.ids.begin(), tb.ids.end(), 0);
+    tb.nodes.reserve(nf / 4 + 1);
+    tb.build(0, static_cast<int>(nf), 0);
+    // leaf-ordered faces with their unit normals (verification.cpp:71-73)
+    std::vector<double> fr(std::max<uint64_t>(nf, 1) * kFaceWords, 0.0);
+    for (uint64_t k = 0; k < nf; ++k) {
+      const int f = tb.ids[k];
+      V3 c[3];
+      for (int j = 0; j < 3; ++j) {
+        const int v = in.f[j][f];
+        c[j] = v3(tb.x[v], tb.y[v], tb.z[v]);
+      }
+      const V3 n = unit(cross(c[1] - c[0], c[2] - c[0]));
+      double *o = fr.data() + k * kFaceWords;
+      for (int j = 0; j < 3; ++j)
+        o[3 * j] = c[j].x, o[3 * j + 1] = c[j].y, o[3 * j + 2] = c[j].z;
+      o[9] = n.x, o[10] = n.y, o[11] = n.z;
+    }
+    std::vector<Motion> mo(nc);
+    for (uint64_t c = 0; c < nc; ++c) {
+      std::memset(&mo[c], 0, sizeof(Motion));
+      for (int k = 0; k < 9; ++k)
+        mo[c].r[k] = in.rotation9[9 * c + k];
+      for (int k = 0; k < 3; ++k) {
+        mo[c].t[k] = in.translation3[3 * c + k];
+        mo[c].n[k] = in.plane_normal3[3 * c + k];
+      }
+      mo[c].off = in.plane_offset[c];
+      mo[c].reflection = in.reflection[c] ? 1 : 0;
+    }
+    const int stack_size = tb.depth + 1;
+    const size_t lds = static_cast<size_t>(stack_size) * kVThreads * sizeof(int);
+    if (lds > 64 * 1024)
+      raise(MVS_EOVERFLOW, "verification: surface tree too deep");
+
+    const size_t node_bytes = tb.nodes.size() * sizeof(SNode);
+    SNode *d_nodes = static_cast<SNode *>(scratch(s, kSlotVerNodes, node_bytes));
+    double *d_faces = scratch_as<double>(s, kSlotVerFaces, fr.size());
+    double *d_pts = scratch_as<double>(s, kSlotVerPoints, 6 * nv);
+    Motion *d_mo = static_cast<Motion *>(scratch(s, kSlotVerMotions, nc * sizeof(Motion)));
+    uint64_t *d_noff = scratch_as<uint64_t>(s, kSlotVerNbrOff, nv + 1);
+    int32_t *d_nidx = scratch_as<int32_t>(s, kSlotVerNbrIdx, std::max<uint64_t>(nnz, 1));
+    uint32_t *d_acc = scratch_as<uint32_t>(s, kSlotVerAccepted, rows);
+    int32_t *d_label = scratch_as<int32_t>(s, kSlotVerLabel, rows);
+    double *d_dist = scratch_as<double>(s, kSlotVerDist, rows);
+    uint64_t *d_pos = scratch_as<uint64_t>(s, kSlotVerPos, rows + 1);
+    uint64_t *d_bnd = scratch_as<uint64_t>(s, kSlotVerBounds, segs + 1);
+
+    copy_h2d(s, d_nodes, tb.nodes.data(), node_bytes);
+    copy_h2d(s, d_faces, fr.data(), fr.size() * sizeof(double));
+    const double *src[6] = {in.vertices->x, in.vertices->y, in.vertices->z,
+                            in.normals->x,  in.normals->y,  in.normals->z};
+    for (int k = 0; k < 6; ++k)
+      copy_h2d(s, d_pts + k * nv, src[k], nv * sizeof(double));
+    copy_h2d(s, d_mo, mo.data(), nc * sizeof(Motion));
+    copy_h2d(s, d_noff, in.nbr_off, (nv + 1) * sizeof(uint64_t));
+    copy_h2d(s, d_nidx, in.nbr_idx, nnz * sizeof(int32_t));
+
+    const unsigned grid = blocks_for(rows, kVThreads);
+    const uint32_t nv32 = static_cast<uint32_t>(nv);
+    MVS_LAUNCH(s, "verify_rows", k_verify, grid, kVThreads, lds, d_pts, d_pts + nv,
+               d_pts + 2 * nv, d_pts + 3 * nv, d_pts + 4 * nv, d_pts + 5 * nv, nv32, rows,
+               d_nodes, d_faces, d_mo, threshold, d_acc, d_label, d_dist);
+    MVS_LAUNCH(s, "verify_cc_hook", k_cc_hook, grid, kVThreads, 0, d_noff, d_nidx, nv32, rows,
+               d_label);
+    MVS_LAUNCH(s, "verify_cc_flatten", k_cc_flatten, grid, kVThreads, 0, nv32, rows, d_label);
+    exclusive_scan_u32(s, d_acc, d_pos, rows);
+    MVS_LAUNCH(s, "verify_bounds", k_bounds, blocks_for(segs + 1, 64), 64, 0, d_pos, nv32,
+               static_cast<uint32_t>(segs), d_bnd);
+    copy_d2h(s, bnd.data(), d_bnd, (segs + 1) * sizeof(uint64_t));
+    const uint64_t total = bnd[segs];
+    int32_t *d_vid = scratch_as<int32_t>(s, kSlotVerVid, std::max<uint64_t>(total, 1));
+    int32_t *d_root = scratch_as<int32_t>(s, kSlotVerRoot, std::max<uint64_t>(total, 1));
+    double *d_d2 = scratch_as<double>(s, kSlotVerD2, std::max<uint64_t>(total, 1));
+    MVS_LAUNCH(s, "verify_compact", k_compact, grid, kVThreads, 0, d_acc, d_pos, d_label,
+               d_dist, nv32, rows, d_vid, d_root, d_d2);
+    vid.resize(total);
+    root.resize(total);
+    d2.resize(total);
+    copy_d2h(s, vid.data(), d_vid, total * sizeof(int32_t));
+    copy_d2h(s, root.data(), d_root, total * sizeof(int32_t));
+    copy_d2h(s, d2.data(), d_d2, total * sizeof(double));
+    s->ver_rows = rows;
+  }
+
+  // assemble in the reference's order (verification.cpp:57-107)
+  double total_area = 0;
+  for (uint64_t i = 0; i < nv; ++i)
+    total_area += in.weights[i];
+  std::vector<int32_t> slot_of(std::max<uint64_t>(nv, 1), -1);
+  for (uint64_t c = 0; c < nc; ++c) {
+    double error = 0, area = 0;
+    for (int sd = 0; sd < 2; ++sd) {
+      const uint64_t b = bnd[2 * c + sd], e = bnd[2 * c + sd + 1];
+      auto &side = s->ver_side[sd];
+      side.insert(side.end(), vid.begin() + b, vid.begin() + e);
+      s->ver_side_off[sd].push_back(side.size());
+      for (uint64_t k = b; k < e; ++k) {
+        if (sd == 0)
+          error += d2[k];
+        area += in.weights[vid[k]];
+      }
+      group_patches(vid.data() + b, root.data() + b, e - b, slot_of, s->ver_patch_off[sd],
+                    s->ver_members[sd]);
+      s->ver_groups[sd].push_back(s->ver_patch_off[sd].size() - 1);
+    }
+    const uint64_t na = bnd[2 * c + 1] - bnd[2 * c];
+    s->ver_rms[c] = na == 0 ? 0 : std::sqrt(error / static_cast<double>(na));
+    s->ver_cov[c] = total_area > 0 ? std::min(1.0, area / total_area) : 0;
+  }
+  counts->side_a = s->ver_side[0].size();
+  counts->side_b = s->ver_side[1].size();
+  counts->patches_a = s->ver_patch_off[0].size() - 1;
+  counts->patches_b = s->ver_patch_off[1].size() - 1;
+}
+
+void put_side(Session *s, int sd, const mvs_verification_side *o) {
+  if (!o)
+    return;
+  auto put = [](auto *dst, const auto &v) {
+    if (dst && !v.empty())
+      std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+  };
+  put(o->side_offsets, s->ver_side_off[sd]);
+  put(o->side, s->ver_side[sd]);
+  put(o->patch_groups, s->ver_groups[sd]);
+  put(o->patch_offsets, s->ver_patch_off[sd]);
+  put(o->patch_members, s->ver_members[sd]);
+}
+
+}  // namespace
+}  // namespace mvs
+
+using namespace mvs;
+
+extern "C" {
+
+int mvs_verification(mvs_session *sp, const mvs_points *vertices, uint64_t face_count,
+                     const int32_t *f0, const int32_t *f1, const int32_t *f2,
+                     const mvs_points *normals, const double *weights,
+                     const uint64_t *neighbor_offsets, const int32_t *neighbor_indices,
+                     uint64_t cluster_count, const uint8_t *reflection, const double *rotation9,
+                     const double *translation3, const double *plane_normal3,
+                     const double *plane_offset, double verify_threshold, double box,
+                     mvs_verification_counts *counts) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    if (!vertices || !normals || !counts || !neighbor_offsets)
+      raise(MVS_EINVAL, "verification: null argument");
+    const uint64_t nv = vertices->count;
+    if (nv && (!vertices->x || !vertices->y || !vertices->z || !normals->x || !normals->y ||
+               !normals->z || !weights))
+      raise(MVS_EINVAL, "verification: null vertex array");
+    if (face_count && (!f0 || !f1 || !f2))
+      raise(MVS_EINVAL, "verification: null face array");
+    if (cluster_count && (!reflection || !rotation9 || !translation3 || !plane_normal3 ||
+                          !plane_offset))
+      raise(MVS_EINVAL, "verification: null cluster array");
+    if (nv && neighbor_offsets[nv] && !neighbor_indices)
+      raise(MVS_EINVAL, "verification: null neighbor indices");
+    VerifyInput in{vertices, face_count, {f0, f1, f2}, normals, weights, neighbor_offsets,
+                   neighbor_indices, cluster_count, reflection, rotation9, translation3,
+                   plane_normal3, plane_offset, verify_threshold, box};
+    verification(s, in, counts);
+  });
+}
+
+int mvs_verification_fetch(mvs_session *sp, const mvs_verification_side *side_a,
+                           const mvs_verification_side *side_b, double *coverage, double *rms) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    put_side(s, 0, side_a);
+    put_side(s, 1, side_b);
+    if (coverage && !s->ver_cov.empty())
+      std::memcpy(coverage, s->ver_cov.data(), s->ver_cov.size() * sizeof(double));
+    if (rms && !s->ver_rms.empty())
+      std::memcpy(rms, s->ver_rms.data(), s->ver_rms.size() * sizeof(double));
+  });
+}
+
+int mvs_verification_rows(mvs_session *sp, uint8_t *accepted, double *dist2) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    const uint64_t rows = s->ver_rows;
+    if (!rows)
+      return;
+    if (accepted) {
+      std::vector<uint32_t> a(rows);
+      copy_d2h(s, a.data(), scratch(s, kSlotVerAccepted, rows * sizeof(uint32_t)),
+               rows * sizeof(uint32_t));
+      for (uint64_t r = 0; r < rows; ++r)
+        accepted[r] = static_cast<uint8_t>(a[r]);
+    }
+    if (dist2)
+      copy_d2h(s, dist2, scratch(s, kSlotVerDist, rows * sizeof(double)),
+               rows * sizeof(double));
+  });
+}
+
+}  // extern "C"

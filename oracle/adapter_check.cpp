@@ -130,8 +130,29 @@ static int run(int argc, char **argv) {
     rng_eq = rng_eq && ga == gb && ca.clusters.size() == cb.clusters.size();
   }
 
+  // verification of the CPU run's clusters through both backends: sets,
+  // patches and coverage identical, rms identical
+  bool verify_eq = true;
+  {
+    Prepared p = prepare(read_mesh(argv[1]), o);
+    Verification va = cpu->verification(p.mesh.as_view(), p.topology.as_view(),
+                                        a.clusters.as_view(), p.context, o);
+    Verification vb = hip->verification(p.mesh.as_view(), p.topology.as_view(),
+                                        a.clusters.as_view(), p.context, o);
+    verify_eq = va.side_a.offsets == vb.side_a.offsets && va.side_a.indices == vb.side_a.indices &&
+                va.side_b.offsets == vb.side_b.offsets && va.side_b.indices == vb.side_b.indices &&
+                va.patches_a.groups == vb.patches_a.groups &&
+                va.patches_a.rows.offsets == vb.patches_a.rows.offsets &&
+                va.patches_a.rows.indices == vb.patches_a.rows.indices &&
+                va.patches_b.groups == vb.patches_b.groups &&
+                va.patches_b.rows.offsets == vb.patches_b.rows.offsets &&
+                va.patches_b.rows.indices == vb.patches_b.rows.indices &&
+                va.coverage == vb.coverage && va.rms == vb.rms &&
+                a.clusters.verification.side_a.indices == va.side_a.indices;
+  }
+
   bool ok = valid_eq && feat.ok && sample_eq && pruned_eq && pairs_eq && votes_eq && vote.ok &&
-            clusters_eq && clus.ok && rng_eq;
+            clusters_eq && clus.ok && rng_eq && verify_eq;
   std::ostringstream js;
   js.precision(17);
   js << std::boolalpha;
@@ -143,7 +164,9 @@ static int run(int argc, char **argv) {
      << ", \"pruned_equal\": " << pruned_eq << ", \"pairs_equal\": " << pairs_eq
      << ", \"votes_order_equal\": " << votes_eq << ", \"vote_max_rel\": " << vote.worst
      << ", \"clusters_equal\": " << clusters_eq << ", \"cluster_max_rel\": " << clus.worst
-     << ", \"rng_equal\": " << rng_eq << ", \"metrics\": [";
+     << ", \"rng_equal\": " << rng_eq << ", \"verification_equal\": " << verify_eq
+     << ", \"side_a\": " << b.clusters.verification.side_a.indices.size()
+     << ", \"metrics\": [";
   for (size_t i = 0; i < metrics.size(); ++i)
     js << (i ? ", " : "") << "{\"stage\": \"" << metrics[i].stage
        << "\", \"interval_ms\": " << metrics[i].gpu_interval_ms

@@ -905,3 +905,167 @@ int oracle_meanshift_batched(int64_t nv, int dim, const double *pts,
   }
   return OR_OK;
 }
+
+/* ------------------------------------------------------- verification ---
+ * src/core/math.cpp:65-112: squared distance from p to segment ab and to
+ * triangle abc (Voronoi regions of Ericson, Real-Time Collision Detection,
+ * 5.1.5; degenerate triangles fall back to their three edges). */
+static double or_clamp01(double v) { return v < 0.0 ? 0.0 : (1.0 < v ? 1.0 : v); }
+static double or_min(double a, double b) { return b < a ? b : a; }
+static double or_seg_d2(vec3 p, vec3 a, vec3 b) {
+  vec3 d = vsub(b, a);
+  double l = vdot(d, d);
+  double t = l > 1e-14 ? or_clamp01(vdot(vsub(p, a), d) / l) : 0;
+  vec3 e = vsub(p, vadd(a, vscale(d, t)));
+  return vdot(e, e);
+}
+double oracle_point_triangle_distance2(const double *pp, const double *pa, const double *pb,
+                                       const double *pc) {
+  vec3 p = {pp[0], pp[1], pp[2]}, a = {pa[0], pa[1], pa[2]}, b = {pb[0], pb[1], pb[2]},
+       c = {pc[0], pc[1], pc[2]};
+  vec3 ab = vsub(b, a), ac = vsub(c, a), ap = vsub(p, a);
+  if (vdot(vcross(ab, ac), vcross(ab, ac)) < 1e-14 * 1e-14)
+    return or_min(or_min(or_seg_d2(p, a, b), or_seg_d2(p, a, c)), or_seg_d2(p, b, c));
+  double d1 = vdot(ab, ap), d2 = vdot(ac, ap);
+  if (d1 <= 0 && d2 <= 0)
+    return vdot(ap, ap);
+  vec3 bp = vsub(p, b);
+  double d3 = vdot(ab, bp), d4 = vdot(ac, bp);
+  if (d3 >= 0 && d4 <= d3)
+    return vdot(bp, bp);
+  double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+    vec3 e = vsub(p, vadd(a, vscale(ab, d1 / (d1 - d3))));
+    return vdot(e, e);
+  }
+  vec3 cp = vsub(p, c);
+  double d5 = vdot(ab, cp), d6 = vdot(ac, cp);
+  if (d6 >= 0 && d5 <= d6)
+    return vdot(cp, cp);
+  double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+    vec3 e = vsub(p, vadd(a, vscale(ac, d2 / (d2 - d6))));
+    return vdot(e, e);
+  }
+  double va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    vec3 e = vsub(p, vadd(b, vscale(vsub(c, b), (d4 - d3) / ((d4 - d3) + (d5 - d6)))));
+    return vdot(e, e);
+  }
+  vec3 e = vsub(p, vadd(vadd(a, vscale(ab, vb / (va + vb + vc))), vscale(ac, vc / (va + vb + vc))));
+  double r = vdot(e, e);
+  return 0.0 < r ? r : 0.0;
+}
+
+/* src/cpu/verification.cpp:57-94, one row per (cluster, side, vertex), with
+ * a brute-force scan of every face in place of the surface tree
+ * (src/cpu/surface.h:85-95).  The minimum squared distance is the same value
+ * (the tree skips only boxes strictly farther than the best so far); the
+ * face the tree reports among several at exactly that distance depends on
+ * its visiting order, so the normal test is evaluated for every such face. */
+int oracle_verification_rows(int64_t nv, const double *x, const double *y, const double *z,
+                             int64_t nf, const int32_t *f0, const int32_t *f1,
+                             const int32_t *f2, const double *nx, const double *ny,
+                             const double *nz, int64_t nc, const uint8_t *refl,
+                             const double *rot, const double *trans, const double *pn,
+                             const double *po, double verify_threshold, double box,
+                             uint8_t *state, double *dist2) {
+  double threshold = verify_threshold * box;
+  for (int64_t c = 0; c < nc; ++c) {
+    mat3 r, inv;
+    for (int k = 0; k < 9; ++k)
+      r.m[k] = rot[9 * c + k];
+    inv = mtranspose(&r);
+    vec3 t = {trans[3 * c], trans[3 * c + 1], trans[3 * c + 2]};
+    vec3 n = {pn[3 * c], pn[3 * c + 1], pn[3 * c + 2]};
+    for (int side = 0; side < 2; ++side) {
+      const mat3 *m = side == 0 ? &r : &inv;
+      for (int64_t i = 0; i < nv; ++i) {
+        int64_t row = (2 * c + side) * nv + i;
+        vec3 p = {x[i], y[i], z[i]};
+        vec3 q = side == 0 ? vadd(mapply(&r, p), t) : mapply(&inv, vsub(p, t));
+        state[row] = OR_VERIFY_EARLY;
+        dist2[row] = NAN;
+        if (vnorm(vsub(q, p)) < threshold * 2)
+          continue;
+        if (refl[c]) {
+          double s = vdot(n, p) + po[c];
+          if (side == 0 ? s >= -threshold : s <= threshold)
+            continue;
+        }
+        double qq[3] = {q.x, q.y, q.z}, best = INFINITY;
+        for (int64_t f = 0; f < nf; ++f) {
+          double a[3] = {x[f0[f]], y[f0[f]], z[f0[f]]}, b[3] = {x[f1[f]], y[f1[f]], z[f1[f]]},
+                 cc[3] = {x[f2[f]], y[f2[f]], z[f2[f]]};
+          double d = oracle_point_triangle_distance2(qq, a, b, cc);
+          if (d < best)
+            best = d;
+        }
+        dist2[row] = best;
+        if (best > threshold * threshold) {
+          state[row] = OR_VERIFY_FAR;
+          continue;
+        }
+        vec3 vn = mapply(m, (vec3){nx[i], ny[i], nz[i]});
+        int pass = 0, failed = 0;
+        for (int64_t f = 0; f < nf; ++f) {
+          vec3 a = {x[f0[f]], y[f0[f]], z[f0[f]]}, b = {x[f1[f]], y[f1[f]], z[f1[f]]},
+               cc = {x[f2[f]], y[f2[f]], z[f2[f]]};
+          double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z}, pc[3] = {cc.x, cc.y, cc.z};
+          if (oracle_point_triangle_distance2(qq, pa, pb, pc) != best)
+            continue;
+          vec3 fnrm = vunit(vcross(vsub(b, a), vsub(cc, a)));
+          if (vdot(vn, fnrm) < 0.5)
+            failed = 1;
+          else
+            pass = 1;
+        }
+        state[row] = pass && failed ? OR_VERIFY_TIE : (pass ? OR_VERIFY_PASS : OR_VERIFY_NORMAL);
+      }
+    }
+  }
+  return OR_OK;
+}
+
+/* src/cpu/verification.cpp:22-49: breadth-first components of the vertices
+ * `ids` over the neighbour rings, seeded in the order of `ids`; each patch
+ * sorted.  off: patches+1 (capacity n_ids+1), members: n_ids. */
+int oracle_components(int64_t nv, const uint64_t *nbr_off, const int32_t *nbr_idx, int64_t n_ids,
+                      const int32_t *ids, uint64_t *off, int32_t *members, int64_t *patches) {
+  uint8_t *allowed = calloc((size_t)(nv > 0 ? nv : 1), 1);
+  uint8_t *seen = calloc((size_t)(nv > 0 ? nv : 1), 1);
+  if (!allowed || !seen) {
+    free(allowed);
+    free(seen);
+    return fail(OR_ERUNTIME, "allocation failed");
+  }
+  for (int64_t k = 0; k < n_ids; ++k)
+    allowed[ids[k]] = 1;
+  int64_t np = 0, pos = 0;
+  off[0] = 0;
+  for (int64_t k = 0; k < n_ids; ++k) {
+    int32_t seed = ids[k];
+    if (seen[seed])
+      continue;
+    /* the queue is the patch itself: members[start .. tail) */
+    int64_t start = pos, head = pos;
+    members[pos++] = seed;
+    seen[seed] = 1;
+    while (head < pos) {
+      int32_t v = members[head++];
+      for (uint64_t j = nbr_off[v]; j < nbr_off[v + 1]; ++j) {
+        int32_t w = nbr_idx[j];
+        if (allowed[w] && !seen[w]) {
+          seen[w] = 1;
+          members[pos++] = w;
+        }
+      }
+    }
+    qsort(members + start, (size_t)(pos - start), sizeof(int32_t), cmp_i32);
+    off[++np] = (uint64_t)pos;
+  }
+  *patches = np;
+  free(allowed);
+  free(seen);
+  return OR_OK;
+}

@@ -33,6 +33,14 @@
  *                        2378/5630, 458/1122 (benchmarks/results/
  *                        rtx4070_wsl_cli.json) and 4470/10356 at t=.15
  *                        (CUDA_STAGE3_RU.md:125).
+ *   oracle_point_triangle_distance2  known answers (tests/geometry.cpp:8-20)
+ *                        and the compiled reference on random and degenerate
+ *                        triangles (the query set of
+ *                        tests/cuda_verification.cpp:77-99).
+ *   oracle_verification_rows, oracle_components  the compiled reference's
+ *                        Backend::verification on the example meshes
+ *                        (sides, patches, rms, coverage); mirror and half-turn
+ *                        closed forms on exactly symmetric grids.
  *   oracle_meanshift_batched  has no reference counterpart: parity unpinned
  *                        beyond closed forms (single tight cluster converges
  *                        to the kernel-weighted mean).
@@ -146,6 +154,34 @@ int oracle_meanshift_batched(int64_t nv, int dim, const double *points,
                              int dim_reflection, int64_t ns, const double *seeds,
                              const uint8_t *seed_kind, double h, double cutoff,
                              int iterations, double *out, double *weight);
+
+/* src/core/math.cpp:72-112. */
+double oracle_point_triangle_distance2(const double *p, const double *a, const double *b,
+                                       const double *c);
+
+/* Row states of oracle_verification_rows. */
+enum {
+  OR_VERIFY_EARLY = 0,   /* moved less than 2 tau, or on the plane's near side */
+  OR_VERIFY_FAR = 1,     /* nearest face farther than tau */
+  OR_VERIFY_NORMAL = 2,  /* normal test fails for every face at the minimum */
+  OR_VERIFY_PASS = 3,    /* accepted: passes for every face at the minimum */
+  OR_VERIFY_TIE = 4      /* faces at the minimum disagree: the reference's
+                            visiting order decides */
+};
+/* src/cpu/verification.cpp:57-94 per row (cluster c, side s, vertex i) at
+ * (2c+s)*nv + i, brute-force nearest face: state and the minimum squared
+ * distance (NaN for OR_VERIFY_EARLY).  rot: 9*nc row-major, trans/pn: 3*nc. */
+int oracle_verification_rows(int64_t nv, const double *x, const double *y, const double *z,
+                             int64_t nf, const int32_t *f0, const int32_t *f1,
+                             const int32_t *f2, const double *nx, const double *ny,
+                             const double *nz, int64_t nc, const uint8_t *refl,
+                             const double *rot, const double *trans, const double *pn,
+                             const double *po, double verify_threshold, double box,
+                             uint8_t *state, double *dist2);
+/* src/cpu/verification.cpp:22-49: components of `ids` (breadth-first, seeds
+ * in the order of ids, each patch sorted). */
+int oracle_components(int64_t nv, const uint64_t *nbr_off, const int32_t *nbr_idx, int64_t n_ids,
+                      const int32_t *ids, uint64_t *off, int32_t *members, int64_t *patches);
 
 #ifdef __cplusplus
 }

@@ -305,6 +305,87 @@ class Oracle:
         self._check(rc)
         return out, w
 
+    # ------------------------------------------------------ verification
+    def point_triangle_distance2(self, p, a, b, c) -> float:
+        f = getattr(self.lib, self.px + "point_triangle_distance2")
+        f.restype = C.c_double
+        arr = [np.ascontiguousarray(v, dtype=np.float64) for v in (p, a, b, c)]
+        return float(f(*[_p(v) for v in arr]))
+
+    @staticmethod
+    def _verify_args(vertices, faces, normals, clusters):
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+        n = np.ascontiguousarray(normals, dtype=np.float64).reshape(3, len(v))
+        cols = [np.ascontiguousarray(v[:, k]) for k in range(3)]
+        fc = [np.ascontiguousarray(f[:, k]) for k in range(3)]
+        nc = len(clusters["reflection"])
+        cl = [np.ascontiguousarray(clusters["reflection"], dtype=np.uint8)] + [
+            np.ascontiguousarray(clusters[k], dtype=np.float64).reshape(nc, -1)
+            for k in ("rotation", "translation", "plane_normal", "plane_offset")]
+        return len(v), len(f), nc, cols, fc, [n[0], n[1], n[2]], cl
+
+    def verification_rows(self, vertices, faces, normals, clusters, verify_threshold, box):
+        """oracle_verification_rows: (state, dist2), each (clusters, 2, nv)."""
+        assert self.which == "oracle"
+        nv, nf, nc, cols, fc, nrm, cl = self._verify_args(vertices, faces, normals, clusters)
+        state = np.zeros(max(2 * nc * nv, 1), np.uint8)
+        d2 = np.zeros(max(2 * nc * nv, 1))
+        rc = self._fn("verification_rows")(
+            C.c_int64(nv), *[_p(a) for a in cols], C.c_int64(nf), *[_p(a) for a in fc],
+            *[_p(np.ascontiguousarray(a)) for a in nrm], C.c_int64(nc), *[_p(a) for a in cl],
+            C.c_double(verify_threshold), C.c_double(box), _p(state), _p(d2))
+        self._check(rc)
+        m = 2 * nc * nv
+        return state[:m].reshape(nc, 2, nv), d2[:m].reshape(nc, 2, nv)
+
+    def components(self, nbr_off, nbr_idx, ids):
+        """oracle_components: list of sorted patches of `ids`."""
+        assert self.which == "oracle"
+        off = np.ascontiguousarray(nbr_off, dtype=np.uint64)
+        idx = np.ascontiguousarray(nbr_idx, dtype=np.int32)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        po = np.zeros(len(ids) + 1, np.uint64)
+        pm = np.zeros(max(len(ids), 1), np.int32)
+        npat = C.c_int64()
+        self._check(self._fn("components")(C.c_int64(len(off) - 1), _p(off), _p(idx),
+                                           C.c_int64(len(ids)), _p(ids), _p(po), _p(pm),
+                                           C.byref(npat)))
+        return [pm[po[k]:po[k + 1]].copy() for k in range(npat.value)]
+
+    def verification(self, vertices, faces, topo: Topology, clusters, verify_threshold, box):
+        """The compiled reference's Backend::verification (ref only): dict of
+        side_a / side_b (offsets, indices), patches_a / patches_b (groups, row
+        offsets, members), coverage, rms."""
+        assert self.which == "ref"
+        nv, nf, nc, cols, fc, nrm, cl = self._verify_args(vertices, faces, topo.normals, clusters)
+        cap = max(nc * nv, 1)
+        bufs = []
+        for _ in range(2):
+            bufs.append([np.zeros(nc + 1, np.uint64), np.zeros(cap, np.int32)])
+        pats = []
+        for _ in range(2):
+            pats.append([np.zeros(nc + 1, np.uint64), np.zeros(cap + 1, np.uint64),
+                         np.zeros(cap, np.int32)])
+        cov, rms = np.zeros(max(nc, 1)), np.zeros(max(nc, 1))
+        counts = np.zeros(4, np.uint64)
+        rc = self._fn("verification")(
+            C.c_int64(nv), *[_p(a) for a in cols], C.c_int64(nf), *[_p(a) for a in fc],
+            *[_p(np.ascontiguousarray(a)) for a in nrm],
+            _p(np.ascontiguousarray(topo.weights, dtype=np.float64)),
+            _p(np.ascontiguousarray(topo.nbr_off, dtype=np.uint64)),
+            _p(np.ascontiguousarray(topo.nbr_idx, dtype=np.int32)), C.c_int64(nc),
+            *[_p(a) for a in cl], C.c_double(verify_threshold), C.c_double(box),
+            _p(bufs[0][0]), _p(bufs[0][1]), _p(bufs[1][0]), _p(bufs[1][1]),
+            *[_p(a) for a in pats[0]], *[_p(a) for a in pats[1]], _p(cov), _p(rms), _p(counts))
+        self._check(rc)
+        sa, sb, pa, pb = (int(c) for c in counts)
+        return dict(side_a=(bufs[0][0], bufs[0][1][:sa].copy()),
+                    side_b=(bufs[1][0], bufs[1][1][:sb].copy()),
+                    patches_a=(pats[0][0], pats[0][1][:pa + 1].copy(), pats[0][2][:sa].copy()),
+                    patches_b=(pats[1][0], pats[1][1][:pb + 1].copy(), pats[1][2][:sb].copy()),
+                    coverage=cov[:nc].copy(), rms=rms[:nc].copy())
+
 
 def read_off(path: str):
     """Minimal OFF reader (triangles only) for the example fixtures."""

@@ -286,6 +286,67 @@ int ref_clustering(int64_t nv, const uint8_t *kind, const double *desc,
                 r.clusters.votes.indices.size() * sizeof(int32_t));
   });
 }
+// Backend::verification of the CPU backend (src/cpu/verification.cpp:50-110).
+// Sides: offsets (nc+1) and indices (capacity nc*nv each); patches: groups
+// (nc+1), row offsets (capacity nc*nv+1) and members (capacity nc*nv);
+// counts[4] = side_a, side_b, patches_a, patches_b totals.
+int ref_verification(int64_t nv, const double *x, const double *y, const double *z,
+                     int64_t nf, const int32_t *f0, const int32_t *f1, const int32_t *f2,
+                     const double *nx, const double *ny, const double *nz,
+                     const double *weights, const uint64_t *nbr_off, const int32_t *nbr_idx,
+                     int64_t nc, const uint8_t *refl, const double *rot, const double *trans,
+                     const double *pn, const double *po, double verify_threshold, double box,
+                     uint64_t *a_off, int32_t *a_idx, uint64_t *b_off, int32_t *b_idx,
+                     uint64_t *pa_groups, uint64_t *pa_off, int32_t *pa_idx,
+                     uint64_t *pb_groups, uint64_t *pb_off, int32_t *pb_idx,
+                     double *coverage, double *rms, uint64_t *counts) {
+  return guarded([&] {
+    MeshView m{pv(x, y, z, nv), {}};
+    m.faces.component[0] = av(f0, nf);
+    m.faces.component[1] = av(f1, nf);
+    m.faces.component[2] = av(f2, nf);
+    TopologyView t{};
+    t.normals = pv(nx, ny, nz, nv);
+    t.weights = av(weights, nv);
+    t.neighbors = {av(nbr_off, nv + 1), av(nbr_idx, static_cast<int64_t>(nbr_off[nv]))};
+    std::vector<ClusterRecord> recs(nc);
+    for (int64_t c = 0; c < nc; ++c) {
+      recs[c].reflection = refl[c] != 0;
+      for (int k = 0; k < 9; ++k)
+        recs[c].rotation[k] = rot[9 * c + k];
+      recs[c].translation = {trans[3 * c], trans[3 * c + 1], trans[3 * c + 2]};
+      recs[c].plane_normal = {pn[3 * c], pn[3 * c + 1], pn[3 * c + 2]};
+      recs[c].plane_offset = po[c];
+    }
+    std::vector<Offset> voff(nc + 1, 0);
+    ClustersView cv{av(recs.data(), nc), {av(voff.data(), nc + 1), {}}};
+    Options o;
+    o.verify_threshold = verify_threshold;
+    auto cpu = make_cpu_backend();
+    Verification r = cpu->verification(m, t, cv, ModelContext{box, {}}, o);
+    auto put = [](auto *dst, const auto &v) {
+      if (!v.empty())
+        std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    put(a_off, r.side_a.offsets);
+    put(a_idx, r.side_a.indices);
+    put(b_off, r.side_b.offsets);
+    put(b_idx, r.side_b.indices);
+    put(pa_groups, r.patches_a.groups);
+    put(pa_off, r.patches_a.rows.offsets);
+    put(pa_idx, r.patches_a.rows.indices);
+    put(pb_groups, r.patches_b.groups);
+    put(pb_off, r.patches_b.rows.offsets);
+    put(pb_idx, r.patches_b.rows.indices);
+    put(coverage, r.coverage);
+    put(rms, r.rms);
+    counts[0] = r.side_a.indices.size();
+    counts[1] = r.side_b.indices.size();
+    counts[2] = r.patches_a.rows.size();
+    counts[3] = r.patches_b.rows.size();
+  });
+}
+
 // The reference's exact squared point-triangle distance (math.cpp:72-112).
 double ref_point_triangle_distance2(const double *p, const double *a, const double *b,
                                     const double *c) {

@@ -38,8 +38,10 @@ def test_detect_through_adapter_matches_cpu(name):
     out = _run(os.path.join(GOLDEN, name + ".off"), "0.08", "1.0", "42")
     assert (out["pairs"], out["votes"]) == EXPECTED[name]
     assert out["clusters"] == 20
+    assert out["verification_equal"] and out["side_a"] > 0
     stages = {m["stage"] for m in out["metrics"]}
-    assert {"curvature", "sample_prune", "pairing", "transformation", "mean_shift"} <= stages
+    assert {"curvature", "sample_prune", "pairing", "transformation", "mean_shift",
+            "verification"} <= stages
 
 
 def test_detect_through_adapter_default_options():
