@@ -363,6 +363,111 @@ def bench_precision(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_verify(args):
+    """Verification (Backend::verification) of 20 cluster records on the
+    2.1M-triangle mesh: 16 rigid copy-to-copy motions and 4 reflections
+    through the centre, verify threshold .025 box.  The surface tree is built
+    once (timed apart); one step = one mvs_dev_verification call with the
+    mesh, normals and rings resident in HBM: 2 x 20 x N row searches, patches,
+    compaction, and the sides, patches, rms and coverage handed back on the
+    host.  value = verified rows (cluster x side x vertex) per second."""
+    import numpy as np
+    import torch
+    from mitra_algo_amd import capi, meshgen, workload
+    mesh = meshgen.cfg2_mesh(args.grid)
+    topo = capi.prepare(mesh.vertices, mesh.faces)
+    v, faces = mesh.vertices, topo["faces"]
+    nv = len(v)
+    cl = workload.copy_motions(mesh)
+    nc = len(cl["reflection"])
+    verify, box = 0.025, topo["box_size"]
+    sess = capi.Session(0)
+    t0 = time.perf_counter()
+    sess.verification_surface(v, faces)
+    surface_s = time.perf_counter() - t0
+    pts = torch.from_numpy(np.ascontiguousarray(np.concatenate([v.T, topo["normals"]]))).cuda()
+    off = torch.from_numpy(topo["nbr_off"].astype(np.int64)).cuda()
+    idx = torch.from_numpy(topo["nbr_idx"]).cuda()
+    w = topo["weights"]
+    torch.cuda.synchronize()
+
+    def step():
+        return sess.dev_verification(nv, pts, off, idx, w, cl, verify, box)
+
+    for _ in range(args.warmup):
+        step()
+    names = ("verify_rows", "verify_cc_init", "verify_cc_hook", "verify_cc_flatten",
+             "verify_compact")
+    sess.enable_timing(True, only=names)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    kern = {}
+    for k in names:
+        ms_, n = sess.timing(k)
+        if n:
+            kern[k] = {"ms_avg": ms_ / n, "launches_per_step": n / args.steps}
+    sess.enable_timing(False)
+    stats = sess.verification_stats()
+    rows = 2 * nc * nv
+    # the whole drop-in call (surface build, uploads, searches, results) once
+    t0 = time.perf_counter()
+    full = sess.verification(v, faces, topo, cl, verify, box)
+    full_s = time.perf_counter() - t0
+    assert all(np.array_equal(a, b) for a, b in zip(full["side_a"], r["side_a"]))
+    tflops = stats["fp64_ops"] / (kern["verify_rows"]["ms_avg"] * 1e-3) / 1e12
+    roof = {"bound": "fp64", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "verify_rows",
+            "ms_avg": kern["verify_rows"]["ms_avg"], "algorithmic_ops": stats["fp64_ops"]}
+    out = {
+        "metric": "verified rows/s (cluster x side x vertex) of Backend::verification",
+        "value": rows / dt, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"height field {args.grid}x{args.grid} ({len(faces)} triangles, "
+                               f"{nv} vertices), {nc} clusters (16 copy-to-copy rigid motions, "
+                               "4 reflections through the centre), verify threshold .025",
+                   "parallelism": "single GPU"},
+        "accepted": {"side_a": int(r["side_a"][0][-1]), "side_b": int(r["side_b"][0][-1]),
+                     "patches_a": int(len(r["patches_a"][1]) - 1)},
+        "search": {"triangle_tests_per_row": stats["triangle_tests"] / rows,
+                   "node_visits_per_row": stats["node_visits"] / rows,
+                   "fp64_ops": stats["fp64_ops"]},
+        "phases_ms": {"device": stats["device_ms"], "download": stats["download_ms"],
+                      "assembly": stats["assembly_ms"]},
+        "surface_build_s": surface_s, "host_call_s": full_s,
+        "host_call_rows_per_s": rows / full_s, "kernels": kern, "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_verify(v, faces, topo, cl, verify, box)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_verify(v, faces, topo, cl, verify, box):
+    """The reference's CPU verification (oracle/_ref, single thread) of the
+    first cluster on the full mesh, its own tree build included."""
+    import numpy as np
+    from oracle.pyoracle import Oracle
+    try:
+        lib, kind = Oracle("ref"), "reference"
+    except FileNotFoundError:
+        return None
+    T = type("T", (), {})()
+    T.normals, T.weights = topo["normals"], topo["weights"]
+    T.nbr_off, T.nbr_idx = topo["nbr_off"], topo["nbr_idx"]
+    one = {k: np.asarray(cl[k])[:1] for k in cl}
+    t0 = time.perf_counter()
+    lib.verification(v, faces, T, one, verify, box)
+    dt = time.perf_counter() - t0
+    return {"value": 2 * len(v) / dt, "unit": "rows/s", "cores": 1, "kind": kind,
+            "sample": f"Backend::verification of 1 of {len(cl['reflection'])} clusters on the "
+                      f"full mesh ({2 * len(v)} rows, tree build included) in {dt:.2f} s, "
+                      "single thread"}
+
+
 def cpu_baseline_refine(probs):
     """The FP64 CPU reference of one energy + gradient evaluation of every
     mode (closest points of the last GPU step held fixed), one thread; the
@@ -402,11 +507,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("cfg2", "batch", "refine", "precision"),
+    ap.add_argument("--workload", choices=("cfg2", "batch", "refine", "precision", "verify"),
                     default="cfg2",
                     help="cfg2: the 2.1M-triangle mesh (default); batch: 512 meshes; "
                          "refine: Gauss-Newton refinement of the top-64 modes of the 96x96 mesh; "
-                         "precision: FP32-guarded against FP64 on the 96x96 mesh")
+                         "precision: FP32-guarded against FP64 on the 96x96 mesh; "
+                         "verify: verification of 20 clusters on the 2.1M-triangle mesh")
     ap.add_argument("--meshes", type=int, default=512)
     ap.add_argument("--check-meshes", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -425,6 +531,9 @@ def main():
         return
     if args.workload == "precision":
         bench_precision(args)
+        return
+    if args.workload == "verify":
+        bench_verify(args)
         return
     import torch
     from mitra_algo_amd import capi, workload

@@ -4,7 +4,8 @@
  *
  * Two layers:
  *   1. Host layer (mvs_curvature, mvs_prune, mvs_pairing, mvs_transformations,
- *      mvs_clustering): borrowed HOST views in, host results out, synchronous.
+ *      mvs_clustering, mvs_verification): borrowed HOST views in, host
+ *      results out, synchronous.
  *      Each call is the drop-in replacement of one method of the reference's
  *      `symmetry::Backend` (include/symmetry/pipeline.h:54-69) with the same
  *      argument meaning, output order and error classes.
@@ -244,6 +245,24 @@ int mvs_verification(mvs_session *s, const mvs_points *vertices, uint64_t face_c
                      const double *translation3, const double *plane_normal3,
                      const double *plane_offset, double verify_threshold, double box,
                      mvs_verification_counts *counts);
+/* The same verification in two steps, for callers that keep the mesh in HBM
+ * and verify several cluster sets against one surface:
+ * mvs_verification_surface builds the surface tree of the mesh (as
+ * mvs_verification does) and keeps it on the session;
+ * mvs_dev_verification then runs against it with DEVICE inputs: pts6 holds
+ * 6*nv doubles (x, y, z, then the vertex normal x, y, z, each nv long) of the
+ * same vertices, neighbor_offsets (nv+1) and neighbor_indices the one-ring
+ * CSR; weights and the cluster records are HOST arrays.  The caller
+ * guarantees the device CSR is well formed (offsets non-decreasing from 0,
+ * indices below nv).  Results and fetch as for mvs_verification. */
+int mvs_verification_surface(mvs_session *s, const mvs_points *vertices, uint64_t face_count,
+                             const int32_t *f0, const int32_t *f1, const int32_t *f2);
+int mvs_dev_verification(mvs_session *s, uint64_t nv, const double *pts6,
+                         const uint64_t *neighbor_offsets, const int32_t *neighbor_indices,
+                         const double *weights, uint64_t cluster_count, const uint8_t *reflection,
+                         const double *rotation9, const double *translation3,
+                         const double *plane_normal3, const double *plane_offset,
+                         double verify_threshold, double box, mvs_verification_counts *counts);
 /* Copies the last verification out: both sides, coverage and rms
  * (cluster_count each).  Any pointer may be NULL. */
 int mvs_verification_fetch(mvs_session *s, const mvs_verification_side *side_a,
@@ -252,6 +271,13 @@ int mvs_verification_fetch(mvs_session *s, const mvs_verification_side *side_a,
  * (cluster, side, vertex): accepted 0/1 and the nearest squared distance
  * (NaN for rows rejected before the surface search).  Either may be NULL. */
 int mvs_verification_rows(mvs_session *s, uint8_t *accepted, double *dist2);
+/* Work of the last verification, summed over all rows: out[0] triangle
+ * distance tests, out[1] tree nodes visited, out[2] FP64 additions,
+ * subtractions, multiplications and divisions of the box and triangle
+ * distances (the path each test took); then host-clock microseconds of
+ * out[3] the device phase (launches to the segment sizes), out[4] the
+ * download of the accepted rows, out[5] the assembly of sides and patches. */
+int mvs_verification_stats(mvs_session *s, uint64_t out[6]);
 
 /* ---------------------------------------------------------- device layer
  * All pointers below are device pointers.  Work is ordered on the session

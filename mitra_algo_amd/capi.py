@@ -32,6 +32,7 @@ EXPORTS = [
     "mvs_session_set_tuning", "mvs_bvh_build", "mvs_dev_closest_points",
     "mvs_dev_closest_modes", "mvs_dev_align_energy",
     "mvs_verification", "mvs_verification_fetch", "mvs_verification_rows",
+    "mvs_verification_stats", "mvs_verification_surface", "mvs_dev_verification",
 ]
 
 
@@ -247,10 +248,25 @@ def sample_vertices(weights: np.ndarray, ratio: float, g: MT64) -> np.ndarray:
     return out[:cnt.value].copy()
 
 
+def _torch_first():
+    """Initialises torch's HIP runtime, when torch is installed and sees a GPU,
+    before libmvs touches the device.  torch ships its own HIP runtime beside
+    the ROCm one libmvs links; both drive the same device through one HSA
+    runtime (so device pointers pass between them), but torch's reports no
+    GPU when it initialises after the other in the same process."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
+
+
 class Session:
     """One libmvs session (device context + stream + scratch)."""
 
     def __init__(self, device: int = 0):
+        _torch_first()
         self.h = C.c_void_p()
         check(lib().mvs_session_create(C.c_int(device), C.byref(self.h)))
 
@@ -437,6 +453,9 @@ class Session:
             ptr(f1), ptr(f2), C.byref(Points(n, ptr(nrm[0]), ptr(nrm[1]), ptr(nrm[2]))), ptr(w),
             ptr(off), ptr(idx), C.c_uint64(nc), ptr(refl), ptr(rot), ptr(tr), ptr(pn), ptr(po),
             C.c_double(verify_threshold), C.c_double(box), C.byref(cnt)))
+        return self._verification_fetch(cnt, nc)
+
+    def _verification_fetch(self, cnt, nc: int) -> dict:
         out, sides = {}, []
         for name, total, patches in (("a", cnt.side_a, cnt.patches_a),
                                      ("b", cnt.side_b, cnt.patches_b)):
@@ -454,6 +473,42 @@ class Session:
                                            ptr(cov), ptr(rms)))
         out["coverage"], out["rms"] = cov[:nc], rms[:nc]
         return out
+
+    def verification_surface(self, vertices, faces):
+        """mvs_verification_surface: keep the mesh's surface tree on the session."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        x, y, z = (np.ascontiguousarray(v[:, k]) for k in range(3))
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+        f0, f1, f2 = (np.ascontiguousarray(f[:, k]) for k in range(3))
+        check(lib().mvs_verification_surface(self.h, C.byref(Points(len(v), ptr(x), ptr(y), ptr(z))),
+                                             C.c_uint64(len(f)), ptr(f0), ptr(f1), ptr(f2)))
+
+    def dev_verification(self, nv: int, pts6, nbr_off, nbr_idx, weights, clusters: dict,
+                         verify_threshold: float, box: float) -> dict:
+        """mvs_dev_verification: pts6 (6, nv) float64, nbr_off, nbr_idx device
+        tensors; weights a host array; results as verification()."""
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        refl = np.ascontiguousarray(clusters["reflection"], dtype=np.uint8)
+        nc = len(refl)
+        rot = np.ascontiguousarray(clusters["rotation"], dtype=np.float64).reshape(nc, 9)
+        tr = np.ascontiguousarray(clusters["translation"], dtype=np.float64).reshape(nc, 3)
+        pn = np.ascontiguousarray(clusters["plane_normal"], dtype=np.float64).reshape(nc, 3)
+        po = np.ascontiguousarray(clusters["plane_offset"], dtype=np.float64).reshape(nc)
+        cnt = VerificationCounts()
+        check(lib().mvs_dev_verification(self.h, C.c_uint64(nv), ptr(pts6), ptr(nbr_off),
+                                         ptr(nbr_idx), ptr(w), C.c_uint64(nc), ptr(refl), ptr(rot),
+                                         ptr(tr), ptr(pn), ptr(po), C.c_double(verify_threshold),
+                                         C.c_double(box), C.byref(cnt)))
+        return self._verification_fetch(cnt, nc)
+
+    def verification_stats(self) -> dict:
+        """Triangle tests, tree nodes visited and FP64 operations of the
+        surface searches of the last verification."""
+        out = np.zeros(6, np.uint64)
+        check(lib().mvs_verification_stats(self.h, ptr(out)))
+        return {"triangle_tests": int(out[0]), "node_visits": int(out[1]),
+                "fp64_ops": int(out[2]), "device_ms": out[3] / 1e3, "download_ms": out[4] / 1e3,
+                "assembly_ms": out[5] / 1e3}
 
     def verification_rows(self, clusters: int, n: int):
         """Per-row results of the last verification: accepted (clusters, 2, n)

@@ -115,9 +115,9 @@ enum Slot : int {
   kSlotBatchMeta, kSlotBatchSeg, kSlotBatchPairOff, kSlotBatchReflOff, kSlotBatchMs0,
   kSlotBatchMs1, kSlotBatchMs2, kSlotBatchMs3, kSlotBatchMs4, kSlotBatchMs5,
   // verification
-  kSlotVerNodes, kSlotVerFaces, kSlotVerPoints, kSlotVerMotions, kSlotVerNbrOff,
+  kSlotVerPoints, kSlotVerMotions, kSlotVerNbrOff,
   kSlotVerNbrIdx, kSlotVerAccepted, kSlotVerLabel, kSlotVerDist, kSlotVerPos, kSlotVerBounds,
-  kSlotVerVid, kSlotVerRoot, kSlotVerD2,
+  kSlotVerVid, kSlotVerRoot, kSlotVerD2, kSlotVerStats,
   kSlotCount
 };
 

@@ -110,7 +110,14 @@ struct Session {
   std::vector<uint64_t> ver_side_off[2], ver_groups[2], ver_patch_off[2];
   std::vector<int32_t> ver_side[2], ver_members[2];
   std::vector<double> ver_cov, ver_rms;
-  uint64_t ver_rows = 0, ver_nv = 0;  // per-row results stay on the device
+  uint64_t ver_rows = 0;  // per-row results of the last call stay on the device
+  uint64_t ver_us[3] = {0, 0, 0};  // host clock: device phase, download, assembly
+  std::vector<std::vector<int32_t>> ver_slot;  // per-thread patch scratch
+  // verification surface: reference-order FP64 tree and leaf-ordered faces
+  DBuf<unsigned char> ver_nodes, ver_faces;
+  int ver_stack = 0;          // LDS stack entries per thread (tree height + 1)
+  uint64_t ver_surface_nv = 0;
+  bool ver_surface = false;
 
   void begin(const char *name);
   void end();

@@ -26,14 +26,19 @@
 //
 // Device work: one thread per row (k_verify, FP64, per-thread stack in LDS),
 // a union-find over the accepted vertices whose roots are the smallest member
-// (k_cc_hook / k_cc_flatten), and an ordered compaction of the accepted rows.
+// (k_cc_init / k_cc_hook / k_cc_flatten), and an ordered compaction of the
+// accepted rows.
 // The host groups the compact lists into patches in O(|side|).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "session.hpp"
@@ -60,35 +65,52 @@ struct Motion {
 };
 
 // ------------------------------------------------------------ host tree
+// Surface tree in the reference's build order (surface.h:16-48), pre-order
+// node numbering.  A node of n faces splits into n / 2 faces on the left and
+// the rest on the right, so the node count and height of a subtree depend on
+// n alone: the right subtree's first node is known before the left one is
+// built, and independent subtrees are built by separate threads.  The
+// partitions are those of std::nth_element run on the same ranges with the
+// same comparisons, whichever thread runs them.
 struct TreeBuilder {
   const double *x, *y, *z;
   const int32_t *f[3];
   std::vector<double> cen[3];
+  std::vector<double> fbox;  // per face: corner minimum xyz, maximum xyz
   std::vector<int> ids;
   std::vector<SNode> nodes;
-  int depth = 0;
+  std::unordered_map<int, std::pair<int, int>> shape;  // faces -> (nodes, height)
 
   double coord(int v, int k) const { return k == 0 ? x[v] : (k == 1 ? y[v] : z[v]); }
 
-  // surface.h:16-48, pre-order node numbering
-  int build(int begin, int end, int level) {
-    depth = std::max(depth, level);
+  std::pair<int, int> shape_of(int n) {
+    auto it = shape.find(n);
+    if (it != shape.end())
+      return it->second;
+    std::pair<int, int> r{1, 0};
+    if (n > kLeafFaces) {
+      const auto l = shape_of(n / 2), h = shape_of(n - n / 2);
+      r = {1 + l.first + h.first, 1 + std::max(l.second, h.second)};
+    }
+    shape.emplace(n, r);
+    return r;
+  }
+
+  void build(int index, int begin, int end, int spawn) {
     SNode n;
     for (int k = 0; k < 3; ++k)
       n.lo[k] = 1e300, n.hi[k] = -1e300;
-    for (int i = begin; i < end; ++i)
-      for (int c = 0; c < 3; ++c) {
-        const int v = f[c][ids[i]];
-        for (int k = 0; k < 3; ++k) {
-          n.lo[k] = std::min(n.lo[k], coord(v, k));
-          n.hi[k] = std::max(n.hi[k], coord(v, k));
-        }
+    // the minimum over the faces' corner minima is the minimum over corners
+    for (int i = begin; i < end; ++i) {
+      const double *b = fbox.data() + 6 * static_cast<size_t>(ids[i]);
+      for (int k = 0; k < 3; ++k) {
+        n.lo[k] = std::min(n.lo[k], b[k]);
+        n.hi[k] = std::max(n.hi[k], b[3 + k]);
       }
+    }
     n.left = n.right = -1;
     n.begin = begin;
     n.end = end;
-    const int index = static_cast<int>(nodes.size());
-    nodes.push_back(n);
     if (end - begin > kLeafFaces) {
       const double ex = n.hi[0] - n.lo[0], ey = n.hi[1] - n.lo[1], ez = n.hi[2] - n.lo[2];
       int axis = ey > ex ? 1 : 0;
@@ -98,14 +120,38 @@ struct TreeBuilder {
       const double *c = cen[axis].data();
       std::nth_element(ids.begin() + begin, ids.begin() + mid, ids.begin() + end,
                        [c](int a, int b) { return c[a] < c[b]; });
-      const int l = build(begin, mid, level + 1);
-      const int r = build(mid, end, level + 1);
-      nodes[index].left = l;
-      nodes[index].right = r;
+      n.left = index + 1;
+      n.right = index + 1 + shape.at(mid - begin).first;
+      nodes[index] = n;
+      if (spawn > 0 && end - begin > (1 << 15)) {
+        std::thread t([&, n, mid, end, spawn] { build(n.right, mid, end, spawn - 1); });
+        build(n.left, begin, mid, spawn - 1);
+        t.join();
+      } else {
+        build(n.left, begin, mid, 0);
+        build(n.right, mid, end, 0);
+      }
+      return;
     }
-    return index;
+    nodes[index] = n;
   }
 };
+
+// fn(first, last) over [0, n) in contiguous chunks on up to 16 threads.
+template <class F> void parallel_chunks(uint64_t n, F fn) {
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const uint64_t parts = std::min<uint64_t>(hw, (n + 65535) / 65536);
+  if (parts <= 1) {
+    fn(uint64_t{0}, n);
+    return;
+  }
+  std::vector<std::thread> ts;
+  for (uint64_t t = 1; t < parts; ++t)
+    ts.emplace_back([=] { fn(n * t / parts, n * (t + 1) / parts); });
+  fn(uint64_t{0}, n / parts);
+  for (auto &t : ts)
+    t.join();
+}
 
 // -------------------------------------------------------- device geometry
 // std::max({a, b, c}) and std::clamp as the reference evaluates them.
@@ -144,37 +190,51 @@ __device__ inline double seg_d2(V3 p, V3 a, V3 b) {
 // Squared distance from p to triangle abc by the Voronoi regions of the
 // vertices, edges and face (Ericson, Real-Time Collision Detection, 2005,
 // 5.1.5), with the reference's degenerate fallback and operation order
-// (src/core/math.cpp:72-112).
-__device__ inline double tri_d2(V3 p, V3 a, V3 b, V3 c) {
+// (src/core/math.cpp:72-112).  ops += the FP64 additions, subtractions,
+// multiplications and divisions of the path taken (23 to 96; 116 for a
+// degenerate triangle).
+__device__ inline double tri_d2(V3 p, V3 a, V3 b, V3 c, uint32_t &ops) {
   V3 ab = b - a, ac = c - a, ap = p - a;
-  if (dot(cross(ab, ac), cross(ab, ac)) < 1e-14 * 1e-14)
+  if (dot(cross(ab, ac), cross(ab, ac)) < 1e-14 * 1e-14) {
+    ops += 116;
     return min2(min2(seg_d2(p, a, b), seg_d2(p, a, c)), seg_d2(p, b, c));
+  }
   double d1 = dot(ab, ap), d2 = dot(ac, ap);
-  if (d1 <= 0 && d2 <= 0)
+  if (d1 <= 0 && d2 <= 0) {
+    ops += 38;
     return dot(ap, ap);
+  }
   V3 bp = p - b;
   double d3 = dot(ab, bp), d4 = dot(ac, bp);
-  if (d3 >= 0 && d4 <= d3)
+  if (d3 >= 0 && d4 <= d3) {
+    ops += 51;
     return dot(bp, bp);
+  }
   double vc = d1 * d4 - d3 * d2;
   if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+    ops += 65;
     V3 e = p - (a + ab * (d1 / (d1 - d3)));
     return dot(e, e);
   }
   V3 cp = p - c;
   double d5 = dot(ab, cp), d6 = dot(ac, cp);
-  if (d6 >= 0 && d5 <= d6)
+  if (d6 >= 0 && d5 <= d6) {
+    ops += 67;
     return dot(cp, cp);
+  }
   double vb = d5 * d2 - d1 * d6;
   if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+    ops += 81;
     V3 e = p - (a + ac * (d2 / (d2 - d6)));
     return dot(e, e);
   }
   double va = d3 * d6 - d5 * d4;
   if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    ops += 89;
     V3 e = p - (b + (c - b) * ((d4 - d3) / ((d4 - d3) + (d5 - d6))));
     return dot(e, e);
   }
+  ops += 96;
   V3 e = p - (a + ab * (vb / (va + vb + vc)) + ac * (vc / (va + vb + vc)));
   double r = dot(e, e);
   return 0.0 < r ? r : 0.0;
@@ -188,19 +248,18 @@ __device__ inline V3 face_corner(const double *fr, int k) {
 // Row r = (2c + s) * nv + i.  accepted[r] in {0, 1}; label[r] = i when
 // accepted, else -1 (the union-find parent array); dist2[r] = the nearest
 // squared distance when the row reached the surface search, else NaN.
-__global__ void __launch_bounds__(kVThreads)
-k_verify(const double *__restrict__ vx, const double *__restrict__ vy,
-         const double *__restrict__ vz, const double *__restrict__ nx,
-         const double *__restrict__ ny, const double *__restrict__ nz, uint32_t nv,
-         uint64_t rows, const SNode *__restrict__ nodes, const double *__restrict__ faces,
-         const Motion *__restrict__ motions, double threshold,
-         uint32_t *__restrict__ accepted, int32_t *__restrict__ label,
-         double *__restrict__ dist2) {
-  extern __shared__ int stack_lds[];
-  int *stack = stack_lds + threadIdx.x;
-  const uint64_t row = static_cast<uint64_t>(blockIdx.x) * kVThreads + threadIdx.x;
-  if (row >= rows)
-    return;
+// One row: transform, early tests, tree search, normal test; tests, visits
+// and ops count the triangle tests, the tree nodes taken off the stack and
+// the FP64 operations of the search (box and triangle distances).
+__device__ inline void verify_row(const double *__restrict__ vx, const double *__restrict__ vy,
+                                  const double *__restrict__ vz, const double *__restrict__ nx,
+                                  const double *__restrict__ ny, const double *__restrict__ nz,
+                                  uint32_t nv, uint64_t row, const SNode *__restrict__ nodes,
+                                  const double *__restrict__ faces,
+                                  const Motion *__restrict__ motions, double threshold,
+                                  int *stack, uint32_t *__restrict__ accepted,
+                                  int32_t *__restrict__ label, double *__restrict__ dist2,
+                                  uint32_t &tests, uint32_t &visits, uint32_t &ops) {
   const uint64_t seg = row / nv;
   const uint32_t i = static_cast<uint32_t>(row - seg * nv);
   const Motion &mo = motions[seg >> 1];
@@ -229,12 +288,16 @@ k_verify(const double *__restrict__ vx, const double *__restrict__ vy,
     stack[0] = 0;
     while (top > 0) {
       const SNode nd = nodes[stack[--top * kVThreads]];
+      ++visits;
+      ops += 12;  // box distance: 2 subtractions, a product and a sum per axis
       if (box_bound(q, nd) > best)
         continue;
       if (nd.left < 0) {
         for (int k = nd.begin; k < nd.end; ++k) {
           const double *fr = faces + static_cast<uint64_t>(k) * kFaceWords;
-          const double d = tri_d2(q, face_corner(fr, 0), face_corner(fr, 1), face_corner(fr, 2));
+          const double d =
+              tri_d2(q, face_corner(fr, 0), face_corner(fr, 1), face_corner(fr, 2), ops);
+          ++tests;
           if (d < best) {
             best = d;
             hit = k;
@@ -242,6 +305,7 @@ k_verify(const double *__restrict__ vx, const double *__restrict__ vy,
         }
       } else {
         int first = nd.left, second = nd.right;
+        ops += 24;
         if (box_bound(q, nodes[first]) > box_bound(q, nodes[second])) {
           first = nd.right;
           second = nd.left;
@@ -264,49 +328,106 @@ k_verify(const double *__restrict__ vx, const double *__restrict__ vy,
   dist2[row] = best;
 }
 
+__global__ void __launch_bounds__(kVThreads)
+k_verify(const double *__restrict__ vx, const double *__restrict__ vy,
+         const double *__restrict__ vz, const double *__restrict__ nx,
+         const double *__restrict__ ny, const double *__restrict__ nz, uint32_t nv,
+         uint64_t rows, const SNode *__restrict__ nodes, const double *__restrict__ faces,
+         const Motion *__restrict__ motions, double threshold,
+         uint32_t *__restrict__ accepted, int32_t *__restrict__ label,
+         double *__restrict__ dist2, unsigned long long *__restrict__ stats) {
+  extern __shared__ int stack_lds[];
+  int *stack = stack_lds + threadIdx.x;
+  const uint64_t row = static_cast<uint64_t>(blockIdx.x) * kVThreads + threadIdx.x;
+  uint32_t tests = 0, visits = 0, ops = 0;
+  if (row < rows)
+    verify_row(vx, vy, vz, nx, ny, nz, nv, row, nodes, faces, motions, threshold, stack,
+               accepted, label, dist2, tests, visits, ops);
+  // work counters: one atomic per wave
+  const uint32_t wt = wave_sum(tests), wv = wave_sum(visits);
+  const unsigned long long wo = wave_sum(static_cast<unsigned long long>(ops));
+  if (lane_id() == 0) {
+    atomicAdd(stats, static_cast<unsigned long long>(wt));
+    atomicAdd(stats + 1, static_cast<unsigned long long>(wv));
+    atomicAdd(stats + 2, wo);
+  }
+}
+
 __device__ inline int32_t load_parent(const int32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Root of x: parents only point to smaller vertices, so the walk ends.
-__device__ inline int32_t cc_find(const int32_t *par, int32_t x) {
-  int32_t p = load_parent(par + x);
-  while (p != x) {
-    x = p;
-    p = load_parent(par + x);
+__device__ inline void store_parent(int32_t *p, int32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Root of x.  Every parent is smaller than its child (a root is its own
+// parent), so the walk ends; on the way each visited vertex is pointed at
+// its grandparent, still an ancestor (intermediate pointer jumping).
+__device__ inline int32_t cc_find(int32_t *par, int32_t x) {
+  int32_t cur = load_parent(par + x);
+  if (cur == x)
+    return x;
+  int32_t prev = x, next;
+  while (cur > (next = load_parent(par + cur))) {
+    store_parent(par + prev, next);
+    prev = cur;
+    cur = next;
   }
-  return x;
+  return cur;
 }
 
-// Union-find over the accepted vertices of each (cluster, side) segment:
-// every ring edge (u, v), u < v, with both ends accepted joins their trees by
-// hooking the larger root under the smaller (compare-and-swap on the root), so
-// every parent is smaller than its child and each root is the smallest
-// member of its component (Jaiganesh and Burtscher, "A high-performance
-// connected components implementation for GPUs", HPDC 2018).
+// Union-find over the accepted vertices of each (cluster, side) segment,
+// after Jaiganesh and Burtscher, "A high-performance connected components
+// implementation for GPUs" (HPDC 2018): every parent is smaller than its
+// child, so each root is the smallest member of its component.
+// k_cc_init: an accepted vertex starts under its smallest accepted
+// neighbour below it (itself when none).
+__global__ void __launch_bounds__(kVThreads)
+k_cc_init(const uint64_t *__restrict__ nbr_off, const int32_t *__restrict__ nbr_idx,
+          const uint32_t *__restrict__ accepted, uint32_t nv, uint64_t rows, int32_t *label) {
+  const uint64_t row = static_cast<uint64_t>(blockIdx.x) * kVThreads + threadIdx.x;
+  if (row >= rows || !accepted[row])
+    return;
+  const uint64_t seg = row / nv;
+  const int32_t u = static_cast<int32_t>(row - seg * nv);
+  const uint32_t *acc = accepted + seg * nv;
+  int32_t best = u;
+  for (uint64_t j = nbr_off[u]; j < nbr_off[u + 1]; ++j) {
+    const int32_t v = nbr_idx[j];
+    if (v < best && v >= 0 && acc[v])
+      best = v;
+  }
+  label[row] = best;
+}
+
+// k_cc_hook: every ring edge (u, v), u < v, with both ends accepted joins
+// their trees by hooking the larger root under the smaller with a
+// compare-and-swap; a failed swap moves on to the root's new parent.
 __global__ void __launch_bounds__(kVThreads)
 k_cc_hook(const uint64_t *__restrict__ nbr_off, const int32_t *__restrict__ nbr_idx,
-          uint32_t nv, uint64_t rows, int32_t *label) {
+          const uint32_t *__restrict__ accepted, uint32_t nv, uint64_t rows, int32_t *label) {
   const uint64_t row = static_cast<uint64_t>(blockIdx.x) * kVThreads + threadIdx.x;
-  if (row >= rows)
+  if (row >= rows || !accepted[row])
     return;
   const uint64_t seg = row / nv;
   const int32_t u = static_cast<int32_t>(row - seg * nv);
   int32_t *par = label + seg * nv;
-  if (load_parent(par + u) < 0)
-    return;
+  const uint32_t *acc = accepted + seg * nv;
   for (uint64_t j = nbr_off[u]; j < nbr_off[u + 1]; ++j) {
     const int32_t v = nbr_idx[j];
-    if (v <= u || load_parent(par + v) < 0)
+    if (v <= u || static_cast<uint32_t>(v) >= nv || !acc[v])
       continue;
-    while (true) {
-      const int32_t a = cc_find(par, u), b = cc_find(par, v);
-      if (a == b)
-        break;
+    int32_t a = cc_find(par, u), b = cc_find(par, v);
+    while (a != b) {
       const int32_t hi = a > b ? a : b, lo = a > b ? b : a;
       int32_t expect = hi;
       if (__hip_atomic_compare_exchange_strong(par + hi, &expect, lo, __ATOMIC_RELAXED,
                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
         break;
+      // hi was no longer a root: continue from its parent (smaller than hi)
+      if (a == hi)
+        a = expect;
+      else
+        b = expect;
     }
   }
 }
@@ -322,8 +443,7 @@ k_cc_flatten(uint32_t nv, uint64_t rows, int32_t *label) {
   int32_t *par = label + seg * nv;
   if (load_parent(par + u) < 0)
     return;
-  const int32_t root = cc_find(par, u);
-  __hip_atomic_store(par + u, root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  store_parent(par + u, cc_find(par, u));
 }
 
 // Accepted rows in row order: vertex, patch root and squared distance.
@@ -355,73 +475,128 @@ __global__ void k_bounds(const uint64_t *__restrict__ pos, uint32_t nv, uint32_t
 // ------------------------------------------------------------- host side
 namespace {
 
-struct VerifyInput {
-  const mvs_points *vertices;
-  uint64_t nf;
-  const int32_t *f[3];
-  const mvs_points *normals;
-  const double *weights;
-  const uint64_t *nbr_off;
-  const int32_t *nbr_idx;
+struct Clusters {
   uint64_t nc;
   const uint8_t *reflection;
   const double *rotation9, *translation3, *plane_normal3, *plane_offset;
-  double verify_threshold, box;
 };
 
 // Groups one segment's compact rows (ascending vertices with their patch
 // roots) into patches: a root is its patch's smallest vertex, so patches
 // appear in the order of their roots, which is the reference's order of
 // breadth-first seeds (verification.cpp:27-47); members stay ascending.
-void group_patches(const int32_t *vid, const int32_t *root, uint64_t n,
-                   std::vector<int32_t> &slot_of, std::vector<uint64_t> &offsets,
-                   std::vector<int32_t> &members) {
-  std::vector<uint64_t> count;
+// Writes the n members to `members` and appends the patch sizes to `sizes`;
+// slot_of is scratch indexed by vertex (read only where written first).
+void group_segment(const int32_t *vid, const int32_t *root, uint64_t n, int32_t *slot_of,
+                   std::vector<uint64_t> &sizes, int32_t *members) {
+  sizes.clear();
   for (uint64_t k = 0; k < n; ++k) {
     if (root[k] == vid[k]) {
-      slot_of[vid[k]] = static_cast<int32_t>(count.size());
-      count.push_back(0);
+      slot_of[vid[k]] = static_cast<int32_t>(sizes.size());
+      sizes.push_back(0);
     }
-    ++count[static_cast<size_t>(slot_of[root[k]])];
+    ++sizes[static_cast<size_t>(slot_of[root[k]])];
   }
-  const uint64_t base = members.size();
-  std::vector<uint64_t> next(count.size());
-  uint64_t run = base;
-  for (size_t g = 0; g < count.size(); ++g) {
+  std::vector<uint64_t> next(sizes.size());
+  uint64_t run = 0;
+  for (size_t g = 0; g < sizes.size(); ++g) {
     next[g] = run;
-    run += count[g];
-    offsets.push_back(run);
+    run += sizes[g];
   }
-  members.resize(run);
   for (uint64_t k = 0; k < n; ++k)
     members[next[static_cast<size_t>(slot_of[root[k]])]++] = vid[k];
 }
 
-void verification(Session *s, const VerifyInput &in, mvs_verification_counts *counts) {
-  const uint64_t nv = in.vertices->count, nf = in.nf, nc = in.nc;
+// The surface (tree and leaf-ordered faces) of a mesh, kept on the session.
+void surface_build(Session *s, const mvs_points *vertices, uint64_t nf, const int32_t *const f[3]) {
+  const uint64_t nv = vertices->count;
   if (nv > static_cast<uint64_t>(INT32_MAX) || nf > static_cast<uint64_t>(INT32_MAX))
     raise(MVS_EOVERFLOW, "verification: vertex and face ids must fit int32");
-  if (in.normals->count != nv)
-    raise(MVS_EINVAL, "verification: normals and vertices differ in size");
-  if (nc > 0 && 2 * nc > UINT32_MAX / 2)
-    raise(MVS_EOVERFLOW, "verification: too many clusters");
-  for (uint64_t f = 0; f < nf; ++f)
+  if (nf && (!vertices->x || !vertices->y || !vertices->z))
+    raise(MVS_EINVAL, "verification: null vertex array");
+  for (uint64_t i = 0; i < nf; ++i)
     for (int c = 0; c < 3; ++c)
-      if (in.f[c][f] < 0 || static_cast<uint64_t>(in.f[c][f]) >= nv)
+      if (f[c][i] < 0 || static_cast<uint64_t>(f[c][i]) >= nv)
         raise(MVS_EINVAL, "verification: face vertex index out of range");
-  if (in.nbr_off[0] != 0)
-    raise(MVS_EINVAL, "verification: neighbor offsets must start at 0");
-  for (uint64_t i = 0; i < nv; ++i)
-    if (in.nbr_off[i + 1] < in.nbr_off[i])
-      raise(MVS_EINVAL, "verification: neighbor offsets must not decrease");
-  const uint64_t nnz = in.nbr_off[nv];
-  for (uint64_t j = 0; j < nnz; ++j)
-    if (in.nbr_idx[j] < 0 || static_cast<uint64_t>(in.nbr_idx[j]) >= nv)
-      raise(MVS_EINVAL, "verification: neighbor index out of range");
+  s->ver_surface = false;
+  TreeBuilder tb;
+  tb.x = vertices->x, tb.y = vertices->y, tb.z = vertices->z;
+  for (int c = 0; c < 3; ++c)
+    tb.f[c] = f[c];
+  for (int k = 0; k < 3; ++k)
+    tb.cen[k].resize(nf);
+  tb.fbox.resize(6 * nf);
+  parallel_chunks(nf, [&](uint64_t b, uint64_t e) {
+    for (uint64_t i = b; i < e; ++i) {
+      const int v0 = f[0][i], v1 = f[1][i], v2 = f[2][i];
+      for (int k = 0; k < 3; ++k) {
+        const double a = tb.coord(v0, k), bb = tb.coord(v1, k), c = tb.coord(v2, k);
+        // the reference's centroid, (a + b + c) / 3 (surface.h:36-40)
+        tb.cen[k][i] = (a + bb + c) / 3;
+        tb.fbox[6 * i + k] = std::min(std::min(a, bb), c);
+        tb.fbox[6 * i + 3 + k] = std::max(std::max(a, bb), c);
+      }
+    }
+  });
+  tb.ids.resize(nf);
+  std::iota(tb.ids.begin(), tb.ids.end(), 0);
+  const auto shape = tb.shape_of(static_cast<int>(nf));
+  tb.nodes.resize(static_cast<size_t>(shape.first));
+  tb.build(0, 0, static_cast<int>(nf), 4);
+  // leaf-ordered faces with their unit normals (verification.cpp:71-73)
+  std::vector<double> fr(std::max<uint64_t>(nf, 1) * kFaceWords, 0.0);
+  parallel_chunks(nf, [&](uint64_t b, uint64_t e) {
+    for (uint64_t k = b; k < e; ++k) {
+      const int face = tb.ids[k];
+      V3 c[3];
+      for (int j = 0; j < 3; ++j) {
+        const int v = f[j][face];
+        c[j] = v3(tb.x[v], tb.y[v], tb.z[v]);
+      }
+      const V3 n = unit(cross(c[1] - c[0], c[2] - c[0]));
+      double *o = fr.data() + k * kFaceWords;
+      for (int j = 0; j < 3; ++j)
+        o[3 * j] = c[j].x, o[3 * j + 1] = c[j].y, o[3 * j + 2] = c[j].z;
+      o[9] = n.x, o[10] = n.y, o[11] = n.z;
+    }
+  });
+  s->ver_stack = shape.second + 1;
+  if (static_cast<size_t>(s->ver_stack) * kVThreads * sizeof(int) > 64 * 1024)
+    raise(MVS_EOVERFLOW, "verification: surface tree too deep");
+  // queued work may still read the previous surface
+  sync(s);
+  s->ver_nodes.alloc(tb.nodes.size() * sizeof(SNode));
+  s->ver_faces.alloc(fr.size() * sizeof(double));
+  copy_h2d(s, s->ver_nodes.ptr, tb.nodes.data(), tb.nodes.size() * sizeof(SNode));
+  copy_h2d(s, s->ver_faces.ptr, fr.data(), fr.size() * sizeof(double));
+  sync(s);  // the host vectors go out of scope
+  s->ver_surface_nv = nv;
+  s->ver_surface = true;
+}
 
-  const double threshold = in.verify_threshold * in.box;
+// Rows, patches and compaction on the device; sides, patches, rms and
+// coverage assembled on the host in the reference's order
+// (verification.cpp:57-107).  pts: 6 nv device doubles (x, y, z, normal
+// x, y, z); neighbours on the device; weights on the host.
+void run(Session *s, uint64_t nv, const double *pts, const uint64_t *noff, const int32_t *nidx,
+         const double *weights, const Clusters &cl, double verify_threshold, double box,
+         mvs_verification_counts *counts) {
+  if (!s->ver_surface)
+    raise(MVS_EINVAL, "verification: no surface (mvs_verification_surface)");
+  if (nv != s->ver_surface_nv)
+    raise(MVS_EINVAL, "verification: vertex count differs from the surface's");
+  if (nv > static_cast<uint64_t>(INT32_MAX))
+    raise(MVS_EOVERFLOW, "verification: vertex ids must fit int32");
+  const uint64_t nc = cl.nc;
+  if (nc > UINT32_MAX / 4)
+    raise(MVS_EOVERFLOW, "verification: too many clusters");
+  if (nv && !weights)
+    raise(MVS_EINVAL, "verification: null weights");
+  if (nc && (!cl.reflection || !cl.rotation9 || !cl.translation3 || !cl.plane_normal3 ||
+             !cl.plane_offset))
+    raise(MVS_EINVAL, "verification: null cluster array");
+  const double threshold = verify_threshold * box;
   const uint64_t segs = 2 * nc, rows = segs * nv;
-  // results of this call (fetch reads them)
   for (int sd = 0; sd < 2; ++sd) {
     s->ver_side_off[sd].assign(1, 0);
     s->ver_side[sd].clear();
@@ -432,88 +607,46 @@ void verification(Session *s, const VerifyInput &in, mvs_verification_counts *co
   s->ver_cov.assign(nc, 0.0);
   s->ver_rms.assign(nc, 0.0);
   s->ver_rows = 0;
-  s->ver_nv = nv;
 
   std::vector<uint64_t> bnd(segs + 1, 0);
-  std::vector<int32_t> vid, root;
-  std::vector<double> d2;
+  const int32_t *h_vid = nullptr, *h_root = nullptr;
+  const double *h_d2 = nullptr;
+  const auto t_start = std::chrono::steady_clock::now();
+  auto t_dev = t_start;
   if (rows > 0) {
-    // tree (surface.h:89-94); an empty surface is a single empty leaf
-    TreeBuilder tb;
-    tb.x = in.vertices->x, tb.y = in.vertices->y, tb.z = in.vertices->z;
-    for (int c = 0; c < 3; ++c)
-      tb.f[c] = in.f[c];
-    for (int k = 0; k < 3; ++k) {
-      tb.cen[k].resize(nf);
-      for (uint64_t f = 0; f < nf; ++f)
-        tb.cen[k][f] = (tb.coord(in.f[0][f], k) + tb.coord(in.f[1][f], k) +
-                        tb.coord(in.f[2][f], k)) / 3;
-    }
-    tb.ids.resize(nf);
-    std::iota(tb.ids.begin(), tb.ids.end(), 0);
-    tb.nodes.reserve(nf / 4 + 1);
-    tb.build(0, static_cast<int>(nf), 0);
-    // leaf-ordered faces with their unit normals (verification.cpp:71-73)
-    std::vector<double> fr(std::max<uint64_t>(nf, 1) * kFaceWords, 0.0);
-    for (uint64_t k = 0; k < nf; ++k) {
-      const int f = tb.ids[k];
-      V3 c[3];
-      for (int j = 0; j < 3; ++j) {
-        const int v = in.f[j][f];
-        c[j] = v3(tb.x[v], tb.y[v], tb.z[v]);
-      }
-      const V3 n = unit(cross(c[1] - c[0], c[2] - c[0]));
-      double *o = fr.data() + k * kFaceWords;
-      for (int j = 0; j < 3; ++j)
-        o[3 * j] = c[j].x, o[3 * j + 1] = c[j].y, o[3 * j + 2] = c[j].z;
-      o[9] = n.x, o[10] = n.y, o[11] = n.z;
-    }
     std::vector<Motion> mo(nc);
     for (uint64_t c = 0; c < nc; ++c) {
       std::memset(&mo[c], 0, sizeof(Motion));
       for (int k = 0; k < 9; ++k)
-        mo[c].r[k] = in.rotation9[9 * c + k];
+        mo[c].r[k] = cl.rotation9[9 * c + k];
       for (int k = 0; k < 3; ++k) {
-        mo[c].t[k] = in.translation3[3 * c + k];
-        mo[c].n[k] = in.plane_normal3[3 * c + k];
+        mo[c].t[k] = cl.translation3[3 * c + k];
+        mo[c].n[k] = cl.plane_normal3[3 * c + k];
       }
-      mo[c].off = in.plane_offset[c];
-      mo[c].reflection = in.reflection[c] ? 1 : 0;
+      mo[c].off = cl.plane_offset[c];
+      mo[c].reflection = cl.reflection[c] ? 1 : 0;
     }
-    const int stack_size = tb.depth + 1;
-    const size_t lds = static_cast<size_t>(stack_size) * kVThreads * sizeof(int);
-    if (lds > 64 * 1024)
-      raise(MVS_EOVERFLOW, "verification: surface tree too deep");
-
-    const size_t node_bytes = tb.nodes.size() * sizeof(SNode);
-    SNode *d_nodes = static_cast<SNode *>(scratch(s, kSlotVerNodes, node_bytes));
-    double *d_faces = scratch_as<double>(s, kSlotVerFaces, fr.size());
-    double *d_pts = scratch_as<double>(s, kSlotVerPoints, 6 * nv);
+    const size_t lds = static_cast<size_t>(s->ver_stack) * kVThreads * sizeof(int);
     Motion *d_mo = static_cast<Motion *>(scratch(s, kSlotVerMotions, nc * sizeof(Motion)));
-    uint64_t *d_noff = scratch_as<uint64_t>(s, kSlotVerNbrOff, nv + 1);
-    int32_t *d_nidx = scratch_as<int32_t>(s, kSlotVerNbrIdx, std::max<uint64_t>(nnz, 1));
     uint32_t *d_acc = scratch_as<uint32_t>(s, kSlotVerAccepted, rows);
     int32_t *d_label = scratch_as<int32_t>(s, kSlotVerLabel, rows);
     double *d_dist = scratch_as<double>(s, kSlotVerDist, rows);
     uint64_t *d_pos = scratch_as<uint64_t>(s, kSlotVerPos, rows + 1);
     uint64_t *d_bnd = scratch_as<uint64_t>(s, kSlotVerBounds, segs + 1);
-
-    copy_h2d(s, d_nodes, tb.nodes.data(), node_bytes);
-    copy_h2d(s, d_faces, fr.data(), fr.size() * sizeof(double));
-    const double *src[6] = {in.vertices->x, in.vertices->y, in.vertices->z,
-                            in.normals->x,  in.normals->y,  in.normals->z};
-    for (int k = 0; k < 6; ++k)
-      copy_h2d(s, d_pts + k * nv, src[k], nv * sizeof(double));
-    copy_h2d(s, d_mo, mo.data(), nc * sizeof(Motion));
-    copy_h2d(s, d_noff, in.nbr_off, (nv + 1) * sizeof(uint64_t));
-    copy_h2d(s, d_nidx, in.nbr_idx, nnz * sizeof(int32_t));
+    auto *d_stats = scratch_as<unsigned long long>(s, kSlotVerStats, 3);
+    copy_h2d(s, d_mo, mo.data(), nc * sizeof(Motion));  // mo outlives the sync below
+    zero_device(s, d_stats, 3 * sizeof(unsigned long long));
 
     const unsigned grid = blocks_for(rows, kVThreads);
     const uint32_t nv32 = static_cast<uint32_t>(nv);
-    MVS_LAUNCH(s, "verify_rows", k_verify, grid, kVThreads, lds, d_pts, d_pts + nv,
-               d_pts + 2 * nv, d_pts + 3 * nv, d_pts + 4 * nv, d_pts + 5 * nv, nv32, rows,
-               d_nodes, d_faces, d_mo, threshold, d_acc, d_label, d_dist);
-    MVS_LAUNCH(s, "verify_cc_hook", k_cc_hook, grid, kVThreads, 0, d_noff, d_nidx, nv32, rows,
+    MVS_LAUNCH(s, "verify_rows", k_verify, grid, kVThreads, lds, pts, pts + nv, pts + 2 * nv,
+               pts + 3 * nv, pts + 4 * nv, pts + 5 * nv, nv32, rows,
+               reinterpret_cast<const SNode *>(s->ver_nodes.ptr),
+               reinterpret_cast<const double *>(s->ver_faces.ptr), d_mo, threshold, d_acc,
+               d_label, d_dist, d_stats);
+    MVS_LAUNCH(s, "verify_cc_init", k_cc_init, grid, kVThreads, 0, noff, nidx, d_acc, nv32, rows,
+               d_label);
+    MVS_LAUNCH(s, "verify_cc_hook", k_cc_hook, grid, kVThreads, 0, noff, nidx, d_acc, nv32, rows,
                d_label);
     MVS_LAUNCH(s, "verify_cc_flatten", k_cc_flatten, grid, kVThreads, 0, nv32, rows, d_label);
     exclusive_scan_u32(s, d_acc, d_pos, rows);
@@ -526,40 +659,87 @@ void verification(Session *s, const VerifyInput &in, mvs_verification_counts *co
     double *d_d2 = scratch_as<double>(s, kSlotVerD2, std::max<uint64_t>(total, 1));
     MVS_LAUNCH(s, "verify_compact", k_compact, grid, kVThreads, 0, d_acc, d_pos, d_label,
                d_dist, nv32, rows, d_vid, d_root, d_d2);
-    vid.resize(total);
-    root.resize(total);
-    d2.resize(total);
-    copy_d2h(s, vid.data(), d_vid, total * sizeof(int32_t));
-    copy_d2h(s, root.data(), d_root, total * sizeof(int32_t));
-    copy_d2h(s, d2.data(), d_d2, total * sizeof(double));
+    // the compact rows through pinned staging: ids, roots, squared distances
+    const uint64_t tb = std::max<uint64_t>(total, 1);
+    auto *st = static_cast<unsigned char *>(host_stage(s, 16 * tb));
+    int32_t *sv = reinterpret_cast<int32_t *>(st), *sr = sv + tb;
+    double *sd2 = reinterpret_cast<double *>(st + 8 * tb);
+    t_dev = std::chrono::steady_clock::now();
+    copy_d2h(s, sv, d_vid, total * sizeof(int32_t));
+    copy_d2h(s, sr, d_root, total * sizeof(int32_t));
+    copy_d2h(s, sd2, d_d2, total * sizeof(double));
+    h_vid = sv, h_root = sr, h_d2 = sd2;
     s->ver_rows = rows;
   }
+  const auto t_copy = std::chrono::steady_clock::now();
 
-  // assemble in the reference's order (verification.cpp:57-107)
   double total_area = 0;
   for (uint64_t i = 0; i < nv; ++i)
-    total_area += in.weights[i];
-  std::vector<int32_t> slot_of(std::max<uint64_t>(nv, 1), -1);
-  for (uint64_t c = 0; c < nc; ++c) {
-    double error = 0, area = 0;
-    for (int sd = 0; sd < 2; ++sd) {
-      const uint64_t b = bnd[2 * c + sd], e = bnd[2 * c + sd + 1];
-      auto &side = s->ver_side[sd];
-      side.insert(side.end(), vid.begin() + b, vid.begin() + e);
-      s->ver_side_off[sd].push_back(side.size());
-      for (uint64_t k = b; k < e; ++k) {
-        if (sd == 0)
-          error += d2[k];
-        area += in.weights[vid[k]];
-      }
-      group_patches(vid.data() + b, root.data() + b, e - b, slot_of, s->ver_patch_off[sd],
-                    s->ver_members[sd]);
-      s->ver_groups[sd].push_back(s->ver_patch_off[sd].size() - 1);
-    }
-    const uint64_t na = bnd[2 * c + 1] - bnd[2 * c];
-    s->ver_rms[c] = na == 0 ? 0 : std::sqrt(error / static_cast<double>(na));
-    s->ver_cov[c] = total_area > 0 ? std::min(1.0, area / total_area) : 0;
+    total_area += weights[i];
+  for (int sd = 0; sd < 2; ++sd) {
+    auto &off = s->ver_side_off[sd];
+    off.assign(nc + 1, 0);
+    for (uint64_t c = 0; c < nc; ++c)
+      off[c + 1] = off[c] + (bnd[2 * c + sd + 1] - bnd[2 * c + sd]);
+    s->ver_side[sd].resize(off[nc]);
+    s->ver_members[sd].resize(off[nc]);
   }
+  // clusters in parallel; within a cluster the reference's order: the error
+  // over side A, the area over side A then side B (verification.cpp:57-96)
+  std::vector<std::vector<uint64_t>> sizes(2 * nc);
+  const unsigned workers = static_cast<unsigned>(std::max<uint64_t>(
+      1, std::min<uint64_t>(nc, std::min(16u, std::max(1u, std::thread::hardware_concurrency())))));
+  if (s->ver_slot.size() < workers)
+    s->ver_slot.resize(workers);
+  std::atomic<uint64_t> next{0};
+  auto work = [&](unsigned w) {
+    auto &slot = s->ver_slot[w];
+    if (slot.size() < nv)
+      slot.resize(nv);
+    for (uint64_t c; (c = next.fetch_add(1)) < nc;) {
+      double error = 0, area = 0;
+      for (int sd = 0; sd < 2; ++sd) {
+        const uint64_t b = bnd[2 * c + sd], e = bnd[2 * c + sd + 1];
+        const uint64_t o = s->ver_side_off[sd][c];
+        if (e > b)
+          std::memcpy(s->ver_side[sd].data() + o, h_vid + b, (e - b) * sizeof(int32_t));
+        for (uint64_t k = b; k < e; ++k) {
+          if (sd == 0)
+            error += h_d2[k];
+          area += weights[h_vid[k]];
+        }
+        group_segment(h_vid + b, h_root + b, e - b, slot.data(), sizes[2 * c + sd],
+                      s->ver_members[sd].data() + o);
+      }
+      const uint64_t na = bnd[2 * c + 1] - bnd[2 * c];
+      s->ver_rms[c] = na == 0 ? 0 : std::sqrt(error / static_cast<double>(na));
+      s->ver_cov[c] = total_area > 0 ? std::min(1.0, area / total_area) : 0;
+    }
+  };
+  {
+    std::vector<std::thread> ts;
+    for (unsigned w = 1; w < workers; ++w)
+      ts.emplace_back(work, w);
+    work(0);
+    for (auto &t : ts)
+      t.join();
+  }
+  for (int sd = 0; sd < 2; ++sd) {
+    auto &po = s->ver_patch_off[sd];
+    auto &gr = s->ver_groups[sd];
+    for (uint64_t c = 0; c < nc; ++c) {
+      for (uint64_t z : sizes[2 * c + sd])
+        po.push_back(po.back() + z);
+      gr.push_back(po.size() - 1);
+    }
+  }
+  const auto t_host = std::chrono::steady_clock::now();
+  auto us = [](auto a, auto b) {
+    return static_cast<uint64_t>(std::chrono::duration<double, std::micro>(b - a).count());
+  };
+  s->ver_us[0] = us(t_start, t_dev);
+  s->ver_us[1] = us(t_dev, t_copy);
+  s->ver_us[2] = us(t_copy, t_host);
   counts->side_a = s->ver_side[0].size();
   counts->side_b = s->ver_side[1].size();
   counts->patches_a = s->ver_patch_off[0].size() - 1;
@@ -587,6 +767,35 @@ using namespace mvs;
 
 extern "C" {
 
+int mvs_verification_surface(mvs_session *sp, const mvs_points *vertices, uint64_t face_count,
+                             const int32_t *f0, const int32_t *f1, const int32_t *f2) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    if (!vertices)
+      raise(MVS_EINVAL, "verification: null vertices");
+    if (face_count && (!f0 || !f1 || !f2))
+      raise(MVS_EINVAL, "verification: null face array");
+    const int32_t *f[3] = {f0, f1, f2};
+    surface_build(s, vertices, face_count, f);
+  });
+}
+
+int mvs_dev_verification(mvs_session *sp, uint64_t nv, const double *pts6,
+                         const uint64_t *neighbor_offsets, const int32_t *neighbor_indices,
+                         const double *weights, uint64_t cluster_count, const uint8_t *reflection,
+                         const double *rotation9, const double *translation3,
+                         const double *plane_normal3, const double *plane_offset,
+                         double verify_threshold, double box, mvs_verification_counts *counts) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    if (!counts || (nv && (!pts6 || !neighbor_offsets || !neighbor_indices)))
+      raise(MVS_EINVAL, "verification: null argument");
+    run(s, nv, pts6, neighbor_offsets, neighbor_indices, weights,
+        Clusters{cluster_count, reflection, rotation9, translation3, plane_normal3, plane_offset},
+        verify_threshold, box, counts);
+  });
+}
+
 int mvs_verification(mvs_session *sp, const mvs_points *vertices, uint64_t face_count,
                      const int32_t *f0, const int32_t *f1, const int32_t *f2,
                      const mvs_points *normals, const double *weights,
@@ -600,20 +809,40 @@ int mvs_verification(mvs_session *sp, const mvs_points *vertices, uint64_t face_
     if (!vertices || !normals || !counts || !neighbor_offsets)
       raise(MVS_EINVAL, "verification: null argument");
     const uint64_t nv = vertices->count;
+    if (normals->count != nv)
+      raise(MVS_EINVAL, "verification: normals and vertices differ in size");
     if (nv && (!vertices->x || !vertices->y || !vertices->z || !normals->x || !normals->y ||
                !normals->z || !weights))
       raise(MVS_EINVAL, "verification: null vertex array");
     if (face_count && (!f0 || !f1 || !f2))
       raise(MVS_EINVAL, "verification: null face array");
-    if (cluster_count && (!reflection || !rotation9 || !translation3 || !plane_normal3 ||
-                          !plane_offset))
-      raise(MVS_EINVAL, "verification: null cluster array");
-    if (nv && neighbor_offsets[nv] && !neighbor_indices)
+    if (neighbor_offsets[0] != 0)
+      raise(MVS_EINVAL, "verification: neighbor offsets must start at 0");
+    for (uint64_t i = 0; i < nv; ++i)
+      if (neighbor_offsets[i + 1] < neighbor_offsets[i])
+        raise(MVS_EINVAL, "verification: neighbor offsets must not decrease");
+    const uint64_t nnz = nv ? neighbor_offsets[nv] : 0;
+    if (nnz && !neighbor_indices)
       raise(MVS_EINVAL, "verification: null neighbor indices");
-    VerifyInput in{vertices, face_count, {f0, f1, f2}, normals, weights, neighbor_offsets,
-                   neighbor_indices, cluster_count, reflection, rotation9, translation3,
-                   plane_normal3, plane_offset, verify_threshold, box};
-    verification(s, in, counts);
+    for (uint64_t j = 0; j < nnz; ++j)
+      if (neighbor_indices[j] < 0 || static_cast<uint64_t>(neighbor_indices[j]) >= nv)
+        raise(MVS_EINVAL, "verification: neighbor index out of range");
+    const int32_t *f[3] = {f0, f1, f2};
+    surface_build(s, vertices, face_count, f);
+    double *d_pts = scratch_as<double>(s, kSlotVerPoints, std::max<uint64_t>(6 * nv, 1));
+    uint64_t *d_noff = scratch_as<uint64_t>(s, kSlotVerNbrOff, nv + 1);
+    int32_t *d_nidx = scratch_as<int32_t>(s, kSlotVerNbrIdx, std::max<uint64_t>(nnz, 1));
+    if (cluster_count) {
+      const double *src[6] = {vertices->x, vertices->y, vertices->z,
+                              normals->x,  normals->y,  normals->z};
+      for (int k = 0; k < 6; ++k)
+        copy_h2d(s, d_pts + k * nv, src[k], nv * sizeof(double));
+      copy_h2d(s, d_noff, neighbor_offsets, (nv + 1) * sizeof(uint64_t));
+      copy_h2d(s, d_nidx, neighbor_indices, nnz * sizeof(int32_t));
+    }
+    run(s, nv, d_pts, d_noff, d_nidx, weights,
+        Clusters{cluster_count, reflection, rotation9, translation3, plane_normal3, plane_offset},
+        verify_threshold, box, counts);
   });
 }
 
@@ -627,6 +856,19 @@ int mvs_verification_fetch(mvs_session *sp, const mvs_verification_side *side_a,
       std::memcpy(coverage, s->ver_cov.data(), s->ver_cov.size() * sizeof(double));
     if (rms && !s->ver_rms.empty())
       std::memcpy(rms, s->ver_rms.data(), s->ver_rms.size() * sizeof(double));
+  });
+}
+
+int mvs_verification_stats(mvs_session *sp, uint64_t out[6]) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    if (!out)
+      raise(MVS_EINVAL, "verification stats: null output");
+    for (int k = 0; k < 6; ++k)
+      out[k] = k < 3 ? 0 : s->ver_us[k - 3];
+    if (!s->ver_rows)
+      return;
+    copy_d2h(s, out, scratch(s, kSlotVerStats, 3 * sizeof(uint64_t)), 3 * sizeof(uint64_t));
   });
 }
 

@@ -135,6 +135,45 @@ def cfg2(device: str = "cuda:0", n: int = 1024) -> Workload:
     return Workload(meshgen.cfg2_mesh(n), 100000, 0.02, False, device=device)
 
 
+def copy_motions(mesh: meshgen.HeightField, count: int = 20) -> dict:
+    """Cluster records for verifying a height field: `count - 4` rigid
+    motions carrying one motif copy onto another of the same handedness
+    (copy i onto copy (37 i + 11) mod n, in copy order, skipping pairs of
+    unlike handedness), then reflections through the mesh centre about the
+    planes x = 0, y = 0, x = y and x = -y.  A copy at centre C_i turned by
+    theta_i maps onto copy j by the turn R = Rz(theta_j - theta_i) about z and
+    the translation C_j - R C_i."""
+    copies = mesh.copies
+    n = len(copies)
+    refl, rot, tr, pn, po = [], [], [], [], []
+    for i in range(n):
+        if len(refl) == count - 4:
+            break
+        j = (37 * i + 11) % n
+        a, b = copies[i], copies[j]
+        if j == i or a.mirror != b.mirror:
+            continue
+        t = b.theta - a.theta
+        c, s_ = np.cos(t), np.sin(t)
+        R = np.array([[c, -s_, 0.0], [s_, c, 0.0], [0.0, 0.0, 1.0]])
+        refl.append(0)
+        rot.append(R.ravel())
+        tr.append(np.array([b.cx, b.cy, 0.0]) - R @ np.array([a.cx, a.cy, 0.0]))
+        pn.append(np.zeros(3))
+        po.append(0.0)
+    for nrm in ([1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.7071067811865476, -0.7071067811865476, 0.0],
+                [0.7071067811865476, 0.7071067811865476, 0.0]):
+        u = np.array(nrm)
+        refl.append(1)
+        rot.append((np.eye(3) - 2 * np.outer(u, u)).ravel())
+        tr.append(np.zeros(3))
+        pn.append(u)
+        po.append(0.0)
+    return dict(reflection=np.array(refl, np.uint8), rotation=np.array(rot),
+                translation=np.array(tr), plane_normal=np.array(pn),
+                plane_offset=np.array(po))
+
+
 class Batch:
     """A batch of height-field meshes (meshgen.batch_spec) resident on one GPU:
     concatenated vertices and features, per-mesh sample offsets, boxes and

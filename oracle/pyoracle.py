@@ -321,8 +321,9 @@ class Oracle:
         fc = [np.ascontiguousarray(f[:, k]) for k in range(3)]
         nc = len(clusters["reflection"])
         cl = [np.ascontiguousarray(clusters["reflection"], dtype=np.uint8)] + [
-            np.ascontiguousarray(clusters[k], dtype=np.float64).reshape(nc, -1)
-            for k in ("rotation", "translation", "plane_normal", "plane_offset")]
+            np.ascontiguousarray(clusters[k], dtype=np.float64).reshape(nc, w)
+            for k, w in (("rotation", 9), ("translation", 3), ("plane_normal", 3),
+                         ("plane_offset", 1))]
         return len(v), len(f), nc, cols, fc, [n[0], n[1], n[2]], cl
 
     def verification_rows(self, vertices, faces, normals, clusters, verify_threshold, box):

@@ -168,3 +168,29 @@ def test_invalid_inputs_raise(sess, ref):
     with pytest.raises(capi.MvsError):
         sess.verification(v, bad, topo_dict(topo), example_clusters("reflection"), .01,
                           topo.box_size)
+
+
+def test_device_layer_matches_host_layer(sess, ref):
+    # surface once, then two cluster sets against device-resident inputs
+    import torch
+    m = meshgen.height_field(129, 2.0, [meshgen.Copy(.5, .5, 0.0, False),
+                                        meshgen.Copy(-.5, -.5, math.pi, False)],
+                             meshgen.Motif.make(0), 1e-4, 2)
+    topo = ref.prepare(m.vertices, m.faces)
+    g = np.random.default_rng(13)
+    sets = [cfg1_clusters(g), noisy_clusters(g, 3)]
+    want = [sess.verification(m.vertices, topo.faces, topo_dict(topo), cl, .025, topo.box_size)
+            for cl in sets]
+    nv = len(m.vertices)
+    pts = torch.from_numpy(np.ascontiguousarray(
+        np.concatenate([m.vertices.T, topo.normals]))).cuda()
+    off = torch.from_numpy(topo.nbr_off.astype(np.int64)).cuda()
+    idx = torch.from_numpy(topo.nbr_idx).cuda()
+    sess.verification_surface(m.vertices, topo.faces)
+    torch.cuda.synchronize()
+    for cl, w in zip(sets, want):
+        print("dev_verification clusters", len(cl["reflection"]), flush=True)
+        got = sess.dev_verification(nv, pts, off, idx, topo.weights, cl, .025, topo.box_size)
+        same(got, w)
+    with pytest.raises(capi.MvsError):
+        sess.dev_verification(nv - 1, pts, off, idx, topo.weights, sets[0], .025, topo.box_size)
