@@ -280,6 +280,9 @@ public:
       std::vector<Index> members(mem.begin() + off[i], mem.begin() + off[i + 1]);
       out.clusters.append(rec, members);
     }
+    // unverified clusters carry empty sides and zero coverage / rms, as the
+    // CPU backend leaves them (src/cpu/clustering.cpp:152)
+    out.clusters.verification = Verification::empty(out.clusters.size());
     c.h2d = (1 + 21 * 8) * V;
     c.d2h = 4 * nm;
     return out;

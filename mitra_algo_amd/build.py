@@ -1,7 +1,9 @@
 """Build libmvs.so (gfx950) in-tree with hipcc.
 
 Every .hip source under csrc/ is compiled to an object in parallel and the
-objects are linked into mitra_algo_amd/libmvs.so.  FP contraction is off
+objects are linked into mitra_algo_amd/libmvs.so; the command line tool
+(app/*.cpp, host C++ on the C ABI) is linked against it as
+mitra_algo_amd/mvs_cli.  FP contraction is off
 (-ffp-contract=off) so FP64 kernels round every product and sum as written,
 like the reference CPU build; packed FP32 instructions are disabled.
 """
@@ -16,6 +18,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmvs.so")
+CLI = os.path.join(HERE, "mvs_cli")
+APP = os.path.join(HERE, "app")
 BUILD = os.path.join(HERE, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -53,9 +57,23 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_cli()
     if verbose:
-        print(f"built {OUT} from {len(srcs)} sources")
+        print(f"built {OUT} from {len(srcs)} sources, and {CLI}")
     return OUT
+
+
+def build_cli():
+    """mvs_cli: host C++17 over include/mvs.h, linked to libmvs.so (found next
+    to the executable at run time)."""
+    srcs = sorted(os.path.join(APP, f) for f in os.listdir(APP) if f.endswith(".cpp"))
+    tmp = CLI + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra", "-o", tmp,
+           *srcs, f"-L{HERE}", "-lmvs", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"mvs_cli build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CLI)
 
 
 def clean():
