@@ -113,8 +113,10 @@ struct Session {
   uint64_t ver_rows = 0;  // per-row results of the last call stay on the device
   uint64_t ver_us[3] = {0, 0, 0};  // host clock: device phase, download, assembly
   std::vector<std::vector<int32_t>> ver_slot;  // per-thread patch scratch
-  // verification surface: reference-order FP64 tree and leaf-ordered faces
-  DBuf<unsigned char> ver_nodes, ver_faces;
+  // verification surface: search records of the reference-order FP64 tree,
+  // leaf-ordered faces and their normals
+  DBuf<unsigned char> ver_nodes, ver_faces, ver_normals;
+  int ver_root = 0;           // root reference (inner node 0, or a leaf)
   int ver_stack = 0;          // LDS stack entries per thread (tree height + 1)
   uint64_t ver_surface_nv = 0;
   bool ver_surface = false;

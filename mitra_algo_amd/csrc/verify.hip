@@ -48,7 +48,7 @@ namespace {
 
 constexpr int kVThreads = 256;
 constexpr int kLeafFaces = 8;
-constexpr int kFaceWords = 12;  // a, b, c, unit normal
+constexpr int kFaceWords = 10;  // a, b, c, degenerate flag (normals apart)
 
 // Tree node: box, children (left < 0 for a leaf) and the leaf's face range
 // in leaf order.
@@ -57,6 +57,16 @@ struct alignas(16) SNode {
   int32_t left, right, begin, end;
 };
 static_assert(sizeof(SNode) == 64, "node layout");
+
+// Search record of an inner node: its two children's boxes and references
+// (one 128-byte line per visited inner node).  A reference >= 0 is an inner
+// node; a leaf of n faces starting at leaf-ordered face b is -1 - (16 b + n).
+struct alignas(128) Inner {
+  double lo[2][3], hi[2][3];
+  int32_t ref[2];
+};
+static_assert(sizeof(Inner) == 128, "search record layout");
+constexpr uint64_t kMaxLeafStart = uint64_t{1} << 27;  // 16 b + n fits int32
 
 // One cluster's motion (ClusterRecord, include/symmetry/storage.h:365-370).
 struct Motion {
@@ -166,14 +176,14 @@ __device__ inline double max3(double a, double b, double c) {
 __device__ inline double clamp01(double v) { return v < 0.0 ? 0.0 : (1.0 < v ? 1.0 : v); }
 __device__ inline double min2(double a, double b) { return b < a ? b : a; }
 
-// Squared distance from p to a node's box (surface.h:49-56).
-__device__ inline double box_bound(V3 p, const SNode &n) {
+// Squared distance from p to a box (surface.h:49-56).
+__device__ inline double box_bound(V3 p, const double *lo, const double *hi) {
   double s = 0;
-  double d = max3(n.lo[0] - p.x, 0.0, p.x - n.hi[0]);
+  double d = max3(lo[0] - p.x, 0.0, p.x - hi[0]);
   s += d * d;
-  d = max3(n.lo[1] - p.y, 0.0, p.y - n.hi[1]);
+  d = max3(lo[1] - p.y, 0.0, p.y - hi[1]);
   s += d * d;
-  d = max3(n.lo[2] - p.z, 0.0, p.z - n.hi[2]);
+  d = max3(lo[2] - p.z, 0.0, p.z - hi[2]);
   s += d * d;
   return s;
 }
@@ -190,12 +200,14 @@ __device__ inline double seg_d2(V3 p, V3 a, V3 b) {
 // Squared distance from p to triangle abc by the Voronoi regions of the
 // vertices, edges and face (Ericson, Real-Time Collision Detection, 2005,
 // 5.1.5), with the reference's degenerate fallback and operation order
-// (src/core/math.cpp:72-112).  ops += the FP64 additions, subtractions,
-// multiplications and divisions of the path taken (23 to 96; 116 for a
-// degenerate triangle).
-__device__ inline double tri_d2(V3 p, V3 a, V3 b, V3 c, uint32_t &ops) {
+// (src/core/math.cpp:72-112).  `degenerate` is the reference's test
+// |ab x ac|^2 < 1e-28, which does not depend on p and is evaluated once per
+// face on the host with the same operations.  ops += the FP64 additions,
+// subtractions, multiplications and divisions of the path taken (23 to 96;
+// 116 for a degenerate triangle), counting the face test.
+__device__ inline double tri_d2(V3 p, V3 a, V3 b, V3 c, bool degenerate, uint32_t &ops) {
   V3 ab = b - a, ac = c - a, ap = p - a;
-  if (dot(cross(ab, ac), cross(ab, ac)) < 1e-14 * 1e-14) {
+  if (degenerate) {
     ops += 116;
     return min2(min2(seg_d2(p, a, b), seg_d2(p, a, c)), seg_d2(p, b, c));
   }
@@ -249,15 +261,16 @@ __device__ inline V3 face_corner(const double *fr, int k) {
 // accepted, else -1 (the union-find parent array); dist2[r] = the nearest
 // squared distance when the row reached the surface search, else NaN.
 // One row: transform, early tests, tree search, normal test; tests, visits
-// and ops count the triangle tests, the tree nodes taken off the stack and
-// the FP64 operations of the search (box and triangle distances).
+// and ops count the triangle tests, the stack entries taken and the FP64
+// operations of the search (box and triangle distances).
 __device__ inline void verify_row(const double *__restrict__ vx, const double *__restrict__ vy,
                                   const double *__restrict__ vz, const double *__restrict__ nx,
                                   const double *__restrict__ ny, const double *__restrict__ nz,
-                                  uint32_t nv, uint64_t row, const SNode *__restrict__ nodes,
-                                  const double *__restrict__ faces,
+                                  uint32_t nv, uint64_t row, const Inner *__restrict__ inner,
+                                  int root, const double *__restrict__ faces,
+                                  const double *__restrict__ normals,
                                   const Motion *__restrict__ motions, double threshold,
-                                  int *stack, uint32_t *__restrict__ accepted,
+                                  int *stk, float *bnd, uint32_t *__restrict__ accepted,
                                   int32_t *__restrict__ label, double *__restrict__ dist2,
                                   uint32_t &tests, uint32_t &visits, uint32_t &ops) {
   const uint64_t seg = row / nv;
@@ -283,44 +296,62 @@ __device__ inline void verify_row(const double *__restrict__ vx, const double *_
     go = inverse ? !(side <= threshold) : !(side >= -threshold);
   }
   if (go) {
+    // Depth-first, nearer child first (the reference's order, decided on
+    // exact FP64 box distances).  A stack entry carries its box distance
+    // rounded down to FP32: an entry is skipped only when that lower bound
+    // already exceeds the best distance, so no subtree holding a face at the
+    // minimum is skipped; an entry the reference would skip and this search
+    // visits holds only farther faces.  The search alternates between
+    // descending to the next leaf and scanning it, so the lanes of a wave do
+    // the same kind of work at a time.
     best = __builtin_inf();
     int hit = -1, top = 1;
-    stack[0] = 0;
-    while (top > 0) {
-      const SNode nd = nodes[stack[--top * kVThreads]];
-      ++visits;
-      ops += 12;  // box distance: 2 subtractions, a product and a sum per axis
-      if (box_bound(q, nd) > best)
-        continue;
-      if (nd.left < 0) {
-        for (int k = nd.begin; k < nd.end; ++k) {
-          const double *fr = faces + static_cast<uint64_t>(k) * kFaceWords;
-          const double d =
-              tri_d2(q, face_corner(fr, 0), face_corner(fr, 1), face_corner(fr, 2), ops);
-          ++tests;
-          if (d < best) {
-            best = d;
-            hit = k;
-          }
+    stk[0] = root;
+    bnd[0] = -__builtin_inff();
+    while (true) {
+      int leaf = 0;
+      while (top > 0) {
+        --top;
+        const int ref = stk[top * kVThreads];
+        const float lb = bnd[top * kVThreads];
+        ++visits;
+        if (static_cast<double>(lb) > best)
+          continue;
+        if (ref < 0) {
+          leaf = ref;
+          break;
         }
-      } else {
-        int first = nd.left, second = nd.right;
-        ops += 24;
-        if (box_bound(q, nodes[first]) > box_bound(q, nodes[second])) {
-          first = nd.right;
-          second = nd.left;
+        const Inner nd = inner[ref];
+        ops += 24;  // two box distances
+        const double b0 = box_bound(q, nd.lo[0], nd.hi[0]), b1 = box_bound(q, nd.lo[1], nd.hi[1]);
+        const bool swap = b0 > b1;  // the right child first only when strictly nearer
+        stk[top * kVThreads] = swap ? nd.ref[0] : nd.ref[1];
+        bnd[top * kVThreads] = __double2float_rd(swap ? b0 : b1);
+        ++top;
+        // at most one waiting sibling per level plus the two children: tree
+        // depth + 1 entries suffice
+        stk[top * kVThreads] = swap ? nd.ref[1] : nd.ref[0];
+        bnd[top * kVThreads] = __double2float_rd(swap ? b1 : b0);
+        ++top;
+      }
+      if (leaf == 0)
+        break;
+      const int code = -1 - leaf, first = code >> 4, end = first + (code & 15);
+      for (int k = first; k < end; ++k) {
+        const double *fr = faces + static_cast<uint64_t>(k) * kFaceWords;
+        const double d = tri_d2(q, face_corner(fr, 0), face_corner(fr, 1), face_corner(fr, 2),
+                                fr[9] != 0.0, ops);
+        ++tests;
+        if (d < best) {
+          best = d;
+          hit = k;
         }
-        // depth-first, `first` on top: at most one waiting sibling per level
-        // plus the two children, so tree depth + 1 entries suffice
-        stack[top++ * kVThreads] = second;
-        stack[top++ * kVThreads] = first;
       }
     }
     if (hit >= 0 && !(best > threshold * threshold)) {
-      const double *fr = faces + static_cast<uint64_t>(hit) * kFaceWords;
-      const V3 fn = face_corner(fr, 3);
+      const double *fn = normals + 3 * static_cast<uint64_t>(hit);
       const V3 vn = apply(r, v3(nx[i], ny[i], nz[i]));
-      ok = dot(vn, fn) < 0.5 ? 0u : 1u;
+      ok = dot(vn, v3(fn[0], fn[1], fn[2])) < 0.5 ? 0u : 1u;
     }
   }
   accepted[row] = ok;
@@ -332,17 +363,20 @@ __global__ void __launch_bounds__(kVThreads)
 k_verify(const double *__restrict__ vx, const double *__restrict__ vy,
          const double *__restrict__ vz, const double *__restrict__ nx,
          const double *__restrict__ ny, const double *__restrict__ nz, uint32_t nv,
-         uint64_t rows, const SNode *__restrict__ nodes, const double *__restrict__ faces,
-         const Motion *__restrict__ motions, double threshold,
+         uint64_t rows, const Inner *__restrict__ inner, int root,
+         const double *__restrict__ faces, const double *__restrict__ normals,
+         const Motion *__restrict__ motions, double threshold, int stack_size,
          uint32_t *__restrict__ accepted, int32_t *__restrict__ label,
          double *__restrict__ dist2, unsigned long long *__restrict__ stats) {
+  // per thread: stack_size node references, then stack_size FP32 bounds
   extern __shared__ int stack_lds[];
-  int *stack = stack_lds + threadIdx.x;
+  int *stk = stack_lds + threadIdx.x;
+  float *bnd = reinterpret_cast<float *>(stack_lds + stack_size * kVThreads) + threadIdx.x;
   const uint64_t row = static_cast<uint64_t>(blockIdx.x) * kVThreads + threadIdx.x;
   uint32_t tests = 0, visits = 0, ops = 0;
   if (row < rows)
-    verify_row(vx, vy, vz, nx, ny, nz, nv, row, nodes, faces, motions, threshold, stack,
-               accepted, label, dist2, tests, visits, ops);
+    verify_row(vx, vy, vz, nx, ny, nz, nv, row, inner, root, faces, normals, motions, threshold,
+               stk, bnd, accepted, label, dist2, tests, visits, ops);
   // work counters: one atomic per wave
   const uint32_t wt = wave_sum(tests), wv = wave_sum(visits);
   const unsigned long long wo = wave_sum(static_cast<unsigned long long>(ops));
@@ -543,8 +577,10 @@ void surface_build(Session *s, const mvs_points *vertices, uint64_t nf, const in
   const auto shape = tb.shape_of(static_cast<int>(nf));
   tb.nodes.resize(static_cast<size_t>(shape.first));
   tb.build(0, 0, static_cast<int>(nf), 4);
-  // leaf-ordered faces with their unit normals (verification.cpp:71-73)
+  // leaf-ordered faces (corners and the degenerate test of
+  // src/core/math.cpp:74) and their unit normals (verification.cpp:71-73)
   std::vector<double> fr(std::max<uint64_t>(nf, 1) * kFaceWords, 0.0);
+  std::vector<double> fn(std::max<uint64_t>(nf, 1) * 3, 0.0);
   parallel_chunks(nf, [&](uint64_t b, uint64_t e) {
     for (uint64_t k = b; k < e; ++k) {
       const int face = tb.ids[k];
@@ -553,22 +589,53 @@ void surface_build(Session *s, const mvs_points *vertices, uint64_t nf, const in
         const int v = f[j][face];
         c[j] = v3(tb.x[v], tb.y[v], tb.z[v]);
       }
-      const V3 n = unit(cross(c[1] - c[0], c[2] - c[0]));
+      const V3 ab = c[1] - c[0], ac = c[2] - c[0];
+      const V3 n = unit(cross(ab, ac));
       double *o = fr.data() + k * kFaceWords;
       for (int j = 0; j < 3; ++j)
         o[3 * j] = c[j].x, o[3 * j + 1] = c[j].y, o[3 * j + 2] = c[j].z;
-      o[9] = n.x, o[10] = n.y, o[11] = n.z;
+      o[9] = dot(cross(ab, ac), cross(ab, ac)) < 1e-14 * 1e-14 ? 1.0 : 0.0;
+      fn[3 * k] = n.x, fn[3 * k + 1] = n.y, fn[3 * k + 2] = n.z;
     }
   });
+  // search records: inner nodes in pre-order, each with its children's boxes
+  if (nf >= kMaxLeafStart)
+    raise(MVS_EOVERFLOW, "verification: at most 2^27 faces");
+  std::vector<int32_t> inner_of(tb.nodes.size(), -1);
+  int32_t ninner = 0;
+  for (size_t i = 0; i < tb.nodes.size(); ++i)
+    if (tb.nodes[i].left >= 0)
+      inner_of[i] = ninner++;
+  auto ref_of = [&](int node) {
+    const SNode &c = tb.nodes[static_cast<size_t>(node)];
+    return c.left >= 0 ? inner_of[static_cast<size_t>(node)] : -1 - (16 * c.begin + (c.end - c.begin));
+  };
+  std::vector<Inner> inner(std::max<int32_t>(ninner, 1));
+  for (size_t i = 0; i < tb.nodes.size(); ++i) {
+    if (inner_of[i] < 0)
+      continue;
+    Inner &r = inner[static_cast<size_t>(inner_of[i])];
+    std::memset(&r, 0, sizeof(Inner));
+    const int kids[2] = {tb.nodes[i].left, tb.nodes[i].right};
+    for (int c = 0; c < 2; ++c) {
+      const SNode &k = tb.nodes[static_cast<size_t>(kids[c])];
+      for (int a = 0; a < 3; ++a)
+        r.lo[c][a] = k.lo[a], r.hi[c][a] = k.hi[a];
+      r.ref[c] = ref_of(kids[c]);
+    }
+  }
+  s->ver_root = ref_of(0);
   s->ver_stack = shape.second + 1;
-  if (static_cast<size_t>(s->ver_stack) * kVThreads * sizeof(int) > 64 * 1024)
+  if (static_cast<size_t>(s->ver_stack) * kVThreads * (sizeof(int) + sizeof(float)) > 64 * 1024)
     raise(MVS_EOVERFLOW, "verification: surface tree too deep");
   // queued work may still read the previous surface
   sync(s);
-  s->ver_nodes.alloc(tb.nodes.size() * sizeof(SNode));
+  s->ver_nodes.alloc(inner.size() * sizeof(Inner));
   s->ver_faces.alloc(fr.size() * sizeof(double));
-  copy_h2d(s, s->ver_nodes.ptr, tb.nodes.data(), tb.nodes.size() * sizeof(SNode));
+  s->ver_normals.alloc(fn.size() * sizeof(double));
+  copy_h2d(s, s->ver_nodes.ptr, inner.data(), inner.size() * sizeof(Inner));
   copy_h2d(s, s->ver_faces.ptr, fr.data(), fr.size() * sizeof(double));
+  copy_h2d(s, s->ver_normals.ptr, fn.data(), fn.size() * sizeof(double));
   sync(s);  // the host vectors go out of scope
   s->ver_surface_nv = nv;
   s->ver_surface = true;
@@ -626,7 +693,7 @@ void run(Session *s, uint64_t nv, const double *pts, const uint64_t *noff, const
       mo[c].off = cl.plane_offset[c];
       mo[c].reflection = cl.reflection[c] ? 1 : 0;
     }
-    const size_t lds = static_cast<size_t>(s->ver_stack) * kVThreads * sizeof(int);
+    const size_t lds = static_cast<size_t>(s->ver_stack) * kVThreads * (sizeof(int) + sizeof(float));
     Motion *d_mo = static_cast<Motion *>(scratch(s, kSlotVerMotions, nc * sizeof(Motion)));
     uint32_t *d_acc = scratch_as<uint32_t>(s, kSlotVerAccepted, rows);
     int32_t *d_label = scratch_as<int32_t>(s, kSlotVerLabel, rows);
@@ -641,8 +708,10 @@ void run(Session *s, uint64_t nv, const double *pts, const uint64_t *noff, const
     const uint32_t nv32 = static_cast<uint32_t>(nv);
     MVS_LAUNCH(s, "verify_rows", k_verify, grid, kVThreads, lds, pts, pts + nv, pts + 2 * nv,
                pts + 3 * nv, pts + 4 * nv, pts + 5 * nv, nv32, rows,
-               reinterpret_cast<const SNode *>(s->ver_nodes.ptr),
-               reinterpret_cast<const double *>(s->ver_faces.ptr), d_mo, threshold, d_acc,
+               reinterpret_cast<const Inner *>(s->ver_nodes.ptr), s->ver_root,
+               reinterpret_cast<const double *>(s->ver_faces.ptr),
+               reinterpret_cast<const double *>(s->ver_normals.ptr), d_mo, threshold,
+               s->ver_stack, d_acc,
                d_label, d_dist, d_stats);
     MVS_LAUNCH(s, "verify_cc_init", k_cc_init, grid, kVThreads, 0, noff, nidx, d_acc, nv32, rows,
                d_label);
