@@ -117,7 +117,8 @@ enum Slot : int {
   // verification
   kSlotVerPoints, kSlotVerMotions, kSlotVerNbrOff,
   kSlotVerNbrIdx, kSlotVerAccepted, kSlotVerLabel, kSlotVerDist, kSlotVerPos, kSlotVerBounds,
-  kSlotVerVid, kSlotVerRoot, kSlotVerD2, kSlotVerStats,
+  kSlotVerVid, kSlotVerRoot, kSlotVerD2, kSlotVerStats, kSlotVerKeys, kSlotVerKeys2,
+  kSlotVerIds, kSlotVerOrder,
   kSlotCount
 };
 

@@ -117,6 +117,7 @@ struct Session {
   // leaf-ordered faces and their normals
   DBuf<unsigned char> ver_nodes, ver_faces, ver_normals;
   int ver_root = 0;           // root reference (inner node 0, or a leaf)
+  double ver_box[6] = {0, 0, 0, 0, 0, 0};  // surface box: lo xyz, hi xyz
   int ver_stack = 0;          // LDS stack entries per thread (tree height + 1)
   uint64_t ver_surface_nv = 0;
   bool ver_surface = false;

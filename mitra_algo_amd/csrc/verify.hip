@@ -164,26 +164,20 @@ template <class F> void parallel_chunks(uint64_t n, F fn) {
 }
 
 // -------------------------------------------------------- device geometry
-// std::max({a, b, c}) and std::clamp as the reference evaluates them.
-__device__ inline double max3(double a, double b, double c) {
-  double r = a;
-  if (r < b)
-    r = b;
-  if (r < c)
-    r = c;
-  return r;
-}
+// std::clamp and std::min as the reference evaluates them.
 __device__ inline double clamp01(double v) { return v < 0.0 ? 0.0 : (1.0 < v ? 1.0 : v); }
 __device__ inline double min2(double a, double b) { return b < a ? b : a; }
 
-// Squared distance from p to a box (surface.h:49-56).
+// Squared distance from p to a box (surface.h:49-56).  The reference takes
+// std::max({lo - p, 0, p - hi}); fmax gives the same value up to the sign of
+// a zero, which the square discards.
 __device__ inline double box_bound(V3 p, const double *lo, const double *hi) {
   double s = 0;
-  double d = max3(lo[0] - p.x, 0.0, p.x - hi[0]);
+  double d = fmax(fmax(lo[0] - p.x, 0.0), p.x - hi[0]);
   s += d * d;
-  d = max3(lo[1] - p.y, 0.0, p.y - hi[1]);
+  d = fmax(fmax(lo[1] - p.y, 0.0), p.y - hi[1]);
   s += d * d;
-  d = max3(lo[2] - p.z, 0.0, p.z - hi[2]);
+  d = fmax(fmax(lo[2] - p.z, 0.0), p.z - hi[2]);
   s += d * d;
   return s;
 }
@@ -202,53 +196,51 @@ __device__ inline double seg_d2(V3 p, V3 a, V3 b) {
 // 5.1.5), with the reference's degenerate fallback and operation order
 // (src/core/math.cpp:72-112).  `degenerate` is the reference's test
 // |ab x ac|^2 < 1e-28, which does not depend on p and is evaluated once per
-// face on the host with the same operations.  ops += the FP64 additions,
-// subtractions, multiplications and divisions of the path taken (23 to 96;
-// 116 for a degenerate triangle), counting the face test.
+// face on the host with the same operations.
+//
+// Without branches: every dot product of the reference is formed up front
+// (forming one early changes no value), the region is the first whose test
+// holds, in the reference's order, and each region's closest point is
+// written in one form, base + d1 s + d2 t:
+//   vertex a, b, c  base = a, b, c            s = t = 0
+//   edge ab         base = a, d1 = ab,        s = d1 / (d1 - d3)
+//   edge ac         base = a, d1 = ac,        s = d2 / (d2 - d6)
+//   edge bc         base = b, d1 = c - b,     s = (d4-d3) / ((d4-d3) + (d5-d6))
+//   face            base = a, d1 = ab, d2 = ac,
+//                   s = vb / (va + vb + vc),  t = vc / (va + vb + vc)
+// Adding a zero product only flips the sign of a zero coordinate of
+// e = p - point, which its square discards, so every region's squared
+// distance is the reference's to the bit.  The lanes of a wave then run one
+// instruction stream instead of every branch in turn.  ops += the FP64
+// additions, subtractions, multiplications and divisions of the reference's
+// path for that region (24 to 82; 102 for a degenerate triangle).
 __device__ inline double tri_d2(V3 p, V3 a, V3 b, V3 c, bool degenerate, uint32_t &ops) {
-  V3 ab = b - a, ac = c - a, ap = p - a;
   if (degenerate) {
-    ops += 116;
+    ops += 102;
     return min2(min2(seg_d2(p, a, b), seg_d2(p, a, c)), seg_d2(p, b, c));
   }
-  double d1 = dot(ab, ap), d2 = dot(ac, ap);
-  if (d1 <= 0 && d2 <= 0) {
-    ops += 38;
-    return dot(ap, ap);
-  }
-  V3 bp = p - b;
-  double d3 = dot(ab, bp), d4 = dot(ac, bp);
-  if (d3 >= 0 && d4 <= d3) {
-    ops += 51;
-    return dot(bp, bp);
-  }
-  double vc = d1 * d4 - d3 * d2;
-  if (vc <= 0 && d1 >= 0 && d3 <= 0) {
-    ops += 65;
-    V3 e = p - (a + ab * (d1 / (d1 - d3)));
-    return dot(e, e);
-  }
-  V3 cp = p - c;
-  double d5 = dot(ab, cp), d6 = dot(ac, cp);
-  if (d6 >= 0 && d5 <= d6) {
-    ops += 67;
-    return dot(cp, cp);
-  }
-  double vb = d5 * d2 - d1 * d6;
-  if (vb <= 0 && d2 >= 0 && d6 <= 0) {
-    ops += 81;
-    V3 e = p - (a + ac * (d2 / (d2 - d6)));
-    return dot(e, e);
-  }
-  double va = d3 * d6 - d5 * d4;
-  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
-    ops += 89;
-    V3 e = p - (b + (c - b) * ((d4 - d3) / ((d4 - d3) + (d5 - d6))));
-    return dot(e, e);
-  }
-  ops += 96;
-  V3 e = p - (a + ab * (vb / (va + vb + vc)) + ac * (vc / (va + vb + vc)));
-  double r = dot(e, e);
+  const V3 ab = b - a, ac = c - a, ap = p - a, bp = p - b, cp = p - c;
+  const double d1 = dot(ab, ap), d2 = dot(ac, ap), d3 = dot(ab, bp), d4 = dot(ac, bp),
+               d5 = dot(ab, cp), d6 = dot(ac, cp);
+  const double vc = d1 * d4 - d3 * d2, vb = d5 * d2 - d1 * d6, va = d3 * d6 - d5 * d4;
+  const double e43 = d4 - d3, e56 = d5 - d6;
+  const bool ra = d1 <= 0 && d2 <= 0;
+  const bool rb = !ra && d3 >= 0 && d4 <= d3;
+  const bool rab = !ra && !rb && vc <= 0 && d1 >= 0 && d3 <= 0;
+  const bool rc = !ra && !rb && !rab && d6 >= 0 && d5 <= d6;
+  const bool rac = !ra && !rb && !rab && !rc && vb <= 0 && d2 >= 0 && d6 <= 0;
+  const bool rbc = !ra && !rb && !rab && !rc && !rac && va <= 0 && e43 >= 0 && e56 >= 0;
+  const bool face = !(ra || rb || rab || rc || rac || rbc);
+  ops += ra ? 24 : rb ? 37 : rab ? 51 : rc ? 53 : rac ? 67 : rbc ? 75 : 82;
+  const V3 base = (rb || rbc) ? b : (rc ? c : a);
+  const V3 d1v = rac ? ac : (rbc ? c - b : ab);
+  const double num = rab ? d1 : (rac ? d2 : (rbc ? e43 : vb));
+  const double den = rab ? d1 - d3 : (rac ? d2 - d6 : (rbc ? e43 + e56 : va + vb + vc));
+  const bool vertex = ra || rb || rc;
+  const double sv = vertex ? 0.0 : num / den;
+  const double tv = face ? vc / (va + vb + vc) : 0.0;
+  const V3 e = p - (base + d1v * sv + ac * tv);
+  const double r = dot(e, e);
   return 0.0 < r ? r : 0.0;
 }
 
@@ -266,7 +258,8 @@ __device__ inline V3 face_corner(const double *fr, int k) {
 __device__ inline void verify_row(const double *__restrict__ vx, const double *__restrict__ vy,
                                   const double *__restrict__ vz, const double *__restrict__ nx,
                                   const double *__restrict__ ny, const double *__restrict__ nz,
-                                  uint32_t nv, uint64_t row, const Inner *__restrict__ inner,
+                                  uint32_t nv, uint64_t row, const uint32_t *__restrict__ order,
+                                  const Inner *__restrict__ inner,
                                   int root, const double *__restrict__ faces,
                                   const double *__restrict__ normals,
                                   const Motion *__restrict__ motions, double threshold,
@@ -274,7 +267,8 @@ __device__ inline void verify_row(const double *__restrict__ vx, const double *_
                                   int32_t *__restrict__ label, double *__restrict__ dist2,
                                   uint32_t &tests, uint32_t &visits, uint32_t &ops) {
   const uint64_t seg = row / nv;
-  const uint32_t i = static_cast<uint32_t>(row - seg * nv);
+  const uint32_t i = order[row - seg * nv];
+  row = seg * nv + i;
   const Motion &mo = motions[seg >> 1];
   const bool inverse = seg & 1;
   uint32_t ok = 0;
@@ -366,7 +360,7 @@ k_verify(const double *__restrict__ vx, const double *__restrict__ vy,
          uint64_t rows, const Inner *__restrict__ inner, int root,
          const double *__restrict__ faces, const double *__restrict__ normals,
          const Motion *__restrict__ motions, double threshold, int stack_size,
-         uint32_t *__restrict__ accepted, int32_t *__restrict__ label,
+         const uint32_t *__restrict__ order, uint32_t *__restrict__ accepted, int32_t *__restrict__ label,
          double *__restrict__ dist2, unsigned long long *__restrict__ stats) {
   // per thread: stack_size node references, then stack_size FP32 bounds
   extern __shared__ int stack_lds[];
@@ -375,8 +369,8 @@ k_verify(const double *__restrict__ vx, const double *__restrict__ vy,
   const uint64_t row = static_cast<uint64_t>(blockIdx.x) * kVThreads + threadIdx.x;
   uint32_t tests = 0, visits = 0, ops = 0;
   if (row < rows)
-    verify_row(vx, vy, vz, nx, ny, nz, nv, row, inner, root, faces, normals, motions, threshold,
-               stk, bnd, accepted, label, dist2, tests, visits, ops);
+    verify_row(vx, vy, vz, nx, ny, nz, nv, row, order, inner, root, faces, normals, motions,
+               threshold, stk, bnd, accepted, label, dist2, tests, visits, ops);
   // work counters: one atomic per wave
   const uint32_t wt = wave_sum(tests), wv = wave_sum(visits);
   const unsigned long long wo = wave_sum(static_cast<unsigned long long>(ops));
@@ -494,6 +488,35 @@ k_compact(const uint32_t *__restrict__ accepted, const uint64_t *__restrict__ po
   vid[o] = static_cast<int32_t>(row - seg * nv);
   root[o] = label[row];
   d2[o] = dist2[row];
+}
+
+// Spreads the low 16 bits of v to every third bit (bit k -> bit 3k).
+__device__ inline uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0xffffu;
+  x = (x | (x << 16)) & 0x0000ff0000ffull;
+  x = (x | (x << 8)) & 0x00f00f00f00full;
+  x = (x | (x << 4)) & 0x0c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x249249249249ull;
+  return x;
+}
+
+// Morton key (16 bits per axis over the surface's box, x in the lowest bit)
+// and vertex id of every vertex: sorted, the ids give the order in which a
+// workgroup's threads take vertices, so the 64 rows of a wave lie close
+// together and search similar parts of the tree.
+__global__ void __launch_bounds__(kVThreads)
+k_morton(const double *__restrict__ x, const double *__restrict__ y,
+         const double *__restrict__ z, uint32_t nv, double lox, double loy, double loz,
+         double scale, uint64_t *__restrict__ keys, uint32_t *__restrict__ ids) {
+  const uint32_t i = blockIdx.x * kVThreads + threadIdx.x;
+  if (i >= nv)
+    return;
+  auto q = [scale](double v, double lo) {
+    const double t = (v - lo) * scale;
+    return static_cast<uint32_t>(t <= 0.0 ? 0.0 : (t >= 65535.0 ? 65535.0 : t));
+  };
+  keys[i] = spread3(q(x[i], lox)) | (spread3(q(y[i], loy)) << 1) | (spread3(q(z[i], loz)) << 2);
+  ids[i] = i;
 }
 
 // First compact position of every segment: bnd[k] = pos[k * nv], k <= segs.
@@ -625,6 +648,10 @@ void surface_build(Session *s, const mvs_points *vertices, uint64_t nf, const in
     }
   }
   s->ver_root = ref_of(0);
+  for (int k = 0; k < 3; ++k) {
+    s->ver_box[k] = nf ? tb.nodes[0].lo[k] : 0.0;
+    s->ver_box[3 + k] = nf ? tb.nodes[0].hi[k] : 0.0;
+  }
   s->ver_stack = shape.second + 1;
   if (static_cast<size_t>(s->ver_stack) * kVThreads * (sizeof(int) + sizeof(float)) > 64 * 1024)
     raise(MVS_EOVERFLOW, "verification: surface tree too deep");
@@ -706,12 +733,25 @@ void run(Session *s, uint64_t nv, const double *pts, const uint64_t *noff, const
 
     const unsigned grid = blocks_for(rows, kVThreads);
     const uint32_t nv32 = static_cast<uint32_t>(nv);
+    // rows taken in the Morton order of their vertices
+    uint64_t *d_keys = scratch_as<uint64_t>(s, kSlotVerKeys, nv);
+    uint64_t *d_keys2 = scratch_as<uint64_t>(s, kSlotVerKeys2, nv);
+    uint32_t *d_ids = scratch_as<uint32_t>(s, kSlotVerIds, nv);
+    uint32_t *d_order = scratch_as<uint32_t>(s, kSlotVerOrder, nv);
+    {
+      const double *lo = s->ver_box, *hi = s->ver_box + 3;
+      const double ext = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+      const double scale = ext > 0 ? 65535.0 / ext : 0.0;
+      MVS_LAUNCH(s, "verify_morton", k_morton, blocks_for(nv, kVThreads), kVThreads, 0, pts,
+                 pts + nv, pts + 2 * nv, nv32, lo[0], lo[1], lo[2], scale, d_keys, d_ids);
+      radix_sort_pairs_u64(s, d_keys, d_ids, d_keys2, d_order, nv, 48);
+    }
     MVS_LAUNCH(s, "verify_rows", k_verify, grid, kVThreads, lds, pts, pts + nv, pts + 2 * nv,
                pts + 3 * nv, pts + 4 * nv, pts + 5 * nv, nv32, rows,
                reinterpret_cast<const Inner *>(s->ver_nodes.ptr), s->ver_root,
                reinterpret_cast<const double *>(s->ver_faces.ptr),
                reinterpret_cast<const double *>(s->ver_normals.ptr), d_mo, threshold,
-               s->ver_stack, d_acc,
+               s->ver_stack, d_order, d_acc,
                d_label, d_dist, d_stats);
     MVS_LAUNCH(s, "verify_cc_init", k_cc_init, grid, kVThreads, 0, noff, nidx, d_acc, nv32, rows,
                d_label);
