@@ -396,8 +396,8 @@ def bench_verify(args):
 
     for _ in range(args.warmup):
         step()
-    names = ("verify_rows", "verify_cc_init", "verify_cc_hook", "verify_cc_flatten",
-             "verify_compact")
+    names = ("verify_morton", "verify_rows", "verify_cc_init", "verify_cc_jump",
+             "verify_cc_hook", "verify_cc_flatten", "verify_compact")
     sess.enable_timing(True, only=names)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -409,7 +409,8 @@ def bench_verify(args):
     for k in names:
         ms_, n = sess.timing(k)
         if n:
-            kern[k] = {"ms_avg": ms_ / n, "launches_per_step": n / args.steps}
+            kern[k] = {"ms_avg": ms_ / n, "launches_per_step": n / args.steps,
+                       "ms_per_step": ms_ / args.steps}
     sess.enable_timing(False)
     stats = sess.verification_stats()
     rows = 2 * nc * nv

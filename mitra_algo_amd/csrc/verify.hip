@@ -49,6 +49,7 @@ namespace {
 constexpr int kVThreads = 256;
 constexpr int kLeafFaces = 8;
 constexpr int kFaceWords = 10;  // a, b, c, degenerate flag (normals apart)
+constexpr int kJumpRounds = 10;  // pointer-jumping rounds before the hooks
 
 // Tree node: box, children (left < 0 for a leaf) and the leaf's face range
 // in leaf order.
@@ -427,6 +428,24 @@ k_cc_init(const uint64_t *__restrict__ nbr_off, const int32_t *__restrict__ nbr_
   label[row] = best;
 }
 
+// k_cc_jump: one round of pointer jumping, parent <- grandparent (still an
+// ancestor).  A few rounds after k_cc_init shorten the chains the
+// initial parents form (along a grid they run a whole column) before the
+// hooks walk them; the number of rounds does not change the result.
+__global__ void __launch_bounds__(kVThreads)
+k_cc_jump(const uint32_t *__restrict__ accepted, uint32_t nv, uint64_t rows, int32_t *label) {
+  const uint64_t row = static_cast<uint64_t>(blockIdx.x) * kVThreads + threadIdx.x;
+  if (row >= rows || !accepted[row])
+    return;
+  const uint64_t seg = row / nv;
+  int32_t *par = label + seg * nv;
+  const int32_t u = static_cast<int32_t>(row - seg * nv);
+  const int32_t p = load_parent(par + u);
+  const int32_t g = load_parent(par + p);
+  if (g != p)
+    store_parent(par + u, g);
+}
+
 // k_cc_hook: every ring edge (u, v), u < v, with both ends accepted joins
 // their trees by hooking the larger root under the smaller with a
 // compare-and-swap; a failed swap moves on to the root's new parent.
@@ -755,6 +774,8 @@ void run(Session *s, uint64_t nv, const double *pts, const uint64_t *noff, const
                d_label, d_dist, d_stats);
     MVS_LAUNCH(s, "verify_cc_init", k_cc_init, grid, kVThreads, 0, noff, nidx, d_acc, nv32, rows,
                d_label);
+    for (int round = 0; round < kJumpRounds; ++round)
+      MVS_LAUNCH(s, "verify_cc_jump", k_cc_jump, grid, kVThreads, 0, d_acc, nv32, rows, d_label);
     MVS_LAUNCH(s, "verify_cc_hook", k_cc_hook, grid, kVThreads, 0, noff, nidx, d_acc, nv32, rows,
                d_label);
     MVS_LAUNCH(s, "verify_cc_flatten", k_cc_flatten, grid, kVThreads, 0, nv32, rows, d_label);
