@@ -420,8 +420,15 @@ def bench_verify(args):
     full_s = time.perf_counter() - t0
     assert all(np.array_equal(a, b) for a, b in zip(full["side_a"], r["side_a"]))
     tflops = stats["fp64_ops"] / (kern["verify_rows"]["ms_avg"] * 1e-3) / 1e12
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic_verify.json")) as f:
+            t = json.load(f)["kernels"]["k_verify"]
+        traffic = (2.0 * t["fetch_kib"] + t["write_kib"]) * 1024
+    except (OSError, ValueError, KeyError):
+        pass
     roof = {"bound": "fp64", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "verify_rows",
+            "frac": tflops / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": "verify_rows",
             "ms_avg": kern["verify_rows"]["ms_avg"], "algorithmic_ops": stats["fp64_ops"]}
     out = {
         "metric": "verified rows/s (cluster x side x vertex) of Backend::verification",
