@@ -169,6 +169,7 @@ def lib() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built; run mitra_algo_amd/build.py "
                               "(or __graft_entry__.build())")
+        _torch_first()
         L = C.CDLL(LIB_PATH)
         L.mvs_last_error.restype = C.c_char_p
         L.mvs_mt64_next.restype = C.c_uint64
@@ -249,11 +250,12 @@ def sample_vertices(weights: np.ndarray, ratio: float, g: MT64) -> np.ndarray:
 
 
 def _torch_first():
-    """Initialises torch's HIP runtime, when torch is installed and sees a GPU,
-    before libmvs touches the device.  torch ships its own HIP runtime beside
-    the ROCm one libmvs links; both drive the same device through one HSA
-    runtime (so device pointers pass between them), but torch's reports no
-    GPU when it initialises after the other in the same process."""
+    """Loads and initialises torch's HIP runtime, when torch is installed and
+    sees a GPU, before libmvs is loaded.  torch ships its own HIP runtime
+    beside the ROCm one libmvs links; both drive the same device through one
+    HSA runtime (so device pointers pass between them), and they work
+    together when torch's is loaded and initialised first; in the other
+    orders one of the two reports no GPU."""
     try:
         import torch
     except ImportError:
@@ -266,7 +268,6 @@ class Session:
     """One libmvs session (device context + stream + scratch)."""
 
     def __init__(self, device: int = 0):
-        _torch_first()
         self.h = C.c_void_p()
         check(lib().mvs_session_create(C.c_int(device), C.byref(self.h)))
 
