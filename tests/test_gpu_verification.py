@@ -194,3 +194,21 @@ def test_device_layer_matches_host_layer(sess, ref):
         same(got, w)
     with pytest.raises(capi.MvsError):
         sess.dev_verification(nv - 1, pts, off, idx, topo.weights, sets[0], .025, topo.box_size)
+
+
+def test_full_mesh_matches_reference(sess, ref):
+    # the 2.1M-triangle workload mesh: one copy-to-copy rigid motion and one
+    # reflection through the centre, against the reference's CPU code
+    from types import SimpleNamespace
+
+    from mitra_algo_amd import capi as _capi
+    from mitra_algo_amd import workload
+    m = meshgen.cfg2_mesh()
+    t = _capi.prepare(m.vertices, m.faces)
+    topo = SimpleNamespace(normals=t["normals"], weights=t["weights"], nbr_off=t["nbr_off"],
+                           nbr_idx=t["nbr_idx"], box_size=t["box_size"])
+    cl = workload.copy_motions(m)
+    pick = [0, len(cl["reflection"]) - 1]
+    two = {k: np.asarray(v)[pick] for k, v in cl.items()}
+    got = run(sess, ref, m.vertices, t["faces"], topo, two, .025)
+    assert int(got["side_a"][0][-1]) > 100000 and int(got["side_b"][0][-1]) > 100000
