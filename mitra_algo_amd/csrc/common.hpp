@@ -107,6 +107,8 @@ enum Slot : int {
   kSlotPlanSlab, kSlotPlanKw, kSlotPlanLongCnt, kSlotPlanLongOff,
   kSlotVoteBlock, kSlotVoteBlockOff,
   kSlotMs0, kSlotMs1, kSlotMs2, kSlotMs3, kSlotMs4, kSlotMs5, kSlotMs6, kSlotMs7, kSlotMs8,
+  // the second kind's query inputs while the first kind's query still runs
+  kSlotMsB0, kSlotMsB5, kSlotMsB6, kSlotMsB7,
   // fused step outputs (valid until the next fused call)
   kSlotOutPairs, kSlotOutRigidQ, kSlotOutRigidT, kSlotOutRigidPair, kSlotOutReflN,
   kSlotOutReflD, kSlotOutReflPair, kSlotOutMsRigidPts, kSlotOutMsReflPts, kSlotOutMsSeeds,

@@ -214,6 +214,7 @@ struct MsCtl {
   Grid grid;
   unsigned long long sums[4];
   unsigned bad;
+  Grid grid2;  // the second slot set's grid
 };
 
 // One wavefront per segment: reduces the segment's nb partial boxes and
@@ -830,6 +831,10 @@ static MsCtl *ms_ctl(Session *s) {
 void ms_begin(Session *s) { zero_device(s, ms_ctl(s), sizeof(MsCtl)); }
 
 void ms_finish(Session *s, unsigned long long sums[4]) {
+  if (s->side_pending) {  // the side stream's query and sums come first
+    MVS_HIP(hipStreamWaitEvent(s->stream, s->join_event, 0));
+    s->side_pending = false;
+  }
   // the visit sums and the flag only (a captured step compares what it reads)
   struct {
     unsigned long long sums[4];
@@ -855,9 +860,12 @@ void ms_finish(Session *s, unsigned long long sums[4]) {
 template <class R>
 static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns, const R *seeds,
                        int32_t *seed_ids, uint64_t salt, int iterations, double h, double cutoff,
-                       R *out, R *weight, int run_slot, const MsSegs *seg) {
+                       R *out, R *weight, int run_slot, const MsSegs *seg, MsRun run) {
   using V4 = typename Vec<R>::V4;
   using V2 = typename Vec<R>::V2;
+  // slots read by the query (the set-up's own scratch is shared)
+  const int slot_rec = run.set ? kSlotMsB0 : kSlotMs0, slot_off = run.set ? kSlotMsB5 : kSlotMs5,
+            slot_seed = run.set ? kSlotMsB6 : kSlotMs6, slot_stats = run.set ? kSlotMsB7 : kSlotMs7;
   if (D != 4 && D != 6)
     raise(MVS_EINVAL, "batched mean-shift supports 4-D (reflection) and 6-D (rigid) points");
   if (!(h > 0) || !(cutoff > 0) || !std::isfinite(h * cutoff) || iterations < 0)
@@ -874,18 +882,19 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   const uint64_t *seg_off = seg ? seg->seg_off : nullptr;
   MsCtl *ctl = ms_ctl(s);
   // per-seed statistics [0, ns) candidates, [ns, 2ns) in-support visits
-  unsigned long long *stats = scratch_as<unsigned long long>(s, kSlotMs7, 2 * ns);
+  unsigned long long *stats = scratch_as<unsigned long long>(s, slot_stats, 2 * ns);
   if (seg)  // seeds past the drawn count leave their slots untouched
     zero_device(s, stats, 2 * ns * sizeof(unsigned long long));
   const uint64_t np = n ? n : 1;
   // cell-sorted records, then the sentinel record at [n]
   char *sorted =
-      static_cast<char *>(scratch(s, kSlotMs0, (np + 1) * (sizeof(V4) + sizeof(V2))));
+      static_cast<char *>(scratch(s, slot_rec, (np + 1) * (sizeof(V4) + sizeof(V2))));
   V4 *p4 = reinterpret_cast<V4 *>(sorted);
   V2 *q2 = reinterpret_cast<V2 *>(sorted + (np + 1) * sizeof(V4));
   const uint64_t cells_cap = seg ? std::max<uint64_t>(seg->cells_bound, 1) : kMaxCells;
-  uint64_t *off = scratch_as<uint64_t>(s, kSlotMs5, cells_cap + 1);
-  Grid *grids = seg ? scratch_as<Grid>(s, kSlotBatchMs0, meshes) : &ctl->grid;
+  uint64_t *off = scratch_as<uint64_t>(s, slot_off, cells_cap + 1);
+  Grid *grids = seg ? scratch_as<Grid>(s, kSlotBatchMs0, meshes)
+                    : (run.set ? &ctl->grid2 : &ctl->grid);
   uint64_t *cellbase = nullptr;
   const uint64_t *total_cells = reinterpret_cast<const uint64_t *>(
       reinterpret_cast<const char *>(grids) + offsetof(Grid, ncells));
@@ -964,14 +973,14 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   // seeds in cell order
   const uint32_t *order = nullptr, *seed_pos = nullptr;
   if (seed_ids) {
-    uint32_t *pos = scratch_as<uint32_t>(s, kSlotMs6, ns);
+    uint32_t *pos = scratch_as<uint32_t>(s, slot_seed, ns);
     MVS_LAUNCH(s, "ms_seeds", k_pick_sorted, blocks_for(ns, kThreads), kThreads, 0, seg_off,
                seg ? seg->seed_off : nullptr, meshes, n, ns, salt, sorted_ids, pos, seed_ids,
                seg ? seg->seed_mesh : nullptr);
     seed_pos = pos;
   } else if (n && ns > 1) {
     uint64_t *skeys = scratch_as<uint64_t>(s, kSlotMs4, 2 * ns);
-    uint32_t *svals = scratch_as<uint32_t>(s, kSlotMs6, 2 * ns);
+    uint32_t *svals = scratch_as<uint32_t>(s, slot_seed, 2 * ns);
     MVS_LAUNCH(s, "ms_seed_keys", k_seed_keys<R>, blocks_for(ns, kThreads), kThreads, 0, seeds,
                ns, grids, skeys, svals);
     radix_sort_pairs_u64(s, skeys, svals, skeys + ns, svals + ns, ns, kCellBits);
@@ -990,6 +999,24 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   MVS_LAUNCH(s, "ms_query", (k_ms_query<R, DD, WW>), grid, kThreads, 0, p4, q2, off, grids,   \
              cellbase, seed_mesh, ns_dev, seed_pos, seeds, order, ns, logical,                \
              static_cast<uint32_t>(n), iterations, gscale, r2, fw, out, weight, stats)
+  // a side-stream query waits for this set-up and runs beside what the
+  // session stream does next; ms_finish joins it
+  const hipStream_t main = s->stream;
+  if (run.side) {
+    if (!s->side_stream) {
+      MVS_HIP(hipStreamCreateWithFlags(&s->side_stream, hipStreamNonBlocking));
+      MVS_HIP(hipEventCreateWithFlags(&s->fork_event, hipEventDisableTiming));
+      MVS_HIP(hipEventCreateWithFlags(&s->join_event, hipEventDisableTiming));
+    }
+    MVS_HIP(hipEventRecord(s->fork_event, main));
+    MVS_HIP(hipStreamWaitEvent(s->side_stream, s->fork_event, 0));
+    s->stream = s->side_stream;
+  }
+  struct Restore {
+    Session *s;
+    hipStream_t main;
+    ~Restore() { s->stream = main; }
+  } restore{s, main};
   if (D == 6 && wide)
     MVS_MS_QUERY(6, kWavesPerBlock);
   else if (D == 6)
@@ -1004,21 +1031,25 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats + ns, ns,
                ctl->sums + run_slot + 1);
   }
+  if (run.side) {
+    MVS_HIP(hipEventRecord(s->join_event, s->side_stream));
+    s->side_pending = true;
+  }
 }
 
 void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,
                     const float *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
                     double h, double cutoff, float *out, float *weight, int run_slot,
-                    const MsSegs *seg) {
+                    const MsSegs *seg, MsRun run) {
   ms_batched<float>(s, D, n, pts, ns, seeds, seed_ids, salt, iterations, h, cutoff, out, weight,
-                    run_slot, seg);
+                    run_slot, seg, run);
 }
 void ms_batched_f64(Session *s, int D, uint64_t n, const double *pts, uint64_t ns,
                     const double *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
                     double h, double cutoff, double *out, double *weight, int run_slot,
-                    const MsSegs *seg) {
+                    const MsSegs *seg, MsRun run) {
   ms_batched<double>(s, D, n, pts, ns, seeds, seed_ids, salt, iterations, h, cutoff, out, weight,
-                     run_slot, seg);
+                     run_slot, seg, run);
 }
 void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4, const void *t3,
                       uint64_t nrefl, const void *n3, uint64_t n3_stride, const void *d1,
@@ -1071,11 +1102,13 @@ int mvs_dev_ms_batched(mvs_session *sp, int precision, int dim, uint64_t n, cons
     if (precision == MVS_FP32_GUARDED)
       ms_batched<float>(s, dim, n, static_cast<const float *>(pts), ns,
                         static_cast<const float *>(seeds), nullptr, 0, iterations, h, cutoff,
-                        static_cast<float *>(out), static_cast<float *>(weight), 0, nullptr);
+                        static_cast<float *>(out), static_cast<float *>(weight), 0, nullptr,
+                        MsRun{});
     else
       ms_batched<double>(s, dim, n, static_cast<const double *>(pts), ns,
                          static_cast<const double *>(seeds), nullptr, 0, iterations, h, cutoff,
-                         static_cast<double *>(out), static_cast<double *>(weight), 0, nullptr);
+                         static_cast<double *>(out), static_cast<double *>(weight), 0, nullptr,
+                         MsRun{});
     unsigned long long sums[4];
     ms_finish(s, sums);
     if (candidates)

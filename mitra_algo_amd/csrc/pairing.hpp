@@ -77,14 +77,22 @@ void ms_finish(Session *s, unsigned long long sums[4]);
 // Seeds: explicit positions (D x ns SoA), or, with seed_ids, ns of the n
 // points drawn stratified over the cell-sorted order with `salt` (indices
 // written to seed_ids, in cell order).
+// Two runs of one call may overlap: the first with side = true queues its
+// query on the session's side stream (forked after its set-up), the second
+// uses the other slot set (set = 1) for what its query reads, and
+// ms_finish joins the side stream before reading anything back.
+struct MsRun {
+  int set = 0;        // slot set of the query inputs (0 or 1)
+  bool side = false;  // query on the side stream
+};
 void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,
                     const float *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
                     double h, double cutoff, float *out, float *weight, int run_slot,
-                    const MsSegs *seg = nullptr);
+                    const MsSegs *seg = nullptr, MsRun run = {});
 void ms_batched_f64(Session *s, int D, uint64_t n, const double *pts, uint64_t ns,
                     const double *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
                     double h, double cutoff, double *out, double *weight, int run_slot,
-                    const MsSegs *seg = nullptr);
+                    const MsSegs *seg = nullptr, MsRun run = {});
 // Compact votes to mean-shift points; mirror normals are 3 rows of n3_stride.
 void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4, const void *t3,
                       uint64_t nrefl, const void *n3, uint64_t n3_stride, const void *d1,

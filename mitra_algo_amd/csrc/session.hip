@@ -139,6 +139,13 @@ Session::~Session() {
     (void)hipHostFree(stage);
   if (batch_stage)
     (void)hipHostFree(batch_stage);
+  if (side_stream) {
+    (void)hipStreamSynchronize(side_stream);
+    (void)hipStreamDestroy(side_stream);
+  }
+  for (auto e : {fork_event, join_event})
+    if (e)
+      (void)hipEventDestroy(e);
   if (own_stream)
     (void)hipStreamDestroy(own_stream);
 }

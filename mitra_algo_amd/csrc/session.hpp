@@ -59,6 +59,11 @@ struct Session {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
+  // second stream for independent work forked from `stream` and joined back
+  // with events before anything reads its results
+  hipStream_t side_stream = nullptr;
+  hipEvent_t fork_event = nullptr, join_event = nullptr;
+  bool side_pending = false;
   DBuf<unsigned char> slots[kSlotCount];
   void *stage = nullptr;
   size_t stage_bytes = 0;
