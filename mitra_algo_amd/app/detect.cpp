@@ -169,7 +169,10 @@ Result detect(Mesh mesh, const Options &options, int device, const Progress &pro
 
   // prepare (src/core/prepare.cpp:102-149): libmvs validates the mesh,
   // rejects degenerate faces and builds the topology; subdivision runs on the
-  // surviving faces and the topology is rebuilt on the subdivided mesh
+  // surviving faces and the topology is rebuilt on the subdivided mesh.  That
+  // second pass applies the degenerate-face test again, which the reference
+  // does not: a child of a face whose area lies within 4x the test's bound
+  // (1e-14 box^2) would be dropped here and kept there.
   validate_options(options);
   if (mesh.vertices() > static_cast<size_t>(std::numeric_limits<int32_t>::max()) ||
       mesh.faces() > static_cast<size_t>(std::numeric_limits<int32_t>::max()))

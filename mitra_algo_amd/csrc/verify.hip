@@ -710,6 +710,8 @@ void run(Session *s, uint64_t nv, const double *pts, const uint64_t *noff, const
     raise(MVS_EINVAL, "verification: null cluster array");
   const double threshold = verify_threshold * box;
   const uint64_t segs = 2 * nc, rows = segs * nv;
+  if (nv && segs > (static_cast<uint64_t>(UINT32_MAX) * kVThreads) / nv)
+    raise(MVS_EOVERFLOW, "verification: clusters x vertices exceed one launch");
   for (int sd = 0; sd < 2; ++sd) {
     s->ver_side_off[sd].assign(1, 0);
     s->ver_side[sd].clear();
