@@ -1,6 +1,7 @@
 // detect() over the libmvs host layer (src/core/pipeline.cpp:5-52).
 #include "detect.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>

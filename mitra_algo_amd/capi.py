@@ -460,11 +460,12 @@ class Session:
         out, sides = {}, []
         for name, total, patches in (("a", cnt.side_a, cnt.patches_a),
                                      ("b", cnt.side_b, cnt.patches_b)):
-            so = np.zeros(nc + 1, np.uint64)
-            sv = np.zeros(max(total, 1), np.int32)
-            pg = np.zeros(nc + 1, np.uint64)
-            pof = np.zeros(patches + 1, np.uint64)
-            pm = np.zeros(max(total, 1), np.int32)
+            # every element is written by the fetch
+            so = np.empty(nc + 1, np.uint64)
+            sv = np.empty(max(total, 1), np.int32)
+            pg = np.empty(nc + 1, np.uint64)
+            pof = np.empty(patches + 1, np.uint64)
+            pm = np.empty(max(total, 1), np.int32)
             sides.append(VerificationSide(ptr(so), ptr(sv), ptr(pg), ptr(pof), ptr(pm)))
             out["side_" + name] = (so, sv[:total])
             out["patches_" + name] = (pg, pof, pm[:total])
