@@ -141,11 +141,11 @@ int main(int argc, char **argv) {
                                "(--backend hip)");
     if (backend != "hip" && backend != "cuda")
       throw std::runtime_error("Unknown backend: " + backend);
-    std::cout << "backend=hip\n";
+    std::cout << "backend=hip" << std::endl;
     mvs_app::Result r =
         mvs_app::detect(mvs_app::read_mesh(input), o, device,
                         [](const std::string &stage, double ms) {
-                          std::cout << stage << ": " << ms << " ms\n";
+                          std::cout << stage << ": " << ms << " ms" << std::endl;
                         });
     mvs_app::export_result(r, out);
     std::cout << "vertices=" << r.mesh.vertices() << " sample=" << r.sample.size()
