@@ -3,10 +3,15 @@
 // Sequential-seed mode (the reference's semantics, src/cpu/clustering.cpp:
 // 32-208): one trajectory at a time, each run entirely inside one launch of
 // k_trajectory (a single 1024-thread workgroup): per iteration the distances
-// of all same-kind votes are evaluated in parallel, the in-support votes are
-// compacted in vote order, and their weights and weighted descriptors are
-// summed IN VOTE ORDER by one lane per component, so every sum rounds exactly
-// as the reference's sequential loop does.  The host keeps the reference's
+// of the trajectory's candidate votes are evaluated in parallel, the
+// in-support votes are compacted in vote order, and their weights and
+// weighted descriptors are summed IN VOTE ORDER by one lane per component, so
+// every sum rounds exactly as the reference's sequential loop does.  The
+// candidates are the same-kind votes within 2h of an anchor mean, listed in
+// vote order by a full pass over the votes and kept while the mean stays
+// within h of the anchor: a superset of every support met meanwhile, so the
+// pass over all votes runs about 1.6 times per trajectory instead of once
+// per iteration.  The host keeps the reference's
 // driver: seed = k-th unvisited vote (Fenwick tree instead of rebuilding the
 // unvisited list), merge into the first same-kind centre within h^2/4,
 // winners by largest support count, extraction, stable sort, truncation.
@@ -24,24 +29,91 @@ namespace mvs {
 namespace {
 
 constexpr int kTrajThreads = 1024;
+constexpr int kTrajWaves = kTrajThreads / kWave;
+// entries per thread in one round of a scan: 4 x 1024 entries share one set
+// of barriers, and each thread keeps 4 independent distance sums in flight
+constexpr int kTrajPer = 4;
+static_assert(kTrajPer * kTrajWaves == kWave, "one wave scans the round's counts");
+// A trajectory's candidate list holds the same-kind votes within
+// (1 + kAnchorRadius) h of the mean at which the list was built (its anchor);
+// it is rebuilt by a full pass once the mean is kAnchorRadius h or farther
+// from the anchor.  1.0 rebuilds about 1.6 times per trajectory.
+constexpr double kAnchorRadius = 1.0;
+// products staged per chunk of the ordered sums (13 x 512 doubles of LDS)
+constexpr int kSumChunk = 512;
 
 struct TrajState {
   double mean[12];
   int32_t converged, exhausted, iterations, touched;
 };
 
+// Slots of the selected entries of one scan round (entry q * kTrajThreads +
+// tid of the round) in entry order, after the *count entries selected before
+// the round; *count then includes the round.  Every thread of the workgroup
+// calls it.
+__device__ inline void ordered_round(const bool (&in)[kTrajPer], uint32_t (&pos)[kTrajPer],
+                                     uint32_t *wsum, uint32_t *count) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  uint64_t mask[kTrajPer];
+#pragma unroll
+  for (int q = 0; q < kTrajPer; ++q) {
+    mask[q] = ballot(in[q]);
+    if (lane == 0)
+      wsum[q * kTrajWaves + wave] = static_cast<uint32_t>(__popcll(mask[q]));
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const uint32_t c = wsum[lane];
+    uint32_t incl = c;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t t = __shfl_up(incl, d);
+      if (lane >= d)
+        incl += t;
+    }
+    const uint32_t before = *count;
+    wsum[lane] = before + incl - c;
+    if (lane == kWave - 1)
+      *count = before + incl;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kTrajPer; ++q)
+    pos[q] = wsum[q * kTrajWaves + wave] + static_cast<uint32_t>(lanes_below(mask[q]));
+  __syncthreads();
+}
+
+// Squared distance of vote j from m over the first dim components, summed
+// in component order as the reference does.
+__device__ inline double vote_d2(const double *desc, uint64_t V, uint64_t j, const double *m,
+                                 int dim) {
+  double s = 0;
+  for (int k = 0; k < dim; ++k) {
+    const double d = m[k] - desc[static_cast<uint64_t>(k) * V + j];
+    s += d * d;
+  }
+  return s;
+}
+
 // support[] must be zero on entry for every vote; on exit the touched votes
 // are listed in touched_ids/touched_counts and their support entries reset.
+// Exactness of the candidate list: distances carry a relative rounding error
+// below 2e-15, so with reach2 = (1 + r)^2 h^2 (1 + 1e-6) and keep2 =
+// r^2 h^2 (1 - 1e-6) every vote whose computed distance from the mean is
+// below h while the list is kept lies in the list (triangle inequality), and
+// the in-support set, its vote order and every sum are those of a full pass.
 __global__ void __launch_bounds__(kTrajThreads)
     k_trajectory(const double *desc, const uint8_t *kind, uint64_t V, int32_t seed, double h2,
-                 int max_iterations, uint32_t *support, int32_t *list_j, double *list_w,
-                 int32_t *touched_ids, uint32_t *touched_counts, TrajState *out) {
-  __shared__ double mean[12];
+                 double reach2, double keep2, int max_iterations, uint32_t *support,
+                 int32_t *cand, int32_t *list_j, double *list_w, int32_t *touched_ids,
+                 uint32_t *touched_counts, TrajState *out) {
+  __shared__ double mean[12], anchor[12];
   __shared__ double sums[13];
-  __shared__ uint32_t wsum[kTrajThreads / kWave + 1];
-  __shared__ uint32_t n_in, n_touched;
-  __shared__ int flag_exhausted, flag_converged, iters;
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  __shared__ double stage[13][kSumChunk];
+  __shared__ uint32_t wsum[kWave];
+  __shared__ uint32_t n_in, n_cand, n_touched;
+  __shared__ int flag_exhausted, flag_converged, iters, build;
+  const int tid = threadIdx.x;
   const bool refl = kind[seed] != 0;
   const int dim = refl ? 4 : 12;
   if (tid < 12)
@@ -51,74 +123,112 @@ __global__ void __launch_bounds__(kTrajThreads)
     flag_exhausted = flag_converged = 0;
     iters = 0;
   }
+  bool anchored = false;  // meaningful in thread 0 only
   __syncthreads();
   for (int it = 0; it < max_iterations; ++it) {
-    if (tid == 0)
+    if (tid == 0) {
       n_in = 0;
+      bool b = !anchored;
+      if (anchored) {
+        double drift = 0;
+        for (int k = 0; k < dim; ++k) {
+          const double d = mean[k] - anchor[k];
+          drift += d * d;
+        }
+        b = !(drift <= keep2);
+      }
+      build = b;
+      if (b)
+        n_cand = 0;
+      anchored = true;
+    }
     __syncthreads();
+    const bool rebuild = build;
     double m[12];
     for (int k = 0; k < 12; ++k)
       m[k] = mean[k];
-    for (uint64_t base = 0; base < V; base += kTrajThreads) {
-      uint64_t j = base + tid;
-      bool in = false;
-      double w = 0;
-      if (j < V && (kind[j] != 0) == refl) {
-        double s = 0;
-        for (int k = 0; k < dim; ++k) {
-          double d = m[k] - desc[static_cast<uint64_t>(k) * V + j];
-          s += d * d;
+    if (rebuild) {
+      // candidate list: the same-kind votes within reach of the mean, in vote order
+      for (uint64_t base = 0; base < V; base += kTrajPer * kTrajThreads) {
+        bool in[kTrajPer];
+        uint32_t pos[kTrajPer];
+#pragma unroll
+        for (int q = 0; q < kTrajPer; ++q) {
+          const uint64_t j = base + static_cast<uint64_t>(q * kTrajThreads + tid);
+          in[q] = j < V && (kind[j] != 0) == refl && vote_d2(desc, V, j, m, dim) < reach2;
         }
-        if (s < h2) {
-          in = true;
-          w = 1 - s / h2;
-        }
+        ordered_round(in, pos, wsum, &n_cand);
+#pragma unroll
+        for (int q = 0; q < kTrajPer; ++q)
+          if (in[q])
+            cand[pos[q]] = static_cast<int32_t>(base + static_cast<uint64_t>(q * kTrajThreads + tid));
       }
-      // ordered compaction (vote order) of the in-support votes
-      uint64_t mask = ballot(in);
-      if (lane == 0)
-        wsum[wave] = static_cast<uint32_t>(__popcll(mask));
-      __syncthreads();
-      if (tid == 0) {
-        uint32_t run = n_in;
-        for (int q = 0; q < kTrajThreads / kWave; ++q) {
-          uint32_t c = wsum[q];
-          wsum[q] = run;
-          run += c;
-        }
-        wsum[kTrajThreads / kWave] = run;
-      }
-      __syncthreads();
-      if (in) {
-        uint32_t pos = wsum[wave] + static_cast<uint32_t>(lanes_below(mask));
-        list_j[pos] = static_cast<int32_t>(j);
-        list_w[pos] = w;
-        if (support[j]++ == 0) {
-          uint32_t t = atomicAdd(&n_touched, 1u);
-          touched_ids[t] = static_cast<int32_t>(j);
-        }
-      }
-      __syncthreads();
-      if (tid == 0)
-        n_in = wsum[kTrajThreads / kWave];
+      if (tid < 12)
+        anchor[tid] = m[tid];
+      __threadfence_block();
       __syncthreads();
     }
-    // ordered sums: lane 0 the weight, lane k the k-th weighted component
+    // in-support votes among the candidates, compacted in vote order
+    const uint32_t C = n_cand;
+    for (uint32_t base = 0; base < C; base += kTrajPer * kTrajThreads) {
+      bool in[kTrajPer];
+      uint32_t pos[kTrajPer];
+      int32_t jj[kTrajPer];
+      double w[kTrajPer];
+#pragma unroll
+      for (int q = 0; q < kTrajPer; ++q) {
+        const uint32_t e = base + static_cast<uint32_t>(q * kTrajThreads + tid);
+        in[q] = false;
+        jj[q] = 0;
+        w[q] = 0;
+        if (e < C) {
+          jj[q] = cand[e];
+          const double s = vote_d2(desc, V, static_cast<uint64_t>(jj[q]), m, dim);
+          if (s < h2) {
+            in[q] = true;
+            w[q] = 1 - s / h2;
+          }
+        }
+      }
+      ordered_round(in, pos, wsum, &n_in);
+#pragma unroll
+      for (int q = 0; q < kTrajPer; ++q)
+        if (in[q]) {
+          list_j[pos[q]] = jj[q];
+          list_w[pos[q]] = w[q];
+          if (support[jj[q]]++ == 0) {
+            const uint32_t t = atomicAdd(&n_touched, 1u);
+            touched_ids[t] = jj[q];
+          }
+        }
+    }
+    // ordered sums: lane 0 the weights, lane k the k-th weighted components.
+    // The whole workgroup stages a chunk of products in LDS, then each of
+    // the dim + 1 lanes adds its row in vote order, so the sequential chains
+    // wait on LDS reads only.
     __threadfence_block();
     __syncthreads();
     const uint32_t N = n_in;
-    if (tid <= dim) {
-      double acc = 0;
-      if (tid == 0) {
-        for (uint32_t e = 0; e < N; ++e)
-          acc += list_w[e];
-      } else {
-        const double *col = desc + static_cast<uint64_t>(tid - 1) * V;
-        for (uint32_t e = 0; e < N; ++e)
-          acc += list_w[e] * col[list_j[e]];
+    const int cols = dim + 1;
+    double acc = 0;
+    for (uint32_t c0 = 0; c0 < N; c0 += kSumChunk) {
+      const uint32_t n = N - c0 < kSumChunk ? N - c0 : kSumChunk;
+      for (int x = tid; x < cols * kSumChunk; x += kTrajThreads) {
+        const int k = x / kSumChunk, e = x % kSumChunk;
+        if (static_cast<uint32_t>(e) < n) {
+          const double w = list_w[c0 + e];
+          stage[k][e] =
+              k == 0 ? w : w * desc[static_cast<uint64_t>(k - 1) * V + list_j[c0 + e]];
+        }
       }
-      sums[tid] = acc;
+      __syncthreads();
+      if (tid < cols)
+        for (uint32_t e = 0; e < n; ++e)
+          acc += stage[tid][e];
+      __syncthreads();
     }
+    if (tid < cols)
+      sums[tid] = acc;
     __syncthreads();
     if (tid == 0) {
       iters = it + 1;
@@ -229,6 +339,10 @@ static void clustering_run(Session *s, uint64_t V, const uint8_t *kind_h, const 
   int32_t *touched = scratch_as<int32_t>(s, kSlotMs3, V ? V : 1);
   uint32_t *tcount = scratch_as<uint32_t>(s, kSlotMs4, V ? V : 1);
   TrajState *st = scratch_as<TrajState>(s, kSlotMs5, 1);
+  int32_t *cand = scratch_as<int32_t>(s, kSlotMs6, V ? V : 1);
+  const double reach = (1 + kAnchorRadius) * h;
+  const double reach2 = reach * reach * (1 + 1e-6);
+  const double keep2 = kAnchorRadius * kAnchorRadius * h2 * (1 - 1e-6);
   if (V)
     zero_device(s, support, V * sizeof(uint32_t));
   struct Center {
@@ -250,8 +364,8 @@ static void clustering_run(Session *s, uint64_t V, const uint8_t *kind_h, const 
     bool refl = kind_h[seed] != 0;
     int dim = refl ? 4 : 12;
     MVS_LAUNCH(s, "ms_trajectory", k_trajectory, 1, kTrajThreads, 0, desc_d, kind_d, V,
-               static_cast<int32_t>(seed), h2, max_iterations, support, list_j, list_w, touched,
-               tcount, st);
+               static_cast<int32_t>(seed), h2, reach2, keep2, max_iterations, support, cand,
+               list_j, list_w, touched, tcount, st);
     copy_d2h(s, &hs, st, sizeof hs);
     uint32_t T = static_cast<uint32_t>(hs.touched);
     ids.resize(T);
