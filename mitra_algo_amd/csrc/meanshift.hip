@@ -41,6 +41,7 @@ static_assert(kTrajPer * kTrajWaves == kWave, "one wave scans the round's counts
 constexpr double kAnchorRadius = 1.0;
 // products staged per chunk of the ordered sums (13 x 512 doubles of LDS)
 constexpr int kSumChunk = 512;
+static_assert(kTrajThreads == 2 * kSumChunk, "two staging threads per chunk entry");
 
 struct TrajState {
   double mean[12];
@@ -96,7 +97,9 @@ __device__ inline double vote_d2(const double *desc, uint64_t V, uint64_t j, con
 }
 
 // support[] must be zero on entry for every vote; on exit the touched votes
-// are listed in touched_ids/touched_counts and their support entries reset.
+// are listed in out_ids/out_counts and their support entries reset.  out,
+// out_ids and out_counts are pinned host memory: the host reads them after
+// one stream synchronisation, with no copies.
 // Exactness of the candidate list: distances carry a relative rounding error
 // below 2e-15, so with reach2 = (1 + r)^2 h^2 (1 + 1e-6) and keep2 =
 // r^2 h^2 (1 - 1e-6) every vote whose computed distance from the mean is
@@ -106,7 +109,7 @@ __global__ void __launch_bounds__(kTrajThreads)
     k_trajectory(const double *desc, const uint8_t *kind, uint64_t V, int32_t seed, double h2,
                  double reach2, double keep2, int max_iterations, uint32_t *support,
                  int32_t *cand, int32_t *list_j, double *list_w, int32_t *touched_ids,
-                 uint32_t *touched_counts, TrajState *out) {
+                 int32_t *out_ids, uint32_t *out_counts, TrajState *out) {
   __shared__ double mean[12], anchor[12];
   __shared__ double sums[13];
   __shared__ double stage[13][kSumChunk];
@@ -213,18 +216,41 @@ __global__ void __launch_bounds__(kTrajThreads)
     double acc = 0;
     for (uint32_t c0 = 0; c0 < N; c0 += kSumChunk) {
       const uint32_t n = N - c0 < kSumChunk ? N - c0 : kSumChunk;
-      for (int x = tid; x < cols * kSumChunk; x += kTrajThreads) {
-        const int k = x / kSumChunk, e = x % kSumChunk;
+      {
+        // two threads per entry, each with its entry's vote and weight and
+        // every other component, so all of a thread's gathers are in flight
+        const int e = tid % kSumChunk, half = tid / kSumChunk;
         if (static_cast<uint32_t>(e) < n) {
+          const uint64_t j = static_cast<uint64_t>(list_j[c0 + e]);
           const double w = list_w[c0 + e];
-          stage[k][e] =
-              k == 0 ? w : w * desc[static_cast<uint64_t>(k - 1) * V + list_j[c0 + e]];
+          if (half == 0)
+            stage[0][e] = w;
+#pragma unroll
+          for (int u = 0; u < 6; ++u) {
+            const int k = 1 + half + 2 * u;
+            if (k < cols)
+              stage[k][e] = w * desc[static_cast<uint64_t>(k - 1) * V + j];
+          }
         }
       }
       __syncthreads();
-      if (tid < cols)
-        for (uint32_t e = 0; e < n; ++e)
-          acc += stage[tid][e];
+      if (tid < cols) {
+        // 16 reads in flight, then 16 adds in order: one LDS round trip per
+        // 16 entries instead of one per entry
+        const double *row = stage[tid];
+        uint32_t e = 0;
+        for (; e + 16 <= n; e += 16) {
+          double v[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            v[u] = row[e + u];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            acc += v[u];
+        }
+        for (; e < n; ++e)
+          acc += row[e];
+      }
       __syncthreads();
     }
     if (tid < cols)
@@ -257,8 +283,9 @@ __global__ void __launch_bounds__(kTrajThreads)
   __syncthreads();
   const uint32_t T = n_touched;
   for (uint32_t e = tid; e < T; e += kTrajThreads) {
-    int32_t j = touched_ids[e];
-    touched_counts[e] = support[j];
+    const int32_t j = touched_ids[e];
+    out_ids[e] = j;
+    out_counts[e] = support[j];
     support[j] = 0;
   }
   if (tid == 0) {
@@ -337,8 +364,16 @@ static void clustering_run(Session *s, uint64_t V, const uint8_t *kind_h, const 
   int32_t *list_j = scratch_as<int32_t>(s, kSlotMs1, V ? V : 1);
   double *list_w = scratch_as<double>(s, kSlotMs2, V ? V : 1);
   int32_t *touched = scratch_as<int32_t>(s, kSlotMs3, V ? V : 1);
-  uint32_t *tcount = scratch_as<uint32_t>(s, kSlotMs4, V ? V : 1);
-  TrajState *st = scratch_as<TrajState>(s, kSlotMs5, 1);
+  // pinned results of one trajectory: state, touched ids, their counts
+  const uint64_t vcap = V ? V : 1;
+  const size_t ids_at = (sizeof(TrajState) + 255) / 256 * 256;
+  const size_t counts_at = ids_at + (vcap * sizeof(int32_t) + 255) / 256 * 256;
+  char *pin = static_cast<char *>(host_stage(s, counts_at + vcap * sizeof(uint32_t)));
+  char *pin_dev = nullptr;
+  MVS_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&pin_dev), pin, 0));
+  const TrajState &hs = *reinterpret_cast<const TrajState *>(pin);
+  const int32_t *pin_ids = reinterpret_cast<const int32_t *>(pin + ids_at);
+  const uint32_t *pin_counts = reinterpret_cast<const uint32_t *>(pin + counts_at);
   int32_t *cand = scratch_as<int32_t>(s, kSlotMs6, V ? V : 1);
   const double reach = (1 + kAnchorRadius) * h;
   const double reach2 = reach * reach * (1 + 1e-6);
@@ -355,9 +390,6 @@ static void clustering_run(Session *s, uint64_t V, const uint8_t *kind_h, const 
   Fenwick fw(static_cast<int64_t>(V));
   uint64_t support_entries = 0;
   int32_t limited = 0;
-  std::vector<int32_t> ids;
-  std::vector<uint32_t> cnts;
-  TrajState hs;
   while (fw.total > 0) {
     uint64_t r = mt64_next(rng) % static_cast<uint64_t>(fw.total);
     int64_t seed = fw.kth(static_cast<int64_t>(r));
@@ -365,16 +397,16 @@ static void clustering_run(Session *s, uint64_t V, const uint8_t *kind_h, const 
     int dim = refl ? 4 : 12;
     MVS_LAUNCH(s, "ms_trajectory", k_trajectory, 1, kTrajThreads, 0, desc_d, kind_d, V,
                static_cast<int32_t>(seed), h2, reach2, keep2, max_iterations, support, cand,
-               list_j, list_w, touched, tcount, st);
-    copy_d2h(s, &hs, st, sizeof hs);
-    uint32_t T = static_cast<uint32_t>(hs.touched);
-    ids.resize(T);
-    cnts.resize(T);
-    copy_d2h(s, ids.data(), touched, T * sizeof(int32_t));
-    copy_d2h(s, cnts.data(), tcount, T * sizeof(uint32_t));
+               list_j, list_w, touched, reinterpret_cast<int32_t *>(pin_dev + ids_at),
+               reinterpret_cast<uint32_t *>(pin_dev + counts_at),
+               reinterpret_cast<TrajState *>(pin_dev));
+    sync(s);
+    const uint32_t T = static_cast<uint32_t>(hs.touched);
+    if (T > V)
+      raise(MVS_ERUNTIME, "mean-shift trajectory touched more votes than exist");
     std::vector<Entry> sup(T);
     for (uint32_t e = 0; e < T; ++e)
-      sup[e] = {ids[e], cnts[e]};
+      sup[e] = {pin_ids[e], pin_counts[e]};
     std::sort(sup.begin(), sup.end(), [](const Entry &a, const Entry &b) { return a.vote < b.vote; });
     for (const Entry &e : sup)
       if (!visited[e.vote]) {

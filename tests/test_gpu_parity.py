@@ -250,6 +250,51 @@ def test_clustering_fixtures(sess, oracle, count):
         assert np.array_equal(b["rotation"], c["rotation"])
 
 
+def _ramp_votes(count: int):
+    """Votes whose density rises linearly along one descriptor axis (rigid:
+    component 9, mirror: component 3, values sqrt(u) on [0, 1]) with the
+    other components fixed, so mean-shift trajectories climb the ramp by a
+    few percent of the bandwidth per iteration and end far from their seeds:
+    every trajectory's candidate list is rebuilt several times."""
+    u = (np.arange(count) * 0.6180339887498949) % 1.0
+    x = np.sqrt(u)
+    kind = (np.arange(count) % 3 == 2).astype(np.uint8)
+    desc = np.zeros((12, count))
+    rot = np.zeros((9, count))
+    ident = np.eye(3).reshape(9)
+    for i in range(count):
+        if kind[i]:
+            rot[:, i] = np.diag([-1.0, 1.0, 1.0]).reshape(9)
+            desc[0:4, i] = (1.0, 0.0, 0.0, x[i])
+        else:
+            rot[:, i] = ident
+            desc[0:9, i] = ident / 2
+            desc[9, i] = x[i]
+            desc[10, i] = 1e-3 * np.sin(i)
+    return kind, desc, rot
+
+
+def test_clustering_drifting_trajectories(sess, oracle):
+    kind, desc, rot = _ramp_votes(3000)
+    h = 0.05
+    # the fixture does what it is for: from the lowest rigid vote a
+    # trajectory under the same weights moves more than 2h in 100 iterations
+    rig = desc[:, kind == 0]
+    m = rig[:, np.argmin(rig[9])].copy()
+    start = m.copy()
+    for _ in range(100):
+        d2 = ((rig - m[:, None]) ** 2).sum(0)
+        w = np.where(d2 < h * h, 1 - d2 / (h * h), 0.0)
+        m = (rig * w).sum(1) / w.sum()
+    assert np.sqrt(((m - start) ** 2).sum()) > 2 * h
+    for iters in (100, 300):
+        g1, g2 = oracle.rng(7), capi.rng(7)
+        a = oracle.clustering(kind, desc, rot, 4.0, [0.5, 0, 0], h, iters, 20, g1)
+        b = sess.clustering(kind, desc, rot, 4.0, [0.5, 0, 0], h, iters, 20, g2)
+        assert bytes(g1) == bytes(g2)
+        _cmp_clusters(a, b)
+
+
 # ------------------------------------------------------------ end to end
 @pytest.mark.parametrize("name", EXAMPLES)
 def test_end_to_end_examples(sess, oracle, golden_dir, name):
