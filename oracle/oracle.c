@@ -860,7 +860,10 @@ done:
 }
 
 /* ------------------------------------------------ batched-seed mean-shift ---
- * No reference counterpart (the reference runs sequential seeds only). */
+ * No reference counterpart (the reference runs sequential seeds only).  Each
+ * seed follows m <- m + sum w (x - m) / sum w with the shifted, truncated
+ * Gaussian weight w = exp(-d^2 / 2h^2) - exp(-c^2 / 2) for d < c h, which is
+ * the mean-shift ascent of the density with profile derivative -w. */
 int oracle_meanshift_batched(int64_t nv, int dim, const double *pts,
                              const uint8_t *kind, int dim_rigid, int dim_refl,
                              int64_t ns, const double *seeds,

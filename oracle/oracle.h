@@ -41,9 +41,17 @@
  *                        Backend::verification on the example meshes
  *                        (sides, patches, rms, coverage); mirror and half-turn
  *                        closed forms on exactly symmetric grids.
- *   oracle_meanshift_batched  has no reference counterpart: parity unpinned
- *                        beyond closed forms (single tight cluster converges
- *                        to the kernel-weighted mean).
+ *   oracle_meanshift_batched  has no reference counterpart (the reference's
+ *                        seeds are sequential).  Pinned by what the
+ *                        mathematics fixes (tests/test_oracle_meanshift.py):
+ *                        converged modes are stationary points and local
+ *                        maxima of the density whose profile derivative is
+ *                        the weight (FP64 finite differences of the density
+ *                        alone), the density never decreases along the
+ *                        iterates, symmetric point sets keep a seed on the
+ *                        centre or symmetry hyperplane exactly, points at
+ *                        (1 -/+ 1e-9) c h are inside / outside the support,
+ *                        and each kind sees only its own votes and lanes.
  */
 #ifndef MVS_ORACLE_H
 #define MVS_ORACLE_H
