@@ -368,6 +368,10 @@ typedef struct {
   const void *refl_modes, *refl_weight;    /* 4 x refl_seeds SoA, refl_seeds */
   uint64_t candidates, in_support;
   int precision;
+  /* want_stats only: updates computed over all seeds (a seed stops once an
+   * update leaves it unchanged), and the visit counts of each kind */
+  uint64_t updates;
+  uint64_t rigid_candidates, rigid_in_support, refl_candidates, refl_in_support;
 } mvs_meanshift_result;
 int mvs_dev_meanshift_votes(mvs_session *s, uint64_t seeds, uint64_t seed_value, int iterations,
                             double h, double cutoff, int want_stats, mvs_meanshift_result *out);
@@ -443,6 +447,28 @@ int mvs_batch_graph_capture(mvs_session *s, uint64_t n, const double *x, const d
                             mvs_batch_meanshift_result *ms);
 int mvs_batch_graph_launch(mvs_session *s);
 int mvs_batch_graph_check(mvs_session *s);
+
+/* One one-mesh step (mvs_dev_pairvote then mvs_dev_meanshift_votes with these
+ * arguments, want_stats off) as a HIP graph owned by the session, with the
+ * same record / capture / check rules as mvs_batch_graph_capture: the counts
+ * the step reads back (pair total, long sources, mirror total, mean-shift
+ * flags) are frozen, so replays are for the captured inputs.  The two
+ * mean-shift queries keep their fork onto the side stream inside the graph.
+ * mvs_graph_launch / mvs_graph_check replay and check whichever step the
+ * session holds (the batch ones are the same calls).  No reference
+ * counterpart: the reference runs every stage eagerly
+ * (src/core/pipeline.cpp:5-52); a caller that repeats a step over resident
+ * inputs (a benchmark, a serving loop) replays it without host
+ * synchronisations or per-kernel launches. */
+int mvs_step_graph_capture(mvs_session *s, uint64_t n, const double *x, const double *y,
+                           const double *z, const double *feat, uint64_t S,
+                           const int32_t *selected, double box, const double center[3],
+                           double threshold, uint64_t max_pairs, uint64_t max_votes,
+                           int precision, uint64_t seeds, uint64_t seed_value, int iterations,
+                           double h, double cutoff, mvs_pairvote_result *pv,
+                           mvs_meanshift_result *ms);
+int mvs_graph_launch(mvs_session *s);
+int mvs_graph_check(mvs_session *s);
 
 /* Launch table of a session: pair-walk work quota (sources a persistent wave
  * claims at a time; 0 = one wave per source) and resident blocks, and the
@@ -521,14 +547,16 @@ int mvs_dev_align_energy(mvs_session *s, int kind, uint32_t modes, uint32_t npts
  * it unchanged (a fixed point: further updates would repeat it exactly).
  * Seeds and out: dim*ns SoA; weight: ns (sum of weights at the result).
  * MVS_FP32_GUARDED selects FP32 points and accumulation, MVS_FP64 FP64.
- * candidates / in_support (optional) receive the numbers of point visits and
- * of visits inside the support (synchronises when requested). */
+ * candidates / in_support / updates (optional) receive the numbers of point
+ * visits, of visits inside the support and of updates computed (seeds stop
+ * at fixed points, so at most ns * iterations). */
 int mvs_dev_ms_points(mvs_session *s, int precision, uint64_t nrigid, const void *q4,
                       const void *t3, uint64_t nrefl, const void *n3, const void *d1,
                       void *rigid_pts, void *refl_pts);
 int mvs_dev_ms_batched(mvs_session *s, int precision, int dim, uint64_t n, const void *pts,
                        uint64_t ns, const void *seeds, int iterations, double h, double cutoff,
-                       void *out, void *weight, uint64_t *candidates, uint64_t *in_support);
+                       void *out, void *weight, uint64_t *candidates, uint64_t *in_support,
+                       uint64_t *updates);
 
 #ifdef __cplusplus
 }

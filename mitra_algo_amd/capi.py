@@ -33,6 +33,7 @@ EXPORTS = [
     "mvs_dev_closest_modes", "mvs_dev_align_energy",
     "mvs_verification", "mvs_verification_fetch", "mvs_verification_rows",
     "mvs_verification_stats", "mvs_verification_surface", "mvs_dev_verification",
+    "mvs_step_graph_capture", "mvs_graph_launch", "mvs_graph_check",
 ]
 
 
@@ -132,7 +133,10 @@ class MeanShiftResult(C.Structure):
                 ("seed_votes", C.c_void_p), ("rigid_modes", C.c_void_p),
                 ("rigid_weight", C.c_void_p), ("refl_modes", C.c_void_p),
                 ("refl_weight", C.c_void_p), ("candidates", C.c_uint64),
-                ("in_support", C.c_uint64), ("precision", C.c_int)]
+                ("in_support", C.c_uint64), ("precision", C.c_int),
+                ("updates", C.c_uint64), ("rigid_candidates", C.c_uint64),
+                ("rigid_in_support", C.c_uint64), ("refl_candidates", C.c_uint64),
+                ("refl_in_support", C.c_uint64)]
 
 
 class TuneEntry(C.Structure):
@@ -639,6 +643,26 @@ class Session:
                                          C.c_uint32(npts), ptr(theta), ptr(base), ptr(pts),
                                          ptr(q), ptr(nrm), ptr(energy), ptr(grad), ptr(hess)))
 
+    def step_graph_capture(self, n, x, y, z, feat, S, sel, box, center, t, max_pairs, max_votes,
+                           precision, seeds, seed_value, iterations, h, cutoff):
+        """Captures one pair + vote + mean-shift step as a HIP graph (see
+        mvs_step_graph_capture); returns the recorded run's results, whose
+        buffers every replay rewrites."""
+        pv, ms = PairVoteResult(), MeanShiftResult()
+        ctr = (C.c_double * 3)(*[float(c) for c in center])
+        check(lib().mvs_step_graph_capture(
+            self.h, C.c_uint64(n), ptr(x), ptr(y), ptr(z), ptr(feat), C.c_uint64(S), ptr(sel),
+            C.c_double(box), ctr, C.c_double(t), C.c_uint64(max_pairs), C.c_uint64(max_votes),
+            C.c_int(precision), C.c_uint64(seeds), C.c_uint64(seed_value), C.c_int(iterations),
+            C.c_double(h), C.c_double(cutoff), C.byref(pv), C.byref(ms)))
+        return pv, ms
+
+    def graph_launch(self):
+        check(lib().mvs_graph_launch(self.h))
+
+    def graph_check(self):
+        check(lib().mvs_graph_check(self.h))
+
     def batch_graph_launch(self):
         check(lib().mvs_batch_graph_launch(self.h))
 
@@ -690,12 +714,15 @@ class Session:
 
     def dev_ms_batched(self, precision, dim, n, pts, ns, seeds, iterations, h, cutoff, out,
                        weight, stats: bool = False):
-        cand, ins = C.c_uint64(), C.c_uint64()
+        cand, ins, upd = C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(lib().mvs_dev_ms_batched(self.h, C.c_int(precision), C.c_int(dim), C.c_uint64(n),
                                        ptr(pts), C.c_uint64(ns), ptr(seeds), C.c_int(iterations),
                                        C.c_double(h), C.c_double(cutoff), ptr(out), ptr(weight),
                                        C.byref(cand) if stats else None,
-                                       C.byref(ins) if stats else None))
+                                       C.byref(ins) if stats else None,
+                                       C.byref(upd) if stats else None))
+        if stats == "updates":
+            return cand.value, ins.value, upd.value
         return (cand.value, ins.value) if stats else None
 
     def dev_vote_count(self, P, a, b, precision) -> int:

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 
 #include "pairing.hpp"
@@ -259,7 +260,7 @@ void batch_meanshift(Session *s, const uint64_t *soff_host, double ratio, uint64
   const uint64_t salt_r = splitmix64(seed_value);
   const uint64_t salt_f = splitmix64(seed_value ^ 0x6d6972726f72ULL);  // "mirror"
   ms_begin(s);
-  const int slot_r = want_stats ? 0 : -1, slot_f = want_stats ? 2 : -1;
+  const int slot_r = want_stats ? 0 : -1, slot_f = want_stats ? 1 : -1;
   if (prec == MVS_FP32_GUARDED) {
     if (cr)
       ms_batched_f32(s, 6, nr, static_cast<const float *>(rpts), cr, nullptr, ids, salt_r,
@@ -279,7 +280,7 @@ void batch_meanshift(Session *s, const uint64_t *soff_host, double ratio, uint64
                      iterations, h, cutoff, static_cast<double *>(ofl),
                      static_cast<double *>(owf), slot_f, &sf);
   }
-  unsigned long long sums[4] = {0, 0, 0, 0};
+  unsigned long long sums[kMsSums] = {0, 0, 0, 0, 0, 0};
   ms_finish(s, sums);
   mvs_meanshift_result m{};
   m.rigid_seeds = cr;
@@ -289,8 +290,13 @@ void batch_meanshift(Session *s, const uint64_t *soff_host, double ratio, uint64
   m.rigid_weight = owr;
   m.refl_modes = ofl;
   m.refl_weight = owf;
-  m.candidates = sums[0] + sums[2];
-  m.in_support = sums[1] + sums[3];
+  m.candidates = sums[0] + sums[3];
+  m.in_support = sums[1] + sums[4];
+  m.updates = sums[2] + sums[5];
+  m.rigid_candidates = sums[0];
+  m.rigid_in_support = sums[1];
+  m.refl_candidates = sums[3];
+  m.refl_in_support = sums[4];
   m.precision = prec;
   out->all = m;
   out->meshes = meshes;
@@ -325,9 +331,9 @@ static void batch_step(Session *s, const BatchStepArgs &a, mvs_batch_pairvote_re
 }
 
 // Records the step eagerly, then captures it on the session's own stream with
-// the recorded counts frozen (see Replay).
-static void batch_graph_capture(Session *s, const BatchStepArgs &a, mvs_batch_pairvote_result *pv,
-                                mvs_batch_meanshift_result *ms) {
+// the recorded counts frozen (see Replay).  `step(false)` is the recorded run
+// (its results are the caller's), `step(true)` the captured one.
+void graph_capture(Session *s, const std::function<void(bool)> &step) {
   s->drop_graph();
   hipStream_t user = s->stream;
   MVS_HIP(hipStreamSynchronize(user));  // the caller's inputs are ready
@@ -346,7 +352,7 @@ static void batch_graph_capture(Session *s, const BatchStepArgs &a, mvs_batch_pa
   s->timing = false;
   s->replay = Replay::kRecord;
   s->rec.clear();
-  batch_step(s, a, pv, ms);
+  step(false);
   // recorded values on the device, mismatch word cleared
   s->rec_off.clear();
   size_t total = 0;
@@ -369,9 +375,7 @@ static void batch_graph_capture(Session *s, const BatchStepArgs &a, mvs_batch_pa
   MVS_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
   hipGraph_t graph = nullptr;
   try {
-    mvs_batch_pairvote_result pv2{};
-    mvs_batch_meanshift_result ms2{};
-    batch_step(s, a, &pv2, &ms2);
+    step(true);
   } catch (...) {
     (void)hipStreamEndCapture(s->stream, &graph);
     if (graph)
@@ -387,9 +391,22 @@ static void batch_graph_capture(Session *s, const BatchStepArgs &a, mvs_batch_pa
   }
 }
 
+static void batch_graph_capture(Session *s, const BatchStepArgs &a, mvs_batch_pairvote_result *pv,
+                                mvs_batch_meanshift_result *ms) {
+  graph_capture(s, [&](bool captured) {
+    if (!captured) {
+      batch_step(s, a, pv, ms);
+      return;
+    }
+    mvs_batch_pairvote_result pv2{};
+    mvs_batch_meanshift_result ms2{};
+    batch_step(s, a, &pv2, &ms2);
+  });
+}
+
 // Replays the captured step on the session's own stream, ordered after the
 // caller's stream and before its later work.
-static void batch_graph_launch(Session *s) {
+void graph_launch(Session *s) {
   if (!s->graph_exec)
     raise(MVS_EINVAL, "no captured batch step on this session");
   if (!s->graph_events[0]) {
@@ -408,7 +425,7 @@ static void batch_graph_launch(Session *s) {
   }
 }
 
-static void batch_graph_check(Session *s) {
+void graph_check(Session *s) {
   if (!s->graph_exec)
     raise(MVS_EINVAL, "no captured batch step on this session");
   MVS_HIP(hipStreamSynchronize(s->own_stream));
@@ -454,8 +471,8 @@ static void batch_autotune(Session *s, const BatchStepArgs &a, mvs_tune_entry *t
   for (uint32_t c = 0; c < n; ++c) {
     s->tune = kCandidates[c];
     batch_graph_capture(s, a, pv, ms);
-    batch_graph_launch(s);  // warm
-    batch_graph_check(s);
+    graph_launch(s);  // warm
+    graph_check(s);
     float t[3];
     for (int r = 0; r < 3; ++r) {
       MVS_HIP(hipEventRecord(ev[0], s->own_stream));
@@ -464,7 +481,7 @@ static void batch_autotune(Session *s, const BatchStepArgs &a, mvs_tune_entry *t
       MVS_HIP(hipEventSynchronize(ev[1]));
       MVS_HIP(hipEventElapsedTime(&t[r], ev[0], ev[1]));
     }
-    batch_graph_check(s);
+    graph_check(s);
     std::sort(t, t + 3);
     table[c].walk_quota = kCandidates[c].walk_quota;
     table[c].walk_blocks = kCandidates[c].walk_blocks;
@@ -577,11 +594,11 @@ int mvs_session_set_tuning(mvs_session *sp, const mvs_tune_entry *in) {
 }
 
 int mvs_batch_graph_launch(mvs_session *sp) {
-  return guarded([&] { batch_graph_launch(as_session(sp)); });
+  return guarded([&] { graph_launch(as_session(sp)); });
 }
 
 int mvs_batch_graph_check(mvs_session *sp) {
-  return guarded([&] { batch_graph_check(as_session(sp)); });
+  return guarded([&] { graph_check(as_session(sp)); });
 }
 
 }  // extern "C"

@@ -87,7 +87,7 @@ static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, in
   ms_points_device(s, prec, kr ? nr : 0, r.rigid_q4, r.rigid_t3, kf ? nf : 0, r.refl_n3,
                    r.refl_stride, r.refl_d, rpts, fpts);
   ms_begin(s);
-  const int slot_r = want_stats ? 0 : -1, slot_f = want_stats ? 2 : -1;
+  const int slot_r = want_stats ? 0 : -1, slot_f = want_stats ? 1 : -1;
   // the mirror kind first, its query on the side stream, beside the rigid
   // set-up and query (the two kinds share nothing the queries read); the
   // mirror query is the longer one, so it starts as early as it can
@@ -111,7 +111,7 @@ static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, in
                      iterations, h, cutoff, static_cast<double *>(orig),
                      static_cast<double *>(owr), slot_r, nullptr, kf ? second : MsRun{});
   }
-  unsigned long long sums[4] = {0, 0, 0, 0};
+  unsigned long long sums[kMsSums] = {0, 0, 0, 0, 0, 0};
   ms_finish(s, sums);
   mvs_meanshift_result m{};
   m.rigid_seeds = kr;
@@ -121,10 +121,43 @@ static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, in
   m.rigid_weight = owr;
   m.refl_modes = ofl;
   m.refl_weight = owf;
-  m.candidates = sums[0] + sums[2];
-  m.in_support = sums[1] + sums[3];
+  m.candidates = sums[0] + sums[3];
+  m.in_support = sums[1] + sums[4];
+  m.updates = sums[2] + sums[5];
+  m.rigid_candidates = sums[0];
+  m.rigid_in_support = sums[1];
+  m.refl_candidates = sums[3];
+  m.refl_in_support = sums[4];
   m.precision = prec;
   *out = m;
+}
+
+// One step of the voting core (pairs + votes, then mean-shift over them).
+struct StepArgs {
+  uint64_t n;
+  const double *x, *y, *z, *feat;
+  uint64_t S;
+  const int32_t *sel;
+  double box;
+  double center[3];
+  double threshold;
+  uint64_t max_pairs, max_votes;
+  int precision;
+  uint64_t seeds, seed_value;
+  int iterations;
+  double h, cutoff;
+};
+
+static void step_capture(Session *s, const StepArgs &a, mvs_pairvote_result *pv,
+                         mvs_meanshift_result *ms) {
+  graph_capture(s, [&](bool captured) {
+    mvs_pairvote_result pv2{};
+    mvs_meanshift_result ms2{};
+    pairvote(s, a.n, a.x, a.y, a.z, a.feat, a.S, a.sel, a.box, a.center, a.threshold, a.max_pairs,
+             a.max_votes, a.precision, captured ? &pv2 : pv, Segs{});
+    meanshift_votes(s, a.seeds, a.seed_value, a.iterations, a.h, a.cutoff, false,
+                    captured ? &ms2 : ms);
+  });
 }
 
 }  // namespace mvs
@@ -153,6 +186,29 @@ int mvs_dev_meanshift_votes(mvs_session *sp, uint64_t seeds, uint64_t seed_value
     meanshift_votes(as_session(sp), seeds, seed_value, iterations, h, cutoff, want_stats != 0,
                     out);
   });
+}
+
+int mvs_step_graph_capture(mvs_session *sp, uint64_t n, const double *x, const double *y,
+                           const double *z, const double *feat, uint64_t S, const int32_t *sel,
+                           double box, const double center[3], double t, uint64_t max_pairs,
+                           uint64_t max_votes, int precision, uint64_t seeds,
+                           uint64_t seed_value, int iterations, double h, double cutoff,
+                           mvs_pairvote_result *pv, mvs_meanshift_result *ms) {
+  return guarded([&] {
+    if (!pv || !ms || !center)
+      raise(MVS_EINVAL, "null argument");
+    StepArgs a{n, x, y, z, feat, S, sel, box, {center[0], center[1], center[2]}, t, max_pairs,
+               max_votes, precision, seeds, seed_value, iterations, h, cutoff};
+    step_capture(as_session(sp), a, pv, ms);
+  });
+}
+
+int mvs_graph_launch(mvs_session *sp) {
+  return guarded([&] { graph_launch(as_session(sp)); });
+}
+
+int mvs_graph_check(mvs_session *sp) {
+  return guarded([&] { graph_check(as_session(sp)); });
 }
 
 }  // extern "C"

@@ -197,6 +197,7 @@ struct Grid {
   int perm[kMaxD];
   double lo[3];
   double cell;
+  double inv;  // 1 / cell: cell_of multiplies, in the keys and the queries alike
   int dims[3];
   int reach;
   uint64_t ncells;
@@ -206,29 +207,32 @@ constexpr uint32_t kMapPerWave = 1024;  // LDS candidate-map entries per wave
 constexpr int kMaxRows = (2 * kMaxReach + 1) * (2 * kMaxReach + 1);
 constexpr int kCellBits = 22;  // at most 2^22 cells: three 8-bit radix passes
 constexpr uint64_t kMaxCells = 1ULL << kCellBits;
-constexpr unsigned kBboxBlocks = 256;
+constexpr unsigned kBboxBlocks = 1024;
 
-// Device-side control block of one mean-shift call: the grid, visit sums of
-// up to two runs and the non-finite-input flag, read back once at the end.
+// Device-side control block of one mean-shift call: the grid, the sums of
+// up to two runs (run k: candidates, in-support visits, seed-iterations at
+// sums[3k..3k+2]) and the non-finite-input flag, read back once at the end.
 struct MsCtl {
   Grid grid;
-  unsigned long long sums[4];
+  unsigned long long sums[kMsSums];
   unsigned bad;
   Grid grid2;  // the second slot set's grid
 };
 
-// One wavefront per segment: reduces the segment's nb partial boxes and
-// moments and lays its grid on the three coordinates of largest spread
-// (variance; ties: lower index first).  Spread, not extent: a few outlying
-// votes can stretch a coordinate whose votes are otherwise concentrated.
+// One block per segment: reduces the segment's nb partial boxes and moments
+// (lanes over partials, then the four waves through LDS) and lays its grid
+// on the three coordinates of largest spread (variance; ties: lower index
+// first).  Spread, not extent: a few outlying votes can stretch a coordinate
+// whose votes are otherwise concentrated.
 // Cells have side >= cell0 / 2 when that grid has at most
 // clamp(2 n, 1024, kMaxCells) cells, else side cell0, doubled until it fits.
 // No points: a single cell.  ncells[m] (optional) receives the cell count.
 template <int D>
-__global__ void k_grid_setup(const double *partial, unsigned nb, const uint64_t *seg_off,
-                             uint64_t n, double cell0, int max_reach, Grid *grids,
-                             uint64_t *ncells, unsigned *bad) {
-  const int lane = threadIdx.x;
+__global__ void __launch_bounds__(kThreads)
+    k_grid_setup(const double *partial, unsigned nb, const uint64_t *seg_off, uint64_t n,
+                 double cell0, int max_reach, Grid *grids, uint64_t *ncells, unsigned *bad) {
+  __shared__ double red[kWavesPerBlock][4 * kMaxD];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const uint32_t mesh = blockIdx.x;
   partial += static_cast<uint64_t>(mesh) * nb * kPart;
   if (seg_off)
@@ -237,7 +241,7 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, const uint64_t 
   double lo[kMaxD], hi[kMaxD], sum[kMaxD], sq[kMaxD];
   for (int k = 0; k < kMaxD; ++k)
     lo[k] = INFINITY, hi[k] = -INFINITY, sum[k] = 0, sq[k] = 0;
-  for (unsigned b = lane; b < nb; b += kWave)
+  for (unsigned b = threadIdx.x; b < nb; b += kThreads)
     for (int k = 0; k < D; ++k) {
       lo[k] = fmin(lo[k], partial[b * kPart + k]);
       hi[k] = fmax(hi[k], partial[b * kPart + kMaxD + k]);
@@ -251,8 +255,23 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, const uint64_t 
       sum[k] += __shfl_xor(sum[k], o, kWave);
       sq[k] += __shfl_xor(sq[k], o, kWave);
     }
-  if (lane != 0)
+  if (lane == 0)
+    for (int k = 0; k < D; ++k) {
+      red[wave][k] = lo[k];
+      red[wave][kMaxD + k] = hi[k];
+      red[wave][2 * kMaxD + k] = sum[k];
+      red[wave][3 * kMaxD + k] = sq[k];
+    }
+  __syncthreads();
+  if (threadIdx.x != 0)
     return;
+  for (int w = 1; w < kWavesPerBlock; ++w)
+    for (int k = 0; k < D; ++k) {
+      lo[k] = fmin(lo[k], red[w][k]);
+      hi[k] = fmax(hi[k], red[w][kMaxD + k]);
+      sum[k] += red[w][2 * kMaxD + k];
+      sq[k] += red[w][3 * kMaxD + k];
+    }
   double var[kMaxD];
   for (int k = 0; k < D; ++k) {
     const double mean = n ? sum[k] / static_cast<double>(n) : 0.0;
@@ -263,6 +282,7 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, const uint64_t 
   for (int k = 0; k < kMaxD; ++k)
     g.perm[k] = k;
   g.cell = cell0;
+  g.inv = 1.0 / cell0;
   g.dims[0] = g.dims[1] = g.dims[2] = 1;
   g.reach = 1;
   g.ncells = 1;
@@ -303,8 +323,9 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, const uint64_t 
     // the grid fits
     auto fits = [&](double cell) {
       uint64_t cells = 1;
+      const double inv = 1.0 / cell;
       for (int k = 0; k < 3; ++k) {
-        const double d = floor(ext[k] / cell) + 1;
+        const double d = floor(ext[k] * inv) + 1;
         g.dims[k] = d < 2147483647.0 ? static_cast<int>(d) : 2147483647;
         cells = cells > cap ? cells : cells * static_cast<uint64_t>(g.dims[k]);
       }
@@ -327,23 +348,26 @@ __global__ void k_grid_setup(const double *partial, unsigned nb, const uint64_t 
     for (int k = 0; k < kMaxD; ++k)
       g.perm[k] = perm[k];
     g.cell = cell;
+    g.inv = 1.0 / cell;
   }
   grids[mesh] = g;
   if (ncells)
     ncells[mesh] = g.ncells;
 }
 
-// Cell index along one axis; NaN and underflow go to cell 0.
-__device__ inline int cell_of(double v, double lo, double cell, int dim) {
-  double c = floor((v - lo) / cell);
+// Cell index along one axis (inv = 1 / cell side); NaN and underflow go to
+// cell 0.  Keys and queries use this one formula, so they agree on every
+// point's cell; cells are monotone in v.
+__device__ inline int cell_of(double v, double lo, double inv, int dim) {
+  double c = floor((v - lo) * inv);
   if (!(c > 0))
     return 0;
   return c >= dim ? dim - 1 : static_cast<int>(c);
 }
 template <class R> __device__ inline uint64_t cell_key(const Grid &g, R a, R b, R c) {
-  int x = cell_of(static_cast<double>(a), g.lo[0], g.cell, g.dims[0]);
-  int y = cell_of(static_cast<double>(b), g.lo[1], g.cell, g.dims[1]);
-  int z = cell_of(static_cast<double>(c), g.lo[2], g.cell, g.dims[2]);
+  int x = cell_of(static_cast<double>(a), g.lo[0], g.inv, g.dims[0]);
+  int y = cell_of(static_cast<double>(b), g.lo[1], g.inv, g.dims[1]);
+  int z = cell_of(static_cast<double>(c), g.lo[2], g.inv, g.dims[2]);
   return (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0] + x;
 }
 
@@ -509,16 +533,17 @@ __global__ void k_seed_keys(const R *seeds, uint64_t ns, const Grid *grids, uint
 // candidates).  Seeds are sorted points (seed_pos, already in cell order) or
 // explicit positions (seeds, D x ns SoA, run in `order`).  All iterations
 // run in registers.  Per iteration the rows of the neighbourhood (each row
-// 2 reach + 1 cells, contiguous in the sorted order) are walked as one
-// flattened sequence of candidates: the row bounds sit in lanes 0..24, a
-// scalar cursor follows the row of the current 64-candidate window and lanes
-// past a row start take that row's base, so short rows do not leave lanes
-// idle; the next batch's records are in flight while the current
-// batch is tested, and batch k goes to wave k mod W.  With W = 4 the waves
-// add their partial sums through LDS in a fixed order, so every wave holds
-// the same new mean.  A seed whose update leaves every coordinate unchanged
-// has reached a fixed point of the iteration and stops (further iterations
-// would repeat it bit for bit).
+// 2 reach + 1 cells, contiguous in the sorted order) are flattened into one
+// candidate range: a map in LDS sends each flattened candidate to its
+// record, and the waves walk the range in 64-candidate batches with the next
+// batches' records in flight (batch k goes to wave k mod W).  With W = 4 the
+// first wave alone finds the rows (the mean's cell, the row ranges cut to the
+// support ball, their prefix sum) and leaves them in LDS; each wave fills
+// the map for its share of the rows; after the walk the waves' partial sums
+// meet in LDS lane by lane in a fixed order, the first wave reduces them and
+// updates the mean, and the others read the new mean back.  A seed whose
+// update leaves every coordinate unchanged has reached a fixed point of the
+// iteration and stops (further iterations would repeat it bit for bit).
 template <class R, int D, int W>
 __global__ void __launch_bounds__(kThreads)
     k_ms_query(const typename Vec<R>::V4 *p4, const typename Vec<R>::V2 *q2,
@@ -528,15 +553,23 @@ __global__ void __launch_bounds__(kThreads)
                uint32_t sentinel, int iterations, R gscale, R r2, R floor_w, R *out, R *weight,
                unsigned long long *stats) {
   static_assert(W == 1 || W == kWavesPerBlock, "one seed per wave or per block");
-  __shared__ R part[kWavesPerBlock][D + 1];
-  // candidate -> record map: one slice per wave (W = 1) or one per block
-  constexpr uint32_t kMapCap = W == 1 ? kMapPerWave : kMapPerWave * kWavesPerBlock;
-  __shared__ uint32_t cmap[kMapPerWave * kWavesPerBlock];
+  // W = 1: candidate -> record map, one slice per wave.  W = 4: the
+  // non-empty rows of this iteration (first record, length, first batch;
+  // entry [nrc] of the batch starts holds the total), the waves' lane-wise
+  // partial sums, the new mean with its weight, and the stop flag
+  constexpr uint32_t kMapCap = kMapPerWave;
+  constexpr int kRowSlots = W == 1 ? 1 : kMaxRows + 1;
+  __shared__ uint32_t cmap[W == 1 ? kMapPerWave * kWavesPerBlock : 1];
+  __shared__ uint32_t s_rb[kRowSlots], s_len[kRowSlots], s_bstart[kRowSlots];
+  __shared__ uint32_t s_count[3];  // W = 4: non-empty rows, batches, candidates
+  __shared__ R red[W == 1 ? 1 : (D + 1) * kThreads];
+  __shared__ R s_mean[W == 1 ? 1 : D + 1];
+  __shared__ int s_moved;
   const int lane = threadIdx.x & (kWave - 1);
   // wave index, made visibly wave-uniform so the row cursor stays scalar
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
   const int wsub = W == 1 ? 0 : wave;  // this wave's share of the seed's batches
-  uint32_t *map = W == 1 ? cmap + wave * kMapPerWave : cmap;
+  uint32_t *map = cmap + (W == 1 ? wave * kMapPerWave : 0);
   // XCD-aware block order: dies take contiguous runs of (cell-ordered) seeds
   const unsigned per = (logical_blocks + kXcds - 1) / kXcds;
   const unsigned logical = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
@@ -566,74 +599,115 @@ __global__ void __launch_bounds__(kThreads)
   }
   R wsum = 0;
   unsigned long long cand = 0, inside = 0;
+  int done = 0;  // updates computed
   const int reach = g.reach, span = 2 * reach + 1, nrows = span * span;
   const int dz = lane / span - reach, dy = lane % span - reach;
   // row cutting (below): slab slop, support radius with margins, 1 / cell
   const double slop =
       1e-9 * (1.0 + fabs(g.lo[1]) + fabs(g.lo[2]) + g.cell * (g.dims[1] + g.dims[2]));
   const double rlim = sqrt(static_cast<double>(r2)) * (1.0 + 1e-5) + slop;
-  const double rlim2 = rlim * rlim, inv_cell = 1.0 / g.cell;
+  const double rlim2 = rlim * rlim, inv_cell = g.inv;
   for (int it = 0; it < iterations; ++it) {
-    const int cx = cell_of(static_cast<double>(m[0]), g.lo[0], g.cell, g.dims[0]);
-    const int cy = cell_of(static_cast<double>(m[1]), g.lo[1], g.cell, g.dims[1]);
-    const int cz = cell_of(static_cast<double>(m[2]), g.lo[2], g.cell, g.dims[2]);
-    const int x0 = cx > reach ? cx - reach : 0;
-    const int x1 = cx + reach < g.dims[0] ? cx + reach : g.dims[0] - 1;
-    // lanes 0..nrows-1 fetch the ranges of the neighbourhood rows, each cut
-    // to the cells its slice of the support ball can reach: a point of row
-    // (y, z) is at least ey, ez from the seed in those two coordinates, so
-    // within the support only if |x - m0| <= sqrt(rlim^2 - ey^2 - ez^2).
-    // Cells are monotone in each coordinate (also where clamped at the grid
-    // edges), rlim and the slab bounds carry margins far above the rounding
-    // of the FP32 / FP64 distance test: no point inside the support is cut.
-    // The cut pays where a seed has thousands of candidates (a block per
-    // seed); with one wave per seed and a few hundred candidates its cost
-    // exceeds what it saves, so those rows stay whole.
-    uint32_t rb = 0, len = 0;
-    if (W == 1 && lane < nrows) {
-      const int z = cz + dz, y = cy + dy;
-      if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
-        const uint64_t row = cb + (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
-        const uint64_t b = offsets[row + x0];
-        rb = static_cast<uint32_t>(b);
-        len = static_cast<uint32_t>(offsets[row + x1 + 1] - b);
-      }
-    } else if (W > 1 && lane < nrows) {
-      const int z = cz + dz, y = cy + dy;
-      if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
-        const double my = static_cast<double>(m[1]), mz = static_cast<double>(m[2]);
-        double ey = y > cy ? g.lo[1] + y * g.cell - my
-                           : (y < cy ? my - (g.lo[1] + (y + 1) * g.cell) : 0.0);
-        double ez = z > cz ? g.lo[2] + z * g.cell - mz
-                           : (z < cz ? mz - (g.lo[2] + (z + 1) * g.cell) : 0.0);
-        ey = ey > slop ? ey - slop : 0.0;
-        ez = ez > slop ? ez - slop : 0.0;
-        const double rem = rlim2 - ey * ey - ez * ez;
-        if (rem > 0) {
-          // half-width (FP32 root, relative margin 1e-6) and its cells, by
-          // the inverse cell side with a margin of 1e-6 cells: xa never
-          // exceeds, xb never falls short of, the cells the divisions of
-          // cell_of give to points inside [mx - hx, mx + hx]
-          const double hx = static_cast<double>(sqrtf(static_cast<float>(rem))) * (1.0 + 1e-6);
-          const double mx = static_cast<double>(m[0]);
-          const double ua = floor((mx - hx - g.lo[0]) * inv_cell - 1e-6);
-          const double ub = floor((mx + hx - g.lo[0]) * inv_cell + 1e-6);
-          const int xa = ua > x0 ? (ua < x1 + 1 ? static_cast<int>(ua) : x1 + 1) : x0;
-          const int xb = ub < x1 ? (ub > x0 - 1 ? static_cast<int>(ub) : x0 - 1) : x1;
-          if (xa <= xb) {
-            const uint64_t row = cb + (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
-            const uint64_t b = offsets[row + xa];
-            rb = static_cast<uint32_t>(b);
-            len = static_cast<uint32_t>(offsets[row + xb + 1] - b);
+    // lanes 0..nrows-1 fetch the ranges of the neighbourhood rows.  With
+    // W = 4 each row is cut to the cells its slice of the support ball can
+    // reach: a point of row (y, z) is at least ey, ez from the seed in those
+    // two coordinates, so within the support only if |x - m0| <=
+    // sqrt(rlim^2 - ey^2 - ez^2).  Cells are monotone in each coordinate
+    // (also where clamped at the grid edges), rlim and the slab bounds carry
+    // margins far above the rounding of the FP32 / FP64 distance test: no
+    // point inside the support is cut.  The cut pays where a seed has
+    // thousands of candidates (a block per seed); with one wave per seed and
+    // a few hundred candidates its cost exceeds what it saves, so those rows
+    // stay whole.
+    uint32_t rb = 0, startv = 0, rend = 0, total = 0;
+    if (W == 1 || wave == 0) {
+      const int cx = cell_of(static_cast<double>(m[0]), g.lo[0], g.inv, g.dims[0]);
+      const int cy = cell_of(static_cast<double>(m[1]), g.lo[1], g.inv, g.dims[1]);
+      const int cz = cell_of(static_cast<double>(m[2]), g.lo[2], g.inv, g.dims[2]);
+      const int x0 = cx > reach ? cx - reach : 0;
+      const int x1 = cx + reach < g.dims[0] ? cx + reach : g.dims[0] - 1;
+      uint32_t len = 0;
+      if (W == 1 && lane < nrows) {
+        const int z = cz + dz, y = cy + dy;
+        if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
+          const uint64_t row = cb + (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
+          const uint64_t b = offsets[row + x0];
+          rb = static_cast<uint32_t>(b);
+          len = static_cast<uint32_t>(offsets[row + x1 + 1] - b);
+        }
+      } else if (W > 1 && lane < nrows) {
+        const int z = cz + dz, y = cy + dy;
+        if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
+          const double my = static_cast<double>(m[1]), mz = static_cast<double>(m[2]);
+          double ey = y > cy ? g.lo[1] + y * g.cell - my
+                             : (y < cy ? my - (g.lo[1] + (y + 1) * g.cell) : 0.0);
+          double ez = z > cz ? g.lo[2] + z * g.cell - mz
+                             : (z < cz ? mz - (g.lo[2] + (z + 1) * g.cell) : 0.0);
+          ey = ey > slop ? ey - slop : 0.0;
+          ez = ez > slop ? ez - slop : 0.0;
+          const double rem = rlim2 - ey * ey - ez * ez;
+          if (rem > 0) {
+            // half-width (FP32 root, relative margin 1e-6) and its cells, by
+            // the inverse cell side with a margin of 1e-6 cells: xa never
+            // exceeds, xb never falls short of, the cells cell_of gives to
+            // points inside [mx - hx, mx + hx]
+            const double hx = static_cast<double>(sqrtf(static_cast<float>(rem))) * (1.0 + 1e-6);
+            const double mx = static_cast<double>(m[0]);
+            const double ua = floor((mx - hx - g.lo[0]) * inv_cell - 1e-6);
+            const double ub = floor((mx + hx - g.lo[0]) * inv_cell + 1e-6);
+            const int xa = ua > x0 ? (ua < x1 + 1 ? static_cast<int>(ua) : x1 + 1) : x0;
+            const int xb = ub < x1 ? (ub > x0 - 1 ? static_cast<int>(ub) : x0 - 1) : x1;
+            if (xa <= xb) {
+              const uint64_t row = cb + (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
+              const uint64_t b = offsets[row + xa];
+              rb = static_cast<uint32_t>(b);
+              len = static_cast<uint32_t>(offsets[row + xb + 1] - b);
+            }
           }
         }
       }
+      const uint32_t incl = wave_incl_scan(len);  // lanes >= nrows add 0
+      // row r covers flattened [start_r, start_r + len_r), start_r = incl_{r-1};
+      // lane r holds start_r (lane nrows: the total) and the row's first record
+      startv = incl - len;
+      rend = incl;
+      total = __builtin_amdgcn_readlane(incl, nrows - 1);
+      if (W > 1) {
+        // the non-empty rows, compacted in row order, with their batches of
+        // 64 candidates numbered across the rows
+        const uint32_t nb = (len + kWave - 1) / kWave;
+        const uint64_t ne = ballot(len > 0);
+        const uint32_t binc = wave_incl_scan(nb);
+        const int idx = lanes_below(ne);
+        if (len > 0) {
+          s_rb[idx] = rb;
+          s_len[idx] = len;
+          s_bstart[idx] = binc - nb;
+        }
+        if (lane == 0) {
+          const uint32_t nrc = static_cast<uint32_t>(__popcll(ne));
+          const uint32_t totb = __builtin_amdgcn_readlane(binc, nrows - 1);
+          s_bstart[nrc] = totb;
+          s_count[0] = nrc;
+          s_count[1] = totb;
+          s_count[2] = total;
+        }
+      }
     }
-    const uint32_t incl = wave_incl_scan(len);  // lanes >= nrows add 0
-    // row r covers flattened [start_r, start_r + len_r), start_r = incl_{r-1};
-    // lane r holds start_r (lane nrows: the total) and the row's first record
-    const uint32_t startv = incl - len;
-    const uint32_t total = __builtin_amdgcn_readlane(incl, nrows - 1);
+    uint32_t nrc = 0, totb = 0, rbv = 0, lenv = 0, bsv = 0;
+    if (W > 1) {
+      __syncthreads();
+      nrc = __builtin_amdgcn_readfirstlane(s_count[0]);
+      totb = __builtin_amdgcn_readfirstlane(s_count[1]);
+      total = __builtin_amdgcn_readfirstlane(s_count[2]);
+      // lane r holds compacted row r (lane nrc: the batch total)
+      if (lane < static_cast<int>(nrc)) {
+        rbv = s_rb[lane];
+        lenv = s_len[lane];
+      }
+      if (lane <= static_cast<int>(nrc))
+        bsv = s_bstart[lane];
+    }
     if (wsub == 0)
       cand += total;
     R acc[D];
@@ -665,122 +739,181 @@ __global__ void __launch_bounds__(kThreads)
         ++in32;
       }
     };
-    // Chunks of the flattened range: the map (LDS) sends candidate c to its
-    // record index, written row by row (rows split across the seed's waves),
-    // then the waves walk the chunk in 64-candidate batches, batch k + 1
-    // loading while batch k is tested.  Loads are unconditional: lanes past
-    // the chunk's end read the map's last slot, the sentinel record, which
-    // lies outside every support.
-    constexpr uint32_t kStep = static_cast<uint32_t>(W) * kWave;
-    constexpr uint32_t kChunk = kMapCap - 1;  // the last slot holds the sentinel
-    for (uint32_t cbase = 0; cbase < total; cbase += kChunk) {
-      const uint32_t cend = total - cbase < kChunk ? total : cbase + kChunk;
-      // With one wave per seed and short rows (at most kShort candidates a
-      // row on average: small meshes), each lane writes the first kShort slots of
-      // its own row and the wave fills the rest of the longer rows; else
-      // the wave fills the rows one by one.
-      constexpr uint32_t kShort = 8;
-      if (W == 1 && total <= kShort * static_cast<uint32_t>(nrows)) {
-        const uint32_t lo = startv > cbase ? startv : cbase, hi = incl < cend ? incl : cend;
-        for (uint32_t e = 0; e < kShort; ++e)
-          if (lo + e < hi)
-            map[lo + e - cbase] = rb + (lo + e - startv);
-        uint64_t longer = ballot(hi > lo + kShort);
-        while (longer) {
-          const int r = __builtin_ctzll(longer);
-          longer &= longer - 1;
-          const uint32_t rs = __builtin_amdgcn_readlane(startv, r);
-          const uint32_t ra = __builtin_amdgcn_readlane(lo, r) + kShort;
-          const uint32_t rz = __builtin_amdgcn_readlane(hi, r);
-          const uint32_t b = __builtin_amdgcn_readlane(rb, r);
-          for (uint32_t c = ra + lane; c < rz; c += kWave)
-            map[c - cbase] = b + (c - rs);
+    if constexpr (W == 1) {
+      // Chunks of the flattened range: the map (LDS) sends candidate c to
+      // its record index, written row by row, then the wave walks the chunk
+      // in 64-candidate batches, batch k + 1 loading while batch k is tested.
+      // Loads are unconditional: lanes past the chunk's end read the map's
+      // last slot, the sentinel record, which lies outside every support.
+      constexpr uint32_t kChunk = kMapCap - 1;  // the last slot holds the sentinel
+      for (uint32_t cbase = 0; cbase < total; cbase += kChunk) {
+        const uint32_t cend = total - cbase < kChunk ? total : cbase + kChunk;
+        // With short rows (at most kShort candidates a row on average:
+        // small meshes), each lane writes the first kShort slots of its own
+        // row and the wave fills the rest of the longer rows; else the wave
+        // fills the rows one by one.
+        constexpr uint32_t kShort = 8;
+        if (total <= kShort * static_cast<uint32_t>(nrows)) {
+          const uint32_t lo = startv > cbase ? startv : cbase, hi = rend < cend ? rend : cend;
+          for (uint32_t e = 0; e < kShort; ++e)
+            if (lo + e < hi)
+              map[lo + e - cbase] = rb + (lo + e - startv);
+          uint64_t longer = ballot(hi > lo + kShort);
+          while (longer) {
+            const int rr = __builtin_ctzll(longer);
+            longer &= longer - 1;
+            const uint32_t rs = __builtin_amdgcn_readlane(startv, rr);
+            const uint32_t ra = __builtin_amdgcn_readlane(lo, rr) + kShort;
+            const uint32_t rz = __builtin_amdgcn_readlane(hi, rr);
+            const uint32_t b = __builtin_amdgcn_readlane(rb, rr);
+            for (uint32_t c = ra + lane; c < rz; c += kWave)
+              map[c - cbase] = b + (c - rs);
+          }
+        } else {
+          for (int rr = 0; rr < nrows; ++rr) {
+            const uint32_t rs = __builtin_amdgcn_readlane(startv, rr);
+            const uint32_t re = __builtin_amdgcn_readlane(startv, rr + 1);
+            const uint32_t lo = rs > cbase ? rs : cbase, hi = re < cend ? re : cend;
+            const uint32_t b = __builtin_amdgcn_readlane(rb, rr);
+            for (uint32_t c = lo + lane; c < hi; c += kWave)
+              map[c - cbase] = b + (c - rs);
+          }
         }
-      } else {
-        for (int r = wsub; r < nrows; r += W) {
-          const uint32_t rs = __builtin_amdgcn_readlane(startv, r);
-          const uint32_t re = __builtin_amdgcn_readlane(startv, r + 1);
-          const uint32_t lo = rs > cbase ? rs : cbase, hi = re < cend ? re : cend;
-          const uint32_t b = __builtin_amdgcn_readlane(rb, r);
-          for (uint32_t c = lo + lane; c < hi; c += kWave)
-            map[c - cbase] = b + (c - rs);
-        }
-      }
-      const uint32_t send = cend - cbase;  // the sentinel's slot
-      if (wsub == 0 && lane == 0)
-        map[send] = sentinel;
-      if (W > 1) {
-        __syncthreads();
-      } else {
+        const uint32_t send = cend - cbase;  // the sentinel's slot
+        if (lane == 0)
+          map[send] = sentinel;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        // three stages per batch: map slot (LDS), record (global), test; the
+        // map read runs two batches ahead and the record load one batch ahead
+        auto slot = [&](uint32_t c) -> uint32_t {
+          return map[c - cbase < send ? c - cbase : send];
+        };
+        auto load = [&](uint32_t j, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
+          p = p4[j];
+          if (D == 6)
+            q = q2[j];
+        };
+        uint32_t c0 = cbase;
+        if (c0 < cend) {
+          typename Vec<R>::V4 pa, pb;
+          typename Vec<R>::V2 qa{}, qb{};
+          uint32_t ja = slot(c0 + lane), jb = slot(c0 + kWave + lane);
+          load(ja, pa, qa);
+          for (; c0 < cend; c0 += 2 * kWave) {
+            // the scheduling barriers keep each prefetch ahead of the tests
+            // (a load sunk into the iteration that reads it waits in full)
+            load(jb, pb, qb);
+            ja = slot(c0 + 2 * kWave + lane);
+            __builtin_amdgcn_sched_barrier(0);
+            test(pa, qa);
+            load(ja, pa, qa);
+            jb = slot(c0 + 3 * kWave + lane);
+            __builtin_amdgcn_sched_barrier(0);
+            test(pb, qb);
+          }
+        }
+        __builtin_amdgcn_wave_barrier();  // the map is rewritten by the next chunk
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       }
-      // three stages per batch: map slot (LDS), record (global), test; the
-      // map read runs two batches ahead and the record load one batch ahead
-      auto slot = [&](uint32_t c) -> uint32_t {
-        return map[c - cbase < send ? c - cbase : send];
+    } else {
+      // Row-direct walk: batch k of the numbering above is 64 consecutive
+      // records of one row, taken by wave k mod W.  Row state is scalar and
+      // advances with k; lanes past the row's end read the sentinel record,
+      // which lies outside every support.  The next batch's records are in
+      // flight while the current batch is tested.
+      uint32_t r = 0, bs_cur = 0, bs_next = 0, rb_r = 0, len_r = 0;
+      if (nrc) {
+        bs_next = __builtin_amdgcn_readlane(bsv, 1);
+        rb_r = __builtin_amdgcn_readlane(rbv, 0);
+        len_r = __builtin_amdgcn_readlane(lenv, 0);
+      }
+      auto rec_of = [&](uint32_t k) -> uint32_t {
+        if (k >= totb)
+          return sentinel;
+        while (k >= bs_next) {
+          ++r;
+          bs_cur = bs_next;
+          bs_next = __builtin_amdgcn_readlane(bsv, r + 1);
+          rb_r = __builtin_amdgcn_readlane(rbv, r);
+          len_r = __builtin_amdgcn_readlane(lenv, r);
+        }
+        const uint32_t off = (k - bs_cur) * kWave;
+        return static_cast<uint32_t>(lane) < len_r - off ? rb_r + off + lane : sentinel;
       };
       auto load = [&](uint32_t j, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
         p = p4[j];
         if (D == 6)
           q = q2[j];
       };
-      uint32_t c0 = cbase + static_cast<uint32_t>(wsub) * kWave;
-      if (c0 < cend) {
+      uint32_t k = static_cast<uint32_t>(wsub);
+      if (k < totb) {
         typename Vec<R>::V4 pa, pb;
         typename Vec<R>::V2 qa{}, qb{};
-        uint32_t ja = slot(c0 + lane), jb = slot(c0 + kStep + lane);
-        load(ja, pa, qa);
-        for (; c0 < cend; c0 += 2 * kStep) {
-          // the scheduling barriers keep each prefetch ahead of the tests
-          // (a load sunk into the iteration that reads it waits in full)
+        load(rec_of(k), pa, qa);
+        uint32_t jb = rec_of(k + W);
+        for (; k < totb; k += 2 * W) {
           load(jb, pb, qb);
-          ja = slot(c0 + 2 * kStep + lane);
+          const uint32_t ja = rec_of(k + 2 * W);
           __builtin_amdgcn_sched_barrier(0);
           test(pa, qa);
           load(ja, pa, qa);
-          jb = slot(c0 + 3 * kStep + lane);
+          jb = rec_of(k + 3 * W);
           __builtin_amdgcn_sched_barrier(0);
           test(pb, qb);
         }
       }
-      if (W > 1) {
-        __syncthreads();  // the map is rewritten by the next chunk
-      } else {
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      }
     }
     inside += in32;
-    wl = wave_total(wl);
-    for (int k = 0; k < D; ++k)
-      acc[k] = wave_total(acc[k]);
-    if (W > 1) {
-      if (lane == 0) {
-        for (int k = 0; k < D; ++k)
-          part[wave][k] = acc[k];
-        part[wave][D] = wl;
+    ++done;
+    bool moved = false;
+    if (W == 1) {
+      wl = wave_total(wl);
+      for (int k = 0; k < D; ++k)
+        acc[k] = wave_total(acc[k]);
+      wsum = wl;
+      if (wl > R(0)) {
+        const R inv = R(1) / wl;
+        for (int k = 0; k < D; ++k) {
+          const R nk = m[k] + acc[k] * inv;
+          moved = moved || nk != m[k];
+          m[k] = nk;
+        }
+      }
+    } else {
+      // lane-wise partials through LDS, summed over the waves in a fixed
+      // order by wave 0
+      for (int k = 0; k < D; ++k)
+        red[k * kThreads + threadIdx.x] = acc[k];
+      red[D * kThreads + threadIdx.x] = wl;
+      __syncthreads();
+      if (wave == 0) {
+        R v[D + 1];
+        for (int k = 0; k <= D; ++k) {
+          const R *row = red + k * kThreads + lane;
+          v[k] = wave_total((row[0] + row[kWave]) + (row[2 * kWave] + row[3 * kWave]));
+        }
+        wl = v[D];
+        if (wl > R(0)) {
+          const R inv = R(1) / wl;
+          for (int k = 0; k < D; ++k) {
+            const R nk = m[k] + v[k] * inv;
+            moved = moved || nk != m[k];
+            m[k] = nk;
+          }
+        }
+        if (lane == 0) {
+          for (int k = 0; k < D; ++k)
+            s_mean[k] = m[k];
+          s_mean[D] = wl;
+          s_moved = moved ? 1 : 0;
+        }
       }
       __syncthreads();
-      wl = part[0][D];
       for (int k = 0; k < D; ++k)
-        acc[k] = part[0][k];
-      for (int w = 1; w < W; ++w) {
-        wl += part[w][D];
-        for (int k = 0; k < D; ++k)
-          acc[k] += part[w][k];
-      }
-      __syncthreads();  // part is rewritten next iteration
-    }
-    wsum = wl;
-    bool moved = false;
-    if (wl > R(0)) {
-      const R inv = R(1) / wl;
-      for (int k = 0; k < D; ++k) {
-        const R nk = m[k] + acc[k] * inv;
-        moved = moved || nk != m[k];
-        m[k] = nk;
-      }
+        m[k] = s_mean[k];
+      wsum = s_mean[D];
+      moved = s_moved != 0;
     }
     if (!moved)
       break;  // uniform across the seed's waves: they hold the same mean
@@ -799,6 +932,7 @@ __global__ void __launch_bounds__(kThreads)
     weight[s] = wsum;
     stats[s] = cand;  // per-seed slots: no contended atomics
     stats[ns + s] = inside;
+    stats[2 * ns + s] = static_cast<unsigned long long>(done);
   }
 }
 
@@ -830,20 +964,21 @@ static MsCtl *ms_ctl(Session *s) {
 
 void ms_begin(Session *s) { zero_device(s, ms_ctl(s), sizeof(MsCtl)); }
 
-void ms_finish(Session *s, unsigned long long sums[4]) {
+void ms_finish(Session *s, unsigned long long sums[kMsSums]) {
   if (s->side_pending) {  // the side stream's query and sums come first
     MVS_HIP(hipStreamWaitEvent(s->stream, s->join_event, 0));
     s->side_pending = false;
   }
   // the visit sums and the flag only (a captured step compares what it reads)
   struct {
-    unsigned long long sums[4];
+    unsigned long long sums[kMsSums];
     unsigned bad;
   } h;
-  static_assert(offsetof(MsCtl, bad) == offsetof(MsCtl, sums) + 4 * sizeof(unsigned long long),
+  static_assert(offsetof(MsCtl, bad) ==
+                    offsetof(MsCtl, sums) + kMsSums * sizeof(unsigned long long),
                 "sums and flag are adjacent");
   read_back(s, &h, reinterpret_cast<const char *>(ms_ctl(s)) + offsetof(MsCtl, sums),
-            4 * sizeof(unsigned long long) + sizeof(unsigned));
+            kMsSums * sizeof(unsigned long long) + sizeof(unsigned));
   if (h.bad)
     raise(MVS_EINVAL, "non-finite mean-shift points");
   if (sums)
@@ -881,10 +1016,11 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   const uint32_t meshes = seg ? seg->meshes : 1;
   const uint64_t *seg_off = seg ? seg->seg_off : nullptr;
   MsCtl *ctl = ms_ctl(s);
-  // per-seed statistics [0, ns) candidates, [ns, 2ns) in-support visits
-  unsigned long long *stats = scratch_as<unsigned long long>(s, slot_stats, 2 * ns);
+  // per-seed statistics [0, ns) candidates, [ns, 2ns) in-support visits,
+  // [2ns, 3ns) updates computed (a seed stops at a fixed point)
+  unsigned long long *stats = scratch_as<unsigned long long>(s, slot_stats, 3 * ns);
   if (seg)  // seeds past the drawn count leave their slots untouched
-    zero_device(s, stats, 2 * ns * sizeof(unsigned long long));
+    zero_device(s, stats, 3 * ns * sizeof(unsigned long long));
   const uint64_t np = n ? n : 1;
   // cell-sorted records, then the sentinel record at [n]
   char *sorted =
@@ -931,10 +1067,10 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   const bool wide = ns < s->tune.wide_seeds;
   const int max_reach = wide ? kMaxReach : 2;
   if (D == 6)
-    MVS_LAUNCH(s, "ms_grid", k_grid_setup<6>, meshes, kWave, 0, partial, nb, seg_off, n,
+    MVS_LAUNCH(s, "ms_grid", k_grid_setup<6>, meshes, kThreads, 0, partial, nb, seg_off, n,
                cutoff * h, max_reach, grids, ncell, &ctl->bad);
   else
-    MVS_LAUNCH(s, "ms_grid", k_grid_setup<4>, meshes, kWave, 0, partial, nb, seg_off, n,
+    MVS_LAUNCH(s, "ms_grid", k_grid_setup<4>, meshes, kThreads, 0, partial, nb, seg_off, n,
                cutoff * h, max_reach, grids, ncell, &ctl->bad);
   int key_bits = kCellBits;
   if (seg) {  // cells of mesh m follow those of meshes < m
@@ -1026,11 +1162,10 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   else
     MVS_MS_QUERY(4, 1);
 #undef MVS_MS_QUERY
-  if (run_slot >= 0) {
-    MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats, ns, ctl->sums + run_slot);
-    MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats + ns, ns,
-               ctl->sums + run_slot + 1);
-  }
+  if (run_slot >= 0)
+    for (int k = 0; k < 3; ++k)
+      MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats + k * ns, ns,
+                 ctl->sums + 3 * run_slot + k);
   if (run.side) {
     MVS_HIP(hipEventRecord(s->join_event, s->side_stream));
     s->side_pending = true;
@@ -1093,7 +1228,8 @@ int mvs_dev_ms_points(mvs_session *sp, int precision, uint64_t nrigid, const voi
 
 int mvs_dev_ms_batched(mvs_session *sp, int precision, int dim, uint64_t n, const void *pts,
                        uint64_t ns, const void *seeds, int iterations, double h, double cutoff,
-                       void *out, void *weight, uint64_t *candidates, uint64_t *in_support) {
+                       void *out, void *weight, uint64_t *candidates, uint64_t *in_support,
+                       uint64_t *updates) {
   return guarded([&] {
     Session *s = as_session(sp);
     if (precision != MVS_FP32_GUARDED && precision != MVS_FP64)
@@ -1109,12 +1245,14 @@ int mvs_dev_ms_batched(mvs_session *sp, int precision, int dim, uint64_t n, cons
                          static_cast<const double *>(seeds), nullptr, 0, iterations, h, cutoff,
                          static_cast<double *>(out), static_cast<double *>(weight), 0, nullptr,
                          MsRun{});
-    unsigned long long sums[4];
+    unsigned long long sums[kMsSums];
     ms_finish(s, sums);
     if (candidates)
       *candidates = sums[0];
     if (in_support)
       *in_support = sums[1];
+    if (updates)
+      *updates = sums[2];
   });
 }
 

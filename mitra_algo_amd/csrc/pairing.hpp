@@ -1,5 +1,7 @@
 // Shared between the pairing and vote translation units.
 #pragma once
+#include <functional>
+
 #include "common.hpp"
 #include "session.hpp"
 
@@ -61,9 +63,11 @@ void vote_emit_strided(Session *s, uint64_t P, const int32_t *a, const int32_t *
 uint64_t vote_total_checked(Session *s);
 
 // Batched mean-shift (msbatch.hip).  A call is ms_begin, up to two runs
-// (run_slot 0 and 2 receive the (candidates, in-support) visit sums; -1 for
-// none), then ms_finish: one synchronisation that raises MVS_EINVAL for
-// non-finite points and returns the four sums.
+// (run_slot k = 0 or 1 receives the sums of candidate visits, in-support
+// visits and updates computed at sums[3k..3k+2]; -1 for none), then
+// ms_finish: one synchronisation that raises MVS_EINVAL for non-finite
+// points and returns the six sums.
+constexpr int kMsSums = 6;
 // Segments of one mean-shift run (batches of meshes).
 struct MsSegs {
   uint32_t meshes = 1;
@@ -73,7 +77,7 @@ struct MsSegs {
   uint64_t cells_bound = 0;            // bound on the total grid cells
 };
 void ms_begin(Session *s);
-void ms_finish(Session *s, unsigned long long sums[4]);
+void ms_finish(Session *s, unsigned long long sums[kMsSums]);
 // Seeds: explicit positions (D x ns SoA), or, with seed_ids, ns of the n
 // points drawn stratified over the cell-sorted order with `salt` (indices
 // written to seed_ids, in cell order).
@@ -122,6 +126,16 @@ void batch_pairvote(Session *s, uint64_t n, const double *x, const double *y, co
 void batch_meanshift(Session *s, const uint64_t *soff_host, double ratio, uint64_t seed_value,
                      int iterations, double h, double cutoff, bool want_stats,
                      mvs_batch_meanshift_result *out);
+
+// Graph of one step (batch.hip).  graph_capture runs step(false) eagerly,
+// recording every value read back, then captures step(true) on the session's
+// own stream with those values frozen (each read-back site becomes a device
+// check).  graph_launch replays it ordered after the session stream's queued
+// work; graph_check synchronises and raises MVS_ERUNTIME when a replay read
+// other values.
+void graph_capture(Session *s, const std::function<void(bool)> &step);
+void graph_launch(Session *s);
+void graph_check(Session *s);
 
 // Sample records of the last pair count (valid until the next count).
 const double *plan_samples64(Session *s);

@@ -124,6 +124,19 @@ class Workload:
         ms = self.meanshift(seeds, iterations)
         return pv, ms
 
+    def capture(self, precision: int, seeds: int = 20000, iterations: int = 30,
+                h: float = 0.05, cutoff: float = 3.0, seed_value: int = 7,
+                max_pairs: int = 2**31 - 1, max_votes: int = 2**31 - 1):
+        """Captures step() as a HIP graph (see mvs_step_graph_capture); replay()
+        reruns it over the same resident inputs."""
+        return self.sess.step_graph_capture(
+            self.n, self.x, self.y, self.z, self.feat, self.S, self.selected, self.box,
+            self.center, self.threshold, max_pairs, max_votes, precision, seeds, seed_value,
+            iterations, h, cutoff)
+
+    def replay(self):
+        self.sess.graph_launch()
+
 
 def cfg1(device: str = "cuda:0") -> Workload:
     """96x96 height field, 2,000 uniform samples, magnitude drop, t = .05."""
@@ -268,3 +281,75 @@ class Batch:
             self.n, self.x, self.y, self.z, self.feat, self.soff, self.selected, self.boxes,
             self.centers, self.threshold, 2**31 - 1, 2**31 - 1, precision, seeds_per_sample,
             seed_value, iterations, h, cutoff)
+
+
+def converged_mode_errors(sess, D: int, p64, p32, seed_idx, h: float = 0.05,
+                          cutoff: float = 3.0, iterations: int = 2000) -> dict:
+    """FP32 against FP64 mean-shift modes from the same seed positions.
+
+    Both arithmetics run every seed to its fixed point (a seed stops once an
+    update leaves it unchanged; `iterations` caps the updates), over their
+    own points (p64: FP64 points, p32: the FP32 path's points, D x n).  The
+    FP64 endpoints are merged into modes by the reference's rule (a seed joins
+    the first mode within h/2 in the mean-shift coordinates, else starts a
+    new one; src/cpu/clustering.cpp:64-70); a mode's position is the mean of
+    its members' endpoints, in each arithmetic over the same members.  Errors
+    per mode: length units (translation about the centre, or mirror offset)
+    and radians (angle of R32^T R64, or between mirror normals).  A seed
+    counts as converged when one more update moves it by at most 1e-12
+    (FP64) or 8 units in the last place of its largest coordinate (FP32)."""
+    import torch
+    from scipy.spatial.transform import Rotation
+    n = p64.shape[1]
+    s64 = p64[:, seed_idx].contiguous()
+    s32 = s64.float().contiguous()
+    ns = s64.shape[1]
+
+    def run(prec, pts, seeds, it):
+        out = torch.empty_like(seeds)
+        w = torch.empty(ns, dtype=seeds.dtype, device=seeds.device)
+        c, i, u = sess.dev_ms_batched(prec, D, n, pts, ns, seeds, it, h, cutoff, out, w,
+                                      stats="updates")
+        torch.cuda.synchronize()
+        return out.double().cpu().numpy(), u
+
+    x64, u64 = run(capi.FP64, p64, s64, iterations)
+    y64, _ = run(capi.FP64, p64, s64, iterations + 1)
+    x32, u32 = run(capi.FP32_GUARDED, p32, s32, iterations)
+    y32, _ = run(capi.FP32_GUARDED, p32, s32, iterations + 1)
+    conv64 = np.max(np.abs(y64 - x64), axis=0) <= 1e-12
+    ulp32 = np.spacing(np.max(np.abs(x32), axis=0).astype(np.float32)).astype(np.float64)
+    conv32 = np.max(np.abs(y32 - x32), axis=0) <= 8 * ulp32
+    both = conv64 & conv32
+    centres, members = [], []
+    for s in np.flatnonzero(both):
+        for k, c in enumerate(centres):
+            if np.sum((x64[:, s] - c) ** 2) < (h / 2) ** 2:
+                members[k].append(s)
+                break
+        else:
+            centres.append(x64[:, s])
+            members.append([s])
+    lin, ang, size = [], [], []
+    for mem in members:
+        a = x64[:, mem].mean(axis=1)
+        b = x32[:, mem].mean(axis=1)
+        if D == 6:
+            lin.append(float(np.linalg.norm(b[0:3] - a[0:3])))
+            ang.append(float((Rotation.from_rotvec(b[3:6]).inv() *
+                              Rotation.from_rotvec(a[3:6])).magnitude()))
+        else:
+            lin.append(abs(float(b[0] - a[0])))
+            ua, ub = a[1:4] / np.linalg.norm(a[1:4]), b[1:4] / np.linalg.norm(b[1:4])
+            ang.append(float(np.arccos(np.clip(np.dot(ua, ub), -1.0, 1.0))))
+        size.append(len(mem))
+    lin, ang = np.array(lin), np.array(ang)
+    return {"seeds": int(ns), "converged_fp64": int(conv64.sum()),
+            "converged_fp32": int(conv32.sum()), "modes": len(members),
+            "updates_fp64": int(u64), "updates_fp32": int(u32),
+            "units_max": float(lin.max(initial=0)), "units_median": float(np.median(lin))
+            if len(lin) else 0.0, "rad_max": float(ang.max(initial=0)),
+            "rad_median": float(np.median(ang)) if len(ang) else 0.0,
+            "modes_over_1e-6_units": int(np.sum(lin > 1e-6)),
+            "modes_over_1e-5_rad": int(np.sum(ang > 1e-5)),
+            "largest_mode_seeds": int(max(size, default=0))}

@@ -193,3 +193,80 @@ def test_batch_autotune_keeps_results(batch):
         for k in eager:
             assert torch.equal(eager[k], again[k]), (quota, k)
     batch.sess.set_tuning(0, 2048, 8192)
+
+
+@pytest.fixture(scope="module")
+def batch512():
+    """Config 4 at its size: 512 meshes, grid sides log-uniform in [32, 512],
+    1-16 motif copies, samples clamp(0.05 N, 200, 20000)."""
+    return workload.Batch(count=512, keep_single=True)
+
+
+def test_config4_full_batch_equals_single_meshes(batch512, oracle):
+    """Per mesh, over a spread of 16 meshes (the smallest and the largest
+    among them): the batch's pairs, their order and the compact votes equal a
+    one-mesh pass bit for bit, and its mean-shift seeds and modes equal the
+    one-mesh call's; the smallest mesh's pairs equal the oracle's."""
+    b = batch512
+    prec = capi.FP32_GUARDED
+    M = b.count
+    sides = np.array([s["side"] for s in b.single])
+    assert M == 512 and sides.min() >= 32 and sides.max() <= 512 and sides.max() > 4 * sides.min()
+    r = b.pairvote(prec)
+    allr = _fetch(b.sess, r.all, prec)
+    poff = b.sess.fetch(r.pair_offsets, (M + 1,), torch.int64).cpu().numpy()
+    roff = b.sess.fetch(r.refl_offsets, (M + 1,), torch.int64).cpu().numpy()
+    ratio, iters = 0.2, 20
+    ms = b.meanshift(ratio, iters)
+    cr, cf = ms.all.rigid_seeds, ms.all.refl_seeds
+    so_r = b.sess.fetch(ms.rigid_seed_offsets, (M + 1,), torch.int64).cpu().numpy()
+    so_f = b.sess.fetch(ms.refl_seed_offsets, (M + 1,), torch.int64).cpu().numpy()
+    ids = b.sess.fetch(ms.all.seed_votes, (cr + cf,), torch.int32).cpu().numpy()
+    mr = b.sess.fetch(ms.all.rigid_modes, (6, cr), torch.float32, cr).double().cpu().numpy()
+    mf = b.sess.fetch(ms.all.refl_modes, (4, cf), torch.float32, cf).double().cpu().numpy()
+    torch.cuda.synchronize()
+    assert poff[-1] == r.all.pairs > 0
+    picks = set(np.linspace(0, M - 1, 14).astype(int).tolist())
+    picks |= {int(np.argmin(sides)), int(np.argmax(sides))}
+    single = capi.Session(0)
+    single.set_stream(torch.cuda.current_stream().cuda_stream)
+    for m in sorted(picks):
+        s = b.single[m]
+        p0, p1, r0, r1 = poff[m], poff[m + 1], roff[m], roff[m + 1]
+        o = int(b.soff[m])
+        one = _fetch(single, single.dev_pairvote(s["n"], s["x"], s["y"], s["z"], s["feat"],
+                                                s["S"], s["sel"], s["box"], s["center"],
+                                                b.threshold, 2**31 - 1, 2**31 - 1, prec), prec)
+        torch.cuda.synchronize()
+        assert torch.equal(allr["a"][p0:p1] - o, one["a"]), m
+        assert torch.equal(allr["b"][p0:p1] - o, one["b"]), m
+        assert torch.equal(allr["q4"][:, 2 * p0:2 * p1], one["q4"]), m
+        assert torch.equal(allr["t3"][:, 2 * p0:2 * p1], one["t3"]), m
+        assert torch.equal(allr["fp"][r0:r1] - int(p0), one["fp"]), m
+        assert torch.equal(allr["n3"][:, r0:r1], one["n3"]), m
+        assert torch.equal(allr["d1"][r0:r1], one["d1"]), m
+        S = int(b.soff[m + 1] - b.soff[m])
+        ns = min(2 * int(p1 - p0) + int(r1 - r0), int(np.ceil(ratio * S)))
+        if ns:
+            one_ms = single.dev_meanshift_votes(ns, 7, iters, 0.05, 3.0, False)
+            kr, kf = one_ms.rigid_seeds, one_ms.refl_seeds
+            oid = single.fetch(one_ms.seed_votes, (kr + kf,), torch.int32).cpu().numpy()
+            orig = single.fetch(one_ms.rigid_modes, (6, kr), torch.float32, kr).double().cpu().numpy()
+            ofl = single.fetch(one_ms.refl_modes, (4, kf), torch.float32, kf).double().cpu().numpy()
+            torch.cuda.synchronize()
+            assert kr == so_r[m + 1] - so_r[m] and kf == so_f[m + 1] - so_f[m], m
+            assert np.array_equal(ids[so_r[m]:so_r[m + 1]] - 2 * poff[m], oid[:kr]), m
+            assert np.array_equal(ids[cr + so_f[m]:cr + so_f[m + 1]] - roff[m], oid[kr:]), m
+            assert np.max(np.abs(mr[:, so_r[m]:so_r[m + 1]] - orig), initial=0) < 2e-5, m
+            assert np.max(np.abs(mf[:, so_f[m]:so_f[m + 1]] - ofl), initial=0) < 2e-5, m
+    single.close()
+    # the smallest mesh's pairs against the oracle (FP64 predicate)
+    m = int(np.argmin(sides))
+    s = b.single[m]
+    feat = s["feat"].cpu().numpy()
+    v = np.stack([s["x"].cpu().numpy(), s["y"].cpu().numpy(), s["z"].cpu().numpy()], axis=1)
+    sel = s["sel"].cpu().numpy()
+    pa, pb = oracle.pairing(v, feat[9], feat[10], sel, s["box"], b.threshold, 10**9)
+    a = allr["a"][poff[m]:poff[m + 1]].numpy() - int(b.soff[m])
+    bb = allr["b"][poff[m]:poff[m + 1]].numpy() - int(b.soff[m])
+    assert np.array_equal(sel[a], pa) and np.array_equal(sel[bb], pb)

@@ -72,15 +72,16 @@ def test_compact_votes_match_oracle(cfg1, oracle, t):
         assert np.array_equal(fp, refl_pairs)
         rig = np.where(kind == 0)[0]
         assert len(rig) == 2 * r.P
-        tol = 1e-12 if prec == capi.FP64 else 2e-5
+        # the documented bars: FP64 2e-12 (tests/cuda_backend.cpp), FP32 2e-5
+        tol = 2e-12 if prec == capi.FP64 else 2e-5
         R_or = lanes[0:9, rig].T.reshape(-1, 3, 3)
         R_gpu = np.stack([quat_to_R(q[:, i]) for i in range(2 * r.P)])
-        assert np.max(np.abs(R_gpu - R_or)) < tol * 10
+        assert np.max(np.abs(R_gpu - R_or)) < tol
         tc_or = lanes[25:28, rig] * wl.box
-        assert np.max(np.abs(tc - tc_or)) < tol * 10 * wl.box
+        assert np.max(np.abs(tc - tc_or)) < tol * wl.box
         mir = np.where(kind == 1)[0]
-        assert np.max(np.abs(n3 - lanes[12:15, mir]), initial=0) < tol * 10
-        assert np.max(np.abs(d1 - lanes[19, mir] * wl.box), initial=0) < tol * 10 * wl.box
+        assert np.max(np.abs(n3 - lanes[12:15, mir]), initial=0) < tol
+        assert np.max(np.abs(d1 - lanes[19, mir] * wl.box), initial=0) < tol * wl.box
         ref[prec] = r
     st = wl.sess.dev_pair_stats()
     assert st["index_candidates"] <= st["window_candidates"]
@@ -117,14 +118,18 @@ def _clusters(D, seed, spread=1.0):
 
 # spread 1: half-width cells (reach 2); spread 8: cells of side 3h (reach 1);
 # spread 200: reach 1 with cells doubled until the grid fits 2^22 cells
+# wide: a block of four waves per seed (rows cut to the support ball, the
+# first wave finding the rows); narrow: one wave per seed
+@pytest.mark.parametrize("wide", [True, False])
 @pytest.mark.parametrize("spread", [1.0, 8.0, 200.0])
 @pytest.mark.parametrize("D", [6, 4])
-def test_batched_meanshift_matches_oracle(oracle, D, spread):
+def test_batched_meanshift_matches_oracle(oracle, D, spread, wide):
     P, seeds = _clusters(D, 7 + D, spread)
     h, cut, iters = 0.05, 3.0, 25
     want, wsum = oracle.meanshift_batched(P, np.zeros(P.shape[1], np.uint8), D, D, seeds,
                                           np.zeros(seeds.shape[1], np.uint8), h, cut, iters)
     s = capi.Session(0)
+    s.set_tuning(0, 2048, 1 << 40 if wide else 0)
     dev = torch.device("cuda:0")
     s.set_stream(torch.cuda.current_stream(dev).cuda_stream)
     for prec, dt, tol in ((capi.FP64, torch.float64, 1e-9), (capi.FP32_GUARDED, torch.float32,
@@ -138,20 +143,22 @@ def test_batched_meanshift_matches_oracle(oracle, D, spread):
         torch.cuda.synchronize()
         got = out.double().cpu().numpy()
         assert np.max(np.abs(got - want)) < tol, float(np.max(np.abs(got - want)))
-        assert np.max(np.abs(w.double().cpu().numpy() - wsum) / (1 + wsum)) < tol * 100
+        assert np.max(np.abs(w.double().cpu().numpy() - wsum) / (1 + wsum)) < tol
         assert 0 < ins <= cand
     s.close()
 
 
+@pytest.mark.parametrize("wide", [True, False])
 @pytest.mark.parametrize("spread", [1.0, 8.0])
 @pytest.mark.parametrize("D", [6, 4])
-def test_meanshift_support_visits_match_brute_force(D, spread):
+def test_meanshift_support_visits_match_brute_force(D, spread, wide):
     """One update from each seed visits exactly the points inside its support
     (counted by brute force in FP64): the grid walk, with its rows cut to the
     support ball, drops no point of the support."""
     P, seeds = _clusters(D, 3 + D, spread)
     h, cut = 0.05, 3.0
     s = capi.Session(0)
+    s.set_tuning(0, 2048, 1 << 40 if wide else 0)
     dev = torch.device("cuda:0")
     s.set_stream(torch.cuda.current_stream(dev).cuda_stream)
     pts = torch.as_tensor(P, dtype=torch.float64, device=dev).contiguous()

@@ -1,9 +1,10 @@
 """Config 3 on the GPU: the 96x96 height field (2,000 samples, t = .05) in
 both arithmetics through the C-ABI.  The FP32-guarded pair and mirror sets
-equal the FP64 ones, and the FP32 mean-shift modes started from the same
-seed positions stay near the FP64 GPU modes: the median within the bar of
-1e-6 length units / 1e-5 rad, at most 1% of the seeds outside it (seeds that
-are still moving after 30 iterations drift at slightly different speeds)."""
+equal the FP64 ones, and the FP32 mean-shift modes equal the FP64 GPU modes
+within 1e-6 length units and 1e-5 rad: every seed (every k-th FP64 vote) runs
+to its fixed point in both arithmetics, the endpoints are merged into modes
+by the reference's h/2 rule, and the bar applies to the largest error over
+the modes."""
 import numpy as np
 import pytest
 
@@ -48,33 +49,16 @@ def test_pair_and_mirror_sets_identical(both):
 
 
 @pytest.mark.parametrize("D", [6, 4])
-def test_modes_near_fp64(both, D):
-    from scipy.spatial.transform import Rotation
+def test_converged_modes_within_the_bar(both, D):
     wl, out = both
     p64, p32 = out[capi.FP64][1][D], out[capi.FP32_GUARDED][1][D]
     n = p64.shape[1]
     idx = torch.arange(0, n, max(1, n // 500), device="cuda")[:500]
-    s64 = p64[:, idx].contiguous()
-    s32 = s64.float().contiguous()
-    ns = s64.shape[1]
-    o64, o32 = torch.empty_like(s64), torch.empty_like(s32)
-    w64 = torch.empty(ns, dtype=torch.float64, device="cuda")
-    w32 = torch.empty(ns, dtype=torch.float32, device="cuda")
-    wl.sess.dev_ms_batched(capi.FP64, D, n, p64, ns, s64, 30, 0.05, 3.0, o64, w64)
-    wl.sess.dev_ms_batched(capi.FP32_GUARDED, D, n, p32, ns, s32, 30, 0.05, 3.0, o32, w32)
-    torch.cuda.synchronize()
-    x64, x32 = o64.cpu().numpy(), o32.double().cpu().numpy()
-    if D == 6:
-        lin = np.linalg.norm(x32[0:3] - x64[0:3], axis=0)
-        ang = (Rotation.from_rotvec(x32[3:6].T).inv() * Rotation.from_rotvec(x64[3:6].T)).magnitude()
-    else:
-        lin = np.abs(x32[0] - x64[0])
-        u64 = x64[1:4] / np.linalg.norm(x64[1:4], axis=0)
-        u32 = x32[1:4] / np.linalg.norm(x32[1:4], axis=0)
-        ang = np.arccos(np.clip((u64 * u32).sum(0), -1, 1))
-    print(f"D={D}: units median {np.median(lin):.2e} max {lin.max():.2e}; "
-          f"rad median {np.median(ang):.2e} max {ang.max():.2e}")
-    assert np.median(lin) < 1e-6 and np.median(ang) < 1e-5
-    assert np.mean(lin > 1e-6) <= 0.01 and np.mean(ang > 1e-5) <= 0.01
-    # no seed lands in another mode
-    assert lin.max() < 1e-3 and ang.max() < 1e-3
+    r = workload.converged_mode_errors(wl.sess, D, p64, p32, idx)
+    print(D, r)
+    # nearly every seed reaches its fixed point within the update cap
+    assert r["converged_fp64"] >= 0.98 * r["seeds"] and r["converged_fp32"] >= 0.98 * r["seeds"]
+    assert r["modes"] >= 3
+    # the bar: the largest error over the modes
+    assert r["units_max"] <= 1e-6, r
+    assert r["rad_max"] <= 1e-5, r
