@@ -50,9 +50,9 @@ def load_traffic():
 
 def emission_bytes(P, R, S, s):
     """Algorithmic bytes of vote emission: pair ordinals in, compact records
-    out (rigid q4+t3 and mirror n3+d at s bytes per scalar, int32 pair id
-    each), the 14-scalar sample records read once."""
-    return 8 * P + 2 * P * (7 * s + 4) + R * (4 * s + 4) + 14 * s * S
+    out (rigid q4+t3 at s bytes per scalar, the pair being vote / 2; mirror
+    n3+d and an int32 pair id), the 14-scalar sample records read once."""
+    return 8 * P + 2 * P * 7 * s + R * (4 * s + 4) + 14 * s * S
 
 
 def meanshift_flops(cand, inside, D=6):
@@ -69,32 +69,120 @@ def meanshift_flops_kinds(ms):
             meanshift_flops(ms.refl_candidates, ms.refl_in_support, 4))
 
 
-def cpu_baseline(wl, budget_s=20.0):
-    """Reference CPU pairing + transformations (oracle/_ref, single thread) on
-    the first S_sub selected samples; votes/s is size-independent to first
-    order (pairs and tests both grow as S^2)."""
-    from oracle.pyoracle import Oracle
+def host_threads():
+    """Threads for the all-cores leg: the host's CPUs, at most 16 (the host
+    share of one GPU on a shared node)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(16, n or 1))
+
+
+def cpu_model():
     try:
-        lib, kind = Oracle("ref"), "reference"
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def median_time(fn, runs=3):
+    """Median wall time of `runs` calls after one warm-up call (the
+    reference's benchmark rule, benchmarks/pairing.cpp:39-52)."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2], out
+
+
+def cpu_baseline(wl, ms_points=None, votes=None, budget_s=20.0):
+    """The voting step on this host's CPU, on bounded samples of the same
+    workload:
+      * pairing + transformations over the first S_sub selected samples (S_sub
+        grown until one single-thread run takes about budget/8): the
+        reference's own code (oracle/_ref, single thread, as the reference
+        runs) and the oracle's port on all host cores (threads over sources,
+        then over pairs; outputs equal the single-thread ones); medians of 3
+        runs after a warm-up; votes/s is size-independent to first order
+        (pairs and tests both grow as S^2);
+      * batched mean-shift (the port, brute force over every vote of the kind
+        per update, as the reference's evaluator, src/cpu/clustering.cpp:
+        186-208) on all cores for a sample of the seeds over the full FP64
+        vote set, its seed-iteration rate extrapolated to the step's seeds x
+        iterations.
+    value: whole-step votes/s of the all-cores leg, V / (V / votes_rate +
+    extrapolated mean-shift seconds)."""
+    import numpy as np
+    from oracle.pyoracle import Oracle
+    port = Oracle("oracle")
+    try:
+        ref, ref_kind = Oracle("ref"), "reference"
     except FileNotFoundError:
-        lib, kind = Oracle("oracle"), "port"
+        ref, ref_kind = port, "port"
+    thr = host_threads()
     v = wl.mesh.vertices
     feat = wl.feat_np
-    S_sub = min(wl.S, 12000)
+    S_sub = min(wl.S, 8000)
+
+    def pv(lib, sel, threads=1):
+        kw = {"threads": threads} if threads > 1 else {}
+        pa, pb = lib.pairing(v, feat[9], feat[10], sel, wl.box, wl.threshold, 2**31 - 1,
+                             cap=max(1, len(sel) * (len(sel) - 1) // 2 // 40 + 1000000), **kw)
+        pair, kd, lanes = lib.transformations(v, feat, pa, pb, wl.box, wl.center, 2**31 - 1,
+                                              **kw)
+        return len(pa), len(pair)
+
     while True:
         sel = wl.selected_np[:S_sub]
         t0 = time.perf_counter()
-        pa, pb = lib.pairing(v, feat[9], feat[10], sel, wl.box, wl.threshold, 2**31 - 1,
-                             cap=max(1, S_sub * (S_sub - 1) // 2 // 50 + 1000000))
-        pair, kd, lanes = lib.transformations(v, feat, pa, pb, wl.box, wl.center, 2**31 - 1)
+        pv(ref, sel)
         dt = time.perf_counter() - t0
-        if dt > budget_s / 4 or S_sub >= wl.S:
+        if dt > budget_s / 8 or S_sub >= wl.S:
             break
-        S_sub = min(wl.S, int(S_sub * 1.6))
-    return {"value": len(pair) / dt, "unit": "votes/s", "cores": 1, "kind": kind,
+        S_sub = min(wl.S, int(S_sub * 1.5))
+    t1, (P1, V1) = median_time(lambda: pv(ref, sel))
+    tN, (PN, VN) = median_time(lambda: pv(port, sel, thr))
+    assert (P1, V1) == (PN, VN)
+    legs = {"pairvote_1thread": {"votes_per_s": V1 / t1, "seconds": t1, "kind": ref_kind,
+                                 "cores": 1},
+            "pairvote_all_cores": {"votes_per_s": VN / tN, "seconds": tN, "kind": "port",
+                                   "cores": thr}}
+    ms_s = None
+    if ms_points is not None:
+        # seeds: every k-th vote of each kind, 2 x `thr` seeds per kind
+        seconds, per_iter = 0.0, {}
+        for name, (pts, ns_full) in ms_points.items():
+            n = pts.shape[1]
+            k = max(1, n // (2 * thr))
+            idx = np.arange(0, n, k)[:2 * thr]
+            seeds = np.ascontiguousarray(pts[:, idx])
+            zeros_v, zeros_s = np.zeros(n, np.uint8), np.zeros(len(idx), np.uint8)
+            D = pts.shape[0]
+            t, _ = median_time(lambda: port.meanshift_batched(
+                pts, zeros_v, D, D, seeds, zeros_s, H, CUT, ITERS, threads=thr), runs=1)
+            rate = len(idx) * ITERS / t  # seed-iterations per second
+            per_iter[name] = {"seed_iterations_per_s": rate, "sample_seeds": int(len(idx)),
+                              "step_seeds": int(ns_full), "votes": int(n)}
+            seconds += ns_full * ITERS / rate
+        ms_s = seconds
+        legs["meanshift_all_cores"] = {"extrapolated_s": seconds, "kinds": per_iter,
+                                       "kind": "port", "cores": thr}
+    V = votes
+    rate = VN / tN
+    value = rate if ms_s is None or not V else V / (V / rate + ms_s)
+    return {"value": value, "unit": "votes/s", "cores": thr, "kind": "port",
+            "cpu": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"pairing + transformations over the first {S_sub} of {wl.S} selected "
-                      f"samples ({len(pa)} pairs, {len(pair)} votes) in {dt:.2f} s, "
-                      "single thread"}
+                      f"samples ({P1} pairs, {V1} votes), medians of 3 runs after a warm-up; "
+                      + (f"mean-shift: {2 * thr} seeds per kind x {ITERS} iterations over all "
+                         f"FP64 votes, extrapolated to {SEEDS} seeds; value = whole-step "
+                         f"votes/s on {thr} threads" if ms_s is not None else
+                         "pairing + transformations only"),
+            "legs": legs}
 
 
 def bench_batch(args):
@@ -689,7 +777,19 @@ def main():
         "rooflines": rooflines,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl)
+        # the FP64 mean-shift points of the step (the CPU leg's vote set)
+        f = wl.sess.fetch
+        q4 = f(pv64.rigid_q4, (4, 2 * P), torch.float64, 2 * P)
+        t3 = f(pv64.rigid_t3, (3, 2 * P), torch.float64, 2 * P)
+        n3 = f(pv64.refl_n3, (3, R), torch.float64, pv64.refl_stride)
+        d1 = f(pv64.refl_d, (R,), torch.float64)
+        rp = torch.empty((6, 2 * P), dtype=torch.float64, device="cuda")
+        fp_ = torch.empty((4, R), dtype=torch.float64, device="cuda")
+        wl.sess.dev_ms_points(capi.FP64, 2 * P, q4, t3, R, n3, d1, rp, fp_)
+        torch.cuda.synchronize()
+        out["cpu_baseline"] = cpu_baseline(
+            wl, {"rigid": (rp.cpu().numpy(), ms32.rigid_seeds),
+                 "mirror": (fp_.cpu().numpy(), ms32.refl_seeds)}, votes=V)
     print(json.dumps(out), flush=True)
 
 

@@ -154,6 +154,21 @@ int mvs_session_enable_timing(mvs_session *s, int enable);
 /* Restricts timing to the comma-separated kernel names (NULL or "" = all). */
 int mvs_session_timing_filter(mvs_session *s, const char *names);
 int mvs_dev_timing(mvs_session *s, const char *kernel, double *ms, uint64_t *launches);
+/* Host <-> device copies of a timed session are timed too, under the names
+ * "copy_h2d" and "copy_d2h"; a NULL kernel sums kernels only.
+ *
+ * Device memory and copy counters of the calling thread's libmvs calls (all
+ * sessions): bytes its device buffers hold now, their peak since the last
+ * reset, and bytes copied host to device and device to host since then.  The
+ * counterpart of the CUDA backend's per-stage profile (src/cuda/profile.cpp:
+ * 61-93, peak_buffer_bytes, h2d_bytes, d2h_bytes; all_buffers_released is
+ * live_bytes back at its value before the stage).  Reset sets the peak to
+ * the live bytes and the copy counts to 0. */
+typedef struct {
+  uint64_t live_bytes, peak_bytes, h2d_bytes, d2h_bytes;
+} mvs_memory_metrics;
+int mvs_thread_metrics(mvs_memory_metrics *out);
+void mvs_thread_metrics_reset(void);
 
 /* ------------------------------------------------------------ host layer */
 
@@ -316,7 +331,8 @@ int mvs_dev_pair_stats(mvs_session *s, uint64_t *window_candidates, uint64_t *in
  * passed to mvs_dev_vote_emit directly after a count. Layout (SoA, s = 8
  * bytes for MVS_FP64, 4 bytes for MVS_FP32_GUARDED):
  *   rigid stream, 2 per pair (slot 2k + (sign<0)): q[4] (w,x,y,z, w >= 0),
- *       t[3], pair id (int32)
+ *       t[3], pair id (int32, = slot / 2; written only when rigid_pair is
+ *       not NULL)
  *   reflection stream, in pair order: n[3], d, pair id (int32)
  * Global vote order (reference order) is recoverable: pair k's votes start
  * at 2k + (reflections before k); its reflection vote, if any, comes first. */
@@ -338,7 +354,7 @@ typedef struct {
   uint64_t pairs, reflections;
   const int32_t *a, *b;             /* pair ordinals (selected positions) */
   const void *rigid_q4, *rigid_t3;  /* 4 / 3 lanes, stride 2*pairs */
-  const int32_t *rigid_pair;
+  const int32_t *rigid_pair;        /* NULL: rigid vote v belongs to pair v / 2 */
   const void *refl_n3, *refl_d;     /* 3 / 1 lanes, stride refl_stride */
   const int32_t *refl_pair;
   uint64_t refl_stride;
@@ -406,7 +422,11 @@ int mvs_dev_batch_pairvote(mvs_session *s, uint64_t n, const double *x, const do
  * mesh m draws ns = min(V_m, ceil(seeds_per_sample * S_m)) seeds (S_m its
  * samples, V_m its votes), split between its rigid and mirror votes as
  * mvs_dev_meanshift_votes splits `seeds`; each mesh's seeds, grids and
- * iterations equal that one-mesh call with seeds = ns.  Outputs: modes and
+ * iterations equal that one-mesh call with seeds = ns under the same launch
+ * regime (the seed count of the call, here all meshes' seeds, against the
+ * session's wide-seed threshold decides whether a seed gets a block and a
+ * finer grid, so compare with a one-mesh session whose threshold puts its
+ * seeds in the same regime).  Outputs: modes and
  * weights in the layout of mvs_meanshift_result with all.rigid_seeds /
  * all.refl_seeds the capacities (SoA strides), mesh m's seeds at
  * [offsets[m], offsets[m + 1]) of each stream (device arrays of meshes + 1);
@@ -478,12 +498,14 @@ int mvs_graph_check(mvs_session *s);
  * its time to `table` (capacity >= 16), keeps the fastest on the session and
  * leaves the step captured with it (results in *pv, *ms as for
  * mvs_batch_graph_capture).  Pairs and votes do not depend on the table;
- * mean-shift modes only through the order of their FP sums (one wave or a
- * block per seed). */
+ * mean-shift modes only through the order of their FP sums (16, 32 or 64
+ * lanes, or a block, per seed). */
 typedef struct {
   uint32_t walk_quota, walk_blocks;
   uint64_t wide_seeds;
   double ms;
+  uint32_t seed_lanes;  /* lanes per seed of the narrow queries: 16, 32 or 64 */
+  uint32_t reserved;
 } mvs_tune_entry;
 int mvs_batch_autotune(mvs_session *s, uint64_t n, const double *x, const double *y,
                        const double *z, const double *feat, uint32_t meshes,

@@ -85,10 +85,20 @@ mvs_features features(FeaturesView f) {
   return r;
 }
 
-// One libmvs session per call; records a metric when profiling.
+// One libmvs session per call; records a metric when profiling, from the
+// library's own counters: bytes copied each way and the copies' device time
+// (events around each copy), the peak of the device bytes the stage held, and
+// whether every buffer was released once the session is gone
+// (src/cuda/profile.cpp:61-93).
 class Call {
 public:
   Call(int device, bool profile, const char *stage) : profile_(profile), stage_(stage) {
+    if (profile_) {
+      mvs_thread_metrics_reset();
+      mvs_memory_metrics m{};
+      check(mvs_thread_metrics(&m));
+      live0_ = m.live_bytes;
+    }
     check(mvs_session_create(device, &s_));
     if (profile_)
       check(mvs_session_enable_timing(s_, 1));
@@ -98,20 +108,28 @@ public:
     if (s_ && profile_) {
       CudaMetric m;
       m.stage = stage_;
-      m.h2d_bytes = h2d;
-      m.d2h_bytes = d2h;
       double ms = 0;
       uint64_t launches = 0;
       if (mvs_dev_timing(s_, nullptr, &ms, &launches) == MVS_OK) {
         m.custom_kernel_ms = ms;
         m.custom_launches = launches;
       }
+      if (mvs_dev_timing(s_, "copy_h2d", &ms, nullptr) == MVS_OK)
+        m.h2d_ms = ms;
+      if (mvs_dev_timing(s_, "copy_d2h", &ms, nullptr) == MVS_OK)
+        m.d2h_ms = ms;
       m.gpu_interval_ms = std::chrono::duration<double, std::milli>(
                               std::chrono::steady_clock::now() - t0_)
                               .count();
       mvs_session_destroy(s_);
       s_ = nullptr;
-      m.all_buffers_released = true;
+      mvs_memory_metrics c{};
+      if (mvs_thread_metrics(&c) == MVS_OK) {
+        m.h2d_bytes = c.h2d_bytes;
+        m.d2h_bytes = c.d2h_bytes;
+        m.peak_buffer_bytes = c.peak_bytes - live0_;
+        m.all_buffers_released = c.live_bytes == live0_;
+      }
       g_metrics.push_back(m);
     } else if (s_) {
       mvs_session_destroy(s_);
@@ -120,12 +138,12 @@ public:
   Call(const Call &) = delete;
   Call &operator=(const Call &) = delete;
   mvs_session *get() const { return s_; }
-  uint64_t h2d = 0, d2h = 0;
 
 private:
   mvs_session *s_ = nullptr;
   bool profile_;
   const char *stage_;
+  uint64_t live0_ = 0;
   std::chrono::steady_clock::time_point t0_;
 };
 
@@ -162,9 +180,6 @@ public:
     out.lane[10] = f.kmax.data();
     out.valid = f.valid.data();
     check(mvs_curvature(c.get(), &v, &topo, o.smoothing, &out));
-    c.h2d = 48 * n + 24 * t.face_normals.size() + 8 * (n + 1) + 4 * t.neighbors.indices.size() +
-            sizeof(Edge) * t.edges.size();
-    c.d2h = 89 * n;
     return f;
   }
 
@@ -184,8 +199,6 @@ public:
     uint64_t kept = 0;
     check(mvs_prune(c.get(), &fv, count, r.sample.data(), o.prune, r.pruned.data(), &kept));
     r.pruned.resize(kept);
-    c.h2d = 17 * f.size() + 4 * count;
-    c.d2h = 4 * kept;
     return r;
   }
 
@@ -202,8 +215,6 @@ public:
     p.b.resize(P);
     if (P)
       check(mvs_pairing_fetch(c.get(), p.a.data(), p.b.data()));
-    c.h2d = 24 * m.vertices.size() + 88 * f.size() + 4 * sel.size();
-    c.d2h = 8 * P;
     return p;
   }
 
@@ -240,8 +251,6 @@ public:
     if (V)
       check(mvs_transformations_fetch(c.get(), r.pairs.data(),
                                       reinterpret_cast<uint8_t *>(r.kind.data()), lanes));
-    c.h2d = 24 * m.vertices.size() + 88 * f.size() + 8 * p.size();
-    c.d2h = 229 * V;
     return r;
   }
 
@@ -283,8 +292,6 @@ public:
     // unverified clusters carry empty sides and zero coverage / rms, as the
     // CPU backend leaves them (src/cpu/clustering.cpp:152)
     out.clusters.verification = Verification::empty(out.clusters.size());
-    c.h2d = (1 + 21 * 8) * V;
-    c.d2h = 4 * nm;
     return out;
   }
 
@@ -330,8 +337,6 @@ public:
     r.coverage.resize(C);
     r.rms.resize(C);
     check(mvs_verification_fetch(c.get(), &out[0], &out[1], r.coverage.data(), r.rms.data()));
-    c.h2d = 56 * n + 12 * F + 4 * t.neighbors.indices.size() + 136 * C;
-    c.d2h = 16 * (k.side_a + k.side_b);
     return r;
   }
 

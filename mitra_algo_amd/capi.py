@@ -33,7 +33,8 @@ EXPORTS = [
     "mvs_dev_closest_modes", "mvs_dev_align_energy",
     "mvs_verification", "mvs_verification_fetch", "mvs_verification_rows",
     "mvs_verification_stats", "mvs_verification_surface", "mvs_dev_verification",
-    "mvs_step_graph_capture", "mvs_graph_launch", "mvs_graph_check",
+    "mvs_step_graph_capture", "mvs_graph_launch", "mvs_graph_check", "mvs_thread_metrics",
+    "mvs_thread_metrics_reset",
 ]
 
 
@@ -141,11 +142,12 @@ class MeanShiftResult(C.Structure):
 
 class TuneEntry(C.Structure):
     _fields_ = [("walk_quota", C.c_uint32), ("walk_blocks", C.c_uint32),
-                ("wide_seeds", C.c_uint64), ("ms", C.c_double)]
+                ("wide_seeds", C.c_uint64), ("ms", C.c_double), ("seed_lanes", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
     def as_dict(self):
         return {"walk_quota": self.walk_quota, "walk_blocks": self.walk_blocks,
-                "wide_seeds": self.wide_seeds, "ms": self.ms}
+                "wide_seeds": self.wide_seeds, "seed_lanes": self.seed_lanes, "ms": self.ms}
 
 
 class VerificationCounts(C.Structure):
@@ -612,8 +614,9 @@ class Session:
         check(lib().mvs_session_get_tuning(self.h, C.byref(e)))
         return e.as_dict()
 
-    def set_tuning(self, walk_quota: int, walk_blocks: int, wide_seeds: int):
-        e = TuneEntry(walk_quota, walk_blocks, wide_seeds, 0.0)
+    def set_tuning(self, walk_quota: int, walk_blocks: int, wide_seeds: int,
+                   seed_lanes: int = 32):
+        e = TuneEntry(walk_quota, walk_blocks, wide_seeds, 0.0, seed_lanes, 0)
         check(lib().mvs_session_set_tuning(self.h, C.byref(e)))
 
     # ------------------------------------------------------ mode refinement

@@ -444,11 +444,12 @@ void graph_check(Session *s) {
 // times graph replays (HIP events on the replay stream, median of three);
 // the fastest table stays on the session, with its graph captured.
 static const Tuning kCandidates[] = {
-    // walk quota, walk blocks: one wave per source, or persistent claims of
-    // 1-16 sources from 1024-4096 resident blocks; wide seeds: when a
-    // mean-shift seed gets a whole block
-    {8192, 0, 2048},   {8192, 1, 2048}, {8192, 4, 2048}, {8192, 16, 2048}, {8192, 4, 1024},
-    {8192, 4, 4096},   {0, 0, 2048},    {0, 4, 2048},    {65536, 0, 2048}, {65536, 4, 2048},
+    // wide seeds (below this many, a mean-shift seed gets a whole block),
+    // walk quota, walk blocks (one wave per source, or persistent claims of
+    // 1-16 sources from 1024-4096 resident blocks), lanes per narrow seed
+    {8192, 0, 2048, 16},  {8192, 0, 2048, 32},  {8192, 0, 2048, 64}, {8192, 1, 2048, 16},
+    {8192, 4, 2048, 16},  {8192, 16, 2048, 16}, {8192, 4, 1024, 16}, {8192, 4, 4096, 16},
+    {0, 0, 2048, 16},     {0, 4, 2048, 16},     {65536, 0, 2048, 16}, {65536, 4, 2048, 16},
 };
 
 static void batch_autotune(Session *s, const BatchStepArgs &a, mvs_tune_entry *table,
@@ -486,6 +487,8 @@ static void batch_autotune(Session *s, const BatchStepArgs &a, mvs_tune_entry *t
     table[c].walk_quota = kCandidates[c].walk_quota;
     table[c].walk_blocks = kCandidates[c].walk_blocks;
     table[c].wide_seeds = kCandidates[c].wide_seeds;
+    table[c].seed_lanes = kCandidates[c].seed_lanes;
+    table[c].reserved = 0;
     table[c].ms = t[1];
     if (table[c].ms < table[arg].ms)
       arg = c;
@@ -577,19 +580,23 @@ int mvs_session_get_tuning(mvs_session *sp, mvs_tune_entry *out) {
     out->walk_quota = t.walk_quota;
     out->walk_blocks = t.walk_blocks;
     out->wide_seeds = t.wide_seeds;
+    out->seed_lanes = t.seed_lanes;
+    out->reserved = 0;
     out->ms = 0;
   });
 }
 
 int mvs_session_set_tuning(mvs_session *sp, const mvs_tune_entry *in) {
   return guarded([&] {
-    if (!in || in->walk_blocks == 0)
+    if (!in || in->walk_blocks == 0 ||
+        (in->seed_lanes != 16 && in->seed_lanes != 32 && in->seed_lanes != 64))
       raise(MVS_EINVAL, "invalid tuning entry");
     Session *s = as_session(sp);
     s->drop_graph();  // a captured step keeps the launch table it was captured with
     s->tune.walk_quota = in->walk_quota;
     s->tune.walk_blocks = in->walk_blocks;
     s->tune.wide_seeds = in->wide_seeds;
+    s->tune.seed_lanes = in->seed_lanes;
   });
 }
 

@@ -45,6 +45,11 @@ template <class F> int guarded(F &&f) {
 
 // ---------------------------------------------------------------- buffers
 // Device buffer owned by the caller's scope; freed on unwind.
+// Device bytes the calling thread's libmvs buffers hold, and their peak
+// (session.hip; read through mvs_thread_metrics).
+void note_device_alloc(size_t bytes);
+void note_device_free(size_t bytes);
+
 template <class T> struct DBuf {
   T *ptr = nullptr;
   size_t n = 0;
@@ -54,17 +59,21 @@ template <class T> struct DBuf {
     release();
     if (count > SIZE_MAX / sizeof(T))
       raise(MVS_EOVERFLOW, "device allocation size overflow");
-    n = count;
-    if (count)
+    if (count) {
       MVS_HIP(hipMalloc(reinterpret_cast<void **>(&ptr), count * sizeof(T)));
+      note_device_alloc(count * sizeof(T));
+    }
+    n = count;
   }
   void ensure(size_t count) {
     if (count > n)
       alloc(count);
   }
   void release() {
-    if (ptr)
+    if (ptr) {
       (void)hipFree(ptr);
+      note_device_free(n * sizeof(T));
+    }
     ptr = nullptr;
     n = 0;
   }
@@ -92,6 +101,7 @@ template <class T> T *scratch_as(Session *s, int slot, size_t count) {
   return static_cast<T *>(scratch(s, slot, count * sizeof(T)));
 }
 void *host_stage(Session *s, size_t bytes);  // pinned staging for small D2H
+void note_copy(bool to_device, size_t bytes);
 void copy_h2d(Session *s, void *dst, const void *src, size_t bytes);
 void copy_d2h(Session *s, void *dst, const void *src, size_t bytes);
 void sync(Session *s);

@@ -31,14 +31,15 @@ void pairvote(Session *s, uint64_t n, const double *x, const double *y, const do
   int32_t *ab = scratch_as<int32_t>(s, kSlotOutPairs, 2 * Pp);
   void *q4 = scratch(s, kSlotOutRigidQ, 4 * 2 * Pp * sz);
   void *t3 = scratch(s, kSlotOutRigidT, 3 * 2 * Pp * sz);
-  int32_t *rp = scratch_as<int32_t>(s, kSlotOutRigidPair, 2 * Pp);
   void *n3 = scratch(s, kSlotOutReflN, 3 * Pp * sz);
   void *d1 = scratch(s, kSlotOutReflD, Pp * sz);
   int32_t *fp = scratch_as<int32_t>(s, kSlotOutReflPair, Pp);
   pair_fill_device(s, ab, ab + Pp, true);
   uint64_t *rtot = reinterpret_cast<uint64_t *>(plan_flags(s)) + kVoteTotalWord;
   vote_offsets_device(s, P, ab, ab + Pp, precision, rtot, segs);
-  vote_emit_strided(s, P, ab, ab + Pp, center, precision, Pp, q4, t3, rp, n3, d1, fp, segs);
+  // no rigid pair-id stream: rigid vote v belongs to pair v / 2
+  vote_emit_strided(s, P, ab, ab + Pp, center, precision, Pp, q4, t3, nullptr, n3, d1, fp,
+                    segs);
   const uint64_t R = vote_total_checked(s);
   if (2 * P + R > max_votes || 2 * P + R > static_cast<uint64_t>(INT32_MAX))
     raise(MVS_EBUDGET, "Vote budget exceeded; lower --sample-ratio or tighten --pair-threshold");
@@ -49,7 +50,7 @@ void pairvote(Session *s, uint64_t n, const double *x, const double *y, const do
   r.b = ab + Pp;
   r.rigid_q4 = q4;
   r.rigid_t3 = t3;
-  r.rigid_pair = rp;
+  r.rigid_pair = nullptr;
   r.refl_n3 = n3;
   r.refl_d = d1;
   r.refl_pair = fp;

@@ -404,6 +404,8 @@ static void clustering_run(Session *s, uint64_t V, const uint8_t *kind_h, const 
     const uint32_t T = static_cast<uint32_t>(hs.touched);
     if (T > V)
       raise(MVS_ERUNTIME, "mean-shift trajectory touched more votes than exist");
+    // the kernel's stores into pinned host memory: state, ids, counts
+    note_copy(false, sizeof(TrajState) + T * (sizeof(int32_t) + sizeof(uint32_t)));
     std::vector<Entry> sup(T);
     for (uint32_t e = 0; e < T; ++e)
       sup[e] = {pin_ids[e], pin_counts[e]};

@@ -936,6 +936,235 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// Sum over each G-lane group (G = 16: a DPP row; G = 32: two rows), the
+// same bits in every lane of the group: exchanges between lanes i and i ^ 1,
+// i ^ 2, the quads of a half row (uniform by then), the halves of a row and
+// (G = 32) the rows of a pair.  IEEE addition is commutative, so both lanes
+// of each exchange add the same two values.
+template <int G, class R> __device__ inline R group_total(R v) {
+  v += dpp_f<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_f<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_f<0x141>(v);  // row_half_mirror
+  v += dpp_f<0x140>(v);  // row_mirror
+  if (G == 32)
+    v += __shfl_xor(v, 16, kWave);
+  return v;
+}
+
+template <int G> __device__ inline uint32_t group_count(uint32_t v) {
+  v += dpp_u32<0xB1>(0u, v);
+  v += dpp_u32<0x4E>(0u, v);
+  v += dpp_u32<0x141>(0u, v);
+  v += dpp_u32<0x140>(0u, v);
+  if (G == 32)
+    v += static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), 16, kWave));
+  return v;
+}
+
+// Mean-shift queries with G lanes per seed (64 / G seeds per wave, 256 / G
+// per block), for many seeds with few candidates each (one-wave seeds would
+// spend most of their issue slots on the per-iteration work: the mean's cell,
+// the row ranges, the reductions, the update).  Grids of reach <= 2: at most
+// 25 rows, held two (G = 16) or one (G = 32) per lane.  Each lane of a group
+// holds the seed's mean; a row's records are walked in batches of G
+// consecutive records, the batches of all rows numbered in row order with a
+// wave-wide count per row (the most any of the wave's seeds needs), so the
+// loop is uniform and the next batch's records are in flight while the
+// current one is tested.  Lanes past their row's end read the sentinel
+// record.  Group sums come from group_total, so every lane of a group holds
+// the same new mean; a seed whose update leaves it unchanged stops (its rows
+// are then empty), and the wave leaves once all its seeds have.
+constexpr int kSmallRows = 25;
+template <class R, int D, int G>
+__global__ void __launch_bounds__(kThreads)
+    k_ms_query_small(const typename Vec<R>::V4 *p4, const typename Vec<R>::V2 *q2,
+                     const uint64_t *offsets, const Grid *grids, const uint64_t *cellbase,
+                     const uint32_t *seed_mesh, const uint64_t *ns_dev,
+                     const uint32_t *seed_pos, const R *seeds, const uint32_t *order,
+                     uint64_t ns, unsigned logical_blocks, uint32_t sentinel, int iterations,
+                     R gscale, R r2, R floor_w, R *out, R *weight,
+                     unsigned long long *stats) {
+  static_assert(G == 16 || G == 32, "16 or 32 lanes per seed");
+  constexpr int kSlots = (kSmallRows + G - 1) / G;  // rows per lane
+  constexpr int kSeedsPerBlock = kThreads / G;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int sl = lane & (G - 1);   // lane in the group
+  const int gbase = lane & ~(G - 1);  // first lane of the group
+  const int wave = threadIdx.x / kWave;
+  const unsigned per = (logical_blocks + kXcds - 1) / kXcds;
+  const unsigned logical = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  if (logical >= logical_blocks)
+    return;
+  const uint64_t slot = static_cast<uint64_t>(logical) * kSeedsPerBlock +
+                        static_cast<uint64_t>(wave) * (kWave / G) + lane / G;
+  const uint64_t nseeds = ns_dev ? *ns_dev : ns;
+  bool active = slot < nseeds;
+  if (!ballot(active))
+    return;  // the whole wave
+  const uint64_t s = active ? (order ? order[slot] : slot) : 0;
+  const uint32_t gm = active && seed_mesh ? seed_mesh[s] : 0;
+  const Grid g = grids[gm];
+  const uint64_t cb = cellbase ? cellbase[gm] : 0;
+  R m[D];
+  for (int k = 0; k < D; ++k)
+    m[k] = 0;
+  if (active) {
+    if (seed_pos) {
+      const uint64_t p = seed_pos[s];
+      const typename Vec<R>::V4 v = p4[p];
+      m[0] = v.x, m[1] = v.y, m[2] = v.z, m[3] = v.w;
+      if (D == 6) {
+        const typename Vec<R>::V2 u = q2[p];
+        m[4 % D] = u.x, m[5 % D] = u.y;
+      }
+    } else {
+      for (int k = 0; k < D; ++k)
+        m[k] = seeds[g.perm[k] * ns + s];
+    }
+  }
+  R wsum = 0;
+  unsigned long long cand = 0;
+  uint32_t inside = 0;
+  int done = 0;
+  const int reach = g.reach, span = 2 * reach + 1, nrows = span * span;
+  for (int it = 0; it < iterations; ++it) {
+    // this lane's rows sl, sl + G (row k: dz = k / span - reach, dy = k % span
+    // - reach), whole rows of 2 reach + 1 cells
+    uint32_t rb[kSlots], len[kSlots];
+    const int cx = cell_of(static_cast<double>(m[0]), g.lo[0], g.inv, g.dims[0]);
+    const int cy = cell_of(static_cast<double>(m[1]), g.lo[1], g.inv, g.dims[1]);
+    const int cz = cell_of(static_cast<double>(m[2]), g.lo[2], g.inv, g.dims[2]);
+    const int x0 = cx > reach ? cx - reach : 0;
+    const int x1 = cx + reach < g.dims[0] ? cx + reach : g.dims[0] - 1;
+    uint32_t mine = 0;
+    for (int q = 0; q < kSlots; ++q) {
+      const int k = q * G + sl;
+      rb[q] = 0, len[q] = 0;
+      const int z = cz + k / span - reach, y = cy + k % span - reach;
+      if (active && k < nrows && z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
+        const uint64_t row = cb + (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
+        const uint64_t b = offsets[row + x0];
+        rb[q] = static_cast<uint32_t>(b);
+        len[q] = static_cast<uint32_t>(offsets[row + x1 + 1] - b);
+      }
+      mine += len[q];
+    }
+    // batches per row: the most any group of the wave needs; lane l < 32
+    // holds row l's count (slot l / G of lane l), their inclusive prefix
+    // numbers the batches in row order
+    uint32_t nbl = 0;
+    for (int q = 0; q < kSlots; ++q) {
+      uint32_t nb = (len[q] + G - 1) / G;
+      nb = max(nb, static_cast<uint32_t>(__shfl_xor(static_cast<int>(nb), 32, kWave)));
+      if (G == 16)
+        nb = max(nb, static_cast<uint32_t>(__shfl_xor(static_cast<int>(nb), 16, kWave)));
+      nbl = (lane & 31) / G == q ? nb : nbl;
+    }
+    nbl = lane < 32 ? nbl : 0u;
+    const uint32_t bincl = wave_incl_scan(nbl);
+    const uint32_t totb = __builtin_amdgcn_readlane(bincl, 31);
+    R acc[D];
+    for (int k = 0; k < D; ++k)
+      acc[k] = 0;
+    R wl = 0;
+    auto test = [&](const typename Vec<R>::V4 &p, const typename Vec<R>::V2 &q) {
+      R d[D];
+      d[0] = p.x - m[0];
+      d[1] = p.y - m[1];
+      d[2] = p.z - m[2];
+      d[3] = p.w - m[3];
+      if (D == 6) {
+        d[4 % D] = q.x - m[4 % D];
+        d[5 % D] = q.y - m[5 % D];
+      }
+      R dd = d[0] * d[0];
+      for (int k = 1; k < D; ++k)
+        dd = ms_fma(d[k], d[k], dd);
+      if (dd < r2) {
+        const R w = ms_gauss<R>(dd, gscale) - floor_w;
+        wl += w;
+        for (int k = 0; k < D; ++k)
+          acc[k] = ms_fma(w, d[k], acc[k]);
+        ++inside;
+      }
+    };
+    // scalar row cursor over the batch numbering; the group's row data by
+    // lane permutes when the row changes
+    int r = -1;
+    uint32_t b_cur = 0, b_next = 0, row_b = 0, row_len = 0;
+    auto rec_of = [&](uint32_t t) -> uint32_t {
+      if (t >= totb)
+        return sentinel;
+      while (t >= b_next) {
+        ++r;
+        b_cur = b_next;
+        b_next = __builtin_amdgcn_readlane(bincl, r);
+        const int src = (gbase | (r % G)) << 2;
+        const int q = r / G;
+        uint32_t vb = rb[0], vl = len[0];
+        for (int qq = 1; qq < kSlots; ++qq)
+          if (q == qq)
+            vb = rb[qq], vl = len[qq];
+        row_b = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src, static_cast<int>(vb)));
+        row_len = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src, static_cast<int>(vl)));
+      }
+      const uint32_t off = (t - b_cur) * G + sl;
+      return off < row_len ? row_b + off : sentinel;
+    };
+    auto load = [&](uint32_t j, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
+      p = p4[j];
+      if (D == 6)
+        q = q2[j];
+    };
+    if (totb) {
+      typename Vec<R>::V4 pa, pb;
+      typename Vec<R>::V2 qa{}, qb{};
+      load(rec_of(0), pa, qa);
+      uint32_t jb = rec_of(1);
+      for (uint32_t t = 0; t < totb; t += 2) {
+        load(jb, pb, qb);
+        const uint32_t ja = rec_of(t + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        test(pa, qa);
+        load(ja, pa, qa);
+        jb = rec_of(t + 3);
+        __builtin_amdgcn_sched_barrier(0);
+        test(pb, qb);
+      }
+    }
+    mine = group_count<G>(mine);
+    cand += active ? mine : 0u;
+    wl = group_total<G>(wl);
+    for (int k = 0; k < D; ++k)
+      acc[k] = group_total<G>(acc[k]);
+    bool moved = false;
+    if (active) {
+      wsum = wl;
+      ++done;
+      if (wl > R(0)) {
+        const R inv = R(1) / wl;
+        for (int k = 0; k < D; ++k) {
+          const R nk = m[k] + acc[k] * inv;
+          moved = moved || nk != m[k];
+          m[k] = nk;
+        }
+      }
+    }
+    active = active && moved;  // group-uniform: every lane holds the same sums
+    if (!ballot(active))
+      break;
+  }
+  const uint32_t ins = group_count<G>(inside);
+  if (sl == 0 && slot < nseeds) {
+    for (int k = 0; k < D; ++k)
+      out[g.perm[k] * ns + s] = m[k];
+    weight[s] = wsum;
+    stats[s] = cand;
+    stats[ns + s] = ins;
+    stats[2 * ns + s] = static_cast<unsigned long long>(done);
+  }
+}
+
 __global__ void k_sum_u64(const unsigned long long *a, uint64_t n, unsigned long long *out) {
   __shared__ unsigned long long sh[kThreads];
   unsigned long long x = 0;
@@ -1127,7 +1356,12 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
                                                  : -1.0 / (2.0 * h * h));
   const R r2 = static_cast<R>(cutoff * h * cutoff * h);
   const R fw = static_cast<R>(std::exp(-0.5 * cutoff * cutoff));
-  const unsigned logical = wide ? static_cast<unsigned>(ns) : blocks_for(ns, kWavesPerBlock);
+  // narrow queries: 64 (one wave), 32 or 16 lanes per seed
+  const unsigned lanes = s->tune.seed_lanes;
+  if (lanes != 16 && lanes != 32 && lanes != 64)
+    raise(MVS_EINVAL, "lanes per mean-shift seed must be 16, 32 or 64");
+  const unsigned logical =
+      wide ? static_cast<unsigned>(ns) : blocks_for(ns, kThreads / lanes);
   const unsigned grid = ((logical + kXcds - 1) / kXcds) * kXcds;
   const uint32_t *seed_mesh = seg ? seg->seed_mesh : nullptr;
   const uint64_t *ns_dev = seg ? seg->seed_off + meshes : nullptr;
@@ -1153,14 +1387,32 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     hipStream_t main;
     ~Restore() { s->stream = main; }
   } restore{s, main};
-  if (D == 6 && wide)
-    MVS_MS_QUERY(6, kWavesPerBlock);
-  else if (D == 6)
-    MVS_MS_QUERY(6, 1);
-  else if (wide)
-    MVS_MS_QUERY(4, kWavesPerBlock);
-  else
-    MVS_MS_QUERY(4, 1);
+#define MVS_MS_SMALL(DD, GG)                                                                   \
+  MVS_LAUNCH(s, "ms_query", (k_ms_query_small<R, DD, GG>), grid, kThreads, 0, p4, q2, off,     \
+             grids, cellbase, seed_mesh, ns_dev, seed_pos, seeds, order, ns, logical,         \
+             static_cast<uint32_t>(n), iterations, gscale, r2, fw, out, weight, stats)
+  if (wide) {
+    if (D == 6)
+      MVS_MS_QUERY(6, kWavesPerBlock);
+    else
+      MVS_MS_QUERY(4, kWavesPerBlock);
+  } else if (lanes == 64) {
+    if (D == 6)
+      MVS_MS_QUERY(6, 1);
+    else
+      MVS_MS_QUERY(4, 1);
+  } else if (lanes == 32) {
+    if (D == 6)
+      MVS_MS_SMALL(6, 32);
+    else
+      MVS_MS_SMALL(4, 32);
+  } else {
+    if (D == 6)
+      MVS_MS_SMALL(6, 16);
+    else
+      MVS_MS_SMALL(4, 16);
+  }
+#undef MVS_MS_SMALL
 #undef MVS_MS_QUERY
   if (run_slot >= 0)
     for (int k = 0; k < 3; ++k)

@@ -715,6 +715,7 @@ void read_back(Session *s, void *dst, const void *src, size_t bytes) {
   }
   void *h = host_stage(s, bytes);
   MVS_HIP(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, s->stream));
+  note_copy(false, bytes);
   MVS_HIP(hipStreamSynchronize(s->stream));
   std::memcpy(dst, h, bytes);
   if (s->replay == Replay::kRecord) {

@@ -14,7 +14,19 @@ namespace mvs {
 namespace {
 thread_local std::string g_error;
 thread_local int g_code = MVS_OK;
+// per-thread device-memory and host-copy counters (mvs_thread_metrics)
+thread_local mvs_memory_metrics g_mem{};
 }  // namespace
+
+void note_device_alloc(size_t bytes) {
+  g_mem.live_bytes += bytes;
+  if (g_mem.live_bytes > g_mem.peak_bytes)
+    g_mem.peak_bytes = g_mem.live_bytes;
+}
+void note_device_free(size_t bytes) { g_mem.live_bytes -= bytes; }
+void note_copy(bool to_device, size_t bytes) {
+  (to_device ? g_mem.h2d_bytes : g_mem.d2h_bytes) += bytes;
+}
 
 void set_error(int code, const char *msg) {
   g_code = code;
@@ -56,14 +68,23 @@ void *host_stage(Session *s, size_t bytes) {
   return s->stage;
 }
 
+// Host <-> device copies are counted, and timed under "copy_h2d" /
+// "copy_d2h" when the session times its launches.
 void copy_h2d(Session *s, void *dst, const void *src, size_t bytes) {
-  if (bytes)
-    MVS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream));
+  if (!bytes)
+    return;
+  s->begin("copy_h2d");
+  MVS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream));
+  s->end();
+  note_copy(true, bytes);
 }
 void copy_d2h(Session *s, void *dst, const void *src, size_t bytes) {
   if (!bytes)
     return;
+  s->begin("copy_d2h");
   MVS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream));
+  s->end();
+  note_copy(false, bytes);
   MVS_HIP(hipStreamSynchronize(s->stream));
 }
 void sync(Session *s) { MVS_HIP(hipStreamSynchronize(s->stream)); }
@@ -233,7 +254,7 @@ int mvs_dev_timing(mvs_session *s, const char *kernel, double *ms, uint64_t *lau
     double t = 0;
     uint64_t n = 0;
     for (const auto &kv : p->timings)
-      if (!kernel || kv.first == kernel) {
+      if (kernel ? kv.first == kernel : kv.first.rfind("copy_", 0) != 0) {
         t += kv.second.first;
         n += kv.second.second;
       }
@@ -242,6 +263,20 @@ int mvs_dev_timing(mvs_session *s, const char *kernel, double *ms, uint64_t *lau
     if (launches)
       *launches = n;
   });
+}
+
+int mvs_thread_metrics(mvs_memory_metrics *out) {
+  if (!out) {
+    set_error(MVS_EINVAL, "null argument");
+    return MVS_EINVAL;
+  }
+  *out = g_mem;
+  return MVS_OK;
+}
+
+void mvs_thread_metrics_reset(void) {
+  g_mem.peak_bytes = g_mem.live_bytes;
+  g_mem.h2d_bytes = g_mem.d2h_bytes = 0;
 }
 
 }  // extern "C"

@@ -53,6 +53,7 @@ struct Tuning {
   uint64_t wide_seeds = 8192;  // below this many seeds, a seed gets four waves
   unsigned walk_quota = 0;     // sources per claim of a persistent pair walk (0: one per wave)
   unsigned walk_blocks = 2048; // resident blocks of the persistent pair walk
+  unsigned seed_lanes = 32;    // lanes per seed of the narrow mean-shift query (16, 32, 64)
 };
 
 struct Session {

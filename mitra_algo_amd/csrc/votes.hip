@@ -68,35 +68,30 @@ __device__ inline M3 rigid64(const Geo64 &g, double sign) {
   return mul_bt(frame(g.bd * sign, g.bx * sign, g.bn), frame(g.ad, g.ax, g.an));
 }
 
-// Rotation matrix -> unit quaternion (w, x, y, z) with w >= 0 (Shepperd).
+// Rotation matrix -> unit quaternion (w, x, y, z) with w >= 0 (Shepperd's
+// method: the largest of the four diagonal combinations gives the pivot
+// component, the off-diagonal sums and differences the other three).  The
+// pivot is chosen by selects, not branches, so the lanes of a wave that pick
+// different pivots do not serialise; one root and one reciprocal per matrix.
+__device__ inline float quat_inv(float s) { return __builtin_amdgcn_rcpf(s); }
+__device__ inline double quat_inv(double s) { return 1.0 / s; }
 template <class R> __device__ inline void quat(const R *m, R *q) {
-  R tr = m[0] + m[4] + m[8];
-  R w, x, y, z;
-  if (tr > R(0)) {
-    R s = sqrt(tr + R(1)) * R(2);
-    w = R(0.25) * s;
-    x = (m[7] - m[5]) / s;
-    y = (m[2] - m[6]) / s;
-    z = (m[3] - m[1]) / s;
-  } else if (m[0] > m[4] && m[0] > m[8]) {
-    R s = sqrt(R(1) + m[0] - m[4] - m[8]) * R(2);
-    w = (m[7] - m[5]) / s;
-    x = R(0.25) * s;
-    y = (m[1] + m[3]) / s;
-    z = (m[2] + m[6]) / s;
-  } else if (m[4] > m[8]) {
-    R s = sqrt(R(1) + m[4] - m[0] - m[8]) * R(2);
-    w = (m[2] - m[6]) / s;
-    x = (m[1] + m[3]) / s;
-    y = R(0.25) * s;
-    z = (m[5] + m[7]) / s;
-  } else {
-    R s = sqrt(R(1) + m[8] - m[0] - m[4]) * R(2);
-    w = (m[3] - m[1]) / s;
-    x = (m[2] + m[6]) / s;
-    y = (m[5] + m[7]) / s;
-    z = R(0.25) * s;
-  }
+  const R tr = m[0] + m[4] + m[8];
+  const bool c0 = tr > R(0);
+  const bool c1 = !c0 && m[0] > m[4] && m[0] > m[8];
+  const bool c2 = !c0 && !c1 && m[4] > m[8];
+  const bool c3 = !c0 && !c1 && !c2;
+  const R d = c0 ? tr + R(1)
+                 : (c1 ? R(1) + m[0] - m[4] - m[8]
+                       : (c2 ? R(1) + m[4] - m[0] - m[8] : R(1) + m[8] - m[0] - m[4]));
+  const R s = sqrt(d) * R(2), piv = R(0.25) * s, inv = quat_inv(s);
+  const R a = (m[7] - m[5]) * inv, b = (m[2] - m[6]) * inv, c = (m[3] - m[1]) * inv;
+  const R e = (m[1] + m[3]) * inv, f = (m[2] + m[6]) * inv, g = (m[5] + m[7]) * inv;
+  R w = c0 ? piv : (c1 ? a : (c2 ? b : c));
+  R x = c0 ? a : (c1 ? piv : (c2 ? e : f));
+  R y = c0 ? b : (c1 ? e : (c2 ? piv : g));
+  R z = c0 ? c : (c1 ? f : (c2 ? g : piv));
+  (void)c3;
   if (w < R(0))
     w = -w, x = -x, y = -y, z = -z;
   q[0] = w, q[1] = x, q[2] = y, q[3] = z;
@@ -246,7 +241,7 @@ __global__ void __launch_bounds__(kThreads) k_vote_count(VoteArgs Ain, uint32_t 
 template <class R> struct VoteOut {
   R *q4;        // rigid: 4 lanes, stride 2P
   R *t3;        // rigid: 3 lanes, stride 2P
-  int32_t *rp;  // rigid pair ids
+  int32_t *rp;  // rigid pair ids (null: not written, the pair of vote v is v / 2)
   R *n3;        // mirror: 3 lanes, stride R_total
   R *d1;        // mirror offsets
   int32_t *fp;  // mirror pair ids
@@ -309,15 +304,23 @@ __global__ void __launch_bounds__(kThreads)
     F3 p = rf3(ra, L_X), q = rf3(rb, L_X);
     F3 an = rf3(ra, L_NX), ad = rf3(ra, L_DX), ax = rf3(ra, L_XX);
     F3 bn = rf3(rb, L_NX), bd = rf3(rb, L_DX), bx = rf3(rb, L_XX);
-    for (int sg = 0; sg < 2; ++sg) {
-      float s = sg == 0 ? 1.0f : -1.0f;
-      F3 b0 = bd * s, b1 = bx * s;
-      float Bc[3][3] = {{b0.x, b1.x, bn.x}, {b0.y, b1.y, bn.y}, {b0.z, b1.z, bn.z}};
-      float Ac[3][3] = {{ad.x, ax.x, an.x}, {ad.y, ax.y, an.y}, {ad.z, ax.z, an.z}};
-      float m[9];
+    // R = B A^T with B = [+-dmin_b, +-dmax_b, n_b]: the two signs share the
+    // first two column products u and the third v (R+ = u + v, R- = -u + v,
+    // the same roundings as forming each product in full)
+    float u[9], v9[9];
+    {
+      const float Bc[3][3] = {{bd.x, bx.x, bn.x}, {bd.y, bx.y, bn.y}, {bd.z, bx.z, bn.z}};
+      const float Ac[3][3] = {{ad.x, ax.x, an.x}, {ad.y, ax.y, an.y}, {ad.z, ax.z, an.z}};
       for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c)
-          m[r * 3 + c] = Bc[r][0] * Ac[c][0] + Bc[r][1] * Ac[c][1] + Bc[r][2] * Ac[c][2];
+        for (int c = 0; c < 3; ++c) {
+          u[r * 3 + c] = Bc[r][0] * Ac[c][0] + Bc[r][1] * Ac[c][1];
+          v9[r * 3 + c] = Bc[r][2] * Ac[c][2];
+        }
+    }
+    for (int sg = 0; sg < 2; ++sg) {
+      float m[9];
+      for (int e = 0; e < 9; ++e)
+        m[e] = (sg == 0 ? u[e] : -u[e]) + v9[e];
       float qq[4];
       quat<float>(m, qq);
       for (int kk = 0; kk < 4; ++kk)
@@ -365,9 +368,11 @@ __global__ void __launch_bounds__(kThreads)
     v.y = rt[1][kk];
     *reinterpret_cast<V2 *>(out.t3 + kk * P2 + 2 * k) = v;
   }
-  int2 ids;
-  ids.x = ids.y = static_cast<int32_t>(k);
-  *reinterpret_cast<int2 *>(out.rp + 2 * k) = ids;
+  if (out.rp) {  // optional: rigid vote v belongs to pair v / 2
+    int2 ids;
+    ids.x = ids.y = static_cast<int32_t>(k);
+    *reinterpret_cast<int2 *>(out.rp + 2 * k) = ids;
+  }
   if (refl && slot < out.R_total) {
     const uint64_t RT = out.R_total;
     out.n3[slot] = pn[0];

@@ -157,7 +157,8 @@ static int run(int argc, char **argv) {
   js.precision(17);
   js << std::boolalpha;
   js << "{\"ok\": " << (ok ? "true" : "false") << ", \"backend\": \"" << hip->name()
-     << "\", \"samples\": " << b.sample.size() << ", \"pruned\": " << b.pruned.size()
+     << "\", \"vertices\": " << mesh.vertices.size()
+     << ", \"samples\": " << b.sample.size() << ", \"pruned\": " << b.pruned.size()
      << ", \"pairs\": " << b.pairs.size() << ", \"votes\": " << b.votes.size()
      << ", \"clusters\": " << b.clusters.size() << ", \"valid_equal\": " << valid_eq
      << ", \"feature_max_rel\": " << feat.worst << ", \"sample_equal\": " << sample_eq
@@ -171,7 +172,12 @@ static int run(int argc, char **argv) {
     js << (i ? ", " : "") << "{\"stage\": \"" << metrics[i].stage
        << "\", \"interval_ms\": " << metrics[i].gpu_interval_ms
        << ", \"kernel_ms\": " << metrics[i].custom_kernel_ms
-       << ", \"launches\": " << metrics[i].custom_launches << "}";
+       << ", \"launches\": " << metrics[i].custom_launches
+       << ", \"h2d_bytes\": " << metrics[i].h2d_bytes
+       << ", \"d2h_bytes\": " << metrics[i].d2h_bytes
+       << ", \"h2d_ms\": " << metrics[i].h2d_ms << ", \"d2h_ms\": " << metrics[i].d2h_ms
+       << ", \"peak_buffer_bytes\": " << metrics[i].peak_buffer_bytes
+       << ", \"released\": " << (metrics[i].all_buffers_released ? "true" : "false") << "}";
   js << "]}";
   std::printf("%s\n", js.str().c_str());
   return ok ? 0 : 1;

@@ -436,16 +436,16 @@ int oracle_prune(int64_t nv, const double *kmin, const double *kmax,
 
 /* ------------------------------------------------------------- pairing ---
  * src/cpu/pairing.cpp:14-35 (brute force over ordinals a < b). */
-int oracle_pairing(int64_t nv, const double *x, const double *y, const double *z,
-                   const double *kmin, const double *kmax, int64_t ns,
-                   const int32_t *sel, double box, double t, uint64_t max_pairs,
-                   int64_t cap, int32_t *out_a, int32_t *out_b, int64_t *count) {
+/* Sources [a0, a1) of the pair loop; the pairs of source a follow those of
+ * source a - 1, so consecutive ranges concatenate into the CPU order. */
+static int pairing_range(int64_t nv, const double *x, const double *y, const double *z,
+                         const double *kmin, const double *kmax, int64_t ns,
+                         const int32_t *sel, double box, double t, uint64_t max_pairs,
+                         int64_t cap, int32_t *out_a, int32_t *out_b, int64_t *count,
+                         int64_t a0, int64_t a1) {
   int64_t c = 0;
   double dist = 0.03 * box;
-  for (int64_t s = 0; s < ns; ++s)
-    if (sel[s] < 0 || sel[s] >= nv)
-      return fail(OR_EINVAL, "selected vertex index out of range");
-  for (int64_t a = 0; a < ns; ++a) {
+  for (int64_t a = a0; a < a1; ++a) {
     int32_t i = sel[a];
     double pk0 = kmin[i], pk1 = kmax[i];
     vec3 pi = {x[i], y[i], z[i]};
@@ -474,6 +474,16 @@ int oracle_pairing(int64_t nv, const double *x, const double *y, const double *z
   }
   *count = c;
   return OR_OK;
+}
+int oracle_pairing(int64_t nv, const double *x, const double *y, const double *z,
+                   const double *kmin, const double *kmax, int64_t ns,
+                   const int32_t *sel, double box, double t, uint64_t max_pairs,
+                   int64_t cap, int32_t *out_a, int32_t *out_b, int64_t *count) {
+  for (int64_t s = 0; s < ns; ++s)
+    if (sel[s] < 0 || sel[s] >= nv)
+      return fail(OR_EINVAL, "selected vertex index out of range");
+  return pairing_range(nv, x, y, z, kmin, kmax, ns, sel, box, t, max_pairs, cap, out_a, out_b,
+                       count, 0, ns);
 }
 
 /* ----------------------------------------------------- transformations ---
@@ -1071,4 +1081,248 @@ int oracle_components(int64_t nv, const uint64_t *nbr_off, const int32_t *nbr_id
   free(allowed);
   free(seen);
   return OR_OK;
+}
+
+/* --------------------------------------------------- multi-threaded runs ---
+ * The CPU baseline on all host cores: the single-thread functions above run on
+ * independent units (pairing: ranges of sources; transformations: ranges of
+ * pairs; batched mean-shift: ranges of seeds), chunks claimed in order by
+ * `threads` POSIX threads, results concatenated in unit order, so every
+ * output equals the single-thread call's (checked in tests/test_oracle.py).
+ * Budgets are checked on the totals. */
+#include <pthread.h>
+#include <stdatomic.h>
+
+typedef struct {
+  int kind; /* 0 pairing, 1 transformations, 2 batched mean-shift */
+  int64_t units, chunk;
+  atomic_llong next;
+  int rc;
+  pthread_mutex_t lock;
+  /* pairing */
+  int64_t nv;
+  const double *x, *y, *z, *kmin, *kmax, *feat;
+  int64_t ns;
+  const int32_t *sel;
+  double box, t;
+  const double *center;
+  const int32_t *pa, *pb;
+  int32_t **ca, **cb;   /* per chunk outputs */
+  int64_t *cn;          /* per chunk counts */
+  int32_t **cpair;
+  uint8_t **ckind;
+  double **clanes;
+  int64_t *ccap;
+  /* mean-shift */
+  int dim, dim_rigid, dim_refl, iterations;
+  const double *pts;
+  const uint8_t *vkind, *seed_kind;
+  const double *seeds;
+  double h, cutoff;
+  double *out, *weight;
+} mt_job;
+
+static void *mt_worker(void *arg) {
+  mt_job *J = (mt_job *)arg;
+  for (;;) {
+    long long c = atomic_fetch_add(&J->next, 1);
+    int64_t u0 = (int64_t)c * J->chunk;
+    if (u0 >= J->units)
+      break;
+    int64_t u1 = u0 + J->chunk < J->units ? u0 + J->chunk : J->units;
+    int rc = OR_OK;
+    if (J->kind == 0) {
+      /* at most one pair per (source, later sample) */
+      int64_t cap = 0;
+      for (int64_t a = u0; a < u1; ++a)
+        cap += J->ns - 1 - a;
+      J->ca[c] = (int32_t *)malloc((size_t)(cap ? cap : 1) * sizeof(int32_t));
+      J->cb[c] = (int32_t *)malloc((size_t)(cap ? cap : 1) * sizeof(int32_t));
+      if (!J->ca[c] || !J->cb[c])
+        rc = fail(OR_EOVERFLOW, "out of host memory");
+      else
+        rc = pairing_range(J->nv, J->x, J->y, J->z, J->kmin, J->kmax, J->ns, J->sel, J->box,
+                           J->t, UINT64_MAX, cap, J->ca[c], J->cb[c], &J->cn[c], u0, u1);
+    } else if (J->kind == 1) {
+      int64_t cap = 3 * (u1 - u0);
+      J->ccap[c] = cap;
+      J->cpair[c] = (int32_t *)malloc((size_t)cap * sizeof(int32_t));
+      J->ckind[c] = (uint8_t *)malloc((size_t)cap);
+      J->clanes[c] = (double *)malloc((size_t)cap * 28 * sizeof(double));
+      if (!J->cpair[c] || !J->ckind[c] || !J->clanes[c])
+        rc = fail(OR_EOVERFLOW, "out of host memory");
+      else
+        rc = oracle_transformations(J->nv, J->x, J->y, J->z, J->feat, u1 - u0, J->pa + u0,
+                                    J->pb + u0, J->box, J->center, UINT64_MAX, cap, J->cpair[c],
+                                    J->ckind[c], J->clanes[c], &J->cn[c]);
+    } else {
+      int64_t m = u1 - u0;
+      double *sd = (double *)malloc((size_t)(J->dim * m) * sizeof(double));
+      double *so = (double *)malloc((size_t)(J->dim * m) * sizeof(double));
+      if (!sd || !so) {
+        rc = fail(OR_EOVERFLOW, "out of host memory");
+      } else {
+        for (int k = 0; k < J->dim; ++k)
+          memcpy(sd + (int64_t)k * m, J->seeds + (int64_t)k * J->units + u0,
+                 (size_t)m * sizeof(double));
+        rc = oracle_meanshift_batched(J->nv, J->dim, J->pts, J->vkind, J->dim_rigid,
+                                      J->dim_refl, m, sd, J->seed_kind + u0, J->h, J->cutoff,
+                                      J->iterations, so, J->weight + u0);
+        for (int k = 0; k < J->dim && rc == OR_OK; ++k)
+          memcpy(J->out + (int64_t)k * J->units + u0, so + (int64_t)k * m,
+                 (size_t)m * sizeof(double));
+      }
+      free(sd);
+      free(so);
+    }
+    if (rc != OR_OK) {
+      pthread_mutex_lock(&J->lock);
+      J->rc = rc;
+      pthread_mutex_unlock(&J->lock);
+    }
+  }
+  return NULL;
+}
+
+static int mt_run(mt_job *J, int threads) {
+  if (threads < 1)
+    threads = 1;
+  if (threads > 256)
+    threads = 256;
+  atomic_init(&J->next, 0);
+  J->rc = OR_OK;
+  pthread_mutex_init(&J->lock, NULL);
+  pthread_t tid[256];
+  int started = 0;
+  for (int k = 1; k < threads; ++k)
+    if (pthread_create(&tid[k], NULL, mt_worker, J) == 0)
+      ++started;
+    else
+      break;
+  mt_worker(J);
+  for (int k = 1; k <= started; ++k)
+    pthread_join(tid[k], NULL);
+  pthread_mutex_destroy(&J->lock);
+  return J->rc;
+}
+
+int oracle_pairing_mt(int threads, int64_t nv, const double *x, const double *y,
+                      const double *z, const double *kmin, const double *kmax, int64_t ns,
+                      const int32_t *sel, double box, double t, uint64_t max_pairs, int64_t cap,
+                      int32_t *out_a, int32_t *out_b, int64_t *count) {
+  if (ns < 0 || cap < 0)
+    return fail(OR_EINVAL, "negative size");
+  for (int64_t s = 0; s < ns; ++s)
+    if (sel[s] < 0 || sel[s] >= nv)
+      return fail(OR_EINVAL, "selected vertex index out of range");
+  mt_job J;
+  memset(&J, 0, sizeof J);
+  J.kind = 0;
+  J.units = ns;
+  J.chunk = 64;
+  J.nv = nv, J.x = x, J.y = y, J.z = z, J.kmin = kmin, J.kmax = kmax, J.ns = ns, J.sel = sel;
+  J.box = box, J.t = t;
+  int64_t nc = (ns + J.chunk - 1) / J.chunk;
+  J.ca = (int32_t **)calloc((size_t)(nc ? nc : 1), sizeof(int32_t *));
+  J.cb = (int32_t **)calloc((size_t)(nc ? nc : 1), sizeof(int32_t *));
+  J.cn = (int64_t *)calloc((size_t)(nc ? nc : 1), sizeof(int64_t));
+  int rc = J.ca && J.cb && J.cn ? mt_run(&J, threads) : fail(OR_EOVERFLOW, "out of host memory");
+  int64_t c = 0;
+  for (int64_t k = 0; k < nc && rc == OR_OK; ++k) {
+    if ((uint64_t)(c + J.cn[k]) > max_pairs) {
+      rc = fail(OR_EBUDGET, "Pair budget exceeded; lower --sample-ratio "
+                            "or tighten --pair-threshold");
+      break;
+    }
+    if (c + J.cn[k] > cap) {
+      rc = fail(OR_EOVERFLOW, "pair output capacity exceeded");
+      break;
+    }
+    memcpy(out_a + c, J.ca[k], (size_t)J.cn[k] * sizeof(int32_t));
+    memcpy(out_b + c, J.cb[k], (size_t)J.cn[k] * sizeof(int32_t));
+    c += J.cn[k];
+  }
+  for (int64_t k = 0; k < nc && J.ca; ++k) {
+    free(J.ca[k]);
+    free(J.cb[k]);
+  }
+  free(J.ca);
+  free(J.cb);
+  free(J.cn);
+  if (rc == OR_OK)
+    *count = c;
+  return rc;
+}
+
+int oracle_transformations_mt(int threads, int64_t nv, const double *x, const double *y,
+                              const double *z, const double *feat, int64_t np,
+                              const int32_t *pa, const int32_t *pb, double box,
+                              const double *center, uint64_t max_votes, int64_t cap,
+                              int32_t *pair, uint8_t *kind, double *lanes, int64_t *count) {
+  if (np < 0 || cap < 0)
+    return fail(OR_EINVAL, "negative size");
+  mt_job J;
+  memset(&J, 0, sizeof J);
+  J.kind = 1;
+  J.units = np;
+  J.chunk = 4096;
+  J.nv = nv, J.x = x, J.y = y, J.z = z, J.feat = feat, J.pa = pa, J.pb = pb, J.box = box;
+  J.center = center;
+  int64_t nc = (np + J.chunk - 1) / J.chunk;
+  J.cpair = (int32_t **)calloc((size_t)(nc ? nc : 1), sizeof(int32_t *));
+  J.ckind = (uint8_t **)calloc((size_t)(nc ? nc : 1), sizeof(uint8_t *));
+  J.clanes = (double **)calloc((size_t)(nc ? nc : 1), sizeof(double *));
+  J.ccap = (int64_t *)calloc((size_t)(nc ? nc : 1), sizeof(int64_t));
+  J.cn = (int64_t *)calloc((size_t)(nc ? nc : 1), sizeof(int64_t));
+  int rc = J.cpair && J.ckind && J.clanes && J.ccap && J.cn
+               ? mt_run(&J, threads)
+               : fail(OR_EOVERFLOW, "out of host memory");
+  int64_t c = 0;
+  for (int64_t k = 0; k < nc && rc == OR_OK; ++k) {
+    if ((uint64_t)(c + J.cn[k]) > max_votes) {
+      rc = fail(OR_EBUDGET, "Vote budget exceeded; lower --sample-ratio or "
+                            "tighten --pair-threshold");
+      break;
+    }
+    if (c + J.cn[k] > cap) {
+      rc = fail(OR_EOVERFLOW, "vote output capacity exceeded");
+      break;
+    }
+    for (int64_t v = 0; v < J.cn[k]; ++v) {
+      pair[c + v] = J.cpair[k][v] + (int32_t)(k * J.chunk);
+      kind[c + v] = J.ckind[k][v];
+    }
+    for (int l = 0; l < 28; ++l)
+      memcpy(lanes + (int64_t)l * cap + c, J.clanes[k] + (int64_t)l * J.ccap[k],
+             (size_t)J.cn[k] * sizeof(double));
+    c += J.cn[k];
+  }
+  for (int64_t k = 0; k < nc && J.cpair; ++k) {
+    free(J.cpair[k]);
+    free(J.ckind[k]);
+    free(J.clanes[k]);
+  }
+  free(J.cpair);
+  free(J.ckind);
+  free(J.clanes);
+  free(J.ccap);
+  free(J.cn);
+  if (rc == OR_OK)
+    *count = c;
+  return rc;
+}
+
+int oracle_meanshift_batched_mt(int threads, int64_t nv, int dim, const double *pts,
+                                const uint8_t *kind, int dim_rigid, int dim_refl, int64_t ns,
+                                const double *seeds, const uint8_t *seed_kind, double h,
+                                double cutoff, int iterations, double *out, double *weight) {
+  mt_job J;
+  memset(&J, 0, sizeof J);
+  J.kind = 2;
+  J.units = ns;
+  J.chunk = 16;
+  J.nv = nv, J.dim = dim, J.pts = pts, J.vkind = kind, J.dim_rigid = dim_rigid;
+  J.dim_refl = dim_refl, J.seeds = seeds, J.seed_kind = seed_kind, J.h = h, J.cutoff = cutoff;
+  J.iterations = iterations, J.out = out, J.weight = weight;
+  return mt_run(&J, threads);
 }

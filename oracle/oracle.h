@@ -163,6 +163,23 @@ int oracle_meanshift_batched(int64_t nv, int dim, const double *points,
                              const uint8_t *seed_kind, double h, double cutoff,
                              int iterations, double *out, double *weight);
 
+/* The same three computations on `threads` host threads (units: sources,
+ * pairs, seeds), outputs identical to the single-thread calls'. */
+int oracle_pairing_mt(int threads, int64_t nv, const double *x, const double *y,
+                      const double *z, const double *kmin, const double *kmax, int64_t ns,
+                      const int32_t *sel, double box, double t, uint64_t max_pairs, int64_t cap,
+                      int32_t *out_a, int32_t *out_b, int64_t *count);
+int oracle_transformations_mt(int threads, int64_t nv, const double *x, const double *y,
+                              const double *z, const double *feat, int64_t np,
+                              const int32_t *pa, const int32_t *pb, double box,
+                              const double *center, uint64_t max_votes, int64_t cap,
+                              int32_t *pair, uint8_t *kind, double *lanes, int64_t *count);
+int oracle_meanshift_batched_mt(int threads, int64_t nv, int dim, const double *points,
+                                const uint8_t *kind, int dim_rigid, int dim_reflection,
+                                int64_t ns, const double *seeds, const uint8_t *seed_kind,
+                                double h, double cutoff, int iterations, double *out,
+                                double *weight);
+
 /* src/core/math.cpp:72-112. */
 double oracle_point_triangle_distance2(const double *p, const double *a, const double *b,
                                        const double *c);

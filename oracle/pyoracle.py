@@ -216,8 +216,17 @@ class Oracle:
         self._check(rc)
         return sample[:ns.value].copy(), pruned[:npr.value].copy()
 
+    def _mt(self, name, threads):
+        """The single-thread entry, or (threads > 1, this oracle only) its
+        multi-threaded counterpart with the thread count bound first."""
+        if threads <= 1:
+            return self._fn(name)
+        assert self.which == "oracle", "only the oracle has multi-threaded runs"
+        f = self._fn(name + "_mt")
+        return lambda *a: f(C.c_int(threads), *a)
+
     def pairing(self, vertices, kmin, kmax, selected, box: float, threshold: float,
-                max_pairs: int = 200000, cap: int | None = None):
+                max_pairs: int = 200000, cap: int | None = None, threads: int = 1):
         v = np.ascontiguousarray(vertices, dtype=np.float64)
         x, y, z = (np.ascontiguousarray(v[:, k]) for k in range(3))
         s = np.ascontiguousarray(selected, dtype=np.int32)
@@ -228,7 +237,7 @@ class Oracle:
         a = np.zeros(cap, np.int32)
         b = np.zeros(cap, np.int32)
         cnt = C.c_int64()
-        rc = self._fn("pairing")(C.c_int64(len(v)), _p(x), _p(y), _p(z),
+        rc = self._mt("pairing", threads)(C.c_int64(len(v)), _p(x), _p(y), _p(z),
                                  _p(np.ascontiguousarray(kmin, dtype=np.float64)),
                                  _p(np.ascontiguousarray(kmax, dtype=np.float64)),
                                  C.c_int64(ns), _p(s), C.c_double(box), C.c_double(threshold),
@@ -238,7 +247,7 @@ class Oracle:
         return a[:cnt.value].copy(), b[:cnt.value].copy()
 
     def transformations(self, vertices, feat, pa, pb, box: float, center,
-                        max_votes: int = 600000, cap: int | None = None):
+                        max_votes: int = 600000, cap: int | None = None, threads: int = 1):
         v = np.ascontiguousarray(vertices, dtype=np.float64)
         x, y, z = (np.ascontiguousarray(v[:, k]) for k in range(3))
         feat = np.ascontiguousarray(feat, dtype=np.float64)
@@ -252,7 +261,7 @@ class Oracle:
         lanes = np.zeros((28, cap))
         ctr = np.ascontiguousarray(center, dtype=np.float64)
         cnt = C.c_int64()
-        rc = self._fn("transformations")(C.c_int64(len(v)), _p(x), _p(y), _p(z), _p(feat),
+        rc = self._mt("transformations", threads)(C.c_int64(len(v)), _p(x), _p(y), _p(z), _p(feat),
                                          C.c_int64(len(pa)), _p(pa), _p(pb), C.c_double(box),
                                          _p(ctr), C.c_uint64(max_votes), C.c_int64(cap),
                                          _p(pair), _p(kind), _p(lanes), C.byref(cnt))
@@ -289,7 +298,7 @@ class Oracle:
                         lim.value)
 
     def meanshift_batched(self, points, kind, dim_rigid, dim_reflection, seeds,
-                          seed_kind, h, cutoff, iterations):
+                          seed_kind, h, cutoff, iterations, threads: int = 1):
         assert self.which == "oracle"
         pts = np.ascontiguousarray(points, dtype=np.float64)
         dim, nv = pts.shape
@@ -297,7 +306,7 @@ class Oracle:
         ns = sd.shape[1]
         out = np.zeros((dim, ns))
         w = np.zeros(ns)
-        rc = self._fn("meanshift_batched")(
+        rc = self._mt("meanshift_batched", threads)(
             C.c_int64(nv), C.c_int(dim), _p(pts), _p(np.ascontiguousarray(kind, dtype=np.uint8)),
             C.c_int(dim_rigid), C.c_int(dim_reflection), C.c_int64(ns), _p(sd),
             _p(np.ascontiguousarray(seed_kind, dtype=np.uint8)), C.c_double(h),

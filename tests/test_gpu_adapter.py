@@ -42,6 +42,17 @@ def test_detect_through_adapter_matches_cpu(name):
     stages = {m["stage"] for m in out["metrics"]}
     assert {"curvature", "sample_prune", "pairing", "transformation", "mean_shift",
             "verification"} <= stages
+    # measured profile (src/cuda/profile.cpp:61-93, tests/cuda_profile.cpp:36-42):
+    # every stage copies both ways, holds device buffers and releases them all
+    for m in out["metrics"]:
+        assert m["released"] and m["peak_buffer_bytes"] > 0, m
+        assert m["h2d_bytes"] > 0 and m["d2h_bytes"] > 0 and m["h2d_ms"] >= 0, m
+    # exact bytes of the select stage: kmin, kmax (FP64) and valid per
+    # vertex plus the sample ids in; the range flag, the kept count and the
+    # kept ids out
+    sp = next(m for m in out["metrics"] if m["stage"] == "sample_prune")
+    assert sp["h2d_bytes"] == 17 * out["vertices"] + 4 * out["samples"], (sp, out["vertices"])
+    assert sp["d2h_bytes"] == 4 + 8 + 4 * out["pruned"], sp
 
 
 def test_detect_through_adapter_default_options():

@@ -118,18 +118,18 @@ def _clusters(D, seed, spread=1.0):
 
 # spread 1: half-width cells (reach 2); spread 8: cells of side 3h (reach 1);
 # spread 200: reach 1 with cells doubled until the grid fits 2^22 cells
-# wide: a block of four waves per seed (rows cut to the support ball, the
-# first wave finding the rows); narrow: one wave per seed
-@pytest.mark.parametrize("wide", [True, False])
+# lanes 256: a block of four waves per seed (rows cut to the support ball,
+# the first wave finding the rows); 64, 32, 16: lanes per narrow seed
+@pytest.mark.parametrize("lanes", [256, 64, 32, 16])
 @pytest.mark.parametrize("spread", [1.0, 8.0, 200.0])
 @pytest.mark.parametrize("D", [6, 4])
-def test_batched_meanshift_matches_oracle(oracle, D, spread, wide):
+def test_batched_meanshift_matches_oracle(oracle, D, spread, lanes):
     P, seeds = _clusters(D, 7 + D, spread)
     h, cut, iters = 0.05, 3.0, 25
     want, wsum = oracle.meanshift_batched(P, np.zeros(P.shape[1], np.uint8), D, D, seeds,
                                           np.zeros(seeds.shape[1], np.uint8), h, cut, iters)
     s = capi.Session(0)
-    s.set_tuning(0, 2048, 1 << 40 if wide else 0)
+    s.set_tuning(0, 2048, 1 << 40 if lanes == 256 else 0, 64 if lanes == 256 else lanes)
     dev = torch.device("cuda:0")
     s.set_stream(torch.cuda.current_stream(dev).cuda_stream)
     for prec, dt, tol in ((capi.FP64, torch.float64, 1e-9), (capi.FP32_GUARDED, torch.float32,
@@ -148,17 +148,17 @@ def test_batched_meanshift_matches_oracle(oracle, D, spread, wide):
     s.close()
 
 
-@pytest.mark.parametrize("wide", [True, False])
+@pytest.mark.parametrize("lanes", [256, 64, 32, 16])
 @pytest.mark.parametrize("spread", [1.0, 8.0])
 @pytest.mark.parametrize("D", [6, 4])
-def test_meanshift_support_visits_match_brute_force(D, spread, wide):
+def test_meanshift_support_visits_match_brute_force(D, spread, lanes):
     """One update from each seed visits exactly the points inside its support
     (counted by brute force in FP64): the grid walk, with its rows cut to the
     support ball, drops no point of the support."""
     P, seeds = _clusters(D, 3 + D, spread)
     h, cut = 0.05, 3.0
     s = capi.Session(0)
-    s.set_tuning(0, 2048, 1 << 40 if wide else 0)
+    s.set_tuning(0, 2048, 1 << 40 if lanes == 256 else 0, 64 if lanes == 256 else lanes)
     dev = torch.device("cuda:0")
     s.set_stream(torch.cuda.current_stream(dev).cuda_stream)
     pts = torch.as_tensor(P, dtype=torch.float64, device=dev).contiguous()
@@ -208,7 +208,7 @@ def _fetch_pairvote(sess, r, prec):
         a=sess.fetch(r.a, (P,), torch.int32), b=sess.fetch(r.b, (P,), torch.int32),
         q4=sess.fetch(r.rigid_q4, (4, 2 * P), dt, 2 * P),
         t3=sess.fetch(r.rigid_t3, (3, 2 * P), dt, 2 * P),
-        rp=sess.fetch(r.rigid_pair, (2 * P,), torch.int32),
+
         n3=sess.fetch(r.refl_n3, (3, R), dt, st), d1=sess.fetch(r.refl_d, (R,), dt),
         fp=sess.fetch(r.refl_pair, (R,), torch.int32))
 
@@ -226,11 +226,17 @@ def test_fused_pairvote_matches_stepwise(cfg1, prec):
     assert torch.equal(fused["q4"], step.q4[:, :2 * P]) and torch.equal(fused["t3"],
                                                                        step.t3[:, :2 * P])
     assert torch.equal(fused["n3"], step.n3[:, :R]) and torch.equal(fused["d1"], step.d1[:R])
-    assert torch.equal(fused["fp"], step.fp[:R]) and torch.equal(fused["rp"], step.rp[:2 * P])
+    assert torch.equal(fused["fp"], step.fp[:R])
+    assert r_fused_rigid_pair_is_implicit(wl, prec)
     with pytest.raises(capi.MvsBudgetError):
         wl.pairvote(prec, max_pairs=max(P - 1, 0))
     with pytest.raises(capi.MvsBudgetError):
         wl.pairvote(prec, max_votes=2 * P + R - 1)
+
+
+def r_fused_rigid_pair_is_implicit(wl, prec):
+    # the fused step writes no rigid pair ids: vote v belongs to pair v / 2
+    return wl.pairvote(prec).rigid_pair is None
 
 
 def _rodrigues(r):

@@ -255,3 +255,35 @@ def test_tetrahedron_topology(oracle):
     assert [tuple(e[:2]) for e in t.edges] == [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
     assert np.all(t.edges[:, 5] >= 0)
     assert t.inc_idx.tolist()[:3] == [0, 1, 2]
+
+
+def test_multithreaded_oracle_equals_single_thread(oracle):
+    """The all-cores CPU baseline runs the single-thread code on ranges of
+    sources, pairs and seeds; its outputs equal the single-thread call's."""
+    from mitra_algo_amd import meshgen
+    mesh = meshgen.cfg1_mesh()
+    v, f = mesh.vertices, mesh.faces
+    topo = oracle.prepare(v, f)
+    feat, valid = oracle.curvature(v, topo, 3)
+    g = oracle.rng(9)
+    _, sel = oracle.select(topo.weights, feat[9], feat[10], valid, 0.3, 0.85, g)
+    one = oracle.pairing(v, feat[9], feat[10], sel, topo.box_size, 0.1, 10**8)
+    many = oracle.pairing(v, feat[9], feat[10], sel, topo.box_size, 0.1, 10**8, threads=5)
+    assert len(one[0]) > 10000
+    assert np.array_equal(one[0], many[0]) and np.array_equal(one[1], many[1])
+    with pytest.raises(OracleError):
+        oracle.pairing(v, feat[9], feat[10], sel, topo.box_size, 0.1, len(one[0]) - 1,
+                       threads=3)
+    t1 = oracle.transformations(v, feat, one[0], one[1], topo.box_size, topo.center, 10**8)
+    t4 = oracle.transformations(v, feat, one[0], one[1], topo.box_size, topo.center, 10**8,
+                                threads=4)
+    for x, y in zip(t1, t4):
+        assert np.array_equal(x, y)
+    rng = np.random.default_rng(5)
+    pts = rng.normal(0, 0.1, size=(6, 3000))
+    kind = (rng.random(3000) < 0.3).astype(np.uint8)
+    seeds = pts[:, :70].copy()
+    sk = kind[:70].copy()
+    a = oracle.meanshift_batched(pts, kind, 6, 4, seeds, sk, 0.05, 3.0, 12)
+    b = oracle.meanshift_batched(pts, kind, 6, 4, seeds, sk, 0.05, 3.0, 12, threads=6)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
