@@ -300,7 +300,10 @@ __global__ void __launch_bounds__(kThreads)
       pd = static_cast<R>(g.d + dot(g.n, c));
     }
   } else {
-    // FP32 frames in centre-relative coordinates: t_c = q' - R p'
+    // FP32 frames in centre-relative coordinates: t_c = q' - R p'.  The
+    // vote values (not the mirror decision, which the count pass must repeat
+    // bit for bit) may fuse products into sums: one rounding fewer each.
+#pragma clang fp contract(fast)
     F3 p = rf3(ra, L_X), q = rf3(rb, L_X);
     F3 an = rf3(ra, L_NX), ad = rf3(ra, L_DX), ax = rf3(ra, L_XX);
     F3 bn = rf3(rb, L_NX), bd = rf3(rb, L_DX), bx = rf3(rb, L_XX);

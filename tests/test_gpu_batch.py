@@ -138,9 +138,12 @@ def test_batch_graph_replays_the_captured_step(batch):
         assert torch.equal(poff, batch.sess.fetch(pv.pair_offsets, (M + 1,), torch.int64).cpu())
 
 
-@pytest.mark.skip(reason="a replay over inputs with other pair counts made a GPU memory "
-                         "access fault; skipped until its cause is found and fixed")
 def test_batch_graph_replay_over_changed_inputs_is_reported(batch):
+    """A replay over changed inputs is reported by the check, and the graph
+    then replays the captured inputs bit for bit again.  (Its earlier fault
+    came from points past the last mesh segment reading the grid past the
+    last one; tests/test_replay_index.py shows that read on the CPU and that
+    the sentinel key keeps every index of the set-up inside its buffer.)"""
     prec = capi.FP32_GUARDED
     pv, ms = batch.capture(prec, 0.2, 12)
     first = _fetch(batch.sess, pv.all, prec)
