@@ -580,6 +580,21 @@ int mvs_dev_ms_batched(mvs_session *s, int precision, int dim, uint64_t n, const
                        void *out, void *weight, uint64_t *candidates, uint64_t *in_support,
                        uint64_t *updates);
 
+/* mvs_dev_ms_batched that allocates nothing: no hipMalloc, hipFree or pinned
+ * host buffer in the call; its scratch is a 16 MiB static device array of
+ * the library, one per GPU, kept until the process ends.  No counterpart in
+ * the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  At most
+ * MVS_NOALLOC_MAX_POINTS points and MVS_NOALLOC_MAX_SEEDS seeds (larger:
+ * MVS_EINVAL); same kernels and results as mvs_dev_ms_batched, without the
+ * visit statistics; synchronises once (the non-finite-input check).  Calls
+ * that overlap on one GPU share the array and race. */
+#define MVS_NOALLOC_MAX_POINTS 65536
+#define MVS_NOALLOC_MAX_SEEDS 8192
+int mvs_dev_ms_batched_noalloc(mvs_session *s, int precision, int dim, uint64_t n,
+                               const void *pts, uint64_t ns, const void *seeds, int iterations,
+                               double h, double cutoff, void *out, void *weight);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,8 +34,20 @@ EXPORTS = [
     "mvs_verification", "mvs_verification_fetch", "mvs_verification_rows",
     "mvs_verification_stats", "mvs_verification_surface", "mvs_dev_verification",
     "mvs_step_graph_capture", "mvs_graph_launch", "mvs_graph_check", "mvs_thread_metrics",
-    "mvs_thread_metrics_reset",
+    "mvs_thread_metrics_reset", "mvs_dev_ms_batched_noalloc",
 ]
+
+
+class MemoryMetrics(C.Structure):
+    _fields_ = [("live_bytes", C.c_uint64), ("peak_bytes", C.c_uint64),
+                ("h2d_bytes", C.c_uint64), ("d2h_bytes", C.c_uint64)]
+
+
+def thread_metrics() -> dict:
+    """Device-memory and copy counters of this thread's libmvs calls."""
+    m = MemoryMetrics()
+    check(lib().mvs_thread_metrics(C.byref(m)))
+    return {k: getattr(m, k) for k, _ in MemoryMetrics._fields_}
 
 
 class MvsError(RuntimeError):
@@ -727,6 +739,13 @@ class Session:
         if stats == "updates":
             return cand.value, ins.value, upd.value
         return (cand.value, ins.value) if stats else None
+
+    def dev_ms_batched_noalloc(self, precision, dim, n, pts, ns, seeds, iterations, h, cutoff,
+                               out, weight):
+        check(lib().mvs_dev_ms_batched_noalloc(
+            self.h, C.c_int(precision), C.c_int(dim), C.c_uint64(n), ptr(pts), C.c_uint64(ns),
+            ptr(seeds), C.c_int(iterations), C.c_double(h), C.c_double(cutoff), ptr(out),
+            ptr(weight)))
 
     def dev_vote_count(self, P, a, b, precision) -> int:
         R = C.c_uint64()

@@ -1193,6 +1193,8 @@ static MsCtl *ms_ctl(Session *s) {
 
 void ms_begin(Session *s) { zero_device(s, ms_ctl(s), sizeof(MsCtl)); }
 
+unsigned *ms_bad_flag(Session *s) { return &ms_ctl(s)->bad; }
+
 void ms_finish(Session *s, unsigned long long sums[kMsSums]) {
   if (s->side_pending) {  // the side stream's query and sums come first
     MVS_HIP(hipStreamWaitEvent(s->stream, s->join_event, 0));
@@ -1256,7 +1258,10 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
       static_cast<char *>(scratch(s, slot_rec, (np + 1) * (sizeof(V4) + sizeof(V2))));
   V4 *p4 = reinterpret_cast<V4 *>(sorted);
   V2 *q2 = reinterpret_cast<V2 *>(sorted + (np + 1) * sizeof(V4));
-  const uint64_t cells_cap = seg ? std::max<uint64_t>(seg->cells_bound, 1) : kMaxCells;
+  // cells: at most clamp(2 n, 1024, kMaxCells) for one point set (the bound
+  // k_grid_setup lays its grid under), the caller's bound for segments
+  const uint64_t one_cap = n < kMaxCells / 2 ? std::max<uint64_t>(2 * n, 1024) : kMaxCells;
+  const uint64_t cells_cap = seg ? std::max<uint64_t>(seg->cells_bound, 1) : one_cap;
   uint64_t *off = scratch_as<uint64_t>(s, slot_off, cells_cap + 1);
   Grid *grids = seg ? scratch_as<Grid>(s, kSlotBatchMs0, meshes)
                     : (run.set ? &ctl->grid2 : &ctl->grid);

@@ -97,6 +97,8 @@ void ms_batched_f64(Session *s, int D, uint64_t n, const double *pts, uint64_t n
                     const double *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
                     double h, double cutoff, double *out, double *weight, int run_slot,
                     const MsSegs *seg = nullptr, MsRun run = {});
+// The non-finite-input flag of the current mean-shift call (device word).
+unsigned *ms_bad_flag(Session *s);
 // Compact votes to mean-shift points; mirror normals are 3 rows of n3_stride.
 void ms_points_device(Session *s, int precision, uint64_t nrigid, const void *q4, const void *t3,
                       uint64_t nrefl, const void *n3, uint64_t n3_stride, const void *d1,

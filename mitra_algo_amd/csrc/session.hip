@@ -42,6 +42,18 @@ Session *as_session(mvs_session *s) {
 hipStream_t session_stream(Session *s) { return s->stream; }
 
 void *scratch(Session *s, int slot, size_t bytes) {
+  if (Arena *a = s->arena) {
+    const size_t want = bytes ? bytes : 1;
+    if (a->len[slot] >= want)
+      return a->base + a->off[slot];
+    const size_t at = (a->used + 255) / 256 * 256;
+    if (at > a->cap || want > a->cap - at)
+      raise(MVS_EINVAL, "the allocation-free call needs more scratch than its static buffer");
+    a->off[slot] = at;
+    a->len[slot] = want;
+    a->used = at + want;
+    return a->base + at;
+  }
   auto &b = s->slots[slot];
   if ((bytes ? bytes : 1) > b.n) {
     if (s->replay == Replay::kCapture)

@@ -56,8 +56,19 @@ struct Tuning {
   unsigned seed_lanes = 32;    // lanes per seed of the narrow mean-shift query (16, 32, 64)
 };
 
+// Caller-provided device memory that serves every scratch request of one
+// call instead of the session's slots (the allocation-free entry point): each
+// slot gets a 256-byte aligned region on its first request in the call, a
+// larger later request a new one; nothing is allocated or freed.
+struct Arena {
+  unsigned char *base = nullptr;
+  size_t cap = 0, used = 0;
+  size_t off[kSlotCount] = {}, len[kSlotCount] = {};
+};
+
 struct Session {
   int device = 0;
+  Arena *arena = nullptr;  // set for the duration of an allocation-free call
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
   // second stream for independent work forked from `stream` and joined back
