@@ -48,6 +48,93 @@ def load_traffic():
         return {}
 
 
+def load_curvature_traffic():
+    """HBM bytes per curvature call from the committed rocprofv3 --pmc
+    summary of the curvature kernels, or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic_curvature.json")) as f:
+            return json.load(f)["bytes_per_call"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def curvature_bytes(N, nnz, E, smoothing, s=8):
+    """Algorithmic HBM bytes of one curvature call: per smoothing pass N*9s
+    written, (N + nnz)*9s read (each ring entry's tensor once: an upper
+    bound, as caches serve repeats) and the ring CSR 8(N+1) + 4 nnz; the
+    edge pass E*(24 + 9s); the frames N*(9s + 3s + 11s + 1)."""
+    per = N * 9 * s + (N + nnz) * 9 * s + 8 * (N + 1) + 4 * nnz
+    return smoothing * per + E * (24 + 9 * s) + N * (9 * s + 3 * s + 11 * s + 1)
+
+
+def bench_curvature(wl, steps):
+    """mvs_dev_curvature on the resident mesh and topology: wall time per
+    call (synchronised), per-kernel device times, and the HBM roofline of
+    the whole call."""
+    import torch
+    names = ("curv_sum", "curv_mean", "curv_greater", "curv_gather", "curv_smooth", "curv_frames")
+    wl.curvature()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        wl.curvature()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    wl.sess.enable_timing(True, only=names)
+    for _ in range(steps):
+        wl.curvature()
+    torch.cuda.synchronize()
+    kern = {}
+    for k in names:
+        ms_, n = wl.sess.timing(k)
+        if n:
+            kern[k] = {"ms_per_call": ms_ / steps, "launches_per_call": n / steps}
+    wl.sess.enable_timing(False)
+    N, nnz, E = wl.n, int(wl.nbr_idx.numel()), int(wl.edges.shape[0])
+    B = curvature_bytes(N, nnz, E, wl.smoothing)
+    gbs = B / dt / 1e9
+    return {"ms_per_call": dt * 1e3, "vertices": N, "ring_entries": nnz, "edges": E,
+            "smoothing": wl.smoothing, "kernels": kern,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": load_curvature_traffic(),
+                         "algorithmic_bytes": B, "kernel": "whole call (gather, 3 smoothing "
+                                                           "passes, frames, edge-length sum)"}}
+
+
+def bench_end_to_end(mesh, precision, runs=3):
+    """Host mesh to modes on the device: device prepare, curvature, prune,
+    pairs + votes, mean-shift (workload.end_to_end), median of `runs` after a
+    warm-up, once more with a synchronisation after every stage for the
+    split; the host prepare (mvs_prepare) is timed beside it."""
+    import numpy as np
+    import torch
+    from mitra_algo_amd import capi, meshgen, workload
+    soa = [np.ascontiguousarray(mesh.vertices[:, k]) for k in range(3)] + \
+          [np.ascontiguousarray(mesh.faces[:, k].astype(np.int32)) for k in range(3)]
+    samples = meshgen.uniform_samples(100000, len(mesh.vertices), 3)
+    sess = capi.Session(0)
+    sess.set_stream(torch.cuda.current_stream().cuda_stream)
+    run = lambda sync=False: workload.end_to_end(sess, soa, samples, 0.02, precision, SEEDS,
+                                                 ITERS, H, CUT, stage_sync=sync)
+    run()
+    totals = []
+    for _ in range(runs):
+        d, S, pv, ms, t = run()
+        totals.append(t["total"])
+    stages = run(True)[4]
+    t0 = time.perf_counter()
+    capi.prepare(mesh.vertices, mesh.faces)
+    host_prepare = time.perf_counter() - t0
+    sess.close()
+    return {"ms": float(np.median(totals)) * 1e3, "runs_ms": [x * 1e3 for x in totals],
+            "stages_ms": {k: v * 1e3 for k, v in stages.items()},
+            "selected": S, "pairs": pv.pairs, "reflection_votes": pv.reflections,
+            "host_prepare_ms": host_prepare * 1e3,
+            "what": "host SoA mesh -> mvs_dev_prepare -> mvs_dev_curvature -> mvs_dev_prune "
+                    "(100,000 uniform samples) -> mvs_dev_pairvote -> mvs_dev_meanshift_votes "
+                    "(20,000 seeds x 30 iterations), modes on the device"}
+
+
 def emission_bytes(P, R, S, s):
     """Algorithmic bytes of vote emission: pair ordinals in, compact records
     out (rigid q4+t3 at s bytes per scalar, the pair being vote / 2; mirror
@@ -776,6 +863,13 @@ def main():
         "roofline": roof,
         "rooflines": rooflines,
     }
+    out["curvature"] = bench_curvature(wl, max(3, args.steps // 2))
+    e2e = bench_end_to_end(wl.mesh, capi.FP32_GUARDED)
+    e2e["pairs_equal_resident_step"] = (e2e["pairs"] == P and e2e["reflection_votes"] == R)
+    out["end_to_end_fp32g"] = e2e
+    print(f"[bench] curvature {out['curvature']['ms_per_call']:.3f} ms, end-to-end "
+          f"{e2e['ms']:.2f} ms (host prepare alone {e2e['host_prepare_ms']:.1f} ms)",
+          file=sys.stderr, flush=True)
     if not args.no_cpu:
         # the FP64 mean-shift points of the step (the CPU leg's vote set)
         f = wl.sess.fetch

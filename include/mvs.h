@@ -120,6 +120,18 @@ typedef struct {
 int mvs_prepare(int64_t nv, const double *x, const double *y, const double *z, int64_t nf,
                 const int32_t *f0, const int32_t *f1, const int32_t *f2, mvs_prepared *out);
 
+/* prepare() on the device: the same outputs as mvs_prepare, bit for bit, but
+ * every array pointer in *out is a DEVICE pointer (same capacities) and the
+ * mesh is also written to xyz, 3*nv device doubles (x, then y, then z), so
+ * curvature and the stages after it start from HBM.  The mesh itself comes
+ * from host arrays, as the reference's Mesh does; counts, box size, centre
+ * and diagnostics come back in *out on the host.  Same errors and messages
+ * as mvs_prepare (src/core/prepare.cpp:102-149); at most 2^32/3 faces.
+ * Work is queued on the session stream; the call reads back two counts. */
+int mvs_dev_prepare(mvs_session *s, int64_t nv, const double *x, const double *y,
+                    const double *z, int64_t nf, const int32_t *f0, const int32_t *f1,
+                    const int32_t *f2, double *xyz, mvs_prepared *out);
+
 /* std::mt19937_64 on a caller-held state (libstdc++ layout). */
 void mvs_mt64_seed(mvs_mt64 *g, uint64_t seed);
 uint64_t mvs_mt64_next(mvs_mt64 *g);
@@ -182,6 +194,13 @@ int mvs_curvature(mvs_session *s, const mvs_points *vertices, const mvs_topology
  * |kmin-kmax|/max > 1-prune, order preserved.  out capacity: sample_count. */
 int mvs_prune(mvs_session *s, const mvs_features *f, uint64_t sample_count,
               const int32_t *sample, double prune, int32_t *out, uint64_t *count);
+/* mvs_prune on device arrays: feat is the 11-lane SoA of mvs_dev_curvature
+ * (stride n; kmin lane 9, kmax lane 10), valid n bytes, sample ns device ids;
+ * the kept ids go to out (device, capacity ns) in order, their count to
+ * *count (host). */
+int mvs_dev_prune(mvs_session *s, uint64_t n, const double *feat, const uint8_t *valid,
+                  uint64_t sample_count, const int32_t *sample, double prune, int32_t *out,
+                  uint64_t *count);
 
 /* Backend::pairing (pipeline.h:60-61; src/cpu/pairing.cpp:14-35).  Computes
  * the pair set in CPU order and keeps it on the device; *count receives P.

@@ -24,6 +24,7 @@ EXPORTS = [
     "mvs_dev_timing",
     "mvs_curvature", "mvs_prune", "mvs_pairing", "mvs_pairing_fetch", "mvs_transformations",
     "mvs_transformations_fetch", "mvs_clustering", "mvs_clustering_fetch",
+    "mvs_dev_prepare", "mvs_dev_prune",
     "mvs_dev_curvature", "mvs_dev_pair_count", "mvs_dev_pair_fill", "mvs_dev_pair_stats",
     "mvs_dev_vote_count", "mvs_dev_vote_emit", "mvs_dev_ms_points", "mvs_dev_ms_batched",
     "mvs_dev_pairvote", "mvs_dev_meanshift_votes", "mvs_dev_copy", "mvs_dev_batch_pairvote",
@@ -540,6 +541,49 @@ class Session:
         return acc[:m].reshape(clusters, 2, n), d2[:m].reshape(clusters, 2, n)
 
     # ---------------------------------------------------- device layer
+    def dev_prepare(self, vertices: np.ndarray, faces: np.ndarray, device=None) -> dict:
+        """mvs_dev_prepare: the host mesh in, the prepared mesh and topology
+        as device tensors (capacities as mvs_prepare's; face lanes have stride
+        nf, the input face count) plus the host scalars."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        return self.dev_prepare_soa(*(np.ascontiguousarray(v[:, k]) for k in range(3)),
+                                    *(np.ascontiguousarray(f[:, k]) for k in range(3)),
+                                    device=device)
+
+    def dev_prepare_soa(self, x, y, z, f0, f1, f2, device=None) -> dict:
+        """dev_prepare over the reference's SoA mesh layout (contiguous
+        float64 x, y, z and int32 face corner lanes)."""
+        import torch
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        nv, nf = len(x), len(f0)
+        cap = max(nf, 1)
+        e = lambda *shape, dt=torch.float64: torch.empty(shape, dtype=dt, device=dev)
+        out = dict(xyz=e(3, max(nv, 1)), faces3=e(3, cap, dt=torch.int32),
+                   normals=e(3, max(nv, 1)), face_normals=e(3, cap), weights=e(max(nv, 1)),
+                   nbr_off=e(nv + 1, dt=torch.int64), nbr_idx=e(6 * cap, dt=torch.int32),
+                   inc_off=e(nv + 1, dt=torch.int64), inc_idx=e(3 * cap, dt=torch.int32),
+                   edges=e(3 * cap, 6, dt=torch.int32))
+        P = Prepared()
+        P.f0, P.f1, P.f2 = (out["faces3"][k].data_ptr() for k in range(3))
+        P.nx, P.ny, P.nz = (out["normals"][k].data_ptr() for k in range(3))
+        P.fnx, P.fny, P.fnz = (out["face_normals"][k].data_ptr() for k in range(3))
+        for k in ("weights", "nbr_off", "nbr_idx", "inc_off", "inc_idx", "edges"):
+            setattr(P, k, out[k].data_ptr())
+        check(lib().mvs_dev_prepare(self.h, C.c_int64(nv), ptr(x), ptr(y), ptr(z), C.c_int64(nf),
+                                    ptr(f0), ptr(f1), ptr(f2), ptr(out["xyz"]), C.byref(P)))
+        out.update(nv=nv, nf=nf, face_count=P.face_count, nbr_nnz=P.nbr_nnz,
+                   inc_nnz=P.inc_nnz, edge_count=P.edge_count, box_size=P.box_size,
+                   center=np.array(list(P.center)), rejected_faces=P.rejected_faces)
+        return out
+
+    def dev_prune(self, n, feat, valid, sample, prune: float, out) -> int:
+        cnt = C.c_uint64()
+        check(lib().mvs_dev_prune(self.h, C.c_uint64(n), ptr(feat), ptr(valid),
+                                  C.c_uint64(sample.numel()), ptr(sample), C.c_double(prune),
+                                  ptr(out), C.byref(cnt)))
+        return cnt.value
+
     def dev_curvature(self, n, x, y, z, normals3, nf, fn3, off, idx, ne, edges, smoothing,
                       feat, valid):
         check(lib().mvs_dev_curvature(self.h, C.c_uint64(n), ptr(x), ptr(y), ptr(z), ptr(normals3),

@@ -131,6 +131,9 @@ enum Slot : int {
   kSlotVerNbrIdx, kSlotVerAccepted, kSlotVerLabel, kSlotVerDist, kSlotVerPos, kSlotVerBounds,
   kSlotVerVid, kSlotVerRoot, kSlotVerD2, kSlotVerStats, kSlotVerKeys, kSlotVerKeys2,
   kSlotVerIds, kSlotVerOrder,
+  kSlotPrep0, kSlotPrep1, kSlotPrep2, kSlotPrep3, kSlotPrep4, kSlotPrep5, kSlotPrep6,
+  kSlotPrep7, kSlotPrep8, kSlotPrep9, kSlotPrep10, kSlotPrep11, kSlotPrep12, kSlotPrep13,
+  kSlotPrep14, kSlotPrep15, kSlotPrep16, kSlotPrep17, kSlotPrep18,
   kSlotCount
 };
 
@@ -158,8 +161,20 @@ __host__ __device__ inline uint64_t splitmix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-inline unsigned blocks_for(uint64_t n, unsigned threads) {
+__host__ __device__ inline unsigned blocks_for(uint64_t n, unsigned threads) {
   return static_cast<unsigned>((n + threads - 1) / threads);
+}
+
+// MI355X deals the blocks of a launch to its 8 XCDs round-robin, and each XCD
+// has its own L2.  A kernel launched on xcd_grid(nb) blocks takes its logical
+// block from xcd_block(nb): each XCD then works on one contiguous range of
+// logical blocks, so data shared by neighbouring blocks (mesh rings) stays in
+// one L2.  Logical blocks >= nb are past the end.
+constexpr unsigned kXcdCount = 8;
+inline unsigned xcd_grid(unsigned nb) { return (nb + kXcdCount - 1) / kXcdCount * kXcdCount; }
+__device__ inline unsigned xcd_block(unsigned nb) {
+  const unsigned per = (nb + kXcdCount - 1) / kXcdCount;
+  return (blockIdx.x % kXcdCount) * per + blockIdx.x / kXcdCount;
 }
 
 // ------------------------------------------------------------ device math

@@ -1,19 +1,21 @@
 // Per-vertex curvature tensor and principal frames (FP64).
 //
 // Restates src/cpu/curvature.cpp:14-79 for gfx950:
-//   * edge pass: one thread per edge computes the dihedral contribution
-//     d d^T beta len / mean once (curvature.cpp:21-37);
-//   * gather pass: one thread per vertex sums its interior-edge
-//     contributions.  The reference adds them in edge order; edges are sorted
-//     by (lo, hi), so a vertex's incident edges in edge order are exactly its
-//     ring neighbours in ascending order (edges (u, v) with u < v first, then
-//     (v, w) with w > v).  The gather walks the sorted ring, which gives the
-//     same summation order without an incident-edge CSR;
+//   * mean edge length: a fixed-order tree sum that computes the lengths as
+//     it sums them (deterministic; it agrees with the reference's sequential
+//     sum to a few ulps);
+//   * gather pass: one thread per vertex computes and sums its interior-edge
+//     contributions d d^T beta len / mean (curvature.cpp:21-40).  The
+//     reference adds them in edge order; edges are sorted by (lo, hi), so a
+//     vertex's incident edges in edge order are exactly its ring neighbours
+//     in ascending order (edges (u, v) with u < v first, then (v, w) with
+//     w > v).  The gather walks the sorted ring, which gives the same
+//     summation order without an incident-edge CSR, and recomputes each
+//     edge's contribution at both endpoints (same operations, same bits)
+//     rather than writing 9 doubles per edge and reading them back twice;
 //   * smoothing: ping-pong (T_i + sum_nbr T_j) / (deg + 1) (curvature.cpp:41-51);
 //   * frames: tangent-plane 2x2 eigen-decomposition (curvature.cpp:54-77).
 // Tensors are 9-lane SoA so every lane access is coalesced.
-// The mean edge length is a fixed-order tree sum (deterministic); it agrees
-// with the reference's sequential sum to a few ulps.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -24,23 +26,46 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__global__ void k_edge_length(const double *x, const double *y, const double *z,
-                              const mvs_edge *edges, uint64_t ne, double *len) {
-  uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= ne)
-    return;
-  int32_t h = edges[e].hi, l = edges[e].lo;
-  len[e] = norm(v3(x[h], y[h], z[h]) - v3(x[l], y[l], z[l]));
+// Edge length of edge i (curvature.cpp:17-18).
+__device__ inline double edge_length(const double *x, const double *y, const double *z,
+                                     const mvs_edge &E) {
+  return norm(v3(x[E.hi], y[E.hi], z[E.hi]) - v3(x[E.lo], y[E.lo], z[E.lo]));
 }
 
-// Fixed-shape tree reduction: each block sums a contiguous tile in a fixed
-// order; a second launch sums the block partials the same way.
-__global__ void k_sum_tiles(const double *in, uint64_t n, uint64_t tile, double *out) {
+// Fixed-shape tree reduction: each block sums a tile of kSumItems values per
+// thread in a fixed order (thread t adds items t, t + 256, ... of the tile),
+// then a tree over the threads; a second launch sums the block partials the
+// same way.  The first level computes the edge lengths it sums (no length
+// array), issuing every load of a thread before the first sum.
+constexpr int kSumItems = 8;
+constexpr uint64_t kSumTile = kThreads * kSumItems;
+
+template <bool kLengths>
+__global__ void __launch_bounds__(kThreads)
+    k_sum_tiles(const double *in, const double *x, const double *y, const double *z,
+                const mvs_edge *edges, uint64_t n, double *out) {
   __shared__ double s[kThreads];
-  uint64_t base = static_cast<uint64_t>(blockIdx.x) * tile;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSumTile + threadIdx.x;
+  double val[kSumItems];
+  if constexpr (kLengths) {
+    int32_t hi[kSumItems], lo[kSumItems];
+    for (int r = 0; r < kSumItems; ++r) {
+      const uint64_t i = base + r * kThreads;
+      hi[r] = i < n ? edges[i].hi : 0;
+      lo[r] = i < n ? edges[i].lo : 0;
+    }
+    for (int r = 0; r < kSumItems; ++r)
+      val[r] = base + r * kThreads < n
+                   ? norm(v3(x[hi[r]], y[hi[r]], z[hi[r]]) - v3(x[lo[r]], y[lo[r]], z[lo[r]]))
+                   : 0.0;
+  } else {
+    for (int r = 0; r < kSumItems; ++r)
+      val[r] = base + r * kThreads < n ? in[base + r * kThreads] : 0.0;
+  }
   double acc = 0;
-  for (uint64_t i = base + threadIdx.x; i < base + tile && i < n; i += kThreads)
-    acc += in[i];
+  for (int r = 0; r < kSumItems; ++r)
+    if (base + r * kThreads < n)
+      acc += val[r];
   s[threadIdx.x] = acc;
   __syncthreads();
   for (int o = kThreads / 2; o > 0; o >>= 1) {
@@ -56,58 +81,7 @@ __global__ void k_mean_finish(const double *sum, uint64_t ne, double *mean) {
   *mean = sum[0] / static_cast<double>(ne);
 }
 
-// Contribution of each interior edge (9 lanes, stride ne).  Boundary edges
-// get a NaN marker in lane 0 so the gather can skip them without re-reading
-// the edge record.
-__global__ void k_edge_tensor(const double *x, const double *y, const double *z,
-                              const double *fn, uint64_t nf, const mvs_edge *edges, uint64_t ne,
-                              const double *mean, double *contrib) {
-  uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= ne)
-    return;
-  mvs_edge E = edges[e];
-  if (E.face1 < 0) {
-    contrib[e] = __builtin_nan("");
-    return;
-  }
-  V3 d = v3(x[E.b], y[E.b], z[E.b]) - v3(x[E.a], y[E.a], z[E.a]);
-  double len = norm(d);
-  d = unit(d);
-  V3 n1 = v3(fn[E.face0], fn[nf + E.face0], fn[2 * nf + E.face0]);
-  V3 n2 = v3(fn[E.face1], fn[nf + E.face1], fn[2 * nf + E.face1]);
-  double c = dot(n1, n2);
-  c = c < -1.0 ? -1.0 : (1.0 < c ? 1.0 : c);
-  double beta = atan2(dot(cross(n1, n2), d), c);
-  double m = *mean;
-  double dd[3] = {d.x, d.y, d.z};
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      contrib[(i * 3 + j) * ne + e] = dd[i] * dd[j] * beta * len / m;
-}
-
-// Edge id of (lo, hi): edges are sorted by (lo, hi), so it is the first edge
-// with this lo plus the rank of hi among lo's ring neighbours greater than lo.
-// Returns ne (no edge) when hi is not a greater neighbour of lo, which a
-// consistent topology never produces.
-__device__ inline uint64_t edge_id(const uint64_t *off, const int32_t *idx,
-                                   const uint64_t *edge_start, int32_t lo, int32_t hi,
-                                   uint64_t ne) {
-  uint64_t r0 = off[lo], r1 = off[lo + 1];
-  uint64_t first_greater = r0, pos = r1;
-  for (uint64_t j = r0; j < r1; ++j) {
-    int32_t u = idx[j];
-    if (u <= lo)
-      first_greater = j + 1;
-    if (u == hi)
-      pos = j;
-  }
-  if (pos == r1 || pos < first_greater)
-    return ne;
-  uint64_t e = edge_start[lo] + (pos - first_greater);
-  return e < ne ? e : ne;
-}
-
-// edge_start[v] = index of the first edge with lo == v (count of edges with lo < v).
+// estart[v] = index of the first edge with lo == v (count of edges with lo < v).
 __global__ void k_greater_count(const uint64_t *off, const int32_t *idx, uint64_t n,
                                 uint32_t *cnt) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -119,56 +93,160 @@ __global__ void k_greater_count(const uint64_t *off, const int32_t *idx, uint64_
   cnt[v] = c;
 }
 
-__global__ void k_gather(const uint64_t *off, const int32_t *idx, const uint64_t *edge_start,
-                         const double *contrib, uint64_t ne, uint64_t n, double *T) {
-  uint64_t v = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+// Per vertex, the dihedral contributions d d^T beta len / mean of its
+// interior edges (curvature.cpp:21-37), summed in edge order, then divided
+// by their count (curvature.cpp:38-40).  Edges are sorted by (lo, hi), so
+// the vertex's edges in edge order are its ring neighbours in ascending
+// order: for a neighbour u > v the edge is the next one of v's own block
+// (estart[v], ...); for u < v it is the edge of u's block whose hi is v.
+// Each edge's contribution is computed by both of its endpoints, with the
+// same operations, instead of being stored and gathered.
+//
+// The ring is taken kGatherChunk slots at a time, phase by phase (ring entries,
+// edge ids, edge records, then positions and face normals), so a thread has
+// a chunk's loads in flight together instead of one dependent chain per
+// slot; the sums still run in slot order.
+constexpr int kChunk = 8;        // smoothing
+constexpr int kGatherChunk = 4;  // gather (more registers per slot)
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_gather(const uint64_t *__restrict__ off, const int32_t *__restrict__ idx,
+             const uint64_t *__restrict__ estart, const mvs_edge *__restrict__ edges, uint64_t ne,
+             const double *__restrict__ x, const double *__restrict__ y,
+             const double *__restrict__ z, const double *__restrict__ fn, uint64_t nf,
+             const double *__restrict__ mean, uint64_t n, double *__restrict__ T) {
+  uint64_t v = static_cast<uint64_t>(xcd_block(blocks_for(n, kThreads))) * kThreads + threadIdx.x;
   if (v >= n)
     return;
+  const double m = *mean;
   double acc[9];
   for (int k = 0; k < 9; ++k)
     acc[k] = 0;
   int count = 0;
-  int32_t self = static_cast<int32_t>(v);
-  for (uint64_t j = off[v]; j < off[v + 1]; ++j) {
-    int32_t u = idx[j];
-    uint64_t e = u < self ? edge_id(off, idx, edge_start, u, self, ne)
-                          : edge_id(off, idx, edge_start, self, u, ne);
-    if (e >= ne)
-      continue;
-    double c0 = contrib[e];
-    if (c0 != c0)  // boundary edge
-      continue;
-    ++count;
-    acc[0] += c0;
-    for (int k = 1; k < 9; ++k)
-      acc[k] += contrib[k * ne + e];
+  const int32_t self = static_cast<int32_t>(v);
+  uint64_t up = estart[v];
+  const uint64_t up_end = estart[v + 1] < ne ? estart[v + 1] : ne;
+  const uint64_t j1 = off[v + 1];
+  for (uint64_t j0 = off[v]; j0 < j1; j0 += kGatherChunk) {
+    const int cnt = j1 - j0 < kGatherChunk ? static_cast<int>(j1 - j0) : kGatherChunk;
+    int32_t u[kGatherChunk];
+#pragma unroll
+    for (int r = 0; r < kGatherChunk; ++r)
+      u[r] = r < cnt ? idx[j0 + r] : self;
+    // edge ids: upper neighbours in order from v's block; lower ones from
+    // their own block (its first entries read together)
+    uint64_t e[kGatherChunk], b0[kGatherChunk], b1[kGatherChunk];
+#pragma unroll
+    for (int r = 0; r < kGatherChunk; ++r) {
+      e[r] = ne;
+      b0[r] = b1[r] = 0;
+      if (r < cnt) {
+        if (u[r] > self) {
+          e[r] = up < up_end ? up : ne;
+          ++up;
+        } else {
+          b0[r] = estart[u[r]];
+          b1[r] = estart[u[r] + 1] < ne ? estart[u[r] + 1] : ne;
+        }
+      }
+    }
+    int32_t ehi[kGatherChunk];
+#pragma unroll
+    for (int r = 0; r < kGatherChunk; ++r)
+      ehi[r] = e[r] < ne ? edges[e[r]].hi : (b0[r] < b1[r] ? edges[b0[r]].hi : -1);
+#pragma unroll
+    for (int r = 0; r < kGatherChunk; ++r) {
+      if (r >= cnt)
+        continue;
+      if (u[r] > self) {
+        if (e[r] < ne && ehi[r] != u[r])
+          e[r] = ne;
+      } else if (b0[r] < b1[r] && ehi[r] == self) {
+        e[r] = b0[r];
+      } else {
+        for (uint64_t k = b0[r] + 1; k < b1[r]; ++k)
+          if (edges[k].hi == self) {
+            e[r] = k;
+            break;
+          }
+      }
+    }
+    mvs_edge E[kGatherChunk];
+#pragma unroll
+    for (int r = 0; r < kGatherChunk; ++r)
+      E[r] = e[r] < ne ? edges[e[r]] : mvs_edge{0, 0, 0, 0, 0, -1};
+#pragma unroll
+    for (int r = 0; r < kGatherChunk; ++r) {
+      if (r >= cnt || E[r].face1 < 0)  // no edge, or a boundary edge
+        continue;
+      ++count;
+      V3 d = v3(x[E[r].b], y[E[r].b], z[E[r].b]) - v3(x[E[r].a], y[E[r].a], z[E[r].a]);
+      const double len = norm(d);
+      d = unit(d);
+      const int32_t f0 = E[r].face0, f1 = E[r].face1;
+      const V3 n1 = v3(fn[f0], fn[nf + f0], fn[2 * nf + f0]);
+      const V3 n2 = v3(fn[f1], fn[nf + f1], fn[2 * nf + f1]);
+      double c = dot(n1, n2);
+      c = c < -1.0 ? -1.0 : (1.0 < c ? 1.0 : c);
+      const double beta = atan2(dot(cross(n1, n2), d), c);
+      // d_i d_k and d_k d_i are the same product, so the tensor is
+      // symmetric to the bit: six values serve the nine lanes
+      const double dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = i; k < 3; ++k) {
+          const double t = dd[i] * dd[k] * beta * len / m;
+          acc[i * 3 + k] += t;
+          if (k != i)
+            acc[k * 3 + i] += t;
+        }
+    }
   }
-  int c = count > 1 ? count : 1;
+  const int c = count > 1 ? count : 1;
   for (int k = 0; k < 9; ++k)
     T[k * n + v] = acc[k] / c;
 }
 
-__global__ void k_smooth(const uint64_t *off, const int32_t *idx, uint64_t n, const double *T,
-                         double *out) {
-  uint64_t v = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+// One smoothing pass: (T_v + sum of the ring's T_u in ring order) / (deg + 1)
+// (curvature.cpp:41-51).  Lane by lane, a chunk of the ring's values is
+// loaded together and then added in order.
+__global__ void __launch_bounds__(kThreads)
+    k_smooth(const uint64_t *__restrict__ off, const int32_t *__restrict__ idx, uint64_t n,
+             const double *__restrict__ T, double *__restrict__ out) {
+  uint64_t v = static_cast<uint64_t>(xcd_block(blocks_for(n, kThreads))) * kThreads + threadIdx.x;
   if (v >= n)
     return;
   double acc[9];
   for (int k = 0; k < 9; ++k)
     acc[k] = T[k * n + v];
-  for (uint64_t j = off[v]; j < off[v + 1]; ++j) {
-    uint64_t u = static_cast<uint64_t>(idx[j]);
-    for (int k = 0; k < 9; ++k)
-      acc[k] += T[k * n + u];
+  const uint64_t j1 = off[v + 1], j0s = off[v];
+  for (uint64_t j0 = j0s; j0 < j1; j0 += kChunk) {
+    const int cnt = j1 - j0 < kChunk ? static_cast<int>(j1 - j0) : kChunk;
+    uint64_t u[kChunk];
+#pragma unroll
+    for (int r = 0; r < kChunk; ++r)
+      u[r] = r < cnt ? static_cast<uint64_t>(idx[j0 + r]) : v;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      double t[kChunk];
+#pragma unroll
+      for (int r = 0; r < kChunk; ++r)
+        t[r] = T[k * n + u[r]];
+#pragma unroll
+      for (int r = 0; r < kChunk; ++r)
+        if (r < cnt)
+          acc[k] += t[r];
+    }
   }
-  double deg1 = static_cast<double>(off[v + 1] - off[v] + 1);
+  double deg1 = static_cast<double>(j1 - j0s + 1);
   for (int k = 0; k < 9; ++k)
     out[k * n + v] = acc[k] / deg1;
 }
 
 __global__ void k_frames(const double *T, const double *normals, const uint64_t *off, uint64_t n,
                          double *feat, uint8_t *valid) {
-  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint64_t i = static_cast<uint64_t>(xcd_block(blocks_for(n, kThreads))) * kThreads + threadIdx.x;
   if (i >= n)
     return;
   double vals[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -207,41 +285,49 @@ void curvature_device(Session *s, uint64_t n, const double *x, const double *y, 
     return;
   if (smoothing < 0)
     raise(MVS_EINVAL, "smoothing must be non-negative");
-  double *len = scratch_as<double>(s, kSlotMisc0, ne ? ne : 1);
-  double *contrib = scratch_as<double>(s, kSlotMisc1, 9 * (ne ? ne : 1));
   double *Ta = scratch_as<double>(s, kSlotMisc2, 9 * n);
   double *Tb = scratch_as<double>(s, kSlotMisc3, 9 * n);
   uint32_t *gcnt = scratch_as<uint32_t>(s, kSlotCounts, n);
   uint64_t *estart = scratch_as<uint64_t>(s, kSlotOffsets, n + 1);
-  const uint64_t tile = 8192;
-  unsigned tiles = blocks_for(ne ? ne : 1, tile);
-  double *partial = scratch_as<double>(s, kSlotBlock, tiles + 1);
-  double *mean = partial + tiles;
+  // partial sums: level one over the edges, then levels over the partials
+  // until one value is left
+  const uint64_t tiles = blocks_for(ne ? ne : 1, kSumTile);
+  uint64_t total = 0;
+  for (uint64_t c = tiles;; c = blocks_for(c, kSumTile)) {
+    total += c;
+    if (c == 1)
+      break;
+  }
+  double *partial = scratch_as<double>(s, kSlotBlock, total + 1);
+  double *mean = partial + total;
   if (ne) {
-    MVS_LAUNCH(s, "curv_edge_length", k_edge_length, blocks_for(ne, kThreads), kThreads, 0, x, y,
-               z, edges, ne, len);
-    MVS_LAUNCH(s, "curv_sum", k_sum_tiles, tiles, kThreads, 0, len, ne, tile, partial);
-    if (tiles > 1)
-      MVS_LAUNCH(s, "curv_sum", k_sum_tiles, 1, kThreads, 0, partial, static_cast<uint64_t>(tiles),
-                 static_cast<uint64_t>(tiles), partial);
-    MVS_LAUNCH(s, "curv_mean", k_mean_finish, 1, 1, 0, partial, ne, mean);
-    MVS_LAUNCH(s, "curv_edge_tensor", k_edge_tensor, blocks_for(ne, kThreads), kThreads, 0, x, y,
-               z, fn, nf, edges, ne, mean, contrib);
+    MVS_LAUNCH(s, "curv_sum", k_sum_tiles<true>, static_cast<unsigned>(tiles), kThreads, 0,
+               nullptr, x, y, z, edges, ne, partial);
+    double *cur = partial;
+    for (uint64_t c = tiles; c > 1;) {
+      const uint64_t nb = blocks_for(c, kSumTile);
+      MVS_LAUNCH(s, "curv_sum", k_sum_tiles<false>, static_cast<unsigned>(nb), kThreads, 0, cur,
+                 nullptr, nullptr, nullptr, nullptr, c, cur + c);
+      cur += c;
+      c = nb;
+    }
+    MVS_LAUNCH(s, "curv_mean", k_mean_finish, 1, 1, 0, cur, ne, mean);
   }
   MVS_LAUNCH(s, "curv_greater", k_greater_count, blocks_for(n, kThreads), kThreads, 0, off, idx, n,
              gcnt);
   exclusive_scan_u32(s, gcnt, estart, n);
-  MVS_LAUNCH(s, "curv_gather", k_gather, blocks_for(n, kThreads), kThreads, 0, off, idx, estart,
-             contrib, ne, n, Ta);
+  const unsigned vgrid = xcd_grid(blocks_for(n, kThreads));
+  MVS_LAUNCH(s, "curv_gather", k_gather, vgrid, kThreads, 0, off, idx, estart,
+             edges, ne, x, y, z, fn, nf, mean, n, Ta);
   double *cur = Ta, *nxt = Tb;
   for (int it = 0; it < smoothing; ++it) {
-    MVS_LAUNCH(s, "curv_smooth", k_smooth, blocks_for(n, kThreads), kThreads, 0, off, idx, n, cur,
+    MVS_LAUNCH(s, "curv_smooth", k_smooth, vgrid, kThreads, 0, off, idx, n, cur,
                nxt);
     double *t = cur;
     cur = nxt;
     nxt = t;
   }
-  MVS_LAUNCH(s, "curv_frames", k_frames, blocks_for(n, kThreads), kThreads, 0, cur, normals, off,
+  MVS_LAUNCH(s, "curv_frames", k_frames, vgrid, kThreads, 0, cur, normals, off,
              n, feat, valid);
 }
 

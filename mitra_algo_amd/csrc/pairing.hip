@@ -995,6 +995,37 @@ int mvs_prune(mvs_session *sp, const mvs_features *f, uint64_t ns, const int32_t
   });
 }
 
+int mvs_dev_prune(mvs_session *sp, uint64_t n, const double *feat, const uint8_t *valid,
+                  uint64_t ns, const int32_t *sample, double prune, int32_t *out,
+                  uint64_t *count) {
+  return guarded([&] {
+    Session *s = as_session(sp);
+    if (!count || (ns && (!feat || !valid || !sample || !out)))
+      raise(MVS_EINVAL, "null argument");
+    *count = 0;
+    if (ns == 0)
+      return;
+    uint32_t *flag = scratch_as<uint32_t>(s, kSlotFlags, ns);
+    uint64_t *off = scratch_as<uint64_t>(s, kSlotOffsets, ns + 2);
+    unsigned *bad = reinterpret_cast<unsigned *>(off + ns + 1);  // past the scan's total
+    zero_device(s, bad, sizeof(unsigned));
+    MVS_LAUNCH(s, "prune_flags", k_prune_flags, blocks_for(ns, 256), 256, 0, feat + 9 * n,
+               feat + 10 * n, valid, n, sample, ns, 1 - prune, flag, bad);
+    exclusive_scan_u32(s, flag, off, ns);
+    MVS_LAUNCH(s, "prune_compact", k_compact, blocks_for(ns, 256), 256, 0, sample, flag, off, ns,
+               out);
+    struct {
+      unsigned bad;
+      uint64_t kept;
+    } h{};
+    copy_d2h(s, &h.bad, bad, sizeof h.bad);
+    copy_d2h(s, &h.kept, off + ns, sizeof h.kept);
+    if (h.bad)
+      raise(MVS_EINVAL, "sample vertex index out of range");
+    *count = h.kept;
+  });
+}
+
 int mvs_pairing(mvs_session *sp, const mvs_points *v, const mvs_features *f, uint64_t S,
                 const int32_t *selected, double box, double t, uint64_t max_pairs, int precision,
                 uint64_t *count) {

@@ -10,6 +10,7 @@ meshgen.magnitude_keep, then the reference's anisotropy prune (.85).
 """
 from __future__ import annotations
 
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -136,6 +137,51 @@ class Workload:
 
     def replay(self):
         self.sess.graph_launch()
+
+
+def end_to_end(sess, soa, samples: np.ndarray, threshold: float, precision: int,
+               seeds: int = 20000, iterations: int = 30, h: float = 0.05, cutoff: float = 3.0,
+               seed_value: int = 7, smoothing: int = 3, stage_sync: bool = False):
+    """One pass from the host mesh to modes on the device: prepare on the
+    device (soa: the reference's host SoA mesh, x, y, z, f0, f1, f2),
+    curvature, the prune (.85) of the host sample ids, pairs + votes, batched
+    mean-shift.  Returns (prepared, S, pair-vote result, mean-shift result,
+    stage seconds); with stage_sync each stage is timed to completion, else
+    only the total is."""
+    dev = torch.device("cuda", torch.cuda.current_device())
+    times = {}
+    t = [time.perf_counter()]
+
+    def mark(name):
+        if stage_sync:
+            torch.cuda.synchronize(dev)
+            now = time.perf_counter()
+            times[name] = now - t[0]
+            t[0] = now
+
+    t0 = t[0]
+    d = sess.dev_prepare_soa(*soa, device=dev)
+    mark("prepare")
+    n, nf = d["nv"], d["nf"]
+    feat = torch.empty((11, n), dtype=torch.float64, device=dev)
+    valid = torch.empty(n, dtype=torch.uint8, device=dev)
+    xyz = d["xyz"]
+    sess.dev_curvature(n, xyz[0], xyz[1], xyz[2], d["normals"], nf, d["face_normals"],
+                       d["nbr_off"], d["nbr_idx"], d["edge_count"], d["edges"], smoothing, feat,
+                       valid)
+    mark("curvature")
+    sample = torch.from_numpy(np.ascontiguousarray(samples, dtype=np.int32)).to(dev)
+    sel = torch.empty(max(len(samples), 1), dtype=torch.int32, device=dev)
+    S = sess.dev_prune(n, feat, valid, sample, 0.85, sel)
+    mark("prune")
+    pv = sess.dev_pairvote(n, xyz[0], xyz[1], xyz[2], feat, S, sel, d["box_size"], d["center"],
+                           threshold, 2**31 - 1, 2**31 - 1, precision)
+    mark("pairvote")
+    ms = sess.dev_meanshift_votes(seeds, seed_value, iterations, h, cutoff, False)
+    torch.cuda.synchronize(dev)
+    mark("meanshift")
+    times["total"] = time.perf_counter() - t0
+    return d, S, pv, ms, times
 
 
 def cfg1(device: str = "cuda:0") -> Workload:
