@@ -134,6 +134,10 @@ enum Slot : int {
   kSlotPrep0, kSlotPrep1, kSlotPrep2, kSlotPrep3, kSlotPrep4, kSlotPrep5, kSlotPrep6,
   kSlotPrep7, kSlotPrep8, kSlotPrep9, kSlotPrep10, kSlotPrep11, kSlotPrep12, kSlotPrep13,
   kSlotPrep14, kSlotPrep15, kSlotPrep16, kSlotPrep17, kSlotPrep18,
+  // scratch bank 1: the sort, the scan and the mean-shift set-up of a run
+  // queued on the side stream beside bank-0 work
+  kSlotSortB0, kSlotSortB1, kSlotSortB2, kSlotSortBHist, kSlotScanB,
+  kSlotMsBankPart, kSlotMsBankKeys, kSlotMsBankVals, kSlotMsBankSeedKeys, kSlotMsBankAos,
   kSlotCount
 };
 

@@ -89,28 +89,29 @@ static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, in
                    r.refl_stride, r.refl_d, rpts, fpts);
   ms_begin(s);
   const int slot_r = want_stats ? 0 : -1, slot_f = want_stats ? 1 : -1;
-  // the mirror kind first, its query on the side stream, beside the rigid
-  // set-up and query (the two kinds share nothing the queries read); the
-  // mirror query is the longer one, so it starts as early as it can
-  const MsRun first{0, kr > 0}, second{1, false};
+  // the rigid kind whole on the side stream (its own scratch bank), queued
+  // first, beside the mirror kind on the session stream: the two set-ups run
+  // together, then the two queries.  A set-up queued behind a running query
+  // waits for its blocks to drain and is slowed several times over.
+  const MsRun rigid_run = kf ? MsRun{1, true} : MsRun{}, mirror_run{};
   if (prec == MVS_FP32_GUARDED) {
-    if (kf)
-      ms_batched_f32(s, 4, nf, static_cast<const float *>(fpts), kf, nullptr, dseeds + kr,
-                     salt_f, iterations, h, cutoff, static_cast<float *>(ofl),
-                     static_cast<float *>(owf), slot_f, nullptr, first);
     if (kr)
       ms_batched_f32(s, 6, nr, static_cast<const float *>(rpts), kr, nullptr, dseeds, salt_r,
                      iterations, h, cutoff, static_cast<float *>(orig), static_cast<float *>(owr),
-                     slot_r, nullptr, kf ? second : MsRun{});
-  } else {
+                     slot_r, nullptr, rigid_run);
     if (kf)
-      ms_batched_f64(s, 4, nf, static_cast<const double *>(fpts), kf, nullptr, dseeds + kr,
-                     salt_f, iterations, h, cutoff, static_cast<double *>(ofl),
-                     static_cast<double *>(owf), slot_f, nullptr, first);
+      ms_batched_f32(s, 4, nf, static_cast<const float *>(fpts), kf, nullptr, dseeds + kr,
+                     salt_f, iterations, h, cutoff, static_cast<float *>(ofl),
+                     static_cast<float *>(owf), slot_f, nullptr, mirror_run);
+  } else {
     if (kr)
       ms_batched_f64(s, 6, nr, static_cast<const double *>(rpts), kr, nullptr, dseeds, salt_r,
                      iterations, h, cutoff, static_cast<double *>(orig),
-                     static_cast<double *>(owr), slot_r, nullptr, kf ? second : MsRun{});
+                     static_cast<double *>(owr), slot_r, nullptr, rigid_run);
+    if (kf)
+      ms_batched_f64(s, 4, nf, static_cast<const double *>(fpts), kf, nullptr, dseeds + kr,
+                     salt_f, iterations, h, cutoff, static_cast<double *>(ofl),
+                     static_cast<double *>(owf), slot_f, nullptr, mirror_run);
   }
   unsigned long long sums[kMsSums] = {0, 0, 0, 0, 0, 0};
   ms_finish(s, sums);

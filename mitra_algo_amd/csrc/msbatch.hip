@@ -564,7 +564,6 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ uint32_t s_count[3];  // W = 4: non-empty rows, batches, candidates
   __shared__ R red[W == 1 ? 1 : (D + 1) * kThreads];
   __shared__ R s_mean[W == 1 ? 1 : D + 1];
-  __shared__ int s_moved;
   const int lane = threadIdx.x & (kWave - 1);
   // wave index, made visibly wave-uniform so the row cursor stays scalar
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
@@ -818,18 +817,27 @@ __global__ void __launch_bounds__(kThreads)
       }
     } else {
       // Row-direct walk: batch k of the numbering above is 64 consecutive
-      // records of one row, taken by wave k mod W.  Row state is scalar and
-      // advances with k; lanes past the row's end read the sentinel record,
-      // which lies outside every support.  The next batch's records are in
-      // flight while the current batch is tested.
+      // records of one row.  Wave w takes the contiguous batches [k0, k1) of
+      // a W-way split, so its row cursor advances over about a W-th of the
+      // rows; it starts at the row holding k0 (the rows whose first batch is
+      // <= k0, counted by one ballot).  Lanes past the row's end read the
+      // sentinel record, which lies outside every support.  The next batch's
+      // records are in flight while the current batch is tested.
+      const uint32_t k0 = static_cast<uint32_t>(
+                         (static_cast<uint64_t>(totb) * static_cast<uint32_t>(wsub)) / W),
+                     k1 = static_cast<uint32_t>(
+                         (static_cast<uint64_t>(totb) * static_cast<uint32_t>(wsub + 1)) / W);
       uint32_t r = 0, bs_cur = 0, bs_next = 0, rb_r = 0, len_r = 0;
-      if (nrc) {
-        bs_next = __builtin_amdgcn_readlane(bsv, 1);
-        rb_r = __builtin_amdgcn_readlane(rbv, 0);
-        len_r = __builtin_amdgcn_readlane(lenv, 0);
+      if (k0 < k1) {
+        const uint64_t below = ballot(lane < static_cast<int>(nrc) && bsv <= k0);
+        r = static_cast<uint32_t>(__popcll(below)) - 1;
+        bs_cur = __builtin_amdgcn_readlane(bsv, r);
+        bs_next = __builtin_amdgcn_readlane(bsv, r + 1);
+        rb_r = __builtin_amdgcn_readlane(rbv, r);
+        len_r = __builtin_amdgcn_readlane(lenv, r);
       }
       auto rec_of = [&](uint32_t k) -> uint32_t {
-        if (k >= totb)
+        if (k >= k1)
           return sentinel;
         while (k >= bs_next) {
           ++r;
@@ -846,19 +854,19 @@ __global__ void __launch_bounds__(kThreads)
         if (D == 6)
           q = q2[j];
       };
-      uint32_t k = static_cast<uint32_t>(wsub);
-      if (k < totb) {
+      uint32_t k = k0;
+      if (k < k1) {
         typename Vec<R>::V4 pa, pb;
         typename Vec<R>::V2 qa{}, qb{};
         load(rec_of(k), pa, qa);
-        uint32_t jb = rec_of(k + W);
-        for (; k < totb; k += 2 * W) {
+        uint32_t jb = rec_of(k + 1);
+        for (; k < k1; k += 2) {
           load(jb, pb, qb);
-          const uint32_t ja = rec_of(k + 2 * W);
+          const uint32_t ja = rec_of(k + 2);
           __builtin_amdgcn_sched_barrier(0);
           test(pa, qa);
           load(ja, pa, qa);
-          jb = rec_of(k + 3 * W);
+          jb = rec_of(k + 3);
           __builtin_amdgcn_sched_barrier(0);
           test(pb, qb);
         }
@@ -881,39 +889,35 @@ __global__ void __launch_bounds__(kThreads)
         }
       }
     } else {
-      // lane-wise partials through LDS, summed over the waves in a fixed
-      // order by wave 0
+      // lane-wise partials through LDS; component c (the D sums, then the
+      // weight) is summed over the waves in a fixed order by wave c mod W,
+      // then every wave takes the D + 1 totals and makes the same update
       for (int k = 0; k < D; ++k)
         red[k * kThreads + threadIdx.x] = acc[k];
       red[D * kThreads + threadIdx.x] = wl;
       __syncthreads();
-      if (wave == 0) {
-        R v[D + 1];
-        for (int k = 0; k <= D; ++k) {
-          const R *row = red + k * kThreads + lane;
-          v[k] = wave_total((row[0] + row[kWave]) + (row[2 * kWave] + row[3 * kWave]));
-        }
-        wl = v[D];
-        if (wl > R(0)) {
-          const R inv = R(1) / wl;
-          for (int k = 0; k < D; ++k) {
-            const R nk = m[k] + v[k] * inv;
-            moved = moved || nk != m[k];
-            m[k] = nk;
-          }
-        }
-        if (lane == 0) {
-          for (int k = 0; k < D; ++k)
-            s_mean[k] = m[k];
-          s_mean[D] = wl;
-          s_moved = moved ? 1 : 0;
-        }
+      for (int c = wave; c <= D; c += W) {
+        const R *row = red + c * kThreads + lane;
+        const R v = wave_total((row[0] + row[kWave]) + (row[2 * kWave] + row[3 * kWave]));
+        if (lane == 0)
+          s_mean[c] = v;
       }
       __syncthreads();
-      for (int k = 0; k < D; ++k)
-        m[k] = s_mean[k];
-      wsum = s_mean[D];
-      moved = s_moved != 0;
+      R v[D + 1];
+      for (int k = 0; k <= D; ++k)
+        v[k] = s_mean[k];
+      wl = v[D];
+      wsum = wl;
+      if (wl > R(0)) {
+        const R inv = R(1) / wl;
+        for (int k = 0; k < D; ++k) {
+          const R nk = m[k] + v[k] * inv;
+          moved = moved || nk != m[k];
+          m[k] = nk;
+        }
+      }
+      // no barrier here: the totals are rewritten only after the next
+      // iteration's two barriers, which every wave reaches after reading them
     }
     if (!moved)
       break;  // uniform across the seed's waves: they hold the same mean
@@ -1244,6 +1248,32 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     raise(MVS_EINVAL, "segmented mean-shift draws its seeds");
   if (ns == 0)
     return;
+  // a side run forks from the session stream here: its set-up (in the bank-1
+  // scratch, sort and scan slots) and its query run beside what the session
+  // stream does next; ms_finish joins it
+  const hipStream_t main = s->stream;
+  const int main_bank = s->bank;
+  if (run.side) {
+    if (!s->side_stream) {
+      MVS_HIP(hipStreamCreateWithFlags(&s->side_stream, hipStreamNonBlocking));
+      MVS_HIP(hipEventCreateWithFlags(&s->fork_event, hipEventDisableTiming));
+      MVS_HIP(hipEventCreateWithFlags(&s->join_event, hipEventDisableTiming));
+    }
+    MVS_HIP(hipEventRecord(s->fork_event, main));
+    MVS_HIP(hipStreamWaitEvent(s->side_stream, s->fork_event, 0));
+    s->stream = s->side_stream;
+  }
+  s->bank = run.set;
+  struct Restore {
+    Session *s;
+    hipStream_t main;
+    int bank;
+    ~Restore() {
+      s->stream = main;
+      s->bank = bank;
+    }
+  } restore{s, main, main_bank};
+  const bool bank1 = run.set != 0;
   const uint32_t meshes = seg ? seg->meshes : 1;
   const uint64_t *seg_off = seg ? seg->seg_off : nullptr;
   MsCtl *ctl = ms_ctl(s);
@@ -1273,7 +1303,8 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   unsigned nb = 0;
   double *partial = nullptr;
   if (!seg) {
-    partial = scratch_as<double>(s, kSlotMs1, kPart * kBboxBlocks);
+    // bank 0 shares its slot with the control block, which follows the partials
+    partial = scratch_as<double>(s, bank1 ? kSlotMsBankPart : kSlotMs1, kPart * kBboxBlocks);
     if (n) {
       nb = blocks_for(n, kThreads);
       if (nb > kBboxBlocks)
@@ -1317,9 +1348,9 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   }
   uint32_t *sorted_ids = nullptr;
   if (n) {
-    uint64_t *keys = scratch_as<uint64_t>(s, kSlotMs2, 2 * n);
-    uint32_t *vals = scratch_as<uint32_t>(s, kSlotMs3, 2 * n);
-    R *aos = scratch_as<R>(s, kSlotMs8, kAos * n);
+    uint64_t *keys = scratch_as<uint64_t>(s, bank1 ? kSlotMsBankKeys : kSlotMs2, 2 * n);
+    uint32_t *vals = scratch_as<uint32_t>(s, bank1 ? kSlotMsBankVals : kSlotMs3, 2 * n);
+    R *aos = scratch_as<R>(s, bank1 ? kSlotMsBankAos : kSlotMs8, kAos * n);
     if (D == 6)
       MVS_LAUNCH(s, "ms_cell_keys", (k_cell_keys<R, 6>), blocks_for(n, kThreads), kThreads, 0,
                  pts, n, grids, seg_off, meshes, cellbase, cells_cap, keys, vals, aos);
@@ -1349,7 +1380,7 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
                seg ? seg->seed_mesh : nullptr);
     seed_pos = pos;
   } else if (n && ns > 1) {
-    uint64_t *skeys = scratch_as<uint64_t>(s, kSlotMs4, 2 * ns);
+    uint64_t *skeys = scratch_as<uint64_t>(s, bank1 ? kSlotMsBankSeedKeys : kSlotMs4, 2 * ns);
     uint32_t *svals = scratch_as<uint32_t>(s, slot_seed, 2 * ns);
     MVS_LAUNCH(s, "ms_seed_keys", k_seed_keys<R>, blocks_for(ns, kThreads), kThreads, 0, seeds,
                ns, grids, skeys, svals);
@@ -1374,24 +1405,6 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   MVS_LAUNCH(s, "ms_query", (k_ms_query<R, DD, WW>), grid, kThreads, 0, p4, q2, off, grids,   \
              cellbase, seed_mesh, ns_dev, seed_pos, seeds, order, ns, logical,                \
              static_cast<uint32_t>(n), iterations, gscale, r2, fw, out, weight, stats)
-  // a side-stream query waits for this set-up and runs beside what the
-  // session stream does next; ms_finish joins it
-  const hipStream_t main = s->stream;
-  if (run.side) {
-    if (!s->side_stream) {
-      MVS_HIP(hipStreamCreateWithFlags(&s->side_stream, hipStreamNonBlocking));
-      MVS_HIP(hipEventCreateWithFlags(&s->fork_event, hipEventDisableTiming));
-      MVS_HIP(hipEventCreateWithFlags(&s->join_event, hipEventDisableTiming));
-    }
-    MVS_HIP(hipEventRecord(s->fork_event, main));
-    MVS_HIP(hipStreamWaitEvent(s->side_stream, s->fork_event, 0));
-    s->stream = s->side_stream;
-  }
-  struct Restore {
-    Session *s;
-    hipStream_t main;
-    ~Restore() { s->stream = main; }
-  } restore{s, main};
 #define MVS_MS_SMALL(DD, GG)                                                                   \
   MVS_LAUNCH(s, "ms_query", (k_ms_query_small<R, DD, GG>), grid, kThreads, 0, p4, q2, off,     \
              grids, cellbase, seed_mesh, ns_dev, seed_pos, seeds, order, ns, logical,         \

@@ -81,13 +81,14 @@ void ms_finish(Session *s, unsigned long long sums[kMsSums]);
 // Seeds: explicit positions (D x ns SoA), or, with seed_ids, ns of the n
 // points drawn stratified over the cell-sorted order with `salt` (indices
 // written to seed_ids, in cell order).
-// Two runs of one call may overlap: the first with side = true queues its
-// query on the session's side stream (forked after its set-up), the second
-// uses the other slot set (set = 1) for what its query reads, and
-// ms_finish joins the side stream before reading anything back.
+// Two runs of one call may overlap: one with set = 1 and side = true runs
+// whole (set-up and query) on the session's side stream, forked when it is
+// queued, in its own slot set and scratch bank; the other runs on the
+// session stream with set 0; ms_finish joins the side stream before reading
+// anything back.
 struct MsRun {
-  int set = 0;        // slot set of the query inputs (0 or 1)
-  bool side = false;  // query on the side stream
+  int set = 0;        // slot set and scratch bank (0 or 1)
+  bool side = false;  // the run on the side stream
 };
 void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,
                     const float *seeds, int32_t *seed_ids, uint64_t salt, int iterations,

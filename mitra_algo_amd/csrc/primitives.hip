@@ -280,10 +280,10 @@ void zero_device(Session *s, void *ptr, size_t bytes) {
 }
 
 void exclusive_scan_u64(Session *s, const uint64_t *in, uint64_t *out, uint64_t n) {
-  exclusive_scan_impl<uint64_t>(s, in, out, n, kSlotScan);
+  exclusive_scan_impl<uint64_t>(s, in, out, n, s->bank ? kSlotScanB : kSlotScan);
 }
 void exclusive_scan_u32(Session *s, const uint32_t *in, uint64_t *out, uint64_t n) {
-  exclusive_scan_impl<uint32_t>(s, in, out, n, kSlotScan);
+  exclusive_scan_impl<uint32_t>(s, in, out, n, s->bank ? kSlotScanB : kSlotScan);
 }
 
 void radix_sort_pairs_u64(Session *s, const uint64_t *keys_in, const uint32_t *vals_in,
@@ -291,10 +291,13 @@ void radix_sort_pairs_u64(Session *s, const uint64_t *keys_in, const uint32_t *v
   if (n == 0)
     return;
   unsigned tiles = blocks_for(n, kSortTile);
-  uint64_t *k2 = scratch_as<uint64_t>(s, kSlotSort0, n);
-  uint32_t *v2 = scratch_as<uint32_t>(s, kSlotSort1, n);
-  uint32_t *hist = scratch_as<uint32_t>(s, kSlotSortHist, static_cast<size_t>(tiles) * kRadix);
-  uint64_t *off = scratch_as<uint64_t>(s, kSlotSort2, static_cast<size_t>(tiles) * kRadix + 1);
+  const bool b = s->bank != 0;
+  uint64_t *k2 = scratch_as<uint64_t>(s, b ? kSlotSortB0 : kSlotSort0, n);
+  uint32_t *v2 = scratch_as<uint32_t>(s, b ? kSlotSortB1 : kSlotSort1, n);
+  uint32_t *hist = scratch_as<uint32_t>(s, b ? kSlotSortBHist : kSlotSortHist,
+                                        static_cast<size_t>(tiles) * kRadix);
+  uint64_t *off = scratch_as<uint64_t>(s, b ? kSlotSortB2 : kSlotSort2,
+                                       static_cast<size_t>(tiles) * kRadix + 1);
   int passes = (bits + 7) / 8;
   // ping-pong so that the last pass writes keys_out/vals_out
   const uint64_t *ksrc = keys_in;

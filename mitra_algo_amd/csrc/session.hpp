@@ -76,6 +76,9 @@ struct Session {
   hipStream_t side_stream = nullptr;
   hipEvent_t fork_event = nullptr, join_event = nullptr;
   bool side_pending = false;
+  // scratch bank of the sort and scan primitives (1: a side-stream run's own
+  // slots, so it can overlap bank-0 work)
+  int bank = 0;
   DBuf<unsigned char> slots[kSlotCount];
   void *stage = nullptr;
   size_t stage_bytes = 0;
