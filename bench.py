@@ -636,6 +636,115 @@ def bench_verify(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_fixtures(args):
+    """The reference's own benchmark fixtures through the drop-in calls, timed
+    as the reference times them (host inputs to host outputs, transfers
+    included; median of 3 after a warm-up, benchmarks/pairing.cpp:39-52 and
+    benchmarks/clustering.cpp:39-58), beside its published RTX 4070 rows and
+    the reference's CPU code (oracle/_ref) on this host:
+      * pairing: 10,000 selected samples in 100 curvature groups, t = 1e-6
+        -> 495,000 pairs (benchmarks/pairing.cpp:12-36), FP64 and
+        FP32-guarded, plus the device-resident count + fill;
+      * clustering: V = 1,024 / 4,096 / 16,384 votes around 4 modes,
+        h = .03, 4 clusters, 100 iterations, rng seed 42
+        (benchmarks/clustering.cpp:17-37)."""
+    import numpy as np
+    import torch
+    from mitra_algo_amd import capi
+    from tests import fixtures as fx
+    try:
+        from oracle.pyoracle import Oracle
+        ref = Oracle("ref")
+    except (ImportError, FileNotFoundError, OSError):
+        ref = None
+    sess = capi.Session(0)
+
+    def med(fn, runs=3):
+        fn()
+        ts = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3, out
+
+    # ------------------------------------------------------------- pairing
+    b = fx.pairing_benchmark()
+    pairing = {"workload": "benchmarks/pairing.cpp:12-36 (10,000 selected, 100 groups, "
+                           "t=1e-6, max_pairs 600,000)",
+               "published": {"cpu_median_ms": 93.54, "cuda_median_ms": 11.47,
+                             "hardware": "RTX 4070 Laptop + host (rtx4070_wsl_pairing.txt)"}}
+    want = None
+    if ref is not None:
+        ms_, want = med(lambda: ref.pairing(b.vertices, b.kmin, b.kmax, b.selected, b.box, 1e-6,
+                                            600000))
+        pairing["reference_cpu_here_ms"] = ms_
+    for name, prec in (("fp64", capi.FP64), ("fp32g", capi.FP32_GUARDED)):
+        ms_, got = med(lambda: sess.pairing(b.vertices, b.feat, b.selected, b.box, 1e-6, 600000,
+                                            prec))
+        entry = {"median_ms": ms_, "pairs": int(len(got[0])),
+                 "pairs_per_s": len(got[0]) / (ms_ * 1e-3)}
+        if want is not None:
+            entry["equal_to_reference"] = bool(np.array_equal(got[0], want[0]) and
+                                               np.array_equal(got[1], want[1]))
+        pairing["drop_in_" + name] = entry
+    # device-resident count + fill (inputs already in HBM)
+    dev = torch.device("cuda:0")
+    sess.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    v = torch.as_tensor(np.ascontiguousarray(b.vertices.T)).to(dev)
+    feat = torch.as_tensor(np.ascontiguousarray(b.feat)).to(dev)
+    sel = torch.as_tensor(b.selected).to(dev)
+    n = len(b.vertices)
+
+    def resident():
+        P = sess.dev_pair_count(n, v[0], v[1], v[2], feat, len(b.selected), sel, b.box,
+                                [0.0, 0.0, 0.0], 1e-6, 600000, capi.FP64)
+        a_ = torch.empty(P, dtype=torch.int32, device=dev)
+        b_ = torch.empty(P, dtype=torch.int32, device=dev)
+        sess.dev_pair_fill(a_, b_, True)
+        torch.cuda.synchronize()
+        return P
+    ms_, P = med(resident)
+    pairing["device_resident_fp64"] = {"median_ms": ms_, "pairs": int(P)}
+    # ---------------------------------------------------------- clustering
+    clustering = {"workload": "benchmarks/clustering.cpp:17-37 (alternating rigid/reflection "
+                              "votes, 4 modes, h=.03, box 10, 4 clusters, rng 42)",
+                  "published": {"votes": [1024, 4096, 16384],
+                                "cpu_median_ms": [0.121, 0.381, 2.033],
+                                "cuda_median_ms": [5.459, 10.383, 32.670],
+                                "hardware": "RTX 4070 Laptop + host "
+                                            "(rtx4070_wsl_clustering.json)"},
+                  "rows": []}
+    for count in (1024, 4096, 16384):
+        kind, desc, rot = fx.cluster_votes(count)
+        row = {"votes": count}
+        ms_, got = med(lambda: sess.clustering(kind, desc, rot, 10.0, [0, 0, 0], 0.03, 100, 4,
+                                               capi.rng(42)))
+        row.update(hip_median_ms=ms_, clusters=int(len(got["reflection"])))
+        if ref is not None:
+            ms_r, want_c = med(lambda: ref.clustering(kind, desc, rot, 10.0, [0, 0, 0], 0.03, 100,
+                                                      4, ref.rng(42)))
+            row["reference_cpu_here_ms"] = ms_r
+            row["members_equal_to_reference"] = bool(
+                np.array_equal(got["members"], want_c.members) and
+                np.array_equal(got["member_off"], want_c.member_off))
+        clustering["rows"].append(row)
+    sess.close()
+    out = {
+        "metric": "reference benchmark fixtures: pairing median ms (495,000 pairs) and "
+                  "clustering median ms (V = 1,024 / 4,096 / 16,384)",
+        "value": pairing["drop_in_fp64"]["median_ms"], "unit": "ms (pairing, drop-in FP64)",
+        "n_gpus": 1, "steps": 3, "warmup": 1,
+        "ms_per_step": pairing["drop_in_fp64"]["median_ms"], "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "the reference's benchmarks/pairing.cpp and "
+                               "benchmarks/clustering.cpp inputs", "parallelism": "single GPU"},
+        "pairing": pairing, "clustering": clustering,
+        "cpu_model": cpu_model(), "host_threads": host_threads(),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline_verify(v, faces, topo, cl, verify, box):
     """The reference's CPU verification (oracle/_ref, single thread) of the
     first cluster on the full mesh, its own tree build included."""
@@ -697,12 +806,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("cfg2", "batch", "refine", "precision", "verify"),
+    ap.add_argument("--workload", choices=("cfg2", "batch", "refine", "precision", "verify",
+                                           "fixtures"),
                     default="cfg2",
                     help="cfg2: the 2.1M-triangle mesh (default); batch: 512 meshes; "
                          "refine: Gauss-Newton refinement of the top-64 modes of the 96x96 mesh; "
                          "precision: FP32-guarded against FP64 on the 96x96 mesh; "
-                         "verify: verification of 20 clusters on the 2.1M-triangle mesh")
+                         "verify: verification of 20 clusters on the 2.1M-triangle mesh; "
+                         "fixtures: the reference's pairing and clustering benchmark inputs")
     ap.add_argument("--meshes", type=int, default=512)
     ap.add_argument("--check-meshes", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -724,6 +835,9 @@ def main():
         return
     if args.workload == "verify":
         bench_verify(args)
+        return
+    if args.workload == "fixtures":
+        bench_fixtures(args)
         return
     import torch
     from mitra_algo_amd import capi, workload
