@@ -189,20 +189,29 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // perm lists the point coordinates in storage (slot) order: the three grid
-// axes ascending, then the remaining coordinates ascending.
-// reach: neighbourhood half-width in cells (cell side >= cutoff*h / reach),
-// so a query visits (2 reach + 1)^2 rows of 2 reach + 1 cells.
+// axes ascending, then the remaining coordinates ascending.  Rows run along
+// grid axis 0 (x): cells have side cellx along it and side cell along axes 1
+// and 2, with cell >= cutoff*h / reach, so a query visits the (2 reach + 1)^2
+// rows around its (y, z) cell, each cut to the x-cells its slice of the
+// support ball reaches.
 struct Grid {
   int axis[3];
   int perm[kMaxD];
   double lo[3];
-  double cell;
-  double inv;  // 1 / cell: cell_of multiplies, in the keys and the queries alike
+  double cell;   // side along axes 1 and 2
+  double inv;    // 1 / cell: cell_of multiplies, in the keys and the queries alike
+  double cellx;  // side along axis 0 (the rows)
+  double invx;   // 1 / cellx
   int dims[3];
   int reach;
   uint64_t ncells;
 };
 constexpr int kMaxReach = 3;  // 7 x 7 = 49 rows: one row range per lane
+// Grids of the narrow queries (many seeds, few candidates each): rows one
+// cell of side cutoff*h to either side (3 x 3 rows, so few short batches),
+// cells a quarter of that along the rows (a tight cut to the ball).
+constexpr int kNarrowReach = 1;
+constexpr int kNarrowXdiv = 4;
 constexpr uint32_t kMapPerWave = 1024;  // LDS candidate-map entries per wave
 constexpr int kMaxRows = (2 * kMaxReach + 1) * (2 * kMaxReach + 1);
 constexpr int kCellBits = 22;  // at most 2^22 cells: three 8-bit radix passes
@@ -224,13 +233,17 @@ struct MsCtl {
 // on the three coordinates of largest spread (variance; ties: lower index
 // first).  Spread, not extent: a few outlying votes can stretch a coordinate
 // whose votes are otherwise concentrated.
-// Cells have side >= cell0 / 2 when that grid has at most
-// clamp(2 n, 1024, kMaxCells) cells, else side cell0, doubled until it fits.
+// The first (reach, x subdivision) pair whose grid has at most
+// clamp(2 n, 1024, kMaxCells) cells wins: reach from max_reach down to 1,
+// and for each reach the x subdivision from max(xdiv, reach) down to reach
+// (cells of side cell0 / reach across the rows, cell0 / xdiv along them);
+// if none fits, cells of side cell0, doubled until the grid fits.
 // No points: a single cell.  ncells[m] (optional) receives the cell count.
 template <int D>
 __global__ void __launch_bounds__(kThreads)
     k_grid_setup(const double *partial, unsigned nb, const uint64_t *seg_off, uint64_t n,
-                 double cell0, int max_reach, Grid *grids, uint64_t *ncells, unsigned *bad) {
+                 double cell0, int max_reach, int xdiv, Grid *grids, uint64_t *ncells,
+                 unsigned *bad) {
   __shared__ double red[kWavesPerBlock][4 * kMaxD];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const uint32_t mesh = blockIdx.x;
@@ -283,6 +296,8 @@ __global__ void __launch_bounds__(kThreads)
     g.perm[k] = k;
   g.cell = cell0;
   g.inv = 1.0 / cell0;
+  g.cellx = cell0;
+  g.invx = 1.0 / cell0;
   g.dims[0] = g.dims[1] = g.dims[2] = 1;
   g.reach = 1;
   g.ncells = 1;
@@ -318,29 +333,34 @@ __global__ void __launch_bounds__(kThreads)
         if (perm[k] == j)
           alo[k] = lo[j], ext[k] = hi[j] - lo[j];
     }
-    // cells of side cutoff*h / reach for the largest reach up to max_reach
-    // whose grid fits, else cells of side cutoff*h (reach 1), doubled until
-    // the grid fits
-    auto fits = [&](double cell) {
+    // cells of side cx along the rows and cyz across them (see above)
+    auto fits = [&](double cx, double cyz) {
       uint64_t cells = 1;
-      const double inv = 1.0 / cell;
       for (int k = 0; k < 3; ++k) {
-        const double d = floor(ext[k] * inv) + 1;
+        const double d = floor(ext[k] * (1.0 / (k == 0 ? cx : cyz))) + 1;
         g.dims[k] = d < 2147483647.0 ? static_cast<int>(d) : 2147483647;
         cells = cells > cap ? cells : cells * static_cast<uint64_t>(g.dims[k]);
       }
       g.ncells = cells;
       return cells <= cap;
     };
-    int reach = max_reach;
-    double cell = cell0 / reach;
-    while (reach > 1 && !fits(cell)) {
-      --reach;
-      cell = cell0 / reach;
+    int reach = 1;
+    double cell = cell0, cellx = cell0;
+    bool found = false;
+    for (int r = max_reach; r >= 1 && !found; --r)
+      for (int xd = xdiv > r ? xdiv : r; xd >= r && !found; --xd)
+        if (fits(cell0 / xd, cell0 / r)) {
+          found = true;
+          reach = r;
+          cell = cell0 / r;
+          cellx = cell0 / xd;
+        }
+    if (!found) {
+      while (!fits(cell, cell))
+        cell *= 2;
+      cellx = cell;
     }
     g.reach = reach;
-    while (!fits(cell))
-      cell *= 2;
     for (int k = 0; k < 3; ++k) {
       g.axis[k] = perm[k];
       g.lo[k] = alo[k];
@@ -349,6 +369,8 @@ __global__ void __launch_bounds__(kThreads)
       g.perm[k] = perm[k];
     g.cell = cell;
     g.inv = 1.0 / cell;
+    g.cellx = cellx;
+    g.invx = 1.0 / cellx;
   }
   grids[mesh] = g;
   if (ncells)
@@ -365,10 +387,60 @@ __device__ inline int cell_of(double v, double lo, double inv, int dim) {
   return c >= dim ? dim - 1 : static_cast<int>(c);
 }
 template <class R> __device__ inline uint64_t cell_key(const Grid &g, R a, R b, R c) {
-  int x = cell_of(static_cast<double>(a), g.lo[0], g.inv, g.dims[0]);
+  int x = cell_of(static_cast<double>(a), g.lo[0], g.invx, g.dims[0]);
   int y = cell_of(static_cast<double>(b), g.lo[1], g.inv, g.dims[1]);
   int z = cell_of(static_cast<double>(c), g.lo[2], g.inv, g.dims[2]);
   return (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0] + x;
+}
+
+// Neighbourhood rows.  Row k (0 <= k < (2 reach + 1)^2: dz = k / span -
+// reach, dy = k % span - reach around the seed's (y, z) cell (cy, cz)) is
+// cut to the x-cells its slice of the support ball can reach: a point of row
+// (y, z) is at least ey, ez from the seed in those two coordinates, so within
+// the support only if |x - m0| <= sqrt(rlim^2 - ey^2 - ez^2).  Cells are
+// monotone in each coordinate (also where clamped at the grid edges); rlim
+// and the slab bounds (slop) carry margins far above the rounding of the
+// FP32 / FP64 distance test, the half-width has a relative margin of 1e-6
+// over its FP32 root and the x-cells one of 1e-6 cells over the product by
+// 1 / cellx: no point inside the support is cut.  Returns the row's first
+// sorted record and its length (0: the row misses the ball or the grid).
+struct RowCut {
+  double slop, rlim2;
+};
+__device__ inline RowCut row_cut(const Grid &g, double r2) {
+  RowCut c;
+  c.slop = 1e-9 * (1.0 + fabs(g.lo[1]) + fabs(g.lo[2]) + g.cell * (g.dims[1] + g.dims[2]));
+  const double rlim = sqrt(r2) * (1.0 + 1e-5) + c.slop;
+  c.rlim2 = rlim * rlim;
+  return c;
+}
+__device__ inline void cut_row(const Grid &g, const RowCut &c, uint64_t cb,
+                               const uint64_t *offsets, double mx, double my, double mz, int cy,
+                               int cz, int k, uint32_t &rb, uint32_t &len) {
+  rb = 0, len = 0;
+  const int span = 2 * g.reach + 1;
+  const int z = cz + k / span - g.reach, y = cy + k % span - g.reach;
+  if (k >= span * span || z < 0 || z >= g.dims[2] || y < 0 || y >= g.dims[1])
+    return;
+  double ey = y > cy ? g.lo[1] + y * g.cell - my : (y < cy ? my - (g.lo[1] + (y + 1) * g.cell) : 0.0);
+  double ez = z > cz ? g.lo[2] + z * g.cell - mz : (z < cz ? mz - (g.lo[2] + (z + 1) * g.cell) : 0.0);
+  ey = ey > c.slop ? ey - c.slop : 0.0;
+  ez = ez > c.slop ? ez - c.slop : 0.0;
+  const double rem = c.rlim2 - ey * ey - ez * ez;
+  if (!(rem > 0))
+    return;
+  const double hx = static_cast<double>(sqrtf(static_cast<float>(rem))) * (1.0 + 1e-6);
+  const double ua = floor((mx - hx - g.lo[0]) * g.invx - 1e-6);
+  const double ub = floor((mx + hx - g.lo[0]) * g.invx + 1e-6);
+  const int last = g.dims[0] - 1;
+  const int xa = ua > 0 ? (ua < last + 1 ? static_cast<int>(ua) : last + 1) : 0;
+  const int xb = ub < last ? (ub > -1 ? static_cast<int>(ub) : -1) : last;
+  if (xa > xb)
+    return;
+  const uint64_t row = cb + (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
+  const uint64_t b = offsets[row + xa];
+  rb = static_cast<uint32_t>(b);
+  len = static_cast<uint32_t>(offsets[row + xb + 1] - b);
 }
 
 // Storage of the cell-sorted points: slot coordinates 0-3 (three grid axes
@@ -599,72 +671,19 @@ __global__ void __launch_bounds__(kThreads)
   R wsum = 0;
   unsigned long long cand = 0, inside = 0;
   int done = 0;  // updates computed
-  const int reach = g.reach, span = 2 * reach + 1, nrows = span * span;
-  const int dz = lane / span - reach, dy = lane % span - reach;
-  // row cutting (below): slab slop, support radius with margins, 1 / cell
-  const double slop =
-      1e-9 * (1.0 + fabs(g.lo[1]) + fabs(g.lo[2]) + g.cell * (g.dims[1] + g.dims[2]));
-  const double rlim = sqrt(static_cast<double>(r2)) * (1.0 + 1e-5) + slop;
-  const double rlim2 = rlim * rlim, inv_cell = g.inv;
+  const int span = 2 * g.reach + 1, nrows = span * span;
+  const RowCut rc = row_cut(g, static_cast<double>(r2));
   for (int it = 0; it < iterations; ++it) {
-    // lanes 0..nrows-1 fetch the ranges of the neighbourhood rows.  With
-    // W = 4 each row is cut to the cells its slice of the support ball can
-    // reach: a point of row (y, z) is at least ey, ez from the seed in those
-    // two coordinates, so within the support only if |x - m0| <=
-    // sqrt(rlim^2 - ey^2 - ez^2).  Cells are monotone in each coordinate
-    // (also where clamped at the grid edges), rlim and the slab bounds carry
-    // margins far above the rounding of the FP32 / FP64 distance test: no
-    // point inside the support is cut.  The cut pays where a seed has
-    // thousands of candidates (a block per seed); with one wave per seed and
-    // a few hundred candidates its cost exceeds what it saves, so those rows
-    // stay whole.
+    // lanes 0..nrows-1 fetch the ranges of the neighbourhood rows, each cut
+    // to the support ball (cut_row)
     uint32_t rb = 0, startv = 0, rend = 0, total = 0;
     if (W == 1 || wave == 0) {
-      const int cx = cell_of(static_cast<double>(m[0]), g.lo[0], g.inv, g.dims[0]);
       const int cy = cell_of(static_cast<double>(m[1]), g.lo[1], g.inv, g.dims[1]);
       const int cz = cell_of(static_cast<double>(m[2]), g.lo[2], g.inv, g.dims[2]);
-      const int x0 = cx > reach ? cx - reach : 0;
-      const int x1 = cx + reach < g.dims[0] ? cx + reach : g.dims[0] - 1;
       uint32_t len = 0;
-      if (W == 1 && lane < nrows) {
-        const int z = cz + dz, y = cy + dy;
-        if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
-          const uint64_t row = cb + (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
-          const uint64_t b = offsets[row + x0];
-          rb = static_cast<uint32_t>(b);
-          len = static_cast<uint32_t>(offsets[row + x1 + 1] - b);
-        }
-      } else if (W > 1 && lane < nrows) {
-        const int z = cz + dz, y = cy + dy;
-        if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
-          const double my = static_cast<double>(m[1]), mz = static_cast<double>(m[2]);
-          double ey = y > cy ? g.lo[1] + y * g.cell - my
-                             : (y < cy ? my - (g.lo[1] + (y + 1) * g.cell) : 0.0);
-          double ez = z > cz ? g.lo[2] + z * g.cell - mz
-                             : (z < cz ? mz - (g.lo[2] + (z + 1) * g.cell) : 0.0);
-          ey = ey > slop ? ey - slop : 0.0;
-          ez = ez > slop ? ez - slop : 0.0;
-          const double rem = rlim2 - ey * ey - ez * ez;
-          if (rem > 0) {
-            // half-width (FP32 root, relative margin 1e-6) and its cells, by
-            // the inverse cell side with a margin of 1e-6 cells: xa never
-            // exceeds, xb never falls short of, the cells cell_of gives to
-            // points inside [mx - hx, mx + hx]
-            const double hx = static_cast<double>(sqrtf(static_cast<float>(rem))) * (1.0 + 1e-6);
-            const double mx = static_cast<double>(m[0]);
-            const double ua = floor((mx - hx - g.lo[0]) * inv_cell - 1e-6);
-            const double ub = floor((mx + hx - g.lo[0]) * inv_cell + 1e-6);
-            const int xa = ua > x0 ? (ua < x1 + 1 ? static_cast<int>(ua) : x1 + 1) : x0;
-            const int xb = ub < x1 ? (ub > x0 - 1 ? static_cast<int>(ub) : x0 - 1) : x1;
-            if (xa <= xb) {
-              const uint64_t row = cb + (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
-              const uint64_t b = offsets[row + xa];
-              rb = static_cast<uint32_t>(b);
-              len = static_cast<uint32_t>(offsets[row + xb + 1] - b);
-            }
-          }
-        }
-      }
+      if (lane < nrows)
+        cut_row(g, rc, cb, offsets, static_cast<double>(m[0]), static_cast<double>(m[1]),
+                static_cast<double>(m[2]), cy, cz, lane, rb, len);
       const uint32_t incl = wave_incl_scan(len);  // lanes >= nrows add 0
       // row r covers flattened [start_r, start_r + len_r), start_r = incl_{r-1};
       // lane r holds start_r (lane nrows: the total) and the row's first record
@@ -730,12 +749,14 @@ __global__ void __launch_bounds__(kThreads)
       R dd = d[0] * d[0];
       for (int k = 1; k < D; ++k)
         dd = ms_fma(d[k], d[k], dd);
-      if (dd < r2) {
+      // in-support visits: the wave's count, kept in a scalar register
+      const bool inb = dd < r2;
+      in32 += static_cast<uint32_t>(__popcll(ballot(inb)));
+      if (inb) {
         const R w = ms_gauss<R>(dd, gscale) - floor_w;
         wl += w;
         for (int k = 0; k < D; ++k)
           acc[k] = ms_fma(w, d[k], acc[k]);
-        ++in32;
       }
     };
     if constexpr (W == 1) {
@@ -836,9 +857,15 @@ __global__ void __launch_bounds__(kThreads)
         rb_r = __builtin_amdgcn_readlane(rbv, r);
         len_r = __builtin_amdgcn_readlane(lenv, r);
       }
+      // batch k -> each lane's record, as a byte offset into p4 (the q2
+      // record sits at half that offset): one add of the lane's own offset
+      // to the batch's scalar base, one compare and one select per batch,
+      // and the loads take the array as a scalar base
+      constexpr uint32_t kRec = sizeof(typename Vec<R>::V4);
+      const uint32_t lane_b = static_cast<uint32_t>(lane) * kRec, sent_b = sentinel * kRec;
       auto rec_of = [&](uint32_t k) -> uint32_t {
         if (k >= k1)
-          return sentinel;
+          return sent_b;
         while (k >= bs_next) {
           ++r;
           bs_cur = bs_next;
@@ -847,12 +874,14 @@ __global__ void __launch_bounds__(kThreads)
           len_r = __builtin_amdgcn_readlane(lenv, r);
         }
         const uint32_t off = (k - bs_cur) * kWave;
-        return static_cast<uint32_t>(lane) < len_r - off ? rb_r + off + lane : sentinel;
+        const uint32_t base = (rb_r + off) * kRec;
+        return static_cast<uint32_t>(lane) < len_r - off ? base + lane_b : sent_b;
       };
-      auto load = [&](uint32_t j, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
-        p = p4[j];
+      auto load = [&](uint32_t b, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
+        p = *reinterpret_cast<const typename Vec<R>::V4 *>(reinterpret_cast<const char *>(p4) + b);
         if (D == 6)
-          q = q2[j];
+          q = *reinterpret_cast<const typename Vec<R>::V2 *>(reinterpret_cast<const char *>(q2) +
+                                                             (b >> 1));
       };
       uint32_t k = k0;
       if (k < k1) {
@@ -922,7 +951,6 @@ __global__ void __launch_bounds__(kThreads)
     if (!moved)
       break;  // uniform across the seed's waves: they hold the same mean
   }
-  inside = wave_sum(inside);
   if (W > 1) {
     __shared__ unsigned long long ins[kWavesPerBlock];
     if (lane == 0)
@@ -978,7 +1006,7 @@ template <int G> __device__ inline uint32_t group_count(uint32_t v) {
 // record.  Group sums come from group_total, so every lane of a group holds
 // the same new mean; a seed whose update leaves it unchanged stops (its rows
 // are then empty), and the wave leaves once all its seeds have.
-constexpr int kSmallRows = 25;
+constexpr int kSmallRows = (2 * kNarrowReach + 1) * (2 * kNarrowReach + 1);
 template <class R, int D, int G>
 __global__ void __launch_bounds__(kThreads)
     k_ms_query_small(const typename Vec<R>::V4 *p4, const typename Vec<R>::V2 *q2,
@@ -990,6 +1018,11 @@ __global__ void __launch_bounds__(kThreads)
                      unsigned long long *stats) {
   static_assert(G == 16 || G == 32, "16 or 32 lanes per seed");
   constexpr int kSlots = (kSmallRows + G - 1) / G;  // rows per lane
+#ifndef MVS_MS_DEPTH32
+#define MVS_MS_DEPTH32 2
+#define MVS_MS_DEPTH64 2
+#endif
+  constexpr int kDepth = sizeof(R) == 4 ? MVS_MS_DEPTH32 : MVS_MS_DEPTH64;  // batches in flight
   constexpr int kSeedsPerBlock = kThreads / G;
   const int lane = threadIdx.x & (kWave - 1);
   const int sl = lane & (G - 1);   // lane in the group
@@ -1030,27 +1063,18 @@ __global__ void __launch_bounds__(kThreads)
   unsigned long long cand = 0;
   uint32_t inside = 0;
   int done = 0;
-  const int reach = g.reach, span = 2 * reach + 1, nrows = span * span;
+  const RowCut rc = row_cut(g, static_cast<double>(r2));
   for (int it = 0; it < iterations; ++it) {
-    // this lane's rows sl, sl + G (row k: dz = k / span - reach, dy = k % span
-    // - reach), whole rows of 2 reach + 1 cells
+    // this lane's rows sl, sl + G (cut_row: each cut to the support ball)
     uint32_t rb[kSlots], len[kSlots];
-    const int cx = cell_of(static_cast<double>(m[0]), g.lo[0], g.inv, g.dims[0]);
     const int cy = cell_of(static_cast<double>(m[1]), g.lo[1], g.inv, g.dims[1]);
     const int cz = cell_of(static_cast<double>(m[2]), g.lo[2], g.inv, g.dims[2]);
-    const int x0 = cx > reach ? cx - reach : 0;
-    const int x1 = cx + reach < g.dims[0] ? cx + reach : g.dims[0] - 1;
     uint32_t mine = 0;
     for (int q = 0; q < kSlots; ++q) {
-      const int k = q * G + sl;
       rb[q] = 0, len[q] = 0;
-      const int z = cz + k / span - reach, y = cy + k % span - reach;
-      if (active && k < nrows && z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
-        const uint64_t row = cb + (static_cast<uint64_t>(z) * g.dims[1] + y) * g.dims[0];
-        const uint64_t b = offsets[row + x0];
-        rb[q] = static_cast<uint32_t>(b);
-        len[q] = static_cast<uint32_t>(offsets[row + x1 + 1] - b);
-      }
+      if (active)
+        cut_row(g, rc, cb, offsets, static_cast<double>(m[0]), static_cast<double>(m[1]),
+                static_cast<double>(m[2]), cy, cz, q * G + sl, rb[q], len[q]);
       mine += len[q];
     }
     // batches per row: the most any group of the wave needs; lane l < 32
@@ -1121,19 +1145,24 @@ __global__ void __launch_bounds__(kThreads)
         q = q2[j];
     };
     if (totb) {
-      typename Vec<R>::V4 pa, pb;
-      typename Vec<R>::V2 qa{}, qb{};
-      load(rec_of(0), pa, qa);
-      uint32_t jb = rec_of(1);
-      for (uint32_t t = 0; t < totb; t += 2) {
-        load(jb, pb, qb);
-        const uint32_t ja = rec_of(t + 2);
-        __builtin_amdgcn_sched_barrier(0);
-        test(pa, qa);
-        load(ja, pa, qa);
-        jb = rec_of(t + 3);
-        __builtin_amdgcn_sched_barrier(0);
-        test(pb, qb);
+      // kDepth batches in flight: the records of batches t + 1 .. t + kDepth
+      // - 1 are loading while batch t is tested (rows are short, so a batch
+      // rarely has the arithmetic to cover one load's latency); batches past
+      // totb read the sentinel record
+      typename Vec<R>::V4 p[kDepth];
+      typename Vec<R>::V2 q[kDepth];
+      for (int j = 0; j < kDepth; ++j) {
+        q[j] = {};
+        load(rec_of(j), p[j], q[j]);
+      }
+      for (uint32_t t = 0; t < totb; t += kDepth) {
+#pragma unroll
+        for (int j = 0; j < kDepth; ++j) {
+          const uint32_t nxt = rec_of(t + kDepth + j);
+          __builtin_amdgcn_sched_barrier(0);
+          test(p[j], q[j]);
+          load(nxt, p[j], q[j]);
+        }
       }
     }
     mine = group_count<G>(mine);
@@ -1240,7 +1269,9 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     raise(MVS_EINVAL, "batched mean-shift supports 4-D (reflection) and 6-D (rigid) points");
   if (!(h > 0) || !(cutoff > 0) || !std::isfinite(h * cutoff) || iterations < 0)
     raise(MVS_EINVAL, "invalid mean-shift parameters");
-  if (n > static_cast<uint64_t>(UINT32_MAX) || ns > static_cast<uint64_t>(UINT32_MAX))
+  // the queries address records by 32-bit byte offsets (the sentinel at [n])
+  if (n > static_cast<uint64_t>(UINT32_MAX) / sizeof(V4) - 1 ||
+      ns > static_cast<uint64_t>(UINT32_MAX))
     raise(MVS_EOVERFLOW, "too many points or seeds");
   if (seed_ids && !seg && ns > n)
     raise(MVS_EINVAL, "more seed votes than votes");
@@ -1330,13 +1361,14 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   // support ball); with one wave per seed the per-row cost of finer cells
   // outweighs the candidates they save (reach 2).
   const bool wide = ns < s->tune.wide_seeds;
-  const int max_reach = wide ? kMaxReach : 2;
+  const int max_reach = wide ? kMaxReach : kNarrowReach;
+  const int xdiv = wide ? kMaxReach : kNarrowXdiv;
   if (D == 6)
     MVS_LAUNCH(s, "ms_grid", k_grid_setup<6>, meshes, kThreads, 0, partial, nb, seg_off, n,
-               cutoff * h, max_reach, grids, ncell, &ctl->bad);
+               cutoff * h, max_reach, xdiv, grids, ncell, &ctl->bad);
   else
     MVS_LAUNCH(s, "ms_grid", k_grid_setup<4>, meshes, kThreads, 0, partial, nb, seg_off, n,
-               cutoff * h, max_reach, grids, ncell, &ctl->bad);
+               cutoff * h, max_reach, xdiv, grids, ncell, &ctl->bad);
   int key_bits = kCellBits;
   if (seg) {  // cells of mesh m follow those of meshes < m
     cellbase = scratch_as<uint64_t>(s, kSlotBatchMs3, meshes + 1);
