@@ -138,6 +138,8 @@ enum Slot : int {
   // queued on the side stream beside bank-0 work
   kSlotSortB0, kSlotSortB1, kSlotSortB2, kSlotSortBHist, kSlotScanB,
   kSlotMsBankPart, kSlotMsBankKeys, kSlotMsBankVals, kSlotMsBankSeedKeys, kSlotMsBankAos,
+  // pairing plan: positions at bucketed positions (FP64, FP32)
+  kSlotPlanPosB, kSlotPlanPosB32,
   kSlotCount
 };
 

@@ -127,10 +127,12 @@ __global__ void k_gather_samples(const double *x, const double *y, const double 
 }
 
 // One wavefront per bucket of 64 kmin-sorted samples: sort by (kmax, ordinal)
-// with a 64-lane bitonic network and write the bucketed index.
+// with a 64-lane bitonic network and write the bucketed index, with each
+// entry's curvatures and position beside it (the walk reads a candidate's
+// fields with coalesced loads at its bucketed position, not at its ordinal).
 __global__ void __launch_bounds__(kThreads)
     k_bucket_index(const uint32_t *ord, const double *r64, const float *r32, uint64_t S,
-                   uint32_t *bord, double2 *bk, float2 *bkf) {
+                   uint32_t *bord, double2 *bk, float2 *bkf, double4 *pb, float4 *pbf) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
   const uint64_t nb = (S + kBucket - 1) / kBucket;
@@ -157,10 +159,15 @@ __global__ void __launch_bounds__(kThreads)
       }
     }
   if (j < S) {
+    const double *r = r64 + static_cast<uint64_t>(o) * kRec;
     bord[j] = o;
-    bk[j] = make_double2(r64[static_cast<uint64_t>(o) * kRec + L_K0], key);
-    if (bkf)
-      bkf[j] = make_float2(r32[static_cast<uint64_t>(o) * kRec + L_K0], r32[static_cast<uint64_t>(o) * kRec + L_K1]);
+    bk[j] = make_double2(r[L_K0], key);
+    pb[j] = make_double4(r[L_X], r[L_X + 1], r[L_X + 2], 0.0);
+    if (bkf) {
+      const float *f = r32 + static_cast<uint64_t>(o) * kRec;
+      bkf[j] = make_float2(f[L_K0], f[L_K1]);
+      pbf[j] = make_float4(f[L_X], f[L_X + 1], f[L_X + 2], 0.f);
+    }
   }
 }
 
@@ -263,6 +270,8 @@ struct PairArgs {
   const uint32_t *bord;
   const double2 *bk;   // (kmin, kmax) at bucketed positions, kmax sorted per bucket
   const float2 *bkf;   // FP32 copy
+  const double4 *pb;   // positions (x, y, z, 0) at bucketed positions
+  const float4 *pbf;   // FP32 copy
   const uint64_t *win;
   const double2 *kw;
   uint64_t S;
@@ -325,9 +334,8 @@ __device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, 
     double tol = A.t * m;
     if (fabs(src.a0 - b.x) > tol || fabs(src.a1 - b.y) > tol)
       return false;
-    const double2 *p = reinterpret_cast<const double2 *>(A.r64 + static_cast<uint64_t>(o) * kRec);
-    double2 p0 = p[0], p1 = p[1];
-    return accept64(src.a0, src.a1, b.x, b.y, src.ax, src.ay, src.az, p0.x, p0.y, p1.x, A.t,
+    const double4 p = A.pb[j];
+    return accept64(src.a0, src.a1, b.x, b.y, src.ax, src.ay, src.az, p.x, p.y, p.z, A.t,
                     src.dist);
   } else {
     const float2 b = bkj;
@@ -335,7 +343,7 @@ __device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, 
     if (r == 0)
       return false;
     if (r == 1) {
-      float4 p = *reinterpret_cast<const float4 *>(A.r32 + static_cast<uint64_t>(o) * kRec);
+      const float4 p = A.pbf[j];
       int d = screen32_dist(src.fx - p.x, src.fy - p.y, src.fz - p.z, src.dist32, A.gd);
       if (d != 2)
         return d == 1;
@@ -741,6 +749,8 @@ struct PairState {
   uint32_t *ord, *bord, *slab;
   double2 *bk, *kw;
   float2 *bkf;
+  double4 *pb;
+  float4 *pbf;
   uint64_t *keys, *win, *counts, *offsets;
   uint32_t *long_list, *cand, *cand2, *rechk;
 };
@@ -753,6 +763,8 @@ static PairState pair_state(Session *s, uint64_t S, bool guarded) {
   st.ord = scratch_as<uint32_t>(s, kSlotPlanOrd, 2 * Sp);  // [0,S) sort input, [S,2S) sorted
   st.bk = scratch_as<double2>(s, kSlotPlanSortedK, Sp);
   st.bkf = guarded ? scratch_as<float2>(s, kSlotPlanSortedK32, Sp) : nullptr;
+  st.pb = scratch_as<double4>(s, kSlotPlanPosB, Sp);
+  st.pbf = guarded ? scratch_as<float4>(s, kSlotPlanPosB32, Sp) : nullptr;
   st.bord = scratch_as<uint32_t>(s, kSlotWindows, Sp);
   st.slab = scratch_as<uint32_t>(s, kSlotPlanSlab, kSlab * Sp);
   st.keys = scratch_as<uint64_t>(s, kSlotPlanKeys, 2 * Sp);
@@ -777,6 +789,8 @@ static PairArgs pair_args(const PairState &st, uint64_t S, double t, double box,
   A.bord = st.bord;
   A.bk = st.bk;
   A.bkf = st.bkf;
+  A.pb = st.pb;
+  A.pbf = st.pbf;
   A.win = st.win;
   A.kw = st.kw;
   A.S = S;
@@ -847,7 +861,7 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   radix_sort_pairs_u64(s, st.keys, st.ord, st.keys + S, st.ord + S, S, key_bits);
   const uint64_t nb = (S + kBucket - 1) / kBucket;
   MVS_LAUNCH(s, "pair_index", k_bucket_index, blocks_for(nb, kWaves), kThreads, 0, st.ord + S,
-             st.r64, st.r32, S, st.bord, st.bk, st.bkf);
+             st.r64, st.r32, S, st.bord, st.bk, st.bkf, st.pb, st.pbf);
   MVS_LAUNCH(s, "pair_windows", k_windows, blocks_for(S, kThreads), kThreads, 0, st.r64,
              st.keys + S, S, t, segs.seg, st.win, st.kw);
   unsigned *long_count = flags + 1;
