@@ -140,6 +140,8 @@ enum Slot : int {
   kSlotMsBankPart, kSlotMsBankKeys, kSlotMsBankVals, kSlotMsBankSeedKeys, kSlotMsBankAos,
   // pairing plan: positions at bucketed positions (FP64, FP32)
   kSlotPlanPosB, kSlotPlanPosB32,
+  // radix sort: digit counts, tile tickets and tile states (banks 0 and 1)
+  kSlotSortState, kSlotSortBState,
   kSlotCount
 };
 

@@ -2,6 +2,8 @@
 // used for the curvature-signature sort.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "session.hpp"
 
@@ -255,6 +257,190 @@ __global__ void __launch_bounds__(kSortThreads)
   }
 }
 
+// ------------------------------------------------ single-pass digit passes
+// Each LSD pass is one launch (after one launch that counts every pass's
+// digits): a tile takes a ticket in launch order, ranks its keys in LDS,
+// publishes its per-digit counts, and finds the counts of the tiles before
+// it by looking back over their published states (decoupled look-back,
+// Merrill & Garland 2016; one pass per digit as in Adinets & Merrill 2022).
+// A state word holds a flag (2 bits) and a count (30 bits): kStAgg, the
+// tile's own counts; kStPre, the inclusive prefix over tiles 0..t.  Tickets
+// follow the order in which blocks start, so every tile a block waits on
+// belongs to a block that is already running and publishes its counts
+// before it looks back itself: every wait ends.  The spin is also bounded
+// (kStSpins windows); past the bound the tile sets the error word and goes
+// on with what it has, so a broken invariant shows as wrong order, never as
+// a hang.
+constexpr uint32_t kStAgg = 1u << 30, kStPre = 2u << 30, kStCount = (1u << 30) - 1;
+constexpr int kStWindow = 16;           // predecessor states read together
+constexpr uint32_t kStSpins = 1u << 22;  // windows read before giving up
+constexpr int kMaxPasses = 8;
+// Up to this many tiles a pass is one launch.  Tiles that run together all
+// publish before any finds a prefix, so the look-back front spreads over
+// about sqrt(2 tiles / kStWindow) rounds of reads; past a few hundred tiles
+// that costs as much as the separate count and scan launches it saves.
+constexpr unsigned kStMaxTiles = 256;
+
+__device__ inline uint32_t st_load(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_store(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Digit counts of every pass in one read of the keys: ghist[p * 256 + d]
+// (cleared before).
+__global__ void __launch_bounds__(kSortThreads)
+    k_os_hist(const uint64_t *keys, uint64_t n, int passes, uint32_t *ghist) {
+  __shared__ uint32_t h[kMaxPasses][kRadix];
+  for (int p = 0; p < passes; ++p)
+    h[p][threadIdx.x] = 0;
+  __syncthreads();
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kSortThreads + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * kSortThreads) {
+    const uint64_t k = keys[i];
+    for (int p = 0; p < passes; ++p)
+      atomicAdd(&h[p][(k >> (8 * p)) & 0xFF], 1u);
+  }
+  __syncthreads();
+  for (int p = 0; p < passes; ++p)
+    if (h[p][threadIdx.x])
+      atomicAdd(&ghist[p * kRadix + threadIdx.x], h[p][threadIdx.x]);
+}
+
+// One pass: ghist the pass's digit counts, ticket its tile counter, state
+// its tiles' state words (all cleared before the first pass).
+__global__ void __launch_bounds__(kSortThreads)
+    k_os_scatter(const uint64_t *keys_in, const uint32_t *vals_in, uint64_t n, int shift,
+                 const uint32_t *ghist, unsigned *ticket, uint32_t *state, unsigned *err,
+                 uint64_t *keys_out, uint32_t *vals_out) {
+  __shared__ uint32_t wave_hist[kSortWaves][kRadix];
+  __shared__ uint32_t tile_off[kRadix];
+  __shared__ uint64_t s_keys[kSortTile];
+  __shared__ uint32_t s_vals[kSortTile];
+  __shared__ uint64_t g_off[kRadix];
+  __shared__ uint32_t wsum[kSortWaves];
+  __shared__ uint32_t s_tile;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  if (threadIdx.x == 0)
+    s_tile = atomicAdd(ticket, 1u);
+  for (int w = 0; w < kSortWaves; ++w)
+    wave_hist[w][threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t tile_base = static_cast<uint64_t>(tile) * kSortTile;
+  uint64_t k[kSortItems];
+  uint32_t v[kSortItems];
+  uint32_t rank[kSortItems];
+  uint32_t dig[kSortItems];
+  for (int r = 0; r < kSortItems; ++r) {
+    uint64_t j = tile_base + wave * (kSortItems * kWave) + r * kWave + lane;
+    bool in = j < n;
+    k[r] = in ? keys_in[j] : 0;
+    v[r] = in ? vals_in[j] : 0;
+    uint32_t d = in ? static_cast<uint32_t>((k[r] >> shift) & 0xFF) : 0u;
+    dig[r] = d;
+    uint64_t match = ballot(in);
+    for (int b = 0; b < 8; ++b) {
+      uint64_t bb = ballot(((d >> b) & 1u) != 0);
+      match &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t base = wave_hist[wave][d];
+    uint32_t below = static_cast<uint32_t>(lanes_below(match));
+    rank[r] = base + below;
+    if (in && below == 0)
+      wave_hist[wave][d] = base + static_cast<uint32_t>(__popcll(match));
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  const int d = threadIdx.x;
+  uint32_t tot = 0;
+  {
+    uint32_t pw[kSortWaves];
+    for (int w = 0; w < kSortWaves; ++w) {
+      pw[w] = tot;
+      tot += wave_hist[w][d];
+    }
+    // this tile's count of digit d, published before anything else
+    uint32_t *mine = state + static_cast<uint64_t>(tile) * kRadix + d;
+    if (tile == 0)
+      st_store(mine, kStPre | tot);
+    else
+      st_store(mine, kStAgg | tot);
+    __syncthreads();
+    for (int w = 0; w < kSortWaves; ++w)
+      wave_hist[w][d] = pw[w];
+  }
+  // tile offsets of the digits (exclusive scan of tot over d) and the pass's
+  // digit starts (exclusive scan of ghist over d)
+  uint32_t x = tot, gx = ghist[d];
+  const uint32_t gd = gx;
+  for (int o = 1; o < kWave; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, kWave), gy = __shfl_up(gx, o, kWave);
+    if (lane >= o)
+      x += y, gx += gy;
+  }
+  __shared__ uint32_t gsum[kSortWaves];
+  if (lane == kWave - 1)
+    wsum[wave] = x, gsum[wave] = gx;
+  // tiles before this one: the look-back over their states
+  uint32_t excl = 0;
+  if (tile > 0) {
+    int64_t j = static_cast<int64_t>(tile) - 1;
+    uint32_t windows = 0;
+    bool done = false;
+    while (!done) {
+      uint32_t w[kStWindow];
+#pragma unroll
+      for (int q = 0; q < kStWindow; ++q)
+        w[q] = j - q >= 0 ? st_load(state + static_cast<uint64_t>(j - q) * kRadix + d) : kStPre;
+      int q = 0;
+      for (; q < kStWindow; ++q) {
+        const uint32_t flag = w[q] & ~kStCount;
+        if (flag == 0)
+          break;  // not yet published: read again from here
+        excl += w[q] & kStCount;
+        if (flag == kStPre) {
+          done = true;
+          break;
+        }
+      }
+      j -= q;
+      if (!done && ++windows >= kStSpins) {
+        atomicOr(err, 1u);
+        break;
+      }
+    }
+    st_store(state + static_cast<uint64_t>(tile) * kRadix + d, kStPre | (excl + tot));
+  }
+  __syncthreads();
+  {
+    uint32_t add = 0, gadd = 0;
+    for (int w = 0; w < wave; ++w)
+      add += wsum[w], gadd += gsum[w];
+    tile_off[d] = x - tot + add;
+    g_off[d] = static_cast<uint64_t>(gx - gd + gadd) + excl;
+  }
+  __syncthreads();
+  for (int r = 0; r < kSortItems; ++r) {
+    uint64_t j = tile_base + wave * (kSortItems * kWave) + r * kWave + lane;
+    if (j < n) {
+      uint32_t p = tile_off[dig[r]] + wave_hist[wave][dig[r]] + rank[r];
+      s_keys[p] = k[r];
+      s_vals[p] = v[r];
+    }
+  }
+  __syncthreads();
+  uint64_t count = n - tile_base < kSortTile ? n - tile_base : kSortTile;
+  for (int i = threadIdx.x; i < static_cast<int>(count); i += kSortThreads) {
+    uint64_t key = s_keys[i];
+    uint32_t dd = static_cast<uint32_t>((key >> shift) & 0xFF);
+    uint64_t dst = g_off[dd] + (i - tile_off[dd]);
+    keys_out[dst] = key;
+    vals_out[dst] = s_vals[i];
+  }
+}
+
 __global__ void k_zero_bytes(unsigned char *p, uint64_t bytes) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   uint64_t words = bytes / 4;
@@ -294,11 +480,36 @@ void radix_sort_pairs_u64(Session *s, const uint64_t *keys_in, const uint32_t *v
   const bool b = s->bank != 0;
   uint64_t *k2 = scratch_as<uint64_t>(s, b ? kSlotSortB0 : kSlotSort0, n);
   uint32_t *v2 = scratch_as<uint32_t>(s, b ? kSlotSortB1 : kSlotSort1, n);
+  const int passes = (bits + 7) / 8;
+  if (passes <= kMaxPasses && tiles <= kStMaxTiles) {
+    // one launch clears the counts, tickets and states, one counts every
+    // pass's digits, then one launch per pass
+    const size_t words = static_cast<size_t>(passes) * kRadix + passes + 1 +
+                         static_cast<size_t>(passes) * tiles * kRadix;
+    uint32_t *st = scratch_as<uint32_t>(s, b ? kSlotSortBState : kSlotSortState, words);
+    zero_device(s, st, words * sizeof(uint32_t));
+    uint32_t *ghist = st, *ticket = st + passes * kRadix, *err = ticket + passes,
+             *state = err + 1;
+    MVS_LAUNCH(s, "radix_hist", k_os_hist, std::min<unsigned>(tiles, 1024u), kSortThreads, 0,
+               keys_in, n, passes, ghist);
+    const uint64_t *ksrc = keys_in;
+    const uint32_t *vsrc = vals_in;
+    for (int p = 0; p < passes; ++p) {
+      bool last_to_out = ((passes - 1 - p) % 2) == 0;
+      uint64_t *kdst = last_to_out ? keys_out : k2;
+      uint32_t *vdst = last_to_out ? vals_out : v2;
+      MVS_LAUNCH(s, "radix_scatter", k_os_scatter, tiles, kSortThreads, 0, ksrc, vsrc, n, 8 * p,
+                 ghist + p * kRadix, ticket + p, state + static_cast<size_t>(p) * tiles * kRadix,
+                 err, kdst, vdst);
+      ksrc = kdst;
+      vsrc = vdst;
+    }
+    return;
+  }
   uint32_t *hist = scratch_as<uint32_t>(s, b ? kSlotSortBHist : kSlotSortHist,
                                         static_cast<size_t>(tiles) * kRadix);
   uint64_t *off = scratch_as<uint64_t>(s, b ? kSlotSortB2 : kSlotSort2,
                                        static_cast<size_t>(tiles) * kRadix + 1);
-  int passes = (bits + 7) / 8;
   // ping-pong so that the last pass writes keys_out/vals_out
   const uint64_t *ksrc = keys_in;
   const uint32_t *vsrc = vals_in;
