@@ -275,11 +275,13 @@ constexpr uint32_t kStAgg = 1u << 30, kStPre = 2u << 30, kStCount = (1u << 30) -
 constexpr int kStWindow = 16;           // predecessor states read together
 constexpr uint32_t kStSpins = 1u << 22;  // windows read before giving up
 constexpr int kMaxPasses = 8;
-// Up to this many tiles a pass is one launch.  Tiles that run together all
-// publish before any finds a prefix, so the look-back front spreads over
-// about sqrt(2 tiles / kStWindow) rounds of reads; past a few hundred tiles
-// that costs as much as the separate count and scan launches it saves.
+// Up to kStMaxTiles tiles of 2048 keys, or of 4096 keys up to kStMaxTiles
+// of those, a pass is one launch.  Tiles that run together all publish
+// before any finds a prefix, so the look-back front spreads over about
+// sqrt(2 tiles / kStWindow) rounds of reads; past a few hundred tiles that
+// costs as much as the separate count and scan launches it saves.
 constexpr unsigned kStMaxTiles = 256;
+constexpr unsigned kStMaxTiles2 = 512;
 
 __device__ inline uint32_t st_load(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -310,14 +312,16 @@ __global__ void __launch_bounds__(kSortThreads)
 
 // One pass: ghist the pass's digit counts, ticket its tile counter, state
 // its tiles' state words (all cleared before the first pass).
+template <int kItems>
 __global__ void __launch_bounds__(kSortThreads)
     k_os_scatter(const uint64_t *keys_in, const uint32_t *vals_in, uint64_t n, int shift,
                  const uint32_t *ghist, unsigned *ticket, uint32_t *state, unsigned *err,
                  uint64_t *keys_out, uint32_t *vals_out) {
   __shared__ uint32_t wave_hist[kSortWaves][kRadix];
   __shared__ uint32_t tile_off[kRadix];
-  __shared__ uint64_t s_keys[kSortTile];
-  __shared__ uint32_t s_vals[kSortTile];
+  constexpr int kTile = kSortThreads * kItems;
+  __shared__ uint64_t s_keys[kTile];
+  __shared__ uint32_t s_vals[kTile];
   __shared__ uint64_t g_off[kRadix];
   __shared__ uint32_t wsum[kSortWaves];
   __shared__ uint32_t s_tile;
@@ -328,13 +332,13 @@ __global__ void __launch_bounds__(kSortThreads)
     wave_hist[w][threadIdx.x] = 0;
   __syncthreads();
   const uint32_t tile = s_tile;
-  const uint64_t tile_base = static_cast<uint64_t>(tile) * kSortTile;
-  uint64_t k[kSortItems];
-  uint32_t v[kSortItems];
-  uint32_t rank[kSortItems];
-  uint32_t dig[kSortItems];
-  for (int r = 0; r < kSortItems; ++r) {
-    uint64_t j = tile_base + wave * (kSortItems * kWave) + r * kWave + lane;
+  const uint64_t tile_base = static_cast<uint64_t>(tile) * kTile;
+  uint64_t k[kItems];
+  uint32_t v[kItems];
+  uint32_t rank[kItems];
+  uint32_t dig[kItems];
+  for (int r = 0; r < kItems; ++r) {
+    uint64_t j = tile_base + wave * (kItems * kWave) + r * kWave + lane;
     bool in = j < n;
     k[r] = in ? keys_in[j] : 0;
     v[r] = in ? vals_in[j] : 0;
@@ -422,8 +426,8 @@ __global__ void __launch_bounds__(kSortThreads)
     g_off[d] = static_cast<uint64_t>(gx - gd + gadd) + excl;
   }
   __syncthreads();
-  for (int r = 0; r < kSortItems; ++r) {
-    uint64_t j = tile_base + wave * (kSortItems * kWave) + r * kWave + lane;
+  for (int r = 0; r < kItems; ++r) {
+    uint64_t j = tile_base + wave * (kItems * kWave) + r * kWave + lane;
     if (j < n) {
       uint32_t p = tile_off[dig[r]] + wave_hist[wave][dig[r]] + rank[r];
       s_keys[p] = k[r];
@@ -431,7 +435,7 @@ __global__ void __launch_bounds__(kSortThreads)
     }
   }
   __syncthreads();
-  uint64_t count = n - tile_base < kSortTile ? n - tile_base : kSortTile;
+  uint64_t count = n - tile_base < kTile ? n - tile_base : kTile;
   for (int i = threadIdx.x; i < static_cast<int>(count); i += kSortThreads) {
     uint64_t key = s_keys[i];
     uint32_t dd = static_cast<uint32_t>((key >> shift) & 0xFF);
@@ -481,11 +485,14 @@ void radix_sort_pairs_u64(Session *s, const uint64_t *keys_in, const uint32_t *v
   uint64_t *k2 = scratch_as<uint64_t>(s, b ? kSlotSortB0 : kSlotSort0, n);
   uint32_t *v2 = scratch_as<uint32_t>(s, b ? kSlotSortB1 : kSlotSort1, n);
   const int passes = (bits + 7) / 8;
-  if (passes <= kMaxPasses && tiles <= kStMaxTiles) {
+  const unsigned tiles2 = blocks_for(n, 2 * kSortTile);
+  const bool one = tiles <= kStMaxTiles, two = !one && tiles2 <= kStMaxTiles2;
+  if (passes <= kMaxPasses && (one || two)) {
     // one launch clears the counts, tickets and states, one counts every
     // pass's digits, then one launch per pass
+    const unsigned ntile = one ? tiles : tiles2;
     const size_t words = static_cast<size_t>(passes) * kRadix + passes + 1 +
-                         static_cast<size_t>(passes) * tiles * kRadix;
+                         static_cast<size_t>(passes) * ntile * kRadix;
     uint32_t *st = scratch_as<uint32_t>(s, b ? kSlotSortBState : kSlotSortState, words);
     zero_device(s, st, words * sizeof(uint32_t));
     uint32_t *ghist = st, *ticket = st + passes * kRadix, *err = ticket + passes,
@@ -498,9 +505,13 @@ void radix_sort_pairs_u64(Session *s, const uint64_t *keys_in, const uint32_t *v
       bool last_to_out = ((passes - 1 - p) % 2) == 0;
       uint64_t *kdst = last_to_out ? keys_out : k2;
       uint32_t *vdst = last_to_out ? vals_out : v2;
-      MVS_LAUNCH(s, "radix_scatter", k_os_scatter, tiles, kSortThreads, 0, ksrc, vsrc, n, 8 * p,
-                 ghist + p * kRadix, ticket + p, state + static_cast<size_t>(p) * tiles * kRadix,
-                 err, kdst, vdst);
+      uint32_t *pstate = state + static_cast<size_t>(p) * ntile * kRadix;
+      if (one)
+        MVS_LAUNCH(s, "radix_scatter", k_os_scatter<kSortItems>, ntile, kSortThreads, 0, ksrc,
+                   vsrc, n, 8 * p, ghist + p * kRadix, ticket + p, pstate, err, kdst, vdst);
+      else
+        MVS_LAUNCH(s, "radix_scatter", k_os_scatter<2 * kSortItems>, ntile, kSortThreads, 0,
+                   ksrc, vsrc, n, 8 * p, ghist + p * kRadix, ticket + p, pstate, err, kdst, vdst);
       ksrc = kdst;
       vsrc = vdst;
     }
