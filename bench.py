@@ -48,6 +48,18 @@ def load_traffic():
         return {}
 
 
+def load_counters():
+    """Per-kernel SQ counters of the cfg2 bench from the committed rocprofv3
+    --pmc summary (profiles/counters_cfg2.json), or {}: wave-level VALU
+    instructions and the fraction of the SIMDs' issue cycles they fill
+    (4 cycles per wave64 VALU instruction on a 16-lane SIMD)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "counters_cfg2.json")) as f:
+            return json.load(f).get("kernels", {})
+    except (OSError, ValueError):
+        return {}
+
+
 def load_curvature_traffic():
     """HBM bytes per curvature call from the committed rocprofv3 --pmc
     summary of the curvature kernels, or None."""
@@ -858,19 +870,14 @@ def main():
         for _ in range(warmup):
             wl.step(precision, SEEDS, ITERS)
         torch.cuda.synchronize()
-        # timed region: K full steps; the roofline kernels carry HIP events
+        # timed region: K full steps back to back, a device synchronisation
+        # on either side; the roofline kernels carry HIP events
         wl.sess.enable_timing(True, only=TIMED)
-        t_pv = t_ms = 0.0
         t0 = time.perf_counter()
         for _ in range(steps):
-            a = time.perf_counter()
             pv = wl.pairvote(precision)
-            b = time.perf_counter()
             ms = wl.meanshift(SEEDS, ITERS)
-            torch.cuda.synchronize()
-            c = time.perf_counter()
-            t_pv += b - a
-            t_ms += c - b
+        torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
         kernels = {}
         for k in TIMED:
@@ -878,6 +885,19 @@ def main():
             if n:
                 kernels[k] = {"ms_total_per_step": ms_ / steps, "ms_avg": ms_ / n,
                               "launches_per_step": n / steps}
+        wl.sess.enable_timing(False)
+        # the split into the two calls, untimed: a synchronisation after each
+        t_pv = t_ms = 0.0
+        ns = max(3, steps // 4)
+        for _ in range(ns):
+            a = time.perf_counter()
+            pv = wl.pairvote(precision)
+            b = time.perf_counter()
+            ms = wl.meanshift(SEEDS, ITERS)
+            torch.cuda.synchronize()
+            c = time.perf_counter()
+            t_pv += (b - a) / ns
+            t_ms += (c - b) / ns
         # untimed breakdown of every kernel (events around each launch)
         wl.sess.enable_timing(True)
         nb = 2
@@ -891,7 +911,7 @@ def main():
                                 "launches_per_step": n / nb}
         wl.sess.enable_timing(False)
         stats = wl.meanshift(SEEDS, ITERS, want_stats=True)
-        return pv, stats, dt, t_pv / steps, t_ms / steps, kernels, breakdown
+        return pv, stats, dt, t_pv, t_ms, kernels, breakdown
 
     pv32, ms32, dt32, pv32_s, ms32_s, k32, b32 = run(capi.FP32_GUARDED, args.steps, args.warmup)
     pair32 = (wl.sess.fetch(pv32.a, (pv32.pairs,), torch.int32).cpu(),
@@ -919,15 +939,24 @@ def main():
     q64_ms = k64["ms_query"]["ms_total_per_step"]
     q64_tflops = meanshift_flops_kinds(ms64) / (q64_ms * 1e-3) / 1e12
     traffic = load_traffic()
+    counters = load_counters()
+
+    def valu(*names):
+        return {n: {k: counters[n][k] for k in ("valu_insts", "valu_issue_frac", "us")
+                    if k in counters[n]} for n in names if n in counters}
     rooflines = {
         "vote_emit": {"bound": "hbm", "achieved": emit_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "valu_counters": valu("k_vote_emit<true, float>"),
                       "frac": emit_gbs / HBM_PEAK_GBS,
                       "traffic": traffic.get("vote_emit", {}).get("bytes"),
                       "algorithmic_bytes": emission_bytes(P, R, wl.S, 4),
                       "ms_avg": emit_ms},
         "ms_query": {"bound": "valu",
-                     "pipe": "VALU FP32 (MFMA not used: a 16-seed tile's candidate union "
-                             "is 3.9x (mirror) / 15x (rigid) its members' supports)",
+                     "pipe": "VALU FP32, issue-bound (packed FP32 off); MFMA not used: "
+                             "the support balls of a seed and its 15 nearest seeds cover "
+                             "2.1-2.9x (mirror) / 14-15x (rigid) the points of one ball",
+                     "valu_counters": valu("k_ms_query<float, 4, 4>",
+                                           "k_ms_query_small<float, 6, 32>"),
                      "achieved": q_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": q_tflops / FP32_PEAK_TFLOPS,
                      "traffic": traffic.get("ms_query", {}).get("bytes"),
