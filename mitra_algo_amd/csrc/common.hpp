@@ -139,7 +139,7 @@ enum Slot : int {
   kSlotSortB0, kSlotSortB1, kSlotSortB2, kSlotSortBHist, kSlotScanB,
   kSlotMsBankPart, kSlotMsBankKeys, kSlotMsBankVals, kSlotMsBankSeedKeys, kSlotMsBankAos,
   // pairing plan: positions at bucketed positions (FP64, FP32)
-  kSlotPlanPosB, kSlotPlanPosB32,
+  kSlotPlanPosB, kSlotPlanPosB32, kSlotPlanPivots,
   // radix sort: digit counts, tile tickets and tile states (banks 0 and 1)
   kSlotSortState, kSlotSortBState,
   kSlotCount

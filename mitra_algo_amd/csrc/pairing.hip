@@ -43,6 +43,8 @@ constexpr int kWaves = kThreads / kWave;  // sources per block
 constexpr int kSlab = 128;                // pairs parked per source by the walk
 constexpr uint64_t kLongChunk = 4096;     // ordinals per wave in the long path
 constexpr int kBucket = 64;               // samples per kmin bucket
+constexpr int kPivot = 8;                 // bucket entries per search segment
+constexpr int kRankSort = 24;             // partners placed by rank, not by the network
 constexpr double kEps = 2.220446049250313e-16;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
@@ -132,7 +134,8 @@ __global__ void k_gather_samples(const double *x, const double *y, const double 
 // fields with coalesced loads at its bucketed position, not at its ordinal).
 __global__ void __launch_bounds__(kThreads)
     k_bucket_index(const uint32_t *ord, const double *r64, const float *r32, uint64_t S,
-                   uint32_t *bord, double2 *bk, float2 *bkf, double4 *pb, float4 *pbf) {
+                   uint32_t *bord, double2 *bk, float2 *bkf, double4 *pb, float4 *pbf,
+                   double *piv) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
   const uint64_t nb = (S + kBucket - 1) / kBucket;
@@ -158,6 +161,10 @@ __global__ void __launch_bounds__(kThreads)
         o = po;
       }
     }
+  // every kPivot-th kmax of the bucket (infinite past the samples), for a
+  // search that reads them in one go before one segment of kPivot entries
+  if (lane % kPivot == 0)
+    piv[b * (kBucket / kPivot) + lane / kPivot] = key;
   if (j < S) {
     const double *r = r64 + static_cast<uint64_t>(o) * kRec;
     bord[j] = o;
@@ -272,6 +279,7 @@ struct PairArgs {
   const float2 *bkf;   // FP32 copy
   const double4 *pb;   // positions (x, y, z, 0) at bucketed positions
   const float4 *pbf;   // FP32 copy
+  const double *piv;   // per bucket: the kmax of every kPivot-th entry
   const uint64_t *win;
   const double2 *kw;
   uint64_t S;
@@ -398,27 +406,44 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
     if (b <= b1) {
       const uint64_t base = b * kBucket;
       const uint64_t nb = S - base < static_cast<uint64_t>(kBucket) ? S - base : kBucket;
-      // lower and upper bound of the kmax interval, searched together
-      // (32-bit counters: a bucket holds at most kBucket entries)
-      uint32_t fl = 0, cl = static_cast<uint32_t>(nb), fu = 0, cu = static_cast<uint32_t>(nb);
-      while (cl | cu) {
-        if (cl) {
-          uint32_t h = cl / 2, m = fl + h;
-          if (A.bk[base + m].y < kw.x) {
-            fl = m + 1;
-            cl -= h + 1;
-          } else
-            cl = h;
-        }
-        if (cu) {
-          uint32_t h = cu / 2, m = fu + h;
-          if (!(A.bk[base + m].y > kw.y)) {
-            fu = m + 1;
-            cu -= h + 1;
-          } else
-            cu = h;
-        }
+      // lower bound fl = entries with kmax < kw.x and upper bound fu =
+      // entries with kmax <= kw.y of the kmax-sorted bucket, by counting:
+      // the kPivot pivots (every kPivot-th entry) locate each bound's
+      // segment, whose other entries are then counted; two rounds of
+      // independent loads instead of one dependent load per bisection step
+      constexpr int kSeg = kBucket / kPivot;
+      double pv[kSeg];
+      for (int q = 0; q < kSeg; ++q)
+        pv[q] = A.piv[b * kSeg + q];
+      int sl = 0, su = 0;  // pivots below the lower bound / not above the upper
+      for (int q = 0; q < kSeg; ++q) {
+        sl += pv[q] < kw.x;
+        su += !(pv[q] > kw.y);
       }
+      // segment s - 1 holds the bound (entries 1 .. kPivot - 1 of it are
+      // counted; its pivot already was); s = 0: the bound is 0
+      uint32_t fl = 0, fu = 0;
+      double el[kPivot - 1], eu[kPivot - 1];
+      const uint64_t bl = base + static_cast<uint64_t>(sl > 0 ? sl - 1 : 0) * kPivot;
+      const uint64_t bu = base + static_cast<uint64_t>(su > 0 ? su - 1 : 0) * kPivot;
+      for (int q = 0; q < kPivot - 1; ++q) {
+        el[q] = bl + 1 + q < base + nb ? A.bk[bl + 1 + q].y : INFINITY;
+        eu[q] = bu + 1 + q < base + nb ? A.bk[bu + 1 + q].y : INFINITY;
+      }
+      if (sl > 0) {
+        fl = static_cast<uint32_t>((sl - 1) * kPivot + 1);
+        for (int q = 0; q < kPivot - 1; ++q)
+          fl += bl + 1 + q < base + nb && el[q] < kw.x;
+      }
+      if (su > 0) {
+        fu = static_cast<uint32_t>((su - 1) * kPivot + 1);
+        for (int q = 0; q < kPivot - 1; ++q)
+          fu += bu + 1 + q < base + nb && !(eu[q] > kw.y);
+      }
+      // an infinite bound also counts the infinite pivots past a short
+      // bucket's end
+      fl = fl < nb ? fl : static_cast<uint32_t>(nb);
+      fu = fu < nb ? fu : static_cast<uint32_t>(nb);
       u = static_cast<uint32_t>(base + fl);
       len = fu > fl ? static_cast<uint32_t>(fu - fl) : 0u;
     }
@@ -494,7 +519,9 @@ __global__ void __launch_bounds__(kThreads)
     });
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    if (cnt <= static_cast<uint64_t>(kWave)) {
+    // partners in ordinal order: a lane's rank among few partners costs
+    // fewer instructions than the 21-step bitonic network over the wave
+    if (cnt > static_cast<uint64_t>(kRankSort) && cnt <= static_cast<uint64_t>(kWave)) {
       uint32_t v = static_cast<uint64_t>(lane) < cnt ? L[lane] : kNone;
       v = wave_sort_u32(v, lane);
       if (static_cast<uint64_t>(lane) < cnt)
@@ -751,6 +778,7 @@ struct PairState {
   float2 *bkf;
   double4 *pb;
   float4 *pbf;
+  double *piv;
   uint64_t *keys, *win, *counts, *offsets;
   uint32_t *long_list, *cand, *cand2, *rechk;
 };
@@ -765,6 +793,7 @@ static PairState pair_state(Session *s, uint64_t S, bool guarded) {
   st.bkf = guarded ? scratch_as<float2>(s, kSlotPlanSortedK32, Sp) : nullptr;
   st.pb = scratch_as<double4>(s, kSlotPlanPosB, Sp);
   st.pbf = guarded ? scratch_as<float4>(s, kSlotPlanPosB32, Sp) : nullptr;
+  st.piv = scratch_as<double>(s, kSlotPlanPivots, (Sp + kBucket - 1) / kBucket * (kBucket / kPivot));
   st.bord = scratch_as<uint32_t>(s, kSlotWindows, Sp);
   st.slab = scratch_as<uint32_t>(s, kSlotPlanSlab, kSlab * Sp);
   st.keys = scratch_as<uint64_t>(s, kSlotPlanKeys, 2 * Sp);
@@ -791,6 +820,7 @@ static PairArgs pair_args(const PairState &st, uint64_t S, double t, double box,
   A.bkf = st.bkf;
   A.pb = st.pb;
   A.pbf = st.pbf;
+  A.piv = st.piv;
   A.win = st.win;
   A.kw = st.kw;
   A.S = S;
@@ -861,7 +891,7 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   radix_sort_pairs_u64(s, st.keys, st.ord, st.keys + S, st.ord + S, S, key_bits);
   const uint64_t nb = (S + kBucket - 1) / kBucket;
   MVS_LAUNCH(s, "pair_index", k_bucket_index, blocks_for(nb, kWaves), kThreads, 0, st.ord + S,
-             st.r64, st.r32, S, st.bord, st.bk, st.bkf, st.pb, st.pbf);
+             st.r64, st.r32, S, st.bord, st.bk, st.bkf, st.pb, st.pbf, st.piv);
   MVS_LAUNCH(s, "pair_windows", k_windows, blocks_for(S, kThreads), kThreads, 0, st.r64,
              st.keys + S, S, t, segs.seg, st.win, st.kw);
   unsigned *long_count = flags + 1;
