@@ -554,6 +554,20 @@ def bench_precision(args):
         "pairvote_ms": {"fp64": a64["ms"], "fp32g": a32["ms"]},
         "mode_error_fp32_vs_fp64": modes,
     }
+    # the bar on converged modes: every 500th-of-n seed run to its fixed
+    # point in both arithmetics, endpoints merged by the reference's h/2
+    # rule, largest error over the modes (the GPU precision test's check)
+    conv = {}
+    for D, name in ((6, "rigid"), (4, "mirror")):
+        p64, p32 = a64["pts"][D], a32["pts"][D]
+        n = p64.shape[1]
+        if n == 0:
+            continue
+        idx = torch.arange(0, n, max(1, n // 500), device="cuda")[:500]
+        conv[name] = workload.converged_mode_errors(wl.sess, D, p64, p32, idx)
+    out["converged_modes_fp32_vs_fp64"] = conv
+    out["converged_modes_within_bar"] = all(
+        c["modes_over_1e-6_units"] == 0 and c["modes_over_1e-5_rad"] == 0 for c in conv.values())
     print(json.dumps(out), flush=True)
 
 
