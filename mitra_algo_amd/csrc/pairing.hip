@@ -44,7 +44,6 @@ constexpr int kSlab = 128;                // pairs parked per source by the walk
 constexpr uint64_t kLongChunk = 4096;     // ordinals per wave in the long path
 constexpr int kBucket = 64;               // samples per kmin bucket
 constexpr int kPivot = 8;                 // bucket entries per search segment
-constexpr int kRankSort = 24;             // partners placed by rank, not by the network
 constexpr double kEps = 2.220446049250313e-16;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
@@ -368,42 +367,65 @@ __device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, 
   }
 }
 
-// Per-wave LDS for concatenating the bucket sub-ranges.
-struct WaveRanges {
-  uint32_t start[kWave];
-  uint32_t pre[kWave + 1];
+// Two sources per wave, one per 32-lane half: a source's window spans ~17
+// buckets and ~19 tested candidates (config 2), so a whole wave per source
+// leaves most lanes idle in both loops.
+constexpr int kHalf = kWave / 2;
+
+// Per-source LDS for concatenating the bucket sub-ranges.
+struct HalfRanges {
+  uint32_t start[kHalf];
+  uint32_t pre[kHalf + 1];
 };
 
-// Walks every candidate of source i; f(accepted, ordinal) is called with all
-// lanes active once per candidate chunk.
+// Lanes of my half below me in a wave-wide ballot, and the half's count.
+__device__ inline uint32_t half_below(uint64_t mask, int lane) {
+  const uint32_t below = static_cast<uint32_t>(lanes_below(mask));
+  return lane < kHalf ? below : below - static_cast<uint32_t>(__popc(static_cast<uint32_t>(mask)));
+}
+__device__ inline uint32_t half_count(uint64_t mask, int lane) {
+  return static_cast<uint32_t>(__popc(static_cast<uint32_t>(lane < kHalf ? mask : mask >> 32)));
+}
+// The larger of the two halves' values (uniform over the wave).
+__device__ inline uint64_t halves_max(uint64_t v) {
+  const uint64_t w = static_cast<uint64_t>(__shfl_xor(static_cast<long long>(v), kHalf, kWave));
+  return v > w ? v : w;
+}
+
+// Walks every candidate of source i in this lane's half (live: the half has
+// a source); f(accepted, ordinal) is called with all lanes active once per
+// candidate chunk of both halves.
 template <bool kGuarded, class F>
-__device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W, int lane,
-                                   unsigned &rechecks, uint64_t &cand2, F &&f) {
+__device__ inline void walk_source(const PairArgs &A, uint64_t i, bool live, HalfRanges &W,
+                                   int lane, unsigned &rechecks, uint64_t &cand2, F &&f) {
   const uint64_t S = A.S;
-  const uint64_t jlo = A.win[i], jhi = A.win[S + i];
-  if (jlo >= jhi)
-    return;
-  const double2 kw = A.kw[i];
+  const int hl = lane & (kHalf - 1);
+  const uint64_t ii = live ? i : 0;  // loads stay in range for an idle half
+  const uint64_t jlo = A.win[ii], jhi = A.win[S + ii];
+  const bool has = live && jlo < jhi;
+  const double2 kw = A.kw[ii];
   Source src;
-  src.i = i;
+  src.i = ii;
   {
     if (!kGuarded) {
-      const double2 *p = reinterpret_cast<const double2 *>(A.r64 + i * kRec);
+      const double2 *p = reinterpret_cast<const double2 *>(A.r64 + ii * kRec);
       double2 p0 = p[0], p1 = p[1], p2 = p[2];
       src.ax = p0.x, src.ay = p0.y, src.az = p1.x, src.a0 = p1.y, src.a1 = p2.x;
     } else {
-      const float4 *q = reinterpret_cast<const float4 *>(A.r32 + i * kRec);
+      const float4 *q = reinterpret_cast<const float4 *>(A.r32 + ii * kRec);
       float4 q0 = q[0], q1 = q[1];
       src.fx = q0.x, src.fy = q0.y, src.fz = q0.z, src.f0 = q0.w, src.f1 = q1.x;
     }
-    source_mesh(A, i, src.dist, src.ohi);
+    source_mesh(A, ii, src.dist, src.ohi);
     src.dist32 = static_cast<float>(src.dist);
   }
-  const uint64_t b0 = jlo / kBucket, b1 = (jhi - 1) / kBucket;
-  for (uint64_t bb = b0; bb <= b1; bb += kWave) {
-    const uint64_t b = bb + lane;
+  const uint64_t b0 = jlo / kBucket;
+  const uint64_t nbk = has ? (jhi - 1) / kBucket - b0 + 1 : 0;  // buckets of the window
+  const uint64_t nbk_max = halves_max(nbk);
+  for (uint64_t off = 0; off < nbk_max; off += kHalf) {
+    const uint64_t b = b0 + off + static_cast<uint64_t>(hl);
     uint32_t u = 0, len = 0;
-    if (b <= b1) {
+    if (off + static_cast<uint64_t>(hl) < nbk) {
       const uint64_t base = b * kBucket;
       const uint64_t nb = S - base < static_cast<uint64_t>(kBucket) ? S - base : kBucket;
       // lower bound fl = entries with kmax < kw.x and upper bound fu =
@@ -447,24 +469,29 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
       u = static_cast<uint32_t>(base + fl);
       len = fu > fl ? static_cast<uint32_t>(fu - fl) : 0u;
     }
-    // exclusive prefix of the sub-range lengths across the wave
-    const uint32_t x = wave_incl_scan(len);
-    const uint32_t total = __builtin_amdgcn_readlane(x, kWave - 1);
-    W.start[lane] = u;
-    W.pre[lane] = x - len;
-    if (lane == 0)
-      W.pre[kWave] = total;
+    // exclusive prefix of the sub-range lengths within each half: one scan
+    // over the wave, the second half less the first half's total
+    const uint32_t xw = wave_incl_scan(len);
+    const uint32_t t0 = __builtin_amdgcn_readlane(xw, kHalf - 1);
+    const uint32_t t1 = __builtin_amdgcn_readlane(xw, kWave - 1) - t0;
+    const uint32_t x = lane < kHalf ? xw : xw - t0;
+    const uint32_t total = lane < kHalf ? t0 : t1;
+    W.start[hl] = u;
+    W.pre[hl] = x - len;
+    if (hl == 0)
+      W.pre[kHalf] = total;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     cand2 += total;
-    for (uint32_t c0 = 0; c0 < total; c0 += kWave) {
-      const uint32_t c = c0 + lane;
+    const uint32_t tmax = t0 > t1 ? t0 : t1;
+    for (uint32_t c0 = 0; c0 < tmax; c0 += kHalf) {
+      const uint32_t c = c0 + static_cast<uint32_t>(hl);
       bool ok = false;
       uint32_t o = 0;
       if (c < total) {
         // owner sub-range: last l with pre[l] <= c
         int l = 0;
-        for (int step = kWave / 2; step > 0; step >>= 1)
+        for (int step = kHalf / 2; step > 0; step >>= 1)
           if (W.pre[l + step] <= c)
             l += step;
         const uint64_t j = W.start[l] + (c - W.pre[l]);
@@ -479,21 +506,10 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, WaveRanges &W,
   }
 }
 
-// 64-lane bitonic sort of one uint32 per lane, ascending.
-__device__ inline uint32_t wave_sort_u32(uint32_t v, int lane) {
-  for (int k = 2; k <= kWave; k <<= 1)
-    for (int m = k >> 1; m > 0; m >>= 1) {
-      uint32_t p = __shfl_xor(v, m, kWave);
-      bool up = (lane & k) == 0, lower = (lane & m) == 0;
-      bool keep = (lower == up) ? (v <= p) : (v >= p);
-      v = keep ? v : p;
-    }
-  return v;
-}
-
 // Single walk per source: counts every accepted pair and parks up to kSlab
-// of them, sorted, in the source's slab.  quota == 0: wave w of the grid
-// walks source w.  quota > 0 (persistent): the resident waves claim `quota`
+// of them, in ordinal order, in the source's slab.  Two sources per wave
+// (one per 32-lane half).  quota == 0: wave w of the grid walks sources 2w
+// and 2w + 1.  quota > 0 (persistent): the resident waves claim `quota`
 // sources at a time from the queue counter until every source is taken, so
 // sources of uneven cost (and meshes of uneven size) balance across the
 // chip; every wave's last claim returns a start past the end, so all exit.
@@ -501,45 +517,43 @@ template <bool kGuarded>
 __global__ void __launch_bounds__(kThreads)
     k_pair_walk(PairArgs Ain, uint64_t *counts, uint32_t *slab, uint32_t *long_list,
                 unsigned *long_count, unsigned quota, unsigned *queue) {
-  __shared__ WaveRanges ranges[kWaves];
-  __shared__ uint32_t list[kWaves][kSlab];
+  __shared__ HalfRanges ranges[2 * kWaves];
+  __shared__ uint32_t list[2 * kWaves][kSlab];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int half = lane / kHalf, hl = lane & (kHalf - 1);
   const PairArgs A = with_bands<kGuarded>(Ain);
-  uint32_t *L = list[wave];
-  auto walk_one = [&](uint64_t i) {
-    uint64_t cnt = 0, cand2 = 0;
+  HalfRanges &W = ranges[2 * wave + half];
+  uint32_t *L = list[2 * wave + half];
+  // walks source i in this lane's half (live: the half has one)
+  auto walk_one = [&](uint64_t i, bool live) {
+    uint32_t cnt = 0;
+    uint64_t cand2 = 0;
     unsigned rechecks = 0;
-    walk_source<kGuarded>(A, i, ranges[wave], lane, rechecks, cand2, [&](bool ok, uint32_t o) {
-      uint64_t mask = ballot(ok);
-      uint64_t pos = cnt + static_cast<uint64_t>(lanes_below(mask));
-      if (ok && pos < static_cast<uint64_t>(kSlab))
+    walk_source<kGuarded>(A, i, live, W, lane, rechecks, cand2, [&](bool ok, uint32_t o) {
+      const uint64_t mask = ballot(ok);
+      const uint32_t pos = cnt + half_below(mask, lane);
+      if (ok && pos < static_cast<uint32_t>(kSlab))
         L[pos] = o;
-      cnt += __popcll(mask);
+      cnt += half_count(mask, lane);
     });
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    // partners in ordinal order: a lane's rank among few partners costs
-    // fewer instructions than the 21-step bitonic network over the wave
-    if (cnt > static_cast<uint64_t>(kRankSort) && cnt <= static_cast<uint64_t>(kWave)) {
-      uint32_t v = static_cast<uint64_t>(lane) < cnt ? L[lane] : kNone;
-      v = wave_sort_u32(v, lane);
-      if (static_cast<uint64_t>(lane) < cnt)
-        slab[i * kSlab + lane] = v;
-    } else if (cnt <= static_cast<uint64_t>(kSlab)) {
-      // rank = number of smaller ordinals (ordinals are distinct)
-      const uint32_t c = static_cast<uint32_t>(cnt);
-      for (uint32_t e = lane; e < c; e += kWave) {
-        uint32_t v = L[e], r = 0;
-        for (uint32_t k = 0; k < c; ++k)
+    // partners in ordinal order: each lane places its entries by rank (the
+    // number of smaller ordinals; ordinals are distinct)
+    if (live && cnt <= static_cast<uint32_t>(kSlab))
+      for (uint32_t e = hl; e < cnt; e += kHalf) {
+        const uint32_t v = L[e];
+        uint32_t r = 0;
+        for (uint32_t k = 0; k < cnt; ++k)
           r += L[k] < v;
         slab[i * kSlab + r] = v;
       }
-    }
-    rechecks = wave_sum(rechecks);
-    if (lane == 0) {
+    for (int o = kHalf / 2; o > 0; o >>= 1)  // the half's recheck count
+      rechecks += static_cast<unsigned>(__shfl_xor(static_cast<int>(rechecks), o, kWave));
+    if (live && hl == 0) {
       counts[i] = cnt;
-      if (cnt > static_cast<uint64_t>(kSlab)) {
+      if (cnt > static_cast<uint32_t>(kSlab)) {
         unsigned s = atomicAdd(long_count, 1u);
         long_list[s] = static_cast<uint32_t>(i);
       }
@@ -548,14 +562,16 @@ __global__ void __launch_bounds__(kThreads)
       A.cand2[i] = static_cast<uint32_t>(cand2 < kNone ? cand2 : kNone);
       A.rechk[i] = rechecks;
     }
-    // the next source rewrites this wave's list and ranges
+    // the next sources rewrite this wave's lists and ranges
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   };
   if (quota == 0) {
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kWaves + wave;
-    if (i < A.S)
-      walk_one(i);
+    const uint64_t i = (static_cast<uint64_t>(blockIdx.x) * kWaves + wave) * 2 + half;
+    const bool live = i < A.S;
+    if (!ballot(live))
+      return;
+    walk_one(i, live);
     return;
   }
   for (;;) {
@@ -566,8 +582,8 @@ __global__ void __launch_bounds__(kThreads)
     if (base >= A.S)
       break;
     const uint64_t end = base + quota < A.S ? base + quota : A.S;
-    for (uint64_t i = base; i < end; ++i)
-      walk_one(i);
+    for (uint64_t t = base; t < end; t += 2)
+      walk_one(t + half, t + half < end);
   }
 }
 
@@ -898,7 +914,7 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   unsigned *queue = flags + 12;  // cleared with the flags above
   PairArgs A = pair_args(st, S, t, box, guarded, flags, segs);
   const unsigned quota = s->tune.walk_quota;
-  unsigned grid = blocks_for(S, kWaves);
+  unsigned grid = blocks_for(S, 2 * kWaves);  // two sources per wave
   if (quota)  // persistent: a fixed set of resident blocks
     grid = std::min<unsigned>(grid, s->tune.walk_blocks);
   if (guarded)
