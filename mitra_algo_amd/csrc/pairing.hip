@@ -587,27 +587,32 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// Copies every slab (count <= kSlab) to its output offset.
+// Copies every slab (count <= kSlab) to its output offset, kPlaceLanes
+// lanes per source (a source parks ~8 pairs at config 2).
 // Writes stay below `cap` (the pair capacity), also when a graph replay
 // meets counts that differ from the ones it was captured with.
+constexpr int kPlaceLanes = 8;
 __global__ void __launch_bounds__(kThreads)
     k_pair_place(uint64_t S, const uint64_t *counts, const uint64_t *offsets,
                  const uint32_t *slab, const int32_t *sel, int32_t *out_a, int32_t *out_b,
                  uint64_t cap) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  const uint64_t i = t / kPlaceLanes;
+  const uint32_t sub = static_cast<uint32_t>(t % kPlaceLanes);
   if (i >= S)
     return;
   const uint64_t c = counts[i];
   if (c > static_cast<uint64_t>(kSlab))
     return;  // written by the long path
-  // every parked pair, also beyond the first 64 (slabs hold up to kSlab)
-  for (uint64_t e = lane; e < c; e += kWave) {
+  const uint64_t o0 = offsets[i];
+  const int32_t src = sel ? sel[i] : static_cast<int32_t>(i);
+  // every parked pair, also beyond the first kPlaceLanes (slabs hold up to kSlab)
+  for (uint64_t e = sub; e < c; e += kPlaceLanes) {
     const uint32_t o = slab[i * kSlab + e];
-    const uint64_t p = offsets[i] + e;
+    const uint64_t p = o0 + e;
     if (p >= cap)
       break;
-    out_a[p] = sel ? sel[i] : static_cast<int32_t>(i);
+    out_a[p] = src;
     out_b[p] = sel ? sel[o] : static_cast<int32_t>(o);
   }
 }
@@ -954,7 +959,8 @@ void pair_fill_device(Session *s, int32_t *a, int32_t *b, bool ordinals) {
   PairState st = pair_state(s, S, guarded);
   PairArgs A = pair_args(st, S, plan.threshold, plan.box, guarded, plan_flags(s), plan.segs);
   const int32_t *sel = ordinals ? nullptr : plan.selected;
-  MVS_LAUNCH(s, "pair_place", k_pair_place, blocks_for(S, kWaves), kThreads, 0, S, st.counts,
+  MVS_LAUNCH(s, "pair_place", k_pair_place, blocks_for(S, kThreads / kPlaceLanes), kThreads, 0,
+             S, st.counts,
              st.offsets, st.slab, sel, a, b, plan.P);
   if (plan.long_sources) {
     const uint64_t L = plan.long_sources, chunks = (S + kLongChunk - 1) / kLongChunk;
