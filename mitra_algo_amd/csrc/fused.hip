@@ -85,15 +85,21 @@ static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, in
   void *owr = scratch(s, kSlotOutMsRigidW, (kr ? kr : 1) * sz);
   void *ofl = scratch(s, kSlotOutMsRefl, 4 * (kf ? kf : 1) * sz);
   void *owf = scratch(s, kSlotOutMsReflW, (kf ? kf : 1) * sz);
-  ms_points_device(s, prec, kr ? nr : 0, r.rigid_q4, r.rigid_t3, kf ? nf : 0, r.refl_n3,
-                   r.refl_stride, r.refl_d, rpts, fpts);
+  if (prec != MVS_FP32_GUARDED && prec != MVS_FP64)
+    raise(MVS_EINVAL, "unknown precision");
   ms_begin(s);
-  const int slot_r = want_stats ? 0 : -1, slot_f = want_stats ? 1 : -1;
   // the rigid kind whole on the side stream (its own scratch bank), queued
   // first, beside the mirror kind on the session stream: the two set-ups run
   // together, then the two queries.  A set-up queued behind a running query
-  // waits for its blocks to drain and is slowed several times over.
-  const MsRun rigid_run = kf ? MsRun{1, true} : MsRun{}, mirror_run{};
+  // waits for its blocks to drain and is slowed several times over.  Each
+  // run writes its kind's points from the votes in the launch that takes
+  // their box and moments.
+  MsRun rigid_run = kf ? MsRun{1, true} : MsRun{};
+  rigid_run.from_a = r.rigid_q4, rigid_run.from_b = r.rigid_t3;
+  MsRun mirror_run{};
+  mirror_run.from_a = r.refl_n3, mirror_run.from_b = r.refl_d;
+  mirror_run.from_stride = r.refl_stride;
+  const int slot_r = want_stats ? 0 : -1, slot_f = want_stats ? 1 : -1;
   if (prec == MVS_FP32_GUARDED) {
     if (kr)
       ms_batched_f32(s, 6, nr, static_cast<const float *>(rpts), kr, nullptr, dseeds, salt_r,

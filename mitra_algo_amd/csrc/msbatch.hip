@@ -239,17 +239,14 @@ struct MsCtl {
 // (cells of side cell0 / reach across the rows, cell0 / xdiv along them);
 // if none fits, cells of side cell0, doubled until the grid fits.
 // No points: a single cell.  ncells[m] (optional) receives the cell count.
+// (One block: the segment's nb partials from `partial`, its n points; the
+// grid goes to *gout, the cell count to *ncells when given.)
 template <int D>
-__global__ void __launch_bounds__(kThreads)
-    k_grid_setup(const double *partial, unsigned nb, const uint64_t *seg_off, uint64_t n,
-                 double cell0, int max_reach, int xdiv, Grid *grids, uint64_t *ncells,
-                 unsigned *bad) {
+__device__ void grid_block(const double *partial, unsigned nb, uint64_t n, double cell0,
+                           int max_reach, int xdiv, Grid *gout, uint64_t *ncells,
+                           unsigned *bad) {
   __shared__ double red[kWavesPerBlock][4 * kMaxD];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const uint32_t mesh = blockIdx.x;
-  partial += static_cast<uint64_t>(mesh) * nb * kPart;
-  if (seg_off)
-    n = seg_off[mesh + 1] - seg_off[mesh];
   const uint64_t cap = n < kMaxCells / 2 ? (2 * n > 1024 ? 2 * n : 1024) : kMaxCells;
   double lo[kMaxD], hi[kMaxD], sum[kMaxD], sq[kMaxD];
   for (int k = 0; k < kMaxD; ++k)
@@ -372,9 +369,80 @@ __global__ void __launch_bounds__(kThreads)
     g.cellx = cellx;
     g.invx = 1.0 / cellx;
   }
-  grids[mesh] = g;
+  *gout = g;
   if (ncells)
-    ncells[mesh] = g.ncells;
+    *ncells = g.ncells;
+}
+template <int D>
+__global__ void __launch_bounds__(kThreads)
+    k_grid_setup(const double *partial, unsigned nb, const uint64_t *seg_off, uint64_t n,
+                 double cell0, int max_reach, int xdiv, Grid *grids, uint64_t *ncells,
+                 unsigned *bad) {
+  const uint32_t mesh = blockIdx.x;
+  if (seg_off)
+    n = seg_off[mesh + 1] - seg_off[mesh];
+  grid_block<D>(partial + static_cast<uint64_t>(mesh) * nb * kPart, nb, n, cell0, max_reach,
+                xdiv, grids + mesh, ncells ? ncells + mesh : nullptr, bad);
+}
+
+// The points of one kind and, in the same launch, their box and moments.
+// Rigid (D = 6): a = q4, b = t3 (rows of n), as k_points_rigid; mirror (D =
+// 4): a = n3 (rows of `stride`), b = d1, as k_points_refl.  Each block
+// reduces its points' box and moments as k_bbox does into partial[block]
+// (saving the box launch and its re-read of the points).  A last-block grid
+// in the same launch was slower: the device-scope fence each block needs
+// before its ticket writes the die's L2 back, full of the points just
+// written.
+template <class R, int D>
+__global__ void __launch_bounds__(kThreads)
+    k_points_moments(const R *a, const R *b, uint64_t stride, uint64_t n, R *pts,
+                     double *partial) {
+  __shared__ double sh[kWavesPerBlock][kPart];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  double v[kPart];
+  for (int k = 0; k < kMaxD; ++k)
+    v[k] = INFINITY, v[kMaxD + k] = -INFINITY, v[2 * kMaxD + k] = 0, v[3 * kMaxD + k] = 0;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * kThreads) {
+    R c[D];
+    if constexpr (D == 6) {
+      const R w = a[i], x = a[n + i], y = a[2 * n + i], z = a[3 * n + i];
+      const R sn = sqrt(x * x + y * y + z * z);
+      const R f = sn > R(1e-12) ? R(2) * atan2(sn, w) / sn : R(2) / (w > R(0.5) ? w : R(1));
+      c[0] = b[i], c[1] = b[n + i], c[2] = b[2 * n + i];
+      c[3] = f * x, c[4 % D] = f * y, c[5 % D] = f * z;
+    } else {
+      c[0] = b[i], c[1] = a[i], c[2] = a[stride + i], c[3] = a[2 * stride + i];
+    }
+    for (int k = 0; k < D; ++k) {
+      pts[k * n + i] = c[k];
+      const double x = static_cast<double>(c[k]);
+      v[k] = fmin(v[k], x);
+      v[kMaxD + k] = fmax(v[kMaxD + k], x);
+      v[2 * kMaxD + k] += x;
+      v[3 * kMaxD + k] += x * x;
+    }
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1)
+    for (int k = 0; k < kMaxD; ++k) {
+      v[k] = fmin(v[k], __shfl_xor(v[k], o, kWave));
+      v[kMaxD + k] = fmax(v[kMaxD + k], __shfl_xor(v[kMaxD + k], o, kWave));
+      v[2 * kMaxD + k] += __shfl_xor(v[2 * kMaxD + k], o, kWave);
+      v[3 * kMaxD + k] += __shfl_xor(v[3 * kMaxD + k], o, kWave);
+    }
+  if (lane == 0)
+    for (int k = 0; k < kPart; ++k)
+      sh[wave][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < kPart) {
+    const int k = threadIdx.x;
+    double r = sh[0][k];
+    for (int w = 1; w < kWavesPerBlock; ++w) {
+      const double x = sh[w][k];
+      r = k < kMaxD ? fmin(r, x) : (k < 2 * kMaxD ? fmax(r, x) : r + x);
+    }
+    partial[static_cast<uint64_t>(blockIdx.x) * kPart + k] = r;
+  }
 }
 
 // Cell index along one axis (inv = 1 / cell side); NaN and underflow go to
@@ -1340,7 +1408,15 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
       nb = blocks_for(n, kThreads);
       if (nb > kBboxBlocks)
         nb = kBboxBlocks;
-      if (D == 6)
+      R *wpts = const_cast<R *>(pts);  // the caller's point buffer, written here
+      const R *fa = static_cast<const R *>(run.from_a), *fb = static_cast<const R *>(run.from_b);
+      if (fa && D == 6)
+        MVS_LAUNCH(s, "ms_points", (k_points_moments<R, 6>), nb, kThreads, 0, fa, fb, n, n,
+                   wpts, partial);
+      else if (fa)
+        MVS_LAUNCH(s, "ms_points", (k_points_moments<R, 4>), nb, kThreads, 0, fa, fb,
+                   run.from_stride, n, wpts, partial);
+      else if (D == 6)
         MVS_LAUNCH(s, "ms_bbox", (k_bbox<R, 6>), nb, kThreads, 0, pts, n, partial);
       else
         MVS_LAUNCH(s, "ms_bbox", (k_bbox<R, 4>), nb, kThreads, 0, pts, n, partial);

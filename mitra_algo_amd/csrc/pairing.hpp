@@ -89,6 +89,11 @@ void ms_finish(Session *s, unsigned long long sums[kMsSums]);
 struct MsRun {
   int set = 0;        // slot set and scratch bank (0 or 1)
   bool side = false;  // the run on the side stream
+  // optional: the votes the points come from (rigid: q4, t3; mirror: n3 with
+  // row stride `from_stride`, d1); the run then writes the points into its
+  // `pts` argument, in the launch that takes their box and moments
+  const void *from_a = nullptr, *from_b = nullptr;
+  uint64_t from_stride = 0;
 };
 void ms_batched_f32(Session *s, int D, uint64_t n, const float *pts, uint64_t ns,
                     const float *seeds, int32_t *seed_ids, uint64_t salt, int iterations,
