@@ -90,13 +90,16 @@ static void meanshift_votes(Session *s, uint64_t nseeds, uint64_t seed_value, in
   ms_begin(s);
   // the rigid kind whole on the side stream (its own scratch bank), queued
   // first, beside the mirror kind on the session stream: the two set-ups run
-  // together, then the two queries.  A set-up queued behind a running query
-  // waits for its blocks to drain and is slowed several times over.  Each
-  // run writes its kind's points from the votes in the launch that takes
-  // their box and moments.
+  // together, then the two queries, each held until both set-ups are queued
+  // and started behind them (ms_release).  A set-up queued behind a running
+  // query waits for its blocks to drain and is slowed several times over.
+  // Each run writes its kind's points from the votes in the launch that
+  // takes their box and moments.
   MsRun rigid_run = kf ? MsRun{1, true} : MsRun{};
+  rigid_run.hold = true;
   rigid_run.from_a = r.rigid_q4, rigid_run.from_b = r.rigid_t3;
   MsRun mirror_run{};
+  mirror_run.hold = true;
   mirror_run.from_a = r.refl_n3, mirror_run.from_b = r.refl_d;
   mirror_run.from_stride = r.refl_stride;
   const int slot_r = want_stats ? 0 : -1, slot_f = want_stats ? 1 : -1;

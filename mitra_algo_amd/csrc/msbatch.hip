@@ -15,8 +15,8 @@
 // horizontal normal components for mirror votes), so a cell holds few points
 // outside the support; reach is 2 (half-width cells) when that grid fits in
 // 2^22 cells, else 1.  Points are radix-sorted by cell (stable: sums are
-// deterministic) and indexed by per-cell offsets found by binary search (no
-// atomics).  With the first grid axis fastest, the 2 reach + 1 cells of one
+// deterministic) and indexed by per-cell offsets, each sorted point writing
+// the run of cells that starts at it (no atomics).  With the first grid axis fastest, the 2 reach + 1 cells of one
 // row are contiguous, so a query visits (2 reach + 1)^2 contiguous point
 // ranges (9 or 25).  Queries run all iterations in registers and reduce
 // with shuffles.  Seeds run in cell order and blocks are mapped so that each
@@ -445,6 +445,12 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+__device__ inline uint64_t lane_u64(uint64_t v, int l) {
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(v), l);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(v >> 32), l);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 // Cell index along one axis (inv = 1 / cell side); NaN and underflow go to
 // cell 0.  Keys and queries use this one formula, so they agree on every
 // point's cell; cells are monotone in v.
@@ -565,21 +571,39 @@ __global__ void k_cell_keys(const R *pts, uint64_t n, const Grid *grids, const u
 
 // offsets[c] = first sorted position with key >= c, for c in [0, ncells]
 // (grid-stride: the cell count lives on the device, at *total).
-__global__ void k_cell_offsets(const uint64_t *skeys, uint64_t n, const uint64_t *total,
-                               uint64_t *offsets) {
+// offsets[c] = the first sorted record whose key is >= c, c in [0, ncells]:
+// record i (0 <= i <= n) starts the cells (key[i - 1], key[i]] (key[-1] = -1,
+// key[n] = ncells; keys past ncells, the segments' sentinels, clamp), so each
+// record writes its run of cells.  Runs of up to kShortRun cells are written
+// by their lane, longer runs (empty stretches of the grid) by the whole wave.
+constexpr uint64_t kShortRun = 8;
+__global__ void __launch_bounds__(kThreads)
+    k_cell_offsets(const uint64_t *skeys, uint64_t n, const uint64_t *total,
+                   uint64_t *offsets) {
   const uint64_t ncells = *total;
-  for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c <= ncells;
-       c += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    uint64_t f = 0, cnt = n;
-    while (cnt) {
-      uint64_t h = cnt / 2, m = f + h;
-      if (skeys[m] < c) {
-        f = m + 1;
-        cnt -= h + 1;
-      } else
-        cnt = h;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + (threadIdx.x - lane);
+       base <= n; base += stride) {
+    const uint64_t i = base + static_cast<uint64_t>(lane);
+    uint64_t lo = 0, hi = 0;  // the run: cells [lo, hi)
+    if (i <= n) {
+      lo = i == 0 ? 0 : min(skeys[i - 1] + 1, ncells + 1);
+      hi = i == n ? ncells + 1 : min(skeys[i] + 1, ncells + 1);
+      hi = hi > lo ? hi : lo;
     }
-    offsets[c] = f;
+    const bool shortrun = hi - lo <= kShortRun;
+    if (shortrun)
+      for (uint64_t c = lo; c < hi; ++c)
+        offsets[c] = i;
+    uint64_t longer = ballot(!shortrun);
+    while (longer) {
+      const int l = __builtin_ctzll(longer);
+      longer &= longer - 1;
+      const uint64_t rlo = lane_u64(lo, l), rhi = lane_u64(hi, l);
+      for (uint64_t c = rlo + static_cast<uint64_t>(lane); c < rhi; c += kWave)
+        offsets[c] = base + static_cast<uint64_t>(l);
+    }
   }
 }
 
@@ -1064,19 +1088,30 @@ template <int G> __device__ inline uint32_t group_count(uint32_t v) {
 // Mean-shift queries with G lanes per seed (64 / G seeds per wave, 256 / G
 // per block), for many seeds with few candidates each (one-wave seeds would
 // spend most of their issue slots on the per-iteration work: the mean's cell,
-// the row ranges, the reductions, the update).  Grids of reach <= 2: at most
-// 25 rows, held two (G = 16) or one (G = 32) per lane.  Each lane of a group
-// holds the seed's mean; a row's records are walked in batches of G
-// consecutive records, the batches of all rows numbered in row order with a
-// wave-wide count per row (the most any of the wave's seeds needs), so the
-// loop is uniform and the next batch's records are in flight while the
-// current one is tested.  Lanes past their row's end read the sentinel
-// record.  Group sums come from group_total, so every lane of a group holds
-// the same new mean; a seed whose update leaves it unchanged stops (its rows
-// are then empty), and the wave leaves once all its seeds have.
+// the row ranges, the reductions, the update).  Grids of reach 1: 9 rows, one
+// per lane of the group's first 16.  The rows' records are numbered one after
+// the other (row order) and written, as byte offsets, into the group's map in
+// LDS; the group then walks the map in batches of G entries, so a batch spans
+// rows and no lane idles at a short row's end.  Rows are kept while the mean
+// stays in the (y, z) cell they were found for and within kReuse * cutoff * h
+// of the mean they were cut around (the anchor): they are cut to the support
+// ball widened by that distance, which holds every point of the support of
+// any mean in that range, and the map is kept too.  Loops are wave-uniform
+// (the most any group of the wave needs); lanes past their group's end read
+// the sentinel record.  Group sums come from group_total, so every lane of a
+// group holds the same new mean; a seed whose update leaves it unchanged
+// stops, and the wave leaves once all its seeds have.
 constexpr int kSmallRows = (2 * kNarrowReach + 1) * (2 * kNarrowReach + 1);
+constexpr uint32_t kSmallMapBlock = 4096;  // LDS map entries per block (16 KiB)
+constexpr double kReuse = 0.1;
+// Held queries (ms_release): bit i set makes held run i's query wait for the
+// set-ups of the other held runs.  The fused step holds the rigid run (side
+// stream) first: its query waits for the mirror set-up, whose query, the
+// longer one, starts as soon as that set-up is done (measured against
+// neither, the other and both waiting).
+constexpr unsigned kMsGate = 1;
 template <class R, int D, int G>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads, sizeof(R) == 4 ? 5 : 1)  // FP32: 96 VGPRs
     k_ms_query_small(const typename Vec<R>::V4 *p4, const typename Vec<R>::V2 *q2,
                      const uint64_t *offsets, const Grid *grids, const uint64_t *cellbase,
                      const uint32_t *seed_mesh, const uint64_t *ns_dev,
@@ -1085,22 +1120,23 @@ __global__ void __launch_bounds__(kThreads)
                      R gscale, R r2, R floor_w, R *out, R *weight,
                      unsigned long long *stats) {
   static_assert(G == 16 || G == 32, "16 or 32 lanes per seed");
-  constexpr int kSlots = (kSmallRows + G - 1) / G;  // rows per lane
-#ifndef MVS_MS_DEPTH32
-#define MVS_MS_DEPTH32 2
-#define MVS_MS_DEPTH64 2
-#endif
-  constexpr int kDepth = sizeof(R) == 4 ? MVS_MS_DEPTH32 : MVS_MS_DEPTH64;  // batches in flight
-  constexpr int kSeedsPerBlock = kThreads / G;
+  static_assert(kSmallRows <= 16, "the rows fit the first DPP row of a group");
+  constexpr int kGroups = kThreads / G;
+  constexpr uint32_t kCap = kSmallMapBlock / kGroups;  // map entries per group
+  constexpr uint32_t kRec = sizeof(typename Vec<R>::V4);
+  __shared__ uint32_t smap[kSmallMapBlock];
+  __shared__ uint4 srows[kGroups * kSmallRows];  // (first entry, first byte, length)
   const int lane = threadIdx.x & (kWave - 1);
   const int sl = lane & (G - 1);   // lane in the group
-  const int gbase = lane & ~(G - 1);  // first lane of the group
+  const int grp = static_cast<int>(threadIdx.x) / G;
   const int wave = threadIdx.x / kWave;
+  uint32_t *map = smap + grp * kCap;
+  uint4 *rows = srows + grp * kSmallRows;
   const unsigned per = (logical_blocks + kXcds - 1) / kXcds;
   const unsigned logical = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
   if (logical >= logical_blocks)
     return;
-  const uint64_t slot = static_cast<uint64_t>(logical) * kSeedsPerBlock +
+  const uint64_t slot = static_cast<uint64_t>(logical) * kGroups +
                         static_cast<uint64_t>(wave) * (kWave / G) + lane / G;
   const uint64_t nseeds = ns_dev ? *ns_dev : ns;
   bool active = slot < nseeds;
@@ -1131,34 +1167,63 @@ __global__ void __launch_bounds__(kThreads)
   unsigned long long cand = 0;
   uint32_t inside = 0;
   int done = 0;
-  const RowCut rc = row_cut(g, static_cast<double>(r2));
+  // rows cut to the support ball widened by the reuse distance (and a
+  // relative margin over the FP64 test of that distance below)
+  const double reuse = kReuse * sqrt(static_cast<double>(r2));
+  RowCut rc = row_cut(g, static_cast<double>(r2));
+  {
+    const double rl = sqrt(rc.rlim2) + reuse * (1.0 + 1e-6);
+    rc.rlim2 = rl * rl;
+  }
+  const double reuse2 = reuse * reuse;
+  const uint32_t sent_b = sentinel * kRec;
+  R anc[3] = {R(0), R(0), R(0)};  // the anchor's grid coordinates
+  int acy = -1, acz = -1;
+  uint32_t rb = 0, len = 0;       // lanes 0..8 of a group: its row's first record, length
+  uint32_t gtot = 0, chunks = 0;  // the group's entries; map chunks (wave-uniform)
+  bool mapped = false;            // the map holds the rows' single chunk
   for (int it = 0; it < iterations; ++it) {
-    // this lane's rows sl, sl + G (cut_row: each cut to the support ball)
-    uint32_t rb[kSlots], len[kSlots];
-    const int cy = cell_of(static_cast<double>(m[1]), g.lo[1], g.inv, g.dims[1]);
-    const int cz = cell_of(static_cast<double>(m[2]), g.lo[2], g.inv, g.dims[2]);
-    uint32_t mine = 0;
-    for (int q = 0; q < kSlots; ++q) {
-      rb[q] = 0, len[q] = 0;
-      if (active)
-        cut_row(g, rc, cb, offsets, static_cast<double>(m[0]), static_cast<double>(m[1]),
-                static_cast<double>(m[2]), cy, cz, q * G + sl, rb[q], len[q]);
-      mine += len[q];
+    const double mx = static_cast<double>(m[0]), my = static_cast<double>(m[1]),
+                 mz = static_cast<double>(m[2]);
+    const int cy = cell_of(my, g.lo[1], g.inv, g.dims[1]);
+    const int cz = cell_of(mz, g.lo[2], g.inv, g.dims[2]);
+    const double ex = mx - static_cast<double>(anc[0]), ey = my - static_cast<double>(anc[1]),
+                 ez = mz - static_cast<double>(anc[2]);
+    const bool need = active && (cy != acy || cz != acz || ex * ex + ey * ey + ez * ez > reuse2);
+    if (ballot(need)) {
+      // the groups that need it cut their rows around their means (cut_row:
+      // lanes 0..8 of the group, one row each; a seed's rows depend on its
+      // own path only, so its sums do not depend on the other seeds of the
+      // wave), then every group numbers its rows' records by a prefix over
+      // its first DPP row and lists them in LDS
+      if (need) {
+        anc[0] = m[0], anc[1] = m[1], anc[2] = m[2];
+        acy = cy, acz = cz;
+        rb = 0, len = 0;
+        if (sl < kSmallRows)
+          cut_row(g, rc, cb, offsets, mx, my, mz, cy, cz, sl, rb, len);
+      }
+      if (!active)
+        len = 0;
+      uint32_t incl = len;
+      incl += dpp_u32<0x111>(0u, incl);  // row_shr 1, 2, 4, 8
+      incl += dpp_u32<0x112>(0u, incl);
+      incl += dpp_u32<0x114>(0u, incl);
+      incl += dpp_u32<0x118>(0u, incl);
+      if (sl < kSmallRows)
+        rows[sl] = make_uint4(incl - len, rb * kRec, len, 0u);
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      const uint4 last = rows[kSmallRows - 1];
+      gtot = last.x + last.z;
+      uint32_t mc = 0;
+      for (int q = 0; q < kWave / G; ++q)
+        mc = max(mc, static_cast<uint32_t>(__builtin_amdgcn_readlane(gtot, q * G)));
+      chunks = (mc + kCap - 1) / kCap;
+      mapped = false;
     }
-    // batches per row: the most any group of the wave needs; lane l < 32
-    // holds row l's count (slot l / G of lane l), their inclusive prefix
-    // numbers the batches in row order
-    uint32_t nbl = 0;
-    for (int q = 0; q < kSlots; ++q) {
-      uint32_t nb = (len[q] + G - 1) / G;
-      nb = max(nb, static_cast<uint32_t>(__shfl_xor(static_cast<int>(nb), 32, kWave)));
-      if (G == 16)
-        nb = max(nb, static_cast<uint32_t>(__shfl_xor(static_cast<int>(nb), 16, kWave)));
-      nbl = (lane & 31) / G == q ? nb : nbl;
-    }
-    nbl = lane < 32 ? nbl : 0u;
-    const uint32_t bincl = wave_incl_scan(nbl);
-    const uint32_t totb = __builtin_amdgcn_readlane(bincl, 31);
+    const uint32_t mine = active ? gtot : 0u;
+    cand += mine;
     R acc[D];
     for (int k = 0; k < D; ++k)
       acc[k] = 0;
@@ -1184,57 +1249,62 @@ __global__ void __launch_bounds__(kThreads)
         ++inside;
       }
     };
-    // scalar row cursor over the batch numbering; the group's row data by
-    // lane permutes when the row changes
-    int r = -1;
-    uint32_t b_cur = 0, b_next = 0, row_b = 0, row_len = 0;
-    auto rec_of = [&](uint32_t t) -> uint32_t {
-      if (t >= totb)
-        return sentinel;
-      while (t >= b_next) {
-        ++r;
-        b_cur = b_next;
-        b_next = __builtin_amdgcn_readlane(bincl, r);
-        const int src = (gbase | (r % G)) << 2;
-        const int q = r / G;
-        uint32_t vb = rb[0], vl = len[0];
-        for (int qq = 1; qq < kSlots; ++qq)
-          if (q == qq)
-            vb = rb[qq], vl = len[qq];
-        row_b = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src, static_cast<int>(vb)));
-        row_len = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src, static_cast<int>(vl)));
-      }
-      const uint32_t off = (t - b_cur) * G + sl;
-      return off < row_len ? row_b + off : sentinel;
-    };
-    auto load = [&](uint32_t j, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
-      p = p4[j];
+    auto load = [&](uint32_t b, typename Vec<R>::V4 &p, typename Vec<R>::V2 &q) {
+      p = *reinterpret_cast<const typename Vec<R>::V4 *>(reinterpret_cast<const char *>(p4) + b);
       if (D == 6)
-        q = q2[j];
+        q = *reinterpret_cast<const typename Vec<R>::V2 *>(reinterpret_cast<const char *>(q2) +
+                                                           (b >> 1));
     };
-    if (totb) {
-      // kDepth batches in flight: the records of batches t + 1 .. t + kDepth
-      // - 1 are loading while batch t is tested (rows are short, so a batch
-      // rarely has the arithmetic to cover one load's latency); batches past
-      // totb read the sentinel record
-      typename Vec<R>::V4 p[kDepth];
-      typename Vec<R>::V2 q[kDepth];
-      for (int j = 0; j < kDepth; ++j) {
-        q[j] = {};
-        load(rec_of(j), p[j], q[j]);
+    for (uint32_t ch = 0; ch < chunks; ++ch) {
+      const uint32_t c0 = ch * kCap;
+      // this group's entries in the chunk (none once its seed has stopped)
+      const uint32_t glen = mine > c0 ? min(mine - c0, kCap) : 0u;
+      if (!mapped) {
+        // entry f of row r (first entry x, first byte y, length z) is the
+        // record at byte y + (f - x) * kRec
+#pragma unroll 1
+        for (int r = 0; r < kSmallRows; ++r) {
+          const uint4 t = rows[r];
+          const uint32_t lo = max(t.x, c0), hi = min(t.x + t.z, c0 + kCap);
+          for (uint32_t f = lo + sl; f < hi; f += G)
+            map[f - c0] = t.y + (f - t.x) * kRec;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       }
-      for (uint32_t t = 0; t < totb; t += kDepth) {
-#pragma unroll
-        for (int j = 0; j < kDepth; ++j) {
-          const uint32_t nxt = rec_of(t + kDepth + j);
+      uint32_t nb = 0;
+      for (int q = 0; q < kWave / G; ++q)
+        nb = max(nb, static_cast<uint32_t>(__builtin_amdgcn_readlane(glen, q * G)));
+      nb = (nb + G - 1) / G;
+      // three stages per batch: map entry (LDS), record (global), test; the
+      // map read runs two batches ahead and the record load one batch ahead
+      auto entry = [&](uint32_t t) -> uint32_t {
+        const uint32_t i = t * G + static_cast<uint32_t>(sl);
+        const uint32_t v = map[i < glen ? i : 0u];
+        return i < glen ? v : sent_b;
+      };
+      if (nb) {
+        typename Vec<R>::V4 pa, pb;
+        typename Vec<R>::V2 qa{}, qb{};
+        uint32_t ja = entry(0), jb = entry(1);
+        load(ja, pa, qa);
+        for (uint32_t t = 0; t < nb; t += 2) {
+          load(jb, pb, qb);
+          ja = entry(t + 2);
           __builtin_amdgcn_sched_barrier(0);
-          test(p[j], q[j]);
-          load(nxt, p[j], q[j]);
+          test(pa, qa);
+          load(ja, pa, qa);
+          jb = entry(t + 3);
+          __builtin_amdgcn_sched_barrier(0);
+          test(pb, qb);
         }
       }
+      if (chunks > 1) {  // the next chunk rewrites the map
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      }
     }
-    mine = group_count<G>(mine);
-    cand += active ? mine : 0u;
+    mapped = chunks == 1;
     wl = group_total<G>(wl);
     for (int k = 0; k < D; ++k)
       acc[k] = group_total<G>(acc[k]);
@@ -1292,11 +1362,48 @@ static MsCtl *ms_ctl(Session *s) {
   return reinterpret_cast<MsCtl *>(base + part);
 }
 
-void ms_begin(Session *s) { zero_device(s, ms_ctl(s), sizeof(MsCtl)); }
+void ms_begin(Session *s) {
+  s->ms_held.clear();
+  zero_device(s, ms_ctl(s), sizeof(MsCtl));
+}
+
+void ms_release(Session *s) {
+  std::vector<Session::MsHeld> held;
+  held.swap(s->ms_held);
+  const size_t k = held.size();
+  if (k > 1) {
+    for (size_t i = 0; i < k; ++i) {
+      if (!s->setup_events[i])
+        MVS_HIP(hipEventCreateWithFlags(&s->setup_events[i], hipEventDisableTiming));
+      MVS_HIP(hipEventRecord(s->setup_events[i], held[i].stream));
+    }
+    for (size_t i = 0; i < k; ++i)
+      for (size_t j = 0; j < k; ++j)
+        if (j != i && held[j].stream != held[i].stream && (kMsGate >> i & 1))
+          MVS_HIP(hipStreamWaitEvent(held[i].stream, s->setup_events[j], 0));
+  }
+  const hipStream_t main = s->stream;
+  const int bank = s->bank;
+  struct Restore {
+    Session *s;
+    hipStream_t main;
+    int bank;
+    ~Restore() {
+      s->stream = main;
+      s->bank = bank;
+    }
+  } restore{s, main, bank};
+  for (auto &q : held) {
+    s->stream = q.stream;
+    s->bank = q.bank;
+    q.launch();
+  }
+}
 
 unsigned *ms_bad_flag(Session *s) { return &ms_ctl(s)->bad; }
 
 void ms_finish(Session *s, unsigned long long sums[kMsSums]) {
+  ms_release(s);
   if (s->side_pending) {  // the side stream's query and sums come first
     MVS_HIP(hipStreamWaitEvent(s->stream, s->join_event, 0));
     s->side_pending = false;
@@ -1467,8 +1574,9 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
                  pts, n, grids, seg_off, meshes, cellbase, cells_cap, keys, vals, aos);
     radix_sort_pairs_u64(s, keys, vals, keys + n, vals + n, n, key_bits);
     sorted_ids = vals + n;
-    MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 2048, kThreads, 0, keys + n, n, total_cells,
-               off);
+    MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets,
+               std::min<unsigned>(blocks_for(n + 1, kThreads), 16384u), kThreads, 0, keys + n, n,
+               total_cells, off);
     if (D == 6)
       MVS_LAUNCH(s, "ms_gather", (k_gather_sorted<R, 6>), blocks_for(n, kThreads), kThreads, 0,
                  aos, n, grids, seg_off, meshes, sorted_ids, p4, q2);
@@ -1509,6 +1617,9 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   const unsigned grid = ((logical + kXcds - 1) / kXcds) * kXcds;
   const uint32_t *seed_mesh = seg ? seg->seed_mesh : nullptr;
   const uint64_t *ns_dev = seg ? seg->seed_off + meshes : nullptr;
+  const bool side = run.side;
+  // the query, its visit sums, and the side run's join (on s->stream)
+  auto query = [=]() {
 #define MVS_MS_QUERY(DD, WW)                                                                   \
   MVS_LAUNCH(s, "ms_query", (k_ms_query<R, DD, WW>), grid, kThreads, 0, p4, q2, off, grids,   \
              cellbase, seed_mesh, ns_dev, seed_pos, seeds, order, ns, logical,                \
@@ -1517,36 +1628,44 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   MVS_LAUNCH(s, "ms_query", (k_ms_query_small<R, DD, GG>), grid, kThreads, 0, p4, q2, off,     \
              grids, cellbase, seed_mesh, ns_dev, seed_pos, seeds, order, ns, logical,         \
              static_cast<uint32_t>(n), iterations, gscale, r2, fw, out, weight, stats)
-  if (wide) {
-    if (D == 6)
-      MVS_MS_QUERY(6, kWavesPerBlock);
-    else
-      MVS_MS_QUERY(4, kWavesPerBlock);
-  } else if (lanes == 64) {
-    if (D == 6)
-      MVS_MS_QUERY(6, 1);
-    else
-      MVS_MS_QUERY(4, 1);
-  } else if (lanes == 32) {
-    if (D == 6)
-      MVS_MS_SMALL(6, 32);
-    else
-      MVS_MS_SMALL(4, 32);
-  } else {
-    if (D == 6)
-      MVS_MS_SMALL(6, 16);
-    else
-      MVS_MS_SMALL(4, 16);
-  }
+    if (wide) {
+      if (D == 6)
+        MVS_MS_QUERY(6, kWavesPerBlock);
+      else
+        MVS_MS_QUERY(4, kWavesPerBlock);
+    } else if (lanes == 64) {
+      if (D == 6)
+        MVS_MS_QUERY(6, 1);
+      else
+        MVS_MS_QUERY(4, 1);
+    } else if (lanes == 32) {
+      if (D == 6)
+        MVS_MS_SMALL(6, 32);
+      else
+        MVS_MS_SMALL(4, 32);
+    } else {
+      if (D == 6)
+        MVS_MS_SMALL(6, 16);
+      else
+        MVS_MS_SMALL(4, 16);
+    }
 #undef MVS_MS_SMALL
 #undef MVS_MS_QUERY
-  if (run_slot >= 0)
-    for (int k = 0; k < 3; ++k)
-      MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats + k * ns, ns,
-                 ctl->sums + 3 * run_slot + k);
-  if (run.side) {
-    MVS_HIP(hipEventRecord(s->join_event, s->side_stream));
-    s->side_pending = true;
+    if (run_slot >= 0)
+      for (int k = 0; k < 3; ++k)
+        MVS_LAUNCH(s, "ms_stats", k_sum_u64, 1, kThreads, 0, stats + k * ns, ns,
+                   ctl->sums + 3 * run_slot + k);
+    if (side) {
+      MVS_HIP(hipEventRecord(s->join_event, s->side_stream));
+      s->side_pending = true;
+    }
+  };
+  if (run.hold) {
+    if (s->ms_held.size() >= 2)
+      raise(MVS_EINVAL, "at most two held mean-shift runs");
+    s->ms_held.push_back(Session::MsHeld{s->stream, s->bank, query});
+  } else {
+    query();
   }
 }
 

@@ -77,6 +77,11 @@ struct MsSegs {
   uint64_t cells_bound = 0;            // bound on the total grid cells
 };
 void ms_begin(Session *s);
+// Launches the held queries (MsRun::hold) of the runs queued since
+// ms_begin, each on its run's stream behind the set-ups of the other runs
+// (a query that occupies the chip would slow a set-up queued behind it
+// several times over); ms_finish calls it too.
+void ms_release(Session *s);
 void ms_finish(Session *s, unsigned long long sums[kMsSums]);
 // Seeds: explicit positions (D x ns SoA), or, with seed_ids, ns of the n
 // points drawn stratified over the cell-sorted order with `salt` (indices
@@ -89,6 +94,7 @@ void ms_finish(Session *s, unsigned long long sums[kMsSums]);
 struct MsRun {
   int set = 0;        // slot set and scratch bank (0 or 1)
   bool side = false;  // the run on the side stream
+  bool hold = false;  // the query waits for ms_release
   // optional: the votes the points come from (rigid: q4, t3; mirror: n3 with
   // row stride `from_stride`, d1); the run then writes the points into its
   // `pts` argument, in the launch that takes their box and moments

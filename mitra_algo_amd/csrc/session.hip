@@ -176,7 +176,7 @@ Session::~Session() {
     (void)hipStreamSynchronize(side_stream);
     (void)hipStreamDestroy(side_stream);
   }
-  for (auto e : {fork_event, join_event})
+  for (auto e : {fork_event, join_event, setup_events[0], setup_events[1]})
     if (e)
       (void)hipEventDestroy(e);
   if (own_stream)

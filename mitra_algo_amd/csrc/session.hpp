@@ -1,5 +1,6 @@
 // Session state shared by the stage translation units.
 #pragma once
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -76,6 +77,15 @@ struct Session {
   hipStream_t side_stream = nullptr;
   hipEvent_t fork_event = nullptr, join_event = nullptr;
   bool side_pending = false;
+  // mean-shift queries held back until every run of the call has queued its
+  // set-up (ms_release launches them, each behind the other runs' set-ups)
+  struct MsHeld {
+    hipStream_t stream;
+    int bank;
+    std::function<void()> launch;
+  };
+  std::vector<MsHeld> ms_held;
+  hipEvent_t setup_events[2] = {nullptr, nullptr};
   // scratch bank of the sort and scan primitives (1: a side-stream run's own
   // slots, so it can overlap bank-0 work)
   int bank = 0;
