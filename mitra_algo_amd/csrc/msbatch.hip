@@ -1104,14 +1104,10 @@ template <int G> __device__ inline uint32_t group_count(uint32_t v) {
 constexpr int kSmallRows = (2 * kNarrowReach + 1) * (2 * kNarrowReach + 1);
 constexpr uint32_t kSmallMapBlock = 4096;  // LDS map entries per block (16 KiB)
 constexpr double kReuse = 0.1;
-// Held queries (ms_release): bit i set makes held run i's query wait for the
-// set-ups of the other held runs.  The fused step holds the rigid run (side
-// stream) first: its query waits for the mirror set-up, whose query, the
-// longer one, starts as soon as that set-up is done (measured against
-// neither, the other and both waiting).
-constexpr unsigned kMsGate = 1;
-template <class R, int D, int G>
-__global__ void __launch_bounds__(kThreads, sizeof(R) == 4 ? 5 : 1)  // FP32: 96 VGPRs
+// kSeg: seeds of several segments (meshes) with a grid each; otherwise every
+// seed shares grid 0, held in scalar registers.
+template <class R, int D, int G, bool kSeg>
+__global__ void __launch_bounds__(kThreads, sizeof(R) == 4 ? 5 : 1)  // FP32: <= 96 VGPRs
     k_ms_query_small(const typename Vec<R>::V4 *p4, const typename Vec<R>::V2 *q2,
                      const uint64_t *offsets, const Grid *grids, const uint64_t *cellbase,
                      const uint32_t *seed_mesh, const uint64_t *ns_dev,
@@ -1143,7 +1139,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(R) == 4 ? 5 : 1)  // FP32: 96
   if (!ballot(active))
     return;  // the whole wave
   const uint64_t s = active ? (order ? order[slot] : slot) : 0;
-  const uint32_t gm = active && seed_mesh ? seed_mesh[s] : 0;
+  const uint32_t gm = kSeg && active ? seed_mesh[s] : 0;
   const Grid g = grids[gm];
   const uint64_t cb = cellbase ? cellbase[gm] : 0;
   R m[D];
@@ -1362,9 +1358,19 @@ static MsCtl *ms_ctl(Session *s) {
   return reinterpret_cast<MsCtl *>(base + part);
 }
 
-void ms_begin(Session *s) {
+void ms_begin(Session *s, bool fork) {
   s->ms_held.clear();
   zero_device(s, ms_ctl(s), sizeof(MsCtl));
+  if (!fork)
+    return;
+  // the fork point of a side run: after the call's inputs and the cleared
+  // control block, before any run's work on the session stream
+  if (!s->side_stream) {
+    MVS_HIP(hipStreamCreateWithFlags(&s->side_stream, hipStreamNonBlocking));
+    MVS_HIP(hipEventCreateWithFlags(&s->fork_event, hipEventDisableTiming));
+    MVS_HIP(hipEventCreateWithFlags(&s->join_event, hipEventDisableTiming));
+  }
+  MVS_HIP(hipEventRecord(s->fork_event, s->stream));
 }
 
 void ms_release(Session *s) {
@@ -1379,7 +1385,7 @@ void ms_release(Session *s) {
     }
     for (size_t i = 0; i < k; ++i)
       for (size_t j = 0; j < k; ++j)
-        if (j != i && held[j].stream != held[i].stream && (kMsGate >> i & 1))
+        if (j != i && held[j].stream != held[i].stream && held[i].gate)
           MVS_HIP(hipStreamWaitEvent(held[i].stream, s->setup_events[j], 0));
   }
   const hipStream_t main = s->stream;
@@ -1459,13 +1465,9 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   // stream does next; ms_finish joins it
   const hipStream_t main = s->stream;
   const int main_bank = s->bank;
-  if (run.side) {
-    if (!s->side_stream) {
-      MVS_HIP(hipStreamCreateWithFlags(&s->side_stream, hipStreamNonBlocking));
-      MVS_HIP(hipEventCreateWithFlags(&s->fork_event, hipEventDisableTiming));
-      MVS_HIP(hipEventCreateWithFlags(&s->join_event, hipEventDisableTiming));
-    }
-    MVS_HIP(hipEventRecord(s->fork_event, main));
+  if (run.side) {  // from the fork point recorded by ms_begin
+    if (!s->side_stream)
+      raise(MVS_EINVAL, "a side mean-shift run needs ms_begin(s, true)");
     MVS_HIP(hipStreamWaitEvent(s->side_stream, s->fork_event, 0));
     s->stream = s->side_stream;
   }
@@ -1624,10 +1626,17 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   MVS_LAUNCH(s, "ms_query", (k_ms_query<R, DD, WW>), grid, kThreads, 0, p4, q2, off, grids,   \
              cellbase, seed_mesh, ns_dev, seed_pos, seeds, order, ns, logical,                \
              static_cast<uint32_t>(n), iterations, gscale, r2, fw, out, weight, stats)
-#define MVS_MS_SMALL(DD, GG)                                                                   \
-  MVS_LAUNCH(s, "ms_query", (k_ms_query_small<R, DD, GG>), grid, kThreads, 0, p4, q2, off,     \
-             grids, cellbase, seed_mesh, ns_dev, seed_pos, seeds, order, ns, logical,         \
+#define MVS_MS_SMALL_SEG(DD, GG, SEG)                                                          \
+  MVS_LAUNCH(s, "ms_query", (k_ms_query_small<R, DD, GG, SEG>), grid, kThreads, 0, p4, q2,     \
+             off, grids, cellbase, seed_mesh, ns_dev, seed_pos, seeds, order, ns, logical,    \
              static_cast<uint32_t>(n), iterations, gscale, r2, fw, out, weight, stats)
+#define MVS_MS_SMALL(DD, GG)                                                                   \
+  do {                                                                                         \
+    if (seed_mesh)                                                                             \
+      MVS_MS_SMALL_SEG(DD, GG, true);                                                          \
+    else                                                                                       \
+      MVS_MS_SMALL_SEG(DD, GG, false);                                                         \
+  } while (0)
     if (wide) {
       if (D == 6)
         MVS_MS_QUERY(6, kWavesPerBlock);
@@ -1650,6 +1659,7 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
         MVS_MS_SMALL(4, 16);
     }
 #undef MVS_MS_SMALL
+#undef MVS_MS_SMALL_SEG
 #undef MVS_MS_QUERY
     if (run_slot >= 0)
       for (int k = 0; k < 3; ++k)
@@ -1663,7 +1673,7 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   if (run.hold) {
     if (s->ms_held.size() >= 2)
       raise(MVS_EINVAL, "at most two held mean-shift runs");
-    s->ms_held.push_back(Session::MsHeld{s->stream, s->bank, query});
+    s->ms_held.push_back(Session::MsHeld{s->stream, s->bank, run.gate, query});
   } else {
     query();
   }

@@ -76,7 +76,8 @@ struct MsSegs {
   uint32_t *seed_mesh = nullptr;       // device, seed capacity: mesh of each seed
   uint64_t cells_bound = 0;            // bound on the total grid cells
 };
-void ms_begin(Session *s);
+// fork: record the side stream's fork point (needed by a side run).
+void ms_begin(Session *s, bool fork = false);
 // Launches the held queries (MsRun::hold) of the runs queued since
 // ms_begin, each on its run's stream behind the set-ups of the other runs
 // (a query that occupies the chip would slow a set-up queued behind it
@@ -87,14 +88,15 @@ void ms_finish(Session *s, unsigned long long sums[kMsSums]);
 // points drawn stratified over the cell-sorted order with `salt` (indices
 // written to seed_ids, in cell order).
 // Two runs of one call may overlap: one with set = 1 and side = true runs
-// whole (set-up and query) on the session's side stream, forked when it is
-// queued, in its own slot set and scratch bank; the other runs on the
-// session stream with set 0; ms_finish joins the side stream before reading
-// anything back.
+// whole (set-up and query) on the session's side stream, forked at
+// ms_begin(s, true), in its own slot set and scratch bank; the other runs on
+// the session stream with set 0; ms_finish joins the side stream before
+// reading anything back.
 struct MsRun {
   int set = 0;        // slot set and scratch bank (0 or 1)
   bool side = false;  // the run on the side stream
   bool hold = false;  // the query waits for ms_release
+  bool gate = false;  // held: the query also waits for the other held runs' set-ups
   // optional: the votes the points come from (rigid: q4, t3; mirror: n3 with
   // row stride `from_stride`, d1); the run then writes the points into its
   // `pts` argument, in the launch that takes their box and moments

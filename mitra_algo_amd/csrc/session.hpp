@@ -82,6 +82,7 @@ struct Session {
   struct MsHeld {
     hipStream_t stream;
     int bank;
+    bool gate;  // also waits for the other held runs' set-ups
     std::function<void()> launch;
   };
   std::vector<MsHeld> ms_held;
