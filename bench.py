@@ -970,7 +970,7 @@ def main():
                              "the support balls of a seed and its 15 nearest seeds cover "
                              "2.1-2.9x (mirror) / 14-15x (rigid) the points of one ball",
                      "valu_counters": valu("k_ms_query<float, 4, 4>",
-                                           "k_ms_query_small<float, 6, 32>"),
+                                           "k_ms_query_small<float, 6, 32, false>"),
                      "achieved": q_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": q_tflops / FP32_PEAK_TFLOPS,
                      "traffic": traffic.get("ms_query", {}).get("bytes"),
