@@ -178,6 +178,43 @@ def test_meanshift_support_visits_match_brute_force(D, spread, lanes):
     assert cand >= ins
 
 
+@pytest.mark.parametrize("lanes", [32, 16])
+@pytest.mark.parametrize("D", [6, 4])
+def test_narrow_query_long_candidate_lists(oracle, D, lanes):
+    """Seeds inside a dense cluster have more candidates than a seed group's
+    LDS map holds (512 entries at 32 lanes per seed, 256 at 16), so the
+    narrow query walks its rows in several chunks; seeds between clusters
+    move far enough to cut new rows.  Modes and weights against the oracle."""
+    g = np.random.default_rng(11 + D)
+    dense = g.normal(0.0, 0.03, (2500, D))
+    other = np.array([0.4] + [0.0] * (D - 1)) + g.normal(0.0, 0.05, (800, D))
+    P = np.concatenate([dense, other, g.uniform(-1.0, 1.0, (400, D))]).T.copy()
+    seeds = P[:, g.choice(P.shape[1], 200, replace=False)].copy()
+    h, cut, iters = 0.05, 3.0, 25
+    # most dense points lie inside the support of the cluster centre
+    assert np.sum(np.sum(dense ** 2, axis=1) < (cut * h) ** 2) > 4 * 512
+    want, wsum = oracle.meanshift_batched(P, np.zeros(P.shape[1], np.uint8), D, D, seeds,
+                                          np.zeros(seeds.shape[1], np.uint8), h, cut, iters)
+    s = capi.Session(0)
+    s.set_tuning(0, 2048, 0, lanes)
+    dev = torch.device("cuda:0")
+    s.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    for prec, dt, tol in ((capi.FP64, torch.float64, 1e-9), (capi.FP32_GUARDED, torch.float32,
+                                                             2e-5)):
+        pts = torch.as_tensor(P, dtype=dt, device=dev).contiguous()
+        sd = torch.as_tensor(seeds, dtype=dt, device=dev).contiguous()
+        out = torch.empty_like(sd)
+        w = torch.empty(seeds.shape[1], dtype=dt, device=dev)
+        cand, ins = s.dev_ms_batched(prec, D, P.shape[1], pts, seeds.shape[1], sd, iters, h, cut,
+                                     out, w, stats=True)
+        torch.cuda.synchronize()
+        got = out.double().cpu().numpy()
+        assert np.max(np.abs(got - want)) < tol, float(np.max(np.abs(got - want)))
+        assert np.max(np.abs(w.double().cpu().numpy() - wsum) / (1 + wsum)) < tol
+        assert 0 < ins <= cand
+    s.close()
+
+
 def test_batched_meanshift_empty_and_degenerate(oracle):
     s = capi.Session(0)
     dev = torch.device("cuda:0")
