@@ -324,24 +324,30 @@ __device__ inline void source_mesh(const PairArgs &A, uint64_t i, double &dist, 
 template <bool kGuarded> struct BucketK;
 template <> struct BucketK<true> {
   using T = float2;
+  using P = float4;
   __device__ static T load(const PairArgs &A, uint64_t j) { return A.bkf[j]; }
+  __device__ static P pos(const PairArgs &A, uint64_t j) { return A.pbf[j]; }
 };
 template <> struct BucketK<false> {
   using T = double2;
+  using P = double4;
   __device__ static T load(const PairArgs &A, uint64_t j) { return A.bk[j]; }
+  __device__ static P pos(const PairArgs &A, uint64_t j) { return A.pb[j]; }
 };
 
-// Decide candidate at bucketed position j (ordinal o, curvatures bkj).
+// Decide candidate at bucketed position j (ordinal o, curvatures bkj,
+// position pj: loaded with the ordinal, not after the curvature test).
 template <bool kGuarded>
 __device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, uint32_t o,
-                              const typename BucketK<kGuarded>::T &bkj, unsigned &rechecks) {
+                              const typename BucketK<kGuarded>::T &bkj,
+                              const typename BucketK<kGuarded>::P &pj, unsigned &rechecks) {
   if constexpr (!kGuarded) {
     const double2 b = bkj;
     double m = fmax(fmax(fabs(src.a0), fabs(src.a1)), fmax(fmax(fabs(b.x), fabs(b.y)), 1e-10));
     double tol = A.t * m;
     if (fabs(src.a0 - b.x) > tol || fabs(src.a1 - b.y) > tol)
       return false;
-    const double4 p = A.pb[j];
+    const double4 p = pj;
     return accept64(src.a0, src.a1, b.x, b.y, src.ax, src.ay, src.az, p.x, p.y, p.z, A.t,
                     src.dist);
   } else {
@@ -350,7 +356,7 @@ __device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, 
     if (r == 0)
       return false;
     if (r == 1) {
-      const float4 p = A.pbf[j];
+      const float4 p = pj;
       int d = screen32_dist(src.fx - p.x, src.fy - p.y, src.fz - p.z, src.dist32, A.gd);
       if (d != 2)
         return d == 1;
@@ -497,8 +503,9 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, bool live, Hal
         const uint64_t j = W.start[l] + (c - W.pre[l]);
         o = A.bord[j];
         const typename BucketK<kGuarded>::T bkj = BucketK<kGuarded>::load(A, j);
+        const typename BucketK<kGuarded>::P pj = BucketK<kGuarded>::pos(A, j);
         if (o != kNone && o > i && o < src.ohi)
-          ok = decide<kGuarded>(A, src, j, o, bkj, rechecks);
+          ok = decide<kGuarded>(A, src, j, o, bkj, pj, rechecks);
       }
       f(ok, o);
     }
