@@ -217,6 +217,8 @@ constexpr int kMaxRows = (2 * kMaxReach + 1) * (2 * kMaxReach + 1);
 constexpr int kCellBits = 22;  // at most 2^22 cells: three 8-bit radix passes
 constexpr uint64_t kMaxCells = 1ULL << kCellBits;
 constexpr unsigned kBboxBlocks = 1024;
+// points per thread and round of k_points_moments (FP64: 2, for registers)
+template <class R> constexpr int kPointsPerRound = sizeof(R) == 4 ? 4 : 2;
 
 // Device-side control block of one mean-shift call: the grid, the sums of
 // up to two runs (run k: candidates, in-support visits, seed-iterations at
@@ -399,40 +401,75 @@ __global__ void __launch_bounds__(kThreads)
                      double *partial) {
   __shared__ double sh[kWavesPerBlock][kPart];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  double v[kPart];
-  for (int k = 0; k < kMaxD; ++k)
-    v[k] = INFINITY, v[kMaxD + k] = -INFINITY, v[2 * kMaxD + k] = 0, v[3 * kMaxD + k] = 0;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
-       i += static_cast<uint64_t>(gridDim.x) * kThreads) {
-    R c[D];
-    if constexpr (D == 6) {
-      const R w = a[i], x = a[n + i], y = a[2 * n + i], z = a[3 * n + i];
-      const R sn = sqrt(x * x + y * y + z * z);
-      const R f = sn > R(1e-12) ? R(2) * atan2(sn, w) / sn : R(2) / (w > R(0.5) ? w : R(1));
-      c[0] = b[i], c[1] = b[n + i], c[2] = b[2 * n + i];
-      c[3] = f * x, c[4 % D] = f * y, c[5 % D] = f * z;
-    } else {
-      c[0] = b[i], c[1] = a[i], c[2] = a[stride + i], c[3] = a[2 * stride + i];
+  // box in the points' own type (exact), sums and sums of squares in FP64
+  R lo[D], hi[D];
+  double sm[D], sq[D];
+  for (int k = 0; k < D; ++k)
+    lo[k] = R(INFINITY), hi[k] = R(-INFINITY), sm[k] = 0, sq[k] = 0;
+  // kPointsPerRound<R> points per thread and round, all their loads in flight
+  // before the arithmetic (rigid: q4 then t3; mirror: d then n3)
+  constexpr int kIn = D == 6 ? 7 : 4;
+  const uint64_t step = static_cast<uint64_t>(gridDim.x) * kThreads;
+  for (uint64_t i0 = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i0 < n;
+       i0 += kPointsPerRound<R> * step) {
+    R in[kPointsPerRound<R>][kIn];
+    for (int u = 0; u < kPointsPerRound<R>; ++u) {
+      const uint64_t i = i0 + u * step;
+      if (i >= n)
+        continue;
+      // 32-bit lane offsets from per-row scalar bases (n < 2^32)
+      const uint32_t iu = static_cast<uint32_t>(i);
+      if constexpr (D == 6) {
+        for (int k = 0; k < 4; ++k)
+          in[u][k] = (a + k * n)[iu];
+        for (int k = 0; k < 3; ++k)
+          in[u][4 + k] = (b + k * n)[iu];
+      } else {
+        in[u][0] = b[iu];
+        for (int k = 0; k < 3; ++k)
+          in[u][1 + k] = (a + k * stride)[iu];
+      }
     }
-    for (int k = 0; k < D; ++k) {
-      pts[k * n + i] = c[k];
-      const double x = static_cast<double>(c[k]);
-      v[k] = fmin(v[k], x);
-      v[kMaxD + k] = fmax(v[kMaxD + k], x);
-      v[2 * kMaxD + k] += x;
-      v[3 * kMaxD + k] += x * x;
+    for (int u = 0; u < kPointsPerRound<R>; ++u) {
+      const uint64_t i = i0 + u * step;
+      if (i >= n)
+        continue;
+      R c[D];
+      if constexpr (D == 6) {
+        const R w = in[u][0], x = in[u][1], y = in[u][2], z = in[u][3];
+        const R sn = sqrt(x * x + y * y + z * z);
+        const R f = sn > R(1e-12) ? R(2) * atan2(sn, w) / sn : R(2) / (w > R(0.5) ? w : R(1));
+        c[0] = in[u][4], c[1] = in[u][5], c[2] = in[u][6];
+        c[3] = f * x, c[4 % D] = f * y, c[5 % D] = f * z;
+      } else {
+        for (int k = 0; k < D; ++k)
+          c[k] = in[u][k];
+      }
+      for (int k = 0; k < D; ++k) {
+        (pts + k * n)[static_cast<uint32_t>(i)] = c[k];
+        const double x = static_cast<double>(c[k]);
+        lo[k] = fmin(lo[k], c[k]);
+        hi[k] = fmax(hi[k], c[k]);
+        sm[k] += x;
+        sq[k] += x * x;
+      }
     }
   }
   for (int o = kWave / 2; o > 0; o >>= 1)
-    for (int k = 0; k < kMaxD; ++k) {
-      v[k] = fmin(v[k], __shfl_xor(v[k], o, kWave));
-      v[kMaxD + k] = fmax(v[kMaxD + k], __shfl_xor(v[kMaxD + k], o, kWave));
-      v[2 * kMaxD + k] += __shfl_xor(v[2 * kMaxD + k], o, kWave);
-      v[3 * kMaxD + k] += __shfl_xor(v[3 * kMaxD + k], o, kWave);
+    for (int k = 0; k < D; ++k) {
+      lo[k] = fmin(lo[k], __shfl_xor(lo[k], o, kWave));
+      hi[k] = fmax(hi[k], __shfl_xor(hi[k], o, kWave));
+      sm[k] += __shfl_xor(sm[k], o, kWave);
+      sq[k] += __shfl_xor(sq[k], o, kWave);
     }
   if (lane == 0)
-    for (int k = 0; k < kPart; ++k)
-      sh[wave][k] = v[k];
+    for (int k = 0; k < kMaxD; ++k) {
+      const bool in = k < D;
+      sh[wave][k] = in ? static_cast<double>(lo[k % D]) : INFINITY;
+      sh[wave][kMaxD + k] = in ? static_cast<double>(hi[k % D]) : -INFINITY;
+      sh[wave][2 * kMaxD + k] = in ? sm[k % D] : 0.0;
+      sh[wave][3 * kMaxD + k] = in ? sq[k % D] : 0.0;
+    }
   __syncthreads();
   if (threadIdx.x < kPart) {
     const int k = threadIdx.x;
@@ -569,8 +606,6 @@ __global__ void k_cell_keys(const R *pts, uint64_t n, const Grid *grids, const u
   keys[i] = (cellbase ? cellbase[m] : 0) + cell_key(g, a, b, c);
 }
 
-// offsets[c] = first sorted position with key >= c, for c in [0, ncells]
-// (grid-stride: the cell count lives on the device, at *total).
 // offsets[c] = the first sorted record whose key is >= c, c in [0, ncells]:
 // record i (0 <= i <= n) starts the cells (key[i - 1], key[i]] (key[-1] = -1,
 // key[n] = ncells; keys past ncells, the segments' sentinels, clamp), so each
@@ -1514,11 +1549,12 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
     // bank 0 shares its slot with the control block, which follows the partials
     partial = scratch_as<double>(s, bank1 ? kSlotMsBankPart : kSlotMs1, kPart * kBboxBlocks);
     if (n) {
-      nb = blocks_for(n, kThreads);
+      const R *fa = static_cast<const R *>(run.from_a), *fb = static_cast<const R *>(run.from_b);
+      // points from votes: a round of kPointsPerRound<R> points per thread
+      nb = blocks_for(n, fa ? kThreads * kPointsPerRound<R> : kThreads);
       if (nb > kBboxBlocks)
         nb = kBboxBlocks;
       R *wpts = const_cast<R *>(pts);  // the caller's point buffer, written here
-      const R *fa = static_cast<const R *>(run.from_a), *fb = static_cast<const R *>(run.from_b);
       if (fa && D == 6)
         MVS_LAUNCH(s, "ms_points", (k_points_moments<R, 6>), nb, kThreads, 0, fa, fb, n, n,
                    wpts, partial);
