@@ -146,10 +146,14 @@ enum Slot : int {
 };
 
 // ------------------------------------------------------------- primitives
-// Exclusive prefix sum of n uint64 values into out[0..n] (out[n] = total).
-void exclusive_scan_u64(Session *s, const uint64_t *in, uint64_t *out, uint64_t n);
+// Exclusive prefix sum of n uint64 values into out[0..n] (out[n] = total);
+// the total also goes to *total_also when given (one launch fewer for a
+// caller that reads it back).
+void exclusive_scan_u64(Session *s, const uint64_t *in, uint64_t *out, uint64_t n,
+                        uint64_t *total_also = nullptr);
 // Exclusive prefix sum of n uint32 values into uint64 out[0..n].
-void exclusive_scan_u32(Session *s, const uint32_t *in, uint64_t *out, uint64_t n);
+void exclusive_scan_u32(Session *s, const uint32_t *in, uint64_t *out, uint64_t n,
+                        uint64_t *total_also = nullptr);
 // Stable LSD radix sort of (key, value) by the low `bits` bits of key.
 void radix_sort_pairs_u64(Session *s, const uint64_t *keys_in, const uint32_t *vals_in,
                           uint64_t *keys_out, uint32_t *vals_out, uint64_t n, int bits);

@@ -702,8 +702,6 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-__global__ void k_read_total(const uint64_t *off, uint64_t n, uint64_t *dst) { *dst = off[n]; }
-
 // Sets *mismatch when the device bytes differ from the recorded ones.
 __global__ void k_check_equal(const unsigned char *a, const unsigned char *b, uint64_t bytes,
                               unsigned *mismatch) {
@@ -935,10 +933,10 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   else
     MVS_LAUNCH(s, "pair_walk", k_pair_walk<false>, grid, kThreads, 0, A, st.counts, st.slab,
                st.long_list, long_count, quota, queue);
-  exclusive_scan_u64(s, st.counts, st.offsets, S);
   // flags, long-source count and total read back in one synchronisation
+  // (the scan writes the total beside the flags too)
   uint64_t *tot = reinterpret_cast<uint64_t *>(flags + 2);
-  MVS_LAUNCH(s, "read_total", k_read_total, 1, 1, 0, st.offsets, S, tot);
+  exclusive_scan_u64(s, st.counts, st.offsets, S, tot);
   unsigned host[4];
   read_back(s, host, flags, sizeof host);
   if (host[0] & 1u)

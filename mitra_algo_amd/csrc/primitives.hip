@@ -65,7 +65,8 @@ __global__ void __launch_bounds__(kScanThreads)
 constexpr unsigned kSelfPrefixTiles = 8192;
 template <class TIn, bool kSelfPrefix>
 __global__ void __launch_bounds__(kScanThreads)
-    k_scan_apply(const TIn *in, uint64_t n, const uint64_t *partial, uint64_t *out) {
+    k_scan_apply(const TIn *in, uint64_t n, const uint64_t *partial, uint64_t *out,
+                 uint64_t *total_also) {
   __shared__ uint64_t lds[kScanThreads / kWave + 1];
   uint64_t tile_off = 0;
   if (kSelfPrefix) {
@@ -92,37 +93,46 @@ __global__ void __launch_bounds__(kScanThreads)
       out[base + i] = run;
     run += v[i];
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
     out[n] = tile_off + total;
+    if (total_also)
+      *total_also = tile_off + total;
+  }
 }
 
-__global__ void k_zero_u64(uint64_t *p) { *p = 0; }
+__global__ void k_zero_u64(uint64_t *p, uint64_t *q) {
+  *p = 0;
+  if (q)
+    *q = 0;
+}
 
 // Exclusive scan: one launch for one tile, two (tile totals, then tiles that
 // add their own prefix) up to kSelfPrefixTiles tiles, recursive beyond.
 // `pool` holds the partial arrays of all levels (carved from one scratch
 // slot, sized by scan_pool_words).
 template <class TIn>
-void exclusive_scan_level(Session *s, const TIn *in, uint64_t *out, uint64_t n, uint64_t *pool) {
+void exclusive_scan_level(Session *s, const TIn *in, uint64_t *out, uint64_t n, uint64_t *pool,
+                          uint64_t *total_also) {
   if (n == 0) {
-    MVS_LAUNCH(s, "scan_zero", k_zero_u64, 1, 1, 0, out);
+    MVS_LAUNCH(s, "scan_zero", k_zero_u64, 1, 1, 0, out, total_also);
     return;
   }
   unsigned nb = blocks_for(n, kScanTile);
   uint64_t *part = pool, *part_scan = pool + nb;
   if (nb == 1) {
     MVS_LAUNCH(s, "scan_apply", (k_scan_apply<TIn, false>), 1, kScanThreads, 0, in, n, nullptr,
-               out);
+               out, total_also);
     return;
   }
   MVS_LAUNCH(s, "scan_reduce", (k_scan_reduce<TIn>), nb, kScanThreads, 0, in, n, part);
   if (nb <= kSelfPrefixTiles) {
-    MVS_LAUNCH(s, "scan_apply", (k_scan_apply<TIn, true>), nb, kScanThreads, 0, in, n, part, out);
+    MVS_LAUNCH(s, "scan_apply", (k_scan_apply<TIn, true>), nb, kScanThreads, 0, in, n, part, out,
+               total_also);
     return;
   }
-  exclusive_scan_level<uint64_t>(s, part, part_scan, nb, pool + 2 * nb + 1);
+  exclusive_scan_level<uint64_t>(s, part, part_scan, nb, pool + 2 * nb + 1, nullptr);
   MVS_LAUNCH(s, "scan_apply", (k_scan_apply<TIn, false>), nb, kScanThreads, 0, in, n, part_scan,
-             out);
+             out, total_also);
 }
 uint64_t scan_pool_words(uint64_t n) {
   uint64_t words = 0;
@@ -137,9 +147,10 @@ uint64_t scan_pool_words(uint64_t n) {
   }
 }
 template <class TIn>
-void exclusive_scan_impl(Session *s, const TIn *in, uint64_t *out, uint64_t n, int slot) {
+void exclusive_scan_impl(Session *s, const TIn *in, uint64_t *out, uint64_t n, int slot,
+                         uint64_t *total_also) {
   uint64_t *pool = scratch_as<uint64_t>(s, slot, scan_pool_words(n));
-  exclusive_scan_level<TIn>(s, in, out, n, pool);
+  exclusive_scan_level<TIn>(s, in, out, n, pool, total_also);
 }
 
 // ------------------------------------------------------------- radix sort
@@ -469,11 +480,13 @@ void zero_device(Session *s, void *ptr, size_t bytes) {
              static_cast<uint64_t>(bytes));
 }
 
-void exclusive_scan_u64(Session *s, const uint64_t *in, uint64_t *out, uint64_t n) {
-  exclusive_scan_impl<uint64_t>(s, in, out, n, s->bank ? kSlotScanB : kSlotScan);
+void exclusive_scan_u64(Session *s, const uint64_t *in, uint64_t *out, uint64_t n,
+                        uint64_t *total_also) {
+  exclusive_scan_impl<uint64_t>(s, in, out, n, s->bank ? kSlotScanB : kSlotScan, total_also);
 }
-void exclusive_scan_u32(Session *s, const uint32_t *in, uint64_t *out, uint64_t n) {
-  exclusive_scan_impl<uint32_t>(s, in, out, n, s->bank ? kSlotScanB : kSlotScan);
+void exclusive_scan_u32(Session *s, const uint32_t *in, uint64_t *out, uint64_t n,
+                        uint64_t *total_also) {
+  exclusive_scan_impl<uint32_t>(s, in, out, n, s->bank ? kSlotScanB : kSlotScan, total_also);
 }
 
 void radix_sort_pairs_u64(Session *s, const uint64_t *keys_in, const uint32_t *vals_in,

@@ -449,8 +449,6 @@ __global__ void k_trans_fill(TransArgs A, const uint64_t *off, uint64_t V, doubl
   }
 }
 
-__global__ void k_read_total64(const uint64_t *off, uint64_t n, uint64_t *dst) { *dst = off[n]; }
-
 }  // namespace
 
 static VoteArgs vote_args(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
@@ -503,8 +501,7 @@ void vote_offsets_device(Session *s, uint64_t P, const int32_t *a, const int32_t
     else
       MVS_LAUNCH(s, "vote_count", k_vote_count<false>, grid, kThreads, 0, A, wr);
   }
-  exclusive_scan_u32(s, wr, off, nw);
-  MVS_LAUNCH(s, "read_total", k_read_total64, 1, 1, 0, off, nw, total_dev);
+  exclusive_scan_u32(s, wr, off, nw, total_dev);
 }
 
 void vote_emit_strided(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
