@@ -46,6 +46,14 @@ constexpr int kBucket = 64;               // samples per kmin bucket
 constexpr int kPivot = 8;                 // bucket entries per search segment
 constexpr double kEps = 2.220446049250313e-16;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+// Signature sort key: the top kSigBits of the order-preserving 32-bit key of
+// kmin (sign, exponent and 12 mantissa bits: classes 2^-12 of the value
+// wide), sorted in three 8-bit passes instead of four.  Samples of one class
+// keep their ordinal order.  Windows are searched on the same classes, so a
+// window holds every sample of the classes its kmin interval touches: a
+// superset of the exact window (every candidate is decided exactly), and the
+// pairs and their order do not depend on the sorted order.
+constexpr int kSigBits = 24;
 
 __device__ inline uint32_t order_key32(double v) {
   if (v == 0.0)
@@ -115,7 +123,7 @@ __global__ void k_gather_samples(const double *x, const double *y, const double 
         f4[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
       r = fmaxf(fabsf(f[L_X]), fmaxf(fabsf(f[L_Y]), fabsf(f[L_Z])));
     }
-    keys[i] = (mesh << 32) | order_key32(v[L_K0]);
+    keys[i] = (mesh << kSigBits) | (order_key32(v[L_K0]) >> (32 - kSigBits));
     vals[i] = static_cast<uint32_t>(i);
   }
   // largest centre-relative coordinate (sets the FP32 guard bands); every
@@ -190,7 +198,7 @@ __global__ void k_windows(const double *r64, const uint64_t *skeys, uint64_t S, 
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= S)
     return;
-  const uint64_t mesh = seg ? static_cast<uint64_t>(seg[i]) << 32 : 0;
+  const uint64_t mesh = seg ? static_cast<uint64_t>(seg[i]) << kSigBits : 0;
   uint32_t qlo = 0, qhi = 0xFFFFFFFFu;
   double klo = -INFINITY, khi = INFINITY;
   double tp = t * (1 + 16 * kEps);
@@ -204,7 +212,7 @@ __global__ void k_windows(const double *r64, const uint64_t *skeys, uint64_t S, 
     klo = b - r - pb;
     khi = b + r + pb;
   }
-  const uint64_t kl = mesh | qlo, kh = mesh | qhi;
+  const uint64_t kl = mesh | (qlo >> (32 - kSigBits)), kh = mesh | (qhi >> (32 - kSigBits));
   uint64_t f = 0, c = S;
   while (c) {  // first position with key >= kl
     uint64_t h = c / 2, m = f + h;
@@ -910,9 +918,9 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
     *P_out = 0;
     return;
   }
-  int key_bits = 32;  // batches: the mesh id sits above the curvature key
+  int key_bits = kSigBits;  // batches: the mesh id sits above the curvature key
   if (segs.seg)
-    while (key_bits < 64 && (s->batch_meshes >> (key_bits - 32)) != 0)
+    while (key_bits < 64 && (s->batch_meshes >> (key_bits - kSigBits)) != 0)
       ++key_bits;
   radix_sort_pairs_u64(s, st.keys, st.ord, st.keys + S, st.ord + S, S, key_bits);
   const uint64_t nb = (S + kBucket - 1) / kBucket;
