@@ -139,12 +139,12 @@ __global__ void k_gather_samples(const double *x, const double *y, const double 
 // with a 64-lane bitonic network and write the bucketed index, with each
 // entry's curvatures and position beside it (the walk reads a candidate's
 // fields with coalesced loads at its bucketed position, not at its ordinal).
-__global__ void __launch_bounds__(kThreads)
-    k_bucket_index(const uint32_t *ord, const double *r64, const float *r32, uint64_t S,
+__device__ inline void bucket_index_block(
+    unsigned blk, const uint32_t *ord, const double *r64, const float *r32, uint64_t S,
                    uint32_t *bord, double2 *bk, float2 *bkf, double4 *pb, float4 *pbf,
                    double *piv) {
   const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
+  const uint64_t b = static_cast<uint64_t>(blk) * kWaves + threadIdx.x / kWave;
   const uint64_t nb = (S + kBucket - 1) / kBucket;
   if (b >= nb)
     return;
@@ -193,9 +193,10 @@ __global__ void __launch_bounds__(kThreads)
 // endpoints by 16 eps of their magnitude; t >= .5 or t below the normal
 // range use the sample's whole mesh.
 // win: [0,S) first sorted position, [S,2S) end position; kw: kmax interval.
-__global__ void k_windows(const double *r64, const uint64_t *skeys, uint64_t S, double t,
-                          const uint32_t *seg, uint64_t *win, double2 *kw) {
-  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ inline void windows_block(unsigned blk, const double *r64, const uint64_t *skeys,
+                                     uint64_t S, double t, const uint32_t *seg, uint64_t *win,
+                                     double2 *kw) {
+  uint64_t i = static_cast<uint64_t>(blk) * kThreads + threadIdx.x;
   if (i >= S)
     return;
   const uint64_t mesh = seg ? static_cast<uint64_t>(seg[i]) << kSigBits : 0;
@@ -236,6 +237,19 @@ __global__ void k_windows(const double *r64, const uint64_t *skeys, uint64_t S, 
   win[i] = lo;
   win[S + i] = f;
   kw[i] = make_double2(klo, khi);
+}
+
+// The bucket index (blocks [0, index_blocks)) and the windows (the rest)
+// in one launch: both read the sorted order, neither the other's output.
+__global__ void __launch_bounds__(kThreads)
+    k_index_windows(unsigned index_blocks, const uint32_t *ord, const double *r64,
+                    const float *r32, uint64_t S, uint32_t *bord, double2 *bk, float2 *bkf,
+                    double4 *pb, float4 *pbf, double *piv, const uint64_t *skeys, double t,
+                    const uint32_t *seg, uint64_t *win, double2 *kw) {
+  if (blockIdx.x < index_blocks)
+    bucket_index_block(blockIdx.x, ord, r64, r32, S, bord, bk, bkf, pb, pbf, piv);
+  else
+    windows_block(blockIdx.x - index_blocks, r64, skeys, S, t, seg, win, kw);
 }
 
 // Exact reference predicate in FP64 (pairing.cpp:22-28).
@@ -924,10 +938,10 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
       ++key_bits;
   radix_sort_pairs_u64(s, st.keys, st.ord, st.keys + S, st.ord + S, S, key_bits);
   const uint64_t nb = (S + kBucket - 1) / kBucket;
-  MVS_LAUNCH(s, "pair_index", k_bucket_index, blocks_for(nb, kWaves), kThreads, 0, st.ord + S,
-             st.r64, st.r32, S, st.bord, st.bk, st.bkf, st.pb, st.pbf, st.piv);
-  MVS_LAUNCH(s, "pair_windows", k_windows, blocks_for(S, kThreads), kThreads, 0, st.r64,
-             st.keys + S, S, t, segs.seg, st.win, st.kw);
+  const unsigned index_blocks = blocks_for(nb, kWaves);
+  MVS_LAUNCH(s, "pair_index", k_index_windows, index_blocks + blocks_for(S, kThreads), kThreads,
+             0, index_blocks, st.ord + S, st.r64, st.r32, S, st.bord, st.bk, st.bkf, st.pb,
+             st.pbf, st.piv, st.keys + S, t, segs.seg, st.win, st.kw);
   unsigned *long_count = flags + 1;
   unsigned *queue = flags + 12;  // cleared with the flags above
   PairArgs A = pair_args(st, S, t, box, guarded, flags, segs);
