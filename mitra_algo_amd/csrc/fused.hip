@@ -36,7 +36,7 @@ void pairvote(Session *s, uint64_t n, const double *x, const double *y, const do
   int32_t *fp = scratch_as<int32_t>(s, kSlotOutReflPair, Pp);
   pair_fill_device(s, ab, ab + Pp, true);
   uint64_t *rtot = reinterpret_cast<uint64_t *>(plan_flags(s)) + kVoteTotalWord;
-  vote_offsets_device(s, P, ab, ab + Pp, precision, rtot, segs);
+  vote_offsets_device(s, P, ab, ab + Pp, precision, rtot, segs, true);
   // no rigid pair-id stream: rigid vote v belongs to pair v / 2
   vote_emit_strided(s, P, ab, ab + Pp, center, precision, Pp, q4, t3, nullptr, n3, d1, fp,
                     segs);

@@ -920,7 +920,9 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   const bool guarded = precision == MVS_FP32_GUARDED;
   PairState st = pair_state(s, S, guarded);
   unsigned *flags = plan_flags(s);
-  zero_device(s, flags, 16 * sizeof(unsigned));
+  // the pairing words and the vote phase's out-of-range flag (kVoteBadWord)
+  static_assert(kVoteBadWord < 22, "the flag words cleared here");
+  zero_device(s, flags, 22 * sizeof(unsigned));
   double cx = center ? center[0] : 0, cy = center ? center[1] : 0, cz = center ? center[2] : 0;
   if (S)
     MVS_LAUNCH(s, "pair_gather", k_gather_samples, blocks_for(S, kThreads), kThreads, 0, x, y, z,

@@ -52,9 +52,11 @@ void vote_emit_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b
                       void *n3, void *d1, int32_t *fp);
 // Vote phases without the reflection read-back: mirror votes go to a stream
 // of capacity (and SoA stride) `stride`; the total is left on the device at
-// *total_dev (uint64).
+// *total_dev (uint64).  bad_cleared: the out-of-range flag word was cleared
+// with the pair-count flags and is not cleared again.
 void vote_offsets_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
-                         int precision, uint64_t *total_dev, const Segs &segs = Segs{});
+                         int precision, uint64_t *total_dev, const Segs &segs = Segs{},
+                         bool bad_cleared = false);
 void vote_emit_strided(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
                        const double *center, int precision, uint64_t stride, void *q4, void *t3,
                        int32_t *rp, void *n3, void *d1, int32_t *fp, const Segs &segs = Segs{});

@@ -485,7 +485,8 @@ static VoteArgs vote_args(Session *s, uint64_t P, const int32_t *a, const int32_
 }
 
 void vote_offsets_device(Session *s, uint64_t P, const int32_t *a, const int32_t *b,
-                         int precision, uint64_t *total_dev, const Segs &segs) {
+                         int precision, uint64_t *total_dev, const Segs &segs,
+                         bool bad_cleared) {
   if (precision != MVS_FP64 && precision != MVS_FP32_GUARDED)
     raise(MVS_EINVAL, "unknown precision");
   const bool guarded = precision == MVS_FP32_GUARDED;
@@ -493,7 +494,8 @@ void vote_offsets_device(Session *s, uint64_t P, const int32_t *a, const int32_t
   const uint64_t nw = (P + kWave - 1) / kWave;
   uint32_t *wr = scratch_as<uint32_t>(s, kSlotVoteBlock, nw ? nw : 1);
   uint64_t *off = scratch_as<uint64_t>(s, kSlotVoteBlockOff, nw + 1);
-  zero_device(s, A.bad, sizeof(unsigned));
+  if (!bad_cleared)
+    zero_device(s, A.bad, sizeof(unsigned));
   if (P) {
     unsigned grid = pair_grid(nw);
     if (guarded)
