@@ -32,7 +32,8 @@ KERNELS = ("batch_seg", "batch_offsets", "batch_seeds",
            "pair_gather", "radix_hist", "radix_scatter", "scan_reduce", "scan_apply", "scan_zero",
            "pair_index", "pair_walk", "pair_place", "pair_long", "vote_count",
            "vote_emit", "zero", "ms_seeds", "ms_points", "ms_bbox", "ms_grid",
-           "ms_cell_keys", "ms_cell_offsets", "ms_gather", "ms_seed_keys", "ms_query", "ms_stats")
+           "ms_cell_keys", "ms_cell_offsets", "ms_gather", "ms_after_sort", "ms_seed_keys",
+           "ms_query", "ms_stats")
 # kernels timed with HIP events inside the timed region (the roofline kernels)
 TIMED = ("pair_walk", "vote_emit", "ms_query")
 

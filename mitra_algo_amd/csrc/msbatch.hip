@@ -612,13 +612,13 @@ __global__ void k_cell_keys(const R *pts, uint64_t n, const Grid *grids, const u
 // record writes its run of cells.  Runs of up to kShortRun cells are written
 // by their lane, longer runs (empty stretches of the grid) by the whole wave.
 constexpr uint64_t kShortRun = 8;
-__global__ void __launch_bounds__(kThreads)
-    k_cell_offsets(const uint64_t *skeys, uint64_t n, const uint64_t *total,
-                   uint64_t *offsets) {
+// (block blk of nblk)
+__device__ inline void cell_offsets_block(unsigned blk, unsigned nblk, const uint64_t *skeys,
+                                          uint64_t n, const uint64_t *total, uint64_t *offsets) {
   const uint64_t ncells = *total;
   const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + (threadIdx.x - lane);
+  const uint64_t stride = static_cast<uint64_t>(nblk) * kThreads;
+  for (uint64_t base = static_cast<uint64_t>(blk) * kThreads + (threadIdx.x - lane);
        base <= n; base += stride) {
     const uint64_t i = base + static_cast<uint64_t>(lane);
     uint64_t lo = 0, hi = 0;  // the run: cells [lo, hi)
@@ -642,11 +642,18 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+__global__ void __launch_bounds__(kThreads)
+    k_cell_offsets(const uint64_t *skeys, uint64_t n, const uint64_t *total,
+                   uint64_t *offsets) {
+  cell_offsets_block(blockIdx.x, gridDim.x, skeys, n, total, offsets);
+}
+
 template <class R, int D>
-__global__ void k_gather_sorted(const R *aos, uint64_t n, const Grid *grids,
-                                const uint64_t *seg_off, uint32_t meshes, const uint32_t *order,
-                                typename Vec<R>::V4 *p4, typename Vec<R>::V2 *q2) {
-  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ inline void gather_sorted_block(unsigned blk, const R *aos, uint64_t n,
+                                           const Grid *grids, const uint64_t *seg_off,
+                                           uint32_t meshes, const uint32_t *order,
+                                           typename Vec<R>::V4 *p4, typename Vec<R>::V2 *q2) {
+  uint64_t i = static_cast<uint64_t>(blk) * kThreads + threadIdx.x;
   if (i >= n)
     return;
   if (i == 0) {  // the sentinel record at [n]: outside every support
@@ -694,10 +701,11 @@ __global__ void k_gather_sorted(const R *aos, uint64_t n, const Grid *grids,
 // distinct and in cell order.  ids receive the seeds' point indices in the
 // caller's (unsorted) numbering, seed_mesh their segment.  Segments: seed_off
 // (meshes + 1) splits the seeds, seg_off the points; one segment otherwise.
-__global__ void k_pick_sorted(const uint64_t *seg_off, const uint64_t *seed_off, uint32_t meshes,
-                              uint64_t n, uint64_t k, uint64_t salt, const uint32_t *sorted_ids,
-                              uint32_t *pos, int32_t *ids, uint32_t *seed_mesh) {
-  const uint64_t j = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ inline void pick_sorted_block(unsigned blk, const uint64_t *seg_off,
+                                         const uint64_t *seed_off, uint32_t meshes, uint64_t n,
+                                         uint64_t k, uint64_t salt, const uint32_t *sorted_ids,
+                                         uint32_t *pos, int32_t *ids, uint32_t *seed_mesh) {
+  const uint64_t j = static_cast<uint64_t>(blk) * kThreads + threadIdx.x;
   if (j >= (seed_off ? seed_off[meshes] : k))
     return;
   const uint32_t m = seg_of(seed_off, meshes, j);
@@ -711,6 +719,46 @@ __global__ void k_pick_sorted(const uint64_t *seg_off, const uint64_t *seed_off,
   ids[j] = static_cast<int32_t>(sorted_ids[p]);
   if (seed_mesh)
     seed_mesh[j] = m;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_pick_sorted(const uint64_t *seg_off, const uint64_t *seed_off, uint32_t meshes, uint64_t n,
+                  uint64_t k, uint64_t salt, const uint32_t *sorted_ids, uint32_t *pos,
+                  int32_t *ids, uint32_t *seed_mesh) {
+  pick_sorted_block(blockIdx.x, seg_off, seed_off, meshes, n, k, salt, sorted_ids, pos, ids,
+                    seed_mesh);
+}
+
+// What follows the cell sort, in one launch of three block ranges that read
+// the sorted keys and order only: the cell offsets (nb_off blocks), the
+// gather of the sorted records (nb_gather), and the drawn seeds (the rest;
+// none without seed_ids).
+struct SortedArgs {
+  const uint64_t *skeys;
+  const uint64_t *total;
+  uint64_t *offsets;
+  const uint64_t *seg_off, *seed_off;
+  uint32_t meshes;
+  uint64_t n, ns, salt;
+  const uint32_t *sorted_ids;
+  uint32_t *pos;
+  int32_t *ids;
+  uint32_t *seed_mesh;
+  const Grid *grids;
+};
+template <class R, int D>
+__global__ void __launch_bounds__(kThreads)
+    k_after_sort(unsigned nb_off, unsigned nb_gather, SortedArgs A, const R *aos,
+                 typename Vec<R>::V4 *p4, typename Vec<R>::V2 *q2) {
+  const unsigned b = blockIdx.x;
+  if (b < nb_off)
+    cell_offsets_block(b, nb_off, A.skeys, A.n, A.total, A.offsets);
+  else if (b < nb_off + nb_gather)
+    gather_sorted_block<R, D>(b - nb_off, aos, A.n, A.grids, A.seg_off, A.meshes, A.sorted_ids,
+                              p4, q2);
+  else
+    pick_sorted_block(b - nb_off - nb_gather, A.seg_off, A.seed_off, A.meshes, A.n, A.ns, A.salt,
+                      A.sorted_ids, A.pos, A.ids, A.seed_mesh);
 }
 
 // Explicit seed positions -> cell keys (to run those seeds in cell order).
@@ -1600,6 +1648,7 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
       ++key_bits;
   }
   uint32_t *sorted_ids = nullptr;
+  const uint32_t *seed_drawn = nullptr;  // seeds drawn in the launch after the sort
   if (n) {
     uint64_t *keys = scratch_as<uint64_t>(s, bank1 ? kSlotMsBankKeys : kSlotMs2, 2 * n);
     uint32_t *vals = scratch_as<uint32_t>(s, bank1 ? kSlotMsBankVals : kSlotMs3, 2 * n);
@@ -1612,22 +1661,33 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
                  pts, n, grids, seg_off, meshes, cellbase, cells_cap, keys, vals, aos);
     radix_sort_pairs_u64(s, keys, vals, keys + n, vals + n, n, key_bits);
     sorted_ids = vals + n;
-    MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets,
-               std::min<unsigned>(blocks_for(n + 1, kThreads), 16384u), kThreads, 0, keys + n, n,
-               total_cells, off);
+    // offsets, gathered records and (drawn seeds) the seeds' positions
+    SortedArgs A{keys + n, total_cells, off, seg_off, seg ? seg->seed_off : nullptr, meshes,
+                 n, ns, salt, sorted_ids, nullptr, seed_ids, seg ? seg->seed_mesh : nullptr,
+                 grids};
+    const unsigned nb_off = std::min<unsigned>(blocks_for(n + 1, kThreads), 16384u),
+                   nb_gather = blocks_for(n, kThreads);
+    unsigned nb_pick = 0;
+    if (seed_ids) {
+      A.pos = scratch_as<uint32_t>(s, slot_seed, ns);
+      nb_pick = blocks_for(ns, kThreads);
+      seed_drawn = A.pos;
+    }
     if (D == 6)
-      MVS_LAUNCH(s, "ms_gather", (k_gather_sorted<R, 6>), blocks_for(n, kThreads), kThreads, 0,
-                 aos, n, grids, seg_off, meshes, sorted_ids, p4, q2);
+      MVS_LAUNCH(s, "ms_after_sort", (k_after_sort<R, 6>), nb_off + nb_gather + nb_pick, kThreads,
+                 0, nb_off, nb_gather, A, aos, p4, q2);
     else
-      MVS_LAUNCH(s, "ms_gather", (k_gather_sorted<R, 4>), blocks_for(n, kThreads), kThreads, 0,
-                 aos, n, grids, seg_off, meshes, sorted_ids, p4, q2);
+      MVS_LAUNCH(s, "ms_after_sort", (k_after_sort<R, 4>), nb_off + nb_gather + nb_pick, kThreads,
+                 0, nb_off, nb_gather, A, aos, p4, q2);
   } else {
     MVS_LAUNCH(s, "ms_cell_offsets", k_cell_offsets, 1, kThreads, 0, nullptr, 0, total_cells,
                off);
   }
   // seeds in cell order
   const uint32_t *order = nullptr, *seed_pos = nullptr;
-  if (seed_ids) {
+  if (seed_drawn) {
+    seed_pos = seed_drawn;
+  } else if (seed_ids) {
     uint32_t *pos = scratch_as<uint32_t>(s, slot_seed, ns);
     MVS_LAUNCH(s, "ms_seeds", k_pick_sorted, blocks_for(ns, kThreads), kThreads, 0, seg_off,
                seg ? seg->seed_off : nullptr, meshes, n, ns, salt, sorted_ids, pos, seed_ids,
