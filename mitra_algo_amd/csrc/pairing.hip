@@ -142,7 +142,7 @@ __global__ void k_gather_samples(const double *x, const double *y, const double 
 __device__ inline void bucket_index_block(
     unsigned blk, const uint32_t *ord, const double *r64, const float *r32, uint64_t S,
                    uint32_t *bord, double2 *bk, float2 *bkf, double4 *pb, float4 *pbf,
-                   double *piv) {
+                   double *piv, float *pivf) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t b = static_cast<uint64_t>(blk) * kWaves + threadIdx.x / kWave;
   const uint64_t nb = (S + kBucket - 1) / kBucket;
@@ -170,8 +170,11 @@ __device__ inline void bucket_index_block(
     }
   // every kPivot-th kmax of the bucket (infinite past the samples), for a
   // search that reads them in one go before one segment of kPivot entries
-  if (lane % kPivot == 0)
+  if (lane % kPivot == 0) {
     piv[b * (kBucket / kPivot) + lane / kPivot] = key;
+    if (pivf)  // rounded as the FP32 record rounds it
+      pivf[b * (kBucket / kPivot) + lane / kPivot] = static_cast<float>(key);
+  }
   if (j < S) {
     const double *r = r64 + static_cast<uint64_t>(o) * kRec;
     bord[j] = o;
@@ -244,15 +247,31 @@ __device__ inline void windows_block(unsigned blk, const double *r64, const uint
 __global__ void __launch_bounds__(kThreads)
     k_index_windows(unsigned index_blocks, const uint32_t *ord, const double *r64,
                     const float *r32, uint64_t S, uint32_t *bord, double2 *bk, float2 *bkf,
-                    double4 *pb, float4 *pbf, double *piv, const uint64_t *skeys, double t,
-                    const uint32_t *seg, uint64_t *win, double2 *kw) {
+                    double4 *pb, float4 *pbf, double *piv, float *pivf, const uint64_t *skeys,
+                    double t, const uint32_t *seg, uint64_t *win, double2 *kw) {
   if (blockIdx.x < index_blocks)
-    bucket_index_block(blockIdx.x, ord, r64, r32, S, bord, bk, bkf, pb, pbf, piv);
+    bucket_index_block(blockIdx.x, ord, r64, r32, S, bord, bk, bkf, pb, pbf, piv, pivf);
   else
     windows_block(blockIdx.x - index_blocks, r64, skeys, S, t, seg, win, kw);
 }
 
-// Exact reference predicate in FP64 (pairing.cpp:22-28).
+// Exact reference predicate in FP64 (pairing.cpp:22-28): the curvature
+// test, then the distance test.
+__device__ inline bool curv_ok64(double a0, double a1, double b0, double b1, double t) {
+  double m = fabs(a0);
+  double c1 = fabs(a1), c2 = fabs(b0), c3 = fabs(b1);
+  m = m < c1 ? c1 : m;
+  m = m < c2 ? c2 : m;
+  m = m < c3 ? c3 : m;
+  m = m < 1e-10 ? 1e-10 : m;
+  double tol = t * m;
+  return !(fabs(a0 - b0) > tol || fabs(a1 - b1) > tol);
+}
+__device__ inline bool dist_ok64(double ax, double ay, double az, double bx, double by, double bz,
+                                 double dist) {
+  double dx = ax - bx, dy = ay - by, dz = az - bz;
+  return !(sqrt(dx * dx + dy * dy + dz * dz) < dist);
+}
 __device__ inline bool accept64(double a0, double a1, double b0, double b1, double ax, double ay,
                                 double az, double bx, double by, double bz, double t,
                                 double dist) {
@@ -301,6 +320,7 @@ struct PairArgs {
   const double4 *pb;   // positions (x, y, z, 0) at bucketed positions
   const float4 *pbf;   // FP32 copy
   const double *piv;   // per bucket: the kmax of every kPivot-th entry
+  const float *pivf;   // FP32 copy
   const uint64_t *win;
   const double2 *kw;
   uint64_t S;
@@ -384,14 +404,15 @@ __device__ inline bool decide(const PairArgs &A, const Source &src, uint64_t j, 
         return d == 1;
     }
     // FP64 recheck: the source's FP64 fields are read here, not kept live
-    // through the walk (rechecks are rare; registers are not)
+    // through the walk, curvatures first and positions after (rechecks are
+    // rare; registers are not)
     ++rechecks;
-    double2 bb = A.bk[j];
-    const double2 *p = reinterpret_cast<const double2 *>(A.r64 + static_cast<uint64_t>(o) * kRec);
-    double2 p0 = p[0], p1 = p[1];
-    const double2 *ps = reinterpret_cast<const double2 *>(A.r64 + src.i * kRec);
-    const double2 s0 = ps[0], s1 = ps[1], s2 = ps[2];
-    return accept64(s1.y, s2.x, bb.x, bb.y, s0.x, s0.y, s1.x, p0.x, p0.y, p1.x, A.t, src.dist);
+    const double *rs = A.r64 + src.i * kRec;
+    const double2 bb = A.bk[j];
+    if (!curv_ok64(rs[L_K0], rs[L_K1], bb.x, bb.y, A.t))
+      return false;
+    const double4 q = A.pb[j];
+    return dist_ok64(rs[L_X], rs[L_Y], rs[L_Z], q.x, q.y, q.z, src.dist);
   }
 }
 
@@ -420,18 +441,84 @@ __device__ inline uint64_t halves_max(uint64_t v) {
   return v > w ? v : w;
 }
 
+// A source's window and kmax interval: loaded one source ahead in the
+// walk, so they arrive beside the current source's first loads (the
+// bucket pivots depend on them; the source's record does not).
+struct SrcPre {
+  uint64_t jlo, jhi;
+  double2 kw;
+};
+__device__ inline SrcPre preload_source(const PairArgs &A, uint64_t ii) {
+  SrcPre r;
+  r.jlo = A.win[ii];
+  r.jhi = A.win[A.S + ii];
+  r.kw = A.kw[ii];
+  return r;
+}
+
+// Lower bound fl = entries with kmax < lo and upper bound fu = entries with
+// kmax <= hi of the kmax-sorted bucket at [base, base + nb), by counting:
+// the kPivot pivots (every kPivot-th entry's kmax) locate each bound's
+// segment, whose other entries are then counted; two rounds of independent
+// loads instead of one dependent load per bisection step.  K = float: the
+// FP32 copies of kmax (rounded to nearest, so still sorted) against bounds
+// rounded outwards, which counts at most as many below lo and at least as
+// many up to hi: the interval holds the exact one.
+__device__ inline float f_down(double x) {
+  const float f = static_cast<float>(x);
+  return static_cast<double>(f) > x ? nextafterf(f, -INFINITY) : f;
+}
+__device__ inline float f_up(double x) {
+  const float f = static_cast<float>(x);
+  return static_cast<double>(f) < x ? nextafterf(f, INFINITY) : f;
+}
+template <class K, class K2>
+__device__ inline void bucket_bounds(const K *pv_in, const K2 *kk, uint64_t base, uint64_t nb,
+                                     K lo, K hi, uint32_t &fl, uint32_t &fu) {
+  constexpr int kSeg = kBucket / kPivot;
+  K pv[kSeg];
+  for (int q = 0; q < kSeg; ++q)
+    pv[q] = pv_in[q];
+  int sl = 0, su = 0;  // pivots below the lower bound / not above the upper
+  for (int q = 0; q < kSeg; ++q) {
+    sl += pv[q] < lo;
+    su += !(pv[q] > hi);
+  }
+  // segment s - 1 holds the bound (entries 1 .. kPivot - 1 of it are
+  // counted; its pivot already was); s = 0: the bound is 0
+  K el[kPivot - 1], eu[kPivot - 1];
+  const uint64_t bl = base + static_cast<uint64_t>(sl > 0 ? sl - 1 : 0) * kPivot;
+  const uint64_t bu = base + static_cast<uint64_t>(su > 0 ? su - 1 : 0) * kPivot;
+  for (int q = 0; q < kPivot - 1; ++q) {
+    el[q] = bl + 1 + q < base + nb ? kk[bl + 1 + q].y : K(INFINITY);
+    eu[q] = bu + 1 + q < base + nb ? kk[bu + 1 + q].y : K(INFINITY);
+  }
+  fl = 0, fu = 0;
+  if (sl > 0) {
+    fl = static_cast<uint32_t>((sl - 1) * kPivot + 1);
+    for (int q = 0; q < kPivot - 1; ++q)
+      fl += bl + 1 + q < base + nb && el[q] < lo;
+  }
+  if (su > 0) {
+    fu = static_cast<uint32_t>((su - 1) * kPivot + 1);
+    for (int q = 0; q < kPivot - 1; ++q)
+      fu += bu + 1 + q < base + nb && !(eu[q] > hi);
+  }
+}
+
 // Walks every candidate of source i in this lane's half (live: the half has
 // a source); f(accepted, ordinal) is called with all lanes active once per
 // candidate chunk of both halves.
 template <bool kGuarded, class F>
-__device__ inline void walk_source(const PairArgs &A, uint64_t i, bool live, HalfRanges &W,
-                                   int lane, unsigned &rechecks, uint64_t &cand2, F &&f) {
+__device__ inline void walk_source(const PairArgs &A, uint64_t i, bool live, const SrcPre &pre,
+                                   HalfRanges &W, int lane, unsigned &rechecks, uint64_t &cand2,
+                                   F &&f) {
   const uint64_t S = A.S;
   const int hl = lane & (kHalf - 1);
-  const uint64_t ii = live ? i : 0;  // loads stay in range for an idle half
-  const uint64_t jlo = A.win[ii], jhi = A.win[S + ii];
+  const uint64_t ii = live ? i : 0;  // loads stay in range for an idle half (pre: for ii)
+  const uint64_t jlo = pre.jlo, jhi = pre.jhi;
   const bool has = live && jlo < jhi;
-  const double2 kw = A.kw[ii];
+  const double2 kw = pre.kw;
   Source src;
   src.i = ii;
   {
@@ -456,40 +543,12 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, bool live, Hal
     if (off + static_cast<uint64_t>(hl) < nbk) {
       const uint64_t base = b * kBucket;
       const uint64_t nb = S - base < static_cast<uint64_t>(kBucket) ? S - base : kBucket;
-      // lower bound fl = entries with kmax < kw.x and upper bound fu =
-      // entries with kmax <= kw.y of the kmax-sorted bucket, by counting:
-      // the kPivot pivots (every kPivot-th entry) locate each bound's
-      // segment, whose other entries are then counted; two rounds of
-      // independent loads instead of one dependent load per bisection step
-      constexpr int kSeg = kBucket / kPivot;
-      double pv[kSeg];
-      for (int q = 0; q < kSeg; ++q)
-        pv[q] = A.piv[b * kSeg + q];
-      int sl = 0, su = 0;  // pivots below the lower bound / not above the upper
-      for (int q = 0; q < kSeg; ++q) {
-        sl += pv[q] < kw.x;
-        su += !(pv[q] > kw.y);
-      }
-      // segment s - 1 holds the bound (entries 1 .. kPivot - 1 of it are
-      // counted; its pivot already was); s = 0: the bound is 0
       uint32_t fl = 0, fu = 0;
-      double el[kPivot - 1], eu[kPivot - 1];
-      const uint64_t bl = base + static_cast<uint64_t>(sl > 0 ? sl - 1 : 0) * kPivot;
-      const uint64_t bu = base + static_cast<uint64_t>(su > 0 ? su - 1 : 0) * kPivot;
-      for (int q = 0; q < kPivot - 1; ++q) {
-        el[q] = bl + 1 + q < base + nb ? A.bk[bl + 1 + q].y : INFINITY;
-        eu[q] = bu + 1 + q < base + nb ? A.bk[bu + 1 + q].y : INFINITY;
-      }
-      if (sl > 0) {
-        fl = static_cast<uint32_t>((sl - 1) * kPivot + 1);
-        for (int q = 0; q < kPivot - 1; ++q)
-          fl += bl + 1 + q < base + nb && el[q] < kw.x;
-      }
-      if (su > 0) {
-        fu = static_cast<uint32_t>((su - 1) * kPivot + 1);
-        for (int q = 0; q < kPivot - 1; ++q)
-          fu += bu + 1 + q < base + nb && !(eu[q] > kw.y);
-      }
+      if constexpr (kGuarded)  // FP32 keys and widened bounds: a superset
+        bucket_bounds(A.pivf + b * (kBucket / kPivot), A.bkf, base, nb, f_down(kw.x),
+                      f_up(kw.y), fl, fu);
+      else
+        bucket_bounds(A.piv + b * (kBucket / kPivot), A.bk, base, nb, kw.x, kw.y, fl, fu);
       // an infinite bound also counts the infinite pivots past a short
       // bucket's end
       fl = fl < nb ? fl : static_cast<uint32_t>(nb);
@@ -555,11 +614,11 @@ __global__ void __launch_bounds__(kThreads)
   HalfRanges &W = ranges[2 * wave + half];
   uint32_t *L = list[2 * wave + half];
   // walks source i in this lane's half (live: the half has one)
-  auto walk_one = [&](uint64_t i, bool live) {
+  auto walk_one = [&](uint64_t i, bool live, const SrcPre &pre) {
     uint32_t cnt = 0;
     uint64_t cand2 = 0;
     unsigned rechecks = 0;
-    walk_source<kGuarded>(A, i, live, W, lane, rechecks, cand2, [&](bool ok, uint32_t o) {
+    walk_source<kGuarded>(A, i, live, pre, W, lane, rechecks, cand2, [&](bool ok, uint32_t o) {
       const uint64_t mask = ballot(ok);
       const uint32_t pos = cnt + half_below(mask, lane);
       if (ok && pos < static_cast<uint32_t>(kSlab))
@@ -586,7 +645,7 @@ __global__ void __launch_bounds__(kThreads)
         unsigned s = atomicAdd(long_count, 1u);
         long_list[s] = static_cast<uint32_t>(i);
       }
-      uint64_t w = A.win[A.S + i] - A.win[i];
+      uint64_t w = pre.jhi - pre.jlo;
       A.cand[i] = static_cast<uint32_t>(w < kNone ? w : kNone);
       A.cand2[i] = static_cast<uint32_t>(cand2 < kNone ? cand2 : kNone);
       A.rechk[i] = rechecks;
@@ -595,12 +654,37 @@ __global__ void __launch_bounds__(kThreads)
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   };
-  if (quota == 0) {
-    const uint64_t i = (static_cast<uint64_t>(blockIdx.x) * kWaves + wave) * 2 + half;
-    const bool live = i < A.S;
-    if (!ballot(live))
+  // source of pair slot q in this lane's half (sample 0 stands in past S)
+  auto source_of = [&](uint64_t q) -> uint64_t {
+    const uint64_t i = 2 * q + static_cast<uint64_t>(half);
+    return i < A.S ? i : 0;
+  };
+  if (quota == 0 && !kGuarded) {  // one pair slot per wave
+    const uint64_t q = static_cast<uint64_t>(blockIdx.x) * kWaves + wave;
+    const uint64_t i = 2 * q + static_cast<uint64_t>(half);
+    if (!ballot(i < A.S))
       return;
-    walk_one(i, live);
+    walk_one(i, i < A.S, preload_source(A, source_of(q)));
+    return;
+  }
+  if (quota == 0) {
+    // static (FP32-guarded walk): wave w of the grid walks pair slots w,
+    // w + waves, ... (two sources each), the next slot's window loading
+    // while the current one is walked
+    const uint64_t waves = static_cast<uint64_t>(gridDim.x) * kWaves;
+    const uint64_t slots = (A.S + 1) / 2;
+    uint64_t q = static_cast<uint64_t>(blockIdx.x) * kWaves + wave;
+    if (q >= slots)
+      return;  // the whole wave
+    SrcPre cur = preload_source(A, source_of(q));
+    for (; q < slots; q += waves) {
+      const uint64_t i = 2 * q + static_cast<uint64_t>(half);
+      SrcPre nxt = cur;
+      if (q + waves < slots)
+        nxt = preload_source(A, source_of(q + waves));
+      walk_one(i, i < A.S, cur);
+      cur = nxt;
+    }
     return;
   }
   for (;;) {
@@ -611,8 +695,10 @@ __global__ void __launch_bounds__(kThreads)
     if (base >= A.S)
       break;
     const uint64_t end = base + quota < A.S ? base + quota : A.S;
-    for (uint64_t t = base; t < end; t += 2)
-      walk_one(t + half, t + half < end);
+    for (uint64_t t = base; t < end; t += 2) {
+      const uint64_t i = t + static_cast<uint64_t>(half);
+      walk_one(i, i < end, preload_source(A, i < end ? i : 0));
+    }
   }
 }
 
@@ -827,6 +913,7 @@ struct PairState {
   double4 *pb;
   float4 *pbf;
   double *piv;
+  float *pivf;
   uint64_t *keys, *win, *counts, *offsets;
   uint32_t *long_list, *cand, *cand2, *rechk;
 };
@@ -841,7 +928,11 @@ static PairState pair_state(Session *s, uint64_t S, bool guarded) {
   st.bkf = guarded ? scratch_as<float2>(s, kSlotPlanSortedK32, Sp) : nullptr;
   st.pb = scratch_as<double4>(s, kSlotPlanPosB, Sp);
   st.pbf = guarded ? scratch_as<float4>(s, kSlotPlanPosB32, Sp) : nullptr;
-  st.piv = scratch_as<double>(s, kSlotPlanPivots, (Sp + kBucket - 1) / kBucket * (kBucket / kPivot));
+  {  // FP64 pivots, then (guarded) their FP32 copy
+    const uint64_t np = (Sp + kBucket - 1) / kBucket * (kBucket / kPivot);
+    st.piv = scratch_as<double>(s, kSlotPlanPivots, np + (np + 1) / 2);
+    st.pivf = guarded ? reinterpret_cast<float *>(st.piv + np) : nullptr;
+  }
   st.bord = scratch_as<uint32_t>(s, kSlotWindows, Sp);
   st.slab = scratch_as<uint32_t>(s, kSlotPlanSlab, kSlab * Sp);
   st.keys = scratch_as<uint64_t>(s, kSlotPlanKeys, 2 * Sp);
@@ -869,6 +960,7 @@ static PairArgs pair_args(const PairState &st, uint64_t S, double t, double box,
   A.pb = st.pb;
   A.pbf = st.pbf;
   A.piv = st.piv;
+  A.pivf = st.pivf;
   A.win = st.win;
   A.kw = st.kw;
   A.S = S;
@@ -943,13 +1035,16 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   const unsigned index_blocks = blocks_for(nb, kWaves);
   MVS_LAUNCH(s, "pair_index", k_index_windows, index_blocks + blocks_for(S, kThreads), kThreads,
              0, index_blocks, st.ord + S, st.r64, st.r32, S, st.bord, st.bk, st.bkf, st.pb,
-             st.pbf, st.piv, st.keys + S, t, segs.seg, st.win, st.kw);
+             st.pbf, st.piv, st.pivf, st.keys + S, t, segs.seg, st.win, st.kw);
   unsigned *long_count = flags + 1;
   unsigned *queue = flags + 12;  // cleared with the flags above
   PairArgs A = pair_args(st, S, t, box, guarded, flags, segs);
   const unsigned quota = s->tune.walk_quota;
-  unsigned grid = blocks_for(S, 2 * kWaves);  // two sources per wave
-  if (quota)  // persistent: a fixed set of resident blocks
+  // two sources per wave; the FP32-guarded walk and the persistent one
+  // (quota > 0) loop on a fixed set of resident blocks (static slots or
+  // claims from the queue), the FP64 walk takes one pair slot per wave
+  unsigned grid = blocks_for(S, 2 * kWaves);
+  if (guarded || quota)
     grid = std::min<unsigned>(grid, s->tune.walk_blocks);
   if (guarded)
     MVS_LAUNCH(s, "pair_walk", k_pair_walk<true>, grid, kThreads, 0, A, st.counts, st.slab,
