@@ -602,7 +602,7 @@ __device__ inline void walk_source(const PairArgs &A, uint64_t i, bool live, con
 // sources of uneven cost (and meshes of uneven size) balance across the
 // chip; every wave's last claim returns a start past the end, so all exit.
 template <bool kGuarded>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads, kGuarded ? 5 : 1)  // guarded: <= 96 registers
     k_pair_walk(PairArgs Ain, uint64_t *counts, uint32_t *slab, uint32_t *long_list,
                 unsigned *long_count, unsigned quota, unsigned *queue) {
   __shared__ HalfRanges ranges[2 * kWaves];
@@ -1041,11 +1041,22 @@ void pair_count_device(Session *s, uint64_t n, const double *x, const double *y,
   PairArgs A = pair_args(st, S, t, box, guarded, flags, segs);
   const unsigned quota = s->tune.walk_quota;
   // two sources per wave; the FP32-guarded walk and the persistent one
-  // (quota > 0) loop on a fixed set of resident blocks (static slots or
-  // claims from the queue), the FP64 walk takes one pair slot per wave
+  // (quota > 0) loop on a fixed set of blocks (static slots: the blocks the
+  // device holds at once; claims from the queue: the tuned count), the FP64
+  // walk takes one pair slot per wave
   unsigned grid = blocks_for(S, 2 * kWaves);
-  if (guarded || quota)
+  if (quota) {
     grid = std::min<unsigned>(grid, s->tune.walk_blocks);
+  } else if (guarded) {
+    if (!s->walk_resident) {
+      int per_cu = 0, cus = 0;
+      MVS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair_walk<true>, kThreads,
+                                                           0));
+      MVS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
+      s->walk_resident = static_cast<unsigned>(std::max(1, per_cu) * std::max(1, cus));
+    }
+    grid = std::min<unsigned>(grid, s->walk_resident);
+  }
   if (guarded)
     MVS_LAUNCH(s, "pair_walk", k_pair_walk<true>, grid, kThreads, 0, A, st.counts, st.slab,
                st.long_list, long_count, quota, queue);

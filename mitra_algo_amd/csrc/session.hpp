@@ -69,6 +69,7 @@ struct Arena {
 
 struct Session {
   int device = 0;
+  unsigned walk_resident = 0;  // blocks of the static pair walk the device holds at once
   Arena *arena = nullptr;  // set for the duration of an allocation-free call
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
