@@ -9,19 +9,22 @@
 //   w = exp(-d^2 / 2h^2) - exp(-cutoff^2 / 2)   for d < cutoff*h,
 // so the support boundary carries no weight jump.
 //
-// Grid: cells of side >= cutoff*h / reach over the three coordinates of
-// largest variance (for height fields: the two horizontal translation
-// components and the rotation about z for rigid votes, plane offset and the
-// horizontal normal components for mirror votes), so a cell holds few points
-// outside the support; reach is 2 (half-width cells) when that grid fits in
-// 2^22 cells, else 1.  Points are radix-sorted by cell (stable: sums are
+// Grid: over the three coordinates of largest variance (for height fields:
+// the two horizontal translation components and the rotation about z for
+// rigid votes, plane offset and the horizontal normal components for mirror
+// votes), cells of side >= cutoff*h / reach across the rows and
+// cutoff*h / xdiv along them (first axis), so a cell holds few points
+// outside the support: reach 3 and xdiv 3 for queries with a block per
+// seed, reach 1 and xdiv 4 for the narrow ones, smaller when the grid would
+// exceed its cell bound.  Points are radix-sorted by cell (stable: sums are
 // deterministic) and indexed by per-cell offsets, each sorted point writing
-// the run of cells that starts at it (no atomics).  With the first grid axis fastest, the 2 reach + 1 cells of one
-// row are contiguous, so a query visits (2 reach + 1)^2 contiguous point
-// ranges (9 or 25).  Queries run all iterations in registers and reduce
-// with shuffles.  Seeds run in cell order and blocks are mapped so that each
-// XCD takes a contiguous run of seeds: neighbouring queries share that XCD's
-// L2.
+// the run of cells that starts at it (no atomics).  With the first grid axis
+// fastest, the cells of one row are contiguous, so a query visits (2 reach +
+// 1)^2 contiguous point ranges (49 or 9), each cut to the x-cells its slice
+// of the support ball reaches.  Queries run all iterations in registers and
+// reduce with shuffles.  Seeds run in cell order and blocks are mapped so
+// that each XCD takes a contiguous run of seeds: neighbouring queries share
+// that XCD's L2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
