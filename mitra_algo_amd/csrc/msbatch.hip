@@ -1629,9 +1629,10 @@ static void ms_batched(Session *s, int D, uint64_t n, const R *pts, uint64_t ns,
   }
   uint64_t *ncell = seg ? scratch_as<uint64_t>(s, kSlotBatchMs2, meshes + 1) : nullptr;
   // few seeds: four waves per seed, so the launch still fills the chip.  A
-  // seed with a block of its own walks finer cells (reach 3, rows cut to the
-  // support ball); with one wave per seed the per-row cost of finer cells
-  // outweighs the candidates they save (reach 2).
+  // seed with a block of its own walks finer cells (reach 3, 49 rows); a
+  // narrow seed (16 to 64 lanes) pays for every row, so it walks 9 rows of
+  // cells cutoff*h wide, a quarter of that along the rows (reach 1, xdiv 4).
+  // Both cut each row to the support ball.
   const bool wide = ns < s->tune.wide_seeds;
   const int max_reach = wide ? kMaxReach : kNarrowReach;
   const int xdiv = wide ? kMaxReach : kNarrowXdiv;
