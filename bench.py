@@ -1042,6 +1042,22 @@ def main():
         out["cpu_baseline"] = cpu_baseline(
             wl, {"rigid": (rp.cpu().numpy(), ms32.rigid_seeds),
                  "mirror": (fp_.cpu().numpy(), ms32.refl_seeds)}, votes=V)
+    # the same FP32-guarded step captured as one HIP graph and replayed over
+    # the resident inputs (a serving loop's view); `value` stays the eager step
+    gpv, _ = wl.capture(capi.FP32_GUARDED, SEEDS, ITERS)
+    for _ in range(args.warmup):
+        wl.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        wl.replay()
+    torch.cuda.synchronize()
+    dtg = (time.perf_counter() - t0) / args.steps
+    wl.sess.graph_check()
+    out["graph_replay_fp32g"] = {"ms_per_step": dtg * 1e3, "votes_per_s": V / dtg,
+                                 "speedup_over_eager": dt32 / dtg,
+                                 "same_vote_count": gpv.pairs == P and gpv.reflections == R}
+    print(f"[bench] graph replay {dtg * 1e3:.3f} ms per step", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 
