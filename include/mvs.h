@@ -542,7 +542,7 @@ int mvs_session_set_tuning(mvs_session *s, const mvs_tune_entry *in);
  * mvs_bvh_build keeps a BVH of the mesh surface on the session: median split
  * of face centroids on the widest box axis, at most 8 faces per leaf, as the
  * reference's verification surface (src/cpu/surface.h:12-96); coordinates
- * relative to `center`, in FP32.  Host arrays in.
+ * relative to `center`, in FP32; 1 to 2^27 faces.  Host arrays in.
  * mvs_dev_closest_points: for n centre-relative FP32 points (3 x n SoA) the
  * closest surface point q (3 x n), the unit normal of its face (3 x n), the
  * squared distance and the face id (optional); device arrays.

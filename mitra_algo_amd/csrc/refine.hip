@@ -27,21 +27,35 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kLeaf = 8;
-constexpr int kStack = 32;  // median split: depth <= log2(faces / 8) + 1 < 32
+constexpr uint64_t kMaxFaces = 1ULL << 27;  // leaf code first * 16 + count < 2^31
 
-// BVH node: bounds, then (left, right) for an inner node or
-// (-(first + 1), end) into the leaf-ordered faces for a leaf.
-struct alignas(16) Node {
-  float lo[3];
-  int a;
-  float hi[3];
-  int b;
+// Inner node of the tree: the boxes and references of both children in one
+// 64-byte line, so a visit is one load.  A reference r >= 0 names an inner
+// record; r < 0 a leaf, with code -1 - r = first * 16 + count (count <= 8
+// faces from `first` in leaf order).
+struct alignas(16) Inner {
+  float lo0[3];
+  int ref0;
+  float hi0[3];
+  int ref1;
+  float lo1[3];
+  int pad0;
+  float hi1[3];
+  int pad1;
 };
+static_assert(sizeof(Inner) == 64, "one 64-byte line per inner node");
 
-__device__ inline float box_dist2(const Node &n, float px, float py, float pz) {
-  float dx = fmaxf(fmaxf(n.lo[0] - px, 0.f), px - n.hi[0]);
-  float dy = fmaxf(fmaxf(n.lo[1] - py, 0.f), py - n.hi[1]);
-  float dz = fmaxf(fmaxf(n.lo[2] - pz, 0.f), pz - n.hi[2]);
+// Face in leaf order: corners a, b, c and the unit normal (48 bytes).
+struct alignas(16) Face {
+  float a[3], b[3], c[3], n[3];
+};
+static_assert(sizeof(Face) == 48, "three 16-byte loads per face");
+
+__device__ inline float box_dist2(const float *lo, const float *hi, float px, float py,
+                                  float pz) {
+  float dx = fmaxf(fmaxf(lo[0] - px, 0.f), px - hi[0]);
+  float dy = fmaxf(fmaxf(lo[1] - py, 0.f), py - hi[1]);
+  float dz = fmaxf(fmaxf(lo[2] - pz, 0.f), pz - hi[2]);
   return dx * dx + dy * dy + dz * dz;
 }
 
@@ -103,64 +117,104 @@ __device__ inline P3 closest_on_triangle(P3 p, P3 a, P3 b, P3 c) {
   return add(a, add(mul(ab, vb * den), mul(ac, vc * den)));
 }
 
-// Nearest surface point to p: depth-first, nearer child first, subtrees
-// whose box is farther than the best distance skipped; `stack` is this
-// thread's column of an LDS stack (stride kThreads).
+__device__ inline float face_d2(P3 p, const Face &f, P3 &c) {
+  c = closest_on_triangle(p, p3(f.a[0], f.a[1], f.a[2]), p3(f.b[0], f.b[1], f.b[2]),
+                          p3(f.c[0], f.c[1], f.c[2]));
+  const P3 e = sub(p, c);
+  return dot3(e, e);
+}
+
+// Nearest surface point to p: depth-first, nearer child first (the left one
+// on a tie), subtrees whose box is farther than the best distance skipped.
+// A stack entry carries its box distance, so an entry that the best hit has
+// overtaken is dropped without a load.  The search alternates between
+// descending to the next leaf and scanning it, so the lanes of a wave do the
+// same kind of work at a time.  stk, bnd: this thread's columns of the LDS
+// stack (stride kThreads, depth + 1 entries: one waiting sibling per level
+// plus the two children just pushed).
 struct Hit {
   P3 q;
   float d2;
   int k;  // leaf-ordered face, -1 when none
 };
 // h: the best hit known before the search (d2 = INFINITY: none).
-__device__ inline Hit nearest(const Node *nodes, const float *tri, P3 p, int *stack, Hit h) {
-  int top = 0;
-  stack[0] = 0;
-  ++top;
-  while (top > 0) {
-    const Node nd = nodes[stack[--top * kThreads]];
-    if (box_dist2(nd, p.x, p.y, p.z) > h.d2)
-      continue;
-    if (nd.a < 0) {
-      for (int k = -nd.a - 1; k < nd.b; ++k) {
-        const float *t = tri + 9 * static_cast<uint64_t>(k);
-        P3 c = closest_on_triangle(p, p3(t[0], t[1], t[2]), p3(t[3], t[4], t[5]),
-                                   p3(t[6], t[7], t[8]));
-        P3 e = sub(p, c);
-        float d = dot3(e, e);
-        if (d < h.d2) {
-          h.d2 = d;
-          h.q = c;
-          h.k = k;
-        }
+__device__ inline Hit nearest(const Inner *inner, const Face *faces, int root, P3 p, int *stk,
+                              float *bnd, Hit h) {
+  int top = 1;
+  stk[0] = root;
+  bnd[0] = -INFINITY;
+  while (true) {
+    int leaf = 0;
+    while (top > 0) {
+      --top;
+      const int ref = stk[top * kThreads];
+      if (bnd[top * kThreads] > h.d2)
+        continue;
+      if (ref < 0) {
+        leaf = ref;
+        break;
       }
-    } else {
-      const Node l = nodes[nd.a], r = nodes[nd.b];
-      const float dl = box_dist2(l, p.x, p.y, p.z), dr = box_dist2(r, p.x, p.y, p.z);
+      const Inner nd = inner[ref];
+      const float d0 = box_dist2(nd.lo0, nd.hi0, p.x, p.y, p.z),
+                  d1 = box_dist2(nd.lo1, nd.hi1, p.x, p.y, p.z);
       // push the farther child first so the nearer one is searched first
-      const int near = dl <= dr ? nd.a : nd.b, far = dl <= dr ? nd.b : nd.a;
-      const float dnear = dl <= dr ? dl : dr, dfar = dl <= dr ? dr : dl;
-      if (dfar <= h.d2 && top < kStack)
-        stack[top++ * kThreads] = far;
-      if (dnear <= h.d2 && top < kStack)
-        stack[top++ * kThreads] = near;
+      const bool swap = d0 > d1;
+      const int near = swap ? nd.ref1 : nd.ref0, far = swap ? nd.ref0 : nd.ref1;
+      const float dnear = swap ? d1 : d0, dfar = swap ? d0 : d1;
+      if (dfar <= h.d2) {
+        stk[top * kThreads] = far;
+        bnd[top * kThreads] = dfar;
+        ++top;
+      }
+      if (dnear <= h.d2) {
+        stk[top * kThreads] = near;
+        bnd[top * kThreads] = dnear;
+        ++top;
+      }
+    }
+    if (leaf == 0)
+      break;
+    const int code = -1 - leaf, first = code >> 4, end = first + (code & 15);
+    for (int k = first; k < end; ++k) {
+      P3 c;
+      const float d = face_d2(p, faces[k], c);
+      if (d < h.d2) {
+        h.d2 = d;
+        h.q = c;
+        h.k = k;
+      }
     }
   }
   return h;
 }
 
+// The LDS stack of a block: `depth` references, then `depth` bounds, per
+// thread (dynamic shared memory of depth * kThreads * 8 bytes).
+struct Stack {
+  int *stk;
+  float *bnd;
+  __device__ explicit Stack(int depth) {
+    extern __shared__ int stack_lds[];
+    stk = stack_lds + threadIdx.x;
+    bnd = reinterpret_cast<float *>(stack_lds + depth * kThreads) + threadIdx.x;
+  }
+};
+
 // One thread per query point (3 x n SoA).
 __global__ void __launch_bounds__(kThreads)
-    k_closest(const Node *nodes, const float *tri, const float *fnrm, const int32_t *face_id,
-              uint64_t n, const float *pts, float *q, float *nrm, float *dist2, int32_t *face) {
-  __shared__ int stack[kStack * kThreads];
+    k_closest(const Inner *inner, const Face *faces, int root, int depth,
+              const int32_t *face_id, uint64_t n, const float *pts, float *q, float *nrm,
+              float *dist2, int32_t *face) {
+  const Stack st(depth);
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
   const P3 p = p3(pts[i], pts[n + i], pts[2 * n + i]);
-  const Hit h = nearest(nodes, tri, p, stack + threadIdx.x, Hit{p, INFINITY, -1});
+  const Hit h = nearest(inner, faces, root, p, st.stk, st.bnd, Hit{p, INFINITY, -1});
   q[i] = h.q.x, q[n + i] = h.q.y, q[2 * n + i] = h.q.z;
   if (h.k >= 0) {
-    nrm[i] = fnrm[3 * h.k], nrm[n + i] = fnrm[3 * h.k + 1], nrm[2 * n + i] = fnrm[3 * h.k + 2];
+    const float *fn = faces[h.k].n;
+    nrm[i] = fn[0], nrm[n + i] = fn[1], nrm[2 * n + i] = fn[2];
   } else {
     nrm[i] = nrm[n + i] = nrm[2 * n + i] = 0.f;
   }
@@ -239,12 +293,12 @@ template <int KIND> struct ModeTransform {
 // and dist2 (modes x npts).  Blocks never span two modes.
 template <int KIND>
 __global__ void __launch_bounds__(kThreads)
-    k_closest_modes(const Node *nodes, const float *tri, const float *fnrm, uint32_t modes,
+    k_closest_modes(const Inner *inner, const Face *faces, int root, int depth, uint32_t modes,
                     uint32_t npts, unsigned blocks_per_mode, const float *theta,
                     const float *base, const float *pts, uint64_t pts_stride, float *q,
                     float *nrm, float *dist2, int32_t *face, const int32_t *face_id,
                     const int32_t *leaf_of, uint32_t nfaces) {
-  __shared__ int stack[kStack * kThreads];
+  const Stack st(depth);
   const uint32_t m = blockIdx.x / blocks_per_mode;
   const uint64_t j = static_cast<uint64_t>(blockIdx.x % blocks_per_mode) * kThreads + threadIdx.x;
   if (m >= modes || j >= npts)
@@ -260,19 +314,17 @@ __global__ void __launch_bounds__(kThreads)
     const int32_t f = face[fo];
     if (f >= 0 && static_cast<uint32_t>(f) < nfaces) {
       const int k = leaf_of[f];
-      const float *t = tri + 9 * static_cast<uint64_t>(k);
-      const P3 c = closest_on_triangle(p, p3(t[0], t[1], t[2]), p3(t[3], t[4], t[5]),
-                                       p3(t[6], t[7], t[8]));
-      const P3 e = sub(p, c);
-      init = Hit{c, dot3(e, e), k};
+      P3 c;
+      const float d = face_d2(p, faces[k], c);
+      init = Hit{c, d, k};
     }
   }
-  const Hit h = nearest(nodes, tri, p, stack + threadIdx.x, init);
+  const Hit h = nearest(inner, faces, root, p, st.stk, st.bnd, init);
   if (face)
     face[fo] = h.k >= 0 ? face_id[h.k] : -1;
   const uint64_t o = static_cast<uint64_t>(m) * 3 * npts + j;
   q[o] = h.q.x, q[o + npts] = h.q.y, q[o + 2 * static_cast<uint64_t>(npts)] = h.q.z;
-  const float *f = h.k >= 0 ? fnrm + 3 * h.k : nullptr;
+  const float *f = h.k >= 0 ? faces[h.k].n : nullptr;
   nrm[o] = f ? f[0] : 0.f, nrm[o + npts] = f ? f[1] : 0.f;
   nrm[o + 2 * static_cast<uint64_t>(npts)] = f ? f[2] : 0.f;
   dist2[fo] = h.d2;
@@ -405,8 +457,8 @@ struct BuildNode {
 static void bvh_build(Session *s, uint64_t nv, const double *x, const double *y, const double *z,
                uint64_t nf, const int32_t *f0, const int32_t *f1, const int32_t *f2,
                const double *center) {
-  if (nf == 0 || nf > static_cast<uint64_t>(INT32_MAX / 2))
-    raise(MVS_EINVAL, "a BVH needs between 1 and 2^30 faces");
+  if (nf == 0 || nf > kMaxFaces)
+    raise(MVS_EINVAL, "a BVH needs between 1 and 2^27 faces");
   for (uint64_t f = 0; f < nf; ++f)
     for (int32_t v : {f0[f], f1[f], f2[f]})
       if (v < 0 || static_cast<uint64_t>(v) >= nv)
@@ -430,14 +482,16 @@ static void bvh_build(Session *s, uint64_t nv, const double *x, const double *y,
   nodes.reserve(2 * (nf / kLeaf + 1));
   // iterative build: (node index, begin, end)
   struct Job {
-    int node, begin, end;
+    int node, begin, end, level;
   };
   std::vector<Job> jobs;
   nodes.emplace_back();
-  jobs.push_back({0, 0, static_cast<int>(nf)});
+  jobs.push_back({0, 0, static_cast<int>(nf), 0});
+  int depth = 0;
   while (!jobs.empty()) {
     Job jb = jobs.back();
     jobs.pop_back();
+    depth = std::max(depth, jb.level);
     BuildNode nd;
     for (int k = 0; k < 3; ++k)
       nd.lo[k] = INFINITY, nd.hi[k] = -INFINITY;
@@ -466,31 +520,44 @@ static void bvh_build(Session *s, uint64_t nv, const double *x, const double *y,
       nd.right = nd.left + 1;
       nodes.emplace_back();
       nodes.emplace_back();
-      jobs.push_back({nd.right, mid, jb.end});
-      jobs.push_back({nd.left, jb.begin, mid});
+      jobs.push_back({nd.right, mid, jb.end, jb.level + 1});
+      jobs.push_back({nd.left, jb.begin, mid, jb.level + 1});
     }
     nodes[jb.node] = nd;
   }
-  std::vector<Node> dn(nodes.size());
+  // inner records numbered in build order (the root first); a leaf is a code
+  std::vector<int> inner_of(nodes.size(), -1);
+  int ninner = 0;
+  for (size_t i = 0; i < nodes.size(); ++i)
+    if (nodes[i].left >= 0)
+      inner_of[i] = ninner++;
+  auto ref = [&](int i) {
+    const BuildNode &b = nodes[i];
+    return b.left >= 0 ? inner_of[i] : -1 - (b.begin * 16 + (b.end - b.begin));
+  };
+  std::vector<Inner> dn(std::max(ninner, 1));
+  std::memset(dn.data(), 0, dn.size() * sizeof(Inner));
   for (size_t i = 0; i < nodes.size(); ++i) {
     const BuildNode &b = nodes[i];
+    if (b.left < 0)
+      continue;
+    Inner &r = dn[inner_of[i]];
+    const BuildNode &l = nodes[b.left], &h = nodes[b.right];
     for (int k = 0; k < 3; ++k)
-      dn[i].lo[k] = b.lo[k], dn[i].hi[k] = b.hi[k];
-    if (b.left >= 0)
-      dn[i].a = b.left, dn[i].b = b.right;
-    else
-      dn[i].a = -(b.begin + 1), dn[i].b = b.end;
+      r.lo0[k] = l.lo[k], r.hi0[k] = l.hi[k], r.lo1[k] = h.lo[k], r.hi1[k] = h.hi[k];
+    r.ref0 = ref(b.left);
+    r.ref1 = ref(b.right);
   }
-  std::vector<float> tri(9 * nf), fn(3 * nf);
+  std::vector<Face> tri(nf);
   std::vector<int32_t> fid(nf), leaf_of(nf);
   for (uint64_t k = 0; k < nf; ++k) {
     const int f = ids[k];
     fid[k] = f;
     leaf_of[f] = static_cast<int32_t>(k);
     const int32_t v[3] = {f0[f], f1[f], f2[f]};
+    float *corner[3] = {tri[k].a, tri[k].b, tri[k].c};
     for (int c = 0; c < 3; ++c)
-      tri[9 * k + 3 * c] = vx[v[c]], tri[9 * k + 3 * c + 1] = vy[v[c]],
-                   tri[9 * k + 3 * c + 2] = vz[v[c]];
+      corner[c][0] = vx[v[c]], corner[c][1] = vy[v[c]], corner[c][2] = vz[v[c]];
     // unit face normal from the FP64 coordinates
     const double ax = x[v[1]] - x[v[0]], ay = y[v[1]] - y[v[0]], az = z[v[1]] - z[v[0]];
     const double bx = x[v[2]] - x[v[0]], by = y[v[2]] - y[v[0]], bz = z[v[2]] - z[v[0]];
@@ -498,23 +565,30 @@ static void bvh_build(Session *s, uint64_t nv, const double *x, const double *y,
     const double l = std::sqrt(cx * cx + cy * cy + cz * cz);
     if (l > 0)
       cx /= l, cy /= l, cz /= l;
-    fn[3 * k] = static_cast<float>(cx), fn[3 * k + 1] = static_cast<float>(cy),
-           fn[3 * k + 2] = static_cast<float>(cz);
+    tri[k].n[0] = static_cast<float>(cx), tri[k].n[1] = static_cast<float>(cy),
+    tri[k].n[2] = static_cast<float>(cz);
   }
-  s->bvh_nodes.alloc(dn.size() * sizeof(Node));
-  s->bvh_tri.alloc(tri.size() * sizeof(float));
-  s->bvh_fn.alloc(fn.size() * sizeof(float));
+  s->bvh_nodes.alloc(dn.size() * sizeof(Inner));
+  s->bvh_tri.alloc(tri.size() * sizeof(Face));
   s->bvh_fid.alloc(fid.size() * sizeof(int32_t));
   s->bvh_leaf_of.alloc(leaf_of.size() * sizeof(int32_t));
   MVS_HIP(hipMemcpy(s->bvh_leaf_of.ptr, leaf_of.data(), leaf_of.size() * sizeof(int32_t),
                     hipMemcpyHostToDevice));
-  MVS_HIP(hipMemcpy(s->bvh_nodes.ptr, dn.data(), dn.size() * sizeof(Node), hipMemcpyHostToDevice));
-  MVS_HIP(hipMemcpy(s->bvh_tri.ptr, tri.data(), tri.size() * sizeof(float),
+  MVS_HIP(hipMemcpy(s->bvh_nodes.ptr, dn.data(), dn.size() * sizeof(Inner),
                     hipMemcpyHostToDevice));
-  MVS_HIP(hipMemcpy(s->bvh_fn.ptr, fn.data(), fn.size() * sizeof(float), hipMemcpyHostToDevice));
+  MVS_HIP(hipMemcpy(s->bvh_tri.ptr, tri.data(), tri.size() * sizeof(Face),
+                    hipMemcpyHostToDevice));
   MVS_HIP(hipMemcpy(s->bvh_fid.ptr, fid.data(), fid.size() * sizeof(int32_t),
                     hipMemcpyHostToDevice));
   s->bvh_faces = nf;
+  s->bvh_root = ref(0);
+  // one waiting sibling per level plus the two children just pushed
+  s->bvh_stack = depth + 1;
+}
+
+// Dynamic LDS of a closest-point launch: the stack of every thread.
+static size_t stack_lds(const Session *s) {
+  return static_cast<size_t>(s->bvh_stack) * kThreads * (sizeof(int) + sizeof(float));
 }
 
 }  // namespace mvs
@@ -542,10 +616,9 @@ int mvs_dev_closest_points(mvs_session *sp, uint64_t n, const float *pts, float 
     if (n && (!pts || !q || !nrm || !dist2))
       raise(MVS_EINVAL, "null argument");
     if (n)
-      MVS_LAUNCH(s, "closest", k_closest, blocks_for(n, kThreads), kThreads, 0,
-                 reinterpret_cast<const Node *>(s->bvh_nodes.ptr),
-                 reinterpret_cast<const float *>(s->bvh_tri.ptr),
-                 reinterpret_cast<const float *>(s->bvh_fn.ptr),
+      MVS_LAUNCH(s, "closest", k_closest, blocks_for(n, kThreads), kThreads, stack_lds(s),
+                 reinterpret_cast<const Inner *>(s->bvh_nodes.ptr),
+                 reinterpret_cast<const Face *>(s->bvh_tri.ptr), s->bvh_root, s->bvh_stack,
                  reinterpret_cast<const int32_t *>(s->bvh_fid.ptr), n, pts, q, nrm, dist2, face);
   });
 }
@@ -569,19 +642,20 @@ int mvs_dev_closest_modes(mvs_session *sp, int kind, uint32_t modes, uint32_t np
     if (static_cast<uint64_t>(per) * modes > 0x7fffffffULL)
       raise(MVS_EOVERFLOW, "too many closest-point queries for one launch");
     const uint64_t stride = shared_pts ? 0 : 3ULL * npts;
-    const Node *nodes = reinterpret_cast<const Node *>(s->bvh_nodes.ptr);
-    const float *tri = reinterpret_cast<const float *>(s->bvh_tri.ptr);
-    const float *fn = reinterpret_cast<const float *>(s->bvh_fn.ptr);
+    const Inner *inner = reinterpret_cast<const Inner *>(s->bvh_nodes.ptr);
+    const Face *faces = reinterpret_cast<const Face *>(s->bvh_tri.ptr);
+    const int root = s->bvh_root, depth = s->bvh_stack;
+    const size_t lds = stack_lds(s);
     const int32_t *fid = reinterpret_cast<const int32_t *>(s->bvh_fid.ptr);
     const int32_t *leaf_of = reinterpret_cast<const int32_t *>(s->bvh_leaf_of.ptr);
     const uint32_t nf = static_cast<uint32_t>(s->bvh_faces);
     if (kind == 0)
-      MVS_LAUNCH(s, "closest", k_closest_modes<0>, per * modes, kThreads, 0, nodes, tri, fn,
-                 modes, npts, per, theta, base, pts, stride, q, nrm, dist2, face, fid, leaf_of,
+      MVS_LAUNCH(s, "closest", k_closest_modes<0>, per * modes, kThreads, lds, inner, faces,
+                 root, depth, modes, npts, per, theta, base, pts, stride, q, nrm, dist2, face, fid, leaf_of,
                  nf);
     else
-      MVS_LAUNCH(s, "closest", k_closest_modes<1>, per * modes, kThreads, 0, nodes, tri, fn,
-                 modes, npts, per, theta, base, pts, stride, q, nrm, dist2, face, fid, leaf_of,
+      MVS_LAUNCH(s, "closest", k_closest_modes<1>, per * modes, kThreads, lds, inner, faces,
+                 root, depth, modes, npts, per, theta, base, pts, stride, q, nrm, dist2, face, fid, leaf_of,
                  nf);
   });
 }

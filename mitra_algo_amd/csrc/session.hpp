@@ -123,8 +123,9 @@ struct Session {
   hipEvent_t graph_events[2] = {nullptr, nullptr};  // orders replays with the caller's stream
   void drop_graph();
   // mesh surface for closest-point queries (centre-relative FP32 BVH)
-  DBuf<unsigned char> bvh_nodes, bvh_tri, bvh_fn, bvh_fid, bvh_leaf_of;
+  DBuf<unsigned char> bvh_nodes, bvh_tri, bvh_fid, bvh_leaf_of;
   uint64_t bvh_faces = 0;
+  int bvh_root = 0, bvh_stack = 1;  // root reference; LDS stack entries per thread
   // outputs of the last fused pair + vote call (session-owned buffers)
   mvs_pairvote_result pairvote{};
   bool pairvote_valid = false;
