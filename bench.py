@@ -406,19 +406,19 @@ def bench_refine(args):
 
     def step():
         reset()
-        return [refine.refine(p, steps=iters) for p in probs]
+        return refine.refine_all(probs, steps=iters)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    wl.sess.enable_timing(True, only=("closest", "align"))
+    wl.sess.enable_timing(True, only=("closest", "align", "align_sum"))
     t0 = time.perf_counter()
     for _ in range(args.steps):
         hist = step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     kern = {}
-    for k in ("closest", "align"):
+    for k in ("closest", "align", "align_sum"):
         ms_, n = wl.sess.timing(k)
         if n:
             kern[k] = {"ms_total_per_step": ms_ / args.steps, "ms_avg": ms_ / n,

@@ -348,18 +348,27 @@ template <int NP> __device__ inline void block_sum(float *v, float *sh) {
   }
 }
 
-// One block per mode: residuals of its npts points, E, g (NP) and the upper
-// triangle of H (NP (NP + 1) / 2), summed in FP32 with a fixed order.
+// Points of one mode per block of the energy kernel (four per thread).
+constexpr uint32_t kAlignChunk = 4 * kThreads;
+
+template <int KIND> struct AlignSizes {
+  static constexpr int NP = KIND == 0 ? 6 : 4;  // parameters
+  static constexpr int NH = NP * (NP + 1) / 2;  // upper triangle of H
+  static constexpr int NV = 1 + NP + NH;        // E, g, H
+};
+
+// Block (m, c) of `chunks` per mode: the residuals of points
+// [c kAlignChunk, (c + 1) kAlignChunk) of mode m, and their E, g (NP) and
+// upper triangle of H (NP (NP + 1) / 2) summed in FP32 in a fixed order into
+// part[(m chunks + c) NV ...]; k_align_sum adds the chunks in order.
 template <int KIND>
 __global__ void __launch_bounds__(kThreads)
-    k_align(uint32_t modes, uint32_t npts, const float *theta, const float *base,
-            const float *pts, const float *q, const float *nrm, float *energy, float *grad,
-            float *hess) {
-  constexpr int NP = KIND == 0 ? 6 : 4;
-  constexpr int NH = NP * (NP + 1) / 2;
-  constexpr int NV = 1 + NP + NH;
+    k_align(uint32_t modes, uint32_t npts, uint32_t chunks, const float *theta,
+            const float *base, const float *pts, const float *q, const float *nrm,
+            float *part) {
+  constexpr int NP = AlignSizes<KIND>::NP, NV = AlignSizes<KIND>::NV;
   __shared__ float sh[(kThreads / kWave) * NV];
-  const uint32_t m = blockIdx.x;
+  const uint32_t m = blockIdx.x / chunks, c = blockIdx.x % chunks;
   if (m >= modes)
     return;
   const float *th = theta + static_cast<uint64_t>(m) * NP;
@@ -387,7 +396,8 @@ __global__ void __launch_bounds__(kThreads)
     u = mul(mm, inv_m);
     d = th[3];
   }
-  for (uint32_t j = threadIdx.x; j < npts; j += kThreads) {
+  const uint32_t j1 = min(npts, (c + 1) * kAlignChunk);
+  for (uint32_t j = c * kAlignChunk + threadIdx.x; j < j1; j += kThreads) {
     const P3 p = p3(px[j], py[j], pz[j]);
     const P3 qq = p3(qx[j], qy[j], qz[j]);
     const P3 n = p3(nx[j], ny[j], nz[j]);
@@ -429,18 +439,39 @@ __global__ void __launch_bounds__(kThreads)
   }
   block_sum<NV>(acc, sh);
   if (threadIdx.x == 0) {
-    energy[m] = acc[0];
-    for (int k = 0; k < NP; ++k)
-      grad[static_cast<uint64_t>(m) * NP + k] = acc[1 + k];
-    if (hess) {
-      int h = 1 + NP;
-      for (int a = 0; a < NP; ++a)
-        for (int b = a; b < NP; ++b) {
-          hess[(static_cast<uint64_t>(m) * NP + a) * NP + b] = acc[h];
-          hess[(static_cast<uint64_t>(m) * NP + b) * NP + a] = acc[h];
-          ++h;
-        }
-    }
+    float *out = part + static_cast<uint64_t>(blockIdx.x) * NV;
+    for (int k = 0; k < NV; ++k)
+      out[k] = acc[k];
+  }
+}
+
+// One wave per mode: lane k < NV adds term k of the mode's chunk partials in
+// chunk order and writes it to E, g or both mirrored entries of H.
+template <int KIND>
+__global__ void __launch_bounds__(kWave)
+    k_align_sum(uint32_t modes, uint32_t chunks, const float *part, float *energy, float *grad,
+                float *hess) {
+  constexpr int NP = AlignSizes<KIND>::NP, NV = AlignSizes<KIND>::NV;
+  const uint32_t m = blockIdx.x;
+  const int k = threadIdx.x;
+  if (m >= modes || k >= NV)
+    return;
+  const float *pm = part + static_cast<uint64_t>(m) * chunks * NV + k;
+  float v = 0.f;
+  for (uint32_t c = 0; c < chunks; ++c)
+    v += pm[static_cast<uint64_t>(c) * NV];
+  if (k == 0) {
+    energy[m] = v;
+  } else if (k <= NP) {
+    grad[static_cast<uint64_t>(m) * NP + k - 1] = v;
+  } else if (hess) {
+    // term k is H[a][b], a <= b, in row order of the upper triangle
+    int h = k - 1 - NP, a = 0;
+    while (h >= NP - a)
+      h -= NP - a, ++a;
+    const int b = a + h;
+    hess[(static_cast<uint64_t>(m) * NP + a) * NP + b] = v;
+    hess[(static_cast<uint64_t>(m) * NP + b) * NP + a] = v;
   }
 }
 
@@ -673,12 +704,23 @@ int mvs_dev_align_energy(mvs_session *sp, int kind, uint32_t modes, uint32_t npt
       raise(MVS_EINVAL, "null argument");
     if (modes == 0)
       return;
-    if (kind == 0)
-      MVS_LAUNCH(s, "align", k_align<0>, modes, kThreads, 0, modes, npts, theta, base, pts, q,
-                 nrm, energy, grad, hess);
-    else
-      MVS_LAUNCH(s, "align", k_align<1>, modes, kThreads, 0, modes, npts, theta, base, pts, q,
-                 nrm, energy, grad, hess);
+    const uint32_t chunks = std::max<uint32_t>(1, (npts + kAlignChunk - 1) / kAlignChunk);
+    if (static_cast<uint64_t>(modes) * chunks > 0x7fffffffULL)
+      raise(MVS_EOVERFLOW, "too many energy blocks for one launch");
+    const int nv = kind == 0 ? AlignSizes<0>::NV : AlignSizes<1>::NV;
+    s->align_part.ensure(static_cast<size_t>(modes) * chunks * nv);
+    float *part = s->align_part.ptr;
+    if (kind == 0) {
+      MVS_LAUNCH(s, "align", k_align<0>, modes * chunks, kThreads, 0, modes, npts, chunks,
+                 theta, base, pts, q, nrm, part);
+      MVS_LAUNCH(s, "align_sum", k_align_sum<0>, modes, kWave, 0, modes, chunks, part, energy,
+                 grad, hess);
+    } else {
+      MVS_LAUNCH(s, "align", k_align<1>, modes * chunks, kThreads, 0, modes, npts, chunks,
+                 theta, base, pts, q, nrm, part);
+      MVS_LAUNCH(s, "align_sum", k_align_sum<1>, modes, kWave, 0, modes, chunks, part, energy,
+                 grad, hess);
+    }
   });
 }
 

@@ -126,6 +126,7 @@ struct Session {
   DBuf<unsigned char> bvh_nodes, bvh_tri, bvh_fid, bvh_leaf_of;
   uint64_t bvh_faces = 0;
   int bvh_root = 0, bvh_stack = 1;  // root reference; LDS stack entries per thread
+  DBuf<float> align_part;            // energy partials: modes x chunks x terms
   // outputs of the last fused pair + vote call (session-owned buffers)
   mvs_pairvote_result pairvote{};
   bool pairvote_valid = false;

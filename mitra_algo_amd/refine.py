@@ -128,52 +128,104 @@ def align_energy(theta: torch.Tensor, problem: Problem) -> torch.Tensor:
 
 
 def refine(problem: Problem, steps: int = 10, damping: float = 1e-3):
+    """Damped Gauss-Newton (Levenberg-Marquardt) steps of one problem; see
+    refine_all.  Returns the per-step energies (steps + 1 rows)."""
+    return refine_all([problem], steps, damping)[0]
+
+
+def _flat_views(buf: torch.Tensor, shapes):
+    """Consecutive views of a flat tensor with the given shapes."""
+    views, o = [], 0
+    for shape in shapes:
+        n = int(np.prod(shape))
+        views.append(buf[o:o + n].view(*shape))
+        o += n
+    return views
+
+
+def refine_all(problems, steps: int = 10, damping: float = 1e-3):
     """Damped Gauss-Newton (Levenberg-Marquardt) steps; closest points are
     recomputed at the start of every step.  A step is kept when it lowers the
     energy (with the step's closest points), else the damping grows.  Rigid
     twists are folded into the base after each accepted step.  Energies,
     gradients and Gauss-Newton matrices come from the device; the 6x6 / 4x4
-    solves and parameter updates run on the host in FP64.  Returns the
-    per-step energies (steps + 1 rows: before the first step and after each,
-    with fresh closest points)."""
-    M = problem.modes
-    dev = problem.pts.device
-    lam = np.full(M, damping)
-    theta = problem.theta.double().cpu().numpy()
-    base = problem.base.double().cpu().numpy() if problem.base is not None else None
-    history = []
+    solves and parameter updates run on the host in FP64.  The problems go
+    in lock step: every launch of a phase is queued before one copy brings
+    all their results to the host, and one copy takes all trial parameters
+    (or all updated parameters) to the device, so a step costs two
+    synchronisations whatever the number of problems.  Returns, per problem,
+    the per-step energies (steps + 1 rows: before the first step and after
+    each, with fresh closest points)."""
+    problems = list(problems)
+    if not problems:
+        return []
+    dev = problems[0].pts.device
+    Ms = [p.modes for p in problems]
+    Ps = [NPARAM[p.kind] for p in problems]
+    lam = [np.full(M, damping) for M in Ms]
+    theta = [p.theta.double().cpu().numpy() for p in problems]
+    base = [p.base.double().cpu().numpy() if p.base is not None else None for p in problems]
+    # device results of a phase, all problems in one buffer: E, g, H each
+    shapes = [s for M, P in zip(Ms, Ps) for s in ((M,), (M, P), (M, P, P))]
+    out = torch.empty(sum(int(np.prod(s)) for s in shapes), dtype=torch.float32, device=dev)
+    outv = _flat_views(out, shapes)
+    history = [[] for _ in problems]
+
+    def launch(k, th, want_hessian):
+        p = problems[k]
+        E, g, H = outv[3 * k:3 * k + 3]
+        p.sess.dev_align_energy(p.kind, p.modes, p.npts, th, p.base, p.pts, p.q, p.nrm, E, g,
+                                H if want_hessian else None)
+
+    def upload(arrays):
+        """FP64 host arrays -> FP32 device tensors of the same shapes, one copy."""
+        flat = torch.from_numpy(np.concatenate([a.ravel() for a in arrays])).float().to(dev)
+        return _flat_views(flat, [a.shape for a in arrays])
+
     for it in range(steps + 1):
-        problem.update_closest(problem.theta)
-        E0, g, H = problem.evaluate(problem.theta)
-        E0 = E0.double().cpu().numpy()
-        history.append(E0)
+        for k, p in enumerate(problems):
+            p.update_closest(p.theta)
+            launch(k, p.theta.detach().float().contiguous(), True)
+        host = _flat_views(out.double().cpu(), shapes)
+        for k in range(len(problems)):
+            history[k].append(host[3 * k].numpy().copy())
         if it == steps:
             break
-        g = g.double().cpu().numpy()
-        H = H.double().cpu().numpy()
-        diag = np.diagonal(H, axis1=1, axis2=2)
-        A = H + (lam[:, None] * (diag + 1e-12 * (1 + diag.sum(1, keepdims=True))))[:, :, None] \
-            * np.eye(H.shape[1])[None]
-        trial = theta - np.linalg.solve(A, g[:, :, None])[:, :, 0]
-        E1, _, _ = problem.evaluate(torch.from_numpy(trial).float().to(dev),
-                                    want_hessian=False)
-        E1 = E1.double().cpu().numpy()
-        ok = (E1 < E0) & np.isfinite(E1)
-        lam = np.where(ok, lam / 3, lam * 4)
-        if problem.kind == RIGID:
-            # fold the accepted twist into the base: the twist restarts at zero
-            for j in np.nonzero(ok)[0]:
-                R = axis_angle_to_matrix(trial[j, :3])
-                R0 = base[j, :9].reshape(3, 3)
-                base[j] = np.concatenate([(R @ R0).ravel(), R @ base[j, 9:] + trial[j, 3:6]])
-                theta[j] = 0
-            problem.base = torch.from_numpy(base).float().to(dev).contiguous()
-        else:
-            for j in np.nonzero(ok)[0]:
-                theta[j] = np.concatenate([trial[j, :3] / np.linalg.norm(trial[j, :3]),
-                                           trial[j, 3:4]])
-        problem.theta = torch.from_numpy(theta).float().to(dev).contiguous()
-    return np.stack(history)
+        trials = []
+        for k in range(len(problems)):
+            g, H = host[3 * k + 1].numpy(), host[3 * k + 2].numpy()
+            diag = np.diagonal(H, axis1=1, axis2=2)
+            A = H + (lam[k][:, None] * (diag + 1e-12 * (1 + diag.sum(1, keepdims=True))))[
+                :, :, None] * np.eye(H.shape[1])[None]
+            trials.append(theta[k] - np.linalg.solve(A, g[:, :, None])[:, :, 0])
+        for k, th in enumerate(upload(trials)):
+            launch(k, th, False)
+        host1 = _flat_views(out.double().cpu(), shapes)
+        updated = []
+        for k, p in enumerate(problems):
+            E0, E1, trial = history[k][-1], host1[3 * k].numpy(), trials[k]
+            ok = (E1 < E0) & np.isfinite(E1)
+            lam[k] = np.where(ok, lam[k] / 3, lam[k] * 4)
+            if p.kind == RIGID:
+                # fold the accepted twist into the base: the twist restarts at zero
+                for j in np.nonzero(ok)[0]:
+                    R = axis_angle_to_matrix(trial[j, :3])
+                    R0 = base[k][j, :9].reshape(3, 3)
+                    base[k][j] = np.concatenate([(R @ R0).ravel(),
+                                                 R @ base[k][j, 9:] + trial[j, 3:6]])
+                    theta[k][j] = 0
+                updated += [base[k], theta[k]]
+            else:
+                for j in np.nonzero(ok)[0]:
+                    theta[k][j] = np.concatenate([trial[j, :3] / np.linalg.norm(trial[j, :3]),
+                                                  trial[j, 3:4]])
+                updated += [theta[k]]
+        dv = iter(upload(updated))
+        for p in problems:
+            if p.kind == RIGID:
+                p.base = next(dv)
+            p.theta = next(dv)
+    return [np.stack(h) for h in history]
 
 
 def surface_samples(vertices: np.ndarray, faces: np.ndarray, count: int,
