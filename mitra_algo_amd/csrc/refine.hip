@@ -176,8 +176,20 @@ __device__ inline Hit nearest(const Inner *inner, const Face *faces, int root, P
       break;
     const int code = -1 - leaf, first = code >> 4, end = first + (code & 15);
     for (int k = first; k < end; ++k) {
+      const Face f = faces[k];
+      // the box of the corners bounds the face's distance from below: a face
+      // whose box is farther than the best hit (by a 2^-10 relative margin,
+      // far above the FP32 rounding of either distance) skips the exact test
+      const float lo[3] = {fminf(fminf(f.a[0], f.b[0]), f.c[0]),
+                           fminf(fminf(f.a[1], f.b[1]), f.c[1]),
+                           fminf(fminf(f.a[2], f.b[2]), f.c[2])};
+      const float hi[3] = {fmaxf(fmaxf(f.a[0], f.b[0]), f.c[0]),
+                           fmaxf(fmaxf(f.a[1], f.b[1]), f.c[1]),
+                           fmaxf(fmaxf(f.a[2], f.b[2]), f.c[2])};
+      if (box_dist2(lo, hi, p.x, p.y, p.z) > h.d2 * (1.f + 0x1p-10f))
+        continue;
       P3 c;
-      const float d = face_d2(p, faces[k], c);
+      const float d = face_d2(p, f, c);
       if (d < h.d2) {
         h.d2 = d;
         h.q = c;
