@@ -186,7 +186,7 @@ def refine_all(problems, steps: int = 10, damping: float = 1e-3):
         for k, p in enumerate(problems):
             p.update_closest(p.theta)
             launch(k, p.theta.detach().float().contiguous(), True)
-        host = _flat_views(out.double().cpu(), shapes)
+        host = _flat_views(out.cpu().double(), shapes)
         for k in range(len(problems)):
             history[k].append(host[3 * k].numpy().copy())
         if it == steps:
@@ -200,7 +200,7 @@ def refine_all(problems, steps: int = 10, damping: float = 1e-3):
             trials.append(theta[k] - np.linalg.solve(A, g[:, :, None])[:, :, 0])
         for k, th in enumerate(upload(trials)):
             launch(k, th, False)
-        host1 = _flat_views(out.double().cpu(), shapes)
+        host1 = _flat_views(out.cpu().double(), shapes)
         updated = []
         for k, p in enumerate(problems):
             E0, E1, trial = history[k][-1], host1[3 * k].numpy(), trials[k]
